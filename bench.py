@@ -1,0 +1,155 @@
+"""Headline benchmark: ant tours/s of plain FACO at TSP1000 on one MI355X.
+
+Workload: 32 uniform random Euclidean TSP1000 instances (integer coordinates
+in [0,10000]^2, EUC_2D), one colony each, 128 ants per colony, MNE 8,
+16/64/20 candidate widths, 2-opt checklist local search with the 100000
+evaluation limit. A step is one FACO iteration of every colony: construction
++ local search for all 4096 ants, then selection and the pheromone update.
+Inputs (instances, candidate rows, pheromone) are resident in HBM before the
+timed region; the timed region is the iteration loop between two stream
+synchronisations, replayed through a HIP graph.
+
+Prints one JSON line with the metric, the roofline of the construction+LS
+kernel (HIP-event average launch time, algorithmic bytes per launch) and the
+CPU baseline (the oracle's OpenMP FACO on the host cores, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_tour(n: int, steps_per_tour: float, evals_per_tour: float) -> float:
+    """Bytes one ant tour must move through the construction+LS kernel:
+    parent read + tour write (4n each), per construction step 16 candidate ids
+    (4 B) and 16 trail*heuristic products (8 B), per LS move evaluation a
+    candidate id, its distance and two distance lookups (16 B)."""
+    return 8.0 * n + 192.0 * steps_per_tour + 16.0 * evals_per_tour
+
+
+def cpu_baseline(xys, keys, seeds, ants, budget_seconds=15.0):
+    from oracle import oracle
+    threads = oracle.max_threads()
+    settings = oracle.default_settings(ants=ants)
+    colonies = min(len(seeds), max(threads, 8))
+    xy = np.stack(xys[:colonies]).astype(np.int32)
+    probe = oracle.run_faco(xy, keys[:colonies], seeds[:colonies], settings, 1, 8, threads=threads)
+    per_batch = max(probe["search_seconds"], 1e-3)
+    batches = int(max(1, min(200, budget_seconds / per_batch)))
+    run = oracle.run_faco(xy, keys[:colonies], seeds[:colonies], settings, batches, 8, threads=threads)
+    tours = colonies * ants * batches
+    return {"value": tours / run["search_seconds"], "unit": "ant-tours/s", "cores": threads, "kind": "port",
+            "sample": f"oracle C++ FACO (OpenMP over colony x ant), TSP1000 uniform, {colonies} colonies x "
+                      f"{ants} ants x {batches} iterations = {tours} tours in {run['search_seconds']:.2f} s; "
+                      f"host nproc={os.cpu_count()}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--n", type=int, default=1000)
+    ap.add_argument("--colonies", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this engine's colonies are independent and every run here uses a "
+              "single MI355X; --gpus above 1 is not measured", file=sys.stderr)
+        return 2
+
+    from gplsaco_amd import gp_faco_ext as ext
+    from gplsaco_amd.instances import instance_key, uniform_instance
+
+    n, colonies = args.n, args.colonies
+    xys = [uniform_instance(n, seed=0, instance=i) for i in range(colonies)]
+    keys = [instance_key(n, 0, i) for i in range(colonies)]
+    seeds = list(range(colonies))
+    settings = ext.FixedFacoSettings()
+    engine = ext.FacoBatchEngine(n, colonies, settings)
+    ants = engine.settings.ants
+    prep_started = time.perf_counter()
+    for k, xy in zip(keys, xys):
+        engine.register_problem(k, xy)
+    prep_seconds = time.perf_counter() - prep_started
+    task_keys = np.array(keys, dtype=np.uint64)
+    task_seeds = np.array(seeds, dtype=np.uint64)
+    print(f"# prepared {colonies} x TSP{n} in {prep_seconds:.2f} s; shape {engine.launch_shape()}", file=sys.stderr,
+          flush=True)
+    if args.warmup:
+        engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.warmup)
+    timed = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=True)
+    print(f"# timed {args.steps} steps in {timed['search_seconds']:.3f} s", file=sys.stderr, flush=True)
+    tours = timed["total_tour_evaluations"]
+    seconds = timed["search_seconds"]
+    value = tours / seconds
+    steps_per_tour = timed["completed_construction_steps"] / tours
+    evals_per_tour = timed["completed_ls_evaluations"] / tours
+
+    prof = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.profile_steps, use_graph=False,
+                                            profile_kernels=True)
+    ptours = prof["total_tour_evaluations"]
+    launch_ms = prof["kernel_milliseconds"]["construction_and_search"] / args.profile_steps
+    colony_ms = prof["kernel_milliseconds"]["reduction_and_pheromone"] / args.profile_steps
+    bytes_per_tour = algorithmic_bytes_per_tour(n, prof["completed_construction_steps"] / ptours,
+                                                prof["completed_ls_evaluations"] / ptours)
+    tours_per_launch = colonies * ants
+    achieved = bytes_per_tour * tours_per_launch / (launch_ms * 1e-3) / 1e9
+    mean_cost = float(np.mean([item["cost"] for item in timed["items"]]))
+
+    line = {
+        "metric": "ant tours/s (construct+2-opt/Or-opt) at TSP1000; gap vs plain FACO at equal FE",
+        "value": value,
+        "unit": "ant-tours/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": seconds / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FACO (no GP, fixed MNE 8), {colonies} uniform random EUC_2D TSP{n} instances, "
+                        f"one colony each, {ants} ants, 16/64/20 candidates, checklist 2-opt",
+            "n": n, "colonies": colonies, "ants": ants, "mne": 8,
+            "construction_steps_per_tour": steps_per_tour, "ls_evaluations_per_tour": evals_per_tour,
+            "mean_final_tour_length": mean_cost, "waves_per_block": engine.launch_shape()["waves_per_block"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "faco_ant_search",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_tour": bytes_per_tour,
+            "tours_per_launch": tours_per_launch,
+            "launch_ms": launch_ms,
+            "colony_update_ms": colony_ms,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(xys, keys, seeds, ants)
+        line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

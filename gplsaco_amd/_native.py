@@ -1,0 +1,173 @@
+"""ctypes binding of libfaco_hip.so, the C ABI in include/faco_engine.h.
+
+The library is built in-tree (gplsaco_amd/libfaco_hip.so) by
+gplsaco_amd.build.build(); importing this module never builds and never falls
+back to Python: a missing library raises ImportError, so a product path can
+not silently run on anything but the HIP engine.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libfaco_hip.so"
+
+FACO_OK, FACO_INVALID_ARGUMENT, FACO_RUNTIME_ERROR = 0, 1, 2
+
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+vp = ctypes.c_void_p
+
+
+class Settings(ctypes.Structure):
+    """faco_settings: the reference FixedFacoSettings layout (64 bytes)."""
+
+    _fields_ = [
+        ("ants", ctypes.c_uint32),
+        ("primary_width", ctypes.c_uint32),
+        ("backup_width", ctypes.c_uint32),
+        ("ls_width", ctypes.c_uint32),
+        ("beta", ctypes.c_double),
+        ("retention", ctypes.c_double),
+        ("p_best", ctypes.c_double),
+        ("epoch_source_probability", ctypes.c_double),
+        ("ls_evaluation_limit", ctypes.c_uint64),
+        ("initial_ls_evaluation_limit", ctypes.c_uint64),
+    ]
+
+
+class AntInfo(ctypes.Structure):
+    _fields_ = [
+        ("construction_cost", ctypes.c_int64),
+        ("final_cost", ctypes.c_int64),
+        ("mne", ctypes.c_uint32),
+        ("steps", ctypes.c_uint32),
+        ("nonidentity_relocations", ctypes.c_uint32),
+        ("legal_exhausted", ctypes.c_uint32),
+        ("processed_nodes", ctypes.c_uint32),
+        ("candidate_checks", ctypes.c_uint32),
+        ("move_evaluations", ctypes.c_uint32),
+        ("accepted_moves", ctypes.c_uint32),
+        ("reactivations", ctypes.c_uint32),
+        ("evaluation_limit_reached", ctypes.c_uint32),
+        ("checklist_size", ctypes.c_uint32),
+        ("start_node", ctypes.c_uint32),
+    ]
+
+
+ANT_INFO_DTYPE = np.dtype([(name, np.int64 if t is ctypes.c_int64 else np.uint32) for name, t in AntInfo._fields_])
+assert ANT_INFO_DTYPE.itemsize == ctypes.sizeof(AntInfo)
+
+STAGE_COUNT = 9
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("evaluation_limit_per_colony", ctypes.c_uint64),
+        ("completed_tour_evaluations_per_colony", ctypes.c_uint64),
+        ("total_tour_evaluations", ctypes.c_uint64),
+        ("launched_batches", ctypes.c_uint64),
+        ("completed_batches", ctypes.c_uint64),
+        ("completed_construction_steps", ctypes.c_uint64),
+        ("completed_ls_evaluations", ctypes.c_uint64),
+        ("allocated_device_bytes", ctypes.c_uint64),
+        ("actual_seconds", ctypes.c_double),
+        ("search_seconds", ctypes.c_double),
+        ("search_gpu_milliseconds", ctypes.c_double),
+        ("kernel_milliseconds", ctypes.c_double * STAGE_COUNT),
+    ]
+
+
+class Program(ctypes.Structure):
+    _fields_ = [
+        ("ir_version", ctypes.c_uint16),
+        ("numeric_spec_id", ctypes.c_uint16),
+        ("feature_spec_id", ctypes.c_uint16),
+        ("length", ctypes.c_uint16),
+        ("constants_count", ctypes.c_uint16),
+        ("opcode", ctypes.c_uint8 * 63),
+        ("operand", ctypes.c_uint8 * 63),
+        ("constants", ctypes.c_float * 63),
+    ]
+
+
+assert ctypes.sizeof(Settings) == 64
+assert ctypes.sizeof(Program) == 388
+
+# (name, restype, argtypes) of every exported entry point of include/faco_engine.h
+SIGNATURES = {
+    "faco_last_error": (ctypes.c_char_p, []),
+    "faco_device_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_size_t,
+                                        ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "faco_prepare_common_initial": (ctypes.c_int, [f64p, ctypes.c_uint32, ctypes.POINTER(Settings),
+                                                   ctypes.POINTER(Settings), u32p, ctypes.POINTER(ctypes.c_int64),
+                                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                                   vp, vp, vp, vp]),
+    "faco_engine_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Settings), ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.POINTER(vp)]),
+    "faco_engine_destroy": (None, [vp]),
+    "faco_engine_settings": (ctypes.c_int, [vp, ctypes.POINTER(Settings)]),
+    "faco_engine_launch_shape": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                                ctypes.POINTER(ctypes.c_uint32)]),
+    "faco_engine_set_waves": (ctypes.c_int, [vp, ctypes.c_uint32]),
+    "faco_engine_register_problem": (ctypes.c_int, [vp, ctypes.c_uint64, f64p, ctypes.c_uint32,
+                                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "faco_engine_preparation_profile": (ctypes.c_int, [vp, ctypes.c_uint64, f64p]),
+    "faco_engine_problem_data": (ctypes.c_int, [vp, ctypes.c_uint64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int64)]),
+    "faco_engine_evaluate_faco": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                                 ctypes.c_uint32, ctypes.POINTER(Result), u32p, i64p, u64p]),
+    "faco_fixed_create": (ctypes.c_int, [f64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(Settings),
+                                         ctypes.POINTER(vp)]),
+    "faco_fixed_destroy": (None, [vp]),
+    "faco_fixed_settings": (ctypes.c_int, [vp, ctypes.POINTER(Settings)]),
+    "faco_fixed_run": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp, vp, vp, vp, vp]),
+    "faco_diagnostic_construct_ls": (ctypes.c_int, [f64p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                                    u32p, u32p, u32p, ctypes.c_uint64, ctypes.c_uint32, u32p, u32p,
+                                                    u32p, u32p, vp]),
+    "faco_diagnostic_distances": (ctypes.c_int, [i32p, ctypes.c_uint32, i64p]),
+    "faco_diagnostic_philox": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_uint32, u32p]),
+    "faco_score_cpu": (ctypes.c_int, [ctypes.POINTER(Program), f32p, u32p, ctypes.c_uint32, f32p, i32p]),
+    "faco_score_gpu": (ctypes.c_int, [ctypes.POINTER(Program), f32p, u32p, ctypes.c_uint32, f32p, i32p]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libfaco_hip.so; raise ImportError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: run gplsaco_amd.build.build() (hipcc, gfx950) first")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    """Map a C status to the reference's Python exceptions."""
+    if status == FACO_OK:
+        return
+    message = lib().faco_last_error().decode(errors="replace")
+    if status == FACO_INVALID_ARGUMENT:
+        raise ValueError(message)
+    raise RuntimeError(message)
+
+
+def ptr(array) -> ctypes.c_void_p | None:
+    return None if array is None else array.ctypes.data_as(ctypes.c_void_p)

@@ -1,0 +1,718 @@
+// Host side of the FACO batch engine: problem preparation, device buffers,
+// the iteration loop (eager or replayed through a HIP graph), the fixed-FACO
+// entry point, diagnostics and the C ABI declared in include/faco_engine.h.
+//
+// Boundary semantics follow the reference's native module gp_faco_ext
+// (cpp/src/bindings.cpp:500-1046) and engine host code
+// (cuda/batch_engine.cu:78-1192, cuda/fixed_faco.cu:44-177).
+#include "faco_common.h"
+#include "host_prepare.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace faco {
+namespace {
+
+thread_local std::string g_error;
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+void require(bool condition, const char* message) {
+    if (!condition) throw std::invalid_argument(message);
+}
+void checked(hipError_t status, const char* what) {
+    if (status != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(status));
+}
+
+template<class F> int guarded(F&& f) {
+    try { f(); return FACO_OK; }
+    catch (const std::invalid_argument& e) { g_error = e.what(); return FACO_INVALID_ARGUMENT; }
+    catch (const std::exception& e) { g_error = e.what(); return FACO_RUNTIME_ERROR; }
+    catch (...) { g_error = "unknown native error"; return FACO_RUNTIME_ERROR; }
+}
+
+// Owning device allocation.
+template<class T> class DeviceArray {
+public:
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() { if (ptr_) (void)hipFree(ptr_); }
+    void allocate(size_t count) {
+        require(ptr_ == nullptr, "device buffer allocated twice");
+        count_ = count;
+        if (count) checked(hipMalloc(reinterpret_cast<void**>(&ptr_), bytes()), "hipMalloc");
+    }
+    T* data() const { return ptr_; }
+    size_t size() const { return count_; }
+    size_t bytes() const { return count_ * sizeof(T); }
+    void upload(const T* src, size_t count, hipStream_t s, size_t offset = 0) {
+        require(offset + count <= count_, "device upload out of range");
+        if (count) checked(hipMemcpyAsync(ptr_ + offset, src, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+    void download(T* dst, size_t count, hipStream_t s, size_t offset = 0) const {
+        require(offset + count <= count_, "device download out of range");
+        if (count) checked(hipMemcpyAsync(dst, ptr_ + offset, count * sizeof(T), hipMemcpyDeviceToHost, s), "download");
+    }
+private:
+    T* ptr_ = nullptr;
+    size_t count_ = 0;
+};
+
+// Device-resident geometry of one engine: per-instance arrays packed by slot.
+struct DeviceGeometry {
+    DeviceArray<uint32_t> coords;     // [G][n]
+    DeviceArray<uint16_t> nbr;        // [G][nbr_stride]
+    DeviceArray<uint16_t> backup;     // [G][n][bw]
+    DeviceArray<double> eta;          // [G][n][pw]
+    DeviceArray<uint32_t> initial;    // [G][n]
+    DeviceArray<int64_t> initial_cost;
+};
+
+struct LaunchShape {
+    uint32_t waves = 1;
+    uint32_t nbr_in_lds = 1;
+    uint32_t nbr_lds_bytes = 0;
+};
+
+int current_device() { int d = 0; checked(hipGetDevice(&d), "hipGetDevice"); return d; }
+
+size_t max_lds_per_block() {
+    int value = 0;
+    checked(hipDeviceGetAttribute(&value, hipDeviceAttributeMaxSharedMemoryPerBlock, current_device()),
+            "hipDeviceGetAttribute");
+    return static_cast<size_t>(value);
+}
+
+// Workgroup shape for the ant kernel: the most ants per workgroup that both
+// divides the ants of a colony and fits the LDS budget; neighbour rows are
+// staged in LDS when at least one ant still fits beside them.
+LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_waves) {
+    LaunchShape shape;
+    const size_t budget = max_lds_per_block();
+    shape.nbr_lds_bytes = round16_host(2u * n * K);
+    const uint32_t per_wave = ant_wave_bytes(n);
+    for (uint32_t lds_rows : {1u, 0u}) {
+        const size_t shared = ant_shared_bytes(n, lds_rows ? shape.nbr_lds_bytes : 0u);
+        for (uint32_t w : {16u, 8u, 4u, 2u, 1u}) {
+            if (forced_waves && w != forced_waves) continue;
+            if (ants % w) continue;
+            if (shared + static_cast<size_t>(w) * per_wave <= budget) {
+                shape.waves = w; shape.nbr_in_lds = lds_rows;
+                return shape;
+            }
+        }
+    }
+    throw std::invalid_argument("instance too large for the LDS budget of one ant");
+}
+
+struct PreparedSlot { uint64_t key; const Prepared* problem; };
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct Engine {
+    uint32_t n, panel, population, colonies_capacity;
+    faco_settings config;
+    uint32_t K, nbr_stride;
+    int device;
+    std::mutex mutex;
+    std::map<uint64_t, Prepared> registry;
+    hipStream_t stream = nullptr;
+    size_t allocated = 0;
+    LaunchShape shape;
+    uint32_t forced_waves = 0, xcd_remap = 1;
+    double p_root = 0;
+    // device state
+    DeviceGeometry geo;
+    DeviceArray<uint32_t> geometry, targets, parent, links, epoch, global, tours;
+    DeviceArray<uint64_t> keys;
+    DeviceArray<ColonyState> state;
+    DeviceArray<double> trails, products;
+    DeviceArray<faco_ant_info> info;
+    std::vector<uint64_t> resident;  // instance key per geometry slot
+    // graph cache
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_colonies = 0, graph_iterations = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+
+    Engine(uint32_t dimension, uint32_t colonies, faco_settings settings, uint32_t pop)
+        : n(dimension), panel(colonies), population(pop), colonies_capacity(colonies * pop) {
+        require(colonies >= 1 && colonies <= 4096 && pop >= 1 && pop <= 128,
+                "panel size must be in 1..4096 and population in 1..128");
+        config = normalized_settings(settings, n);
+        require(config.primary_width <= kMaxPrimary, "primary_width above 64 is not supported");
+        require(config.ls_width <= kMaxLs, "ls_width above 32 is not supported");
+        require(config.backup_width <= kMaxBackup, "backup_width above 256 is not supported");
+        require(n <= 65535, "dimension above 65535 is not supported");
+        K = std::max(config.primary_width, config.ls_width);
+        nbr_stride = round16_host(2u * n * K) / 2u;
+        device = current_device();
+        p_root = std::pow(config.p_best, 1.0 / config.primary_width);
+        shape = choose_shape(n, K, config.ants, 0);
+        checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        checked(hipEventCreate(&ev_start), "hipEventCreate");
+        checked(hipEventCreate(&ev_stop), "hipEventCreate");
+        const size_t C = colonies_capacity, A = C * config.ants, G = C;
+        auto alloc = [&](auto& buffer, size_t count) { buffer.allocate(count); allocated += buffer.bytes(); };
+        alloc(geo.coords, G * n); alloc(geo.nbr, G * nbr_stride); alloc(geo.backup, G * n * std::max(config.backup_width, 1u));
+        alloc(geo.eta, G * n * config.primary_width); alloc(geo.initial, G * n); alloc(geo.initial_cost, G);
+        alloc(geometry, C); alloc(keys, C); alloc(targets, A); alloc(state, C);
+        alloc(parent, C * n); alloc(links, C * n); alloc(epoch, C * n); alloc(global, C * n);
+        alloc(trails, C * n * config.primary_width); alloc(products, C * n * config.primary_width);
+        alloc(tours, A * n); alloc(info, A);
+    }
+    ~Engine() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (ev_start) (void)hipEventDestroy(ev_start);
+        if (ev_stop) (void)hipEventDestroy(ev_stop);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void set_waves(uint32_t waves) {
+        shape = choose_shape(n, K, config.ants, waves);
+        forced_waves = waves;
+        drop_graph();
+    }
+    void drop_graph() {
+        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+        graph_colonies = graph_iterations = 0;
+    }
+
+    const Prepared& problem(uint64_t key) const {
+        const auto it = registry.find(key);
+        require(it != registry.end(), "task instance has not been registered");
+        return it->second;
+    }
+
+    // Upload geometry for the distinct instances of `tasks` (first-appearance
+    // order) and the colony -> slot map.
+    void upload_geometry(const uint64_t* instance_keys, uint32_t count) {
+        std::vector<uint64_t> slots;
+        std::vector<uint32_t> map(count);
+        for (uint32_t c = 0; c < count; ++c) {
+            const auto found = std::find(slots.begin(), slots.end(), instance_keys[c]);
+            map[c] = static_cast<uint32_t>(found - slots.begin());
+            if (found == slots.end()) slots.push_back(instance_keys[c]);
+        }
+        geometry.upload(map.data(), count, stream);
+        if (slots == resident) return;
+        const uint32_t bw = config.backup_width, pw = config.primary_width;
+        for (uint32_t g = 0; g < slots.size(); ++g) {
+            const Prepared& p = problem(slots[g]);
+            geo.coords.upload(p.packed.data(), n, stream, static_cast<size_t>(g) * n);
+            geo.nbr.upload(p.nbr.data(), nbr_stride, stream, static_cast<size_t>(g) * nbr_stride);
+            if (bw) geo.backup.upload(p.backup16.data(), static_cast<size_t>(n) * bw, stream, static_cast<size_t>(g) * n * bw);
+            geo.eta.upload(p.eta.data(), static_cast<size_t>(n) * pw, stream, static_cast<size_t>(g) * n * pw);
+            geo.initial.upload(p.initial.data(), n, stream, static_cast<size_t>(g) * n);
+            geo.initial_cost.upload(&p.initial_cost, 1, stream, g);
+        }
+        checked(hipStreamSynchronize(stream), "geometry upload");
+        resident = slots;
+    }
+
+    AntParams ant_params(uint32_t colonies) const {
+        AntParams p{};
+        p.nbr_in_lds = shape.nbr_in_lds;
+        p.n = n; p.ants = config.ants; p.pw = config.primary_width; p.bw = config.backup_width;
+        p.lw = config.ls_width; p.K = K; p.waves = shape.waves;
+        p.nbr_lds_bytes = shape.nbr_lds_bytes; p.nbr_stride = nbr_stride;
+        p.per_wave_bytes = ant_wave_bytes(n);
+        p.xcd_remap = xcd_remap;
+        p.ls_limit = config.ls_evaluation_limit;
+        p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
+        p.products = products.data(); p.parent = parent.data(); p.links = links.data();
+        p.state = state.data(); p.keys = keys.data(); p.targets = targets.data();
+        p.geometry = geometry.data(); p.out_tours = tours.data(); p.out_info = info.data();
+        (void)colonies;
+        return p;
+    }
+    ColonyParams colony_params() const {
+        ColonyParams p{};
+        p.n = n; p.ants = config.ants; p.pw = config.primary_width; p.K = K; p.nbr_stride = nbr_stride;
+        p.retention = config.retention; p.p_best_root = p_root;
+        p.epoch_probability = config.epoch_source_probability;
+        p.nbr = geo.nbr.data(); p.eta = geo.eta.data(); p.geometry = geometry.data(); p.keys = keys.data();
+        p.tours = tours.data(); p.info = info.data(); p.state = state.data(); p.parent = parent.data();
+        p.links = links.data(); p.epoch = epoch.data(); p.global = global.data(); p.trails = trails.data();
+        p.products = products.data();
+        return p;
+    }
+    void reset(uint32_t colonies) {
+        ResetParams r{};
+        r.n = n; r.ants = config.ants; r.pw = config.primary_width; r.K = K;
+        r.retention = config.retention; r.p_best_root = p_root;
+        r.geometry = geometry.data(); r.initial = geo.initial.data(); r.initial_cost = geo.initial_cost.data();
+        r.eta = geo.eta.data(); r.state = state.data(); r.parent = parent.data(); r.links = links.data();
+        r.epoch = epoch.data(); r.global = global.data(); r.trails = trails.data(); r.products = products.data();
+        checked(launch_colony_reset(r, colonies, stream), "colony reset launch");
+    }
+    void iteration(uint32_t colonies, const AntParams& ap, const ColonyParams& cp) {
+        const uint32_t blocks = colonies * config.ants / shape.waves;
+        checked(launch_ant_search(ap, blocks, stream), "ant search launch");
+        checked(launch_colony_update(cp, colonies, stream), "colony update launch");
+    }
+    // Eager iterations with HIP events around each kernel on the engine
+    // stream; ms[0] += construction+LS, ms[1] += colony update.
+    void run_profiled(uint32_t colonies, uint64_t iterations, double* ms) {
+        const AntParams ap = ant_params(colonies);
+        const ColonyParams cp = colony_params();
+        hipEvent_t ev[3];
+        for (auto& e : ev) checked(hipEventCreate(&e), "hipEventCreate");
+        struct Destroy { hipEvent_t* e; ~Destroy() { for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]); } } d{ev};
+        const uint32_t blocks = colonies * config.ants / shape.waves;
+        for (uint64_t i = 0; i < iterations; ++i) {
+            checked(hipEventRecord(ev[0], stream), "event");
+            checked(launch_ant_search(ap, blocks, stream), "ant search launch");
+            checked(hipEventRecord(ev[1], stream), "event");
+            checked(launch_colony_update(cp, colonies, stream), "colony update launch");
+            checked(hipEventRecord(ev[2], stream), "event");
+            checked(hipEventSynchronize(ev[2]), "event sync");
+            float a = 0, b = 0;
+            checked(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
+            checked(hipEventElapsedTime(&b, ev[1], ev[2]), "event time");
+            ms[0] += a; ms[1] += b;
+        }
+    }
+    // Run `iterations` batches; through a cached HIP graph of up to 16
+    // iterations when use_graph is set, eagerly otherwise.
+    void run(uint32_t colonies, uint64_t iterations, bool use_graph) {
+        const AntParams ap = ant_params(colonies);
+        const ColonyParams cp = colony_params();
+        constexpr uint32_t kGraphIterations = 16;
+        if (use_graph && iterations >= kGraphIterations) {
+            if (!graph || graph_colonies != colonies || graph_iterations != kGraphIterations) {
+                drop_graph();
+                hipGraph_t g = nullptr;
+                checked(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                for (uint32_t i = 0; i < kGraphIterations; ++i) iteration(colonies, ap, cp);
+                checked(hipStreamEndCapture(stream, &g), "end capture");
+                checked(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0), "graph instantiate");
+                (void)hipGraphDestroy(g);
+                graph_colonies = colonies; graph_iterations = kGraphIterations;
+            }
+            const uint64_t rounds = iterations / kGraphIterations;
+            for (uint64_t r = 0; r < rounds; ++r) checked(hipGraphLaunch(graph, stream), "graph launch");
+            iterations -= rounds * kGraphIterations;
+        }
+        for (uint64_t i = 0; i < iterations; ++i) iteration(colonies, ap, cp);
+    }
+};
+
+struct Fixed {
+    std::unique_ptr<Engine> engine;
+    uint64_t key;
+};
+
+void set_last_error(const char* message) { g_error = message; }
+
+}  // namespace faco
+
+using namespace faco;
+
+struct faco_engine { std::unique_ptr<Engine> impl; };
+struct faco_fixed { std::unique_ptr<Engine> impl; uint64_t key; };
+
+namespace {
+
+std::vector<int32_t> integer_coordinates(const double* xy, uint32_t n) {
+    require(xy != nullptr && n >= 3 && n <= 10000, "coordinates must be FP64 [n,2] with 3 <= n <= 10000");
+    std::vector<int32_t> out(2 * static_cast<size_t>(n));
+    for (size_t i = 0; i < out.size(); ++i) {
+        const double v = xy[i];
+        require(std::isfinite(v) && v == std::floor(v) && v >= 0 && v <= kMaxCoordinate,
+                "EUC_2D engine needs integer coordinates in [0, 32767]");
+        out[i] = static_cast<int32_t>(v);
+    }
+    return out;
+}
+
+void fill_result(faco_result* r, const Engine& e, uint32_t colonies, uint64_t limit, uint64_t batches,
+                 uint64_t steps, uint64_t evals, double actual, double search, float gpu_ms) {
+    if (!r) return;
+    std::memset(r, 0, sizeof(*r));
+    r->evaluation_limit_per_colony = limit;
+    r->completed_tour_evaluations_per_colony = batches * e.config.ants;
+    r->total_tour_evaluations = r->completed_tour_evaluations_per_colony * colonies;
+    r->launched_batches = r->completed_batches = batches;
+    r->completed_construction_steps = steps;
+    r->completed_ls_evaluations = evals;
+    r->allocated_device_bytes = e.allocated;
+    r->actual_seconds = actual;
+    r->search_seconds = search;
+    r->search_gpu_milliseconds = gpu_ms;
+}
+
+// Shared evaluation body for the batch engine and the fixed entry point.
+void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks, uint64_t batches,
+              uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
+              uint64_t limit) {
+    const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
+    double kernel_ms[FACO_STAGE_COUNT] = {0};
+    const auto started = Clock::now();
+    std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+    require(lock.owns_lock(), "one evaluation per engine at a time");
+    require(current_device() == e.device, "engine must be used on the device it was created on");
+    require(tasks == e.panel, "task count must equal the engine's colonies");
+    require(mne > 0, "mne_target must be positive");
+    for (uint32_t c = 0; c < tasks; ++c) (void)e.problem(instance_keys[c]);
+    e.upload_geometry(instance_keys, tasks);
+    std::vector<uint64_t> keys(tasks);
+    for (uint32_t c = 0; c < tasks; ++c) keys[c] = colony_key(instance_keys[c], seeds[c]);
+    e.keys.upload(keys.data(), tasks, e.stream);
+    std::vector<uint32_t> targets(static_cast<size_t>(tasks) * e.config.ants, mne);
+    e.targets.upload(targets.data(), targets.size(), e.stream);
+    e.reset(tasks);
+    checked(hipStreamSynchronize(e.stream), "initialisation");
+    const auto search_started = Clock::now();
+    checked(hipEventRecord(e.ev_start, e.stream), "event");
+    if (profile) e.run_profiled(tasks, batches, kernel_ms);
+    else e.run(tasks, batches, use_graph);
+    checked(hipEventRecord(e.ev_stop, e.stream), "event");
+    checked(hipStreamSynchronize(e.stream), "search");
+    const double search = seconds_since(search_started);
+    float gpu_ms = 0;
+    checked(hipEventElapsedTime(&gpu_ms, e.ev_start, e.ev_stop), "event time");
+    std::vector<ColonyState> states(tasks);
+    e.state.download(states.data(), tasks, e.stream);
+    if (tours) e.global.download(tours, static_cast<size_t>(tasks) * e.n, e.stream);
+    checked(hipStreamSynchronize(e.stream), "readback");
+    uint64_t steps = 0, evals = 0;
+    for (uint32_t c = 0; c < tasks; ++c) {
+        if (costs) costs[c] = states[c].global_cost;
+        if (work) { work[2 * c] = states[c].work_construction; work[2 * c + 1] = states[c].work_ls; }
+        steps += states[c].work_construction; evals += states[c].work_ls;
+    }
+    fill_result(result, e, tasks, limit, batches, steps, evals, seconds_since(started), search, gpu_ms);
+    if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* faco_last_error(void) { return g_error.c_str(); }
+
+int faco_device_info(int* device, char* name, size_t name_size, char* arch, size_t arch_size,
+                     int* compute_units, size_t* total_memory, size_t* max_lds) {
+    return guarded([&] {
+        int d = current_device();
+        hipDeviceProp_t prop;
+        checked(hipGetDeviceProperties(&prop, d), "hipGetDeviceProperties");
+        if (device) *device = d;
+        if (name && name_size) { std::strncpy(name, prop.name, name_size - 1); name[name_size - 1] = 0; }
+        if (arch && arch_size) { std::strncpy(arch, prop.gcnArchName, arch_size - 1); arch[arch_size - 1] = 0; }
+        if (compute_units) *compute_units = prop.multiProcessorCount;
+        if (total_memory) *total_memory = prop.totalGlobalMem;
+        if (max_lds) *max_lds = prop.sharedMemPerBlock;
+    });
+}
+
+int faco_prepare_common_initial(const double* xy, uint32_t n, const faco_settings* settings,
+                                faco_settings* normalized, uint32_t* tour, int64_t* cost,
+                                double* cheap_seconds, double* preparation_seconds, uint32_t* primary,
+                                uint32_t* backup, uint32_t* ls, double* local_scale) {
+    return guarded([&] {
+        require(settings != nullptr, "null argument");
+        const auto coords = integer_coordinates(xy, n);
+        const faco_settings c = normalized_settings(*settings, n);
+        const uint32_t K = std::max(c.primary_width, c.ls_width);
+        const Prepared p = prepare_problem(coords, c, K, round16_host(2u * n * K) / 2u);
+        if (normalized) *normalized = c;
+        if (tour) std::copy(p.initial.begin(), p.initial.end(), tour);
+        if (cost) *cost = p.initial_cost;
+        if (cheap_seconds) *cheap_seconds = p.cheap_seconds;
+        if (preparation_seconds) *preparation_seconds = p.preparation_seconds;
+        if (primary) std::copy(p.primary.begin(), p.primary.end(), primary);
+        if (backup) std::copy(p.backup.begin(), p.backup.end(), backup);
+        if (ls) std::copy(p.ls.begin(), p.ls.end(), ls);
+        if (local_scale) std::copy(p.local_scale.begin(), p.local_scale.end(), local_scale);
+    });
+}
+
+int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_settings* settings,
+                       uint32_t constraint_mode, uint32_t population_size, faco_engine** out) {
+    return guarded([&] {
+        require(out && settings, "null argument");
+        require(constraint_mode == 0, "only the unrestricted constraint mode is available");
+        auto engine = std::make_unique<faco_engine>();
+        engine->impl = std::make_unique<Engine>(dimension, colonies, *settings, population_size);
+        *out = engine.release();
+    });
+}
+
+void faco_engine_destroy(faco_engine* engine) { delete engine; }
+
+int faco_engine_settings(const faco_engine* engine, faco_settings* out) {
+    return guarded([&] { require(engine && out, "null argument"); *out = engine->impl->config; });
+}
+
+int faco_engine_set_waves(faco_engine* engine, uint32_t waves) {
+    return guarded([&] { require(engine, "null argument"); engine->impl->set_waves(waves); });
+}
+
+int faco_engine_launch_shape(const faco_engine* engine, uint32_t* waves, uint32_t* nbr_in_lds, uint32_t* lds_bytes) {
+    return guarded([&] {
+        require(engine, "null argument");
+        const auto& e = *engine->impl;
+        if (waves) *waves = e.shape.waves;
+        if (nbr_in_lds) *nbr_in_lds = e.shape.nbr_in_lds;
+        if (lds_bytes) *lds_bytes = ant_shared_bytes(e.n, e.shape.nbr_in_lds ? e.shape.nbr_lds_bytes : 0u) +
+                                    e.shape.waves * ant_wave_bytes(e.n);
+    });
+}
+
+int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double* xy, uint32_t n,
+                                 double* cheap_seconds, double* preparation_seconds) {
+    return guarded([&] {
+        require(engine, "null argument");
+        Engine& e = *engine->impl;
+        std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+        require(lock.owns_lock(), "registration and evaluation of one engine must not overlap");
+        require(n == e.n, "instance dimension does not match the engine");
+        auto coords = integer_coordinates(xy, n);
+        const auto existing = e.registry.find(key);
+        if (existing != e.registry.end()) {
+            require(existing->second.xy == coords, "same instance key registered with different coordinates");
+        } else {
+            e.registry.emplace(key, prepare_problem(coords, e.config, e.K, e.nbr_stride));
+        }
+        const Prepared& p = e.registry.at(key);
+        if (cheap_seconds) *cheap_seconds = p.cheap_seconds;
+        if (preparation_seconds) *preparation_seconds = p.preparation_seconds;
+    });
+}
+
+int faco_engine_preparation_profile(faco_engine* engine, uint64_t key, double out[4]) {
+    return guarded([&] {
+        require(engine && out, "null argument");
+        const Prepared& p = engine->impl->problem(key);
+        std::copy(p.profile, p.profile + 4, out);
+    });
+}
+
+int faco_engine_problem_data(faco_engine* engine, uint64_t key, uint32_t* primary, uint32_t* backup,
+                             uint32_t* ls, uint32_t* initial_tour, int64_t* initial_cost) {
+    return guarded([&] {
+        require(engine, "null argument");
+        const Prepared& p = engine->impl->problem(key);
+        if (primary) std::copy(p.primary.begin(), p.primary.end(), primary);
+        if (backup) std::copy(p.backup.begin(), p.backup.end(), backup);
+        if (ls) std::copy(p.ls.begin(), p.ls.end(), ls);
+        if (initial_tour) std::copy(p.initial.begin(), p.initial.end(), initial_tour);
+        if (initial_cost) *initial_cost = p.initial_cost;
+    });
+}
+
+int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                              uint32_t tasks, uint64_t limit, uint32_t mne, uint32_t flags,
+                              faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work) {
+    return guarded([&] {
+        require(engine && instance_keys && seeds, "null argument");
+        Engine& e = *engine->impl;
+        require(e.population == 1, "population engines evaluate through the population entry point");
+        require(limit % e.config.ants == 0 && limit / e.config.ants <= UINT32_MAX,
+                "evaluation limit must be a whole number of ant batches");
+        evaluate(e, instance_keys, seeds, tasks, limit / e.config.ants, mne, flags, result, tours,
+                 costs, work, limit);
+    });
+}
+
+// ---------------------------------------------------------------------------
+int faco_fixed_create(const double* xy, uint32_t n, uint64_t key, const faco_settings* settings, faco_fixed** out) {
+    return guarded([&] {
+        require(out && settings, "null argument");
+        auto fixed = std::make_unique<faco_fixed>();
+        fixed->impl = std::make_unique<Engine>(n, 1u, *settings, 1u);
+        fixed->key = key;
+        Engine& e = *fixed->impl;
+        auto coords = integer_coordinates(xy, n);
+        e.registry.emplace(key, prepare_problem(coords, e.config, e.K, e.nbr_stride));
+        *out = fixed.release();
+    });
+}
+
+void faco_fixed_destroy(faco_fixed* fixed) { delete fixed; }
+
+int faco_fixed_settings(const faco_fixed* fixed, faco_settings* out) {
+    return guarded([&] { require(fixed && out, "null argument"); *out = fixed->impl->config; });
+}
+
+int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t mne, uint32_t* tour, int64_t* cost,
+                   int64_t* initial_cost, uint64_t* construction_steps, uint64_t* ls_evaluations,
+                   double* solve_seconds, uint32_t* trace_parent, double* trace_trails, int64_t* trace_costs,
+                   uint32_t* trace_best, faco_ant_info* trace_ants, uint32_t* trace_selected,
+                   double* trace_uniforms, uint32_t* trace_tours) {
+    return guarded([&] {
+        require(fixed, "null argument");
+        Engine& e = *fixed->impl;
+        require(mne > 0, "mne_target must be positive");
+        const auto started = Clock::now();
+        const bool trace = trace_parent || trace_trails || trace_costs || trace_best || trace_ants ||
+                           trace_selected || trace_uniforms || trace_tours;
+        const uint64_t keyv = fixed->key;
+        if (!trace) {
+            uint64_t work[2] = {0, 0};
+            int64_t c = 0;
+            evaluate(e, &keyv, &seed, 1, batches, mne, FACO_RUN_GRAPH, nullptr, tour, &c, work,
+                     static_cast<uint64_t>(batches) * e.config.ants);
+            if (cost) *cost = c;
+            if (construction_steps) *construction_steps = work[0];
+            if (ls_evaluations) *ls_evaluations = work[1];
+        } else {
+            std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+            require(lock.owns_lock(), "one evaluation per engine at a time");
+            const uint32_t n = e.n, ants = e.config.ants, pw = e.config.primary_width;
+            e.upload_geometry(&keyv, 1);
+            const uint64_t key = colony_key(keyv, seed);
+            e.keys.upload(&key, 1, e.stream);
+            std::vector<uint32_t> targets(ants, mne);
+            e.targets.upload(targets.data(), ants, e.stream);
+            e.reset(1);
+            DeviceArray<uint32_t> selected;
+            DeviceArray<double> uniforms;
+            selected.allocate(static_cast<size_t>(ants) * n);
+            uniforms.allocate(static_cast<size_t>(ants) * n);
+            AntParams ap = e.ant_params(1);
+            ap.trace_selected = selected.data();
+            ap.trace_uniform = uniforms.data();
+            const ColonyParams cp = e.colony_params();
+            for (uint32_t b = 0; b < batches; ++b) {
+                e.iteration(1, ap, cp);
+                ColonyState st;
+                e.state.download(&st, 1, e.stream);
+                if (trace_parent) e.parent.download(trace_parent + static_cast<size_t>(b) * n, n, e.stream);
+                if (trace_trails) e.trails.download(trace_trails + static_cast<size_t>(b) * n * pw, static_cast<size_t>(n) * pw, e.stream);
+                if (trace_ants) e.info.download(trace_ants + static_cast<size_t>(b) * ants, ants, e.stream);
+                if (trace_selected) selected.download(trace_selected + static_cast<size_t>(b) * ants * n, static_cast<size_t>(ants) * n, e.stream);
+                if (trace_uniforms) uniforms.download(trace_uniforms + static_cast<size_t>(b) * ants * n, static_cast<size_t>(ants) * n, e.stream);
+                if (trace_tours) e.tours.download(trace_tours + static_cast<size_t>(b) * ants * n, static_cast<size_t>(ants) * n, e.stream);
+                checked(hipStreamSynchronize(e.stream), "trace batch");
+                if (trace_costs) {
+                    trace_costs[3 * b] = st.epoch_cost; trace_costs[3 * b + 1] = st.global_cost;
+                    trace_costs[3 * b + 2] = st.parent_cost;
+                }
+                if (trace_best) trace_best[b] = st.iteration_best;
+            }
+            ColonyState st;
+            e.state.download(&st, 1, e.stream);
+            if (tour) e.global.download(tour, n, e.stream);
+            checked(hipStreamSynchronize(e.stream), "readback");
+            if (cost) *cost = st.global_cost;
+            if (construction_steps) *construction_steps = st.work_construction;
+            if (ls_evaluations) *ls_evaluations = st.work_ls;
+        }
+        if (initial_cost) *initial_cost = e.problem(keyv).initial_cost;
+        if (solve_seconds) *solve_seconds = seconds_since(started);
+    });
+}
+
+// ---------------------------------------------------------------------------
+int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* ls, uint32_t lw, uint32_t ants,
+                                 const uint32_t* parents, const uint32_t* visits, const uint32_t* mne_targets,
+                                 uint64_t limit, uint32_t waves, uint32_t* construction_tours, uint32_t* tours,
+                                 uint32_t* positions, uint32_t* checklists, faco_ant_info* info) {
+    return guarded([&] {
+        require(ls && parents && visits && mne_targets && tours && info, "null argument");
+        require(ants >= 1 && lw >= 1 && lw <= kMaxLs && lw <= n - 1, "invalid ant count or LS width");
+        const auto coords = integer_coordinates(xy, n);
+        for (uint32_t a = 0; a < ants; ++a) {
+            require(is_permutation(parents + static_cast<size_t>(a) * n, n), "parent tour is not a permutation");
+            require(mne_targets[a] > 0, "mne_target must be positive");
+        }
+        for (size_t i = 0; i < static_cast<size_t>(n) * lw; ++i) require(ls[i] < n, "LS candidate out of range");
+        if (waves == 0) waves = 1;
+        require(waves <= 16 && ants % waves == 0, "waves per block must divide the ant count");
+        hipStream_t stream;
+        checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+        struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } guard{stream};
+        const uint32_t K = lw, stride = round16_host(2u * n * K) / 2u;
+        std::vector<uint32_t> packed(n);
+        for (uint32_t i = 0; i < n; ++i) packed[i] = pack_xy(coords[2 * i], coords[2 * i + 1]);
+        std::vector<uint16_t> nbr(stride, 0);
+        for (size_t i = 0; i < static_cast<size_t>(n) * lw; ++i) nbr[i] = static_cast<uint16_t>(ls[i]);
+        std::vector<uint32_t> links(static_cast<size_t>(ants) * n);
+        std::vector<int64_t> parent_costs(ants);
+        for (uint32_t a = 0; a < ants; ++a) {
+            const uint32_t* t = parents + static_cast<size_t>(a) * n;
+            int64_t c = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t prev = t[i == 0 ? n - 1 : i - 1], next = t[i + 1 == n ? 0 : i + 1];
+                links[static_cast<size_t>(a) * n + t[i]] = prev | (next << 16);
+                c += euc2d(packed[prev], packed[t[i]]);
+            }
+            parent_costs[a] = c;
+        }
+        const size_t cells = static_cast<size_t>(ants) * n;
+        DeviceArray<uint32_t> d_coords, d_parents, d_links, d_visits, d_targets, d_tours, d_ctours, d_pos, d_check;
+        DeviceArray<uint16_t> d_nbr;
+        DeviceArray<int64_t> d_costs;
+        DeviceArray<faco_ant_info> d_info;
+        d_coords.allocate(n); d_nbr.allocate(stride); d_parents.allocate(cells); d_links.allocate(cells);
+        d_visits.allocate(cells); d_targets.allocate(ants); d_tours.allocate(cells); d_ctours.allocate(cells);
+        d_pos.allocate(cells); d_check.allocate(cells * 5); d_costs.allocate(ants); d_info.allocate(ants);
+        d_coords.upload(packed.data(), n, stream); d_nbr.upload(nbr.data(), stride, stream);
+        d_parents.upload(parents, cells, stream); d_links.upload(links.data(), cells, stream);
+        d_visits.upload(visits, cells, stream); d_targets.upload(mne_targets, ants, stream);
+        d_costs.upload(parent_costs.data(), ants, stream);
+        AntParams p{};
+        p.nbr_in_lds = 1;
+        p.n = n; p.ants = ants; p.pw = 0; p.bw = 0; p.lw = lw; p.K = K; p.waves = waves;
+        p.nbr_lds_bytes = round16_host(2u * n * K); p.nbr_stride = stride; p.per_wave_bytes = ant_wave_bytes(n);
+        p.xcd_remap = 0; p.explicit_mode = 1; p.per_ant_parent = 1; p.log_checklist = 1;
+        p.ls_limit = limit;
+        p.coords = d_coords.data(); p.nbr = d_nbr.data(); p.parent = d_parents.data(); p.links = d_links.data();
+        p.parent_cost = d_costs.data(); p.targets = d_targets.data(); p.visits = d_visits.data();
+        p.out_tours = d_tours.data(); p.out_info = d_info.data(); p.diag_construction = d_ctours.data();
+        p.diag_positions = d_pos.data(); p.diag_checklist = d_check.data();
+        const size_t lds = ant_shared_bytes(n, p.nbr_lds_bytes) + static_cast<size_t>(waves) * p.per_wave_bytes;
+        if (lds > max_lds_per_block()) {
+            p.nbr_in_lds = 0;
+            require(ant_shared_bytes(n, 0) + static_cast<size_t>(waves) * p.per_wave_bytes <= max_lds_per_block(),
+                    "instance too large for the LDS budget");
+        }
+        checked(launch_ant_search(p, ants / waves, stream), "diagnostic launch");
+        d_tours.download(tours, cells, stream);
+        if (construction_tours) d_ctours.download(construction_tours, cells, stream);
+        if (positions) d_pos.download(positions, cells, stream);
+        if (checklists) d_check.download(checklists, cells * 5, stream);
+        d_info.download(info, ants, stream);
+        checked(hipStreamSynchronize(stream), "diagnostic run");
+    });
+}
+
+int faco_diagnostic_distances(const int32_t* pairs, uint32_t count, int64_t* out) {
+    return guarded([&] {
+        require(pairs && out, "null argument");
+        for (size_t i = 0; i < 4ull * count; ++i)
+            require(pairs[i] >= 0 && pairs[i] <= static_cast<int32_t>(kMaxCoordinate), "coordinate out of range");
+        DeviceArray<int32_t> d_pairs; DeviceArray<int64_t> d_out;
+        d_pairs.allocate(4ull * count); d_out.allocate(count);
+        d_pairs.upload(pairs, 4ull * count, nullptr);
+        checked(launch_distance_check(d_pairs.data(), count, d_out.data(), nullptr), "distance launch");
+        d_out.download(out, count, nullptr);
+        checked(hipDeviceSynchronize(), "distance run");
+    });
+}
+
+int faco_diagnostic_philox(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step, uint32_t words, uint32_t* out) {
+    return guarded([&] {
+        require(out && words >= 1 && words <= 4096, "invalid word count");
+        DeviceArray<uint32_t> d_out; d_out.allocate(words);
+        checked(launch_philox_check(key, batch, ant, step, words, d_out.data(), nullptr), "philox launch");
+        d_out.download(out, words, nullptr);
+        checked(hipDeviceSynchronize(), "philox run");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// Shared host/device definitions of the FACO engine: device records, the
+// Philox4x32-10 stream, the EUC_2D distance and the SplitMix64 key mix.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/faco_engine.h"
+
+#define FACO_HD __host__ __device__ __forceinline__
+
+namespace faco {
+
+// Largest candidate widths one wave handles: primary/neighbour rows are one
+// lane per slot; the 2-opt scan puts successor moves in lanes [0,32) and
+// predecessor moves in lanes [32,64).
+constexpr uint32_t kMaxPrimary = 64;
+constexpr uint32_t kMaxLs = 32;
+constexpr uint32_t kMaxBackup = 256;
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kMaxCoordinate = 32767;
+
+// Per-colony device state (the reference's Colony, cuda/colony_state.cuh:59-64,
+// with integer costs and the batch counter kept on the device so one
+// captured iteration body can be replayed).
+struct ColonyState {
+    int64_t global_cost, epoch_cost, parent_cost;
+    double minimum, maximum, default_trail, source_uniform;
+    uint32_t iteration_best, source_is_epoch;
+    uint32_t batch, pad;
+    uint64_t work_construction, work_ls;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), shared by host and device.
+FACO_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c[0];
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c[2];
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = static_cast<uint32_t>(p1);
+        c[3] = static_cast<uint32_t>(p0);
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Stream of 32-bit words for (key, batch, ant, step): counter block
+// (step, 0, ant, batch), four words per block, 128-bit counter increment.
+// The domain split follows cuda/faco_choices.cuh:32-38 of the reference.
+struct Stream {
+    uint32_t ctr[4];
+    uint32_t block[4];
+    uint32_t k0, k1;
+    uint32_t index;
+    FACO_HD Stream(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step) {
+        ctr[0] = step; ctr[1] = 0; ctr[2] = ant; ctr[3] = batch;
+        k0 = static_cast<uint32_t>(key); k1 = static_cast<uint32_t>(key >> 32);
+        index = 4;
+        block[0] = block[1] = block[2] = block[3] = 0;
+    }
+    FACO_HD uint32_t next() {
+        if (index == 4) {
+            block[0] = ctr[0]; block[1] = ctr[1]; block[2] = ctr[2]; block[3] = ctr[3];
+            philox4x32_10(block, k0, k1);
+            index = 0;
+            if (++ctr[0] == 0 && ++ctr[1] == 0 && ++ctr[2] == 0) ++ctr[3];
+        }
+        return block[index++];
+    }
+    FACO_HD double uniform53() {
+        const uint64_t hi = next(), lo = next();
+        return static_cast<double>(((hi << 32) | lo) >> 11) * 0x1.0p-53;
+    }
+};
+
+// First two words of block `step` only: the per-step roulette draw needs
+// exactly one block, so the counter never carries.
+FACO_HD double step_uniform(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step) {
+    uint32_t c[4] = {step, 0u, ant, batch};
+    philox4x32_10(c, static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32));
+    const uint64_t hi = c[0], lo = c[1];
+    return static_cast<double>(((hi << 32) | lo) >> 11) * 0x1.0p-53;
+}
+
+// Rejection-sampled start node over [0,count) in the reserved step
+// 0xffffffff (reference cuda/faco_choices.cuh:71-84).
+FACO_HD uint32_t start_node(uint64_t key, uint32_t batch, uint32_t ant, uint32_t count) {
+    Stream s(key, batch, ant, 0xffffffffu);
+    const uint32_t threshold = (0u - count) % count;
+    uint32_t value = s.next();
+    while (value < threshold) value = s.next();
+    return value % count;
+}
+
+// SplitMix64 finaliser (reference cuda/batch_engine.cu:27-32).
+FACO_HD uint64_t mixed(uint64_t value) {
+    value += 0x9e3779b97f4a7c15ULL;
+    value = (value ^ (value >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    value = (value ^ (value >> 27)) * 0x94d049bb133111ebULL;
+    return value ^ (value >> 31);
+}
+
+FACO_HD uint64_t colony_key(uint64_t instance_key, uint64_t seed) {
+    return mixed(seed ^ mixed(instance_key + 0xd1b54a32d192ed03ULL));
+}
+
+// EUC_2D on packed coordinates (x | y << 16), both in [0, 32767]:
+// s = dx^2 + dy^2 < 2^31, r = isqrt(s), d = r + (s > r^2 + r).
+// The float estimate of sqrt(s) is within 0.01 of the true root, so one
+// correction step each way gives the exact integer root.
+FACO_HD uint32_t euc2d(uint32_t a, uint32_t b) {
+    const int dx = static_cast<int>(a & 0xffffu) - static_cast<int>(b & 0xffffu);
+    const int dy = static_cast<int>(a >> 16) - static_cast<int>(b >> 16);
+    const uint32_t s = static_cast<uint32_t>(dx * dx) + static_cast<uint32_t>(dy * dy);
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_sqrtf(static_cast<float>(s)));
+#else
+    uint32_t r = static_cast<uint32_t>(__builtin_sqrtf(static_cast<float>(s)));
+#endif
+    r -= (r * r > s) ? 1u : 0u;
+    r += ((r + 1) * (r + 1) <= s) ? 1u : 0u;
+    return r + ((s > r * r + r) ? 1u : 0u);
+}
+
+FACO_HD uint32_t pack_xy(uint32_t x, uint32_t y) { return x | (y << 16); }
+
+// ---------------------------------------------------------------------------
+// Kernel parameter blocks (passed by value as kernel arguments).
+struct AntParams {
+    uint32_t nbr_in_lds;                // neighbour rows staged in LDS (else read from HBM)
+    uint32_t n, ants, pw, bw, lw, K;   // K = neighbour row width = max(pw, lw)
+    uint32_t waves;                     // ants (waves) per workgroup
+    uint32_t nbr_lds_bytes;             // neighbour rows in LDS (16-byte rounded)
+    uint32_t nbr_stride;                // u16 elements per geometry (nbr_lds_bytes / 2)
+    uint32_t per_wave_bytes;            // per-ant LDS block
+    uint32_t xcd_remap;                 // map blocks so one colony stays on one XCD
+    uint32_t explicit_mode;             // visits[] drive construction (diagnostic)
+    uint32_t per_ant_parent;            // parent/links/cost indexed by ant, not colony
+    uint32_t log_checklist;             // append every checklist push to diag_checklist
+    uint64_t ls_limit;
+    const uint32_t* coords;             // [G][n] packed x | y << 16
+    const uint16_t* nbr;                // [G][n][K] (d, id)-ordered nearest cities
+    const uint16_t* backup;             // [G][n][bw]
+    const double* products;             // [C][n][pw] trail * heuristic
+    const uint32_t* parent;             // [C][n] (or [A][n])
+    const uint32_t* links;              // [C][n] parent predecessor | successor << 16
+    const int64_t* parent_cost;         // [A] when per_ant_parent
+    const ColonyState* state;           // [C] (parent cost and batch counter)
+    const uint64_t* keys;               // [C]
+    const uint32_t* targets;            // [A] MNE targets
+    const uint32_t* geometry;           // [C] colony -> geometry, or null (all 0)
+    const uint32_t* visits;             // [A][n] explicit choices
+    uint32_t* out_tours;                // [A][n]
+    faco_ant_info* out_info;            // [A]
+    uint32_t* diag_construction;        // [A][n] or null
+    uint32_t* diag_positions;           // [A][n] or null
+    uint32_t* diag_checklist;           // [A][5n] or null
+    uint32_t* trace_selected;           // [A][n] or null
+    double* trace_uniform;              // [A][n] or null
+};
+
+struct ColonyParams {
+    uint32_t n, ants, pw, K, nbr_stride;
+    double retention, p_best_root;      // p = p_best^(1/pw), computed on the host
+    double epoch_probability;
+    const uint16_t* nbr;                // [G][n][K]
+    const double* eta;                  // [G][n][pw]
+    const uint32_t* geometry;           // [C]
+    const uint64_t* keys;               // [C]
+    const uint32_t* tours;              // [A][n]
+    const faco_ant_info* info;          // [A]
+    ColonyState* state;                 // [C]
+    uint32_t* parent;                   // [C][n]
+    uint32_t* links;                    // [C][n]
+    uint32_t* epoch;                    // [C][n]
+    uint32_t* global;                   // [C][n]
+    double* trails;                     // [C][n][pw]
+    double* products;                   // [C][n][pw]
+};
+
+struct ResetParams {
+    uint32_t n, ants, pw, K;
+    double retention, p_best_root;
+    const uint32_t* geometry;
+    const uint32_t* initial;            // [G][n]
+    const int64_t* initial_cost;        // [G]
+    const double* eta;                  // [G][n][pw]
+    ColonyState* state;
+    uint32_t* parent; uint32_t* links; uint32_t* epoch; uint32_t* global;
+    double* trails; double* products;
+};
+
+// Launch wrappers (defined in faco_kernels.hip).
+hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t stream);
+hipError_t launch_colony_update(const ColonyParams& p, uint32_t colonies, hipStream_t stream);
+hipError_t launch_colony_reset(const ResetParams& p, uint32_t colonies, hipStream_t stream);
+hipError_t launch_distance_check(const int32_t* pairs, uint32_t count, int64_t* out, hipStream_t stream);
+hipError_t launch_philox_check(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step, uint32_t words,
+                               uint32_t* out, hipStream_t stream);
+// LDS bytes of one ant-search workgroup.
+inline uint32_t round16_host(uint32_t v) { return (v + 15u) & ~15u; }
+inline uint32_t ant_shared_bytes(uint32_t n, uint32_t nbr_lds_bytes) { return round16_host(4u * n) + nbr_lds_bytes; }
+inline uint32_t ant_wave_bytes(uint32_t n) { return 3u * round16_host(2u * n) + round16_host(n); }
+
+}  // namespace faco

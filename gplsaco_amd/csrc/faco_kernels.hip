@@ -1,0 +1,584 @@
+// HIP/CDNA4 kernels of the FACO batch engine (gfx950).
+//
+// ant_search: one wave64 per ant. The instance's packed coordinates and
+//   distance-ordered neighbour rows sit in LDS once per workgroup; each wave
+//   keeps its ant's tour, inverse positions, FIFO checklist and visited/queued
+//   flags in LDS (7 bytes per city). Scalar FACO state (current city, step,
+//   MNE, queue head/tail, cost) is wave-uniform and lives in SGPRs; lanes work
+//   on the parallel parts: the 16-slot roulette (ballot availability + an
+//   in-order FP64 prefix), relocation shifts and 2-opt flips (64 elements per
+//   instruction), and the 2x20 2-opt move scan with a wave-wide
+//   (gain, lower index) maximum. Distances are EUC_2D from LDS coordinates,
+//   computed in registers (no distance table, no HBM gather).
+//   Semantics: reference cuda/faco_device.cuh:100-501 (construct_and_search),
+//   cuda/faco_choices.cuh:86-142 (StochasticChoices::next).
+//
+// colony_update: one workgroup per colony: work totals, iteration best
+//   (lowest ant on ties), epoch/global update, MMAS bounds, epoch-source coin,
+//   next parent and its predecessor/successor links, and the fused
+//   evaporate/deposit/clamp/product pheromone update over the colony's
+//   [n][primary] entries (each entry owned by one thread, no atomics).
+//   Semantics: reference cuda/colony_state.cuh:148-229.
+#include "faco_common.h"
+
+namespace faco {
+
+// ---------------------------------------------------------------------------
+// Wave helpers. Values that are identical in every lane are made explicit
+// with readfirstlane so the compiler can keep them in SGPRs.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+__device__ __forceinline__ uint32_t read_lane(uint32_t v, uint32_t lane) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
+}
+__device__ __forceinline__ double read_lane(double v, uint32_t lane) {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = read_lane(static_cast<uint32_t>(bits), lane);
+    const uint32_t hi = read_lane(static_cast<uint32_t>(bits >> 32), lane);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// LDS writes by one lane become visible to the other lanes of the same wave
+// in program order; this keeps the compiler from moving LDS accesses across
+// the hand-off points.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ uint32_t ctz64(uint64_t m) { return static_cast<uint32_t>(__builtin_ctzll(m)); }
+__device__ __forceinline__ uint32_t last64(uint64_t m) { return 63u - static_cast<uint32_t>(__builtin_clzll(m)); }
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
+    for (uint32_t i = 0; i < k; ++i) m &= m - 1;
+    return ctz64(m);
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int offset = 32; offset; offset >>= 1) {
+        const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), offset, 64));
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+    for (int offset = 32; offset; offset >>= 1) {
+        const uint64_t o = static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(v), offset, 64));
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+constexpr uint8_t kVisited = 1, kQueued = 2;
+
+
+__device__ __forceinline__ uint32_t round16(uint32_t v) { return (v + 15u) & ~15u; }
+
+template<bool NbrLds>
+__global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const uint32_t n = p.n;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = uniform(threadIdx.x >> 6);
+    uint32_t blk = blockIdx.x;
+    const uint32_t nblocks = gridDim.x;
+    if (p.xcd_remap && (nblocks & 7u) == 0) blk = (blk & 7u) * (nblocks >> 3) + (blk >> 3);
+    const uint32_t ant = blk * p.waves + wave;              // global ant index
+    const uint32_t colony = p.explicit_mode ? 0u : ant / p.ants;
+    const uint32_t local_ant = p.explicit_mode ? ant : ant % p.ants;
+    const uint32_t block_colony = p.explicit_mode ? 0u : (blk * p.waves) / p.ants;
+    const uint32_t geom = p.geometry ? p.geometry[block_colony] : 0u;
+
+    // Shared instance data: packed coordinates + neighbour rows.
+    uint32_t* s_xy = reinterpret_cast<uint32_t*>(lds);
+    const uint16_t* s_nbr;
+    {
+        const uint32_t* gxy = p.coords + static_cast<size_t>(geom) * n;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_xy[i] = gxy[i];
+        const uint16_t* g_nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
+        if constexpr (NbrLds) {
+            uint16_t* l_nbr = reinterpret_cast<uint16_t*>(lds + round16(4u * n));
+            const uint4* src = reinterpret_cast<const uint4*>(g_nbr);
+            uint4* dst = reinterpret_cast<uint4*>(l_nbr);
+            const uint32_t chunks = p.nbr_lds_bytes / 16u;
+            for (uint32_t i = threadIdx.x; i < chunks; i += blockDim.x) dst[i] = src[i];
+            s_nbr = l_nbr;
+        } else {
+            s_nbr = g_nbr;
+        }
+    }
+    unsigned char* mine = lds + round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u) + wave * p.per_wave_bytes;
+    uint16_t* tour = reinterpret_cast<uint16_t*>(mine);
+    uint16_t* pos = reinterpret_cast<uint16_t*>(mine + round16(2u * n));
+    uint16_t* queue = reinterpret_cast<uint16_t*>(mine + 2u * round16(2u * n));
+    uint8_t* flags = reinterpret_cast<uint8_t*>(mine + 3u * round16(2u * n));
+
+    const size_t parent_base = static_cast<size_t>(p.per_ant_parent ? ant : colony) * n;
+    const uint32_t* parent = p.parent + parent_base;
+    const uint32_t* links = p.links + parent_base;
+    for (uint32_t i = lane; i < n; i += kWave) {
+        const uint32_t v = parent[i];
+        tour[i] = static_cast<uint16_t>(v);
+        pos[v] = static_cast<uint16_t>(i);
+        flags[i] = 0;
+    }
+    __syncthreads();
+
+    const uint32_t pw = p.pw, K = p.K, lw = p.lw;
+    const uint64_t key = p.explicit_mode ? 0ull : p.keys[colony];
+    const uint32_t batch = p.explicit_mode ? 0u : p.state[colony].batch;
+    const double* products = p.products + static_cast<size_t>(colony) * n * pw;
+    const uint16_t* backup = p.backup + static_cast<size_t>(geom) * n * p.bw;
+    const uint32_t target = p.targets[ant];
+    const uint32_t* visits = p.visits + static_cast<size_t>(ant) * n;
+    uint32_t* checklist_log = p.diag_checklist ? p.diag_checklist + static_cast<size_t>(ant) * 5u * n : nullptr;
+
+    int64_t cost = p.per_ant_parent ? p.parent_cost[ant] : p.state[colony].parent_cost;
+    uint32_t current = p.explicit_mode ? visits[0] : start_node(key, batch, local_ant, n);
+    current = uniform(current);
+    const uint32_t start = current;
+    if (p.trace_selected && lane == 0) {
+        p.trace_selected[static_cast<size_t>(ant) * n] = current;
+        p.trace_uniform[static_cast<size_t>(ant) * n] = 0.0;
+    }
+    if (lane == 0) flags[current] = kVisited;
+    wave_sync();
+
+    uint32_t steps = 0, mne = 0, relocations = 0, exhausted = 0;
+    uint32_t qhead = 0, qtail = 0, logged = 0;
+
+    // Append `node` to the FIFO checklist unless it is already pending
+    // (reference append_if_absent with queued flags, cuda/faco_device.cuh:81-92).
+    auto push = [&](uint32_t node) -> uint32_t {
+        const uint8_t f = flags[node];
+        if (f & kQueued) return 0u;
+        if (lane == 0) {
+            flags[node] = static_cast<uint8_t>(f | kQueued);
+            queue[qtail % n] = static_cast<uint16_t>(node);
+            if (checklist_log) checklist_log[logged] = node;
+        }
+        ++qtail;
+        ++logged;
+        wave_sync();
+        return 1u;
+    };
+
+    // ------------------------------------------------------------------ construction
+    while (mne < target && steps + 1 < n) {
+        const uint32_t step = steps + 1;
+        uint32_t selected;
+        double u = 0.0;
+        if (p.explicit_mode) {
+            selected = uniform(visits[step]);
+        } else {
+            u = step_uniform(key, batch, local_ant, step);
+            const bool in_row = lane < pw;
+            const uint32_t cand = in_row ? s_nbr[current * K + lane] : n;
+            const double prod = in_row ? products[static_cast<size_t>(current) * pw + lane] : 0.0;
+            const bool avail = in_row && cand < n && !(flags[cand] & kVisited);
+            const uint64_t amask = ballot(avail);
+            if (amask) {
+                // In-order prefix sums of the available weights: lane j holds
+                // ((w0 + w1) + ...) + wj over available slots <= j, the exact
+                // left-to-right sums of select_next.
+                double acc = 0.0;
+                for (uint64_t m = amask; m; m &= m - 1) {
+                    const uint32_t i = ctz64(m);
+                    const double v = read_lane(prod, i);
+                    acc = (i <= lane) ? acc + v : acc;
+                }
+                const double total = read_lane(acc, 63);
+                if (total > 0.0) {
+                    const double threshold = u * total;
+                    const uint64_t hit = ballot(avail && threshold < acc);
+                    selected = read_lane(cand, hit ? ctz64(hit) : last64(amask));
+                } else {
+                    const uint32_t count = static_cast<uint32_t>(__builtin_popcountll(amask));
+                    uint32_t k = static_cast<uint32_t>(u * count);
+                    if (k >= count) k = count - 1;
+                    selected = read_lane(cand, nth_set_bit(amask, k));
+                }
+            } else {
+                selected = n;
+                for (uint32_t base = 0; base < p.bw && selected == n; base += kWave) {
+                    const uint32_t j = base + lane;
+                    const uint32_t b = j < p.bw ? backup[static_cast<size_t>(current) * p.bw + j] : n;
+                    const uint64_t bm = ballot(b < n && !(flags[b] & kVisited));
+                    if (bm) selected = read_lane(b, ctz64(bm));
+                }
+                if (selected == n) {
+                    // Nearest unvisited city, lowest id on ties.
+                    const uint32_t cxy = s_xy[current];
+                    uint64_t best = ~0ull;
+                    for (uint32_t v = lane; v < n; v += kWave) {
+                        if (!(flags[v] & kVisited)) {
+                            const uint64_t k = (static_cast<uint64_t>(euc2d(cxy, s_xy[v])) << 32) | v;
+                            best = k < best ? k : best;
+                        }
+                    }
+                    best = wave_min_u64(best);
+                    selected = best == ~0ull ? n : static_cast<uint32_t>(best);
+                }
+            }
+            selected = uniform(selected);
+        }
+        if (p.trace_selected && lane == 0) {
+            p.trace_selected[static_cast<size_t>(ant) * n + step] = selected;
+            p.trace_uniform[static_cast<size_t>(ant) * n + step] = u;
+        }
+        if (selected >= n) { exhausted = 1; break; }
+
+        ++steps;
+        const uint32_t ps = pos[selected], pc = pos[current];
+        const uint32_t old_prev = tour[ps == 0 ? n - 1 : ps - 1];
+        const uint32_t after = tour[ps + 1 == n ? 0 : ps + 1];
+        const uint32_t target_after = tour[pc + 1 == n ? 0 : pc + 1];
+        if (lane == 0) flags[selected] = static_cast<uint8_t>(flags[selected] | kVisited);
+        if (target_after != selected) {
+            ++relocations;
+            const uint32_t xs = s_xy[selected], xc = s_xy[current], xp = s_xy[old_prev];
+            const uint32_t xa = s_xy[after], xt = s_xy[target_after];
+            cost += static_cast<int64_t>(euc2d(xp, xa)) + euc2d(xc, xs) + euc2d(xs, xt)
+                  - static_cast<int64_t>(euc2d(xp, xs)) - euc2d(xs, xa) - euc2d(xc, xt);
+            // Move `selected` to directly after `current`; only the closed
+            // position range between the two shifts by one place.
+            if (pc < ps) {
+                const int32_t lowest = static_cast<int32_t>(pc) + 2;
+                for (int32_t hi = static_cast<int32_t>(ps); hi >= lowest; hi -= static_cast<int32_t>(kWave)) {
+                    const int32_t i = hi - static_cast<int32_t>(lane);
+                    const bool ok = i >= lowest;
+                    const uint32_t v = ok ? tour[i - 1] : 0u;
+                    if (ok) { tour[i] = static_cast<uint16_t>(v); pos[v] = static_cast<uint16_t>(i); }
+                }
+                if (lane == 0) { tour[pc + 1] = static_cast<uint16_t>(selected); pos[selected] = static_cast<uint16_t>(pc + 1); }
+            } else {
+                for (uint32_t lo = ps; lo < pc; lo += kWave) {
+                    const uint32_t i = lo + lane;
+                    const bool ok = i < pc;
+                    const uint32_t v = ok ? tour[i + 1] : 0u;
+                    if (ok) { tour[i] = static_cast<uint16_t>(v); pos[v] = static_cast<uint16_t>(i); }
+                }
+                if (lane == 0) { tour[pc] = static_cast<uint16_t>(selected); pos[selected] = static_cast<uint16_t>(pc); }
+            }
+        }
+        wave_sync();
+        const uint32_t link = links[current];
+        if (selected != (link & 0xffffu) && selected != (link >> 16)) {
+            ++mne;
+            push(current);
+            push(selected);
+            push(old_prev);
+        }
+        current = selected;
+    }
+
+    if (p.diag_construction) {
+        for (uint32_t i = lane; i < n; i += kWave) p.diag_construction[static_cast<size_t>(ant) * n + i] = tour[i];
+    }
+    const int64_t construction_cost = cost;
+
+    // ------------------------------------------------------------------ checklist 2-opt
+    // Lanes [0, lw) scan successor moves (kind 0), lanes [32, 32 + lw)
+    // predecessor moves (kind 1); slot j of kind k has tie index k*lw + j.
+    const uint32_t kind = lane >> 5, slot = lane & 31u;
+    const bool scan_lane = slot < lw;
+    const uint32_t tie_index = kind * lw + slot;
+    uint32_t processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0, limit_hit = 0;
+    int64_t gain_sum = 0;
+    const uint64_t limit = p.ls_limit;
+    while (qhead != qtail && accepted < n) {
+        if (evaluations >= limit) { limit_hit = 1; break; }
+        const uint32_t a = queue[qhead % n];
+        ++qhead;
+        if (lane == 0) flags[a] = static_cast<uint8_t>(flags[a] & ~kQueued);
+        ++processed;
+        const uint32_t pa = pos[a];
+        const uint32_t a_next = tour[pa + 1 == n ? 0 : pa + 1];
+        const uint32_t a_prev = tour[pa == 0 ? n - 1 : pa - 1];
+        const uint32_t xa = s_xy[a];
+        const int32_t d_next = static_cast<int32_t>(euc2d(xa, s_xy[a_next]));
+        const int32_t d_prev = static_cast<int32_t>(euc2d(s_xy[a_prev], xa));
+        const int32_t current_edge = kind ? d_prev : d_next;
+        const uint32_t b = scan_lane ? s_nbr[a * K + slot] : n;
+        const uint32_t xb = scan_lane && b < n ? s_xy[b] : 0u;
+        const int32_t dab = scan_lane && b < n ? static_cast<int32_t>(euc2d(xa, xb)) : 0;
+        const bool bad = !scan_lane || b >= n || !(current_edge > dab);
+        // First failing slot per kind; lanes past lw count as failing.
+        const uint64_t bad_mask = ballot(bad);
+        const uint32_t bad0 = static_cast<uint32_t>(bad_mask), bad1 = static_cast<uint32_t>(bad_mask >> 32);
+        const uint32_t prefix0 = bad0 ? min(static_cast<uint32_t>(__builtin_ctz(bad0)), lw) : lw;
+        const uint32_t prefix1 = bad1 ? min(static_cast<uint32_t>(__builtin_ctz(bad1)), lw) : lw;
+        // Budget: kind 0 first, then kind 1; the limit check precedes every
+        // evaluation and a node cut short commits nothing.
+        const uint64_t remaining0 = limit - evaluations;
+        const uint32_t used0 = remaining0 < prefix0 ? static_cast<uint32_t>(remaining0) : prefix0;
+        evaluations += used0;
+        checks += used0 + (used0 < lw ? 1u : 0u);
+        if (used0 < prefix0) { limit_hit = 1; break; }
+        const uint64_t remaining1 = limit - evaluations;
+        const uint32_t used1 = remaining1 < prefix1 ? static_cast<uint32_t>(remaining1) : prefix1;
+        evaluations += used1;
+        checks += used1 + (used1 < lw ? 1u : 0u);
+        if (used1 < prefix1) { limit_hit = 1; break; }
+
+        const bool active = scan_lane && slot < (kind ? used1 : used0);
+        uint32_t nb = 0;
+        int32_t gain = 0;
+        if (active) {
+            const uint32_t pb = pos[b];
+            nb = kind ? tour[pb == 0 ? n - 1 : pb - 1] : tour[pb + 1 == n ? 0 : pb + 1];
+            const uint32_t closing = kind ? a_prev : a_next;
+            const uint32_t xn = s_xy[nb];
+            gain = (nb == a || b == closing) ? 0
+                 : current_edge + static_cast<int32_t>(euc2d(xb, xn)) - dab
+                   - static_cast<int32_t>(euc2d(s_xy[closing], xn));
+        }
+        // key: larger gain first, then lower tie index; 0 = no candidate.
+        const uint32_t key_value = active ? ((static_cast<uint32_t>(gain + (1 << 20)) << 6) | (63u - tie_index)) : 0u;
+        const uint32_t best = uniform(wave_max_u32(key_value));
+        if (best == 0u) continue;
+        const int32_t best_gain = static_cast<int32_t>(best >> 6) - (1 << 20);
+        if (best_gain <= 0) continue;
+        const uint32_t best_index = 63u - (best & 63u);
+        const uint32_t best_kind = best_index >= lw ? 1u : 0u;
+        const uint32_t best_lane = best_kind * 32u + (best_index - best_kind * lw);
+        const uint32_t bnode = read_lane(b, best_lane);
+        const uint32_t nnode = read_lane(nb, best_lane);
+        uint32_t move[4];
+        if (best_kind == 0) { move[0] = a_next; move[1] = nnode; move[2] = a; move[3] = bnode; }
+        else { move[0] = a; move[1] = bnode; move[2] = a_prev; move[3] = nnode; }
+        const uint32_t p0 = pos[move[0]], p1 = pos[move[1]];
+        const uint32_t first = p0 < p1 ? p0 : p1, last = p0 < p1 ? p1 : p0;
+        const uint32_t length = last - first;
+        const bool straight = length <= n - length || first == 0;
+        const uint32_t count = straight ? length : n - length;
+        const uint32_t start = straight ? first : last;
+        // Reversal as disjoint pair swaps (the complement side wraps round
+        // the array end); first == 0 keeps the straight side, which is the
+        // array layout the reference's extra lap produces.
+        for (uint32_t off = lane; off < count / 2; off += kWave) {
+            uint32_t left = start + off; if (left >= n) left -= n;
+            uint32_t right = start + count - 1 - off; if (right >= n) right -= n;
+            const uint16_t l = tour[left], r = tour[right];
+            tour[left] = r; tour[right] = l;
+            pos[r] = static_cast<uint16_t>(left); pos[l] = static_cast<uint16_t>(right);
+        }
+        wave_sync();
+        ++accepted;
+        gain_sum += best_gain;
+        for (uint32_t m = 0; m < 4; ++m) reactivations += push(move[m]);
+    }
+
+    // ------------------------------------------------------------------ write back
+    uint32_t* out = p.out_tours + static_cast<size_t>(ant) * n;
+    for (uint32_t i = lane; i < n; i += kWave) out[i] = tour[i];
+    if (p.diag_positions) {
+        for (uint32_t i = lane; i < n; i += kWave) p.diag_positions[static_cast<size_t>(ant) * n + i] = pos[i];
+    }
+    if (lane == 0) {
+        faco_ant_info info;
+        info.construction_cost = construction_cost;
+        info.final_cost = cost - gain_sum;
+        info.mne = mne; info.steps = steps; info.nonidentity_relocations = relocations;
+        info.legal_exhausted = exhausted;
+        info.processed_nodes = processed; info.candidate_checks = checks;
+        info.move_evaluations = evaluations; info.accepted_moves = accepted;
+        info.reactivations = reactivations; info.evaluation_limit_reached = limit_hit;
+        info.checklist_size = logged;
+        info.start_node = start;
+        p.out_info[ant] = info;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+// MMAS bounds from the epoch cost (reference cuda/colony_state.cuh:66-70).
+__device__ __forceinline__ void bounds(ColonyState& s, uint32_t width, double retention, double p) {
+    s.maximum = 1.0 / (static_cast<double>(s.epoch_cost) * (1.0 - retention));
+    const double lo = s.maximum * (1.0 - p) / ((width - 1.0) * p);
+    s.minimum = s.maximum < lo ? s.maximum : lo;
+}
+
+__global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) {
+    const uint32_t colony = blockIdx.x, n = p.n, tid = threadIdx.x;
+    const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
+    ColonyState* st = p.state + colony;
+    const faco_ant_info* info = p.info + static_cast<size_t>(colony) * p.ants;
+    __shared__ uint64_t best_key[256];
+    __shared__ uint64_t work_c[256], work_l[256];
+    __shared__ ColonyState s;
+    extern __shared__ uint32_t s_links[];
+
+    uint64_t best = ~0ull, wc = 0, wl = 0;
+    for (uint32_t a = tid; a < p.ants; a += blockDim.x) {
+        // final costs are positive and < 2^31: (cost, ant) orders ties by ant
+        const uint64_t k = (static_cast<uint64_t>(info[a].final_cost) << 32) | a;
+        best = k < best ? k : best;
+        wc += info[a].steps;
+        wl += info[a].move_evaluations;
+    }
+    best_key[tid] = best; work_c[tid] = wc; work_l[tid] = wl;
+    __syncthreads();
+    for (uint32_t stride = blockDim.x / 2; stride; stride >>= 1) {
+        if (tid < stride) {
+            best_key[tid] = best_key[tid + stride] < best_key[tid] ? best_key[tid + stride] : best_key[tid];
+            work_c[tid] += work_c[tid + stride];
+            work_l[tid] += work_l[tid + stride];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        s = *st;
+        s.work_construction += work_c[0];
+        s.work_ls += work_l[0];
+        const uint32_t ib = static_cast<uint32_t>(best_key[0]);
+        const int64_t cost = static_cast<int64_t>(best_key[0] >> 32);
+        s.iteration_best = ib;
+        const bool improve_epoch = cost < s.epoch_cost, improve_global = cost < s.global_cost;
+        if (improve_epoch) s.epoch_cost = cost;
+        if (improve_global) s.global_cost = cost;
+        bounds(s, p.pw, p.retention, p.p_best_root);
+        Stream coin(p.keys[colony], s.batch, 0xffffffffu, 0u);
+        s.source_uniform = coin.uniform53();
+        s.source_is_epoch = s.source_uniform < p.epoch_probability ? 1u : 0u;
+        s.parent_cost = s.source_is_epoch ? s.epoch_cost : cost;
+        s.pad = (improve_epoch ? 1u : 0u) | (improve_global ? 2u : 0u);
+    }
+    __syncthreads();
+    const uint32_t improve = s.pad;
+    const uint32_t* best_tour = p.tours + (static_cast<size_t>(colony) * p.ants + s.iteration_best) * n;
+    uint32_t* epoch = p.epoch + static_cast<size_t>(colony) * n;
+    uint32_t* global = p.global + static_cast<size_t>(colony) * n;
+    uint32_t* parent = p.parent + static_cast<size_t>(colony) * n;
+    uint32_t* links = p.links + static_cast<size_t>(colony) * n;
+    if (improve & 1u) for (uint32_t i = tid; i < n; i += blockDim.x) epoch[i] = best_tour[i];
+    if (improve & 2u) for (uint32_t i = tid; i < n; i += blockDim.x) global[i] = best_tour[i];
+    __syncthreads();
+    const uint32_t* source = s.source_is_epoch ? epoch : best_tour;
+    for (uint32_t i = tid; i < n; i += blockDim.x) {
+        const uint32_t v = source[i];
+        const uint32_t pred = source[i == 0 ? n - 1 : i - 1], succ = source[i + 1 == n ? 0 : i + 1];
+        parent[i] = v;
+        s_links[v] = pred | (succ << 16);
+        links[v] = pred | (succ << 16);
+    }
+    __syncthreads();
+    // Evaporate, clamp below, deposit on parent edges, clamp above, refresh
+    // products (reference cuda/colony_state.cuh:210-229).
+    const uint32_t cells = n * p.pw;
+    double* trails = p.trails + static_cast<size_t>(colony) * cells;
+    double* products = p.products + static_cast<size_t>(colony) * cells;
+    const double* eta = p.eta + static_cast<size_t>(geom) * cells;
+    const uint16_t* nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
+    const double minimum = s.minimum, maximum = s.maximum, retention = p.retention;
+    const double increment = 1.0 / static_cast<double>(s.parent_cost);
+    for (uint32_t i = tid; i < cells; i += blockDim.x) {
+        const uint32_t from = i / p.pw, j = i - from * p.pw;
+        const uint32_t to = nbr[from * p.K + j];
+        double v = trails[i] * retention;
+        v = v > minimum ? v : minimum;
+        const uint32_t link = s_links[from];
+        if (to == (link & 0xffffu) || to == (link >> 16)) {
+            const double raised = v + increment;
+            v = raised < maximum ? raised : maximum;
+        }
+        trails[i] = v;
+        products[i] = v * eta[i];
+    }
+    if (tid == 0) {
+        const double d = s.default_trail * retention;
+        s.default_trail = d > minimum ? d : minimum;
+        s.batch += 1;
+        s.pad = 0;
+        *st = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Colony reset (reference reset_colony + initialize_products,
+// cuda/colony_state.cuh:72-110): parent = epoch = global = initial tour,
+// bounds from the initial cost, every trail at the upper bound.
+
+__global__ void __launch_bounds__(256) faco_colony_reset(const ResetParams p) {
+    const uint32_t colony = blockIdx.x, n = p.n, tid = threadIdx.x;
+    const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
+    const uint32_t* init = p.initial + static_cast<size_t>(geom) * n;
+    __shared__ ColonyState s;
+    if (tid == 0) {
+        ColonyState z = {};
+        z.global_cost = z.epoch_cost = z.parent_cost = p.initial_cost[geom];
+        bounds(z, p.pw, p.retention, p.p_best_root);
+        z.default_trail = z.maximum;
+        s = z;
+        p.state[colony] = z;
+    }
+    __syncthreads();
+    const size_t base = static_cast<size_t>(colony) * n;
+    for (uint32_t i = tid; i < n; i += blockDim.x) {
+        const uint32_t v = init[i];
+        p.parent[base + i] = p.epoch[base + i] = p.global[base + i] = v;
+        const uint32_t pred = init[i == 0 ? n - 1 : i - 1], succ = init[i + 1 == n ? 0 : i + 1];
+        p.links[base + v] = pred | (succ << 16);
+    }
+    const uint32_t cells = n * p.pw;
+    const double* eta = p.eta + static_cast<size_t>(geom) * cells;
+    for (uint32_t i = tid; i < cells; i += blockDim.x) {
+        p.trails[static_cast<size_t>(colony) * cells + i] = s.maximum;
+        p.products[static_cast<size_t>(colony) * cells + i] = s.maximum * eta[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics: device EUC_2D and Philox streams.
+__global__ void faco_distance_check(const int32_t* pairs, uint32_t count, int64_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t a = pack_xy(static_cast<uint32_t>(pairs[4 * i]), static_cast<uint32_t>(pairs[4 * i + 1]));
+    const uint32_t b = pack_xy(static_cast<uint32_t>(pairs[4 * i + 2]), static_cast<uint32_t>(pairs[4 * i + 3]));
+    out[i] = euc2d(a, b);
+}
+
+__global__ void faco_philox_check(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step,
+                                             uint32_t words, uint32_t* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Stream s(key, batch, ant, step);
+    for (uint32_t i = 0; i < words; ++i) out[i] = s.next();
+    if (words >= 2) {
+        // the single-block fast path must agree with the stream
+        const double u = step_uniform(key, batch, ant, step);
+        Stream t(key, batch, ant, step);
+        if (u != t.uniform53()) out[0] = ~out[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t stream) {
+    const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
+    if (p.nbr_in_lds) hipLaunchKernelGGL(faco_ant_search<true>, dim3(blocks), dim3(64 * p.waves), lds, stream, p);
+    else hipLaunchKernelGGL(faco_ant_search<false>, dim3(blocks), dim3(64 * p.waves), lds, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_colony_update(const ColonyParams& p, uint32_t colonies, hipStream_t stream) {
+    hipLaunchKernelGGL(faco_colony_update, dim3(colonies), dim3(256), 4u * p.n, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_colony_reset(const ResetParams& p, uint32_t colonies, hipStream_t stream) {
+    hipLaunchKernelGGL(faco_colony_reset, dim3(colonies), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_distance_check(const int32_t* pairs, uint32_t count, int64_t* out, hipStream_t stream) {
+    if (count) hipLaunchKernelGGL(faco_distance_check, dim3((count + 255) / 256), dim3(256), 0, stream, pairs, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_philox_check(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step, uint32_t words,
+                               uint32_t* out, hipStream_t stream) {
+    hipLaunchKernelGGL(faco_philox_check, dim3(1), dim3(64), 0, stream, key, batch, ant, step, words, out);
+    return hipGetLastError();
+}
+
+}  // namespace faco

@@ -1,0 +1,426 @@
+"""Drop-in Python surface of the reference's native module ``gp_faco_ext``.
+
+Names, keyword-only flags, dtype strictness and result-dict keys follow the
+reference pybind11 module (reference cpp/src/bindings.cpp:500-1046); every
+call goes through the C ABI of libfaco_hip.so (include/faco_engine.h). A
+caller switches with ``from gplsaco_amd import gp_faco_ext``.
+
+Differences from the reference, by design of this engine:
+- distances are TSPLIB EUC_2D on integer coordinates in [0, 32767] (the
+  reference uses continuous FP64); ``numeric_backend`` says so;
+- ``score_cuda`` runs the HIP kernel (the name is kept for callers).
+"""
+from __future__ import annotations
+
+import ctypes
+import struct
+import time
+
+import numpy as np
+
+from . import _native
+from ._native import Program, Result, Settings, check, lib, ptr
+
+numeric_backend = "euc2d_int32"
+campaign_interface_version = 1
+
+_SETTING_FIELDS = ("ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
+                   "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")
+
+
+class FixedFacoSettings:
+    """Read/write settings record (reference bindings.cpp:508-519)."""
+
+    def __init__(self):
+        self.ants = 0
+        self.primary_width = 16
+        self.backup_width = 64
+        self.ls_width = 20
+        self.beta = 1.0
+        self.retention = 0.5
+        self.p_best = 0.1
+        self.epoch_source_probability = 0.01
+        self.ls_evaluation_limit = 100000
+        self.initial_ls_evaluation_limit = 100000
+
+    def _native(self) -> Settings:
+        s = Settings()
+        for name in _SETTING_FIELDS:
+            value = getattr(self, name)
+            if isinstance(value, bool):
+                raise TypeError(f"{name} must be a number, not bool")
+            setattr(s, name, value)
+        return s
+
+    @classmethod
+    def _from_native(cls, s: Settings) -> "FixedFacoSettings":
+        out = cls()
+        for name in _SETTING_FIELDS:
+            setattr(out, name, getattr(s, name))
+        return out
+
+    def __repr__(self):
+        return "FixedFacoSettings(" + ", ".join(f"{k}={getattr(self, k)!r}" for k in _SETTING_FIELDS) + ")"
+
+
+def _coordinates(coordinates) -> np.ndarray:
+    if not isinstance(coordinates, np.ndarray) or coordinates.dtype != np.float64 or coordinates.ndim != 2 \
+            or coordinates.shape[1] != 2 or not coordinates.flags.c_contiguous:
+        raise TypeError("coordinates must be a C-contiguous float64 array of shape [n, 2]")
+    return coordinates
+
+
+def _keys(values, name) -> np.ndarray:
+    if not isinstance(values, np.ndarray) or values.dtype != np.uint64 or values.ndim != 1 \
+            or not values.flags.c_contiguous:
+        raise TypeError(f"{name} must be a C-contiguous uint64 array")
+    return values
+
+
+def _unsigned(value, name, bits=64) -> int:
+    if type(value) is not int:
+        raise ValueError(f"{name} must be a non-negative integer, not {type(value).__name__}")
+    if value < 0 or value >= 1 << bits:
+        raise ValueError(f"{name} out of the uint{bits} range")
+    return value
+
+
+# ---------------------------------------------------------------------------
+# Program IR (reference bindings.cpp:22-56, python/gp_faco/program_ir.py:20-130)
+
+def read_program(dictionary) -> Program:
+    if not isinstance(dictionary, dict):
+        raise TypeError("program must be a dict")
+    if len(dictionary) != 6 and not (len(dictionary) == 7 and "program_id" in dictionary):
+        raise ValueError("IR field count is wrong")
+    for name in ("ir_version", "numeric_spec_id", "feature_spec_id"):
+        if type(dictionary.get(name)) is not int:
+            raise ValueError("IR versions must be integers")
+    for name in ("opcode", "operand", "constant_bits"):
+        seq = dictionary.get(name)
+        if not isinstance(seq, (list, tuple)) or any(type(v) is not int for v in seq):
+            raise ValueError("IR opcodes and bit patterns must be integers")
+    opcodes, operands, constants = dictionary["opcode"], dictionary["operand"], dictionary["constant_bits"]
+    if not opcodes or len(opcodes) > 63 or len(operands) != len(opcodes) or len(constants) > 63:
+        raise ValueError("IR array size is wrong")
+    if any(not 0 <= v < 256 for v in list(opcodes) + list(operands)) or any(not 0 <= v < 1 << 32 for v in constants):
+        raise ValueError("IR values out of range")
+    for name in ("ir_version", "numeric_spec_id", "feature_spec_id"):
+        if not 0 <= dictionary[name] < 1 << 16:
+            raise ValueError("IR versions out of range")
+    p = Program()
+    p.ir_version = dictionary["ir_version"]
+    p.numeric_spec_id = dictionary["numeric_spec_id"]
+    p.feature_spec_id = dictionary["feature_spec_id"]
+    p.length = len(opcodes)
+    p.constants_count = len(constants)
+    for i, (op, operand) in enumerate(zip(opcodes, operands)):
+        p.opcode[i] = op
+        p.operand[i] = operand
+    for i, bits in enumerate(constants):
+        p.constants[i] = struct.unpack("<f", struct.pack("<I", bits))[0]
+    return p
+
+
+def _score(program, features, masks, entry):
+    p = read_program(program)
+    if not isinstance(features, np.ndarray) or features.dtype != np.float32 or features.ndim != 3 \
+            or features.shape[0] != 12 or features.shape[2] != 32 or not features.flags.c_contiguous:
+        raise TypeError("features must be C-contiguous float32 [12, colonies, 32]")
+    if not isinstance(masks, np.ndarray) or masks.dtype != np.uint32 or masks.ndim != 1 \
+            or masks.shape[0] != features.shape[1] or not masks.flags.c_contiguous:
+        raise TypeError("masks must be C-contiguous uint32 [colonies]")
+    colonies = masks.shape[0]
+    scores = np.zeros((colonies, 32), dtype=np.float32)
+    actions = np.zeros(colonies, dtype=np.int32)
+    check(getattr(lib(), entry)(ctypes.byref(p), features.reshape(-1), masks, colonies, scores.reshape(-1), actions))
+    return {"scores": scores, "actions": actions}
+
+
+def score_cpu(program, features, masks):
+    """Host scorer (reference cpp/src/program_cpu.cpp:79-116)."""
+    return _score(program, features, masks, "faco_score_cpu")
+
+
+def score_cuda(program, features, masks):
+    """HIP scorer (reference cuda/gp_score.cu:52-77); bit-identical to score_cpu."""
+    return _score(program, features, masks, "faco_score_gpu")
+
+
+score_hip = score_cuda
+
+
+# ---------------------------------------------------------------------------
+
+def prepare_common_initial(coordinates, settings: FixedFacoSettings) -> dict:
+    xy = _coordinates(coordinates)
+    n = xy.shape[0]
+    tour = np.zeros(n, dtype=np.uint32)
+    cost = ctypes.c_int64()
+    cheap, prep = ctypes.c_double(), ctypes.c_double()
+    norm = Settings()
+    check(lib().faco_prepare_common_initial(xy.reshape(-1), n, ctypes.byref(settings._native()), ctypes.byref(norm),
+                                            tour, ctypes.byref(cost), ctypes.byref(cheap), ctypes.byref(prep),
+                                            None, None, None, None))
+    return {"tour": tour.tolist(), "cost": float(cost.value), "cheap_seconds": cheap.value,
+            "preparation_seconds": prep.value}
+
+
+def prepare_rows(coordinates, settings: FixedFacoSettings) -> dict:
+    """Candidate rows and initial tour of the host preparation (as arrays)."""
+    xy = _coordinates(coordinates)
+    n = xy.shape[0]
+    norm = Settings()
+    tour = np.zeros(n, dtype=np.uint32)
+    cost = ctypes.c_int64()
+    cheap, prep = ctypes.c_double(), ctypes.c_double()
+    s = settings._native()
+    primary = np.zeros(n * max(1, s.primary_width), dtype=np.uint32)
+    backup = np.zeros(n * max(1, s.backup_width), dtype=np.uint32)
+    ls = np.zeros(n * max(1, s.ls_width), dtype=np.uint32)
+    scale = np.zeros(n, dtype=np.float64)
+    check(lib().faco_prepare_common_initial(xy.reshape(-1), n, ctypes.byref(s), ctypes.byref(norm), tour,
+                                            ctypes.byref(cost), ctypes.byref(cheap), ctypes.byref(prep),
+                                            ptr(primary), ptr(backup), ptr(ls), ptr(scale)))
+    return {"settings": FixedFacoSettings._from_native(norm),
+            "primary": primary[: n * norm.primary_width].reshape(n, norm.primary_width),
+            "backup": backup[: n * norm.backup_width].reshape(n, norm.backup_width),
+            "ls": ls[: n * norm.ls_width].reshape(n, norm.ls_width),
+            "local_scale": scale, "initial_tour": tour, "initial_cost": int(cost.value)}
+
+
+def cuda_device_info() -> dict:
+    device, cus = ctypes.c_int(), ctypes.c_int()
+    mem, lds = ctypes.c_size_t(), ctypes.c_size_t()
+    name, arch = ctypes.create_string_buffer(256), ctypes.create_string_buffer(64)
+    check(lib().faco_device_info(ctypes.byref(device), name, 256, arch, 64, ctypes.byref(cus), ctypes.byref(mem),
+                                 ctypes.byref(lds)))
+    return {"device": device.value, "name": name.value.decode(), "arch": arch.value.decode(),
+            "compute_units": cus.value, "total_memory": mem.value, "max_lds_per_block": lds.value}
+
+
+def _batch_output(result: Result, tours: np.ndarray, costs: np.ndarray, mode: str, elapsed: float) -> dict:
+    items = [{"has_incumbent": True, "tour": tours[i].tolist(), "cost": float(costs[i]),
+              "completed_seconds": elapsed} for i in range(tours.shape[0])]
+    return {
+        "items": items,
+        "budget_seconds": None,
+        "budget_kind": "search_tour_evaluations",
+        "evaluation_limit_per_colony": result.evaluation_limit_per_colony,
+        "completed_tour_evaluations_per_colony": result.completed_tour_evaluations_per_colony,
+        "total_tour_evaluations": result.total_tour_evaluations,
+        "elapsed_seconds": result.actual_seconds,
+        "actual_seconds": result.actual_seconds,
+        "charged_seconds": 0.0,
+        "last_batch_completed_seconds": elapsed,
+        "overrun_seconds": 0.0,
+        "launched_batches": result.launched_batches,
+        "completed_batches": result.completed_batches,
+        "discarded_batches": 0,
+        "completed_construction_steps": result.completed_construction_steps,
+        "completed_ls_evaluations": result.completed_ls_evaluations,
+        "allocated_device_bytes": result.allocated_device_bytes,
+        "preparation_completed": True,
+        "preparation_mode": mode,
+        "search_seconds": result.search_seconds,
+        "search_gpu_milliseconds": result.search_gpu_milliseconds,
+    }
+
+
+class FacoBatchEngine:
+    """Fixed-shape multi-colony engine (reference batch_engine.hpp:68-145)."""
+
+    def __init__(self, dimension, colonies, settings, constraint_mode="unrestricted", population_size=1):
+        if type(dimension) is not int or type(colonies) is not int:
+            raise ValueError("dimension/colonies must be integers")
+        population = _unsigned(population_size, "population_size", 32)
+        if constraint_mode not in ("unrestricted", "hard", "escape"):
+            raise ValueError("unknown constraint mode")
+        if constraint_mode != "unrestricted":
+            raise ValueError("only the unrestricted constraint mode is available in this engine")
+        if not isinstance(settings, FixedFacoSettings):
+            raise TypeError("settings must be FixedFacoSettings")
+        handle = ctypes.c_void_p()
+        check(lib().faco_engine_create(dimension, colonies, ctypes.byref(settings._native()), 0, population,
+                                       ctypes.byref(handle)))
+        self._handle = handle
+        self.dimension, self.colonies, self.population_size = dimension, colonies, population
+        s = Settings()
+        check(lib().faco_engine_settings(self._handle, ctypes.byref(s)))
+        self.settings = FixedFacoSettings._from_native(s)
+        self._charges = {}
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle:
+            lib().faco_engine_destroy(handle)
+            self._handle = None
+
+    def launch_shape(self) -> dict:
+        w, lds_rows, lds = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().faco_engine_launch_shape(self._handle, ctypes.byref(w), ctypes.byref(lds_rows), ctypes.byref(lds)))
+        return {"waves_per_block": w.value, "neighbour_rows_in_lds": bool(lds_rows.value), "lds_bytes": lds.value}
+
+    def set_waves_per_block(self, waves: int) -> None:
+        check(lib().faco_engine_set_waves(self._handle, _unsigned(waves, "waves", 32)))
+
+    def register_problem(self, instance_key, coordinates) -> dict:
+        if type(instance_key) is not int:
+            raise ValueError("instance key must be an integer")
+        key = _unsigned(instance_key, "instance_key")
+        xy = _coordinates(coordinates)
+        cheap, prep = ctypes.c_double(), ctypes.c_double()
+        check(lib().faco_engine_register_problem(self._handle, key, xy.reshape(-1), xy.shape[0], ctypes.byref(cheap),
+                                                 ctypes.byref(prep)))
+        return {"cheap_seconds": cheap.value, "preparation_seconds": prep.value}
+
+    def preparation_profile(self, instance_key) -> dict:
+        out = np.zeros(4, dtype=np.float64)
+        check(lib().faco_engine_preparation_profile(self._handle, _unsigned(instance_key, "instance_key"), out))
+        return {"candidates_and_scales_seconds": out[0], "nearest_neighbor_seconds": out[1],
+                "initial_ls_seconds": out[2], "finalization_seconds": out[3]}
+
+    def set_preparation_charges(self, instance_key, cheap_seconds, preparation_seconds) -> None:
+        key = _unsigned(instance_key, "instance_key")
+        for v in (cheap_seconds, preparation_seconds):
+            if type(v) not in (int, float) or not np.isfinite(v) or v < 0:
+                raise ValueError("frozen charges must be finite non-negative numbers")
+        self.preparation_profile(key)  # raises when not registered
+        old = self._charges.setdefault(key, (float(cheap_seconds), float(preparation_seconds)))
+        if old != (float(cheap_seconds), float(preparation_seconds)):
+            raise ValueError("frozen preparation charges of one instance can not change")
+
+    def problem_data(self, instance_key) -> dict:
+        n, s = self.dimension, self.settings
+        primary = np.zeros(n * s.primary_width, dtype=np.uint32)
+        backup = np.zeros(max(1, n * s.backup_width), dtype=np.uint32)
+        ls = np.zeros(n * s.ls_width, dtype=np.uint32)
+        tour = np.zeros(n, dtype=np.uint32)
+        cost = ctypes.c_int64()
+        check(lib().faco_engine_problem_data(self._handle, _unsigned(instance_key, "instance_key"), ptr(primary),
+                                             ptr(backup), ptr(ls), ptr(tour), ctypes.byref(cost)))
+        return {"primary": primary.reshape(n, -1), "backup": backup[: n * s.backup_width].reshape(n, -1),
+                "ls": ls.reshape(n, -1), "initial_tour": tour, "initial_cost": int(cost.value)}
+
+    def evaluate_faco_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, mne=8,
+                                  preparation_mode="cached", *, checkpoint_iterations=(), profile_events_only=False,
+                                  use_graph=True, profile_kernels=False) -> dict:
+        keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
+        if keys.size != seeds.size:
+            raise ValueError("FACO count entry needs complete task arrays")
+        if preparation_mode not in ("cached", "end_to_end"):
+            raise ValueError("unknown preparation mode")
+        if checkpoint_iterations:
+            raise ValueError("checkpoint iterations are not available in this engine")
+        limit = _unsigned(evaluation_limit_per_colony, "evaluation_limit_per_colony")
+        mne = _unsigned(mne, "mne", 32)
+        tasks = keys.size
+        tours = np.zeros((tasks, self.dimension), dtype=np.uint32)
+        costs = np.zeros(tasks, dtype=np.int64)
+        work = np.zeros((tasks, 2), dtype=np.uint64)
+        result = Result()
+        started = time.perf_counter()
+        flags = (1 if use_graph else 0) | (2 if profile_kernels else 0)
+        check(lib().faco_engine_evaluate_faco(self._handle, keys, seeds, tasks, limit, mne, flags,
+                                              ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1)))
+        out = _batch_output(result, tours, costs, preparation_mode, time.perf_counter() - started)
+        out["colony_work"] = work
+        if profile_kernels:
+            out["kernel_milliseconds"] = {"construction_and_search": result.kernel_milliseconds[0],
+                                          "reduction_and_pheromone": result.kernel_milliseconds[1]}
+        return out
+
+
+class FixedFacoGpu:
+    """Single-instance fixed-iteration FACO (reference fixed_faco_gpu.hpp:44-60)."""
+
+    def __init__(self, coordinates, instance_key, settings):
+        xy = _coordinates(coordinates)
+        if not isinstance(settings, FixedFacoSettings):
+            raise TypeError("settings must be FixedFacoSettings")
+        handle = ctypes.c_void_p()
+        check(lib().faco_fixed_create(xy.reshape(-1), xy.shape[0], _unsigned(instance_key, "instance_key"),
+                                      ctypes.byref(settings._native()), ctypes.byref(handle)))
+        self._handle = handle
+        self.dimension = xy.shape[0]
+        s = Settings()
+        check(lib().faco_fixed_settings(self._handle, ctypes.byref(s)))
+        self.settings = FixedFacoSettings._from_native(s)
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle:
+            lib().faco_fixed_destroy(handle)
+            self._handle = None
+
+    def run_iterations(self, seed, batches, mne_target, *, trace=False) -> dict:
+        seed = _unsigned(seed, "seed")
+        batches = _unsigned(batches, "batches", 32)
+        mne = _unsigned(mne_target, "mne_target", 32)
+        n, ants, pw = self.dimension, self.settings.ants, self.settings.primary_width
+        tour = np.zeros(n, dtype=np.uint32)
+        cost, initial = ctypes.c_int64(), ctypes.c_int64()
+        steps, evals, seconds = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        tr = {}
+        if trace:
+            tr = {
+                "parent": np.zeros((batches, n), dtype=np.uint32),
+                "trails": np.zeros((batches, n * pw), dtype=np.float64),
+                "costs": np.zeros((batches, 3), dtype=np.int64),
+                "iteration_best": np.zeros(batches, dtype=np.uint32),
+                "ants": np.zeros((batches, ants), dtype=_native.ANT_INFO_DTYPE),
+                "selected": np.zeros((batches, ants, n), dtype=np.uint32),
+                "uniforms": np.zeros((batches, ants, n), dtype=np.float64),
+                "tours": np.zeros((batches, ants, n), dtype=np.uint32),
+            }
+        check(lib().faco_fixed_run(self._handle, seed, batches, mne, tour, ctypes.byref(cost), ctypes.byref(initial),
+                                   ctypes.byref(steps), ctypes.byref(evals), ctypes.byref(seconds),
+                                   ptr(tr.get("parent")), ptr(tr.get("trails")), ptr(tr.get("costs")),
+                                   ptr(tr.get("iteration_best")), ptr(tr.get("ants")), ptr(tr.get("selected")),
+                                   ptr(tr.get("uniforms")), ptr(tr.get("tours"))))
+        out = {"tour": tour.tolist(), "cost": float(cost.value), "initial_cost": float(initial.value),
+               "preparation_seconds": 0.0, "solve_seconds": seconds.value, "batches": batches,
+               "construction_steps": steps.value, "ls_evaluations": evals.value,
+               "allocated_device_bytes": 0}
+        if trace:
+            out["trace"] = tr
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Diagnostics (parity harness; reference cuda/faco_operations.cu:60-132)
+
+def diagnostic_construct_ls(coordinates, ls_rows, parents, visits, mne_targets, evaluation_limit, waves_per_block=1):
+    xy = _coordinates(coordinates)
+    n = xy.shape[0]
+    ls_rows = np.ascontiguousarray(ls_rows, dtype=np.uint32)
+    parents = np.ascontiguousarray(parents, dtype=np.uint32)
+    visits = np.ascontiguousarray(visits, dtype=np.uint32)
+    targets = np.ascontiguousarray(mne_targets, dtype=np.uint32)
+    ants = parents.shape[0]
+    if parents.shape != (ants, n) or visits.shape != (ants, n) or targets.shape != (ants,):
+        raise ValueError("parents/visits must be [ants, n] and mne_targets [ants]")
+    out = {k: np.zeros((ants, n), dtype=np.uint32) for k in ("construction_tours", "tours", "positions")}
+    checklists = np.zeros((ants, 5 * n), dtype=np.uint32)
+    info = np.zeros(ants, dtype=_native.ANT_INFO_DTYPE)
+    check(lib().faco_diagnostic_construct_ls(xy.reshape(-1), n, ls_rows.reshape(-1), ls_rows.shape[1], ants,
+                                             parents.reshape(-1), visits.reshape(-1), targets,
+                                             _unsigned(evaluation_limit, "evaluation_limit"), waves_per_block,
+                                             out["construction_tours"].reshape(-1), out["tours"].reshape(-1),
+                                             out["positions"].reshape(-1), checklists.reshape(-1), ptr(info)))
+    out["checklists"] = checklists
+    out["info"] = info
+    return out
+
+
+def diagnostic_distances(pairs) -> np.ndarray:
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 4)
+    out = np.zeros(pairs.shape[0], dtype=np.int64)
+    check(lib().faco_diagnostic_distances(pairs.reshape(-1), pairs.shape[0], out))
+    return out
+
+
+def diagnostic_philox(key, batch, ant, step, words=8) -> np.ndarray:
+    out = np.zeros(words, dtype=np.uint32)
+    check(lib().faco_diagnostic_philox(key, batch, ant, step, words, out))
+    return out

@@ -1,0 +1,212 @@
+/*
+ * faco_engine.h - C ABI of the MI355X FACO batch engine (libfaco_hip.so).
+ *
+ * This is the drop-in boundary for the reference's Python->native module
+ * gp_faco_ext (reference cpp/src/bindings.cpp:500-1046). Every entry point
+ * takes plain pointers and sizes, owns no caller memory and reports errors as
+ *   FACO_OK (0), FACO_INVALID_ARGUMENT (1) -> Python ValueError,
+ *   FACO_RUNTIME_ERROR (2) -> Python RuntimeError,
+ * with a thread-local message from faco_last_error(). This mirrors the
+ * reference's std::invalid_argument / std::runtime_error split
+ * (cuda/colony_state.cuh:10-15).
+ *
+ * Distances: TSPLIB EUC_2D, nint(sqrt(dx^2+dy^2)), on integer coordinates in
+ * [0, 32767]. Coordinates cross the boundary as FP64 [n][2] like the
+ * reference's; non-integral or out-of-range values are rejected.
+ */
+#ifndef FACO_ENGINE_H
+#define FACO_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FACO_OK 0
+#define FACO_INVALID_ARGUMENT 1
+#define FACO_RUNTIME_ERROR 2
+
+/* FixedFacoSettings, same field order and 64-byte layout
+ * (reference cpp/include/gp_faco/fixed_faco_gpu.hpp:10-21). ants == 0 means
+ * 64 * ceil(sqrt(n) / 16) (reference cpp/src/prepared_problem.cpp:19-33). */
+typedef struct faco_settings {
+    uint32_t ants;
+    uint32_t primary_width;
+    uint32_t backup_width;
+    uint32_t ls_width;
+    double beta;
+    double retention;
+    double p_best;
+    double epoch_source_probability;
+    uint64_t ls_evaluation_limit;
+    uint64_t initial_ls_evaluation_limit;
+} faco_settings;
+
+/* Per-ant diagnostic record, the counters of FacoDiagnosticInfo
+ * (reference cpp/include/gp_faco/faco_cuda_diagnostic.hpp:17-25,
+ * cpp/include/gp_faco/faco_cpu.hpp:34-42,70-75). Costs are exact integers. */
+typedef struct faco_ant_info {
+    int64_t construction_cost;
+    int64_t final_cost;
+    uint32_t mne;
+    uint32_t steps;
+    uint32_t nonidentity_relocations;
+    uint32_t legal_exhausted;
+    uint32_t processed_nodes;
+    uint32_t candidate_checks;
+    uint32_t move_evaluations;
+    uint32_t accepted_moves;
+    uint32_t reactivations;
+    uint32_t evaluation_limit_reached;
+    uint32_t checklist_size;
+    uint32_t start_node;
+} faco_ant_info;
+
+/* Batch result summary, the scalar fields of the reference's batch_output
+ * dict (reference cpp/src/bindings.cpp:277-410). */
+#define FACO_STAGE_COUNT 9
+typedef struct faco_result {
+    uint64_t evaluation_limit_per_colony;
+    uint64_t completed_tour_evaluations_per_colony;
+    uint64_t total_tour_evaluations;
+    uint64_t launched_batches;
+    uint64_t completed_batches;
+    uint64_t completed_construction_steps;
+    uint64_t completed_ls_evaluations;
+    uint64_t allocated_device_bytes;
+    double actual_seconds;
+    double search_seconds;       /* host wall clock of the iteration loop */
+    double search_gpu_milliseconds; /* device time of the iteration loop (HIP events) */
+    double kernel_milliseconds[FACO_STAGE_COUNT]; /* filled when profiling is on */
+} faco_result;
+
+/* Program IR v1 record (reference cpp/include/gp_faco/program.hpp:10-19). */
+typedef struct faco_program {
+    uint16_t ir_version;
+    uint16_t numeric_spec_id;
+    uint16_t feature_spec_id;
+    uint16_t length;
+    uint16_t constants_count;
+    uint8_t opcode[63];
+    uint8_t operand[63];
+    float constants[63];
+} faco_program;
+
+typedef struct faco_engine faco_engine;
+typedef struct faco_fixed faco_fixed;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* faco_last_error(void);
+
+/* Device query (reference cpp/src/bindings.cpp:1031-1044, cuda_device_info). */
+int faco_device_info(int* device, char* name, size_t name_size, char* arch, size_t arch_size,
+                     int* compute_units, size_t* total_memory, size_t* max_lds_per_block);
+
+/* prepare_common_initial(coordinates, settings) (reference
+ * cpp/src/bindings.cpp:522-540): host-only preparation of one instance.
+ * Outputs the initial tour [n] and cost, the normalised settings and, when
+ * not NULL, the candidate rows primary [n][pw], backup [n][bw], ls [n][lw]
+ * and local_scale [n]. Needs no GPU. */
+int faco_prepare_common_initial(const double* xy, uint32_t n, const faco_settings* settings,
+                                faco_settings* normalized, uint32_t* tour, int64_t* cost,
+                                double* cheap_seconds, double* preparation_seconds, uint32_t* primary,
+                                uint32_t* backup, uint32_t* ls, double* local_scale);
+
+/* FacoBatchEngine(dimension, colonies, settings, constraint_mode, population_size)
+ * (reference cpp/include/gp_faco/batch_engine.hpp:68-70; bindings.cpp:557-573).
+ * constraint_mode: 0 unrestricted. population_size in [1, 128]. */
+int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_settings* settings,
+                       uint32_t constraint_mode, uint32_t population_size, faco_engine** out);
+void faco_engine_destroy(faco_engine* engine);
+/* The normalised settings the engine runs with. */
+int faco_engine_settings(const faco_engine* engine, faco_settings* out);
+
+/* Launch shape of the ant kernel: ants (waves) per workgroup, whether the
+ * neighbour rows are staged in LDS, and the workgroup's LDS bytes. No
+ * counterpart in the reference (its warps per block is a compile-time
+ * choice, CMakeLists.txt:13-19); set_waves(0) restores the automatic shape. */
+int faco_engine_launch_shape(const faco_engine* engine, uint32_t* waves, uint32_t* nbr_in_lds, uint32_t* lds_bytes);
+int faco_engine_set_waves(faco_engine* engine, uint32_t waves);
+
+/* register_problem(instance_key, coordinates FP64[n][2])
+ * (reference batch_engine.hpp:72, batch_engine.cu:279-295). Host preparation:
+ * k-NN rows, NN tour, initial checklist 2-opt. */
+int faco_engine_register_problem(faco_engine* engine, uint64_t instance_key, const double* xy,
+                                 uint32_t n, double* cheap_seconds, double* preparation_seconds);
+/* preparation_profile: candidates_and_scales, nearest_neighbor, initial_ls,
+ * finalization seconds (reference bindings.cpp:639-650). */
+int faco_engine_preparation_profile(faco_engine* engine, uint64_t instance_key, double out[4]);
+/* Prepared data of a registered instance (for host callers that mirror
+ * FixedFacoGpu::primary_candidates etc.). Any pointer may be NULL. */
+int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_t* primary,
+                             uint32_t* backup, uint32_t* ls, uint32_t* initial_tour, int64_t* initial_cost);
+
+/* Flags of faco_engine_evaluate_faco. */
+#define FACO_RUN_GRAPH 1u   /* replay the iteration body through a captured HIP graph */
+#define FACO_RUN_PROFILE 2u /* eager launches with HIP events around each kernel:
+                               kernel_milliseconds[0] = construction + LS, [1] = colony update */
+
+/* evaluate_faco_evaluations(keys, seeds, evaluation_limit_per_colony, mne)
+ * (reference batch_engine.hpp:87-89, bindings.cpp:701-720): plain FACO on
+ * `tasks` colonies for evaluation_limit / ants batches. Outputs per colony
+ * the global-best tour [tasks][n] and cost [tasks]; `work` (may be NULL)
+ * receives [tasks][2] construction steps and LS evaluations. */
+int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                              uint32_t tasks, uint64_t evaluation_limit_per_colony, uint32_t mne_target,
+                              uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs,
+                              uint64_t* work);
+
+/* FixedFacoGpu(coordinates, instance_key, settings).run_iterations(seed,
+ * batches, mne_target) (reference cpp/include/gp_faco/fixed_faco_gpu.hpp:44-60,
+ * cuda/fixed_faco.cu:105-177). The optional per-batch trace receives, for
+ * batch b: parent after the batch [b][n], trails after the batch
+ * [b][n*primary_width], (epoch, global, parent) costs [b][3], iteration best
+ * [b], the per-ant records [b][ants] and the selection trace: selected nodes
+ * and uniforms [b][ants][n] (index 0 = start node). Pass NULL to skip. */
+int faco_fixed_create(const double* xy, uint32_t n, uint64_t instance_key, const faco_settings* settings,
+                      faco_fixed** out);
+void faco_fixed_destroy(faco_fixed* fixed);
+int faco_fixed_settings(const faco_fixed* fixed, faco_settings* out);
+int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t mne_target,
+                   uint32_t* tour, int64_t* cost, int64_t* initial_cost, uint64_t* construction_steps,
+                   uint64_t* ls_evaluations, double* solve_seconds,
+                   uint32_t* trace_parent, double* trace_trails, int64_t* trace_costs,
+                   uint32_t* trace_best, faco_ant_info* trace_ants, uint32_t* trace_selected,
+                   double* trace_uniforms, uint32_t* trace_tours);
+
+/* Explicit-choice construction + checklist 2-opt for `ants` independent ants
+ * on one instance: the parity harness of cuda_faco_diagnostic (reference
+ * cuda/faco_operations.cu:60-132). parents and visits are [ants][n]
+ * (visits[a][0] = start node); ls is [n][ls_width], distance ordered.
+ * Outputs: construction tours, final tours, final positions [ants][n],
+ * checklists [ants][5n] (the first info.checklist_size entries are valid). */
+int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* ls, uint32_t ls_width,
+                                 uint32_t ants, const uint32_t* parents, const uint32_t* visits,
+                                 const uint32_t* mne_targets, uint64_t evaluation_limit,
+                                 uint32_t waves_per_block, uint32_t* construction_tours, uint32_t* tours,
+                                 uint32_t* positions, uint32_t* checklists, faco_ant_info* info);
+
+/* EUC_2D of `count` coordinate pairs on the device (distance kernel check). */
+int faco_diagnostic_distances(const int32_t* pairs /* [count][4] */, uint32_t count, int64_t* out);
+
+/* Philox words of the device generator for (key, batch, ant, step): the
+ * first `words` 32-bit outputs of the stream (generator check). */
+int faco_diagnostic_philox(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step, uint32_t words,
+                           uint32_t* out);
+
+/* score_cpu / score_cuda (reference cpp/include/gp_faco/program.hpp:37-42,
+ * cpp/src/program_cpu.cpp:79-116, cuda/gp_score.cu:52-77): features are FP32
+ * [12][colonies][32], masks u32 [colonies]; outputs scores [colonies][32] and
+ * actions [colonies]. */
+int faco_score_cpu(const faco_program* program, const float* features, const uint32_t* masks,
+                   uint32_t colonies, float* scores, int32_t* actions);
+int faco_score_gpu(const faco_program* program, const float* features, const uint32_t* masks,
+                   uint32_t colonies, float* scores, int32_t* actions);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FACO_ENGINE_H */

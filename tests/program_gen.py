@@ -1,0 +1,43 @@
+"""Random IR v1 programs (postfix, depth-bounded) for scoring tests."""
+import struct
+
+import numpy as np
+
+ARITY = {2: 2, 3: 2, 4: 2, 5: 2, 6: 2, 7: 1, 8: 2}
+
+
+def random_tree(rng, depth):
+    """Postfix list of (opcode, operand, constant or None)."""
+    if depth == 0 or rng.random() < 0.3:
+        if rng.random() < 0.7:
+            return [(0, int(rng.integers(0, 12)), None)]
+        return [(1, 0, float(np.float32(rng.uniform(-2, 2))))]
+    op = int(rng.choice(list(ARITY)))
+    out = []
+    for _ in range(ARITY[op]):
+        out += random_tree(rng, depth - 1)
+    return out + [(op, 0, None)]
+
+
+def stack_peak(nodes):
+    top = peak = 0
+    for op, _, _ in nodes:
+        top = top + 1 if op in (0, 1) else top - ARITY[op] + 1
+        peak = max(peak, top)
+    return peak
+
+
+def random_program(rng, depth=5, feature_spec_id=1):
+    while True:
+        nodes = random_tree(rng, depth)
+        if len(nodes) <= 63 and stack_peak(nodes) <= 6:
+            break
+    opcode, operand, constants = [], [], []
+    for op, arg, const in nodes:
+        if op == 1:
+            arg = len(constants)
+            constants.append(struct.unpack("<I", struct.pack("<f", const))[0])
+        opcode.append(op)
+        operand.append(arg)
+    return {"ir_version": 1, "numeric_spec_id": 1, "feature_spec_id": feature_spec_id,
+            "opcode": opcode, "operand": operand, "constant_bits": constants}
