@@ -38,32 +38,35 @@ def algorithmic_bytes_per_tour(n: int, steps_per_tour: float, evals_per_tour: fl
     return 8.0 * n + 192.0 * steps_per_tour + 16.0 * evals_per_tour
 
 
-def cpu_baseline(xys, keys, seeds, ants, budget_seconds=15.0):
+def cpu_baseline(xys, keys, seeds, ants, iterations, budget_seconds=20.0):
+    """The oracle's C++ FACO (OpenMP over colony x ant) on the host cores, on
+    the GPU workload's own instances and iteration count; colonies are cut
+    to keep the sample near `budget_seconds`."""
     from oracle import oracle
     threads = oracle.max_threads()
     settings = oracle.default_settings(ants=ants)
-    colonies = min(len(seeds), max(threads, 8))
-    xy = np.stack(xys[:colonies]).astype(np.int32)
-    probe = oracle.run_faco(xy, keys[:colonies], seeds[:colonies], settings, 1, 8, threads=threads)
-    per_batch = max(probe["search_seconds"], 1e-3)
-    batches = int(max(1, min(200, budget_seconds / per_batch)))
-    run = oracle.run_faco(xy, keys[:colonies], seeds[:colonies], settings, batches, 8, threads=threads)
-    tours = colonies * ants * batches
+    xy = np.stack(xys).astype(np.int32)
+    probe_colonies = min(len(seeds), threads)
+    probe = oracle.run_faco(xy[:probe_colonies], keys[:probe_colonies], seeds[:probe_colonies], settings,
+                            min(iterations, 10), 8, threads=threads)
+    per_colony = probe["search_seconds"] / probe_colonies * iterations / min(iterations, 10)
+    colonies = int(max(1, min(len(seeds), budget_seconds / max(per_colony, 1e-6))))
+    run = oracle.run_faco(xy[:colonies], keys[:colonies], seeds[:colonies], settings, iterations, 8, threads=threads)
+    tours = colonies * ants * iterations
     return {"value": tours / run["search_seconds"], "unit": "ant-tours/s", "cores": threads, "kind": "port",
-            "sample": f"oracle C++ FACO (OpenMP over colony x ant), TSP1000 uniform, {colonies} colonies x "
-                      f"{ants} ants x {batches} iterations = {tours} tours in {run['search_seconds']:.2f} s; "
-                      f"host nproc={os.cpu_count()}"}
+            "sample": f"oracle C++ FACO (OpenMP over colony x ant, -O3), the same TSP1000 instances, {colonies} "
+                      f"colonies x {ants} ants x {iterations} iterations = {tours} tours in "
+                      f"{run['search_seconds']:.2f} s of search; host nproc={os.cpu_count()}"}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1000)
     ap.add_argument("--colonies", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine's colonies are independent and every run here uses a "
@@ -98,11 +101,13 @@ def main() -> int:
     steps_per_tour = timed["completed_construction_steps"] / tours
     evals_per_tour = timed["completed_ls_evaluations"] / tours
 
-    prof = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.profile_steps, use_graph=False,
+    # Same iterations again, eagerly, with HIP events around every launch on
+    # the engine's stream: the average launch time of each kernel.
+    prof = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=False,
                                             profile_kernels=True)
     ptours = prof["total_tour_evaluations"]
-    launch_ms = prof["kernel_milliseconds"]["construction_and_search"] / args.profile_steps
-    colony_ms = prof["kernel_milliseconds"]["reduction_and_pheromone"] / args.profile_steps
+    launch_ms = prof["kernel_milliseconds"]["construction_and_search"] / args.steps
+    colony_ms = prof["kernel_milliseconds"]["reduction_and_pheromone"] / args.steps
     bytes_per_tour = algorithmic_bytes_per_tour(n, prof["completed_construction_steps"] / ptours,
                                                 prof["completed_ls_evaluations"] / ptours)
     tours_per_launch = colonies * ants
@@ -141,11 +146,12 @@ def main() -> int:
             "tours_per_launch": tours_per_launch,
             "launch_ms": launch_ms,
             "colony_update_ms": colony_ms,
+            "mean_ant_phase_ms_per_step": {k: v / args.steps for k, v in prof["ant_phase_milliseconds"].items()},
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(xys, keys, seeds, ants)
+        line["cpu_baseline"] = cpu_baseline(xys, keys, seeds, ants, args.steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
     return 0

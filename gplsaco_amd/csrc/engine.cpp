@@ -265,8 +265,16 @@ struct Engine {
     // Eager iterations with HIP events around each kernel on the engine
     // stream; ms[0] += construction+LS, ms[1] += colony update.
     void run_profiled(uint32_t colonies, uint64_t iterations, double* ms) {
-        const AntParams ap = ant_params(colonies);
+        AntParams ap = ant_params(colonies);
         const ColonyParams cp = colony_params();
+        const size_t ants_total = static_cast<size_t>(colonies) * config.ants;
+        DeviceArray<uint64_t> ticks;
+        ticks.allocate(3 * ants_total);
+        ap.phase_ticks = ticks.data();
+        std::vector<uint64_t> host(3 * ants_total);
+        int clock_khz = 0;
+        checked(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
+        const double ms_per_tick = clock_khz > 0 ? 1.0 / clock_khz : 1e-5;
         hipEvent_t ev[3];
         for (auto& e : ev) checked(hipEventCreate(&e), "hipEventCreate");
         struct Destroy { hipEvent_t* e; ~Destroy() { for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]); } } d{ev};
@@ -282,6 +290,13 @@ struct Engine {
             checked(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
             checked(hipEventElapsedTime(&b, ev[1], ev[2]), "event time");
             ms[0] += a; ms[1] += b;
+            ticks.download(host.data(), host.size(), stream);
+            checked(hipStreamSynchronize(stream), "phase ticks");
+            double phase[3] = {0, 0, 0};
+            for (size_t k = 0; k < ants_total; ++k)
+                for (int j = 0; j < 3; ++j) phase[j] += static_cast<double>(host[3 * k + j]);
+            // mean per-ant milliseconds of setup, construction and local search
+            for (int j = 0; j < 3; ++j) ms[2 + j] += phase[j] * ms_per_tick / ants_total;
         }
     }
     // Run `iterations` batches; through a cached HIP graph of up to 16

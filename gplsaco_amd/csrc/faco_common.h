@@ -162,6 +162,7 @@ struct AntParams {
     uint32_t* diag_checklist;           // [A][5n] or null
     uint32_t* trace_selected;           // [A][n] or null
     double* trace_uniform;              // [A][n] or null
+    uint64_t* phase_ticks;              // [A][3] setup, construction, local search (wall clock ticks) or null
 };
 
 struct ColonyParams {

@@ -78,6 +78,7 @@ __device__ __forceinline__ uint32_t round16(uint32_t v) { return (v + 15u) & ~15
 template<bool NbrLds>
 __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const uint64_t tick0 = p.phase_ticks ? wall_clock64() : 0;
     const uint32_t n = p.n;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = uniform(threadIdx.x >> 6);
@@ -147,6 +148,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 
     uint32_t steps = 0, mne = 0, relocations = 0, exhausted = 0;
     uint32_t qhead = 0, qtail = 0, logged = 0;
+    const uint64_t tick1 = p.phase_ticks ? wall_clock64() : 0;
 
     // Append `node` to the FIFO checklist unless it is already pending
     // (reference append_if_absent with queued flags, cuda/faco_device.cuh:81-92).
@@ -277,6 +279,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         for (uint32_t i = lane; i < n; i += kWave) p.diag_construction[static_cast<size_t>(ant) * n + i] = tour[i];
     }
     const int64_t construction_cost = cost;
+    const uint64_t tick2 = p.phase_ticks ? wall_clock64() : 0;
 
     // ------------------------------------------------------------------ checklist 2-opt
     // Lanes [0, lw) scan successor moves (kind 0), lanes [32, 32 + lw)
@@ -370,6 +373,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         for (uint32_t m = 0; m < 4; ++m) reactivations += push(move[m]);
     }
 
+    const uint64_t tick3 = p.phase_ticks ? wall_clock64() : 0;
     // ------------------------------------------------------------------ write back
     uint32_t* out = p.out_tours + static_cast<size_t>(ant) * n;
     for (uint32_t i = lane; i < n; i += kWave) out[i] = tour[i];
@@ -388,6 +392,11 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         info.checklist_size = logged;
         info.start_node = start;
         p.out_info[ant] = info;
+        if (p.phase_ticks) {
+            p.phase_ticks[3 * static_cast<size_t>(ant)] = tick1 - tick0;
+            p.phase_ticks[3 * static_cast<size_t>(ant) + 1] = tick2 - tick1;
+            p.phase_ticks[3 * static_cast<size_t>(ant) + 2] = tick3 - tick2;
+        }
     }
 }
 

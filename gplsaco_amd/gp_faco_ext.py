@@ -328,6 +328,9 @@ class FacoBatchEngine:
         if profile_kernels:
             out["kernel_milliseconds"] = {"construction_and_search": result.kernel_milliseconds[0],
                                           "reduction_and_pheromone": result.kernel_milliseconds[1]}
+            out["ant_phase_milliseconds"] = {"setup": result.kernel_milliseconds[2],
+                                             "construction": result.kernel_milliseconds[3],
+                                             "local_search": result.kernel_milliseconds[4]}
         return out
 
 

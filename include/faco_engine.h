@@ -146,7 +146,10 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_
 /* Flags of faco_engine_evaluate_faco. */
 #define FACO_RUN_GRAPH 1u   /* replay the iteration body through a captured HIP graph */
 #define FACO_RUN_PROFILE 2u /* eager launches with HIP events around each kernel:
-                               kernel_milliseconds[0] = construction + LS, [1] = colony update */
+                               kernel_milliseconds[0] = construction + LS, [1] = colony update,
+                               summed over iterations; [2..4] = mean per-ant setup, construction
+                               and local-search milliseconds (device wall clock), summed over
+                               iterations */
 
 /* evaluate_faco_evaluations(keys, seeds, evaluation_limit_per_colony, mne)
  * (reference batch_engine.hpp:87-89, bindings.cpp:701-720): plain FACO on

@@ -214,8 +214,11 @@ def test_tsp1000_full_size_properties(ext, oracle_mod):
         engine.register_problem(k, xy)
         initial.append(engine.problem_data(k)["initial_cost"])
     announce("tsp1000", instances)
+    ants = engine.settings.ants
+    assert ants == 128
     got = engine.evaluate_faco_evaluations(np.array(keys, dtype=np.uint64), np.arange(instances, dtype=np.uint64),
-                                           s.ants * 20, profile_kernels=True)
+                                           ants * 20, profile_kernels=True)
+    assert got["completed_batches"] == 20
     for c, item in enumerate(got["items"]):
         tour = np.array(item["tour"])
         assert np.array_equal(np.sort(tour), np.arange(n))
