@@ -1,10 +1,13 @@
 """Headline benchmark: ant tours/s of plain FACO at TSP1000 on one MI355X.
 
 Workload: 32 uniform random Euclidean TSP1000 instances (integer coordinates
-in [0,10000]^2, EUC_2D), one colony each, 128 ants per colony, MNE 8,
+in [0,10000]^2, EUC_2D) x 8 seeds = 256 colonies, 128 ants per colony, MNE 8,
 16/64/20 candidate widths, 2-opt checklist local search with the 100000
-evaluation limit. A step is one FACO iteration of every colony: construction
-+ local search for all 4096 ants, then selection and the pheromone update.
+evaluation limit, 100 iterations from the prepared initial tour. A step is one
+FACO iteration of every colony: construction + local search for all 32768
+ants, then selection and the pheromone update. The same run with one seed
+(32 colonies, the batch shape of the reference's own TSP1000 profile row) is
+reported beside it.
 Inputs (instances, candidate rows, pheromone) are resident in HBM before the
 timed region; the timed region is the iteration loop between two stream
 synchronisations, replayed through a HIP graph.
@@ -65,8 +68,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1000)
-    ap.add_argument("--colonies", type=int, default=32)
+    ap.add_argument("--instances", type=int, default=32)
+    ap.add_argument("--seeds", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the bench, run the calibration copy kernel over this many MiB (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine's colonies are independent and every run here uses a "
@@ -76,10 +82,10 @@ def main() -> int:
     from gplsaco_amd import gp_faco_ext as ext
     from gplsaco_amd.instances import instance_key, uniform_instance
 
-    n, colonies = args.n, args.colonies
-    xys = [uniform_instance(n, seed=0, instance=i) for i in range(colonies)]
-    keys = [instance_key(n, 0, i) for i in range(colonies)]
-    seeds = list(range(colonies))
+    n, instances = args.n, args.instances
+    colonies = instances * args.seeds
+    xys = [uniform_instance(n, seed=0, instance=i) for i in range(instances)]
+    keys = [instance_key(n, 0, i) for i in range(instances)]
     settings = ext.FixedFacoSettings()
     engine = ext.FacoBatchEngine(n, colonies, settings)
     ants = engine.settings.ants
@@ -87,10 +93,10 @@ def main() -> int:
     for k, xy in zip(keys, xys):
         engine.register_problem(k, xy)
     prep_seconds = time.perf_counter() - prep_started
-    task_keys = np.array(keys, dtype=np.uint64)
-    task_seeds = np.array(seeds, dtype=np.uint64)
-    print(f"# prepared {colonies} x TSP{n} in {prep_seconds:.2f} s; shape {engine.launch_shape()}", file=sys.stderr,
-          flush=True)
+    task_keys = np.array([keys[c % instances] for c in range(colonies)], dtype=np.uint64)
+    task_seeds = np.array([c // instances for c in range(colonies)], dtype=np.uint64)
+    print(f"# prepared {instances} x TSP{n} in {prep_seconds:.2f} s; shape {engine.launch_shape()}",
+          file=sys.stderr, flush=True)
     if args.warmup:
         engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.warmup)
     timed = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=True)
@@ -114,6 +120,14 @@ def main() -> int:
     achieved = bytes_per_tour * tours_per_launch / (launch_ms * 1e-3) / 1e9
     mean_cost = float(np.mean([item["cost"] for item in timed["items"]]))
 
+    # The reference's profile shape: 32 colonies (one seed per instance).
+    small = ext.FacoBatchEngine(n, instances, settings)
+    for k, xy in zip(keys, xys):
+        small.register_problem(k, xy)
+    small_run = small.evaluate_faco_evaluations(np.array(keys, dtype=np.uint64),
+                                                np.zeros(instances, dtype=np.uint64), ants * args.steps)
+    small_value = small_run["total_tour_evaluations"] / small_run["search_seconds"]
+
     line = {
         "metric": "ant tours/s (construct+2-opt/Or-opt) at TSP1000; gap vs plain FACO at equal FE",
         "value": value,
@@ -128,11 +142,13 @@ def main() -> int:
         "dtype": "int32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FACO (no GP, fixed MNE 8), {colonies} uniform random EUC_2D TSP{n} instances, "
-                        f"one colony each, {ants} ants, 16/64/20 candidates, checklist 2-opt",
-            "n": n, "colonies": colonies, "ants": ants, "mne": 8,
+            "workload": f"plain FACO (no GP, fixed MNE 8), {instances} uniform random EUC_2D TSP{n} instances x "
+                        f"{args.seeds} seeds = {colonies} colonies, {ants} ants each, 16/64/20 candidates, "
+                        f"checklist 2-opt, {args.steps} iterations",
+            "n": n, "colonies": colonies, "ants": ants, "mne": 8, "iterations": args.steps,
+            "value_at_32_colonies": small_value,
             "construction_steps_per_tour": steps_per_tour, "ls_evaluations_per_tour": evals_per_tour,
-            "mean_final_tour_length": mean_cost, "waves_per_block": engine.launch_shape()["waves_per_block"],
+            "mean_final_tour_length": mean_cost, "launch_shape": engine.launch_shape(colonies),
         },
         "roofline": {
             "bound": "hbm",
@@ -147,12 +163,18 @@ def main() -> int:
             "launch_ms": launch_ms,
             "colony_update_ms": colony_ms,
             "mean_ant_phase_ms_per_step": {k: v / args.steps for k, v in prof["ant_phase_milliseconds"].items()},
+            "mean_ant_subphase_megaclocks_per_step": {k: v / args.steps
+                                                      for k, v in prof["ant_subphase_megaclocks"].items()},
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(xys, keys, seeds, ants, args.steps)
+        line["cpu_baseline"] = cpu_baseline(xys, keys, list(range(instances)), ants, args.steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    if args.copy_mib:
+        copy_ms = ext.diagnostic_copy(args.copy_mib << 20, 4)
+        line["calibration_copy"] = {"bytes_read": args.copy_mib << 20, "bytes_written": args.copy_mib << 20,
+                                    "ms": copy_ms, "GB/s": 2 * (args.copy_mib << 20) / copy_ms / 1e6}
     print(json.dumps(line), flush=True)
     return 0
 

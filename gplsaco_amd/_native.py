@@ -64,7 +64,7 @@ class AntInfo(ctypes.Structure):
 ANT_INFO_DTYPE = np.dtype([(name, np.int64 if t is ctypes.c_int64 else np.uint32) for name, t in AntInfo._fields_])
 assert ANT_INFO_DTYPE.itemsize == ctypes.sizeof(AntInfo)
 
-STAGE_COUNT = 9
+STAGE_COUNT = 12
 
 
 class Result(ctypes.Structure):
@@ -114,9 +114,8 @@ SIGNATURES = {
                                           ctypes.c_uint32, ctypes.POINTER(vp)]),
     "faco_engine_destroy": (None, [vp]),
     "faco_engine_settings": (ctypes.c_int, [vp, ctypes.POINTER(Settings)]),
-    "faco_engine_launch_shape": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
-                                                ctypes.POINTER(ctypes.c_uint32)]),
-    "faco_engine_set_waves": (ctypes.c_int, [vp, ctypes.c_uint32]),
+    "faco_engine_launch_shape": (ctypes.c_int, [vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 5),
+    "faco_engine_set_waves": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32]),
     "faco_engine_register_problem": (ctypes.c_int, [vp, ctypes.c_uint64, f64p, ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "faco_engine_preparation_profile": (ctypes.c_int, [vp, ctypes.c_uint64, f64p]),
@@ -132,9 +131,12 @@ SIGNATURES = {
                                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp, vp, vp, vp, vp]),
     "faco_diagnostic_construct_ls": (ctypes.c_int, [f64p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
-                                                    u32p, u32p, u32p, ctypes.c_uint64, ctypes.c_uint32, u32p, u32p,
-                                                    u32p, u32p, vp]),
+                                                    u32p, u32p, u32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                                    u32p, u32p, u32p, u32p, vp]),
     "faco_diagnostic_distances": (ctypes.c_int, [i32p, ctypes.c_uint32, i64p]),
+    "faco_diagnostic_start_nodes": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                                   ctypes.c_uint32, ctypes.c_uint32, u32p]),
+    "faco_diagnostic_copy": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "faco_diagnostic_philox": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_uint32, u32p]),
     "faco_score_cpu": (ctypes.c_int, [ctypes.POINTER(Program), f32p, u32p, ctypes.c_uint32, f32p, i32p]),

@@ -131,7 +131,8 @@ struct Engine {
     hipStream_t stream = nullptr;
     size_t allocated = 0;
     LaunchShape shape;
-    uint32_t forced_waves = 0, xcd_remap = 1;
+    uint32_t forced_waves = 0, forced_blocks_per_colony = 0, xcd_remap = 1;
+    int compute_units = 1;
     double p_root = 0;
     // device state
     DeviceGeometry geo;
@@ -158,6 +159,7 @@ struct Engine {
         K = std::max(config.primary_width, config.ls_width);
         nbr_stride = round16_host(2u * n * K) / 2u;
         device = current_device();
+        checked(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device), "CU count");
         p_root = std::pow(config.p_best, 1.0 / config.primary_width);
         shape = choose_shape(n, K, config.ants, 0);
         checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -179,9 +181,10 @@ struct Engine {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    void set_waves(uint32_t waves) {
+    void set_waves(uint32_t waves, uint32_t blocks_per_colony) {
         shape = choose_shape(n, K, config.ants, waves);
         forced_waves = waves;
+        forced_blocks_per_colony = blocks_per_colony;
         drop_graph();
     }
     void drop_graph() {
@@ -229,6 +232,16 @@ struct Engine {
         p.nbr_lds_bytes = shape.nbr_lds_bytes; p.nbr_stride = nbr_stride;
         p.per_wave_bytes = ant_wave_bytes(n);
         p.xcd_remap = xcd_remap;
+        // Split each colony over just enough workgroups to fill every CU:
+        // fewer, larger chunks let the waves of a workgroup balance uneven
+        // ants among themselves.
+        uint32_t bpc = 1;
+        const uint64_t slots = static_cast<uint64_t>(compute_units) * ant_search_blocks_per_cu(p);
+        while (static_cast<uint64_t>(colonies) * bpc < slots && config.ants % (2 * bpc * shape.waves) == 0) bpc *= 2;
+        if (forced_blocks_per_colony && config.ants % (forced_blocks_per_colony * shape.waves) == 0)
+            bpc = forced_blocks_per_colony;
+        p.blocks_per_colony = bpc;
+        p.chunk = config.ants / bpc;
         p.ls_limit = config.ls_evaluation_limit;
         p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
         p.products = products.data(); p.parent = parent.data(); p.links = links.data();
@@ -258,7 +271,7 @@ struct Engine {
         checked(launch_colony_reset(r, colonies, stream), "colony reset launch");
     }
     void iteration(uint32_t colonies, const AntParams& ap, const ColonyParams& cp) {
-        const uint32_t blocks = colonies * config.ants / shape.waves;
+        const uint32_t blocks = colonies * ap.blocks_per_colony;
         checked(launch_ant_search(ap, blocks, stream), "ant search launch");
         checked(launch_colony_update(cp, colonies, stream), "colony update launch");
     }
@@ -269,16 +282,16 @@ struct Engine {
         const ColonyParams cp = colony_params();
         const size_t ants_total = static_cast<size_t>(colonies) * config.ants;
         DeviceArray<uint64_t> ticks;
-        ticks.allocate(3 * ants_total);
+        ticks.allocate(8 * ants_total);
         ap.phase_ticks = ticks.data();
-        std::vector<uint64_t> host(3 * ants_total);
+        std::vector<uint64_t> host(8 * ants_total);
         int clock_khz = 0;
         checked(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
         const double ms_per_tick = clock_khz > 0 ? 1.0 / clock_khz : 1e-5;
         hipEvent_t ev[3];
         for (auto& e : ev) checked(hipEventCreate(&e), "hipEventCreate");
         struct Destroy { hipEvent_t* e; ~Destroy() { for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]); } } d{ev};
-        const uint32_t blocks = colonies * config.ants / shape.waves;
+        const uint32_t blocks = colonies * ap.blocks_per_colony;
         for (uint64_t i = 0; i < iterations; ++i) {
             checked(hipEventRecord(ev[0], stream), "event");
             checked(launch_ant_search(ap, blocks, stream), "ant search launch");
@@ -292,11 +305,13 @@ struct Engine {
             ms[0] += a; ms[1] += b;
             ticks.download(host.data(), host.size(), stream);
             checked(hipStreamSynchronize(stream), "phase ticks");
-            double phase[3] = {0, 0, 0};
+            double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             for (size_t k = 0; k < ants_total; ++k)
-                for (int j = 0; j < 3; ++j) phase[j] += static_cast<double>(host[3 * k + j]);
+                for (int j = 0; j < 8; ++j) phase[j] += static_cast<double>(host[8 * k + j]);
             // mean per-ant milliseconds of setup, construction and local search
             for (int j = 0; j < 3; ++j) ms[2 + j] += phase[j] * ms_per_tick / ants_total;
+            // mean per-ant shader clocks (millions) of the five sub-phases
+            for (int j = 3; j < 8; ++j) ms[2 + j] += phase[j] / ants_total * 1e-6;
         }
     }
     // Run `iterations` batches; through a cached HIP graph of up to 16
@@ -472,14 +487,18 @@ int faco_engine_settings(const faco_engine* engine, faco_settings* out) {
     return guarded([&] { require(engine && out, "null argument"); *out = engine->impl->config; });
 }
 
-int faco_engine_set_waves(faco_engine* engine, uint32_t waves) {
-    return guarded([&] { require(engine, "null argument"); engine->impl->set_waves(waves); });
+int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony) {
+    return guarded([&] { require(engine, "null argument"); engine->impl->set_waves(waves, blocks_per_colony); });
 }
 
-int faco_engine_launch_shape(const faco_engine* engine, uint32_t* waves, uint32_t* nbr_in_lds, uint32_t* lds_bytes) {
+int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint32_t* waves, uint32_t* nbr_in_lds,
+                             uint32_t* lds_bytes, uint32_t* blocks_per_colony, uint32_t* blocks_per_cu) {
     return guarded([&] {
         require(engine, "null argument");
         const auto& e = *engine->impl;
+        const AntParams ap = e.ant_params(colonies ? colonies : e.panel);
+        if (blocks_per_colony) *blocks_per_colony = ap.blocks_per_colony;
+        if (blocks_per_cu) *blocks_per_cu = static_cast<uint32_t>(ant_search_blocks_per_cu(ap));
         if (waves) *waves = e.shape.waves;
         if (nbr_in_lds) *nbr_in_lds = e.shape.nbr_in_lds;
         if (lds_bytes) *lds_bytes = ant_shared_bytes(e.n, e.shape.nbr_in_lds ? e.shape.nbr_lds_bytes : 0u) +
@@ -635,7 +654,7 @@ int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t 
 // ---------------------------------------------------------------------------
 int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* ls, uint32_t lw, uint32_t ants,
                                  const uint32_t* parents, const uint32_t* visits, const uint32_t* mne_targets,
-                                 uint64_t limit, uint32_t waves, uint32_t* construction_tours, uint32_t* tours,
+                                 uint64_t limit, uint32_t waves, uint32_t chunk, uint32_t* construction_tours, uint32_t* tours,
                                  uint32_t* positions, uint32_t* checklists, faco_ant_info* info) {
     return guarded([&] {
         require(ls && parents && visits && mne_targets && tours && info, "null argument");
@@ -647,7 +666,8 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         }
         for (size_t i = 0; i < static_cast<size_t>(n) * lw; ++i) require(ls[i] < n, "LS candidate out of range");
         if (waves == 0) waves = 1;
-        require(waves <= 16 && ants % waves == 0, "waves per block must divide the ant count");
+        if (chunk == 0) chunk = waves;
+        require(waves <= 16 && chunk >= 1 && ants % chunk == 0, "ants per workgroup must divide the ant count");
         hipStream_t stream;
         checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
         struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } guard{stream};
@@ -683,6 +703,7 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         AntParams p{};
         p.nbr_in_lds = 1;
         p.n = n; p.ants = ants; p.pw = 0; p.bw = 0; p.lw = lw; p.K = K; p.waves = waves;
+        p.chunk = chunk; p.blocks_per_colony = 1;
         p.nbr_lds_bytes = round16_host(2u * n * K); p.nbr_stride = stride; p.per_wave_bytes = ant_wave_bytes(n);
         p.xcd_remap = 0; p.explicit_mode = 1; p.per_ant_parent = 1; p.log_checklist = 1;
         p.ls_limit = limit;
@@ -696,7 +717,7 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
             require(ant_shared_bytes(n, 0) + static_cast<size_t>(waves) * p.per_wave_bytes <= max_lds_per_block(),
                     "instance too large for the LDS budget");
         }
-        checked(launch_ant_search(p, ants / waves, stream), "diagnostic launch");
+        checked(launch_ant_search(p, ants / chunk, stream), "diagnostic launch");
         d_tours.download(tours, cells, stream);
         if (construction_tours) d_ctours.download(construction_tours, cells, stream);
         if (positions) d_pos.download(positions, cells, stream);
@@ -717,6 +738,43 @@ int faco_diagnostic_distances(const int32_t* pairs, uint32_t count, int64_t* out
         checked(launch_distance_check(d_pairs.data(), count, d_out.data(), nullptr), "distance launch");
         d_out.download(out, count, nullptr);
         checked(hipDeviceSynchronize(), "distance run");
+    });
+}
+
+int faco_diagnostic_start_nodes(uint64_t key, uint32_t batch, uint32_t ant, const uint32_t* counts, uint32_t n,
+                                uint32_t on_device, uint32_t* out) {
+    return guarded([&] {
+        require(counts && out, "null argument");
+        for (uint32_t i = 0; i < n; ++i) require(counts[i] >= 1, "count must be positive");
+        if (!on_device) {
+            for (uint32_t i = 0; i < n; ++i) out[i] = start_node(key, batch, ant, counts[i]);
+            return;
+        }
+        DeviceArray<uint32_t> d_counts, d_out;
+        d_counts.allocate(n); d_out.allocate(n);
+        d_counts.upload(counts, n, nullptr);
+        checked(launch_start_check(key, batch, ant, d_counts.data(), n, d_out.data(), nullptr), "start launch");
+        d_out.download(out, n, nullptr);
+        checked(hipDeviceSynchronize(), "start run");
+    });
+}
+
+int faco_diagnostic_copy(uint64_t bytes, uint32_t repeats, double* milliseconds) {
+    return guarded([&] {
+        require(bytes >= 16 && bytes % 16 == 0 && bytes <= (1ull << 34) && repeats >= 1, "invalid copy size");
+        DeviceArray<uint8_t> a, b;
+        a.allocate(bytes); b.allocate(bytes);
+        checked(hipMemset(a.data(), 1, bytes), "memset");
+        hipEvent_t e0, e1;
+        checked(hipEventCreate(&e0), "event"); checked(hipEventCreate(&e1), "event");
+        checked(hipEventRecord(e0, nullptr), "event");
+        for (uint32_t r = 0; r < repeats; ++r) checked(launch_copy(a.data(), b.data(), bytes, nullptr), "copy launch");
+        checked(hipEventRecord(e1, nullptr), "event");
+        checked(hipEventSynchronize(e1), "copy");
+        float ms = 0;
+        checked(hipEventElapsedTime(&ms, e0, e1), "event time");
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        if (milliseconds) *milliseconds = ms / repeats;
     });
 }
 

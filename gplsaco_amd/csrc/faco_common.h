@@ -89,12 +89,27 @@ FACO_HD double step_uniform(uint64_t key, uint32_t batch, uint32_t ant, uint32_t
 
 // Rejection-sampled start node over [0,count) in the reserved step
 // 0xffffffff (reference cuda/faco_choices.cuh:71-84).
+// The first word is accepted unless it falls below (2^32 mod count), so the
+// stream is continued only on that rare path; words are taken in the
+// stream's order, four per counter block.
 FACO_HD uint32_t start_node(uint64_t key, uint32_t batch, uint32_t ant, uint32_t count) {
-    Stream s(key, batch, ant, 0xffffffffu);
     const uint32_t threshold = (0u - count) % count;
-    uint32_t value = s.next();
-    while (value < threshold) value = s.next();
-    return value % count;
+    const uint32_t k0 = static_cast<uint32_t>(key), k1 = static_cast<uint32_t>(key >> 32);
+    uint32_t c[4] = {0xffffffffu, 0u, ant, batch};
+    philox4x32_10(c, k0, k1);
+    if (c[0] >= threshold) return c[0] % count;
+    uint32_t ctr[4] = {0xffffffffu, 0u, ant, batch};
+    uint32_t w1 = c[1], w2 = c[2], w3 = c[3];
+    for (;;) {
+        if (w1 >= threshold) return w1 % count;
+        if (w2 >= threshold) return w2 % count;
+        if (w3 >= threshold) return w3 % count;
+        if (++ctr[0] == 0 && ++ctr[1] == 0 && ++ctr[2] == 0) ++ctr[3];
+        uint32_t b[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+        philox4x32_10(b, k0, k1);
+        if (b[0] >= threshold) return b[0] % count;
+        w1 = b[1]; w2 = b[2]; w3 = b[3];
+    }
 }
 
 // SplitMix64 finaliser (reference cuda/batch_engine.cu:27-32).
@@ -113,18 +128,24 @@ FACO_HD uint64_t colony_key(uint64_t instance_key, uint64_t seed) {
 // s = dx^2 + dy^2 < 2^31, r = isqrt(s), d = r + (s > r^2 + r).
 // The float estimate of sqrt(s) is within 0.01 of the true root, so one
 // correction step each way gives the exact integer root.
+// Every product below fits 24 bits of operand (|dx|, |dy| <= 32767 and
+// r <= 46341), so the device uses full-rate 24-bit multiplies.
 FACO_HD uint32_t euc2d(uint32_t a, uint32_t b) {
     const int dx = static_cast<int>(a & 0xffffu) - static_cast<int>(b & 0xffffu);
     const int dy = static_cast<int>(a >> 16) - static_cast<int>(b >> 16);
-    const uint32_t s = static_cast<uint32_t>(dx * dx) + static_cast<uint32_t>(dy * dy);
 #if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t s = static_cast<uint32_t>(__mul24(dx, dx)) + static_cast<uint32_t>(__mul24(dy, dy));
     uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_sqrtf(static_cast<float>(s)));
+    r -= (__umul24(r, r) > s) ? 1u : 0u;
+    r += (__umul24(r + 1, r + 1) <= s) ? 1u : 0u;
+    return r + ((s > __umul24(r, r) + r) ? 1u : 0u);
 #else
+    const uint32_t s = static_cast<uint32_t>(dx * dx) + static_cast<uint32_t>(dy * dy);
     uint32_t r = static_cast<uint32_t>(__builtin_sqrtf(static_cast<float>(s)));
-#endif
     r -= (r * r > s) ? 1u : 0u;
     r += ((r + 1) * (r + 1) <= s) ? 1u : 0u;
     return r + ((s > r * r + r) ? 1u : 0u);
+#endif
 }
 
 FACO_HD uint32_t pack_xy(uint32_t x, uint32_t y) { return x | (y << 16); }
@@ -134,7 +155,9 @@ FACO_HD uint32_t pack_xy(uint32_t x, uint32_t y) { return x | (y << 16); }
 struct AntParams {
     uint32_t nbr_in_lds;                // neighbour rows staged in LDS (else read from HBM)
     uint32_t n, ants, pw, bw, lw, K;   // K = neighbour row width = max(pw, lw)
-    uint32_t waves;                     // ants (waves) per workgroup
+    uint32_t waves;                     // waves per workgroup
+    uint32_t chunk;                     // ants of one colony handled by one workgroup
+    uint32_t blocks_per_colony;         // workgroups per colony (explicit mode: unused)
     uint32_t nbr_lds_bytes;             // neighbour rows in LDS (16-byte rounded)
     uint32_t nbr_stride;                // u16 elements per geometry (nbr_lds_bytes / 2)
     uint32_t per_wave_bytes;            // per-ant LDS block
@@ -162,7 +185,8 @@ struct AntParams {
     uint32_t* diag_checklist;           // [A][5n] or null
     uint32_t* trace_selected;           // [A][n] or null
     double* trace_uniform;              // [A][n] or null
-    uint64_t* phase_ticks;              // [A][3] setup, construction, local search (wall clock ticks) or null
+    uint64_t* phase_ticks;              // [A][8] or null: setup, construction, LS (wall clock ticks);
+                                        // select, relocate, mne/push, LS scan, LS apply (shader clocks)
 };
 
 struct ColonyParams {
@@ -198,6 +222,10 @@ struct ResetParams {
 
 // Launch wrappers (defined in faco_kernels.hip).
 hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t stream);
+int ant_search_blocks_per_cu(const AntParams& p);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
+hipError_t launch_start_check(uint64_t key, uint32_t batch, uint32_t ant, const uint32_t* counts, uint32_t n,
+                              uint32_t* out, hipStream_t stream);
 hipError_t launch_colony_update(const ColonyParams& p, uint32_t colonies, hipStream_t stream);
 hipError_t launch_colony_reset(const ResetParams& p, uint32_t colonies, hipStream_t stream);
 hipError_t launch_distance_check(const int32_t* pairs, uint32_t count, int64_t* out, hipStream_t stream);

@@ -256,13 +256,16 @@ class FacoBatchEngine:
             lib().faco_engine_destroy(handle)
             self._handle = None
 
-    def launch_shape(self) -> dict:
-        w, lds_rows, lds = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-        check(lib().faco_engine_launch_shape(self._handle, ctypes.byref(w), ctypes.byref(lds_rows), ctypes.byref(lds)))
-        return {"waves_per_block": w.value, "neighbour_rows_in_lds": bool(lds_rows.value), "lds_bytes": lds.value}
+    def launch_shape(self, colonies: int = 0) -> dict:
+        vals = [ctypes.c_uint32() for _ in range(5)]
+        check(lib().faco_engine_launch_shape(self._handle, colonies, *[ctypes.byref(v) for v in vals]))
+        w, lds_rows, lds, bpc, per_cu = (v.value for v in vals)
+        return {"waves_per_block": w, "neighbour_rows_in_lds": bool(lds_rows), "lds_bytes": lds,
+                "blocks_per_colony": bpc, "blocks_per_cu": per_cu}
 
-    def set_waves_per_block(self, waves: int) -> None:
-        check(lib().faco_engine_set_waves(self._handle, _unsigned(waves, "waves", 32)))
+    def set_waves_per_block(self, waves: int, blocks_per_colony: int = 0) -> None:
+        check(lib().faco_engine_set_waves(self._handle, _unsigned(waves, "waves", 32),
+                                          _unsigned(blocks_per_colony, "blocks_per_colony", 32)))
 
     def register_problem(self, instance_key, coordinates) -> dict:
         if type(instance_key) is not int:
@@ -331,6 +334,9 @@ class FacoBatchEngine:
             out["ant_phase_milliseconds"] = {"setup": result.kernel_milliseconds[2],
                                              "construction": result.kernel_milliseconds[3],
                                              "local_search": result.kernel_milliseconds[4]}
+            km = result.kernel_milliseconds
+            out["ant_subphase_megaclocks"] = {"select": km[5], "relocate": km[6], "mne_push": km[7],
+                                              "ls_scan": km[8], "ls_apply": km[9]}
         return out
 
 
@@ -393,7 +399,8 @@ class FixedFacoGpu:
 # ---------------------------------------------------------------------------
 # Diagnostics (parity harness; reference cuda/faco_operations.cu:60-132)
 
-def diagnostic_construct_ls(coordinates, ls_rows, parents, visits, mne_targets, evaluation_limit, waves_per_block=1):
+def diagnostic_construct_ls(coordinates, ls_rows, parents, visits, mne_targets, evaluation_limit, waves_per_block=1,
+                            ants_per_block=0):
     xy = _coordinates(coordinates)
     n = xy.shape[0]
     ls_rows = np.ascontiguousarray(ls_rows, dtype=np.uint32)
@@ -409,6 +416,7 @@ def diagnostic_construct_ls(coordinates, ls_rows, parents, visits, mne_targets, 
     check(lib().faco_diagnostic_construct_ls(xy.reshape(-1), n, ls_rows.reshape(-1), ls_rows.shape[1], ants,
                                              parents.reshape(-1), visits.reshape(-1), targets,
                                              _unsigned(evaluation_limit, "evaluation_limit"), waves_per_block,
+                                             ants_per_block,
                                              out["construction_tours"].reshape(-1), out["tours"].reshape(-1),
                                              out["positions"].reshape(-1), checklists.reshape(-1), ptr(info)))
     out["checklists"] = checklists
@@ -426,4 +434,19 @@ def diagnostic_distances(pairs) -> np.ndarray:
 def diagnostic_philox(key, batch, ant, step, words=8) -> np.ndarray:
     out = np.zeros(words, dtype=np.uint32)
     check(lib().faco_diagnostic_philox(key, batch, ant, step, words, out))
+    return out
+
+
+def diagnostic_copy(nbytes: int, repeats: int = 1) -> float:
+    """Average milliseconds of the calibration copy kernel over `nbytes`."""
+    ms = ctypes.c_double()
+    check(lib().faco_diagnostic_copy(nbytes, repeats, ctypes.byref(ms)))
+    return ms.value
+
+
+def diagnostic_start_nodes(key: int, batch: int, ant: int, counts, on_device: bool = False) -> np.ndarray:
+    """Start cities of the rejection sampler for each count (host or device)."""
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    out = np.zeros(counts.size, dtype=np.uint32)
+    check(lib().faco_diagnostic_start_nodes(key, batch, ant, counts, counts.size, int(on_device), out))
     return out

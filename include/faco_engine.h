@@ -66,7 +66,7 @@ typedef struct faco_ant_info {
 
 /* Batch result summary, the scalar fields of the reference's batch_output
  * dict (reference cpp/src/bindings.cpp:277-410). */
-#define FACO_STAGE_COUNT 9
+#define FACO_STAGE_COUNT 12
 typedef struct faco_result {
     uint64_t evaluation_limit_per_colony;
     uint64_t completed_tour_evaluations_per_colony;
@@ -124,11 +124,15 @@ void faco_engine_destroy(faco_engine* engine);
 int faco_engine_settings(const faco_engine* engine, faco_settings* out);
 
 /* Launch shape of the ant kernel: ants (waves) per workgroup, whether the
- * neighbour rows are staged in LDS, and the workgroup's LDS bytes. No
+ * neighbour rows are staged in LDS, the workgroup's LDS bytes, the
+ * workgroups per colony for a run of `colonies` colonies (each workgroup's
+ * waves share one chunk of a colony's ants dynamically) and the resident
+ * workgroups per CU. No
  * counterpart in the reference (its warps per block is a compile-time
  * choice, CMakeLists.txt:13-19); set_waves(0) restores the automatic shape. */
-int faco_engine_launch_shape(const faco_engine* engine, uint32_t* waves, uint32_t* nbr_in_lds, uint32_t* lds_bytes);
-int faco_engine_set_waves(faco_engine* engine, uint32_t waves);
+int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint32_t* waves, uint32_t* nbr_in_lds,
+                             uint32_t* lds_bytes, uint32_t* blocks_per_colony, uint32_t* blocks_per_cu);
+int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony);
 
 /* register_problem(instance_key, coordinates FP64[n][2])
  * (reference batch_engine.hpp:72, batch_engine.cu:279-295). Host preparation:
@@ -149,7 +153,9 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_
                                kernel_milliseconds[0] = construction + LS, [1] = colony update,
                                summed over iterations; [2..4] = mean per-ant setup, construction
                                and local-search milliseconds (device wall clock), summed over
-                               iterations */
+                               iterations; [5..9] = mean per-ant shader megaclocks of roulette
+                               selection, relocation, MNE/checklist pushes, 2-opt scan and
+                               2-opt apply, summed over iterations */
 
 /* evaluate_faco_evaluations(keys, seeds, evaluation_limit_per_colony, mne)
  * (reference batch_engine.hpp:87-89, bindings.cpp:701-720): plain FACO on
@@ -188,11 +194,23 @@ int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t 
 int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* ls, uint32_t ls_width,
                                  uint32_t ants, const uint32_t* parents, const uint32_t* visits,
                                  const uint32_t* mne_targets, uint64_t evaluation_limit,
-                                 uint32_t waves_per_block, uint32_t* construction_tours, uint32_t* tours,
+                                 uint32_t waves_per_block, uint32_t ants_per_block,
+                                 uint32_t* construction_tours, uint32_t* tours,
                                  uint32_t* positions, uint32_t* checklists, faco_ant_info* info);
 
 /* EUC_2D of `count` coordinate pairs on the device (distance kernel check). */
 int faco_diagnostic_distances(const int32_t* pairs /* [count][4] */, uint32_t count, int64_t* out);
+
+/* Start cities drawn by the generator's rejection sampler for (key, batch,
+ * ant) and each count in counts[n], on the host (on_device = 0, no GPU
+ * needed) or on the device (generator check). */
+int faco_diagnostic_start_nodes(uint64_t key, uint32_t batch, uint32_t ant, const uint32_t* counts, uint32_t n,
+                                uint32_t on_device, uint32_t* out);
+
+/* Streaming copy of `bytes` (multiple of 16) between two device buffers,
+ * `repeats` launches of faco_copy_kernel; average launch milliseconds. The
+ * known-traffic calibration kernel for the HBM byte counters. */
+int faco_diagnostic_copy(uint64_t bytes, uint32_t repeats, double* milliseconds);
 
 /* Philox words of the device generator for (key, batch, ant, step): the
  * first `words` 32-bit outputs of the stream (generator check). */

@@ -61,7 +61,7 @@ def test_device_philox_matches_oracle(ext, oracle_mod):
 CAPS = [0, 1, 7, 127, U64]
 
 
-@pytest.mark.parametrize("n", [3, 4, 5, 7, 8, 17, 100, 500, 1000])
+@pytest.mark.parametrize("n", [3, 4, 5, 7, 8, 17, 100, 500, 1000, 1024, 1025, 1500])
 @pytest.mark.parametrize("metric", ["uniform", "grid", "repeated"])
 def test_explicit_construct_ls_matches_oracle(ext, oracle_mod, n, metric):
     rng = np.random.default_rng(83741 + n + len(metric))
@@ -80,8 +80,9 @@ def test_explicit_construct_ls_matches_oracle(ext, oracle_mod, n, metric):
         visits = np.stack([rng.permutation(n) for _ in range(ants)]).astype(np.uint32)
         targets = np.array([2, 4, 8, 16] * 2, dtype=np.uint32)
         waves = [1, 2, 4, 8][cap % 4 if cap < 8 else 3]
-        announce("construct_ls", n, metric, cap, waves)
-        got = ext.diagnostic_construct_ls(xy.astype(np.float64), rows, parents, visits, targets, cap, waves)
+        chunk = min(8, [waves, 2 * waves, 8][CAPS.index(cap) % 3])
+        announce("construct_ls", n, metric, cap, waves, chunk)
+        got = ext.diagnostic_construct_ls(xy.astype(np.float64), rows, parents, visits, targets, cap, waves, chunk)
         for a in range(ants):
             want = oracle_mod.construct_ls(xy, rows, parents[a], visits[a], int(targets[a]), cap)
             info = got["info"][a]
@@ -155,7 +156,7 @@ def _panel(n, instances, clustered=False):
 
 
 @pytest.mark.parametrize("n,instances,seeds,batches,clustered", [
-    (100, 4, 2, 10, False), (200, 2, 3, 20, True), (1000, 2, 1, 4, False)])
+    (100, 4, 2, 10, False), (200, 2, 3, 20, True), (1000, 2, 1, 4, False), (1300, 2, 1, 3, True)])
 def test_batch_engine_matches_oracle(ext, oracle_mod, n, instances, seeds, batches, clustered):
     xys, keys = _panel(n, instances, clustered)
     s = _settings(ext, ants=32 if n < 1000 else 64)
@@ -193,9 +194,9 @@ def test_graph_replay_and_layouts_are_identical(ext):
         assert [i["tour"] for i in again["items"]] == [i["tour"] for i in base["items"]], rep
         np.testing.assert_array_equal(again["colony_work"], base["colony_work"])
     first = None
-    for waves in (1, 2, 4, 8, 16):
-        engine.set_waves_per_block(waves)
-        announce("layout", waves)
+    for waves, bpc in ((1, 0), (2, 0), (4, 0), (8, 0), (16, 0), (16, 1), (8, 2), (4, 16)):
+        engine.set_waves_per_block(waves, bpc)
+        announce("layout", waves, bpc, engine.launch_shape())
         other = engine.evaluate_faco_evaluations(task_keys, seeds, 64 * 20, use_graph=True)
         if first is None:
             first = other
@@ -240,3 +241,13 @@ def test_score_gpu_matches_score_cpu(ext):
         b = ext.score_cuda(program, features, masks)
         np.testing.assert_array_equal(a["scores"].view(np.uint32), b["scores"].view(np.uint32))
         np.testing.assert_array_equal(a["actions"], b["actions"])
+
+
+def test_device_start_nodes_match_oracle(ext, oracle_mod):
+    counts = np.array([3, 7, 1000, 1024, 10000, 65535, (1 << 31) + 1, (1 << 31) + 7, 0xFFFFFFFF, 3000000001],
+                      dtype=np.uint32)
+    for key, batch, ant in [(0, 0, 0), (0x9E3779B97F4A7C15, 5, 77), (2**64 - 1, 2**32 - 1, 2**32 - 1)]:
+        announce("start", key, batch, ant)
+        dev = ext.diagnostic_start_nodes(key, batch, ant, counts, on_device=True)
+        want = np.array([oracle_mod.start_node(key, batch, ant, int(c)) for c in counts], dtype=np.uint32)
+        np.testing.assert_array_equal(dev, want)
