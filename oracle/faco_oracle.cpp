@@ -619,6 +619,24 @@ int oracle_construct_ls(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, co
     });
 }
 
+// Checklist 2-opt alone from a tour and an explicit checklist (the initial
+// local search of preparation, cpp/src/prepared_problem.cpp:111-113).
+int oracle_two_opt(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, u32* tour, const u32* checklist_in,
+                   u32 checklist_size, u64 limit, std::int64_t* cost, u64* evaluations) {
+    return guarded([&] {
+        check_coordinates(n, xy);
+        Problem p; p.n = n; p.xy = xy;
+        Tour t; t.assign(tour, n, &p);
+        std::vector<u32> checklist(checklist_in, checklist_in + checklist_size);
+        std::vector<std::uint8_t> queued(n, 0);
+        for (u32 v : checklist) queued[v] = 1;
+        const LsStats st = two_opt(t, ls, lw, checklist, queued, limit);
+        std::copy(t.order.begin(), t.order.end(), tour);
+        *cost = t.recomputed();
+        *evaluations = st.evaluations;
+    });
+}
+
 // One roulette decision (select_next) with explicit inputs.
 int oracle_select_next(u32 n, const std::int32_t* xy, u32 pw, u32 bw, const u32* primary_row,
                        const u32* backup_row, const double* products, const std::uint8_t* visited,

@@ -122,6 +122,8 @@ def lib():
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double)]
         L.oracle_max_threads.restype = ctypes.c_int
+        L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
+                                     ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -229,6 +231,19 @@ def construct_ls(xy, ls, parent, visit, mne_target, limit) -> dict:
     out["checklist"] = checklist[: out["counters"]["checklist_size"]].copy()
     out["construction_cost"], out["final_cost"] = int(costs[0]), int(costs[1])
     return out
+
+
+def two_opt(xy, ls, tour, checklist, limit):
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
+    n = xy.size // 2
+    ls = np.ascontiguousarray(ls, dtype=np.uint32)
+    tour = np.ascontiguousarray(tour, dtype=np.uint32).copy()
+    checklist = np.ascontiguousarray(checklist, dtype=np.uint32)
+    cost, evals = ctypes.c_int64(), ctypes.c_uint64()
+    L = lib()
+    _check(L.oracle_two_opt(n, xy, ls.reshape(-1), ls.shape[1], tour, checklist, checklist.size, limit,
+                            ctypes.byref(cost), ctypes.byref(evals)), L)
+    return tour, cost.value, evals.value
 
 
 def select_next(xy, primary_row, backup_row, products, visited, current, u):
@@ -399,6 +414,3 @@ def ref_score_cpu(opcode, operand, constants, features, masks, feature_spec_id=1
                            actions), R)
     return scores.reshape(colonies, 32), np.array(actions[:], dtype=np.int32)
 
-
-if os.environ.get("ORACLE_AUTOBUILD", "1") == "1" and not LIB_PATH.exists():
-    pass  # built lazily by lib()
