@@ -159,6 +159,7 @@ struct Problem {
     Settings s{};
     std::vector<u32> primary, backup, ls;  // row-major [n][width]
     std::vector<double> local_scale;
+    double scale_epsilon = 1e-12;
     std::vector<u32> cheap, initial;
     i64 cheap_cost = 0, initial_cost = 0;
     i64 d(u32 a, u32 b) const { return euc2d(xy, a, b); }
@@ -398,6 +399,14 @@ void prepare(Problem& p) {
     p.backup.assign(static_cast<size_t>(n) * c.backup_width, 0);
     p.ls.assign(static_cast<size_t>(n) * c.ls_width, 0);
     p.local_scale.assign(n, 0);
+    // Scale floor: 1e-12 times the larger coordinate range (at least 1)
+    // (cpp/src/prepared_problem.cpp:68-73).
+    double min_x = p.xy[0], max_x = min_x, min_y = p.xy[1], max_y = min_y;
+    for (u32 i = 0; i < n; ++i) {
+        min_x = std::min<double>(min_x, p.xy[2 * i]); max_x = std::max<double>(max_x, p.xy[2 * i]);
+        min_y = std::min<double>(min_y, p.xy[2 * i + 1]); max_y = std::max<double>(max_y, p.xy[2 * i + 1]);
+    }
+    p.scale_epsilon = 1e-12 * std::max({max_x - min_x, max_y - min_y, 1.0});
     std::vector<std::pair<i64, u32>> row;
     for (u32 a = 0; a < n; ++a) {
         row.clear();
@@ -519,6 +528,500 @@ void colony_select_and_update(const Problem& p, ColonyRun& r, const std::vector<
 AntResult stochastic_ant(const Problem& p, const ColonyRun& r, u64 key, u32 batch, u32 ant, u32 target) {
     const u32 pw = p.s.primary_width;
     const u32 start = start_node(key, batch, ant, p.n);
+    return run_ant(p, r.parent.data(), start, target, p.s.ls_evaluation_limit,
+        [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited) {
+            Stream stream(key, batch, ant, step);
+            const double u = stream.uniform53();
+            int stage = 0;
+            return select_next(p, current, r.products.data() + static_cast<size_t>(current) * pw, visited, u, &stage);
+        });
+}
+
+// ===========================================================================
+// Restart and perturbation control (the GP / baseline controlled loop).
+// Restates cpp/include/gp_faco/control_ops.hpp, cuda/control_state.cuh,
+// cpp/include/gp_faco/baseline_policy.hpp, cpp/include/gp_faco/factorial_policy.hpp,
+// cuda/baseline_policy.cuh, cuda/gp_score.cuh and the per-batch order of
+// cuda/batch_engine.cu:880-1060 (features, decision, action, construction,
+// selection, pheromone, fingerprints, archive and feedback).
+
+constexpr u32 kArchive = 4, kRegion = 16, kSamples = 64;
+constexpr double kBand = 0.02;
+
+struct Ident {
+    u64 first = 0, second = 0;
+    bool operator==(const Ident& o) const { return first == o.first && second == o.second; }
+    bool operator!=(const Ident& o) const { return !(*this == o); }
+    bool operator<(const Ident& o) const { return first < o.first || (first == o.first && second < o.second); }
+};
+
+// fingerprint_edge / fingerprint (control_ops.hpp:43-56): XOR over the
+// undirected edges of two SplitMix64 images of (min << 32 | max).
+Ident edge_ident(u32 a, u32 b) {
+    if (b < a) std::swap(a, b);
+    const u64 e = (static_cast<u64>(a) << 32) | b;
+    return {mixed(e ^ 0xb5ad4eceda1ce2a9ULL), mixed(e ^ 0xd6e8feb86659fd93ULL)};
+}
+
+Ident tour_ident(const u32* t, u32 n) {
+    Ident r;
+    for (u32 i = 0; i < n; ++i) {
+        const Ident e = edge_ident(t[i], t[i + 1 == n ? 0 : i + 1]);
+        r.first ^= e.first; r.second ^= e.second;
+    }
+    return r;
+}
+
+// A tour with positions, integer cost and identity (TourRecord,
+// cpp/include/gp_faco/control_cpu.hpp:8-15).
+struct Rec {
+    std::vector<u32> order, pos;
+    i64 cost = 0;
+    Ident id;
+    u32 n() const { return static_cast<u32>(order.size()); }
+    u32 succ(u32 v) const { return order[pos[v] + 1 == n() ? 0 : pos[v] + 1]; }
+    u32 pred(u32 v) const { return order[pos[v] == 0 ? n() - 1 : pos[v] - 1]; }
+    bool has(u32 a, u32 b) const { return succ(a) == b || pred(a) == b; }
+};
+
+Rec make_rec(const u32* order, u32 n, i64 cost) {
+    Rec r;
+    r.order.assign(order, order + n);
+    r.pos.assign(n, 0);
+    for (u32 i = 0; i < n; ++i) r.pos[order[i]] = i;
+    r.cost = cost;
+    r.id = tour_ident(order, n);
+    return r;
+}
+
+// canonical_compare (control_ops.hpp:60-71): both tours read from city 0 in
+// the direction of its smaller neighbour.
+int canonical_compare(const Rec& a, const Rec& b) {
+    const u32 n = a.n();
+    const bool fa = a.succ(0) < a.pred(0), fb = b.succ(0) < b.pred(0);
+    u32 pa = a.pos[0], pb = b.pos[0];
+    for (u32 i = 1; i < n; ++i) {
+        pa = fa ? (pa + 1 == n ? 0 : pa + 1) : (pa == 0 ? n - 1 : pa - 1);
+        pb = fb ? (pb + 1 == n ? 0 : pb + 1) : (pb == 0 ? n - 1 : pb - 1);
+        if (a.order[pa] != b.order[pb]) return a.order[pa] < b.order[pb] ? -1 : 1;
+    }
+    return 0;
+}
+bool same_tour(const Rec& a, const Rec& b) { return a.n() == b.n() && canonical_compare(a, b) == 0; }
+
+// ControlRandom (control_ops.hpp:81-89): rejection-sampled SplitMix64 words.
+struct ControlRandom {
+    u64 key, counter = 0;
+    u32 bounded(u32 bound) {
+        const u64 threshold = (0ULL - static_cast<u64>(bound)) % bound;
+        u64 word;
+        do { word = mixed(key + counter++); } while (word < threshold);
+        return static_cast<u32>(word % bound);
+    }
+};
+
+// rank-th city not in existing[0..count) (control_ops.hpp:92-102).
+u32 remaining_node(u32 rank, const u32* existing, u32 count) {
+    std::vector<u32> sorted(existing, existing + count);
+    std::sort(sorted.begin(), sorted.end());
+    u32 node = rank;
+    for (u32 v : sorted) if (v <= node) ++node;
+    return node;
+}
+
+// sample_nodes (control_ops.hpp:104-108).
+void sample_nodes(u32 n, u64 instance_key, u32* out) {
+    const u32 count = std::min(n, kSamples);
+    ControlRandom random{mixed(instance_key ^ 0x6a09e667f3bcc909ULL)};
+    for (u32 i = 0; i < count; ++i) out[i] = remaining_node(random.bounded(n - i), out, i);
+}
+
+bool has_node(const u32* nodes, u32 count, u32 v) {
+    for (u32 i = 0; i < count; ++i) if (nodes[i] == v) return true;
+    return false;
+}
+
+// build_region (control_ops.hpp:120-146): regions 0-1 are tour segments of
+// the reference, region 2 a breadth-first candidate neighbourhood of a random
+// city, region 3 (and any shortfall) random distinct cities.
+void build_region(const Rec& ref, const u32* primary, u32 width, u64 key, u32 batch, u32 mode, u32 region,
+                  u32* nodes) {
+    const u32 n = ref.n(), count = std::min(n, kRegion);
+    ControlRandom random{mixed(key ^ 0xbb67ae8584caa73bULL ^ mixed(static_cast<u64>(batch) ^ 0x3c6ef372fe94f82bULL) ^
+                               mixed(static_cast<u64>(mode * 4 + region + 1) ^ 0xa54ff53a5f1d36f1ULL))};
+    u32 used = 0;
+    if (region < 2) {
+        const u32 start = random.bounded(n);
+        for (u32 i = 0; i < count; ++i) nodes[i] = ref.order[(start + i) % n];
+        return;
+    }
+    if (region == 2) {
+        nodes[used++] = random.bounded(n);
+        for (u32 head = 0; head < used && used < count; ++head)
+            for (u32 j = 0; j < width && used < count; ++j) {
+                const u32 v = primary[static_cast<size_t>(nodes[head]) * width + j];
+                if (v < n && !has_node(nodes, used, v)) nodes[used++] = v;
+            }
+    }
+    while (used < count) {
+        const u32 v = remaining_node(random.bounded(n - used), nodes, used);
+        nodes[used++] = v;
+    }
+}
+
+// sampled_difference (control_ops.hpp:148-157).
+u32 sampled_difference(const Rec& cand, const Rec& active, const u32* samples, u32 count) {
+    u32 missing = 0;
+    for (u32 i = 0; i < count; ++i) {
+        const u32 v = samples[i];
+        missing += !active.has(v, cand.succ(v));
+        missing += !active.has(v, cand.pred(v));
+    }
+    return missing;
+}
+
+double unit_clip(double v) { return v < 0 ? 0 : v > 1 ? 1 : v; }
+
+struct Feedback {
+    float return_rate = 0, ls_work = 0;
+    u64 stagnant = 0, epoch_batches = 0, restarts = 0;
+};
+
+// update_feedback / restart_feedback (control_ops.hpp:175-190): EMA with
+// alpha = 1/16 in FP32, saturating counters.
+void update_feedback(Feedback& f, bool improved, double returned_fraction, double work_fraction, u32 ants) {
+    if (!ants) return;
+    const float returned = static_cast<float>(unit_clip(returned_fraction));
+    const float work = static_cast<float>(unit_clip(work_fraction));
+    f.return_rate += (returned - f.return_rate) * 0.0625f;
+    f.ls_work += (work - f.ls_work) * 0.0625f;
+    if (improved) f.stagnant = 0;
+    else if (f.stagnant != UINT64_MAX) ++f.stagnant;
+    if (f.epoch_batches != UINT64_MAX) ++f.epoch_batches;
+}
+void restart_feedback(Feedback& f) {
+    f.return_rate = f.ls_work = 0;
+    f.epoch_batches = 0;
+    if (f.restarts != UINT64_MAX) ++f.restarts;
+}
+
+struct Control {
+    Feedback fb;
+    std::vector<Rec> archive;
+    i64 tracked_global = 0, tracked_epoch = 0;
+    Ident active, epoch_id;
+};
+
+// Baseline and factorial policies: same field layout as the reference's
+// BaselinePolicy (baseline_policy.hpp:20-27) and FactorialPolicy
+// (factorial_policy.hpp:17-20).
+struct Baseline {
+    u32 kind, mne_level, max_mne_level, region, restart_mode, restart_period;
+    double restart_probability;
+    u32 stagnation_step, restart_stagnation, restart_cooldown, pad;
+};
+static_assert(sizeof(Baseline) == 48, "baseline layout");
+
+// baseline_action (baseline_policy.hpp:59-79).
+u32 baseline_action(const Baseline& p, const Feedback& f, u32 batch, double u, u32 mask) {
+    u32 level = p.mne_level;
+    bool restart = false;
+    if (p.kind == 0) {
+        restart = p.restart_mode == 1 ? batch > 0 && batch % p.restart_period == 0
+                                      : p.restart_mode == 2 && u < p.restart_probability;
+    } else {
+        if (p.stagnation_step) {
+            const u64 extra = f.stagnant / p.stagnation_step;
+            const u32 room = p.max_mne_level - p.mne_level;
+            level += extra < room ? static_cast<u32>(extra) : room;
+        }
+        restart = p.restart_stagnation && f.stagnant >= p.restart_stagnation && f.epoch_batches >= p.restart_cooldown;
+    }
+    const u32 keep = p.region * 4 + level, requested = keep + 16;
+    return restart && (mask & (1u << requested)) ? requested : keep;
+}
+
+// baseline_uniform (cuda/baseline_policy.cuh:10-16): colony slot ant
+// 0xffffffff, step 1; only Bernoulli restarts consume it.
+double baseline_uniform(const Baseline& p, u64 key, u32 batch) {
+    if (p.kind != 0 || p.restart_mode != 2) return 0;
+    Stream s(key, batch, 0xffffffffu, 1);
+    return s.uniform53();
+}
+
+// factorial_mask (factorial_policy.hpp:32-45); variant 0..3 = M00, M10, M01, M11.
+u32 factorial_mask(u32 variant, const Baseline& p, const Feedback& f, u32 batch, double u, u32 legal) {
+    if (variant == 3) return legal;
+    if (variant == 0) return 1u << baseline_action(p, f, batch, u, legal);
+    const u32 requested = baseline_action(p, f, batch, u, 0xffffffffu);
+    if (variant == 2) {
+        const u32 keep = requested % 16;
+        return legal & ((1u << keep) | (1u << (keep + 16)));
+    }
+    const u32 allowed = legal & (requested >= 16 ? 0xffff0000u : 0xffffu);
+    return allowed ? allowed : legal & 0xffffu;
+}
+
+// Program IR v1 record (cpp/include/gp_faco/program.hpp:10-19) and the
+// three-tree controller (:27-30).
+struct Program {
+    std::uint16_t ir_version, numeric_spec_id, feature_spec_id, length, constants_count;
+    std::uint8_t opcode[63], operand[63];
+    float constants[63];
+};
+static_assert(sizeof(Program) == 388, "program layout");
+struct Controller {
+    u32 kind;
+    Program trees[3];
+};
+static_assert(sizeof(Controller) == 1168, "controller layout");
+
+// apply_operation (cpp/include/gp_faco/program_ops.hpp:13-31), the CPU
+// formula for AQ.
+float apply_operation(int op, float left, float right) {
+    float r = 0.0f;
+    switch (op) {
+        case 2: r = left + right; break;
+        case 3: r = left - right; break;
+        case 4: r = left * right; break;
+        case 5: r = std::fmin(left, right); break;
+        case 6: r = std::fmax(left, right); break;
+        case 7: r = std::fabs(left); break;
+        case 8: r = left * (1.0f / std::sqrt(1.0f + right * right)); break;
+        default: break;
+    }
+    return std::fmin(8.0f, std::fmax(-8.0f, r));
+}
+
+// Stack VM value of one action; features are [12][32] of one colony.
+float program_value(const Program& p, const float* features, u32 action) {
+    float stack[6] = {0, 0, 0, 0, 0, 0};
+    int top = 0;
+    for (int i = 0; i < p.length; ++i) {
+        const int op = p.opcode[i];
+        if (op == 0) stack[top++] = features[p.operand[i] * 32 + action];
+        else if (op == 1) stack[top++] = p.constants[p.operand[i]];
+        else {
+            const float right = op == 7 ? 0.0f : stack[--top];
+            const float left = stack[--top];
+            stack[top++] = apply_operation(op, left, right);
+        }
+    }
+    return stack[0];
+}
+
+// Smallest legal index within 1e-6 of the largest legal value
+// (cuda/gp_score.cuh:24-34, cpp/src/program_cpu.cpp:104-113); 32 if none.
+u32 choose(const float* values, const bool* legal, u32 width) {
+    float maximum = -std::numeric_limits<float>::infinity();
+    for (u32 i = 0; i < width; ++i) if (legal[i]) maximum = std::max(maximum, values[i]);
+    for (u32 i = 0; i < width; ++i) if (legal[i] && maximum - values[i] <= 1e-6f) return i;
+    return 32;
+}
+
+// score_actions (cuda/gp_score.cuh:64-102): one tree over the 32 actions.
+u32 score_program(const Program& p, const float* features, u32 mask, float* scores) {
+    bool legal[32];
+    for (u32 a = 0; a < 32; ++a) { scores[a] = program_value(p, features, a); legal[a] = (mask >> a) & 1u; }
+    return choose(scores, legal, 32);
+}
+
+// score_controllers (cuda/gp_score.cuh:37-62): kind 0 = one tree; kind 1 =
+// conditional restart (2-way, stride 16), region (4-way, stride 4), MNE level
+// (4-way, stride 1); a block is legal when any action under it is.
+u32 score_controller(const Controller& c, const float* features, u32 mask, float* scores) {
+    if (!c.kind) return score_program(c.trees[0], features, mask, scores);
+    for (u32 a = 0; a < 32; ++a) scores[a] = 0.0f;
+    u32 prefix = 0;
+    for (u32 role = 0; role < 3; ++role) {
+        const u32 width = role == 0 ? 2 : 4, stride = role == 0 ? 16 : role == 1 ? 4 : 1;
+        const u32 slot = role == 0 ? 0 : role == 1 ? 2 : 6;
+        float values[4];
+        bool legal[4];
+        for (u32 l = 0; l < width; ++l) {
+            const u32 action = prefix + l * stride;
+            legal[l] = ((mask >> action) & ((1u << stride) - 1u)) != 0;
+            values[l] = program_value(c.trees[role], features, action);
+            scores[slot + l] = values[l];
+        }
+        prefix += stride * choose(values, legal, width);
+    }
+    return prefix;
+}
+
+struct ControlOut {
+    u32 archive_size, pad;
+    u64 stagnant_batches, epoch_batches, restarts;
+    float return_rate, ls_work;
+};
+
+struct Decision {
+    float features[12 * 32];
+    float scores[32];
+    u32 mask = 0, alternative = kArchive, action = 0;
+    u32 regions[2][4][kRegion];
+    u32 region_count = 0;
+};
+
+// build_control_features (cuda/control_state.cuh:200-307) for one colony.
+void control_features(const Problem& p, const ColonyRun& r, const Control& ctl, const u32* samples, u64 key,
+                      u32 batch, double elapsed, u32 experiment_mask, Decision& d) {
+    const u32 n = p.n, pw = p.s.primary_width, sample_count = std::min(n, kSamples);
+    const Rec active = make_rec(r.parent.data(), n, r.parent_cost);
+    u32 alternative = kArchive, largest = 0;
+    for (u32 i = 0; i < ctl.archive.size(); ++i) {
+        const Rec& cand = ctl.archive[i];
+        if (cand.id == ctl.active && same_tour(cand, active)) continue;
+        const u32 missing = sampled_difference(cand, active, samples, sample_count);
+        bool better_tie = false;
+        if (alternative != kArchive && missing == largest) {
+            const Rec& cur = ctl.archive[alternative];
+            better_tie = cand.cost < cur.cost ||
+                (cand.cost == cur.cost && (cand.id < cur.id || (cand.id == cur.id && canonical_compare(cand, cur) < 0)));
+        }
+        if (alternative == kArchive || missing > largest || better_tie) { alternative = i; largest = missing; }
+    }
+    const double global = static_cast<double>(r.global_cost);
+    float gap[2];
+    gap[0] = static_cast<float>(unit_clip((static_cast<double>(r.parent_cost) / global - 1) / kBand));
+    gap[1] = alternative < kArchive
+        ? static_cast<float>(unit_clip((static_cast<double>(ctl.archive[alternative].cost) / global - 1) / kBand)) : 0;
+    const float difference = alternative < kArchive ? static_cast<float>(largest) / (2 * sample_count) : 0;
+    d.alternative = alternative;
+    d.mask = experiment_mask & (alternative < kArchive ? 0xffffffffu : 0xffffu);
+    d.region_count = std::min(n, kRegion);
+    const u32 count = d.region_count, archive_size = static_cast<u32>(ctl.archive.size());
+    for (u32 mode = 0; mode < 2; ++mode) for (u32 region = 0; region < 4; ++region) {
+        u32* nodes = d.regions[mode][region];
+        for (u32 i = 0; i < kRegion; ++i) nodes[i] = 0;
+        const bool valid = mode == 0 || alternative < kArchive;
+        float regional[4] = {0, 0, 0, 0};
+        if (valid) {
+            const Rec& ref = mode == 0 ? active : ctl.archive[alternative];
+            build_region(ref, p.primary.data(), pw, key, batch, mode, region, nodes);
+            double excess = 0, disagreement = 0, strength = 0, dispersion = 0;
+            for (u32 i = 0; i < count; ++i) {
+                const u32 v = nodes[i], prev = ref.pred(v), next = ref.succ(v);
+                const double scale = p.local_scale[v] < p.scale_epsilon ? p.scale_epsilon : p.local_scale[v];
+                const double near = 0.5 * static_cast<double>(p.d(v, prev)) + 0.5 * static_cast<double>(p.d(v, next));
+                excess += unit_clip((near / scale - 1) / 3);
+                double agree = 0, pher[2];
+                const u32 neighbours[2] = {prev, next};
+                double spread = 0;
+                for (u32 k = 0; k < 2; ++k) {
+                    const u32 w = neighbours[k];
+                    for (u32 j = 0; j < archive_size; ++j) agree += !ctl.archive[j].has(v, w);
+                    double trail = r.default_trail;
+                    for (u32 j = 0; j < pw; ++j)
+                        if (p.primary[static_cast<size_t>(v) * pw + j] == w) { trail = r.trails[static_cast<size_t>(v) * pw + j]; break; }
+                    pher[k] = r.minimum == r.maximum ? 1 : unit_clip((trail - r.minimum) / (r.maximum - r.minimum));
+                    spread += !has_node(nodes, count, w);
+                }
+                disagreement += agree;
+                strength += pher[0]; strength += pher[1];
+                dispersion += spread;
+            }
+            regional[0] = static_cast<float>(excess / count);
+            regional[1] = static_cast<float>(disagreement / (2 * count * archive_size));
+            regional[2] = static_cast<float>(strength / (2 * count));
+            regional[3] = static_cast<float>(dispersion / (2 * count));
+        }
+        for (u32 level = 0; level < 4; ++level) {
+            const u32 action = mode * 16 + region * 4 + level;
+            const float values[12] = {static_cast<float>(unit_clip(elapsed)),
+                static_cast<float>(ctl.fb.stagnant < 32 ? ctl.fb.stagnant / 32.0 : 1),
+                ctl.fb.return_rate, ctl.fb.ls_work, static_cast<float>(mode), (level + 1) * 0.25f,
+                gap[mode], mode ? difference : 0.0f, regional[0], regional[1], regional[2], regional[3]};
+            for (u32 f = 0; f < 12; ++f) d.features[f * 32 + action] = values[f];
+        }
+    }
+}
+
+// apply_control_action (cuda/control_state.cuh:309-364): action >= 16
+// restarts from the archive alternative.
+void control_restart(const Problem& p, ColonyRun& r, Control& ctl, u32 alternative, bool initializing) {
+    const u32 n = p.n, pw = p.s.primary_width;
+    const Rec& alt = ctl.archive[alternative];
+    r.epoch_cost = r.parent_cost = alt.cost;
+    trail_limits(pw, p.s.p_best, p.s.retention, static_cast<double>(r.epoch_cost), &r.minimum, &r.maximum);
+    r.default_trail = r.maximum;
+    r.source_uniform = 0; r.source_is_epoch = 0; r.iteration_best = 0;
+    ctl.tracked_epoch = r.epoch_cost;
+    ctl.active = ctl.epoch_id = alt.id;
+    if (!initializing) restart_feedback(ctl.fb);
+    r.parent = r.epoch = alt.order;
+    for (size_t i = 0; i < static_cast<size_t>(n) * pw; ++i) {
+        r.trails[i] = r.maximum;
+        r.products[i] = r.maximum * r.eta[i];
+    }
+}
+
+// Archive selection of update_control (cuda/control_state.cuh:127-160):
+// slot 0 is the global best `first`; slots 1-3 take, in order, the best
+// pool entries by (cost, identity, canonical order, index) that lie within
+// the 2% band of the global cost and are not the same tour as a chosen one.
+std::vector<u32> archive_select(const std::vector<Rec>& pool, u32 first, i64 global_cost) {
+    const double band = static_cast<double>(global_cost) * (1 + kBand);
+    std::vector<u32> chosen{first};
+    auto duplicate = [&](u32 a, u32 b) { return a == b || (pool[a].id == pool[b].id && same_tour(pool[a], pool[b])); };
+    auto better = [&](u32 a, u32 b) {
+        if (a == UINT32_MAX) return b;
+        if (pool[a].cost != pool[b].cost) return pool[a].cost < pool[b].cost ? a : b;
+        if (pool[a].id != pool[b].id) return pool[a].id < pool[b].id ? a : b;
+        const int order = canonical_compare(pool[a], pool[b]);
+        if (order) return order < 0 ? a : b;
+        return std::min(a, b);
+    };
+    for (u32 slot = 1; slot < kArchive; ++slot) {
+        u32 best = UINT32_MAX;
+        for (u32 c = 0; c < pool.size(); ++c) {
+            if (static_cast<double>(pool[c].cost) > band) continue;
+            bool dup = false;
+            for (u32 j : chosen) if (duplicate(c, j)) { dup = true; break; }
+            if (!dup) best = better(best, c);
+        }
+        if (best == UINT32_MAX) break;
+        chosen.push_back(best);
+    }
+    return chosen;
+}
+
+// update_control (cuda/control_state.cuh:105-198) after selection.
+void control_update(const Problem& p, const ColonyRun& r, Control& ctl, const std::vector<AntResult>& ants) {
+    const u32 n = p.n, A = static_cast<u32>(ants.size());
+    std::vector<Rec> pool = ctl.archive;
+    for (const auto& a : ants) pool.push_back(make_rec(a.tour.data(), n, a.final_cost));
+    const u32 archived = static_cast<u32>(ctl.archive.size());
+    const bool improved = r.global_cost < ctl.tracked_global;
+    const std::vector<u32> chosen = archive_select(pool, improved ? archived + r.iteration_best : 0u, r.global_cost);
+    double returned = 0, work = 0;
+    const u64 limit = p.s.ls_evaluation_limit;
+    for (u32 a = 0; a < A; ++a) {
+        returned += pool[archived + a].id == ctl.active;
+        if (limit) work += static_cast<double>(ants[a].ls.evaluations) / static_cast<double>(limit);
+    }
+    update_feedback(ctl.fb, improved, returned / A, work / A, A);
+    const Ident best_id = pool[archived + r.iteration_best].id;
+    if (r.epoch_cost < ctl.tracked_epoch) ctl.epoch_id = best_id;
+    ctl.active = r.source_is_epoch ? ctl.epoch_id : best_id;
+    ctl.tracked_global = r.global_cost;
+    ctl.tracked_epoch = r.epoch_cost;
+    std::vector<Rec> next;
+    for (u32 c : chosen) next.push_back(pool[c]);
+    ctl.archive = std::move(next);
+}
+
+void control_init(const Problem& p, Control& ctl) {
+    ctl = Control{};
+    ctl.archive.push_back(make_rec(p.initial.data(), p.n, p.initial_cost));
+    ctl.tracked_global = ctl.tracked_epoch = p.initial_cost;
+    ctl.active = ctl.epoch_id = ctl.archive[0].id;
+}
+
+AntResult controlled_ant(const Problem& p, const ColonyRun& r, u64 key, u32 batch, u32 ant, u32 target,
+                         const u32* start_nodes, u32 start_count) {
+    const u32 pw = p.s.primary_width;
+    const u32 start = start_nodes[start_node(key, batch, ant, start_count)];
     return run_ant(p, r.parent.data(), start, target, p.s.ls_evaluation_limit,
         [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited) {
             Stream stream(key, batch, ant, step);
@@ -713,6 +1216,246 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             global_costs[c] = runs[c].global_cost;
             work[2 * c] = runs[c].construction_steps;
             work[2 * c + 1] = runs[c].ls_evaluations;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Control primitives, exported for tests.
+
+void oracle_tour_ident(u32 n, const u32* tour, u64* out) {
+    const Ident id = tour_ident(tour, n);
+    out[0] = id.first; out[1] = id.second;
+}
+
+int oracle_canonical_compare(u32 n, const u32* a, const u32* b) {
+    return canonical_compare(make_rec(a, n, 0), make_rec(b, n, 0));
+}
+
+void oracle_sample_nodes(u32 n, u64 instance_key, u32* out) { sample_nodes(n, instance_key, out); }
+
+u32 oracle_remaining_node(u32 rank, const u32* existing, u32 count) { return remaining_node(rank, existing, count); }
+
+void oracle_build_region(u32 n, const u32* tour, const u32* primary, u32 width, u64 key, u32 batch, u32 mode,
+                         u32 region, u32* nodes) {
+    build_region(make_rec(tour, n, 0), primary, width, key, batch, mode, region, nodes);
+}
+
+u32 oracle_baseline_action(const Baseline* policy, const Feedback* feedback, u32 batch, double uniform, u32 mask) {
+    return baseline_action(*policy, *feedback, batch, uniform, mask);
+}
+
+double oracle_baseline_uniform(const Baseline* policy, u64 key, u32 batch) {
+    return baseline_uniform(*policy, key, batch);
+}
+
+u32 oracle_factorial_mask(u32 variant, const Baseline* policy, const Feedback* feedback, u32 batch, double uniform,
+                          u32 legal) {
+    return factorial_mask(variant, *policy, *feedback, batch, uniform, legal);
+}
+
+void oracle_update_feedback(Feedback* feedback, int improved, double returned, double work, u32 ants) {
+    update_feedback(*feedback, improved != 0, returned, work, ants);
+}
+
+void oracle_restart_feedback(Feedback* feedback) { restart_feedback(*feedback); }
+
+int oracle_score_controller(const Controller* controller, const float* features /*[12][32]*/, u32 mask,
+                            float* scores, u32* action) {
+    return guarded([&] { *action = score_controller(*controller, features, mask, scores); });
+}
+
+namespace {
+i64 tour_length(const std::int32_t* xy, const u32* t, u32 n) {
+    i64 s = 0;
+    for (u32 i = 0; i < n; ++i) s += euc2d(xy, t[i], t[i + 1 == n ? 0 : i + 1]);
+    return s;
+}
+}  // namespace
+
+// ReferenceArchive::update semantics through archive_select: the pool is
+// [global, entries..., completed...] with slot 0 the global tour. Outputs
+// the chosen tours [4][n] and their count.
+int oracle_archive_update(u32 n, const std::int32_t* xy, const u32* global, u32 entries, const u32* entry_tours,
+                          u32 completed, const u32* completed_tours, u32* out_tours, u32* out_count) {
+    return guarded([&] {
+        std::vector<Rec> pool;
+        pool.push_back(make_rec(global, n, tour_length(xy, global, n)));
+        for (u32 i = 0; i < entries; ++i) {
+            const u32* t = entry_tours + static_cast<size_t>(i) * n;
+            pool.push_back(make_rec(t, n, tour_length(xy, t, n)));
+        }
+        for (u32 i = 0; i < completed; ++i) {
+            const u32* t = completed_tours + static_cast<size_t>(i) * n;
+            pool.push_back(make_rec(t, n, tour_length(xy, t, n)));
+        }
+        const auto chosen = archive_select(pool, 0, pool[0].cost);
+        for (u32 i = 0; i < chosen.size(); ++i)
+            std::copy(pool[chosen[i]].order.begin(), pool[chosen[i]].order.end(), out_tours + static_cast<size_t>(i) * n);
+        *out_count = static_cast<u32>(chosen.size());
+    });
+}
+
+// Feature snapshot of one colony (build_control_features) from an explicit
+// state: active (parent) tour, archive tours (slot 0 = global best), sparse
+// trails [n][pw] with their default, bounds, feedback and progress.
+int oracle_action_snapshot(u32 n, const std::int32_t* xy, u32 pw, const u32* primary, const double* local_scale,
+                           double epsilon, const u32* samples, const u32* active, u32 archive_size,
+                           const u32* archive_tours, const double* trails, double default_trail, double minimum,
+                           double maximum, const Feedback* feedback, double elapsed, u64 key, u32 batch,
+                           u32 experiment_mask, float* features, u32* regions, u32* alternative, u32* mask) {
+    return guarded([&] {
+        Problem p; p.n = n; p.xy = xy;
+        p.s.primary_width = pw;
+        p.primary.assign(primary, primary + static_cast<size_t>(n) * pw);
+        p.local_scale.assign(local_scale, local_scale + n);
+        p.scale_epsilon = epsilon;
+        Control ctl;
+        ctl.fb = *feedback;
+        for (u32 i = 0; i < archive_size; ++i) {
+            const u32* t = archive_tours + static_cast<size_t>(i) * n;
+            ctl.archive.push_back(make_rec(t, n, tour_length(xy, t, n)));
+        }
+        ColonyRun r;
+        r.parent.assign(active, active + n);
+        r.parent_cost = tour_length(xy, active, n);
+        r.global_cost = ctl.archive[0].cost;
+        r.trails.assign(trails, trails + static_cast<size_t>(n) * pw);
+        r.default_trail = default_trail; r.minimum = minimum; r.maximum = maximum;
+        ctl.active = tour_ident(active, n);
+        Decision d;
+        control_features(p, r, ctl, samples, key, batch, elapsed, experiment_mask, d);
+        std::copy(d.features, d.features + 12 * 32, features);
+        std::memcpy(regions, d.regions, sizeof(d.regions));
+        *alternative = d.alternative;
+        *mask = d.mask;
+    });
+}
+
+// Controlled FACO run (evaluate_program / population / controller /
+// baseline / factorial evaluations, cuda/batch_engine.cu:376-460,492-1192).
+// Colony c uses instance c % instances and decision maker c / instances.
+// mode: 1 program, 2 controller, 3 baseline, 4 factorial (program + policy).
+// Traces (any may be NULL), per batch b and colony c:
+//   features [b][12][C][32], scores [b][C][32], masks/actions/alternatives
+//   [b][C], regions [b][C][2][4][16], costs [b][C][3] (epoch, global, parent
+//   after the batch).
+int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, const u64* instance_keys,
+                          u32 colonies, const u64* seeds, const Settings* settings, u64 evaluation_limit,
+                          u32 mode, u32 deciders, const Program* programs, const Controller* controllers,
+                          const Baseline* baseline, u32 variant, u32 experiment_mask, int threads,
+                          u32* global_tours, std::int64_t* global_costs, u64* work, ControlOut* states,
+                          float* trace_features, float* trace_scores, u32* trace_masks, std::int32_t* trace_actions,
+                          u32* trace_alternatives, u32* trace_regions, std::int64_t* trace_costs,
+                          double* elapsed_search_seconds) {
+    return guarded([&] {
+        if (!instances || !colonies || colonies % instances) throw std::invalid_argument("invalid panel");
+        if (mode < 1 || mode > 4) throw std::invalid_argument("unknown control mode");
+        if (!(experiment_mask & 0xffffu)) throw std::invalid_argument("experiment mask must keep a hold action");
+        std::vector<Problem> problems(instances);
+        std::vector<std::vector<u32>> samples(instances, std::vector<u32>(kSamples, 0));
+        for (u32 g = 0; g < instances; ++g) {
+            check_coordinates(n, xy_all + static_cast<size_t>(g) * 2 * n);
+            problems[g].n = n; problems[g].xy = xy_all + static_cast<size_t>(g) * 2 * n;
+            problems[g].s = normalized(*settings, n);
+            prepare(problems[g]);
+            sample_nodes(n, instance_keys[g], samples[g].data());
+        }
+        const Settings s = problems[0].s;
+        if (evaluation_limit % s.ants) throw std::invalid_argument("evaluation limit must be whole ant batches");
+        const u32 batches = static_cast<u32>(evaluation_limit / s.ants);
+        std::vector<ColonyRun> runs(colonies);
+        std::vector<Control> controls(colonies);
+        std::vector<u64> keys(colonies);
+        for (u32 c = 0; c < colonies; ++c) {
+            const Problem& p = problems[c % instances];
+            colony_reset(p, runs[c]);
+            control_init(p, controls[c]);
+            control_restart(p, runs[c], controls[c], 0, true);
+            keys[c] = colony_key(instance_keys[c % instances], seeds[c]);
+        }
+        std::vector<std::vector<AntResult>> results(colonies, std::vector<AntResult>(s.ants));
+        std::vector<Decision> decisions(colonies);
+        std::vector<u32> targets(colonies);
+#ifdef _OPENMP
+        if (threads > 0) omp_set_num_threads(threads);
+        const double started = omp_get_wtime();
+#endif
+        for (u32 batch = 0; batch < batches; ++batch) {
+            const double elapsed = static_cast<double>(static_cast<u64>(batch) * s.ants) /
+                                   static_cast<double>(evaluation_limit);
+            for (u32 c = 0; c < colonies; ++c) {
+                const Problem& p = problems[c % instances];
+                Decision& d = decisions[c];
+                control_features(p, runs[c], controls[c], samples[c % instances].data(), keys[c], batch, elapsed,
+                                 experiment_mask, d);
+                const u32 who = c / instances;
+                if (mode == 3 || mode == 4) {
+                    const double u = baseline_uniform(*baseline, keys[c], batch);
+                    if (mode == 3) {
+                        d.action = baseline_action(*baseline, controls[c].fb, batch, u, d.mask);
+                        std::fill(d.scores, d.scores + 32, 0.0f);
+                    } else {
+                        d.mask = factorial_mask(variant, *baseline, controls[c].fb, batch, u, d.mask);
+                        d.action = score_program(programs[who < deciders ? who : 0], d.features, d.mask, d.scores);
+                    }
+                } else if (mode == 1) {
+                    d.action = score_program(programs[who < deciders ? who : 0], d.features, d.mask, d.scores);
+                } else {
+                    d.action = score_controller(controllers[who < deciders ? who : 0], d.features, d.mask, d.scores);
+                }
+                if (d.action >= 32) throw std::runtime_error("no legal control action");
+                targets[c] = 2u << (d.action % 4);
+                if (d.action >= 16) control_restart(p, runs[c], controls[c], d.alternative, false);
+            }
+            const long long total = static_cast<long long>(colonies) * s.ants;
+#pragma omp parallel for schedule(dynamic, 1)
+            for (long long i = 0; i < total; ++i) {
+                const u32 c = static_cast<u32>(i / s.ants), a = static_cast<u32>(i % s.ants);
+                const Decision& d = decisions[c];
+                results[c][a] = controlled_ant(problems[c % instances], runs[c], keys[c], batch, a, targets[c],
+                                               d.regions[d.action / 16][(d.action / 4) % 4], d.region_count);
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+            for (long long c = 0; c < static_cast<long long>(colonies); ++c) {
+                const Problem& p = problems[c % instances];
+                colony_select_and_update(p, runs[c], results[c], keys[c], batch);
+                control_update(p, runs[c], controls[c], results[c]);
+            }
+            for (u32 c = 0; c < colonies; ++c) {
+                const Decision& d = decisions[c];
+                const size_t bc = static_cast<size_t>(batch) * colonies + c;
+                if (trace_features)
+                    for (u32 f = 0; f < 12; ++f)
+                        std::copy(d.features + f * 32, d.features + f * 32 + 32,
+                                  trace_features + ((static_cast<size_t>(batch) * 12 + f) * colonies + c) * 32);
+                if (trace_scores) std::copy(d.scores, d.scores + 32, trace_scores + bc * 32);
+                if (trace_masks) trace_masks[bc] = d.mask;
+                if (trace_actions) trace_actions[bc] = static_cast<std::int32_t>(d.action);
+                if (trace_alternatives) trace_alternatives[bc] = d.alternative;
+                if (trace_regions) std::memcpy(trace_regions + bc * 128, d.regions, sizeof(d.regions));
+                if (trace_costs) {
+                    trace_costs[3 * bc] = runs[c].epoch_cost;
+                    trace_costs[3 * bc + 1] = runs[c].global_cost;
+                    trace_costs[3 * bc + 2] = runs[c].parent_cost;
+                }
+            }
+        }
+#ifdef _OPENMP
+        if (elapsed_search_seconds) *elapsed_search_seconds = omp_get_wtime() - started;
+#else
+        if (elapsed_search_seconds) *elapsed_search_seconds = 0;
+#endif
+        for (u32 c = 0; c < colonies; ++c) {
+            std::copy(runs[c].global.begin(), runs[c].global.end(), global_tours + static_cast<size_t>(c) * n);
+            global_costs[c] = runs[c].global_cost;
+            work[2 * c] = runs[c].construction_steps;
+            work[2 * c + 1] = runs[c].ls_evaluations;
+            if (states) {
+                const Control& k = controls[c];
+                states[c] = ControlOut{static_cast<u32>(k.archive.size()), 0, k.fb.stagnant, k.fb.epoch_batches,
+                                       k.fb.restarts, k.fb.return_rate, k.fb.ls_work};
+            }
         }
     });
 }

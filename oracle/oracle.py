@@ -28,7 +28,23 @@ Parity status of each oracle function and what pins it:
 - run_faco: composition of the above; the per-batch selection rule (argmin
   first index, strict improvement, epoch-source coin) restated from
   cuda/colony_state.cuh:148-208 and checked by replaying its trace through
-  the parts above in tests/test_oracle_colony.py.
+  the parts above in tests/test_oracle.py.
+- control (tour identity, canonical order, sampling, regions, archive update,
+  action snapshot, feedback, baseline and factorial policies): against the
+  reference's control_cpu.cpp / control_ops.hpp / baseline_policy.hpp /
+  factorial_policy.hpp (oracle/_ref) on random states, and the hand-derived
+  cases of tests/cpp/test_control_cpu.cpp and test_baseline_policy.cpp.
+- controller scoring (conditional three-tree): restated from
+  cuda/gp_score.cuh:37-62; pinned by closed forms (a kind-0 controller equals
+  score_cpu; each stage is an argmax over the legal blocks) in
+  tests/test_oracle_control.py. The reference has no CPU scorer for it.
+- run_controlled: composition of the above in the per-batch order of
+  cuda/batch_engine.cu:880-1060; its restart transaction
+  (cuda/control_state.cuh:309-364) and archive/feedback update (:105-198)
+  have no CPU counterpart in the reference and are pinned by the invariants
+  in tests/test_oracle_control.py (archive = archive_update of the batch's
+  pool, restarts reset the trails to the new upper bound, feedback = the
+  reference's update_feedback).
 """
 from __future__ import annotations
 
@@ -68,6 +84,57 @@ class Settings(ctypes.Structure):
         ("ls_evaluation_limit", ctypes.c_uint64),
         ("initial_ls_evaluation_limit", ctypes.c_uint64),
     ]
+
+
+class Feedback(ctypes.Structure):
+    """FeedbackState (reference cpp/include/gp_faco/control_ops.hpp:160-163)."""
+
+    _fields_ = [("return_rate", ctypes.c_float), ("ls_work", ctypes.c_float), ("stagnant_batches", ctypes.c_uint64),
+                ("epoch_batches", ctypes.c_uint64), ("restarts", ctypes.c_uint64)]
+
+
+class Baseline(ctypes.Structure):
+    """BaselinePolicy (reference cpp/include/gp_faco/baseline_policy.hpp:20-27), 48 bytes.
+
+    kind 0 static / 1 rule; restart_mode 0 none / 1 periodic / 2 bernoulli."""
+
+    _fields_ = [("kind", ctypes.c_uint32), ("mne_level", ctypes.c_uint32), ("max_mne_level", ctypes.c_uint32),
+                ("region", ctypes.c_uint32), ("restart_mode", ctypes.c_uint32), ("restart_period", ctypes.c_uint32),
+                ("restart_probability", ctypes.c_double), ("stagnation_step", ctypes.c_uint32),
+                ("restart_stagnation", ctypes.c_uint32), ("restart_cooldown", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class ControlOut(ctypes.Structure):
+    _fields_ = [("archive_size", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("stagnant_batches", ctypes.c_uint64),
+                ("epoch_batches", ctypes.c_uint64), ("restarts", ctypes.c_uint64), ("return_rate", ctypes.c_float),
+                ("ls_work", ctypes.c_float)]
+
+
+class Program(ctypes.Structure):
+    """Program IR v1 record (reference cpp/include/gp_faco/program.hpp:10-19), 388 bytes."""
+
+    _fields_ = [("ir_version", ctypes.c_uint16), ("numeric_spec_id", ctypes.c_uint16),
+                ("feature_spec_id", ctypes.c_uint16), ("length", ctypes.c_uint16),
+                ("constants_count", ctypes.c_uint16), ("opcode", ctypes.c_uint8 * 63),
+                ("operand", ctypes.c_uint8 * 63), ("constants", ctypes.c_float * 63)]
+
+
+class Controller(ctypes.Structure):
+    """ControllerProgram (reference cpp/include/gp_faco/program.hpp:27-30), 1168 bytes."""
+
+    _fields_ = [("kind", ctypes.c_uint32), ("trees", Program * 3)]
+
+
+def make_program(opcode, operand, constants=(), feature_spec_id=2) -> Program:
+    p = Program()
+    p.ir_version, p.numeric_spec_id, p.feature_spec_id = 1, 1, feature_spec_id
+    p.length, p.constants_count = len(opcode), len(constants)
+    for i, (o, a) in enumerate(zip(opcode, operand)):
+        p.opcode[i], p.operand[i] = o, a
+    for i, c in enumerate(constants):
+        p.constants[i] = c
+    return p
 
 
 def default_settings(**overrides) -> Settings:
@@ -122,6 +189,41 @@ def lib():
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double)]
         L.oracle_max_threads.restype = ctypes.c_int
+        L.oracle_tour_ident.argtypes = [ctypes.c_uint32, u32p, u64p]
+        L.oracle_canonical_compare.restype = ctypes.c_int
+        L.oracle_canonical_compare.argtypes = [ctypes.c_uint32, u32p, u32p]
+        L.oracle_sample_nodes.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u32p]
+        L.oracle_remaining_node.restype = ctypes.c_uint32
+        L.oracle_remaining_node.argtypes = [ctypes.c_uint32, u32p, ctypes.c_uint32]
+        L.oracle_build_region.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint64,
+                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p]
+        L.oracle_baseline_action.restype = ctypes.c_uint32
+        L.oracle_baseline_action.argtypes = [ctypes.POINTER(Baseline), ctypes.POINTER(Feedback), ctypes.c_uint32,
+                                             ctypes.c_double, ctypes.c_uint32]
+        L.oracle_baseline_uniform.restype = ctypes.c_double
+        L.oracle_baseline_uniform.argtypes = [ctypes.POINTER(Baseline), ctypes.c_uint64, ctypes.c_uint32]
+        L.oracle_factorial_mask.restype = ctypes.c_uint32
+        L.oracle_factorial_mask.argtypes = [ctypes.c_uint32, ctypes.POINTER(Baseline), ctypes.POINTER(Feedback),
+                                            ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32]
+        L.oracle_update_feedback.argtypes = [ctypes.POINTER(Feedback), ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_uint32]
+        L.oracle_restart_feedback.argtypes = [ctypes.POINTER(Feedback)]
+        L.oracle_score_controller.argtypes = [ctypes.c_void_p, f32p, ctypes.c_uint32, f32p,
+                                              ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_archive_update.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32,
+                                            u32p, u32p, ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_action_snapshot.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double,
+                                             u32p, u32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.POINTER(Feedback),
+                                             ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                             f32p, u32p, ctypes.POINTER(ctypes.c_uint32),
+                                             ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_run_controlled.argtypes = [ctypes.c_uint32, ctypes.c_uint32, i32p, u64p, ctypes.c_uint32, u64p,
+                                            ctypes.POINTER(Settings), ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.POINTER(Baseline), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                            u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
+                                            ctypes.POINTER(ctypes.c_double)]
         L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
@@ -414,3 +516,292 @@ def ref_score_cpu(opcode, operand, constants, features, masks, feature_spec_id=1
                            actions), R)
     return scores.reshape(colonies, 32), np.array(actions[:], dtype=np.int32)
 
+
+
+# ----------------------------------------------------------------------------
+# Control layer.
+
+def tour_ident(tour):
+    tour = np.ascontiguousarray(tour, dtype=np.uint32)
+    out = np.zeros(2, dtype=np.uint64)
+    lib().oracle_tour_ident(tour.size, tour, out)
+    return int(out[0]), int(out[1])
+
+
+def canonical_compare(a, b) -> int:
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return int(lib().oracle_canonical_compare(a.size, a, np.ascontiguousarray(b, dtype=np.uint32)))
+
+
+def sample_nodes(n, instance_key):
+    out = np.zeros(64, dtype=np.uint32)
+    lib().oracle_sample_nodes(n, instance_key, out)
+    return out[: min(n, 64)]
+
+
+def remaining_node(rank, existing) -> int:
+    existing = np.ascontiguousarray(existing, dtype=np.uint32)
+    return int(lib().oracle_remaining_node(rank, existing if existing.size else np.zeros(1, np.uint32),
+                                           existing.size))
+
+
+def build_region(tour, primary, key, batch, mode, region):
+    tour = np.ascontiguousarray(tour, dtype=np.uint32)
+    primary = np.ascontiguousarray(primary, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint32)
+    lib().oracle_build_region(tour.size, tour, primary.reshape(-1), primary.shape[1], key, batch, mode, region, out)
+    return out[: min(tour.size, 16)]
+
+
+def baseline_action(policy: Baseline, feedback: Feedback, batch, uniform, mask) -> int:
+    return int(lib().oracle_baseline_action(ctypes.byref(policy), ctypes.byref(feedback), batch, uniform, mask))
+
+
+def baseline_uniform(policy: Baseline, key, batch) -> float:
+    return float(lib().oracle_baseline_uniform(ctypes.byref(policy), key, batch))
+
+
+def factorial_mask(variant, policy: Baseline, feedback: Feedback, batch, uniform, legal) -> int:
+    return int(lib().oracle_factorial_mask(variant, ctypes.byref(policy), ctypes.byref(feedback), batch, uniform,
+                                           legal))
+
+
+def update_feedback(feedback: Feedback, improved, returned, work, ants) -> None:
+    lib().oracle_update_feedback(ctypes.byref(feedback), int(bool(improved)), returned, work, ants)
+
+
+def restart_feedback(feedback: Feedback) -> None:
+    lib().oracle_restart_feedback(ctypes.byref(feedback))
+
+
+def score_controller(controller: Controller, features, mask):
+    features = np.ascontiguousarray(features, dtype=np.float32).reshape(12 * 32)
+    scores = np.zeros(32, dtype=np.float32)
+    action = ctypes.c_uint32()
+    L = lib()
+    _check(L.oracle_score_controller(ctypes.addressof(controller), features, mask, scores, ctypes.byref(action)), L)
+    return scores, action.value
+
+
+def archive_update(xy, global_tour, entries, completed):
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
+    n = xy.size // 2
+    entries = np.ascontiguousarray(entries, dtype=np.uint32).reshape(-1, n)
+    completed = np.ascontiguousarray(completed, dtype=np.uint32).reshape(-1, n)
+    out = np.zeros((4, n), dtype=np.uint32)
+    count = ctypes.c_uint32()
+    L = lib()
+    _check(L.oracle_archive_update(n, xy, np.ascontiguousarray(global_tour, dtype=np.uint32), entries.shape[0],
+                                   entries.reshape(-1) if entries.size else np.zeros(1, np.uint32),
+                                   completed.shape[0], completed.reshape(-1) if completed.size else np.zeros(1, np.uint32),
+                                   out.reshape(-1), ctypes.byref(count)), L)
+    return out[: count.value]
+
+
+def action_snapshot(xy, primary, local_scale, epsilon, samples, active, archive, trails, default_trail, minimum,
+                    maximum, feedback: Feedback, elapsed, key, batch, mask=0xFFFFFFFF):
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
+    n = xy.size // 2
+    primary = np.ascontiguousarray(primary, dtype=np.uint32)
+    archive = np.ascontiguousarray(archive, dtype=np.uint32).reshape(-1, n)
+    samples = np.ascontiguousarray(samples, dtype=np.uint32)
+    features = np.zeros(12 * 32, dtype=np.float32)
+    regions = np.zeros(128, dtype=np.uint32)
+    alt, legal = ctypes.c_uint32(), ctypes.c_uint32()
+    L = lib()
+    _check(L.oracle_action_snapshot(n, xy, primary.shape[1], primary.reshape(-1),
+                                    np.ascontiguousarray(local_scale, dtype=np.float64), epsilon, samples,
+                                    np.ascontiguousarray(active, dtype=np.uint32), archive.shape[0],
+                                    archive.reshape(-1), np.ascontiguousarray(trails, dtype=np.float64).reshape(-1),
+                                    default_trail, minimum, maximum, ctypes.byref(feedback), elapsed, key, batch,
+                                    mask, features, regions, ctypes.byref(alt), ctypes.byref(legal)), L)
+    return {"features": features.reshape(12, 32), "regions": regions.reshape(2, 4, 16), "alternative": alt.value,
+            "mask": legal.value}
+
+
+CONTROL_MODES = {"program": 1, "controller": 2, "baseline": 3, "factorial": 4}
+FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
+
+
+def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
+                   controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
+                   trace=False):
+    """Controlled FACO: colony c uses instance c % instances and decider c // instances."""
+    xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
+    instances, n = xy_instances.shape[0], xy_instances.shape[1]
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    colonies = seeds.size
+    keys = np.ascontiguousarray(instance_keys, dtype=np.uint64)
+    tours = np.zeros((colonies, n), dtype=np.uint32)
+    costs = np.zeros(colonies, dtype=np.int64)
+    work = np.zeros((colonies, 2), dtype=np.uint64)
+    states = (ControlOut * colonies)()
+    elapsed = ctypes.c_double()
+    norm = prepare(xy_instances[0], settings)["settings"]
+    batches = evaluation_limit // norm.ants
+    traces = {}
+    if trace:
+        traces = {"features": np.zeros((batches, 12, colonies, 32), dtype=np.float32),
+                  "scores": np.zeros((batches, colonies, 32), dtype=np.float32),
+                  "masks": np.zeros((batches, colonies), dtype=np.uint32),
+                  "actions": np.zeros((batches, colonies), dtype=np.int32),
+                  "alternatives": np.zeros((batches, colonies), dtype=np.uint32),
+                  "regions": np.zeros((batches, colonies, 2, 4, 16), dtype=np.uint32),
+                  "costs": np.zeros((batches, colonies, 3), dtype=np.int64)}
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None  # noqa: E731
+    prog_arr = ctrl_arr = None
+    deciders = 1
+    if programs is not None:
+        prog_arr = (Program * len(programs))(*programs)
+        deciders = len(programs)
+    if controllers is not None:
+        ctrl_arr = (Controller * len(controllers))(*controllers)
+        deciders = len(controllers)
+    policy = baseline if baseline is not None else Baseline()
+    L = lib()
+    _check(L.oracle_run_controlled(
+        n, instances, xy_instances.reshape(-1), keys, colonies, seeds, ctypes.byref(settings), evaluation_limit,
+        CONTROL_MODES[mode], deciders, ctypes.addressof(prog_arr) if prog_arr is not None else None,
+        ctypes.addressof(ctrl_arr) if ctrl_arr is not None else None, ctypes.byref(policy),
+        FACTORIAL_VARIANTS[variant], experiment_mask, threads, tours.reshape(-1), costs, work.reshape(-1),
+        ctypes.addressof(states), ptr(traces.get("features")), ptr(traces.get("scores")),
+        ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
+        ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed)), L)
+    out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
+           "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
+                       "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,
+                       "ls_work": s.ls_work} for s in states]}
+    out.update({"trace_" + k: v for k, v in traces.items()})
+    return out
+
+
+# Reference checker, control layer.
+
+def _ref_control():
+    R = ref()
+    if not getattr(R, "_control_bound", False):
+        R.ref_fingerprint.argtypes = [ctypes.c_uint32, u32p, u64p]
+        R.ref_canonical_compare.restype = ctypes.c_int
+        R.ref_canonical_compare.argtypes = [ctypes.c_uint32, u32p, u32p]
+        R.ref_sample_nodes.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u32p]
+        R.ref_build_region.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint64,
+                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p]
+        R.ref_validate_baseline.argtypes = [ctypes.POINTER(Baseline), ctypes.c_uint32]
+        R.ref_baseline_action.restype = ctypes.c_uint32
+        R.ref_baseline_action.argtypes = [ctypes.POINTER(Baseline), ctypes.POINTER(Feedback), ctypes.c_uint32,
+                                          ctypes.c_double, ctypes.c_uint32]
+        R.ref_factorial_mask.restype = ctypes.c_uint32
+        R.ref_factorial_mask.argtypes = [ctypes.c_uint32, ctypes.POINTER(Baseline), ctypes.POINTER(Feedback),
+                                         ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32]
+        R.ref_update_feedback.argtypes = [ctypes.POINTER(Feedback), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_uint32]
+        R.ref_restart_feedback.argtypes = [ctypes.POINTER(Feedback)]
+        R.ref_archive_update.argtypes = [ctypes.c_uint32, f64p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p,
+                                         u32p, ctypes.POINTER(ctypes.c_uint32)]
+        R.ref_action_snapshot.argtypes = [ctypes.c_uint32, f64p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, u32p,
+                                          ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_uint32, u32p, f64p,
+                                          ctypes.c_double, ctypes.c_double, ctypes.POINTER(Feedback),
+                                          ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, f32p,
+                                          u32p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                          f64p, ctypes.POINTER(ctypes.c_double), u32p,
+                                          ctypes.POINTER(ctypes.c_uint32)]
+        R._control_bound = True
+    return R
+
+
+def ref_fingerprint(tour):
+    tour = np.ascontiguousarray(tour, dtype=np.uint32)
+    out = np.zeros(2, dtype=np.uint64)
+    _ref_control().ref_fingerprint(tour.size, tour, out)
+    return int(out[0]), int(out[1])
+
+
+def ref_canonical_compare(a, b) -> int:
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return int(_ref_control().ref_canonical_compare(a.size, a, np.ascontiguousarray(b, dtype=np.uint32)))
+
+
+def ref_sample_nodes(n, key):
+    out = np.zeros(64, dtype=np.uint32)
+    _ref_control().ref_sample_nodes(n, key, out)
+    return out[: min(n, 64)]
+
+
+def ref_build_region(tour, primary, key, batch, mode, region):
+    tour = np.ascontiguousarray(tour, dtype=np.uint32)
+    primary = np.ascontiguousarray(primary, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint32)
+    _ref_control().ref_build_region(tour.size, tour, primary.reshape(-1), primary.shape[1], key, batch, mode,
+                                    region, out)
+    return out[: min(tour.size, 16)]
+
+
+def ref_validate_baseline(policy: Baseline, mask=0xFFFFFFFF) -> bool:
+    return _ref_control().ref_validate_baseline(ctypes.byref(policy), mask) == 0
+
+
+def ref_baseline_action(policy, feedback, batch, uniform, mask) -> int:
+    return int(_ref_control().ref_baseline_action(ctypes.byref(policy), ctypes.byref(feedback), batch, uniform,
+                                                  mask))
+
+
+def ref_factorial_mask(variant, policy, feedback, batch, uniform, legal) -> int:
+    return int(_ref_control().ref_factorial_mask(variant, ctypes.byref(policy), ctypes.byref(feedback), batch,
+                                                 uniform, legal))
+
+
+def ref_update_feedback(feedback, improved, returned, work, ants) -> None:
+    _ref_control().ref_update_feedback(ctypes.byref(feedback), int(bool(improved)), returned, work, ants)
+
+
+def ref_restart_feedback(feedback) -> None:
+    _ref_control().ref_restart_feedback(ctypes.byref(feedback))
+
+
+def ref_archive_update(matrix, global_tour, entries, completed):
+    matrix = np.ascontiguousarray(matrix, dtype=np.float64)
+    n = matrix.shape[0]
+    entries = np.ascontiguousarray(entries, dtype=np.uint32).reshape(-1, n)
+    completed = np.ascontiguousarray(completed, dtype=np.uint32).reshape(-1, n)
+    out = np.zeros((4, n), dtype=np.uint32)
+    count = ctypes.c_uint32()
+    R = _ref_control()
+    _check(R.ref_archive_update(n, matrix.reshape(-1), np.ascontiguousarray(global_tour, dtype=np.uint32),
+                                entries.shape[0], entries.reshape(-1) if entries.size else np.zeros(1, np.uint32),
+                                completed.shape[0],
+                                completed.reshape(-1) if completed.size else np.zeros(1, np.uint32),
+                                out.reshape(-1), ctypes.byref(count)), R)
+    return out[: count.value]
+
+
+def ref_action_snapshot(matrix, primary, local_scale, epsilon, samples, active, archive_tours, pheromone_initial,
+                        retention, evaporation_floor, deposit_parents, deposit_amounts, minimum, maximum,
+                        feedback, elapsed, key, batch, mask=0xFFFFFFFF):
+    matrix = np.ascontiguousarray(matrix, dtype=np.float64)
+    n = matrix.shape[0]
+    primary = np.ascontiguousarray(primary, dtype=np.uint32)
+    pw = primary.shape[1]
+    archive_tours = np.ascontiguousarray(archive_tours, dtype=np.uint32).reshape(-1, n)
+    deposit_parents = np.ascontiguousarray(deposit_parents, dtype=np.uint32).reshape(-1, n)
+    samples = np.ascontiguousarray(samples, dtype=np.uint32)
+    features = np.zeros(12 * 32, dtype=np.float32)
+    regions = np.zeros(128, dtype=np.uint32)
+    alt, legal, count = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    trails = np.zeros(n * pw)
+    default = ctypes.c_double()
+    archive_out = np.zeros((4, n), dtype=np.uint32)
+    R = _ref_control()
+    _check(R.ref_action_snapshot(n, matrix.reshape(-1), pw, primary.reshape(-1),
+                                 np.ascontiguousarray(local_scale, dtype=np.float64), epsilon, samples, samples.size,
+                                 np.ascontiguousarray(active, dtype=np.uint32), archive_tours.shape[0],
+                                 archive_tours.reshape(-1), pheromone_initial, retention, evaporation_floor,
+                                 deposit_parents.shape[0],
+                                 deposit_parents.reshape(-1) if deposit_parents.size else np.zeros(1, np.uint32),
+                                 np.ascontiguousarray(deposit_amounts, dtype=np.float64) if len(deposit_amounts)
+                                 else np.zeros(1), minimum, maximum, ctypes.byref(feedback), elapsed, key, batch,
+                                 mask, features, regions, ctypes.byref(alt), ctypes.byref(legal), trails,
+                                 ctypes.byref(default), archive_out.reshape(-1), ctypes.byref(count)), R)
+    return {"features": features.reshape(12, 32), "regions": regions.reshape(2, 4, 16), "alternative": alt.value,
+            "mask": legal.value, "trails": trails.reshape(n, pw), "default": default.value,
+            "archive": archive_out[: count.value]}

@@ -8,7 +8,10 @@
 // The reference classes take an arbitrary DistanceFunction
 // (cpp/include/gp_faco/faco_cpu.hpp:13), so the shim hands them the same
 // integer EUC_2D distances the oracle uses, as FP64 values.
+#include "gp_faco/baseline_policy.hpp"
+#include "gp_faco/control_cpu.hpp"
 #include "gp_faco/faco_cpu.hpp"
+#include "gp_faco/factorial_policy.hpp"
 #include "gp_faco/program.hpp"
 
 #include <algorithm>
@@ -152,6 +155,163 @@ int ref_score_cpu(const std::uint8_t* opcode, const std::uint8_t* operand, Node 
                                             std::vector<std::uint32_t>(masks, masks + colonies));
         std::copy(out.scores.begin(), out.scores.end(), scores);
         std::copy(out.actions.begin(), out.actions.end(), actions);
+    });
+}
+
+
+// ---------------------------------------------------------------------------
+// Control layer (cpp/include/gp_faco/control_ops.hpp, control_cpu.cpp,
+// baseline_policy.hpp, factorial_policy.hpp).
+
+void ref_fingerprint(Node n, const std::uint32_t* tour, std::uint64_t* out) {
+    std::vector<Node> positions(n);
+    for (Node i = 0; i < n; ++i) positions[tour[i]] = i;
+    const auto id = gp_faco::fingerprint({tour, positions.data(), n});
+    out[0] = id.first; out[1] = id.second;
+}
+
+int ref_canonical_compare(Node n, const std::uint32_t* a, const std::uint32_t* b) {
+    std::vector<Node> pa(n), pb(n);
+    for (Node i = 0; i < n; ++i) { pa[a[i]] = i; pb[b[i]] = i; }
+    return gp_faco::canonical_compare({a, pa.data(), n}, {b, pb.data(), n});
+}
+
+void ref_sample_nodes(Node n, std::uint64_t key, std::uint32_t* out) { gp_faco::sample_nodes(n, key, out); }
+
+void ref_build_region(Node n, const std::uint32_t* tour, const std::uint32_t* primary, Node width,
+                      std::uint64_t key, Node batch, Node mode, Node region, std::uint32_t* nodes) {
+    std::vector<Node> positions(n);
+    for (Node i = 0; i < n; ++i) positions[tour[i]] = i;
+    gp_faco::build_region({tour, positions.data(), n}, primary, width, key, batch, mode, region, nodes);
+}
+
+struct RefFeedback { float return_rate, ls_work; std::uint64_t stagnant, epoch_batches, restarts; };
+
+gp_faco::FeedbackState feedback_of(const RefFeedback& f) {
+    gp_faco::FeedbackState s;
+    s.return_rate = f.return_rate; s.ls_work = f.ls_work; s.stagnant_batches = f.stagnant;
+    s.epoch_batches = f.epoch_batches; s.restarts = f.restarts;
+    return s;
+}
+
+struct RefBaseline {
+    std::uint32_t kind, mne_level, max_mne_level, region, restart_mode, restart_period;
+    double restart_probability;
+    std::uint32_t stagnation_step, restart_stagnation, restart_cooldown, pad;
+};
+
+gp_faco::BaselinePolicy policy_of(const RefBaseline& b) {
+    gp_faco::BaselinePolicy p;
+    p.kind = b.kind ? gp_faco::BaselineKind::Rule : gp_faco::BaselineKind::Static;
+    p.mne_level = b.mne_level; p.max_mne_level = b.max_mne_level; p.region = b.region;
+    p.restart_mode = b.restart_mode == 0 ? gp_faco::StaticRestart::None :
+        b.restart_mode == 1 ? gp_faco::StaticRestart::Periodic : gp_faco::StaticRestart::Bernoulli;
+    p.restart_period = b.restart_period; p.restart_probability = b.restart_probability;
+    p.stagnation_step = b.stagnation_step; p.restart_stagnation = b.restart_stagnation;
+    p.restart_cooldown = b.restart_cooldown;
+    return p;
+}
+
+int ref_validate_baseline(const RefBaseline* b, std::uint32_t mask) {
+    return guarded([&] { gp_faco::validate_baseline(policy_of(*b), mask); });
+}
+
+Node ref_baseline_action(const RefBaseline* b, const RefFeedback* f, Node batch, double uniform, std::uint32_t mask) {
+    return gp_faco::baseline_action(policy_of(*b), feedback_of(*f), batch, uniform, mask);
+}
+
+std::uint32_t ref_factorial_mask(std::uint32_t variant, const RefBaseline* b, const RefFeedback* f, Node batch,
+                                 double uniform, std::uint32_t legal) {
+    gp_faco::FactorialPolicy p;
+    p.variant = static_cast<gp_faco::FactorialVariant>(variant);
+    p.baseline = policy_of(*b);
+    return gp_faco::factorial_mask(p, feedback_of(*f), batch, uniform, legal);
+}
+
+void ref_update_feedback(RefFeedback* f, int improved, double returned, double work, Node ants) {
+    auto s = feedback_of(*f);
+    gp_faco::update_feedback(s, improved != 0, returned, work, ants);
+    *f = {s.return_rate, s.ls_work, s.stagnant_batches, s.epoch_batches, s.restarts};
+}
+
+void ref_restart_feedback(RefFeedback* f) {
+    auto s = feedback_of(*f);
+    gp_faco::restart_feedback(s);
+    *f = {s.return_rate, s.ls_work, s.stagnant_batches, s.epoch_batches, s.restarts};
+}
+
+namespace {
+std::vector<gp_faco::TourRecord> records(const std::uint32_t* tours, Node count, Node n,
+                                         const gp_faco::DistanceFunction& distance) {
+    std::vector<gp_faco::TourRecord> out;
+    for (Node i = 0; i < count; ++i)
+        out.emplace_back(std::vector<Node>(tours + static_cast<std::size_t>(i) * n,
+                                           tours + static_cast<std::size_t>(i + 1) * n), distance);
+    return out;
+}
+}  // namespace
+
+// ReferenceArchive(global).update(global, entries) then .update(global,
+// completed): the archive built from `entries`, then one batch update.
+int ref_archive_update(Node n, const double* matrix, const std::uint32_t* global, Node entries,
+                       const std::uint32_t* entry_tours, Node completed, const std::uint32_t* completed_tours,
+                       std::uint32_t* out_tours, Node* out_count) {
+    return guarded([&] {
+        const auto distance = matrix_distance(matrix, n);
+        const gp_faco::TourRecord g(std::vector<Node>(global, global + n), distance);
+        gp_faco::ReferenceArchive archive(g);
+        archive.update(g, records(entry_tours, entries, n, distance));
+        archive.update(g, records(completed_tours, completed, n, distance));
+        const auto& chosen = archive.entries();
+        for (Node i = 0; i < chosen.size(); ++i)
+            std::copy(chosen[i].order.begin(), chosen[i].order.end(), out_tours + static_cast<std::size_t>(i) * n);
+        *out_count = static_cast<Node>(chosen.size());
+    });
+}
+
+// make_action_snapshot (cpp/src/control_cpu.cpp:75-120) with the archive
+// entries built as in ref_archive_update from `archive_tours` (slot 0 =
+// global) and a SparsePheromone holding `trails` / `default_trail`.
+int ref_action_snapshot(Node n, const double* matrix, Node pw, const std::uint32_t* primary,
+                        const double* local_scale, double epsilon, const std::uint32_t* samples, Node sample_count,
+                        const std::uint32_t* active, Node archive_size, const std::uint32_t* archive_tours,
+                        double pheromone_initial, double retention, double evaporation_floor, Node deposits,
+                        const std::uint32_t* deposit_parents, const double* deposit_amounts, double minimum,
+                        double maximum, const RefFeedback* feedback, double elapsed, std::uint64_t key, Node batch,
+                        std::uint32_t experiment_mask, float* features, std::uint32_t* regions,
+                        Node* alternative, std::uint32_t* mask, double* trails_out, double* default_out,
+                        std::uint32_t* archive_out, Node* archive_count) {
+    return guarded([&] {
+        const auto distance = matrix_distance(matrix, n);
+        gp_faco::ControlStaticData problem{n, pw, std::vector<Node>(primary, primary + static_cast<std::size_t>(n) * pw),
+            std::vector<Node>(samples, samples + sample_count), std::vector<double>(local_scale, local_scale + n), epsilon};
+        const auto rows = rows_of(primary, n, pw);
+        gp_faco::SparsePheromone pheromone(rows, pheromone_initial, true);
+        for (Node s = 0; s < deposits; ++s) {
+            const std::uint32_t* parent = deposit_parents + static_cast<std::size_t>(s) * n;
+            pheromone.evaporate(retention, evaporation_floor);
+            for (Node i = 0; i < n; ++i)
+                pheromone.deposit(parent[(i + n - 1) % n], parent[i], deposit_amounts[s], 1e300);
+        }
+        for (Node a = 0; a < n; ++a) for (Node j = 0; j < pw; ++j)
+            trails_out[static_cast<std::size_t>(a) * pw + j] = pheromone.get(a, primary[static_cast<std::size_t>(a) * pw + j]);
+        *default_out = pheromone.default_value();
+        const auto all = records(archive_tours, archive_size, n, distance);
+        gp_faco::ReferenceArchive archive(all[0]);
+        archive.update(all[0], std::vector<gp_faco::TourRecord>(all.begin() + 1, all.end()));
+        *archive_count = static_cast<Node>(archive.entries().size());
+        for (Node i = 0; i < archive.entries().size(); ++i)
+            std::copy(archive.entries()[i].order.begin(), archive.entries()[i].order.end(),
+                      archive_out + static_cast<std::size_t>(i) * n);
+        const gp_faco::TourRecord act(std::vector<Node>(active, active + n), distance);
+        gp_faco::FeedbackState fb = feedback_of(*feedback);
+        const auto snap = gp_faco::make_action_snapshot(problem, act, archive, fb, pheromone, {minimum, maximum},
+                                                        distance, elapsed, key, batch, experiment_mask);
+        std::copy(snap.features.begin(), snap.features.end(), features);
+        for (Node m = 0; m < 2; ++m) for (Node r = 0; r < 4; ++r)
+            std::copy(snap.regions.nodes[m][r], snap.regions.nodes[m][r] + 16, regions + (m * 4 + r) * 16);
+        *alternative = snap.alternative_slot;
+        *mask = snap.legal_mask;
     });
 }
 
