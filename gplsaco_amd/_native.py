@@ -97,8 +97,79 @@ class Program(ctypes.Structure):
     ]
 
 
+class Controller(ctypes.Structure):
+    """faco_controller: the reference ControllerProgram layout (1168 bytes)."""
+
+    _fields_ = [("kind", ctypes.c_uint32), ("trees", Program * 3)]
+
+
+class BaselinePolicy(ctypes.Structure):
+    """faco_baseline_policy: the reference BaselinePolicy layout (48 bytes)."""
+
+    _fields_ = [
+        ("kind", ctypes.c_uint32),
+        ("mne_level", ctypes.c_uint32),
+        ("max_mne_level", ctypes.c_uint32),
+        ("region", ctypes.c_uint32),
+        ("restart_mode", ctypes.c_uint32),
+        ("restart_period", ctypes.c_uint32),
+        ("restart_probability", ctypes.c_double),
+        ("stagnation_step", ctypes.c_uint32),
+        ("restart_stagnation", ctypes.c_uint32),
+        ("restart_cooldown", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
+class ControlState(ctypes.Structure):
+    _fields_ = [
+        ("archive_size", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("stagnant_batches", ctypes.c_uint64),
+        ("epoch_batches", ctypes.c_uint64),
+        ("restarts", ctypes.c_uint64),
+        ("return_rate", ctypes.c_float),
+        ("ls_work", ctypes.c_float),
+    ]
+
+
+CONTROL_PROGRAM, CONTROL_CONTROLLER, CONTROL_BASELINE, CONTROL_FACTORIAL = 1, 2, 3, 4
+
+
+class ControlSpec(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_uint32),
+        ("deciders", ctypes.c_uint32),
+        ("programs", ctypes.c_void_p),
+        ("controllers", ctypes.c_void_p),
+        ("baseline", BaselinePolicy),
+        ("factorial_variant", ctypes.c_uint32),
+        ("experiment_mask", ctypes.c_uint32),
+        ("force_fingerprint_collisions", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
+class ControlTrace(ctypes.Structure):
+    _fields_ = [
+        ("batches", ctypes.c_void_p),
+        ("count", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("features", ctypes.c_void_p),
+        ("scores", ctypes.c_void_p),
+        ("masks", ctypes.c_void_p),
+        ("actions", ctypes.c_void_p),
+        ("alternatives", ctypes.c_void_p),
+        ("regions", ctypes.c_void_p),
+        ("costs", ctypes.c_void_p),
+    ]
+
+
 assert ctypes.sizeof(Settings) == 64
 assert ctypes.sizeof(Program) == 388
+assert ctypes.sizeof(Controller) == 1168
+assert ctypes.sizeof(BaselinePolicy) == 48
+assert ctypes.sizeof(ControlState) == 40
 
 # (name, restype, argtypes) of every exported entry point of include/faco_engine.h
 SIGNATURES = {
@@ -141,6 +212,14 @@ SIGNATURES = {
                                               ctypes.c_uint32, u32p]),
     "faco_score_cpu": (ctypes.c_int, [ctypes.POINTER(Program), f32p, u32p, ctypes.c_uint32, f32p, i32p]),
     "faco_score_gpu": (ctypes.c_int, [ctypes.POINTER(Program), f32p, u32p, ctypes.c_uint32, f32p, i32p]),
+    "faco_score_controller_cpu": (ctypes.c_int, [ctypes.POINTER(Controller), f32p, u32p, ctypes.c_uint32, f32p,
+                                                 i32p]),
+    "faco_score_controller_gpu": (ctypes.c_int, [ctypes.POINTER(Controller), f32p, u32p, ctypes.c_uint32, f32p,
+                                                 i32p]),
+    "faco_engine_evaluate_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64,
+                                                    ctypes.POINTER(ControlSpec), ctypes.c_uint32,
+                                                    ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,
+                                                    u64p, vp]),
 }
 
 _lib = None

@@ -6,6 +6,7 @@
 // (cpp/src/bindings.cpp:500-1046) and engine host code
 // (cuda/batch_engine.cu:78-1192, cuda/fixed_faco.cu:44-177).
 #include "faco_common.h"
+#include "gp_program.h"
 #include "host_prepare.h"
 
 #include <algorithm>
@@ -136,12 +137,28 @@ struct Engine {
     double p_root = 0;
     // device state
     DeviceGeometry geo;
-    DeviceArray<uint32_t> geometry, targets, parent, links, epoch, global, tours;
+    DeviceArray<uint32_t> geometry, parent, links, epoch, global, tours;
     DeviceArray<uint64_t> keys;
     DeviceArray<ColonyState> state;
     DeviceArray<double> trails, products;
     DeviceArray<faco_ant_info> info;
     std::vector<uint64_t> resident;  // instance key per geometry slot
+    // controlled-loop buffers, allocated on the first controlled evaluation
+    struct ControlBuffers {
+        DeviceArray<ControlState> control;          // [C]
+        DeviceArray<uint32_t> archive_tours, archive_links;  // [C][2][4][n]
+        DeviceArray<uint64_t> ant_identity;         // [A][2]
+        DeviceArray<uint32_t> ant_zero;             // [A]
+        DeviceArray<uint32_t> samples;              // [G][64]
+        DeviceArray<double> local_scale, epsilon;   // [G][n], [G]
+        DeviceArray<faco_program> programs;         // [population]
+        DeviceArray<faco_controller> controllers;   // [population]
+        DeviceArray<float> features, scores;        // [12][C][32], [C][32]
+        DeviceArray<uint32_t> masks, alternatives, regions;  // [C], [C], [C][128]
+        DeviceArray<int32_t> actions;               // [C]
+    };
+    std::unique_ptr<ControlBuffers> ctl;
+    bool resident_control = false;
     // graph cache
     hipGraphExec_t graph = nullptr;
     uint32_t graph_colonies = 0, graph_iterations = 0;
@@ -165,11 +182,11 @@ struct Engine {
         checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         checked(hipEventCreate(&ev_start), "hipEventCreate");
         checked(hipEventCreate(&ev_stop), "hipEventCreate");
-        const size_t C = colonies_capacity, A = C * config.ants, G = C;
+        const size_t C = colonies_capacity, A = C * config.ants, G = panel;
         auto alloc = [&](auto& buffer, size_t count) { buffer.allocate(count); allocated += buffer.bytes(); };
         alloc(geo.coords, G * n); alloc(geo.nbr, G * nbr_stride); alloc(geo.backup, G * n * std::max(config.backup_width, 1u));
         alloc(geo.eta, G * n * config.primary_width); alloc(geo.initial, G * n); alloc(geo.initial_cost, G);
-        alloc(geometry, C); alloc(keys, C); alloc(targets, A); alloc(state, C);
+        alloc(geometry, C); alloc(keys, C); alloc(state, C);
         alloc(parent, C * n); alloc(links, C * n); alloc(epoch, C * n); alloc(global, C * n);
         alloc(trails, C * n * config.primary_width); alloc(products, C * n * config.primary_width);
         alloc(tours, A * n); alloc(info, A);
@@ -179,6 +196,22 @@ struct Engine {
         if (ev_start) (void)hipEventDestroy(ev_start);
         if (ev_stop) (void)hipEventDestroy(ev_stop);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void ensure_control() {
+        if (ctl) return;
+        auto b = std::make_unique<ControlBuffers>();
+        const size_t C = colonies_capacity, A = C * config.ants, G = panel;
+        auto alloc = [&](auto& buffer, size_t count) { buffer.allocate(count); allocated += buffer.bytes(); };
+        alloc(b->control, C);
+        alloc(b->archive_tours, C * 2 * kArchive * n); alloc(b->archive_links, C * 2 * kArchive * n);
+        alloc(b->ant_identity, 2 * A); alloc(b->ant_zero, A);
+        alloc(b->samples, G * kSamples); alloc(b->local_scale, G * n); alloc(b->epsilon, G);
+        alloc(b->programs, population); alloc(b->controllers, population);
+        alloc(b->features, 12 * 32 * C); alloc(b->scores, 32 * C);
+        alloc(b->masks, C); alloc(b->alternatives, C); alloc(b->regions, 128 * C); alloc(b->actions, C);
+        ctl = std::move(b);
+        resident.clear();  // the control geometry is uploaded with the next geometry upload
     }
 
     void set_waves(uint32_t waves, uint32_t blocks_per_colony) {
@@ -209,7 +242,8 @@ struct Engine {
             if (found == slots.end()) slots.push_back(instance_keys[c]);
         }
         geometry.upload(map.data(), count, stream);
-        if (slots == resident) return;
+        require(slots.size() <= panel, "more distinct instances than the engine's panel");
+        if (slots == resident && (!ctl || resident_control)) return;
         const uint32_t bw = config.backup_width, pw = config.primary_width;
         for (uint32_t g = 0; g < slots.size(); ++g) {
             const Prepared& p = problem(slots[g]);
@@ -219,9 +253,16 @@ struct Engine {
             geo.eta.upload(p.eta.data(), static_cast<size_t>(n) * pw, stream, static_cast<size_t>(g) * n * pw);
             geo.initial.upload(p.initial.data(), n, stream, static_cast<size_t>(g) * n);
             geo.initial_cost.upload(&p.initial_cost, 1, stream, g);
+            if (ctl) {
+                const auto samples = sample_nodes(n, slots[g]);
+                ctl->samples.upload(samples.data(), kSamples, stream, static_cast<size_t>(g) * kSamples);
+                ctl->local_scale.upload(p.local_scale.data(), n, stream, static_cast<size_t>(g) * n);
+                ctl->epsilon.upload(&p.scale_epsilon, 1, stream, g);
+            }
         }
         checked(hipStreamSynchronize(stream), "geometry upload");
         resident = slots;
+        resident_control = ctl != nullptr;
     }
 
     AntParams ant_params(uint32_t colonies) const {
@@ -245,7 +286,7 @@ struct Engine {
         p.ls_limit = config.ls_evaluation_limit;
         p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
         p.products = products.data(); p.parent = parent.data(); p.links = links.data();
-        p.state = state.data(); p.keys = keys.data(); p.targets = targets.data();
+        p.state = state.data(); p.keys = keys.data(); p.targets = nullptr;
         p.geometry = geometry.data(); p.out_tours = tours.data(); p.out_info = info.data();
         (void)colonies;
         return p;
@@ -261,9 +302,9 @@ struct Engine {
         p.products = products.data();
         return p;
     }
-    void reset(uint32_t colonies) {
+    void reset(uint32_t colonies, uint32_t mne) {
         ResetParams r{};
-        r.n = n; r.ants = config.ants; r.pw = config.primary_width; r.K = K;
+        r.n = n; r.ants = config.ants; r.pw = config.primary_width; r.K = K; r.target = mne;
         r.retention = config.retention; r.p_best_root = p_root;
         r.geometry = geometry.data(); r.initial = geo.initial.data(); r.initial_cost = geo.initial_cost.data();
         r.eta = geo.eta.data(); r.state = state.data(); r.parent = parent.data(); r.links = links.data();
@@ -400,9 +441,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     std::vector<uint64_t> keys(tasks);
     for (uint32_t c = 0; c < tasks; ++c) keys[c] = colony_key(instance_keys[c], seeds[c]);
     e.keys.upload(keys.data(), tasks, e.stream);
-    std::vector<uint32_t> targets(static_cast<size_t>(tasks) * e.config.ants, mne);
-    e.targets.upload(targets.data(), targets.size(), e.stream);
-    e.reset(tasks);
+    e.reset(tasks, mne);
     checked(hipStreamSynchronize(e.stream), "initialisation");
     const auto search_started = Clock::now();
     checked(hipEventRecord(e.ev_start, e.stream), "event");
@@ -425,6 +464,200 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     }
     fill_result(result, e, tasks, limit, batches, steps, evals, seconds_since(started), search, gpu_ms);
     if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+}
+
+// validate_baseline (cpp/include/gp_faco/baseline_policy.hpp:36-56).
+void validate_baseline(const faco_baseline_policy& p, uint32_t mask) {
+    if (p.kind > 1 || p.mne_level > 3 || p.max_mne_level > 3 || p.max_mne_level < p.mne_level || p.region > 3 ||
+        !std::isfinite(p.restart_probability) || p.restart_probability < 0 || p.restart_probability > 1 ||
+        p.restart_mode > 2)
+        throw std::invalid_argument("baseline kind, MNE level, region or probability invalid");
+    if (p.kind == 0) {
+        if (p.max_mne_level != p.mne_level || p.stagnation_step || p.restart_stagnation || p.restart_cooldown)
+            throw std::invalid_argument("a static baseline takes no feedback rule");
+        const bool ok = (p.restart_mode == 0 && !p.restart_period && !p.restart_probability) ||
+                        (p.restart_mode == 1 && p.restart_period && !p.restart_probability) ||
+                        (p.restart_mode == 2 && !p.restart_period);
+        if (!ok) throw std::invalid_argument("static restart mode does not match its period / probability");
+    } else if (p.restart_mode != 0 || p.restart_period || p.restart_probability ||
+               ((p.max_mne_level == p.mne_level) != (p.stagnation_step == 0)) ||
+               ((p.restart_stagnation == 0) != (p.restart_cooldown == 0))) {
+        throw std::invalid_argument("rule baseline needs a stagnation step and restart cooldown in canonical form");
+    }
+    uint32_t required = 0;
+    for (uint32_t level = p.mne_level; level <= p.max_mne_level; ++level) required |= 1u << (p.region * 4 + level);
+    if ((mask & required) != required)
+        throw std::invalid_argument("the experiment mask must keep every hold action the baseline may choose");
+}
+
+// Controlled evaluation: the panel replicated per decider, per batch
+// decide -> ants -> colony update -> control update (reference
+// cuda/batch_engine.cu:880-1060).
+void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* panel_seeds, uint32_t panel,
+                      uint64_t limit, const faco_control_spec& spec, uint32_t flags, const faco_control_trace* trace,
+                      faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
+                      faco_control_state* states) {
+    const auto started = Clock::now();
+    std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+    require(lock.owns_lock(), "one evaluation per engine at a time");
+    require(current_device() == e.device, "engine must be used on the device it was created on");
+    require(spec.mode >= FACO_CONTROL_PROGRAM && spec.mode <= FACO_CONTROL_FACTORIAL, "unknown control mode");
+    require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
+    const uint32_t ants = e.config.ants;
+    require(limit % ants == 0 && limit / ants <= UINT32_MAX, "evaluation limit must be a whole number of ant batches");
+    const bool scored = spec.mode != FACO_CONTROL_BASELINE;
+    const uint32_t deciders = scored ? spec.deciders : 1u;
+    require(deciders >= 1 && deciders <= e.population, "decider count must be in 1..population_size");
+    require(panel >= 1 && panel <= e.panel, "panel size out of range");
+    if (spec.mode == FACO_CONTROL_PROGRAM || spec.mode == FACO_CONTROL_FACTORIAL) {
+        require(spec.programs != nullptr, "programs missing");
+        for (uint32_t d = 0; d < deciders; ++d) {
+            validate_program(spec.programs[d]);
+            require(spec.programs[d].feature_spec_id == 2, "evaluation-count entry points need feature spec 2");
+        }
+    }
+    if (spec.mode == FACO_CONTROL_CONTROLLER) {
+        require(spec.controllers != nullptr, "controllers missing");
+        for (uint32_t d = 0; d < deciders; ++d) validate_controller(spec.controllers[d]);
+    }
+    if (spec.mode == FACO_CONTROL_BASELINE) validate_baseline(spec.baseline, spec.experiment_mask);
+    if (spec.mode == FACO_CONTROL_FACTORIAL) {
+        require(spec.factorial_variant <= 3, "unknown factorial variant");
+        validate_baseline(spec.baseline, 0xffffffffu);
+        if (spec.factorial_variant == 0 || spec.factorial_variant == 2)
+            validate_baseline(spec.baseline, spec.experiment_mask);
+        require(deciders == 1 || spec.factorial_variant == 1 || spec.factorial_variant == 2,
+                "population factorial runs take M10 or M01 only");
+    }
+    const uint32_t C = panel * deciders;
+    require(C <= e.colonies_capacity, "colonies exceed the engine's capacity");
+    for (uint32_t t = 0; t < panel; ++t) (void)e.problem(panel_keys[t]);
+    std::vector<uint32_t> traced;
+    if (trace && trace->count) {
+        require(trace->batches != nullptr, "trace batches missing");
+        traced.assign(trace->batches, trace->batches + trace->count);
+        for (size_t i = 0; i < traced.size(); ++i)
+            require(traced[i] < limit / ants && (i == 0 || traced[i] > traced[i - 1]),
+                    "trace batches must be ascending and inside the budget");
+    }
+    e.ensure_control();
+    auto& b = *e.ctl;
+    // LDS of the archive update: per candidate cost, identity, start, leader, pair
+    require(36ull * (kArchive + ants) <= max_lds_per_block(), "too many ants for the archive update");
+    std::vector<uint64_t> keys(C), inst(C);
+    for (uint32_t c = 0; c < C; ++c) {
+        inst[c] = panel_keys[c % panel];
+        keys[c] = colony_key(panel_keys[c % panel], panel_seeds[c % panel]);
+    }
+    e.upload_geometry(inst.data(), C);
+    e.keys.upload(keys.data(), C, e.stream);
+    if (spec.mode == FACO_CONTROL_PROGRAM || spec.mode == FACO_CONTROL_FACTORIAL)
+        b.programs.upload(spec.programs, deciders, e.stream);
+    if (spec.mode == FACO_CONTROL_CONTROLLER) b.controllers.upload(spec.controllers, deciders, e.stream);
+
+    ControlParams cp{};
+    cp.n = e.n; cp.ants = ants; cp.pw = e.config.primary_width; cp.K = e.K; cp.nbr_stride = e.nbr_stride;
+    cp.panel = panel; cp.mode = spec.mode; cp.variant = spec.factorial_variant;
+    cp.experiment_mask = spec.experiment_mask; cp.force_collisions = spec.force_fingerprint_collisions ? 1u : 0u;
+    cp.evaluation_limit = limit; cp.ls_limit = e.config.ls_evaluation_limit;
+    cp.retention = e.config.retention; cp.p_best_root = e.p_root; cp.baseline = spec.baseline;
+    cp.coords = e.geo.coords.data(); cp.nbr = e.geo.nbr.data(); cp.eta = e.geo.eta.data();
+    cp.local_scale = b.local_scale.data(); cp.epsilon = b.epsilon.data(); cp.samples = b.samples.data();
+    cp.initial = e.geo.initial.data(); cp.initial_cost = e.geo.initial_cost.data();
+    cp.geometry = e.geometry.data(); cp.keys = e.keys.data();
+    cp.programs = b.programs.data(); cp.controllers = b.controllers.data();
+    cp.state = e.state.data(); cp.control = b.control.data();
+    cp.parent = e.parent.data(); cp.links = e.links.data(); cp.epoch = e.epoch.data();
+    cp.trails = e.trails.data(); cp.products = e.products.data();
+    cp.archive_tours = b.archive_tours.data(); cp.archive_links = b.archive_links.data();
+    cp.tours = e.tours.data(); cp.info = e.info.data();
+    cp.ant_identity = b.ant_identity.data(); cp.ant_zero = b.ant_zero.data();
+    cp.features = b.features.data(); cp.scores = b.scores.data(); cp.masks = b.masks.data();
+    cp.actions = b.actions.data(); cp.alternatives = b.alternatives.data(); cp.regions = b.regions.data();
+
+    AntParams ap = e.ant_params(C);
+    ap.control = b.control.data();
+    ap.out_identity = b.ant_identity.data();
+    ap.out_zero = b.ant_zero.data();
+    const ColonyParams colony = e.colony_params();
+    auto iteration = [&]() {
+        checked(launch_control_decide(cp, C, e.stream), "control decide launch");
+        checked(launch_ant_search(ap, C * ap.blocks_per_colony, e.stream), "ant search launch");
+        checked(launch_colony_update(colony, C, e.stream), "colony update launch");
+        checked(launch_control_update(cp, C, e.stream), "control update launch");
+    };
+
+    e.reset(C, 2u);
+    checked(launch_control_init(cp, C, e.stream), "control init launch");
+    checked(hipStreamSynchronize(e.stream), "initialisation");
+    const uint64_t batches = limit / ants;
+    const auto search_started = Clock::now();
+    checked(hipEventRecord(e.ev_start, e.stream), "event");
+    if (!traced.empty()) {
+        size_t next = 0;
+        std::vector<ColonyState> st(C);
+        for (uint64_t batch = 0; batch < batches; ++batch) {
+            iteration();
+            if (next < traced.size() && traced[next] == batch) {
+                const size_t k = next++;
+                if (trace->features) b.features.download(trace->features + k * 12 * 32 * C, 12 * 32 * C, e.stream);
+                if (trace->scores) b.scores.download(trace->scores + k * 32 * C, 32 * C, e.stream);
+                if (trace->masks) b.masks.download(trace->masks + k * C, C, e.stream);
+                if (trace->actions) b.actions.download(trace->actions + k * C, C, e.stream);
+                if (trace->alternatives) b.alternatives.download(trace->alternatives + k * C, C, e.stream);
+                if (trace->regions) b.regions.download(trace->regions + k * 128 * C, 128 * C, e.stream);
+                e.state.download(st.data(), C, e.stream);
+                checked(hipStreamSynchronize(e.stream), "trace batch");
+                if (trace->costs)
+                    for (uint32_t c = 0; c < C; ++c) {
+                        trace->costs[(k * C + c) * 3] = st[c].epoch_cost;
+                        trace->costs[(k * C + c) * 3 + 1] = st[c].global_cost;
+                        trace->costs[(k * C + c) * 3 + 2] = st[c].parent_cost;
+                    }
+            }
+        }
+    } else if ((flags & FACO_RUN_GRAPH) && batches >= 16) {
+        constexpr uint32_t kGraphIterations = 16;
+        e.drop_graph();
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+        checked(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        for (uint32_t i = 0; i < kGraphIterations; ++i) iteration();
+        checked(hipStreamEndCapture(e.stream, &g), "end capture");
+        checked(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        struct ExecGuard { hipGraphExec_t x; ~ExecGuard() { if (x) (void)hipGraphExecDestroy(x); } } guard{exec};
+        const uint64_t rounds = batches / kGraphIterations;
+        for (uint64_t r = 0; r < rounds; ++r) checked(hipGraphLaunch(exec, e.stream), "graph launch");
+        for (uint64_t i = rounds * kGraphIterations; i < batches; ++i) iteration();
+        checked(hipStreamSynchronize(e.stream), "search");
+    } else {
+        for (uint64_t batch = 0; batch < batches; ++batch) iteration();
+    }
+    checked(hipEventRecord(e.ev_stop, e.stream), "event");
+    checked(hipStreamSynchronize(e.stream), "search");
+    const double search = seconds_since(search_started);
+    float gpu_ms = 0;
+    checked(hipEventElapsedTime(&gpu_ms, e.ev_start, e.ev_stop), "event time");
+    std::vector<ColonyState> st(C);
+    std::vector<ControlState> cs(C);
+    e.state.download(st.data(), C, e.stream);
+    b.control.download(cs.data(), C, e.stream);
+    if (tours) e.global.download(tours, static_cast<size_t>(C) * e.n, e.stream);
+    checked(hipStreamSynchronize(e.stream), "readback");
+    uint64_t steps = 0, evals = 0;
+    for (uint32_t c = 0; c < C; ++c) {
+        if (costs) costs[c] = st[c].global_cost;
+        if (work) { work[2 * c] = st[c].work_construction; work[2 * c + 1] = st[c].work_ls; }
+        if (states) {
+            faco_control_state& o = states[c];
+            o.archive_size = cs[c].archive_size; o.pad = 0;
+            o.stagnant_batches = cs[c].stagnant; o.epoch_batches = cs[c].epoch_batches;
+            o.restarts = cs[c].restarts; o.return_rate = cs[c].return_rate; o.ls_work = cs[c].ls_work;
+        }
+        steps += st[c].work_construction; evals += st[c].work_ls;
+    }
+    fill_result(result, e, C, limit, batches, steps, evals, seconds_since(started), search, gpu_ms);
 }
 
 }  // namespace
@@ -562,6 +795,17 @@ int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys
     });
 }
 
+int faco_engine_evaluate_control(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                                 uint32_t panel, uint64_t limit, const faco_control_spec* spec, uint32_t flags,
+                                 const faco_control_trace* trace, faco_result* result, uint32_t* tours,
+                                 int64_t* costs, uint64_t* work, faco_control_state* states) {
+    return guarded([&] {
+        require(engine && instance_keys && seeds && spec, "null argument");
+        evaluate_control(*engine->impl, instance_keys, seeds, panel, limit, *spec, flags, trace, result, tours, costs,
+                         work, states);
+    });
+}
+
 // ---------------------------------------------------------------------------
 int faco_fixed_create(const double* xy, uint32_t n, uint64_t key, const faco_settings* settings, faco_fixed** out) {
     return guarded([&] {
@@ -610,9 +854,7 @@ int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t 
             e.upload_geometry(&keyv, 1);
             const uint64_t key = colony_key(keyv, seed);
             e.keys.upload(&key, 1, e.stream);
-            std::vector<uint32_t> targets(ants, mne);
-            e.targets.upload(targets.data(), ants, e.stream);
-            e.reset(1);
+            e.reset(1, mne);
             DeviceArray<uint32_t> selected;
             DeviceArray<double> uniforms;
             selected.allocate(static_cast<size_t>(ants) * n);

@@ -29,6 +29,27 @@ struct ColonyState {
     uint32_t iteration_best, source_is_epoch;
     uint32_t batch, pad;
     uint64_t work_construction, work_ls;
+    uint32_t target;       // MNE target of this batch's ants
+    uint32_t parent_zero;  // position of city 0 in the parent | (succ(0) < pred(0)) << 31
+};
+
+// Per-colony control state of the GP / baseline controlled loop (the
+// reference's ControllerState, cpp/include/gp_faco/control_ops.hpp:160-173,
+// plus this batch's decision). Archive tours live in a two-buffer ring
+// ([C][2][4][n] tours and parent-style links); `parity` names the current
+// buffer so the update can write the next archive without a scratch copy.
+constexpr uint32_t kArchive = 4, kRegion = 16, kSamples = 64;
+struct ControlState {
+    float return_rate, ls_work;
+    uint64_t stagnant, epoch_batches, restarts;
+    int64_t tracked_global, tracked_epoch;
+    uint64_t active_id[2], epoch_id[2];
+    int64_t archive_cost[kArchive];
+    uint64_t archive_id[kArchive][2];
+    uint32_t archive_zero[kArchive];   // position of city 0 | forward << 31
+    uint32_t archive_size, parity;
+    uint32_t alternative, action, mask, start_count;
+    uint32_t start_nodes[kRegion];     // region the ants of this batch start in
 };
 
 // ---------------------------------------------------------------------------
@@ -124,6 +145,16 @@ FACO_HD uint64_t colony_key(uint64_t instance_key, uint64_t seed) {
     return mixed(seed ^ mixed(instance_key + 0xd1b54a32d192ed03ULL));
 }
 
+// Undirected edge identity: two SplitMix64 images of (min << 32 | max); a
+// tour's identity is the XOR over its edges (reference
+// cpp/include/gp_faco/control_ops.hpp:43-56).
+FACO_HD void edge_identity(uint32_t a, uint32_t b, uint64_t& first, uint64_t& second) {
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    const uint64_t e = (static_cast<uint64_t>(lo) << 32) | hi;
+    first = mixed(e ^ 0xb5ad4eceda1ce2a9ULL);
+    second = mixed(e ^ 0xd6e8feb86659fd93ULL);
+}
+
 // EUC_2D on packed coordinates (x | y << 16), both in [0, 32767]:
 // s = dx^2 + dy^2 < 2^31, r = isqrt(s), d = r + (s > r^2 + r).
 // The float estimate of sqrt(s) is within 0.01 of the true root, so one
@@ -187,6 +218,9 @@ struct AntParams {
     double* trace_uniform;              // [A][n] or null
     uint64_t* phase_ticks;              // [A][8] or null: setup, construction, LS (wall clock ticks);
                                         // select, relocate, mne/push, LS scan, LS apply (shader clocks)
+    const ControlState* control;        // [C] or null: start cities from the chosen region
+    uint64_t* out_identity;             // [A][2] or null: tour identity of each ant
+    uint32_t* out_zero;                 // [A] or null: position of city 0 | forward << 31
 };
 
 struct ColonyParams {
@@ -209,7 +243,7 @@ struct ColonyParams {
 };
 
 struct ResetParams {
-    uint32_t n, ants, pw, K;
+    uint32_t n, ants, pw, K, target;
     double retention, p_best_root;
     const uint32_t* geometry;
     const uint32_t* initial;            // [G][n]
@@ -219,6 +253,56 @@ struct ResetParams {
     uint32_t* parent; uint32_t* links; uint32_t* epoch; uint32_t* global;
     double* trails; double* products;
 };
+
+// Controlled loop (control_kernels.hip). One workgroup per colony.
+struct ControlParams {
+    uint32_t n, ants, pw, K, nbr_stride;
+    uint32_t panel;                     // colonies per decider: decider = colony / panel
+    uint32_t mode;                      // FACO_CONTROL_*
+    uint32_t variant;                   // factorial variant
+    uint32_t experiment_mask;
+    uint32_t force_collisions;
+    uint64_t evaluation_limit;          // FE per colony (progress feature)
+    uint64_t ls_limit;                  // LS evaluation limit (LS work feature)
+    double retention, p_best_root;
+    faco_baseline_policy baseline;
+    const uint32_t* coords;             // [G][n]
+    const uint16_t* nbr;                // [G][stride] rows, primary = first pw
+    const double* eta;                  // [G][n][pw]
+    const double* local_scale;          // [G][n]
+    const double* epsilon;              // [G]
+    const uint32_t* samples;            // [G][64]
+    const uint32_t* initial;            // [G][n]
+    const int64_t* initial_cost;        // [G]
+    const uint32_t* geometry;           // [C]
+    const uint64_t* keys;               // [C]
+    const faco_program* programs;       // [deciders]
+    const faco_controller* controllers; // [deciders]
+    ColonyState* state;                 // [C]
+    ControlState* control;              // [C]
+    uint32_t* parent;                   // [C][n]
+    uint32_t* links;                    // [C][n]
+    uint32_t* epoch;                    // [C][n]
+    double* trails;                     // [C][n][pw]
+    double* products;                   // [C][n][pw]
+    uint32_t* archive_tours;            // [C][2][4][n]
+    uint32_t* archive_links;            // [C][2][4][n]
+    const uint32_t* tours;              // [A][n]
+    const faco_ant_info* info;          // [A]
+    const uint64_t* ant_identity;       // [A][2]
+    const uint32_t* ant_zero;           // [A]
+    float* features;                    // [12][C][32] or null
+    float* scores;                      // [C][32] or null
+    uint32_t* masks;                    // [C] or null
+    int32_t* actions;                   // [C] or null
+    uint32_t* alternatives;             // [C] or null
+    uint32_t* regions;                  // [C][2][4][16] or null
+};
+hipError_t launch_control_init(const ControlParams& p, uint32_t colonies, hipStream_t stream);
+hipError_t launch_control_decide(const ControlParams& p, uint32_t colonies, hipStream_t stream);
+hipError_t launch_control_update(const ControlParams& p, uint32_t colonies, hipStream_t stream);
+hipError_t launch_score_controller(const faco_controller* controller, const float* features, const uint32_t* masks,
+                                   uint32_t colonies, float* scores, int32_t* actions, hipStream_t stream);
 
 // Launch wrappers (defined in faco_kernels.hip).
 hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t stream);

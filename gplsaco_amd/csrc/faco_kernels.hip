@@ -200,11 +200,20 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         }
         wave_sync();
 
-        const uint32_t target = p.targets[ant];
+        const uint32_t target = p.explicit_mode ? p.targets[ant] : p.state[colony].target;
         const uint32_t* visits = p.visits + static_cast<size_t>(ant) * n;
         uint32_t* checklist_log = p.diag_checklist ? p.diag_checklist + static_cast<size_t>(ant) * 5u * n : nullptr;
         int64_t cost = p.per_ant_parent ? p.parent_cost[ant] : p.state[colony].parent_cost;
-        uint32_t current = p.explicit_mode ? visits[0] : start_node(key, batch, local_ant, n);
+        uint32_t current;
+        if (p.explicit_mode) {
+            current = visits[0];
+        } else if (p.control) {
+            // controlled runs start in the region the colony's action chose
+            const ControlState* ctl = p.control + colony;
+            current = ctl->start_nodes[start_node(key, batch, local_ant, ctl->start_count)];
+        } else {
+            current = start_node(key, batch, local_ant, n);
+        }
         current = uniform(current);
         const uint32_t start = current;
         if (p.trace_selected && lane == 0) {
@@ -592,6 +601,29 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         if (p.diag_positions) {
             for (uint32_t i = lane; i < n; i += kWave) p.diag_positions[static_cast<size_t>(ant) * n + i] = pos[i];
         }
+        if (p.out_identity) {
+            // tour identity: XOR of the edge identities, then the canonical
+            // start (position of city 0 and the direction of its smaller
+            // neighbour) used to compare tours in the archive update
+            uint64_t f = 0, g = 0;
+            for (uint32_t i = lane; i < n; i += kWave) {
+                uint64_t a, b;
+                edge_identity(tour[i], tour[i + 1 == n ? 0 : i + 1], a, b);
+                f ^= a; g ^= b;
+            }
+#pragma unroll
+            for (int offset = 32; offset; offset >>= 1) {
+                f ^= __shfl_xor(f, offset);
+                g ^= __shfl_xor(g, offset);
+            }
+            if (lane == 0) {
+                const uint32_t z = pos[0];
+                const uint32_t next = tour[z + 1 == n ? 0 : z + 1], prev = tour[z == 0 ? n - 1 : z - 1];
+                p.out_identity[2 * static_cast<size_t>(ant)] = f;
+                p.out_identity[2 * static_cast<size_t>(ant) + 1] = g;
+                p.out_zero[ant] = z | (next < prev ? 0x80000000u : 0u);
+            }
+        }
         if (lane == 0) {
             faco_ant_info info;
             info.construction_cost = construction_cost;
@@ -685,6 +717,7 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
         parent[i] = v;
         s_links[v] = pred | (succ << 16);
         links[v] = pred | (succ << 16);
+        if (v == 0) s.parent_zero = i | (succ < pred ? 0x80000000u : 0u);
     }
     __syncthreads();
     // Evaporate, clamp below, deposit on parent edges, clamp above, refresh
@@ -733,8 +766,8 @@ __global__ void __launch_bounds__(256) faco_colony_reset(const ResetParams p) {
         z.global_cost = z.epoch_cost = z.parent_cost = p.initial_cost[geom];
         bounds(z, p.pw, p.retention, p.p_best_root);
         z.default_trail = z.maximum;
+        z.target = p.target;
         s = z;
-        p.state[colony] = z;
     }
     __syncthreads();
     const size_t base = static_cast<size_t>(colony) * n;
@@ -743,7 +776,10 @@ __global__ void __launch_bounds__(256) faco_colony_reset(const ResetParams p) {
         p.parent[base + i] = p.epoch[base + i] = p.global[base + i] = v;
         const uint32_t pred = init[i == 0 ? n - 1 : i - 1], succ = init[i + 1 == n ? 0 : i + 1];
         p.links[base + v] = pred | (succ << 16);
+        if (v == 0) s.parent_zero = i | (succ < pred ? 0x80000000u : 0u);
     }
+    __syncthreads();
+    if (tid == 0) p.state[colony] = s;
     const uint32_t cells = n * p.pw;
     const double* eta = p.eta + static_cast<size_t>(geom) * cells;
     for (uint32_t i = tid; i < cells; i += blockDim.x) {

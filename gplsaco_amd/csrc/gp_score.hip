@@ -4,6 +4,7 @@
 // the legal-masked maximum and the "smallest legal action within 1e-6 of the
 // maximum" rule run as 32-lane butterflies inside each half (reference
 // cuda/gp_score.cuh:64-102, tie rule cpp/src/program_cpu.cpp:106-113).
+#include "faco_common.h"
 #include "gp_program.h"
 
 #include <cmath>
@@ -43,6 +44,26 @@ void validate_program(const faco_program& p) {
         if (top > 6) fail("stack peak above 6");
     }
     if (top != 1) fail("final stack is not a single value");
+}
+
+// validate_controller (cpp/src/program_cpu.cpp:11-24): every tree is a
+// valid feature-spec-2 program, the conditional trees read no feature of a
+// decision not yet taken, and the trees hold at most 63 nodes together.
+void validate_controller(const faco_controller& c) {
+    if (c.kind > 1) throw std::invalid_argument("unknown controller representation");
+    unsigned total = 0;
+    for (unsigned role = 0; role < (c.kind ? 3u : 1u); ++role) {
+        const faco_program& p = c.trees[role];
+        validate_program(p);
+        total += p.length;
+        if (p.feature_spec_id != 2) throw std::invalid_argument("controllers need evaluation-count features (spec 2)");
+        for (unsigned i = 0; c.kind && i < p.length; ++i) {
+            const unsigned f = p.operand[i];
+            if (p.opcode[i] == 0 && ((role == 0 && (f == 5 || f >= 8)) || (role == 1 && f == 5)))
+                throw std::invalid_argument("a stage tree reads a feature of an undecided stage");
+        }
+    }
+    if (total > 63) throw std::invalid_argument("controller holds more than 63 nodes");
 }
 
 void validate_features(const float* features, const uint32_t* masks, uint32_t colonies) {
@@ -146,6 +167,46 @@ extern "C" int faco_score_gpu(const faco_program* program, const float* features
         check(hipMemcpy(d_feat, features, fbytes, hipMemcpyHostToDevice), "upload");
         check(hipMemcpy(d_mask, masks, colonies * sizeof(uint32_t), hipMemcpyHostToDevice), "upload");
         check(faco::launch_score(d_prog, 0, d_feat, d_mask, colonies, d_scores, d_actions, nullptr), "score launch");
+        check(hipMemcpy(scores, d_scores, colonies * 32 * sizeof(float), hipMemcpyDeviceToHost), "download");
+        check(hipMemcpy(actions, d_actions, colonies * sizeof(int32_t), hipMemcpyDeviceToHost), "download");
+    });
+}
+
+extern "C" int faco_score_controller_cpu(const faco_controller* controller, const float* features,
+                                         const uint32_t* masks, uint32_t colonies, float* scores, int32_t* actions) {
+    return guarded_score([&] {
+        if (!controller || !scores || !actions) throw std::invalid_argument("null argument");
+        faco::validate_controller(*controller);
+        faco::validate_features(features, masks, colonies);
+        for (uint32_t c = 0; c < colonies; ++c)
+            actions[c] = faco::controller_decide_host(*controller, features, colonies, c, masks[c],
+                                                      scores + static_cast<size_t>(c) * 32);
+    });
+}
+
+extern "C" int faco_score_controller_gpu(const faco_controller* controller, const float* features,
+                                         const uint32_t* masks, uint32_t colonies, float* scores, int32_t* actions) {
+    return guarded_score([&] {
+        if (!controller || !scores || !actions) throw std::invalid_argument("null argument");
+        faco::validate_controller(*controller);
+        faco::validate_features(features, masks, colonies);
+        auto check = [](hipError_t e, const char* what) {
+            if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+        };
+        faco_controller* d_ctrl = nullptr; float* d_feat = nullptr; uint32_t* d_mask = nullptr;
+        float* d_scores = nullptr; int32_t* d_actions = nullptr;
+        const size_t fbytes = static_cast<size_t>(colonies) * 12 * 32 * sizeof(float);
+        struct Frees { void* p[5]; ~Frees() { for (void* q : p) if (q) (void)hipFree(q); } } frees{};
+        check(hipMalloc(reinterpret_cast<void**>(&d_ctrl), sizeof(faco_controller)), "hipMalloc"); frees.p[0] = d_ctrl;
+        check(hipMalloc(reinterpret_cast<void**>(&d_feat), fbytes), "hipMalloc"); frees.p[1] = d_feat;
+        check(hipMalloc(reinterpret_cast<void**>(&d_mask), colonies * sizeof(uint32_t)), "hipMalloc"); frees.p[2] = d_mask;
+        check(hipMalloc(reinterpret_cast<void**>(&d_scores), colonies * 32 * sizeof(float)), "hipMalloc"); frees.p[3] = d_scores;
+        check(hipMalloc(reinterpret_cast<void**>(&d_actions), colonies * sizeof(int32_t)), "hipMalloc"); frees.p[4] = d_actions;
+        check(hipMemcpy(d_ctrl, controller, sizeof(faco_controller), hipMemcpyHostToDevice), "upload");
+        check(hipMemcpy(d_feat, features, fbytes, hipMemcpyHostToDevice), "upload");
+        check(hipMemcpy(d_mask, masks, colonies * sizeof(uint32_t), hipMemcpyHostToDevice), "upload");
+        check(faco::launch_score_controller(d_ctrl, d_feat, d_mask, colonies, d_scores, d_actions, nullptr),
+              "controller launch");
         check(hipMemcpy(scores, d_scores, colonies * 32 * sizeof(float), hipMemcpyDeviceToHost), "download");
         check(hipMemcpy(actions, d_actions, colonies * sizeof(int32_t), hipMemcpyDeviceToHost), "download");
     });

@@ -146,6 +146,16 @@ Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& c,
     p.ls.assign(static_cast<size_t>(n) * lw, 0);
     p.local_scale.assign(n, 0);
     p.eta.assign(static_cast<size_t>(n) * pw, 0);
+    // scale floor: 1e-12 times the larger coordinate range, at least 1
+    // (cpp/src/prepared_problem.cpp:68-73)
+    {
+        double min_x = xy[0], max_x = xy[0], min_y = xy[1], max_y = xy[1];
+        for (uint32_t i = 0; i < n; ++i) {
+            min_x = std::min<double>(min_x, xy[2 * i]); max_x = std::max<double>(max_x, xy[2 * i]);
+            min_y = std::min<double>(min_y, xy[2 * i + 1]); max_y = std::max<double>(max_y, xy[2 * i + 1]);
+        }
+        p.scale_epsilon = 1e-12 * std::max({max_x - min_x, max_y - min_y, 1.0});
+    }
     auto rows = [&](uint32_t begin, uint32_t end) {
         std::vector<uint64_t> keys(n - 1);
         for (uint32_t a = begin; a < end; ++a) {
@@ -216,6 +226,26 @@ Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& c,
     p.profile[3] = since(t_ls);
     p.preparation_seconds = since(started);
     return p;
+}
+
+// sample_nodes (control_ops.hpp:104-108): min(n, 64) distinct cities from
+// a SplitMix64 stream keyed by the instance; the i-th draw is a rank among
+// the cities not drawn yet.
+std::vector<uint32_t> sample_nodes(uint32_t n, uint64_t instance_key) {
+    const uint32_t count = std::min<uint32_t>(n, kSamples);
+    std::vector<uint32_t> out(kSamples, 0), sorted;
+    uint64_t key = mixed(instance_key ^ 0x6a09e667f3bcc909ULL), counter = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t bound = n - i;
+        const uint64_t threshold = (0ULL - static_cast<uint64_t>(bound)) % bound;
+        uint64_t word;
+        do { word = mixed(key + counter++); } while (word < threshold);
+        uint32_t v = static_cast<uint32_t>(word % bound);
+        for (uint32_t s : sorted) v += s <= v ? 1u : 0u;
+        out[i] = v;
+        sorted.insert(std::upper_bound(sorted.begin(), sorted.end(), v), v);
+    }
+    return out;
 }
 
 }  // namespace faco

@@ -21,6 +21,7 @@ struct Prepared {
     std::vector<uint16_t> backup16;    // [n][bw]
     std::vector<double> eta;           // [n][pw] heuristic 1/d^beta (1 when d == 0)
     std::vector<double> local_scale;   // mean of the 8 nearest distances
+    double scale_epsilon = 1e-12;      // floor of local_scale in the control features
     std::vector<uint32_t> cheap, initial;
     int64_t cheap_cost = 0, initial_cost = 0;
     double cheap_seconds = 0, preparation_seconds = 0;
@@ -33,5 +34,8 @@ Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& se
                          uint32_t nbr_stride);
 bool is_permutation(const uint32_t* tour, uint32_t n);
 int64_t tour_cost(const Prepared& p, const uint32_t* tour);
+// The 64 sampled cities of an instance for the control features ([64],
+// first min(n, 64) valid).
+std::vector<uint32_t> sample_nodes(uint32_t n, uint64_t instance_key);
 
 }  // namespace faco

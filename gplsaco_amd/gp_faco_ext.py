@@ -19,7 +19,8 @@ import time
 import numpy as np
 
 from . import _native
-from ._native import Program, Result, Settings, check, lib, ptr
+from ._native import (BaselinePolicy, ControlSpec, ControlState, ControlTrace, Controller, Program, Result, Settings,
+                      check, lib, ptr)
 
 numeric_backend = "euc2d_int32"
 campaign_interface_version = 1
@@ -151,6 +152,109 @@ score_hip = score_cuda
 
 
 # ---------------------------------------------------------------------------
+
+# ---------------------------------------------------------------------------
+# Controllers and policies (reference bindings.cpp:56-75,147-220)
+
+def read_controller(dictionary) -> Controller:
+    """A program dict (one tree) or {"controller_version": 1, "representation":
+    "joint_single" | "conditional_three", "trees": [...]}."""
+    if not isinstance(dictionary, dict):
+        raise TypeError("controller must be a dict")
+    c = Controller()
+    if "opcode" in dictionary:
+        c.kind = 0
+        c.trees[0] = read_program(dictionary)
+        return c
+    if len(dictionary) != 4 or type(dictionary.get("controller_version")) is not int \
+            or dictionary["controller_version"] != 1:
+        raise ValueError("unknown controller version or fields")
+    kind = dictionary.get("representation")
+    if kind not in ("joint_single", "conditional_three"):
+        raise ValueError("unknown controller representation")
+    c.kind = 1 if kind == "conditional_three" else 0
+    trees = dictionary.get("trees")
+    if not isinstance(trees, (list, tuple)) or len(trees) != (3 if c.kind else 1):
+        raise ValueError("controller tree count is wrong")
+    for i, tree in enumerate(trees):
+        c.trees[i] = read_program(tree)
+    return c
+
+
+_BASELINE_INT_FIELDS = ("mne_level", "max_mne_level", "region", "restart_period", "stagnation_step",
+                        "restart_stagnation", "restart_cooldown")
+
+
+def read_baseline(dictionary) -> BaselinePolicy:
+    if not isinstance(dictionary, dict) or len(dictionary) != 11 or type(dictionary.get("policy_spec_id")) is not int \
+            or dictionary["policy_spec_id"] != 1:
+        raise ValueError("baseline fields or spec version invalid")
+    b = BaselinePolicy()
+    kinds = {"static": 0, "rule": 1}
+    modes = {"none": 0, "periodic": 1, "bernoulli": 2}
+    if dictionary.get("kind") not in kinds:
+        raise ValueError("unknown baseline kind")
+    if dictionary.get("restart_mode") not in modes:
+        raise ValueError("unknown static restart mode")
+    b.kind, b.restart_mode = kinds[dictionary["kind"]], modes[dictionary["restart_mode"]]
+    for name in _BASELINE_INT_FIELDS:
+        setattr(b, name, _unsigned(dictionary.get(name), name, 32))
+    probability = dictionary.get("restart_probability")
+    if type(probability) not in (int, float):
+        raise ValueError("restart probability must be a real number, not bool")
+    b.restart_probability = float(probability)
+    return b
+
+
+def baseline_output(b: BaselinePolicy) -> dict:
+    return {"policy_spec_id": 1, "kind": "static" if b.kind == 0 else "rule", "mne_level": b.mne_level,
+            "max_mne_level": b.max_mne_level, "region": b.region,
+            "restart_mode": ("none", "periodic", "bernoulli")[b.restart_mode], "restart_period": b.restart_period,
+            "restart_probability": b.restart_probability, "stagnation_step": b.stagnation_step,
+            "restart_stagnation": b.restart_stagnation, "restart_cooldown": b.restart_cooldown}
+
+
+FACTORIAL_VARIANTS = ("M00", "M10", "M01", "M11")
+
+
+def read_factorial(dictionary):
+    if not isinstance(dictionary, dict) or len(dictionary) != 3 or set(dictionary) != {
+            "factorial_spec_id", "variant", "baseline_policy"} or type(dictionary["factorial_spec_id"]) is not int \
+            or dictionary["factorial_spec_id"] != 1 or not isinstance(dictionary["baseline_policy"], dict):
+        raise ValueError("factorial fields or spec version invalid")
+    if dictionary["variant"] not in FACTORIAL_VARIANTS:
+        raise ValueError("unknown factorial variant")
+    return FACTORIAL_VARIANTS.index(dictionary["variant"]), read_baseline(dictionary["baseline_policy"])
+
+
+def factorial_output(variant: int, baseline: BaselinePolicy) -> dict:
+    return {"factorial_spec_id": 1, "variant": FACTORIAL_VARIANTS[variant], "baseline_policy": baseline_output(baseline)}
+
+
+def _controller_score(controller, features, masks, entry):
+    c = read_controller(controller)
+    if not isinstance(features, np.ndarray) or features.dtype != np.float32 or features.ndim != 3 \
+            or features.shape[0] != 12 or features.shape[2] != 32 or not features.flags.c_contiguous:
+        raise TypeError("features must be C-contiguous float32 [12, colonies, 32]")
+    if not isinstance(masks, np.ndarray) or masks.dtype != np.uint32 or masks.ndim != 1 \
+            or masks.size != features.shape[1] or not masks.flags.c_contiguous:
+        raise TypeError("masks must be C-contiguous uint32 [colonies]")
+    colonies = masks.size
+    scores = np.zeros((colonies, 32), dtype=np.float32)
+    actions = np.zeros(colonies, dtype=np.int32)
+    check(getattr(lib(), entry)(ctypes.byref(c), features.reshape(-1), masks, colonies, scores.reshape(-1), actions))
+    return {"scores": scores, "actions": actions}
+
+
+def score_controller_cpu(controller, features, masks):
+    """Host scorer of a ControllerProgram (the device kernel's twin)."""
+    return _controller_score(controller, features, masks, "faco_score_controller_cpu")
+
+
+def score_controller_cuda(controller, features, masks):
+    """score_controller_cuda (reference bindings.cpp:1010-1030) on the HIP device."""
+    return _controller_score(controller, features, masks, "faco_score_controller_gpu")
+
 
 def prepare_common_initial(coordinates, settings: FixedFacoSettings) -> dict:
     xy = _coordinates(coordinates)
@@ -338,6 +442,229 @@ class FacoBatchEngine:
             out["ant_subphase_megaclocks"] = {"select": km[5], "relocate": km[6], "mne_push": km[7],
                                               "ls_scan": km[8], "ls_apply": km[9]}
         return out
+
+
+    # ------------------------------------------------------------------
+    # Controlled evaluations (reference bindings.cpp:721-879)
+
+    def evaluate_control_raw(self, instance_keys, seeds, evaluation_limit_per_colony, mode, *, programs=(),
+                             controllers=(), baseline=None, variant=3, experiment_mask=0xFFFFFFFF,
+                             trace_batches=(), use_graph=True, force_fingerprint_collisions=False) -> dict:
+        """Controlled run through faco_engine_evaluate_control; arrays out.
+
+        mode: "program", "controller", "baseline" or "factorial"; colony
+        c = d * panel + t runs task t under decider d. trace_batches
+        (ascending, 0-based) selects the batches whose decisions are kept."""
+        keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
+        if keys.size != seeds.size:
+            raise ValueError("controlled entry points need complete task arrays")
+        limit = _unsigned(evaluation_limit_per_colony, "evaluation_limit_per_colony")
+        spec = ControlSpec()
+        spec.mode = {"program": 1, "controller": 2, "baseline": 3, "factorial": 4}[mode]
+        spec.experiment_mask = _unsigned(experiment_mask, "experiment_mask", 32)
+        spec.factorial_variant = variant
+        spec.force_fingerprint_collisions = 1 if force_fingerprint_collisions else 0
+        prog_arr = ctrl_arr = None
+        if mode in ("program", "factorial"):
+            prog_arr = (Program * len(programs))(*programs)
+            spec.programs = ctypes.addressof(prog_arr)
+            spec.deciders = len(programs)
+        if mode == "controller":
+            ctrl_arr = (Controller * len(controllers))(*controllers)
+            spec.controllers = ctypes.addressof(ctrl_arr)
+            spec.deciders = len(controllers)
+        if mode == "baseline":
+            spec.deciders = 1
+        if baseline is not None:
+            spec.baseline = baseline
+        panel = keys.size
+        C = panel * spec.deciders
+        tours = np.zeros((C, self.dimension), dtype=np.uint32)
+        costs = np.zeros(C, dtype=np.int64)
+        work = np.zeros((C, 2), dtype=np.uint64)
+        states = (ControlState * C)()
+        result = Result()
+        batches = np.ascontiguousarray(sorted(trace_batches), dtype=np.uint32)
+        k = batches.size
+        traces = {}
+        trace = None
+        if k:
+            traces = {"features": np.zeros((k, 12, C, 32), dtype=np.float32),
+                      "scores": np.zeros((k, C, 32), dtype=np.float32),
+                      "masks": np.zeros((k, C), dtype=np.uint32), "actions": np.zeros((k, C), dtype=np.int32),
+                      "alternatives": np.zeros((k, C), dtype=np.uint32),
+                      "regions": np.zeros((k, C, 2, 4, 16), dtype=np.uint32),
+                      "costs": np.zeros((k, C, 3), dtype=np.int64)}
+            trace = ControlTrace()
+            trace.batches = batches.ctypes.data_as(ctypes.c_void_p)
+            trace.count = k
+            for name, array in traces.items():
+                setattr(trace, name, array.ctypes.data_as(ctypes.c_void_p))
+        started = time.perf_counter()
+        check(lib().faco_engine_evaluate_control(self._handle, keys, seeds, panel, limit, ctypes.byref(spec),
+                                                 1 if use_graph else 0, ctypes.byref(trace) if trace else None,
+                                                 ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1),
+                                                 ctypes.addressof(states)))
+        del prog_arr, ctrl_arr
+        return {"result": result, "tours": tours, "costs": costs, "work": work, "elapsed": time.perf_counter() - started,
+                "states": [{"archive_size": st.archive_size, "stagnant_batches": st.stagnant_batches,
+                            "return_rate": st.return_rate, "ls_work": st.ls_work, "epoch_batches": st.epoch_batches,
+                            "restarts": st.restarts} for st in states],
+                **{"trace_" + name: array for name, array in traces.items()}}
+
+    @staticmethod
+    def _decision_batches(decision_iterations, limit, ants):
+        points = list(decision_iterations)
+        if points and (any(type(v) is not int for v in points) or points[0] <= 0 or points[-1] > limit // ants
+                       or sorted(set(points)) != points):
+            raise ValueError("decision iterations must be increasing iteration numbers inside the budget")
+        return [v - 1 for v in points]
+
+    def _control_output(self, raw, decision_iterations, kind) -> dict:
+        out = _batch_output(raw["result"], raw["tours"], raw["costs"], "cached", raw["elapsed"])
+        out["control_states"] = raw["states"]
+        if decision_iterations:
+            rows = []
+            C = raw["tours"].shape[0]
+            for k, iteration in enumerate(decision_iterations):
+                for colony in range(C):
+                    row = {"iteration": iteration, "colony": colony,
+                           "legal_mask": int(raw["trace_masks"][k, colony]),
+                           "action": int(raw["trace_actions"][k, colony]),
+                           "features": raw["trace_features"][k, :, colony, :].tolist()}
+                    scores = raw["trace_scores"][k, colony]
+                    if kind == "conditional":
+                        row["stages"] = _controller_stages(scores, row["legal_mask"], row["action"])
+                    elif kind != "fixed":
+                        row["scores"] = [float(v) if np.isfinite(v) else None for v in scores]
+                    rows.append(row)
+            out["decisions"] = rows
+        return out
+
+    def evaluate_program_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, program,
+                                     preparation_mode="cached", experiment_mask=0xFFFFFFFF, *, record_behavior=False,
+                                     checkpoint_iterations=(), profile=False, profile_events_only=False,
+                                     decision_iterations=(), use_graph=True) -> dict:
+        if type(evaluation_limit_per_colony) is not int or type(experiment_mask) is not int:
+            raise ValueError("count entry needs an integer limit and mask")
+        if preparation_mode not in ("cached", "end_to_end"):
+            raise ValueError("count entry takes cached or end_to_end preparation")
+        if record_behavior or checkpoint_iterations or profile:
+            raise ValueError("behaviour rows, checkpoints and profiles are not available in this engine")
+        p = read_program(program)
+        batches = self._decision_batches(decision_iterations, evaluation_limit_per_colony, self.settings.ants)
+        raw = self.evaluate_control_raw(instance_keys, seeds, evaluation_limit_per_colony, "program", programs=[p],
+                                        experiment_mask=experiment_mask, trace_batches=batches, use_graph=use_graph)
+        return self._control_output(raw, list(decision_iterations), "program")
+
+    def evaluate_controller_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, controller,
+                                        experiment_mask=0xFFFFFFFF, *, decision_iterations=(), use_graph=True) -> dict:
+        c = read_controller(controller)
+        batches = self._decision_batches(decision_iterations, evaluation_limit_per_colony, self.settings.ants)
+        raw = self.evaluate_control_raw(instance_keys, seeds, evaluation_limit_per_colony, "controller",
+                                        controllers=[c], experiment_mask=experiment_mask, trace_batches=batches,
+                                        use_graph=use_graph)
+        return self._control_output(raw, list(decision_iterations), "conditional" if c.kind else "program")
+
+    def evaluate_baseline_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, policy,
+                                      preparation_mode="cached", experiment_mask=0xFFFFFFFF, *, record_behavior=False,
+                                      checkpoint_iterations=(), profile=False, profile_events_only=False,
+                                      decision_iterations=(), use_graph=True) -> dict:
+        if preparation_mode not in ("cached", "end_to_end"):
+            raise ValueError("count entry takes cached or end_to_end preparation")
+        if record_behavior or checkpoint_iterations or profile:
+            raise ValueError("behaviour rows, checkpoints and profiles are not available in this engine")
+        b = read_baseline(policy)
+        batches = self._decision_batches(decision_iterations, evaluation_limit_per_colony, self.settings.ants)
+        raw = self.evaluate_control_raw(instance_keys, seeds, evaluation_limit_per_colony, "baseline", baseline=b,
+                                        experiment_mask=_unsigned(experiment_mask, "experiment_mask", 32),
+                                        trace_batches=batches, use_graph=use_graph)
+        out = self._control_output(raw, list(decision_iterations), "fixed")
+        out["baseline_policy"] = baseline_output(b)
+        return out
+
+    def evaluate_factorial_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, program, policy,
+                                       preparation_mode="cached", experiment_mask=0xFFFFFFFF, *,
+                                       record_behavior=False, use_graph=True) -> dict:
+        if preparation_mode not in ("cached", "end_to_end"):
+            raise ValueError("count entry takes cached or end_to_end preparation")
+        if record_behavior:
+            raise ValueError("behaviour rows are not available in this engine")
+        p = read_program(program)
+        variant, b = read_factorial(policy)
+        raw = self.evaluate_control_raw(instance_keys, seeds, evaluation_limit_per_colony, "factorial", programs=[p],
+                                        baseline=b, variant=variant,
+                                        experiment_mask=_unsigned(experiment_mask, "experiment_mask", 32),
+                                        use_graph=use_graph)
+        out = self._control_output(raw, [], "program")
+        out["factorial_policy"] = factorial_output(variant, b)
+        return out
+
+    def evaluate_population_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, programs,
+                                        experiment_mask=0xFFFFFFFF, *, factorial_policy=None,
+                                        profile_events_only=False, use_graph=True) -> list:
+        """One launch for the whole population (reference batch_engine.cu:376-460):
+        the panel is replicated per individual; per-individual results share
+        the batch time equally."""
+        keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
+        if keys.size != seeds.size:
+            raise ValueError("population entry needs a complete shared panel")
+        if not isinstance(programs, (list, tuple)) or not programs:
+            raise ValueError("population needs at least one program")
+        mask = _unsigned(experiment_mask, "experiment_mask", 32)
+        controllers = [read_controller(d) for d in programs]
+        conditional = any(c.kind for c in controllers)
+        variant, b = (read_factorial(factorial_policy) if factorial_policy is not None else (3, None))
+        if factorial_policy is not None and len(programs) > 1 and variant not in (1, 2):
+            raise ValueError("population factorial runs take M10 or M01 only")
+        if conditional:
+            if factorial_policy is not None:
+                raise ValueError("controller populations take no factorial policy")
+            raw = self.evaluate_control_raw(keys, seeds, evaluation_limit_per_colony, "controller",
+                                            controllers=controllers, experiment_mask=mask, use_graph=use_graph)
+        elif factorial_policy is not None:
+            raw = self.evaluate_control_raw(keys, seeds, evaluation_limit_per_colony, "factorial",
+                                            programs=[c.trees[0] for c in controllers], baseline=b, variant=variant,
+                                            experiment_mask=mask, use_graph=use_graph)
+        else:
+            raw = self.evaluate_control_raw(keys, seeds, evaluation_limit_per_colony, "program",
+                                            programs=[c.trees[0] for c in controllers], experiment_mask=mask,
+                                            use_graph=use_graph)
+        count, panel = len(programs), keys.size
+        r = raw["result"]
+        outputs = []
+        for individual in range(count):
+            sl = slice(individual * panel, (individual + 1) * panel)
+            work = raw["work"][sl]
+            item = _batch_output(r, raw["tours"][sl], raw["costs"][sl], "cached", raw["elapsed"] / count)
+            item.update({
+                "population_size": count, "population_actual_seconds": r.actual_seconds,
+                "time_accounting": "equal_share_of_population_batch",
+                "actual_seconds": r.actual_seconds / count, "elapsed_seconds": r.actual_seconds / count,
+                "total_tour_evaluations": r.completed_tour_evaluations_per_colony * panel,
+                "completed_construction_steps": int(work[:, 0].sum()),
+                "completed_ls_evaluations": int(work[:, 1].sum()),
+                "control_states": raw["states"][sl],
+            })
+            for entry in item["items"]:
+                entry["completed_seconds"] = raw["elapsed"] / count
+            if factorial_policy is not None:
+                item["factorial_policy"] = factorial_output(variant, b)
+            outputs.append(item)
+        return outputs
+
+
+def _controller_stages(scores, mask, action) -> list:
+    """Per-stage view of a conditional controller decision (reference bindings.cpp:77-94)."""
+    stages, prefix = [], 0
+    for role in range(3):
+        width, stride, offset = (2, 16, 0) if role == 0 else (4, 4, 2) if role == 1 else (4, 1, 6)
+        legal = [bool((mask >> (prefix + i * stride)) & ((1 << stride) - 1)) for i in range(width)]
+        selected = (action - prefix) // stride
+        stages.append({"scores": [float(v) for v in scores[offset:offset + width]], "legal": legal,
+                       "selected": selected})
+        prefix += selected * stride
+    return stages
 
 
 class FixedFacoGpu:

@@ -94,6 +94,81 @@ typedef struct faco_program {
     float constants[63];
 } faco_program;
 
+/* ControllerProgram (reference cpp/include/gp_faco/program.hpp:27-30),
+ * 1168 bytes: kind 0 = one tree scoring the 32 actions; kind 1 = conditional
+ * trees choosing restart (2-way), region (4-way) and MNE level (4-way). */
+typedef struct faco_controller {
+    uint32_t kind;
+    faco_program trees[3];
+} faco_controller;
+
+/* BaselinePolicy, same 48-byte layout (reference
+ * cpp/include/gp_faco/baseline_policy.hpp:20-27). kind: 0 static, 1 rule;
+ * restart_mode: 0 none, 1 periodic, 2 bernoulli. */
+typedef struct faco_baseline_policy {
+    uint32_t kind;
+    uint32_t mne_level;
+    uint32_t max_mne_level;
+    uint32_t region;
+    uint32_t restart_mode;
+    uint32_t restart_period;
+    double restart_probability;
+    uint32_t stagnation_step;
+    uint32_t restart_stagnation;
+    uint32_t restart_cooldown;
+    uint32_t pad;
+} faco_baseline_policy;
+
+/* Per-colony control summary after a controlled run: the fields of
+ * ControllerState the reference exports as control_states
+ * (cpp/src/bindings.cpp:394-407; control_ops.hpp:160-173). */
+typedef struct faco_control_state {
+    uint32_t archive_size;
+    uint32_t pad;
+    uint64_t stagnant_batches;
+    uint64_t epoch_batches;
+    uint64_t restarts;
+    float return_rate;
+    float ls_work;
+} faco_control_state;
+
+/* Decision source of a controlled evaluation. */
+#define FACO_CONTROL_PROGRAM 1u    /* GP program(s) score the 32 actions */
+#define FACO_CONTROL_CONTROLLER 2u /* ControllerProgram(s) */
+#define FACO_CONTROL_BASELINE 3u   /* fixed baseline policy */
+#define FACO_CONTROL_FACTORIAL 4u  /* baseline-restricted mask, then program(s) */
+
+typedef struct faco_control_spec {
+    uint32_t mode;                       /* FACO_CONTROL_* */
+    uint32_t deciders;                   /* programs or controllers: 1, or the population size */
+    const faco_program* programs;        /* [deciders] for PROGRAM and FACTORIAL */
+    const faco_controller* controllers;  /* [deciders] for CONTROLLER */
+    faco_baseline_policy baseline;       /* for BASELINE and FACTORIAL */
+    uint32_t factorial_variant;          /* 0 M00, 1 M10, 2 M01, 3 M11 */
+    uint32_t experiment_mask;            /* legal actions; bits 0..15 must not all be clear */
+    uint32_t force_fingerprint_collisions; /* diagnostic: every tour identity reads as zero */
+    uint32_t pad;
+} faco_control_spec;
+
+/* Decision trace at chosen batches (the reference's decision_iterations,
+ * bindings.cpp:721-757, with batch b = iteration - 1): for the k-th listed
+ * batch (ascending), features [k][12][C][32], scores [k][C][32], masks,
+ * actions and alternatives [k][C], start regions [k][C][2][4][16] and the
+ * (epoch, global, parent) costs after that batch [k][C][3]. Any output
+ * pointer may be NULL. */
+typedef struct faco_control_trace {
+    const uint32_t* batches;
+    uint32_t count;
+    uint32_t pad;
+    float* features;
+    float* scores;
+    uint32_t* masks;
+    int32_t* actions;
+    uint32_t* alternatives;
+    uint32_t* regions;
+    int64_t* costs;
+} faco_control_trace;
+
 typedef struct faco_engine faco_engine;
 typedef struct faco_fixed faco_fixed;
 
@@ -166,6 +241,35 @@ int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys
                               uint32_t tasks, uint64_t evaluation_limit_per_colony, uint32_t mne_target,
                               uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs,
                               uint64_t* work);
+
+/* Controlled FACO: the GP-controlled restart and perturbation loop.
+ * One entry point for the reference's evaluate_program_evaluations,
+ * evaluate_controller_evaluations, evaluate_population_evaluations,
+ * evaluate_baseline_evaluations and evaluate_factorial_evaluations
+ * (bindings.cpp:721-879; batch_engine.cu:376-460,492-1192). The panel of
+ * `panel` tasks is replicated once per decider: colony c = d * panel + t runs
+ * task t under program / controller d (C = panel * deciders colonies, at
+ * most the engine's colonies * population_size). Per batch: control features
+ * and start regions, decision, restart transaction, construction + 2-opt,
+ * selection and pheromone, archive and feedback update
+ * (cuda/control_state.cuh). Outputs per colony: global tour [C][n], cost
+ * [C], work [C][2] (may be NULL) and control summary [C] (may be NULL).
+ * flags: FACO_RUN_GRAPH as for faco_engine_evaluate_faco (ignored while a
+ * trace is taken). */
+int faco_engine_evaluate_control(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                                 uint32_t panel, uint64_t evaluation_limit_per_colony,
+                                 const faco_control_spec* spec, uint32_t flags, const faco_control_trace* trace,
+                                 faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
+                                 faco_control_state* states);
+
+/* score_controller_cuda (reference cpp/src/bindings.cpp:1010-1030,
+ * cuda/gp_score.cuh:37-62) and its host twin: features FP32 [12][colonies][32],
+ * masks [colonies] -> scores [colonies][32] (kind 1: restart, region and
+ * level tree values in slots 0-1, 2-5, 6-9) and actions [colonies]. */
+int faco_score_controller_cpu(const faco_controller* controller, const float* features, const uint32_t* masks,
+                              uint32_t colonies, float* scores, int32_t* actions);
+int faco_score_controller_gpu(const faco_controller* controller, const float* features, const uint32_t* masks,
+                              uint32_t colonies, float* scores, int32_t* actions);
 
 /* FixedFacoGpu(coordinates, instance_key, settings).run_iterations(seed,
  * batches, mne_target) (reference cpp/include/gp_faco/fixed_faco_gpu.hpp:44-60,

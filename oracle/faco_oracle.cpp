@@ -584,13 +584,19 @@ struct Rec {
     bool has(u32 a, u32 b) const { return succ(a) == b || pred(a) == b; }
 };
 
+// Diagnostic mode of the controlled run: every tour identity reads as zero,
+// so duplicate detection rests on the full comparison alone
+// (force_fingerprint_collisions, cuda/control_state.cuh:19-64). Set before a
+// run, read by its worker threads.
+bool g_force_collisions = false;
+
 Rec make_rec(const u32* order, u32 n, i64 cost) {
     Rec r;
     r.order.assign(order, order + n);
     r.pos.assign(n, 0);
     for (u32 i = 0; i < n; ++i) r.pos[order[i]] = i;
     r.cost = cost;
-    r.id = tour_ident(order, n);
+    r.id = g_force_collisions ? Ident{} : tour_ident(order, n);
     return r;
 }
 
@@ -1343,7 +1349,8 @@ int oracle_action_snapshot(u32 n, const std::int32_t* xy, u32 pw, const u32* pri
 int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, const u64* instance_keys,
                           u32 colonies, const u64* seeds, const Settings* settings, u64 evaluation_limit,
                           u32 mode, u32 deciders, const Program* programs, const Controller* controllers,
-                          const Baseline* baseline, u32 variant, u32 experiment_mask, int threads,
+                          const Baseline* baseline, u32 variant, u32 experiment_mask, int force_collisions,
+                          int threads,
                           u32* global_tours, std::int64_t* global_costs, u64* work, ControlOut* states,
                           float* trace_features, float* trace_scores, u32* trace_masks, std::int32_t* trace_actions,
                           u32* trace_alternatives, u32* trace_regions, std::int64_t* trace_costs,
@@ -1352,6 +1359,8 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
         if (!instances || !colonies || colonies % instances) throw std::invalid_argument("invalid panel");
         if (mode < 1 || mode > 4) throw std::invalid_argument("unknown control mode");
         if (!(experiment_mask & 0xffffu)) throw std::invalid_argument("experiment mask must keep a hold action");
+        g_force_collisions = force_collisions != 0;
+        struct Reset { ~Reset() { g_force_collisions = false; } } reset_flag;
         std::vector<Problem> problems(instances);
         std::vector<std::vector<u32>> samples(instances, std::vector<u32>(kSamples, 0));
         for (u32 g = 0; g < instances; ++g) {

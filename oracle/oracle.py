@@ -222,6 +222,7 @@ def lib():
                                             ctypes.POINTER(Settings), ctypes.c_uint64, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.POINTER(Baseline), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                            ctypes.c_int,
                                             u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
                                             ctypes.POINTER(ctypes.c_double)]
         L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
@@ -625,7 +626,7 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
-                   trace=False):
+                   trace=False, force_collisions=False):
     """Controlled FACO: colony c uses instance c % instances and decider c // instances."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
@@ -663,7 +664,8 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         n, instances, xy_instances.reshape(-1), keys, colonies, seeds, ctypes.byref(settings), evaluation_limit,
         CONTROL_MODES[mode], deciders, ctypes.addressof(prog_arr) if prog_arr is not None else None,
         ctypes.addressof(ctrl_arr) if ctrl_arr is not None else None, ctypes.byref(policy),
-        FACTORIAL_VARIANTS[variant], experiment_mask, threads, tours.reshape(-1), costs, work.reshape(-1),
+        FACTORIAL_VARIANTS[variant], experiment_mask, int(force_collisions), threads, tours.reshape(-1), costs,
+        work.reshape(-1),
         ctypes.addressof(states), ptr(traces.get("features")), ptr(traces.get("scores")),
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
         ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed)), L)

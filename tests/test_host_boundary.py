@@ -168,3 +168,33 @@ def test_score_cpu_matches_reference_core_live():
         out = ext.score_cpu(p, features, masks)
         assert out["scores"].view(np.uint32).tolist() == s.view(np.uint32).tolist()
         assert out["actions"].tolist() == a.tolist()
+
+
+def test_score_controller_host_matches_oracle():
+    """The product's host controller scorer (the device kernel's twin) against
+    the oracle's restatement of cuda/gp_score.cuh:37-62."""
+    rng = np.random.default_rng(12)
+    features = rng.random((12, 7, 32), dtype=np.float32)
+    masks = rng.integers(1, 2**32, size=7, dtype=np.uint32) | 1
+    masks[:3] = [0xFFFF, 1 << 16 | 1, 1 << 5]
+    for _ in range(60):
+        trees = []
+        for role in range(3):
+            while True:
+                p = {**random_program(rng, depth=3), "feature_spec_id": 2}
+                if not any(op == 0 and ((role == 0 and (f == 5 or f >= 8)) or (role == 1 and f == 5))
+                           for op, f in zip(p["opcode"], p["operand"])):
+                    break
+            trees.append(p)
+        d = {"controller_version": 1, "representation": "conditional_three", "controller_id": "c", "trees": trees}
+        out = ext.score_controller_cpu(d, features, masks)
+        c = O.Controller.from_buffer_copy(bytes(ext.read_controller(d)))
+        for col in range(7):
+            scores, action = O.score_controller(c, features[:, col, :], int(masks[col]))
+            assert out["actions"][col] == action
+            assert out["scores"][col].view(np.uint32).tolist() == scores.view(np.uint32).tolist()
+    with pytest.raises(ValueError):  # a restart tree may not read region features
+        bad = {"controller_version": 1, "representation": "conditional_three", "controller_id": "bad",
+               "trees": [program((0,), (9,)), program((0,), (1,)), program((0,), (1,))]}
+        bad["trees"] = [{**t, "feature_spec_id": 2} for t in bad["trees"]]
+        ext.score_controller_cpu(bad, features, masks)
