@@ -62,6 +62,49 @@ def cpu_baseline(xys, keys, seeds, ants, iterations, budget_seconds=20.0):
                       f"{run['search_seconds']:.2f} s of search; host nproc={os.cpu_count()}"}
 
 
+def measured_traffic(n: int, colonies: int, ants: int):
+    """Calibrated HBM bytes per faco_ant_search launch of this workload from
+    the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for the calibration copy
+    kernel of known size), or None when no summary matches the shape."""
+    path = ROOT / "profiles" / "faco_ant_search_tsp1000_traffic.json"
+    if not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    if (data.get("n"), data.get("colonies"), data.get("ants")) != (n, colonies, ants):
+        return None
+    return data["fetch_bytes_per_launch"] * data["fetch_scale"] + data["write_bytes_per_launch"] * data["write_scale"]
+
+
+def gp_population_rate(ext, steps: int) -> dict:
+    """The GP-controlled population shape: 4 uniform TSP500 instances x 128
+    random feature-spec-2 programs (512 colonies x 128 ants), one launch per
+    iteration stage, `steps` iterations replayed through a HIP graph."""
+    from gplsaco_amd.instances import instance_key, uniform_instance
+    from tests.program_gen import random_program
+    n, instances, individuals = 500, 4, 128
+    rng = np.random.default_rng(0)
+    programs = [{**random_program(rng, depth=5), "feature_spec_id": 2} for _ in range(individuals)]
+    settings = ext.FixedFacoSettings()
+    engine = ext.FacoBatchEngine(n, instances, settings, population_size=individuals)
+    keys = [instance_key(n, 2, i) for i in range(instances)]
+    for i, k in enumerate(keys):
+        engine.register_problem(k, uniform_instance(n, seed=2, instance=i))
+    ants = engine.settings.ants
+    task_keys = np.array(keys, dtype=np.uint64)
+    seeds = np.zeros(instances, dtype=np.uint64)
+    parsed = [ext.read_program(p) for p in programs]
+    engine.evaluate_control_raw(task_keys, seeds, ants * 2, "program", programs=parsed)
+    raw = engine.evaluate_control_raw(task_keys, seeds, ants * steps, "program", programs=parsed, use_graph=True)
+    tours = raw["result"].total_tour_evaluations
+    restarts = sum(st["restarts"] for st in raw["states"])
+    return {"workload": f"GP-controlled FACO, {instances} uniform TSP{n} x {individuals} random programs = "
+                        f"{instances * individuals} colonies x {ants} ants, {steps} iterations",
+            "ant_tours_per_s": tours / raw["result"].search_seconds, "tours": tours,
+            "seconds": raw["result"].search_seconds, "restarts": restarts,
+            "mean_final_tour_length": float(raw["costs"].mean())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +114,8 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--seeds", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--population-steps", type=int, default=50,
+                    help="iterations of the GP-controlled population measurement (0 skips it)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the bench, run the calibration copy kernel over this many MiB (counter scaling)")
     args = ap.parse_args()
@@ -157,7 +202,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(n, colonies, ants),
             "bytes_per_tour": bytes_per_tour,
             "tours_per_launch": tours_per_launch,
             "launch_ms": launch_ms,
@@ -168,6 +213,8 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if args.population_steps:
+        line["config"]["gp_population"] = gp_population_rate(ext, args.population_steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(xys, keys, list(range(instances)), ants, args.steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

@@ -265,3 +265,54 @@ def test_score_controller_gpu_matches_host(ext):
         b = ext.score_controller_cuda(d, features, masks)
         assert (a["actions"] == b["actions"]).all()
         assert (a["scores"].view(np.uint32) == b["scores"].view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("factorial", [None, "M10", "M01"])
+def test_population_matches_every_serial_member(ext, factorial):
+    """Reference tests/python/test_population_cuda.py:26-80 on integer
+    coordinates: every individual of one population launch equals its own
+    serial single-program run; chunks of the population and panel changes
+    do not change any individual's trajectory."""
+    rng = np.random.default_rng(1103)
+    n, replicas, iterations = 67, 4, 40
+    s = settings(ext)
+    values = [spec2(random_program(rng, depth=4)) for _ in range(7)] + [None]
+    values[-1] = values[0]
+    serial = ext.FacoBatchEngine(n, replicas, s)
+    expanded = ext.FacoBatchEngine(n, replicas, s, population_size=len(values))
+    for key in (1, 2, 3):
+        xy = np.random.default_rng(107 + key).integers(0, 10001, size=(n, 2)).astype(np.float64)
+        for engine in (serial, expanded):
+            engine.register_problem(key, xy)
+    policy = None
+    if factorial:
+        policy = {"factorial_spec_id": 1, "variant": factorial, "baseline_policy": {
+            "policy_spec_id": 1, "kind": "static", "mne_level": 0, "max_mne_level": 0, "region": 0,
+            "restart_mode": "none", "restart_period": 0, "restart_probability": 0, "stagnation_step": 0,
+            "restart_stagnation": 0, "restart_cooldown": 0}}
+    ants = serial.settings.ants
+    for panel_keys, panel_seeds in (([1, 1, 2, 2], [17, 29, 17, 29]), ([3, 2, 3, 1], [41, 29, 53, 17])):
+        keys = np.asarray(panel_keys, dtype=np.uint64)
+        seeds = np.asarray(panel_seeds, dtype=np.uint64)
+        announce("population-serial", factorial, panel_keys)
+        results = expanded.evaluate_population_evaluations(keys, seeds, iterations * ants, values,
+                                                           factorial_policy=policy)
+        assert len(results) == len(values)
+        for program, actual in zip(values, results):
+            expected = (serial.evaluate_factorial_evaluations(keys, seeds, iterations * ants, program, policy)
+                        if policy else serial.evaluate_program_evaluations(keys, seeds, iterations * ants, program))
+            assert [(r["tour"], r["cost"]) for r in actual["items"]] == [(r["tour"], r["cost"])
+                                                                         for r in expected["items"]]
+            assert actual["control_states"] == expected["control_states"]
+            for field in ("completed_construction_steps", "completed_ls_evaluations",
+                          "completed_tour_evaluations_per_colony", "total_tour_evaluations"):
+                assert actual[field] == expected[field], field
+        assert results[0]["items"] == results[-1]["items"] or \
+            [(r["tour"], r["cost"]) for r in results[0]["items"]] == [(r["tour"], r["cost"]) for r in results[-1]["items"]]
+        for start, stop in ((3, 8), (0, 1), (0, 3)):
+            chunk = expanded.evaluate_population_evaluations(keys, seeds, iterations * ants, values[start:stop],
+                                                             factorial_policy=policy)
+            for expected, actual in zip(results[start:stop], chunk):
+                assert [(r["tour"], r["cost"]) for r in actual["items"]] == [(r["tour"], r["cost"])
+                                                                             for r in expected["items"]]
+                assert actual["control_states"] == expected["control_states"]
