@@ -253,6 +253,12 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         // row already in registers. A miss loads it; results are identical.
         uint32_t pf_node0 = n, pf_node1 = n, pf_link0 = 0, pf_link1 = 0;
         double pf_row0 = 0.0, pf_row1 = 0.0;
+        // Roulette draws come in pools of 64 computed lane-parallel on the
+        // vector unit: lane l holds the uniform of step u_base + l (one
+        // Philox block per step, as step_uniform), so the serial scalar
+        // Philox of every step becomes one vector Philox per 64 steps.
+        double u_pool = 0.0;
+        uint32_t u_base = 0;
         while (mne < target && steps + 1 < n) {
             if constexpr (Profile) mark = clock64();
             const uint32_t step = steps + 1;
@@ -276,7 +282,11 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 pf_row1 = in_row ? products[static_cast<size_t>(pf_node1) * pw + lane] : 0.0;
                 pf_link0 = links[pf_node0];
                 pf_link1 = links[pf_node1];
-                u = step_uniform(key, batch, local_ant, step);
+                if (u_base == 0 || step - u_base >= kWave) {
+                    u_base = step;
+                    u_pool = step_uniform(key, batch, local_ant, step + lane);
+                }
+                u = read_lane(u_pool, step - u_base);
                 const uint32_t cand = in_row ? s_nbr[current * K + lane] : n;
                 const bool avail = in_row && cand < n && !(flags[cand] & kVisited);
                 const uint64_t amask = ballot(avail);
