@@ -251,8 +251,25 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         // for both parent neighbours of the current city: the roulette
         // mostly follows a parent edge, so the next step usually finds its
         // row already in registers. A miss loads it; results are identical.
-        uint32_t pf_node0 = n, pf_node1 = n, pf_link0 = 0, pf_link1 = 0;
+        // pf_links: lane 0 holds the parent links of pf_node0, lane 1 those
+        // of pf_node1 (one vector load, read out only at the next step, so
+        // no wait is placed on the prefetches in this step)
+        uint32_t pf_node0 = n, pf_node1 = n, pf_links = 0;
         double pf_row0 = 0.0, pf_row1 = 0.0;
+        // Inputs of the next step, loaded as soon as its city is known (right
+        // after selection, overlapping the relocation): candidate row,
+        // unvisited flags (only the just selected city changes state before
+        // the next step, and a city is never its own candidate), product row
+        // and parent links.
+        uint32_t cur_cand = n, cur_link_parent = 0;
+        bool cur_free = false;
+        double cur_prod = 0.0;
+        if (!p.explicit_mode) {
+            cur_cand = in_row ? s_nbr[current * K + lane] : n;
+            cur_free = in_row && cur_cand < n && !(flags[cur_cand] & kVisited);
+            cur_prod = in_row ? products[static_cast<size_t>(current) * pw + lane] : 0.0;
+            cur_link_parent = links[current];
+        }
         // Roulette draws come in pools of 64 computed lane-parallel on the
         // vector unit: lane l holds the uniform of step u_base + l (one
         // Philox block per step, as step_uniform), so the serial scalar
@@ -269,26 +286,20 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 selected = uniform(visits[step]);
                 link = links[current];
             } else {
-                double prod;
-                if (current == pf_node0) { prod = pf_row0; link = pf_link0; }
-                else if (current == pf_node1) { prod = pf_row1; link = pf_link1; }
-                else {
-                    prod = in_row ? products[static_cast<size_t>(current) * pw + lane] : 0.0;
-                    link = links[current];
-                }
+                const double prod = cur_prod;
+                link = cur_link_parent;
                 pf_node0 = link & 0xffffu;
                 pf_node1 = link >> 16;
                 pf_row0 = in_row ? products[static_cast<size_t>(pf_node0) * pw + lane] : 0.0;
                 pf_row1 = in_row ? products[static_cast<size_t>(pf_node1) * pw + lane] : 0.0;
-                pf_link0 = links[pf_node0];
-                pf_link1 = links[pf_node1];
+                pf_links = lane < 2 ? links[lane == 0 ? pf_node0 : pf_node1] : 0u;
                 if (u_base == 0 || step - u_base >= kWave) {
                     u_base = step;
                     u_pool = step_uniform(key, batch, local_ant, step + lane);
                 }
                 u = read_lane(u_pool, step - u_base);
-                const uint32_t cand = in_row ? s_nbr[current * K + lane] : n;
-                const bool avail = in_row && cand < n && !(flags[cand] & kVisited);
+                const uint32_t cand = cur_cand;
+                const bool avail = cur_free;
                 const uint64_t amask = ballot(avail);
                 if (amask) {
                     // In-order prefix sums of the available weights: lane j
@@ -341,6 +352,16 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                     }
                 }
                 selected = uniform(selected);
+                if (selected < n) {
+                    cur_cand = in_row ? s_nbr[selected * K + lane] : n;
+                    cur_free = in_row && cur_cand < n && !(flags[cur_cand] & kVisited);
+                    if (selected == pf_node0) { cur_prod = pf_row0; cur_link_parent = read_lane(pf_links, 0); }
+                    else if (selected == pf_node1) { cur_prod = pf_row1; cur_link_parent = read_lane(pf_links, 1); }
+                    else {
+                        cur_prod = in_row ? products[static_cast<size_t>(selected) * pw + lane] : 0.0;
+                        cur_link_parent = links[selected];
+                    }
+                }
             }
             if (p.trace_selected && lane == 0) {
                 p.trace_selected[static_cast<size_t>(ant) * n + step] = selected;
