@@ -187,6 +187,8 @@ SIGNATURES = {
     "faco_engine_settings": (ctypes.c_int, [vp, ctypes.POINTER(Settings)]),
     "faco_engine_launch_shape": (ctypes.c_int, [vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 5),
     "faco_engine_set_waves": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32]),
+    "faco_engine_set_local_search": (ctypes.c_int, [vp, ctypes.c_uint32]),
+    "faco_fixed_set_local_search": (ctypes.c_int, [vp, ctypes.c_uint32]),
     "faco_engine_register_problem": (ctypes.c_int, [vp, ctypes.c_uint64, f64p, ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "faco_engine_preparation_profile": (ctypes.c_int, [vp, ctypes.c_uint64, f64p]),
@@ -203,7 +205,7 @@ SIGNATURES = {
                                       ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp, vp, vp, vp, vp]),
     "faco_diagnostic_construct_ls": (ctypes.c_int, [f64p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                                     u32p, u32p, u32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
-                                                    u32p, u32p, u32p, u32p, vp]),
+                                                    u32p, u32p, u32p, u32p, vp, ctypes.c_uint32]),
     "faco_diagnostic_distances": (ctypes.c_int, [i32p, ctypes.c_uint32, i64p]),
     "faco_diagnostic_start_nodes": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                                    ctypes.c_uint32, ctypes.c_uint32, u32p]),

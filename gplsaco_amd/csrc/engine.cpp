@@ -133,6 +133,7 @@ struct Engine {
     size_t allocated = 0;
     LaunchShape shape;
     uint32_t forced_waves = 0, forced_blocks_per_colony = 0, xcd_remap = 1;
+    uint32_t local_search = 0;  // 0: checklist 2-opt, 1: 2-opt + Or-opt
     int compute_units = 1;
     double p_root = 0;
     // device state
@@ -284,6 +285,7 @@ struct Engine {
         p.blocks_per_colony = bpc;
         p.chunk = config.ants / bpc;
         p.ls_limit = config.ls_evaluation_limit;
+        p.or_opt = local_search;
         p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
         p.products = products.data(); p.parent = parent.data(); p.links = links.data();
         p.state = state.data(); p.keys = keys.data(); p.targets = nullptr;
@@ -724,6 +726,15 @@ int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_p
     return guarded([&] { require(engine, "null argument"); engine->impl->set_waves(waves, blocks_per_colony); });
 }
 
+int faco_engine_set_local_search(faco_engine* engine, uint32_t mode) {
+    return guarded([&] {
+        require(engine, "null argument");
+        require(mode <= 1, "local search mode must be 0 (2-opt) or 1 (2-opt + Or-opt)");
+        engine->impl->local_search = mode;
+        engine->impl->drop_graph();
+    });
+}
+
 int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint32_t* waves, uint32_t* nbr_in_lds,
                              uint32_t* lds_bytes, uint32_t* blocks_per_colony, uint32_t* blocks_per_cu) {
     return guarded([&] {
@@ -822,6 +833,15 @@ int faco_fixed_create(const double* xy, uint32_t n, uint64_t key, const faco_set
 
 void faco_fixed_destroy(faco_fixed* fixed) { delete fixed; }
 
+int faco_fixed_set_local_search(faco_fixed* fixed, uint32_t mode) {
+    return guarded([&] {
+        require(fixed, "null argument");
+        require(mode <= 1, "local search mode must be 0 (2-opt) or 1 (2-opt + Or-opt)");
+        fixed->impl->local_search = mode;
+        fixed->impl->drop_graph();
+    });
+}
+
 int faco_fixed_settings(const faco_fixed* fixed, faco_settings* out) {
     return guarded([&] { require(fixed && out, "null argument"); *out = fixed->impl->config; });
 }
@@ -897,7 +917,7 @@ int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t 
 int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* ls, uint32_t lw, uint32_t ants,
                                  const uint32_t* parents, const uint32_t* visits, const uint32_t* mne_targets,
                                  uint64_t limit, uint32_t waves, uint32_t chunk, uint32_t* construction_tours, uint32_t* tours,
-                                 uint32_t* positions, uint32_t* checklists, faco_ant_info* info) {
+                                 uint32_t* positions, uint32_t* checklists, faco_ant_info* info, uint32_t local_search) {
     return guarded([&] {
         require(ls && parents && visits && mne_targets && tours && info, "null argument");
         require(ants >= 1 && lw >= 1 && lw <= kMaxLs && lw <= n - 1, "invalid ant count or LS width");
@@ -948,6 +968,8 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         p.chunk = chunk; p.blocks_per_colony = 1;
         p.nbr_lds_bytes = round16_host(2u * n * K); p.nbr_stride = stride; p.per_wave_bytes = ant_wave_bytes(n);
         p.xcd_remap = 0; p.explicit_mode = 1; p.per_ant_parent = 1; p.log_checklist = 1;
+        require(local_search <= 1, "local search mode must be 0 or 1");
+        p.or_opt = local_search;
         p.ls_limit = limit;
         p.coords = d_coords.data(); p.nbr = d_nbr.data(); p.parent = d_parents.data(); p.links = d_links.data();
         p.parent_cost = d_costs.data(); p.targets = d_targets.data(); p.visits = d_visits.data();

@@ -196,6 +196,7 @@ struct AntParams {
     uint32_t explicit_mode;             // visits[] drive construction (diagnostic)
     uint32_t per_ant_parent;            // parent/links/cost indexed by ant, not colony
     uint32_t log_checklist;             // append every checklist push to diag_checklist
+    uint32_t or_opt;                    // local search adds Or-opt segment moves
     uint64_t ls_limit;
     const uint32_t* coords;             // [G][n] packed x | y << 16
     const uint16_t* nbr;                // [G][n][K] (d, id)-ordered nearest cities

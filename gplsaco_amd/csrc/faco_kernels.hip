@@ -533,6 +533,35 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         const uint32_t kind = lane >> 5, cslot = lane & 31u;
         const bool scan_lane = cslot < lw;
         const uint32_t tie_index = kind * lw + cslot;
+        // Reverse the tour between cities u and v (2-opt flip on the array,
+        // reference cpu flip_without_cost): as disjoint pair swaps of the
+        // shorter side (the complement side wraps round the array end);
+        // first == 0 keeps the straight side, which is the array layout the
+        // reference's extra lap produces.
+        auto flip_nodes = [&](uint32_t u, uint32_t v) {
+            const uint32_t p0 = pos[u], p1 = pos[v];
+            const uint32_t first = p0 < p1 ? p0 : p1, last = p0 < p1 ? p1 : p0;
+            const uint32_t length = last - first;
+            const bool straight = length <= n - length || first == 0;
+            const uint32_t count = straight ? length : n - length;
+            const uint32_t begin = straight ? first : last;
+            for (uint32_t off = lane; off < count / 2; off += kWave) {
+                uint32_t left = begin + off; if (left >= n) left -= n;
+                uint32_t right = begin + count - 1 - off; if (right >= n) right -= n;
+                const uint16_t l = tour[left], r = tour[right];
+                tour[left] = r; tour[right] = l;
+                pos[r] = static_cast<uint16_t>(left); pos[l] = static_cast<uint16_t>(right);
+            }
+            wave_sync();
+        };
+        // Remove edges (x1, x2), (y1, y2), add (x1, y1), (x2, y2), where x2
+        // follows x1 and y2 follows y1 in one orientation of the tour.
+        auto two_opt_move = [&](uint32_t x1, uint32_t x2, uint32_t y1, uint32_t y2) {
+            const uint32_t px = pos[x1];
+            const uint32_t succ_x1 = uniform(tour[px + 1 == n ? 0 : px + 1]);
+            if (succ_x1 == x2) flip_nodes(x2, y2);
+            else flip_nodes(x1, y1);
+        };
         uint32_t processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0, limit_hit = 0;
         int64_t gain_sum = 0;
         while (qhead != qtail && accepted < n) {
@@ -587,9 +616,96 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             // key: larger gain first, then lower tie index; 0 = no candidate.
             const uint32_t key_value = active ? ((static_cast<uint32_t>(gain + (1 << 20)) << 6) | (63u - tie_index)) : 0u;
             const uint32_t best = uniform(dpp_max_u32(key_value));
+            int32_t best_gain = best ? static_cast<int32_t>(best >> 6) - (1 << 20) : -1;
+
+            // Or-opt (this engine's extension): the segment a, succ a, ...
+            // of length L = 1..3 reinserted next to a predecessor-scan
+            // neighbour b, forward at (b, succ b) or reversed at (pred b, b);
+            // one lane per b, six moves per lane in (L, forward first) order.
+            uint32_t or_best = 0u;
+            uint32_t s2 = 0, s3 = 0, nx3 = 0, or_succ_b = 0;
+            if (p.or_opt && used1 > 0) {
+                s2 = a_next;
+                s3 = tour[pa + 2 >= n ? pa + 2 - n : pa + 2];
+                nx3 = tour[pa + 3 >= n ? pa + 3 - n : pa + 3];
+                const uint32_t xs2 = s_xy[s2], xs3 = s_xy[s3], xnx3 = s_xy[nx3], xp = s_xy[a_prev];
+                const int32_t dpa = static_cast<int32_t>(euc2d(xp, xa));
+                const int32_t removed[3] = {
+                    dpa + static_cast<int32_t>(euc2d(xa, xs2)) - static_cast<int32_t>(euc2d(xp, xs2)),
+                    dpa + static_cast<int32_t>(euc2d(xs2, xs3)) - static_cast<int32_t>(euc2d(xp, xs3)),
+                    dpa + static_cast<int32_t>(euc2d(xs3, xnx3)) - static_cast<int32_t>(euc2d(xp, xnx3))};
+                const uint32_t seg_xy[3] = {xa, xs2, xs3};
+                const bool or_lane = active && kind == 1u;
+                uint32_t valid_count = 0, lane_key = 0;
+                if (or_lane) {
+                    or_succ_b = tour[pb + 1 == n ? 0 : pb + 1];
+                    const uint32_t pbn = nb;  // predecessor of b (kind 1)
+                    const uint32_t xsb = s_xy[or_succ_b], xpb = s_xy[pbn];
+                    const int32_t d_bsb = static_cast<int32_t>(euc2d(xb, xsb));
+                    const int32_t d_pbb = static_cast<int32_t>(euc2d(xpb, xb));
+                    int32_t lane_best = 0;
+                    uint32_t lane_k = 0;
+                    bool any = false;
+                    for (uint32_t L = 1; L <= 3; ++L) {
+                        if (L + 4 > n) break;
+                        const uint32_t nx = L == 1 ? s2 : (L == 2 ? s3 : nx3);
+                        const uint32_t xsl = seg_xy[L - 1];
+                        auto outside = [&](uint32_t v) {
+                            return v != a_prev && v != nx && v != a && !(L >= 2 && v == s2) && !(L >= 3 && v == s3);
+                        };
+                        // forward: (b, a) + (last, succ b) replace (b, succ b)
+                        if (outside(b) && outside(or_succ_b)) {
+                            ++valid_count;
+                            const int32_t g = removed[L - 1] + d_bsb - dab - static_cast<int32_t>(euc2d(xsl, xsb));
+                            if (!any || g > lane_best) { lane_best = g; lane_k = (L - 1) * 2; any = true; }
+                        }
+                        // reversed: (pred b, last) + (a, b) replace (pred b, b)
+                        if (outside(pbn) && outside(b)) {
+                            ++valid_count;
+                            const int32_t g = removed[L - 1] + d_pbb - static_cast<int32_t>(euc2d(xpb, xsl)) - dab;
+                            if (!any || g > lane_best) { lane_best = g; lane_k = (L - 1) * 2 + 1; any = true; }
+                        }
+                    }
+                    if (any) lane_key = (static_cast<uint32_t>(lane_best + (1 << 20)) << 8) | (255u - (cslot * 8u + lane_k));
+                }
+                // every Or-opt move counts one evaluation after the 2-opt ones;
+                // a node cut short commits nothing
+                uint32_t total = valid_count;
+#pragma unroll
+                for (int offset = 32; offset; offset >>= 1) total += __shfl_xor(total, offset);
+                total = uniform(total);
+                const uint64_t remaining = limit - evaluations;
+                if (total > remaining) { evaluations = static_cast<uint32_t>(limit); limit_hit = 1; break; }
+                evaluations += total;
+                or_best = uniform(dpp_max_u32(lane_key));
+            }
             if constexpr (Profile) { const uint64_t t = clock64(); sub[3] += t - mark; mark = t; }
+            const int32_t or_gain = or_best ? static_cast<int32_t>(or_best >> 8) - (1 << 20) : -1;
+            if (or_best && or_gain > 0 && or_gain > best_gain) {
+                const uint32_t index = 255u - (or_best & 255u);
+                const uint32_t j = index / 8u, k = index % 8u, L = k / 2 + 1;
+                const bool forward = (k & 1u) == 0u;
+                const uint32_t bnode = read_lane(b, 32u + j);
+                const uint32_t pnode = read_lane(nb, 32u + j), snode = read_lane(or_succ_b, 32u + j);
+                const uint32_t sl = L == 1 ? a : (L == 2 ? s2 : s3);
+                const uint32_t nx = L == 1 ? s2 : (L == 2 ? s3 : nx3);
+                const uint32_t c = forward ? bnode : pnode, d = forward ? snode : bnode;
+                // two (reversed) or three (forward) 2-opt reversals
+                two_opt_move(a_prev, a, c, d);
+                two_opt_move(a_prev, c, nx, sl);
+                if (forward) two_opt_move(c, sl, a, d);
+                ++accepted;
+                gain_sum += or_gain;
+                reactivations += push(a_prev);
+                reactivations += push(nx);
+                reactivations += push(a);
+                reactivations += push(sl);
+                reactivations += push(c);
+                reactivations += push(d);
+                if constexpr (Profile) sub[4] += clock64() - mark;
+                continue;
+            }
             if (best == 0u) continue;
-            const int32_t best_gain = static_cast<int32_t>(best >> 6) - (1 << 20);
             if (best_gain <= 0) continue;
             const uint32_t best_index = 63u - (best & 63u);
             const uint32_t best_kind = best_index >= lw ? 1u : 0u;
@@ -599,23 +715,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             uint32_t move[4];
             if (best_kind == 0) { move[0] = a_next; move[1] = nnode; move[2] = a; move[3] = bnode; }
             else { move[0] = a; move[1] = bnode; move[2] = a_prev; move[3] = nnode; }
-            const uint32_t p0 = pos[move[0]], p1 = pos[move[1]];
-            const uint32_t first = p0 < p1 ? p0 : p1, last = p0 < p1 ? p1 : p0;
-            const uint32_t length = last - first;
-            const bool straight = length <= n - length || first == 0;
-            const uint32_t count = straight ? length : n - length;
-            const uint32_t begin = straight ? first : last;
-            // Reversal as disjoint pair swaps (the complement side wraps round
-            // the array end); first == 0 keeps the straight side, which is the
-            // array layout the reference's extra lap produces.
-            for (uint32_t off = lane; off < count / 2; off += kWave) {
-                uint32_t left = begin + off; if (left >= n) left -= n;
-                uint32_t right = begin + count - 1 - off; if (right >= n) right -= n;
-                const uint16_t l = tour[left], r = tour[right];
-                tour[left] = r; tour[right] = l;
-                pos[r] = static_cast<uint16_t>(left); pos[l] = static_cast<uint16_t>(right);
-            }
-            wave_sync();
+            flip_nodes(move[0], move[1]);
             ++accepted;
             gain_sum += best_gain;
             reactivations += push(move[0]);

@@ -367,6 +367,10 @@ class FacoBatchEngine:
         return {"waves_per_block": w, "neighbour_rows_in_lds": bool(lds_rows), "lds_bytes": lds,
                 "blocks_per_colony": bpc, "blocks_per_cu": per_cu}
 
+    def set_local_search(self, mode: str) -> None:
+        """'2opt' (the reference's checklist 2-opt, default) or '2opt+oropt'."""
+        check(lib().faco_engine_set_local_search(self._handle, _local_search_mode(mode)))
+
     def set_waves_per_block(self, waves: int, blocks_per_colony: int = 0) -> None:
         check(lib().faco_engine_set_waves(self._handle, _unsigned(waves, "waves", 32),
                                           _unsigned(blocks_per_colony, "blocks_per_colony", 32)))
@@ -689,6 +693,10 @@ class FixedFacoGpu:
             lib().faco_fixed_destroy(handle)
             self._handle = None
 
+    def set_local_search(self, mode: str) -> None:
+        """'2opt' (the reference's checklist 2-opt, default) or '2opt+oropt'."""
+        check(lib().faco_fixed_set_local_search(self._handle, _local_search_mode(mode)))
+
     def run_iterations(self, seed, batches, mne_target, *, trace=False) -> dict:
         seed = _unsigned(seed, "seed")
         batches = _unsigned(batches, "batches", 32)
@@ -726,8 +734,17 @@ class FixedFacoGpu:
 # ---------------------------------------------------------------------------
 # Diagnostics (parity harness; reference cuda/faco_operations.cu:60-132)
 
+LOCAL_SEARCH_MODES = {"2opt": 0, "2opt+oropt": 1}
+
+
+def _local_search_mode(mode) -> int:
+    if mode not in LOCAL_SEARCH_MODES:
+        raise ValueError("local search must be '2opt' or '2opt+oropt'")
+    return LOCAL_SEARCH_MODES[mode]
+
+
 def diagnostic_construct_ls(coordinates, ls_rows, parents, visits, mne_targets, evaluation_limit, waves_per_block=1,
-                            ants_per_block=0):
+                            ants_per_block=0, local_search="2opt"):
     xy = _coordinates(coordinates)
     n = xy.shape[0]
     ls_rows = np.ascontiguousarray(ls_rows, dtype=np.uint32)
@@ -745,7 +762,8 @@ def diagnostic_construct_ls(coordinates, ls_rows, parents, visits, mne_targets, 
                                              _unsigned(evaluation_limit, "evaluation_limit"), waves_per_block,
                                              ants_per_block,
                                              out["construction_tours"].reshape(-1), out["tours"].reshape(-1),
-                                             out["positions"].reshape(-1), checklists.reshape(-1), ptr(info)))
+                                             out["positions"].reshape(-1), checklists.reshape(-1), ptr(info),
+                                             _local_search_mode(local_search)))
     out["checklists"] = checklists
     out["info"] = info
     return out

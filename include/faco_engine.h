@@ -209,6 +209,16 @@ int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint3
                              uint32_t* lds_bytes, uint32_t* blocks_per_colony, uint32_t* blocks_per_cu);
 int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony);
 
+/* Local search of the ants: 0 = the reference's checklist 2-opt (default),
+ * 1 = checklist 2-opt + Or-opt. Or-opt has no counterpart in the reference:
+ * after a popped city a's 2-opt scan, the segment of 1-3 cities starting at
+ * a is tried at the edges next to each LS neighbour b that passed the
+ * predecessor scan (d(a,b) < d(pred a, a)), forward at (b, succ b) and
+ * reversed at (pred b, b); every such move counts one evaluation, the best
+ * move over both operators wins (2-opt on ties) and an Or-opt move is
+ * applied as two or three 2-opt reversals. */
+int faco_engine_set_local_search(faco_engine* engine, uint32_t mode);
+
 /* register_problem(instance_key, coordinates FP64[n][2])
  * (reference batch_engine.hpp:72, batch_engine.cu:279-295). Host preparation:
  * k-NN rows, NN tour, initial checklist 2-opt. */
@@ -281,6 +291,7 @@ int faco_score_controller_gpu(const faco_controller* controller, const float* fe
 int faco_fixed_create(const double* xy, uint32_t n, uint64_t instance_key, const faco_settings* settings,
                       faco_fixed** out);
 void faco_fixed_destroy(faco_fixed* fixed);
+int faco_fixed_set_local_search(faco_fixed* fixed, uint32_t mode);  /* as faco_engine_set_local_search */
 int faco_fixed_settings(const faco_fixed* fixed, faco_settings* out);
 int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t mne_target,
                    uint32_t* tour, int64_t* cost, int64_t* initial_cost, uint64_t* construction_steps,
@@ -300,7 +311,8 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
                                  const uint32_t* mne_targets, uint64_t evaluation_limit,
                                  uint32_t waves_per_block, uint32_t ants_per_block,
                                  uint32_t* construction_tours, uint32_t* tours,
-                                 uint32_t* positions, uint32_t* checklists, faco_ant_info* info);
+                                 uint32_t* positions, uint32_t* checklists, faco_ant_info* info,
+                                 uint32_t local_search);
 
 /* EUC_2D of `count` coordinate pairs on the device (distance kernel check). */
 int faco_diagnostic_distances(const int32_t* pairs /* [count][4] */, uint32_t count, int64_t* out);

@@ -160,6 +160,7 @@ struct Problem {
     std::vector<u32> primary, backup, ls;  // row-major [n][width]
     std::vector<double> local_scale;
     double scale_epsilon = 1e-12;
+    bool or_opt = false;  // local search adds Or-opt moves (this engine's extension)
     std::vector<u32> cheap, initial;
     i64 cheap_cost = 0, initial_cost = 0;
     i64 d(u32 a, u32 b) const { return euc2d(xy, a, b); }
@@ -239,8 +240,33 @@ struct LsStats {
 // (cpp/src/faco_cpu.cpp:148-217). `checklist` is the whole FIFO history; the
 // unprocessed part starts at `next`. `queued[v]` marks membership of the
 // unprocessed part (the reference's std::find over [next, end)).
+//
+// With `or_opt` (this engine's extension; the reference has no Or-opt), each
+// popped city a also tries Or-opt moves after its 2-opt scan: the segment
+// S = a, succ(a), ... of length L = 1, 2, 3 (L + 4 <= n) is cut out between
+// p = pred(a) and nx = succ(last of S) and reinserted at the edge (c, d) of
+// an LS neighbour b of a, either forward between c = b and d = succ(b)
+// (new edges (b, a), (last, d)) or reversed between c = pred(b) and d = b
+// (new edges (c, last), (a, b)); c and d must lie outside S, p and nx. The
+// neighbours tried are those the predecessor scan of 2-opt passed
+// (d(a, b) < d(pred a, a)), in row order, then L, then forward before
+// reversed; each counts one move evaluation against the same budget, with
+// the same cut-short rule. The best move over 2-opt and Or-opt wins (ties to
+// the earlier in that order, so 2-opt first). An Or-opt move is applied as
+// two (reversed) or three (forward) 2-opt reversals and reactivates p, nx,
+// a, last, c, d.
+namespace {
+// Remove edges (x1, x2), (y1, y2) and add (x1, y1), (x2, y2), where x2
+// follows x1 and y2 follows y1 in one orientation of the tour.
+void two_opt_move(Tour& t, u32 x1, u32 x2, u32 y1, u32 y2) {
+    if (t.succ(x1) == x2 && t.succ(y1) == y2) t.flip(x2, y2);
+    else if (t.succ(x2) == x1 && t.succ(y2) == y1) t.flip(x1, y1);
+    else throw std::runtime_error("Or-opt reversal on edges that are not in the tour");
+}
+}  // namespace
+
 LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
-                std::vector<std::uint8_t>& queued, u64 limit) {
+                std::vector<std::uint8_t>& queued, u64 limit, bool or_opt) {
     LsStats st;
     const u32 N = t.n();
     size_t next = 0;
@@ -254,7 +280,10 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
         const i64 to_next = t.p->d(a, a_next), to_prev = t.p->d(a_prev, a);
         i64 best = -1;
         u32 move[4] = {0, 0, 0, 0};
-        bool interrupted = false;
+        bool interrupted = false, or_best = false;
+        u32 or_move[6] = {0, 0, 0, 0, 0, 0};  // p, a, last, nx, c, d
+        u32 or_forward = 0;
+        u32 passed[2] = {0, 0};
         for (int kind = 0; kind < 2 && !interrupted; ++kind) {
             const i64 current = kind == 0 ? to_next : to_prev;
             for (u32 j = 0; j < lw; ++j) {
@@ -264,6 +293,7 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
                 if (!(current > ab)) break;
                 if (st.evaluations == limit) { interrupted = true; break; }
                 ++st.evaluations;
+                ++passed[kind];
                 const u32 nb = kind == 0 ? t.succ(b) : t.pred(b);
                 const u32 closing = kind == 0 ? a_next : a_prev;
                 const i64 gain = (nb == a || b == closing) ? 0
@@ -275,11 +305,60 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
                 }
             }
         }
+        if (or_opt && !interrupted) {
+            const u32 p = a_prev;
+            for (u32 j = 0; j < passed[1] && !interrupted; ++j) {
+                const u32 b = ls[static_cast<size_t>(a) * lw + j];
+                u32 seg[3] = {a, 0, 0};
+                for (u32 L = 1; L <= 3 && L + 4 <= N && !interrupted; ++L) {
+                    if (L > 1) seg[L - 1] = t.succ(seg[L - 2]);
+                    const u32 last = seg[L - 1], nx = t.succ(last);
+                    auto outside = [&](u32 v) {
+                        if (v == p || v == nx) return false;
+                        for (u32 k = 0; k < L; ++k) if (seg[k] == v) return false;
+                        return true;
+                    };
+                    const i64 removed = t.p->d(p, a) + t.p->d(last, nx) - t.p->d(p, nx);
+                    for (u32 variant = 0; variant < 2 && !interrupted; ++variant) {
+                        const u32 forward = variant == 0 ? 1u : 0u;
+                        const u32 c = forward ? b : t.pred(b), d = forward ? t.succ(b) : b;
+                        if (!outside(c) || !outside(d)) continue;
+                        if (st.evaluations == limit) { interrupted = true; break; }
+                        ++st.evaluations;
+                        const i64 added = forward ? t.p->d(b, a) + t.p->d(last, d) : t.p->d(c, last) + t.p->d(a, b);
+                        const i64 gain = removed + t.p->d(c, d) - added;
+                        if (gain > best) {
+                            best = gain;
+                            or_best = true;
+                            or_move[0] = p; or_move[1] = a; or_move[2] = last; or_move[3] = nx;
+                            or_move[4] = c; or_move[5] = d;
+                            or_forward = forward;
+                        }
+                    }
+                }
+            }
+        }
         if (interrupted) { st.limit_reached = 1; break; }
         if (best > 0) {
-            t.flip(move[0], move[1]);
-            for (u32 v : move) {
-                if (!queued[v]) { queued[v] = 1; checklist.push_back(v); ++st.reactivations; }
+            if (or_best) {
+                const u32 p = or_move[0], s1 = or_move[1], sl = or_move[2], nx = or_move[3];
+                const u32 c = or_move[4], d = or_move[5];
+                if (or_forward) {
+                    two_opt_move(t, p, s1, c, d);     // p c .. nx sl .. s1 d
+                    two_opt_move(t, p, c, nx, sl);    // p nx .. c sl .. s1 d
+                    two_opt_move(t, c, sl, s1, d);    // p nx .. c s1 .. sl d
+                } else {
+                    two_opt_move(t, p, s1, c, d);     // p c .. nx sl .. s1 d
+                    two_opt_move(t, p, c, nx, sl);    // p nx .. c sl .. s1 d
+                }
+                for (u32 v : {p, nx, s1, sl, c, d}) {
+                    if (!queued[v]) { queued[v] = 1; checklist.push_back(v); ++st.reactivations; }
+                }
+            } else {
+                t.flip(move[0], move[1]);
+                for (u32 v : move) {
+                    if (!queued[v]) { queued[v] = 1; checklist.push_back(v); ++st.reactivations; }
+                }
             }
             ++st.accepted;
             gain_sum -= best;
@@ -375,7 +454,7 @@ AntResult run_ant(const Problem& p, const u32* parent, u32 start, u32 target, u6
     }
     r.construction_tour = work.order;
     r.construction_cost = work.cost;
-    r.ls = two_opt(work, p.ls.data(), p.s.ls_width, checklist, queued, limit);
+    r.ls = two_opt(work, p.ls.data(), p.s.ls_width, checklist, queued, limit, p.or_opt);
     r.tour = work.order;
     r.positions = work.pos;
     r.checklist = checklist;
@@ -433,7 +512,7 @@ void prepare(Problem& p) {
     Tour t; t.assign(nn.data(), n, &p);
     std::vector<u32> checklist(nn);
     std::vector<std::uint8_t> queued(n, 1);
-    two_opt(t, p.ls.data(), c.ls_width, checklist, queued, c.initial_ls_evaluation_limit);
+    two_opt(t, p.ls.data(), c.ls_width, checklist, queued, c.initial_ls_evaluation_limit, false);
     p.initial = t.order;
     p.initial_cost = t.recomputed();
     if (p.cheap_cost <= p.initial_cost) { p.initial = p.cheap; p.initial_cost = p.cheap_cost; }
@@ -1108,10 +1187,10 @@ int oracle_prepare(u32 n, const std::int32_t* xy, const Settings* settings, Sett
 int oracle_construct_ls(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, const u32* parent,
                         const u32* visit, u32 mne_target, u64 limit,
                         u32* construction_tour, u32* tour, u32* positions, u32* checklist,
-                        std::int64_t* costs /* construction, final */, u64* counters) {
+                        std::int64_t* costs /* construction, final */, u64* counters, u32 local_search) {
     return guarded([&] {
         check_coordinates(n, xy);
-        Problem p; p.n = n; p.xy = xy; p.s.ls_width = lw;
+        Problem p; p.n = n; p.xy = xy; p.s.ls_width = lw; p.or_opt = local_search != 0;
         p.ls.assign(ls, ls + static_cast<size_t>(n) * lw);
         u32 cursor = 1;
         auto r = run_ant(p, parent, visit[0], mne_target, limit,
@@ -1131,7 +1210,7 @@ int oracle_construct_ls(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, co
 // Checklist 2-opt alone from a tour and an explicit checklist (the initial
 // local search of preparation, cpp/src/prepared_problem.cpp:111-113).
 int oracle_two_opt(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, u32* tour, const u32* checklist_in,
-                   u32 checklist_size, u64 limit, std::int64_t* cost, u64* evaluations) {
+                   u32 checklist_size, u64 limit, std::int64_t* cost, u64* evaluations, u32 local_search) {
     return guarded([&] {
         check_coordinates(n, xy);
         Problem p; p.n = n; p.xy = xy;
@@ -1139,7 +1218,7 @@ int oracle_two_opt(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, u32* to
         std::vector<u32> checklist(checklist_in, checklist_in + checklist_size);
         std::vector<std::uint8_t> queued(n, 0);
         for (u32 v : checklist) queued[v] = 1;
-        const LsStats st = two_opt(t, ls, lw, checklist, queued, limit);
+        const LsStats st = two_opt(t, ls, lw, checklist, queued, limit, local_search != 0);
         std::copy(t.order.begin(), t.order.end(), tour);
         *cost = t.recomputed();
         *evaluations = st.evaluations;
@@ -1171,7 +1250,7 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
                     u32 mne_target, int threads,
                     u32* global_tours, std::int64_t* global_costs, u64* work /* [colonies][2] */,
                     u32* trace_parent, double* trace_trails, std::int64_t* trace_costs /*[batches][3]*/,
-                    double* elapsed_search_seconds) {
+                    double* elapsed_search_seconds, u32 local_search) {
     return guarded([&] {
         if (!instances || !colonies) throw std::invalid_argument("empty panel");
         std::vector<Problem> problems(instances);
@@ -1180,6 +1259,7 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             problems[g].n = n; problems[g].xy = xy_all + static_cast<size_t>(g) * 2 * n;
             problems[g].s = normalized(*settings, n);
             prepare(problems[g]);
+            problems[g].or_opt = local_search != 0;
         }
         const Settings s = problems[0].s;
         std::vector<ColonyRun> runs(colonies);
@@ -1354,7 +1434,7 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           u32* global_tours, std::int64_t* global_costs, u64* work, ControlOut* states,
                           float* trace_features, float* trace_scores, u32* trace_masks, std::int32_t* trace_actions,
                           u32* trace_alternatives, u32* trace_regions, std::int64_t* trace_costs,
-                          double* elapsed_search_seconds) {
+                          double* elapsed_search_seconds, u32 local_search) {
     return guarded([&] {
         if (!instances || !colonies || colonies % instances) throw std::invalid_argument("invalid panel");
         if (mode < 1 || mode > 4) throw std::invalid_argument("unknown control mode");
@@ -1368,6 +1448,7 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
             problems[g].n = n; problems[g].xy = xy_all + static_cast<size_t>(g) * 2 * n;
             problems[g].s = normalized(*settings, n);
             prepare(problems[g]);
+            problems[g].or_opt = local_search != 0;
             sample_nodes(n, instance_keys[g], samples[g].data());
         }
         const Settings s = problems[0].s;

@@ -180,14 +180,15 @@ def lib():
                                      u32p, u32p, u32p, f64p, u32p, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64)]
         L.oracle_construct_ls.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p,
-                                          ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p, u32p, i64p, u64p]
+                                          ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p, u32p, i64p, u64p,
+                                          ctypes.c_uint32]
         L.oracle_select_next.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p,
                                          f64p, u8p, ctypes.c_uint32, ctypes.c_double,
                                          ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int)]
         L.oracle_run_faco.argtypes = [ctypes.c_uint32, ctypes.c_uint32, i32p, u64p, ctypes.c_uint32, u64p,
                                       ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                      ctypes.POINTER(ctypes.c_double)]
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]
         L.oracle_max_threads.restype = ctypes.c_int
         L.oracle_tour_ident.argtypes = [ctypes.c_uint32, u32p, u64p]
         L.oracle_canonical_compare.restype = ctypes.c_int
@@ -224,9 +225,10 @@ def lib():
                                             ctypes.POINTER(Baseline), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                             ctypes.c_int,
                                             u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
-                                            ctypes.POINTER(ctypes.c_double)]
+                                            ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]
         L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
-                                     ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64)]
+                                     ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.c_uint32]
         _lib = L
     return _lib
 
@@ -316,7 +318,7 @@ COUNTER_NAMES = ("mne", "steps", "nonidentity_relocations", "legal_exhausted", "
                  "evaluation_limit_reached", "checklist_size")
 
 
-def construct_ls(xy, ls, parent, visit, mne_target, limit) -> dict:
+def construct_ls(xy, ls, parent, visit, mne_target, limit, or_opt=False) -> dict:
     xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
     n = xy.size // 2
     ls = np.ascontiguousarray(ls, dtype=np.uint32)
@@ -329,14 +331,14 @@ def construct_ls(xy, ls, parent, visit, mne_target, limit) -> dict:
     _check(L.oracle_construct_ls(n, xy, ls.reshape(-1), lw, np.ascontiguousarray(parent, dtype=np.uint32),
                                  np.ascontiguousarray(visit, dtype=np.uint32), mne_target, limit,
                                  out["construction_tour"], out["tour"], out["positions"], checklist, costs,
-                                 counters), L)
+                                 counters, int(bool(or_opt))), L)
     out["counters"] = dict(zip(COUNTER_NAMES, (int(c) for c in counters)))
     out["checklist"] = checklist[: out["counters"]["checklist_size"]].copy()
     out["construction_cost"], out["final_cost"] = int(costs[0]), int(costs[1])
     return out
 
 
-def two_opt(xy, ls, tour, checklist, limit):
+def two_opt(xy, ls, tour, checklist, limit, or_opt=False):
     xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
     n = xy.size // 2
     ls = np.ascontiguousarray(ls, dtype=np.uint32)
@@ -345,7 +347,7 @@ def two_opt(xy, ls, tour, checklist, limit):
     cost, evals = ctypes.c_int64(), ctypes.c_uint64()
     L = lib()
     _check(L.oracle_two_opt(n, xy, ls.reshape(-1), ls.shape[1], tour, checklist, checklist.size, limit,
-                            ctypes.byref(cost), ctypes.byref(evals)), L)
+                            ctypes.byref(cost), ctypes.byref(evals), int(bool(or_opt))), L)
     return tour, cost.value, evals.value
 
 
@@ -364,7 +366,8 @@ def select_next(xy, primary_row, backup_row, products, visited, current, u):
     return chosen.value, stage.value
 
 
-def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False):
+def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False,
+             or_opt=False):
     """Fixed-iteration FACO over colonies (colony i uses instance i % instances)."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
@@ -386,7 +389,7 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
     _check(L.oracle_run_faco(n, instances, xy_instances.reshape(-1), keys, colonies, seeds,
                              ctypes.byref(settings), batches, mne_target, threads, tours.reshape(-1), costs,
                              work.reshape(-1), ptr(trace_parent), ptr(trace_trails), ptr(trace_costs),
-                             ctypes.byref(elapsed)), L)
+                             ctypes.byref(elapsed), int(bool(or_opt))), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value}
     if trace:
         out.update(trace_parent=trace_parent, trace_trails=trace_trails, trace_costs=trace_costs)
@@ -626,7 +629,7 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
-                   trace=False, force_collisions=False):
+                   trace=False, force_collisions=False, or_opt=False):
     """Controlled FACO: colony c uses instance c % instances and decider c // instances."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
@@ -668,7 +671,7 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         work.reshape(-1),
         ctypes.addressof(states), ptr(traces.get("features")), ptr(traces.get("scores")),
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
-        ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed)), L)
+        ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt))), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
                        "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,

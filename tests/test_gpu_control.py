@@ -316,3 +316,22 @@ def test_population_matches_every_serial_member(ext, factorial):
                 assert [(r["tour"], r["cost"]) for r in actual["items"]] == [(r["tour"], r["cost"])
                                                                              for r in expected["items"]]
                 assert actual["control_states"] == expected["control_states"]
+
+
+def test_controlled_or_opt_matches_oracle(ext, oracle_mod):
+    O = oracle_mod
+    n, count, ants, batches = 90, 2, 16, 20
+    xys, keys = panel(n, count, clustered=True)
+    s = settings(ext, ants=ants)
+    e = engine_for(ext, n, xys, keys, s, population=2)
+    e.set_local_search("2opt+oropt")
+    rng = np.random.default_rng(31)
+    parsed = [ext.read_program(RESTARTY), ext.read_program(spec2(random_program(rng, depth=4)))]
+    seeds = np.array([2, 3], dtype=np.uint64)
+    announce("control-oropt", n)
+    raw = e.evaluate_control_raw(np.array(keys, dtype=np.uint64), seeds, ants * batches, "program",
+                                 programs=parsed, trace_batches=range(batches))
+    want = O.run_controlled(np.stack(xys).astype(np.int32), keys, np.tile(seeds, 2), oracle_settings(O, s),
+                            ants * batches, "program", programs=[to_oracle_program(O, p) for p in parsed],
+                            trace=True, or_opt=True)
+    compare(raw, want, 4, batches)

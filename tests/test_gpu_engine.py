@@ -63,7 +63,8 @@ CAPS = [0, 1, 7, 127, U64]
 
 @pytest.mark.parametrize("n", [3, 4, 5, 7, 8, 17, 100, 500, 1000, 1024, 1025, 1500])
 @pytest.mark.parametrize("metric", ["uniform", "grid", "repeated"])
-def test_explicit_construct_ls_matches_oracle(ext, oracle_mod, n, metric):
+@pytest.mark.parametrize("ls_mode", ["2opt", "2opt+oropt"])
+def test_explicit_construct_ls_matches_oracle(ext, oracle_mod, n, metric, ls_mode):
     rng = np.random.default_rng(83741 + n + len(metric))
     if metric == "uniform":
         xy = rng.integers(0, 10001, size=(n, 2))
@@ -81,10 +82,12 @@ def test_explicit_construct_ls_matches_oracle(ext, oracle_mod, n, metric):
         targets = np.array([2, 4, 8, 16] * 2, dtype=np.uint32)
         waves = [1, 2, 4, 8][cap % 4 if cap < 8 else 3]
         chunk = min(8, [waves, 2 * waves, 8][CAPS.index(cap) % 3])
-        announce("construct_ls", n, metric, cap, waves, chunk)
-        got = ext.diagnostic_construct_ls(xy.astype(np.float64), rows, parents, visits, targets, cap, waves, chunk)
+        announce("construct_ls", n, metric, cap, waves, chunk, ls_mode)
+        got = ext.diagnostic_construct_ls(xy.astype(np.float64), rows, parents, visits, targets, cap, waves, chunk,
+                                          local_search=ls_mode)
         for a in range(ants):
-            want = oracle_mod.construct_ls(xy, rows, parents[a], visits[a], int(targets[a]), cap)
+            want = oracle_mod.construct_ls(xy, rows, parents[a], visits[a], int(targets[a]), cap,
+                                           or_opt=ls_mode != "2opt")
             info = got["info"][a]
             c = want["counters"]
             assert np.array_equal(got["construction_tours"][a], want["construction_tour"]), (cap, a)
@@ -176,6 +179,31 @@ def test_batch_engine_matches_oracle(ext, oracle_mod, n, instances, seeds, batch
     np.testing.assert_array_equal(got["colony_work"], want["work"])
     assert got["total_tour_evaluations"] == colonies * s.ants * batches
     assert got["completed_construction_steps"] == int(want["work"][:, 0].sum())
+
+
+@pytest.mark.parametrize("n,instances,seeds,batches", [(60, 2, 2, 12), (300, 2, 1, 6), (1000, 1, 2, 3)])
+def test_batch_engine_or_opt_matches_oracle(ext, oracle_mod, n, instances, seeds, batches):
+    xys, keys = _panel(n, instances, clustered=n == 300)
+    s = _settings(ext, ants=32 if n < 1000 else 64)
+    colonies = instances * seeds
+    task_keys = np.array([keys[c % instances] for c in range(colonies)], dtype=np.uint64)
+    task_seeds = np.array([9 + c // instances for c in range(colonies)], dtype=np.uint64)
+    engine = ext.FacoBatchEngine(n, colonies, s)
+    engine.set_local_search("2opt+oropt")
+    for k, xy in zip(keys, xys):
+        engine.register_problem(k, xy)
+    announce("batch-oropt", n, colonies, batches)
+    got = engine.evaluate_faco_evaluations(task_keys, task_seeds, s.ants * batches, 8, use_graph=False)
+    want = oracle_mod.run_faco(np.stack(xys).astype(np.int32), keys, task_seeds, _oracle_settings(oracle_mod, s),
+                               batches, 8, threads=0, or_opt=True)
+    for c in range(colonies):
+        assert got["items"][c]["tour"] == want["tours"][c].tolist(), c
+    np.testing.assert_array_equal(got["colony_work"], want["work"])
+    plain = ext.FacoBatchEngine(n, colonies, s)
+    for k, xy in zip(keys, xys):
+        plain.register_problem(k, xy)
+    base = plain.evaluate_faco_evaluations(task_keys, task_seeds, s.ants * batches, 8)
+    assert base["completed_ls_evaluations"] < got["completed_ls_evaluations"]
 
 
 def test_graph_replay_and_layouts_are_identical(ext):
