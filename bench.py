@@ -165,6 +165,13 @@ def main() -> int:
     achieved = bytes_per_tour * tours_per_launch / (launch_ms * 1e-3) / 1e9
     mean_cost = float(np.mean([item["cost"] for item in timed["items"]]))
 
+    # The same workload with Or-opt added to the local search.
+    engine.set_local_search("2opt+oropt")
+    oropt = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=True)
+    engine.set_local_search("2opt")
+    oropt_value = oropt["total_tour_evaluations"] / oropt["search_seconds"]
+    oropt_cost = float(np.mean([item["cost"] for item in oropt["items"]]))
+
     # The reference's profile shape: 32 colonies (one seed per instance).
     small = ext.FacoBatchEngine(n, instances, settings)
     for k, xy in zip(keys, xys):
@@ -192,6 +199,8 @@ def main() -> int:
                         f"checklist 2-opt, {args.steps} iterations",
             "n": n, "colonies": colonies, "ants": ants, "mne": 8, "iterations": args.steps,
             "value_at_32_colonies": small_value,
+            "value_with_or_opt": oropt_value, "mean_final_tour_length_with_or_opt": oropt_cost,
+            "ls_evaluations_per_tour_with_or_opt": oropt["completed_ls_evaluations"] / oropt["total_tour_evaluations"],
             "construction_steps_per_tour": steps_per_tour, "ls_evaluations_per_tour": evals_per_tour,
             "mean_final_tour_length": mean_cost, "launch_shape": engine.launch_shape(colonies),
         },

@@ -22,6 +22,7 @@ i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 vp = ctypes.c_void_p
 
 
@@ -218,6 +219,20 @@ SIGNATURES = {
                                                  i32p]),
     "faco_score_controller_gpu": (ctypes.c_int, [ctypes.POINTER(Controller), f32p, u32p, ctypes.c_uint32, f32p,
                                                  i32p]),
+    "faco_prior_param_count": (ctypes.c_uint32, []),
+    "faco_prior_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(vp)]),
+    "faco_prior_destroy": (None, [vp]),
+    "faco_prior_get_params": (ctypes.c_int, [vp, f32p]),
+    "faco_prior_set_params": (ctypes.c_int, [vp, f32p]),
+    "faco_prior_logits": (ctypes.c_int, [vp, f32p, ctypes.c_uint32, f32p]),
+    "faco_prior_gradients": (ctypes.c_int, [vp, f32p, u8p, ctypes.c_uint32, ctypes.c_float, f32p,
+                                            ctypes.POINTER(ctypes.c_float)]),
+    "faco_prior_train": (ctypes.c_int, [vp, f32p, u8p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_float, ctypes.c_float, ctypes.c_uint64, f32p]),
+    "faco_prior_mma_check": (ctypes.c_int, [f32p, f32p, f32p]),
+    "faco_engine_edge_features": (ctypes.c_int, [vp, ctypes.c_uint64, f32p]),
+    "faco_engine_set_edge_prior": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
+    "faco_engine_apply_edge_prior": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp]),
     "faco_engine_evaluate_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64,
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,

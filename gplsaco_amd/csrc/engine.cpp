@@ -8,6 +8,7 @@
 #include "faco_common.h"
 #include "gp_program.h"
 #include "host_prepare.h"
+#include "prior.h"
 
 #include <algorithm>
 #include <chrono>
@@ -251,7 +252,15 @@ struct Engine {
             geo.coords.upload(p.packed.data(), n, stream, static_cast<size_t>(g) * n);
             geo.nbr.upload(p.nbr.data(), nbr_stride, stream, static_cast<size_t>(g) * nbr_stride);
             if (bw) geo.backup.upload(p.backup16.data(), static_cast<size_t>(n) * bw, stream, static_cast<size_t>(g) * n * bw);
-            geo.eta.upload(p.eta.data(), static_cast<size_t>(n) * pw, stream, static_cast<size_t>(g) * n * pw);
+            if (p.prior.empty()) {
+                geo.eta.upload(p.eta.data(), static_cast<size_t>(n) * pw, stream, static_cast<size_t>(g) * n * pw);
+            } else {
+                // learned edge prior: eta' = eta * multiplier, in FP64
+                std::vector<double> scaled(p.eta.size());
+                for (size_t i = 0; i < scaled.size(); ++i) scaled[i] = p.eta[i] * static_cast<double>(p.prior[i]);
+                geo.eta.upload(scaled.data(), scaled.size(), stream, static_cast<size_t>(g) * n * pw);
+                checked(hipStreamSynchronize(stream), "prior upload");
+            }
             geo.initial.upload(p.initial.data(), n, stream, static_cast<size_t>(g) * n);
             geo.initial_cost.upload(&p.initial_cost, 1, stream, g);
             if (ctl) {
@@ -768,6 +777,72 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double
         const Prepared& p = e.registry.at(key);
         if (cheap_seconds) *cheap_seconds = p.cheap_seconds;
         if (preparation_seconds) *preparation_seconds = p.preparation_seconds;
+    });
+}
+
+int faco_engine_edge_features(faco_engine* engine, uint64_t key, float* features) {
+    return guarded([&] {
+        require(engine && features, "null argument");
+        Engine& e = *engine->impl;
+        const Prepared& p = e.problem(key);
+        const uint32_t n = e.n, pw = e.config.primary_width;
+        double mean = 0;
+        for (uint32_t a = 0; a < n; ++a) mean += p.local_scale[a];
+        const float mean_scale = static_cast<float>(std::max(mean / n, p.scale_epsilon));
+        DeviceArray<uint32_t> coords, primary;
+        DeviceArray<double> scale;
+        DeviceArray<float> out;
+        coords.allocate(n); primary.allocate(static_cast<size_t>(n) * pw); scale.allocate(n);
+        out.allocate(static_cast<size_t>(n) * pw * 8);
+        coords.upload(p.packed.data(), n, e.stream);
+        primary.upload(p.primary.data(), static_cast<size_t>(n) * pw, e.stream);
+        scale.upload(p.local_scale.data(), n, e.stream);
+        checked(launch_prior_features(n, pw, coords.data(), primary.data(), scale.data(), p.scale_epsilon, mean_scale,
+                                      out.data(), e.stream), "feature launch");
+        out.download(features, out.size(), e.stream);
+        checked(hipStreamSynchronize(e.stream), "edge features");
+    });
+}
+
+int faco_engine_set_edge_prior(faco_engine* engine, uint64_t key, const float* multipliers) {
+    return guarded([&] {
+        require(engine, "null argument");
+        Engine& e = *engine->impl;
+        std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+        require(lock.owns_lock(), "the prior of an engine can not change during an evaluation");
+        auto it = e.registry.find(key);
+        require(it != e.registry.end(), "task instance has not been registered");
+        const size_t cells = static_cast<size_t>(e.n) * e.config.primary_width;
+        if (multipliers) {
+            for (size_t i = 0; i < cells; ++i)
+                require(std::isfinite(multipliers[i]) && multipliers[i] > 0.0f, "prior multipliers must be positive");
+            it->second.prior.assign(multipliers, multipliers + cells);
+        } else {
+            it->second.prior.clear();
+        }
+        e.resident.clear();
+    });
+}
+
+int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_t key, float* multipliers) {
+    return guarded([&] {
+        require(engine && prior, "null argument");
+        Engine& e = *engine->impl;
+        const size_t cells = static_cast<size_t>(e.n) * e.config.primary_width;
+        std::vector<float> features(cells * 8), logits(cells), mult(cells);
+        if (faco_engine_edge_features(engine, key, features.data()) != FACO_OK) throw std::runtime_error(g_error);
+        DeviceArray<float> f, s;
+        f.allocate(cells * 8); s.allocate(cells);
+        f.upload(features.data(), f.size(), e.stream);
+        checked(launch_prior_infer(prior_device_params(prior), f.data(), static_cast<uint32_t>(cells), s.data(),
+                                   e.stream), "prior inference launch");
+        s.download(logits.data(), cells, e.stream);
+        checked(hipStreamSynchronize(e.stream), "prior inference");
+        for (size_t i = 0; i < cells; ++i)
+            mult[i] = static_cast<float>(1.0 / (1.0 + std::exp(-static_cast<double>(logits[i]))));
+        for (size_t i = 0; i < cells; ++i) mult[i] = mult[i] > 0.0f ? mult[i] : 1e-30f;
+        if (faco_engine_set_edge_prior(engine, key, mult.data()) != FACO_OK) throw std::runtime_error(g_error);
+        if (multipliers) std::copy(mult.begin(), mult.end(), multipliers);
     });
 }
 

@@ -305,6 +305,22 @@ hipError_t launch_control_update(const ControlParams& p, uint32_t colonies, hipS
 hipError_t launch_score_controller(const faco_controller* controller, const float* features, const uint32_t* masks,
                                    uint32_t colonies, float* scores, int32_t* actions, hipStream_t stream);
 
+// Learned edge prior (prior_kernels.hip).
+uint32_t prior_param_count();
+uint32_t prior_grad_rows(uint32_t edges);
+hipError_t launch_prior_features(uint32_t n, uint32_t pw, const uint32_t* coords, const uint32_t* primary,
+                                 const double* local_scale, double epsilon, float mean_scale, float* out,
+                                 hipStream_t stream);
+hipError_t launch_prior_infer(const float* params, const float* features, uint32_t edges, float* logits,
+                              hipStream_t stream);
+hipError_t launch_prior_grad(const float* params, const float* features, const uint8_t* labels,
+                             const uint32_t* indices, uint32_t offset, uint32_t edges, float pos_weight,
+                             float inv_total, float* partial, hipStream_t stream);
+hipError_t launch_prior_reduce(const float* partial, uint32_t rows, float* grad, float* params, float* m, float* v,
+                               bool adam, float lr, float beta1, float beta2, float eps, uint32_t step,
+                               float* loss_sum, hipStream_t stream);
+hipError_t launch_prior_mma_check(const float* A, const float* B, float* C, hipStream_t stream);
+
 // Launch wrappers (defined in faco_kernels.hip).
 hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t stream);
 int ant_search_blocks_per_cu(const AntParams& p);

@@ -27,6 +27,7 @@ struct Prepared {
     double cheap_seconds = 0, preparation_seconds = 0;
     double profile[4] = {0, 0, 0, 0};  // candidates, nearest neighbour, initial LS, finalisation
     uint32_t s_lw = 0;                 // LS row width of `ls`
+    std::vector<float> prior;          // [n][pw] heuristic multipliers of a learned edge prior, or empty
 };
 
 faco_settings normalized_settings(faco_settings settings, uint32_t n);

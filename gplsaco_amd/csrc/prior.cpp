@@ -1,0 +1,218 @@
+// Host side of the learned edge prior: the faco_prior object (FP32 master
+// weights and Adam moments on the device), inference, full-batch gradients
+// and the minibatch training loop. Kernels: prior_kernels.hip.
+#include "faco_common.h"
+#include "prior.h"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace faco {
+void set_last_error(const char* message);
+
+namespace {
+
+void require(bool condition, const char* message) {
+    if (!condition) throw std::invalid_argument(message);
+}
+void checked(hipError_t status, const char* what) {
+    if (status != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(status));
+}
+
+template<class F> int guarded(F&& f) {
+    try { f(); return FACO_OK; }
+    catch (const std::invalid_argument& e) { set_last_error(e.what()); return FACO_INVALID_ARGUMENT; }
+    catch (const std::exception& e) { set_last_error(e.what()); return FACO_RUNTIME_ERROR; }
+    catch (...) { set_last_error("unknown native error"); return FACO_RUNTIME_ERROR; }
+}
+
+template<class T> struct Buffer {
+    T* ptr = nullptr;
+    size_t count = 0;
+    Buffer() = default;
+    explicit Buffer(size_t n) { allocate(n); }
+    Buffer(const Buffer&) = delete;
+    Buffer& operator=(const Buffer&) = delete;
+    ~Buffer() { if (ptr) (void)hipFree(ptr); }
+    void allocate(size_t n) {
+        count = n;
+        if (n) checked(hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)), "hipMalloc");
+    }
+};
+
+// SplitMix64 stream for initial weights and epoch shuffles.
+struct Mix {
+    uint64_t state;
+    uint64_t next() { return mixed(state++); }
+    double uniform() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }
+};
+
+}  // namespace
+
+struct PriorImpl {
+    Buffer<float> params, m, v, grad, loss;
+    uint32_t step = 0;
+    hipStream_t stream = nullptr;
+    PriorImpl() : params(prior_param_count()), m(prior_param_count()), v(prior_param_count()),
+                  grad(prior_param_count() + 1), loss(1) {
+        checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    ~PriorImpl() { if (stream) (void)hipStreamDestroy(stream); }
+    void set(const float* host) {
+        const size_t P = prior_param_count();
+        for (size_t i = 0; i < P; ++i) require(std::isfinite(host[i]), "prior parameters must be finite");
+        checked(hipMemcpyAsync(params.ptr, host, P * sizeof(float), hipMemcpyHostToDevice, stream), "upload");
+        checked(hipMemsetAsync(m.ptr, 0, P * sizeof(float), stream), "memset");
+        checked(hipMemsetAsync(v.ptr, 0, P * sizeof(float), stream), "memset");
+        checked(hipStreamSynchronize(stream), "prior set");
+        step = 0;
+    }
+};
+
+const float* prior_device_params(const faco_prior* prior) { return reinterpret_cast<const PriorImpl*>(prior)->params.ptr; }
+
+}  // namespace faco
+
+using faco::PriorImpl;
+
+extern "C" {
+
+int faco_prior_create(uint64_t seed, faco_prior** out) {
+    return faco::guarded([&] {
+        faco::require(out != nullptr, "null argument");
+        auto impl = std::make_unique<PriorImpl>();
+        // Glorot-uniform weights from a SplitMix64 stream, zero biases.
+        const uint32_t P = faco::prior_param_count();
+        std::vector<float> w(P, 0.0f);
+        faco::Mix mix{faco::mixed(seed ^ 0x5851f42d4c957f2dULL)};
+        auto fill = [&](uint32_t offset, uint32_t count, double fan_in, double fan_out) {
+            const double bound = std::sqrt(6.0 / (fan_in + fan_out));
+            for (uint32_t i = 0; i < count; ++i) w[offset + i] = static_cast<float>((2.0 * mix.uniform() - 1.0) * bound);
+        };
+        fill(0, 512, 8, 64);       // W1
+        fill(576, 4096, 64, 64);   // W2
+        fill(4736, 64, 64, 1);     // w3
+        impl->set(w.data());
+        *out = reinterpret_cast<faco_prior*>(impl.release());
+    });
+}
+
+void faco_prior_destroy(faco_prior* prior) { delete reinterpret_cast<PriorImpl*>(prior); }
+
+uint32_t faco_prior_param_count(void) { return faco::prior_param_count(); }
+
+int faco_prior_get_params(const faco_prior* prior, float* out) {
+    return faco::guarded([&] {
+        faco::require(prior && out, "null argument");
+        const auto* p = reinterpret_cast<const PriorImpl*>(prior);
+        faco::checked(hipMemcpy(out, p->params.ptr, faco::prior_param_count() * sizeof(float), hipMemcpyDeviceToHost),
+                      "download");
+    });
+}
+
+int faco_prior_set_params(faco_prior* prior, const float* params) {
+    return faco::guarded([&] {
+        faco::require(prior && params, "null argument");
+        reinterpret_cast<PriorImpl*>(prior)->set(params);
+    });
+}
+
+int faco_prior_logits(faco_prior* prior, const float* features, uint32_t edges, float* logits) {
+    return faco::guarded([&] {
+        faco::require(prior && features && logits && edges, "null argument or no edges");
+        auto* p = reinterpret_cast<PriorImpl*>(prior);
+        faco::Buffer<float> f(static_cast<size_t>(edges) * 8), s(edges);
+        faco::checked(hipMemcpyAsync(f.ptr, features, f.count * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
+        faco::checked(faco::launch_prior_infer(p->params.ptr, f.ptr, edges, s.ptr, p->stream), "prior inference launch");
+        faco::checked(hipMemcpyAsync(logits, s.ptr, edges * sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");
+        faco::checked(hipStreamSynchronize(p->stream), "prior inference");
+    });
+}
+
+int faco_prior_gradients(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges,
+                         float pos_weight, float* grads, float* loss) {
+    return faco::guarded([&] {
+        faco::require(prior && features && labels && grads && edges, "null argument or no edges");
+        faco::require(std::isfinite(pos_weight) && pos_weight > 0, "positive weight must be positive");
+        auto* p = reinterpret_cast<PriorImpl*>(prior);
+        faco::Buffer<float> f(static_cast<size_t>(edges) * 8);
+        faco::Buffer<uint8_t> y(edges);
+        const uint32_t rows = faco::prior_grad_rows(edges);
+        faco::Buffer<float> partial(static_cast<size_t>(rows) * (faco::prior_param_count() + 1));
+        faco::checked(hipMemcpyAsync(f.ptr, features, f.count * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
+        faco::checked(hipMemcpyAsync(y.ptr, labels, edges, hipMemcpyHostToDevice, p->stream), "upload");
+        faco::checked(faco::launch_prior_grad(p->params.ptr, f.ptr, y.ptr, nullptr, 0, edges, pos_weight,
+                                              1.0f / static_cast<float>(edges), partial.ptr, p->stream), "gradient launch");
+        faco::checked(faco::launch_prior_reduce(partial.ptr, rows, p->grad.ptr, p->params.ptr, p->m.ptr, p->v.ptr, false,
+                                                0, 0, 0, 0, 1, nullptr, p->stream), "reduce launch");
+        std::vector<float> g(faco::prior_param_count() + 1);
+        faco::checked(hipMemcpyAsync(g.data(), p->grad.ptr, g.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream),
+                      "download");
+        faco::checked(hipStreamSynchronize(p->stream), "prior gradients");
+        std::memcpy(grads, g.data(), faco::prior_param_count() * sizeof(float));
+        if (loss) *loss = g.back();
+    });
+}
+
+int faco_prior_train(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges, uint32_t batch,
+                     uint32_t epochs, float lr, float pos_weight, uint64_t shuffle_seed, float* epoch_losses) {
+    return faco::guarded([&] {
+        faco::require(prior && features && labels && edges, "null argument or no edges");
+        faco::require(batch >= 1 && epochs >= 1, "batch and epochs must be positive");
+        faco::require(std::isfinite(lr) && lr > 0 && std::isfinite(pos_weight) && pos_weight > 0,
+                      "learning rate and positive weight must be positive");
+        auto* p = reinterpret_cast<PriorImpl*>(prior);
+        const uint32_t P = faco::prior_param_count();
+        faco::Buffer<float> f(static_cast<size_t>(edges) * 8);
+        faco::Buffer<uint8_t> y(edges);
+        faco::Buffer<uint32_t> order(edges);
+        const uint32_t rows = faco::prior_grad_rows(batch < edges ? batch : edges);
+        faco::Buffer<float> partial(static_cast<size_t>(rows) * (P + 1));
+        faco::checked(hipMemcpyAsync(f.ptr, features, f.count * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
+        faco::checked(hipMemcpyAsync(y.ptr, labels, edges, hipMemcpyHostToDevice, p->stream), "upload");
+        std::vector<uint32_t> perm(edges);
+        for (uint32_t e = 0; e < epochs; ++e) {
+            // Fisher-Yates shuffle from a SplitMix64 stream of (seed, epoch)
+            for (uint32_t i = 0; i < edges; ++i) perm[i] = i;
+            faco::Mix mix{faco::mixed(shuffle_seed ^ faco::mixed(e + 0x2545f4914f6cdd1dULL))};
+            for (uint32_t i = edges - 1; i > 0; --i) {
+                const uint32_t j = static_cast<uint32_t>(mix.next() % (static_cast<uint64_t>(i) + 1));
+                std::swap(perm[i], perm[j]);
+            }
+            faco::checked(hipMemcpyAsync(order.ptr, perm.data(), edges * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                         p->stream), "upload");
+            faco::checked(hipMemsetAsync(p->loss.ptr, 0, sizeof(float), p->stream), "memset");
+            uint32_t batches = 0;
+            for (uint32_t offset = 0; offset < edges; offset += batch, ++batches) {
+                const uint32_t count = edges - offset < batch ? edges - offset : batch;
+                faco::checked(faco::launch_prior_grad(p->params.ptr, f.ptr, y.ptr, order.ptr, offset, count, pos_weight,
+                                                      1.0f / static_cast<float>(count), partial.ptr, p->stream),
+                              "gradient launch");
+                faco::checked(faco::launch_prior_reduce(partial.ptr, faco::prior_grad_rows(count), p->grad.ptr,
+                                                        p->params.ptr, p->m.ptr, p->v.ptr, true, lr, 0.9f, 0.999f, 1e-8f,
+                                                        ++p->step, p->loss.ptr, p->stream), "adam launch");
+            }
+            float total = 0;
+            faco::checked(hipMemcpyAsync(&total, p->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");
+            faco::checked(hipStreamSynchronize(p->stream), "training epoch");
+            if (epoch_losses) epoch_losses[e] = total / static_cast<float>(batches);
+        }
+    });
+}
+
+int faco_prior_mma_check(const float* A, const float* B, float* C) {
+    return faco::guarded([&] {
+        faco::require(A && B && C, "null argument");
+        faco::Buffer<float> a(16 * 32), b(32 * 16), c(16 * 16);
+        faco::checked(hipMemcpy(a.ptr, A, a.count * sizeof(float), hipMemcpyHostToDevice), "upload");
+        faco::checked(hipMemcpy(b.ptr, B, b.count * sizeof(float), hipMemcpyHostToDevice), "upload");
+        faco::checked(faco::launch_prior_mma_check(a.ptr, b.ptr, c.ptr, nullptr), "mma launch");
+        faco::checked(hipMemcpy(C, c.ptr, c.count * sizeof(float), hipMemcpyDeviceToHost), "download");
+    });
+}
+
+}  // extern "C"

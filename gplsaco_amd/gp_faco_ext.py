@@ -371,6 +371,32 @@ class FacoBatchEngine:
         """'2opt' (the reference's checklist 2-opt, default) or '2opt+oropt'."""
         check(lib().faco_engine_set_local_search(self._handle, _local_search_mode(mode)))
 
+    # Learned edge prior (this engine's extension; gplsaco_amd.prior).
+    def edge_features(self, instance_key) -> np.ndarray:
+        """Features [n * primary_width, 8] of a registered instance's candidate edges."""
+        out = np.zeros((self.dimension * self.settings.primary_width, 8), dtype=np.float32)
+        check(lib().faco_engine_edge_features(self._handle, _unsigned(instance_key, "instance_key"), out.reshape(-1)))
+        return out
+
+    def set_edge_prior(self, instance_key, multipliers=None) -> None:
+        """Heuristic multipliers [n, primary_width] of an instance (None: plain FACO)."""
+        key = _unsigned(instance_key, "instance_key")
+        if multipliers is None:
+            check(lib().faco_engine_set_edge_prior(self._handle, key, None))
+            return
+        m = np.ascontiguousarray(multipliers, dtype=np.float32).reshape(-1)
+        if m.size != self.dimension * self.settings.primary_width:
+            raise ValueError("multipliers must be [n, primary_width]")
+        check(lib().faco_engine_set_edge_prior(self._handle, key, m.ctypes.data_as(ctypes.c_void_p)))
+
+    def apply_edge_prior(self, prior, instance_key) -> np.ndarray:
+        """Score the instance's candidate edges with `prior` (an EdgePrior) and use
+        sigmoid(logit) as their heuristic multipliers; returns them."""
+        out = np.zeros((self.dimension, self.settings.primary_width), dtype=np.float32)
+        check(lib().faco_engine_apply_edge_prior(self._handle, prior._handle, _unsigned(instance_key, "instance_key"),
+                                                 out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
     def set_waves_per_block(self, waves: int, blocks_per_colony: int = 0) -> None:
         check(lib().faco_engine_set_waves(self._handle, _unsigned(waves, "waves", 32),
                                           _unsigned(blocks_per_colony, "blocks_per_colony", 32)))

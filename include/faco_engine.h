@@ -169,6 +169,7 @@ typedef struct faco_control_trace {
     int64_t* costs;
 } faco_control_trace;
 
+typedef struct faco_prior faco_prior;
 typedef struct faco_engine faco_engine;
 typedef struct faco_fixed faco_fixed;
 
@@ -280,6 +281,43 @@ int faco_score_controller_cpu(const faco_controller* controller, const float* fe
                               uint32_t colonies, float* scores, int32_t* actions);
 int faco_score_controller_gpu(const faco_controller* controller, const float* features, const uint32_t* masks,
                               uint32_t colonies, float* scores, int32_t* actions);
+
+/* ---------------------------------------------------------------------
+ * Learned edge prior. No counterpart in the reference: an MLP 8 -> 64 ->
+ * 64 -> 1 (tanh GELU) scores each primary candidate edge (a, b) from eight
+ * features (distance over the local scales of a and b, over the nearest
+ * distance of a and over the mean scale; candidate rank; mutual-candidate
+ * flag and rank; shared candidates). Its sigmoid multiplies the edge's
+ * heuristic (eta' = eta * sigmoid(logit)), so pheromone, roulette and
+ * local search are unchanged. All products run on bf16 matrix cores with
+ * FP32 accumulation; training minimises binary cross-entropy (positives
+ * weighted by pos_weight) with Adam on FP32 master weights. */
+#define FACO_PRIOR_FEATURES 8
+uint32_t faco_prior_param_count(void);  /* 4801: W1[64][8] b1[64] W2[64][64] b2[64] w3[64] b3 */
+int faco_prior_create(uint64_t seed, faco_prior** out);  /* Glorot-uniform weights, zero biases */
+void faco_prior_destroy(faco_prior* prior);
+int faco_prior_get_params(const faco_prior* prior, float* params);
+int faco_prior_set_params(faco_prior* prior, const float* params);  /* also clears the Adam state */
+/* logits [edges] of features [edges][8] */
+int faco_prior_logits(faco_prior* prior, const float* features, uint32_t edges, float* logits);
+/* full-batch gradient [param_count] and loss of the weighted BCE (mean over edges) */
+int faco_prior_gradients(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges,
+                         float pos_weight, float* grads, float* loss);
+/* `epochs` passes of Adam (lr, betas 0.9/0.999, eps 1e-8) over minibatches of
+ * `batch` edges in a shuffled order; mean batch loss per epoch */
+int faco_prior_train(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges,
+                     uint32_t batch, uint32_t epochs, float lr, float pos_weight, uint64_t shuffle_seed,
+                     float* epoch_losses);
+/* one 16x16x32 bf16 matrix-core product C = A[16][32] B[32][16] (layout check) */
+int faco_prior_mma_check(const float* A, const float* B, float* C);
+/* edge features [n * primary_width][8] of a registered instance */
+int faco_engine_edge_features(faco_engine* engine, uint64_t instance_key, float* features);
+/* heuristic multipliers [n][primary_width] of a registered instance
+ * (NULL restores plain FACO) */
+int faco_engine_set_edge_prior(faco_engine* engine, uint64_t instance_key, const float* multipliers);
+/* features -> logits on the device -> sigmoid; sets and returns the multipliers */
+int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_t instance_key,
+                                 float* multipliers);
 
 /* FixedFacoGpu(coordinates, instance_key, settings).run_iterations(seed,
  * batches, mne_target) (reference cpp/include/gp_faco/fixed_faco_gpu.hpp:44-60,

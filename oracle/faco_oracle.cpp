@@ -161,6 +161,7 @@ struct Problem {
     std::vector<double> local_scale;
     double scale_epsilon = 1e-12;
     bool or_opt = false;  // local search adds Or-opt moves (this engine's extension)
+    std::vector<float> prior;  // learned heuristic multipliers [n][pw] (this engine's extension), empty: none
     std::vector<u32> cheap, initial;
     i64 cheap_cost = 0, initial_cost = 0;
     i64 d(u32 a, u32 b) const { return euc2d(xy, a, b); }
@@ -557,6 +558,7 @@ void colony_reset(const Problem& p, ColonyRun& r) {
     for (u32 a = 0; a < n; ++a) for (u32 j = 0; j < pw; ++j) {
         const size_t i = static_cast<size_t>(a) * pw + j;
         r.eta[i] = heuristic(p.d(a, p.primary[i]), p.s.beta);
+        if (!p.prior.empty()) r.eta[i] *= static_cast<double>(p.prior[i]);
         r.products[i] = r.trails[i] * r.eta[i];
     }
     r.construction_steps = r.ls_evaluations = 0;
@@ -1250,7 +1252,8 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
                     u32 mne_target, int threads,
                     u32* global_tours, std::int64_t* global_costs, u64* work /* [colonies][2] */,
                     u32* trace_parent, double* trace_trails, std::int64_t* trace_costs /*[batches][3]*/,
-                    double* elapsed_search_seconds, u32 local_search) {
+                    double* elapsed_search_seconds, u32 local_search,
+                    const float* multipliers /* [instances][n * pw] or null */) {
     return guarded([&] {
         if (!instances || !colonies) throw std::invalid_argument("empty panel");
         std::vector<Problem> problems(instances);
@@ -1260,6 +1263,10 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             problems[g].s = normalized(*settings, n);
             prepare(problems[g]);
             problems[g].or_opt = local_search != 0;
+            if (multipliers) {
+                const size_t cells = static_cast<size_t>(n) * problems[g].s.primary_width;
+                problems[g].prior.assign(multipliers + g * cells, multipliers + (g + 1) * cells);
+            }
         }
         const Settings s = problems[0].s;
         std::vector<ColonyRun> runs(colonies);
@@ -1558,4 +1565,54 @@ int oracle_max_threads() {
 #endif
 }
 
+}  // extern "C"
+
+// ===========================================================================
+// Learned edge prior: candidate-edge features (this engine's extension; the
+// reference has no learned prior). FP32 arithmetic in the order written here
+// is the definition; the HIP kernel computes the same operations.
+namespace {
+float prior_clip(float v) { return v < 0.0f ? 0.0f : (v > 4.0f ? 4.0f : v); }
+}  // namespace
+
+extern "C" {
+// features [n * pw][8] for edge (a, j) = (a, primary[a][j]):
+// 0 d/scale(a), 1 j/(pw-1), 2 d/d(a, nearest), 3 mutual (a in b's row),
+// 4 rank of a in b's row / pw (1 when absent), 5 d/scale(b), 6 d/mean scale,
+// 7 shared row entries / pw; the distance ratios are clipped to [0, 4].
+int oracle_prior_features(u32 n, const std::int32_t* xy, u32 pw, const u32* primary, const double* local_scale,
+                          double epsilon, float* out) {
+    return guarded([&] {
+        check_coordinates(n, xy);
+        if (pw < 2 || pw > n - 1) throw std::invalid_argument("primary width out of range");
+        double mean = 0;
+        for (u32 a = 0; a < n; ++a) mean += local_scale[a];
+        const float mean_scale = static_cast<float>(std::max(mean / n, epsilon));
+        for (u32 a = 0; a < n; ++a) {
+            const u32* row = primary + static_cast<size_t>(a) * pw;
+            const float sa = static_cast<float>(std::max(local_scale[a], epsilon));
+            const float d0 = static_cast<float>(std::max<i64>(euc2d(xy, a, row[0]), 1));
+            for (u32 j = 0; j < pw; ++j) {
+                const u32 b = row[j];
+                const u32* brow = primary + static_cast<size_t>(b) * pw;
+                const float sb = static_cast<float>(std::max(local_scale[b], epsilon));
+                const float d = static_cast<float>(euc2d(xy, a, b));
+                u32 rank = pw, shared = 0;
+                for (u32 k = 0; k < pw; ++k) {
+                    if (brow[k] == a && rank == pw) rank = k;
+                    for (u32 m = 0; m < pw; ++m) shared += brow[k] == row[m] ? 1u : 0u;
+                }
+                float* f = out + (static_cast<size_t>(a) * pw + j) * 8;
+                f[0] = prior_clip(d / sa);
+                f[1] = static_cast<float>(j) / static_cast<float>(pw - 1);
+                f[2] = prior_clip(d / d0);
+                f[3] = rank < pw ? 1.0f : 0.0f;
+                f[4] = static_cast<float>(rank) / static_cast<float>(pw);
+                f[5] = prior_clip(d / sb);
+                f[6] = prior_clip(d / mean_scale);
+                f[7] = static_cast<float>(shared) / static_cast<float>(pw);
+            }
+        }
+    });
+}
 }  // extern "C"

@@ -188,7 +188,8 @@ def lib():
         L.oracle_run_faco.argtypes = [ctypes.c_uint32, ctypes.c_uint32, i32p, u64p, ctypes.c_uint32, u64p,
                                       ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p]
+        L.oracle_prior_features.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, f32p]
         L.oracle_max_threads.restype = ctypes.c_int
         L.oracle_tour_ident.argtypes = [ctypes.c_uint32, u32p, u64p]
         L.oracle_canonical_compare.restype = ctypes.c_int
@@ -367,8 +368,10 @@ def select_next(xy, primary_row, backup_row, products, visited, current, u):
 
 
 def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False,
-             or_opt=False):
-    """Fixed-iteration FACO over colonies (colony i uses instance i % instances)."""
+             or_opt=False, multipliers=None):
+    """Fixed-iteration FACO over colonies (colony i uses instance i % instances).
+    `multipliers` [instances, n, pw] (float32): learned heuristic multipliers,
+    eta' = eta * multiplier in FP64."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -384,15 +387,33 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
         trace_parent = np.zeros((batches, n), dtype=np.uint32)
         trace_trails = np.zeros((batches, n * norm.primary_width), dtype=np.float64)
         trace_costs = np.zeros((batches, 3), dtype=np.int64)
+    if multipliers is not None:
+        mult = np.ascontiguousarray(multipliers, dtype=np.float32)
+        if mult.shape[0] != instances:
+            raise ValueError("one multiplier table per instance")
     L = lib()
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None  # noqa: E731
     _check(L.oracle_run_faco(n, instances, xy_instances.reshape(-1), keys, colonies, seeds,
                              ctypes.byref(settings), batches, mne_target, threads, tours.reshape(-1), costs,
                              work.reshape(-1), ptr(trace_parent), ptr(trace_trails), ptr(trace_costs),
-                             ctypes.byref(elapsed), int(bool(or_opt))), L)
+                             ctypes.byref(elapsed), int(bool(or_opt)),
+                             ptr(None if multipliers is None else mult)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value}
     if trace:
         out.update(trace_parent=trace_parent, trace_trails=trace_trails, trace_costs=trace_costs)
+    return out
+
+
+def prior_features(xy, primary, local_scale, epsilon):
+    """Learned-prior edge features [n * pw, 8] (float32) of the candidate edges."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32)
+    primary = np.ascontiguousarray(primary, dtype=np.uint32)
+    n, pw = primary.shape
+    scale = np.ascontiguousarray(local_scale, dtype=np.float64)
+    out = np.zeros((n * pw, 8), dtype=np.float32)
+    L = lib()
+    _check(L.oracle_prior_features(n, xy.reshape(-1), pw, primary.reshape(-1), scale, float(epsilon),
+                                   out.reshape(-1)), L)
     return out
 
 

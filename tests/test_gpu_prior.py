@@ -1,0 +1,197 @@
+"""Learned edge prior on the device against its oracle (oracle/prior_oracle.py
+and oracle_prior_features), and FACO with prior multipliers against the
+oracle's FACO with the same multipliers.
+
+Bars: matrix-core layout exact on integer operands; edge features
+bit-exact; logits and gradients within the stated tolerances (the device sums
+bf16 products in FP32 in its own order and uses FP32 tanh/exp, the oracle
+sums in FP64 at the same bf16 rounding points); FACO with multipliers
+bit-exact (tours, costs, work counters).
+"""
+import numpy as np
+import pytest
+
+from gplsaco_amd.instances import clustered_instance, uniform_instance
+from tests.conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+LOGIT_ATOL = 2e-2       # absolute, on logits of magnitude <= ~5
+GRAD_RTOL = 2e-2        # per parameter block, relative Frobenius norm
+LOSS_RTOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def prior_mod():
+    from gplsaco_amd import prior
+    return prior
+
+
+def _eps(xy):
+    span = np.ptp(np.asarray(xy, dtype=np.float64), axis=0)
+    return 1e-12 * max(span[0], span[1], 1.0)
+
+
+def _oracle_settings(oracle_mod, s):
+    return oracle_mod.default_settings(**{k: getattr(s, k) for k in (
+        "ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
+        "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")})
+
+
+def test_mma_layout_exact_on_integers(prior_mod):
+    rng = np.random.default_rng(0)
+    A = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)
+    B = rng.integers(-8, 9, size=(32, 16)).astype(np.float32)
+    announce("prior-mma")
+    np.testing.assert_array_equal(prior_mod.mma_check(A, B), A @ B)
+    # a permutation-like product that fails any transposed or shifted operand
+    A = np.zeros((16, 32), np.float32); A[np.arange(16), (np.arange(16) * 2 + 1) % 32] = 1
+    B = np.arange(32 * 16, dtype=np.float32).reshape(32, 16) % 97
+    np.testing.assert_array_equal(prior_mod.mma_check(A, B), A @ B)
+
+
+def test_mma_random_within_fp32(prior_mod):
+    from oracle import prior_oracle as PO
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((16, 32)).astype(np.float32)
+    B = rng.standard_normal((32, 16)).astype(np.float32)
+    want = PO.bf16(A).astype(np.float64) @ PO.bf16(B).astype(np.float64)
+    got = prior_mod.mma_check(A, B)
+    bound = 32 * 2.0 ** -23 * (np.abs(PO.bf16(A)).astype(np.float64) @ np.abs(PO.bf16(B)).astype(np.float64))
+    assert np.all(np.abs(got - want) <= bound + 1e-30)
+
+
+@pytest.mark.parametrize("kind,n", [("uniform", 200), ("clustered", 1000), ("duplicates", 80)])
+def test_edge_features_bit_exact(ext, oracle_mod, kind, n):
+    xy = clustered_instance(n, seed=4) if kind == "clustered" else uniform_instance(n, seed=4)
+    if kind == "duplicates":
+        xy[3::5] = xy[0]
+    s = ext.FixedFacoSettings()
+    engine = ext.FacoBatchEngine(n, 1, s)
+    engine.register_problem(77, xy)
+    announce("prior-features", kind, n)
+    got = engine.edge_features(77)
+    P = oracle_mod.prepare(xy, _oracle_settings(oracle_mod, s))
+    np.testing.assert_array_equal(engine.problem_data(77)["primary"], P["primary"])
+    want = oracle_mod.prior_features(xy, P["primary"], P["local_scale"], _eps(xy))
+    np.testing.assert_array_equal(got, want)
+
+
+def _dataset(ext, n=200, seed=4):
+    xy = uniform_instance(n, seed=seed)
+    engine = ext.FacoBatchEngine(n, 1, ext.FixedFacoSettings())
+    engine.register_problem(5, xy)
+    f = engine.edge_features(5)
+    rng = np.random.default_rng(seed)
+    y = (rng.random(f.shape[0]) < 0.15).astype(np.uint8)
+    return f, y
+
+
+@pytest.mark.parametrize("edges", [1, 31, 33, 3200, 70001])
+def test_logits_match_oracle(ext, prior_mod, edges):
+    from oracle import prior_oracle as PO
+    f, _ = _dataset(ext, 1000 if edges > 3200 else 200)
+    rng = np.random.default_rng(edges)
+    f = f[rng.integers(0, f.shape[0], size=edges)]
+    prior = prior_mod.EdgePrior(seed=11)
+    announce("prior-logits", edges)
+    got = prior.logits(f)
+    want = PO.forward(prior.params(), f)["s"]
+    err = np.abs(got - want)
+    assert err.max() < LOGIT_ATOL and err.mean() < LOGIT_ATOL / 10, (err.max(), err.mean())
+
+
+@pytest.mark.parametrize("edges,pos_weight", [(64, 1.0), (3200, 4.0), (20000, 2.5)])
+def test_gradients_match_oracle(ext, prior_mod, edges, pos_weight):
+    from oracle import prior_oracle as PO
+    f, y = _dataset(ext, 1000 if edges > 3200 else 200)
+    f, y = f[:edges], y[:edges]
+    prior = prior_mod.EdgePrior(seed=3)
+    announce("prior-gradients", edges)
+    loss, g = prior.gradients(f, y, pos_weight)
+    want_loss, want = PO.loss_and_grad(prior.params(), f, y, pos_weight)
+    assert loss == pytest.approx(want_loss, rel=LOSS_RTOL)
+    for lo, hi in ((PO.O_W1, PO.O_B1), (PO.O_B1, PO.O_W2), (PO.O_W2, PO.O_B2), (PO.O_B2, PO.O_W3),
+                   (PO.O_W3, PO.O_B3), (PO.O_B3, PO.P)):
+        assert np.linalg.norm(g[lo:hi] - want[lo:hi]) <= GRAD_RTOL * np.linalg.norm(want[lo:hi]) + 1e-7, (lo, hi)
+    # the reduction is deterministic
+    loss2, g2 = prior.gradients(f, y, pos_weight)
+    assert loss2 == loss and np.array_equal(g2, g)
+
+
+def test_training_learns_tour_edges_and_is_deterministic(ext, prior_mod):
+    announce("prior-train")
+    f, y, summary = prior_mod.build_dataset([200], per_size=4, iterations=8, seed=1)
+    assert f.shape[0] == y.size and 0 < y.mean() < 0.5
+    a = prior_mod.EdgePrior(seed=5)
+    start = a.params()
+    losses = a.train(f, y, batch=1024, epochs=6, lr=3e-3, pos_weight=3.0, seed=2)
+    assert np.all(np.isfinite(losses)) and losses[-1] < 0.8 * losses[0], losses
+    s = a.logits(f)
+    # positives score above negatives (ranking quality of the trained prior)
+    pos, neg = s[y == 1], s[y == 0]
+    auc = (np.searchsorted(np.sort(neg), pos).sum()) / (pos.size * neg.size)
+    assert auc > 0.8, auc
+    b = prior_mod.EdgePrior(params=start)
+    b.train(f, y, batch=1024, epochs=6, lr=3e-3, pos_weight=3.0, seed=2)
+    assert np.array_equal(a.params(), b.params())
+
+
+def _panel(n, instances):
+    xys = [(clustered_instance if i % 2 else uniform_instance)(n, seed=6, instance=i) for i in range(instances)]
+    return xys, [300 + i for i in range(instances)]
+
+
+@pytest.mark.parametrize("n,instances,seeds,batches", [(100, 2, 2, 10), (500, 2, 1, 6)])
+def test_faco_with_prior_multipliers_matches_oracle(ext, oracle_mod, n, instances, seeds, batches):
+    xys, keys = _panel(n, instances)
+    s = ext.FixedFacoSettings()
+    s.ants = 32
+    colonies = instances * seeds
+    task_keys = np.array([keys[c % instances] for c in range(colonies)], dtype=np.uint64)
+    task_seeds = np.array([3 + c // instances for c in range(colonies)], dtype=np.uint64)
+    engine = ext.FacoBatchEngine(n, colonies, s)
+    rng = np.random.default_rng(n)
+    mults = []
+    for k, xy in zip(keys, xys):
+        engine.register_problem(k, xy)
+        m = rng.uniform(0.02, 1.0, size=(n, s.primary_width)).astype(np.float32)
+        engine.set_edge_prior(k, m)
+        mults.append(m)
+    announce("faco-prior", n, colonies, batches)
+    got = engine.evaluate_faco_evaluations(task_keys, task_seeds, s.ants * batches, 8, use_graph=False)
+    ostate = _oracle_settings(oracle_mod, s)
+    want = oracle_mod.run_faco(np.stack(xys).astype(np.int32), keys, task_seeds, ostate, batches, 8,
+                               multipliers=np.stack(mults))
+    for c in range(colonies):
+        assert got["items"][c]["tour"] == want["tours"][c].tolist(), c
+        assert got["items"][c]["cost"] == float(want["costs"][c]), c
+    np.testing.assert_array_equal(got["colony_work"], want["work"])
+    # removing the prior gives plain FACO back
+    for k in keys:
+        engine.set_edge_prior(k, None)
+    plain = engine.evaluate_faco_evaluations(task_keys, task_seeds, s.ants * batches, 8, use_graph=False)
+    base = oracle_mod.run_faco(np.stack(xys).astype(np.int32), keys, task_seeds, ostate, batches, 8)
+    for c in range(colonies):
+        assert plain["items"][c]["tour"] == base["tours"][c].tolist(), c
+
+
+def test_apply_edge_prior_uses_sigmoid_of_logits(ext, oracle_mod, prior_mod):
+    n = 300
+    xy = uniform_instance(n, seed=8)
+    s = ext.FixedFacoSettings()
+    s.ants = 32
+    engine = ext.FacoBatchEngine(n, 2, s)
+    engine.register_problem(9, xy)
+    prior = prior_mod.EdgePrior(seed=21)
+    announce("apply-prior", n)
+    m = engine.apply_edge_prior(prior, 9)
+    logits = prior.logits(engine.edge_features(9))
+    want_m = (1.0 / (1.0 + np.exp(-logits.astype(np.float64)))).astype(np.float32).reshape(n, -1)
+    np.testing.assert_array_equal(m, want_m)
+    got = engine.evaluate_faco_evaluations(np.array([9, 9], np.uint64), np.array([1, 2], np.uint64), s.ants * 5, 8,
+                                           use_graph=False)
+    want = oracle_mod.run_faco(xy[None].astype(np.int32), [9], [1, 2], _oracle_settings(oracle_mod, s), 5, 8,
+                               multipliers=m[None])
+    for c in range(2):
+        assert got["items"][c]["tour"] == want["tours"][c].tolist(), c
