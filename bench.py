@@ -105,6 +105,50 @@ def gp_population_rate(ext, steps: int) -> dict:
             "mean_final_tour_length": float(raw["costs"].mean())}
 
 
+def learned_prior_legs(ext, budgets) -> dict:
+    """Learned edge prior: (1) the committed trained network against plain
+    FACO at equal FE on unseen TSP1000 instances (16: half uniform, half
+    clustered, x 8 seeds, k = 20 candidates) for each iteration budget;
+    (2) training throughput of the prior's forward+backward+Adam kernels on
+    those instances' candidate edges (labels: plain FACO's best tours)."""
+    from gplsaco_amd import prior as P
+    if not P.DEFAULT_WEIGHTS.exists():
+        return {"skipped": "no trained weights"}
+    net = P.EdgePrior.load()
+    out = {"weights": P.DEFAULT_WEIGHTS.name, "training": {k: v for k, v in net.info.items() if k != "sizes"}}
+    for its in budgets:
+        out[f"budget_{its}_iterations"] = P.equal_budget_comparison(net, iterations=its, instances=16, seeds=8)
+    # training throughput: 8 TSP1000 instances' edges, replicated to 5.1e6 edges
+    settings = ext.FixedFacoSettings()
+    settings.primary_width = 20
+    engine = ext.FacoBatchEngine(1000, 8, settings)
+    keys = []
+    for i in range(8):
+        key, xy = P.synthetic_instance(1000, i % 2 == 1, i, 104729)
+        engine.register_problem(key, xy)
+        keys.append(key)
+    res = engine.evaluate_faco_evaluations(np.array(keys, np.uint64), np.zeros(8, np.uint64), engine.settings.ants * 20)
+    feats = [engine.edge_features(k) for k in keys]
+    labels = [P.tour_labels(engine.problem_data(k)["primary"], it["tour"]) for k, it in zip(keys, res["items"])]
+    f = np.tile(np.concatenate(feats), (32, 1))
+    y = np.tile(np.concatenate(labels), 32)
+    scratch = P.EdgePrior(seed=9)
+    scratch.train(f[:65536], y[:65536], epochs=1)
+    started = time.perf_counter()
+    scratch.train(f, y, batch=8192, epochs=2)
+    seconds = time.perf_counter() - started
+    flops = 3 * 2 * (8 * 64 + 64 * 64 + 64) * f.shape[0] * 2
+    started = time.perf_counter()
+    scratch.logits(f)
+    infer_seconds = time.perf_counter() - started
+    out["training_throughput"] = {
+        "edges": int(f.shape[0]), "epochs": 2, "batch": 8192, "seconds": seconds,
+        "edges_per_s": 2 * f.shape[0] / seconds, "algorithmic_tflops": flops / seconds / 1e12,
+        "bf16_peak_tflops": 2500.0, "note": "host shuffle and upload per epoch included",
+        "inference_edges_per_s_including_copies": f.shape[0] / infer_seconds}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +160,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--population-steps", type=int, default=50,
                     help="iterations of the GP-controlled population measurement (0 skips it)")
+    ap.add_argument("--prior-budgets", default="100,300",
+                    help="iteration budgets of the learned-prior comparison (empty skips it)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the bench, run the calibration copy kernel over this many MiB (counter scaling)")
     args = ap.parse_args()
@@ -152,13 +198,17 @@ def main() -> int:
     steps_per_tour = timed["completed_construction_steps"] / tours
     evals_per_tour = timed["completed_ls_evaluations"] / tours
 
-    # Same iterations again, eagerly, with HIP events around every launch on
-    # the engine's stream: the average launch time of each kernel.
+    # Same iterations again, eagerly, with HIP events around every launch of
+    # the production kernels on the engine's stream: average launch times.
     prof = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=False,
-                                            profile_kernels=True)
+                                            time_kernels=True)
     ptours = prof["total_tour_evaluations"]
     launch_ms = prof["kernel_milliseconds"]["construction_and_search"] / args.steps
     colony_ms = prof["kernel_milliseconds"]["reduction_and_pheromone"] / args.steps
+    # A few iterations of the clock-instrumented kernel variant: per-ant phases.
+    phase_steps = min(args.steps, 10)
+    phases = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * phase_steps, use_graph=False,
+                                              profile_kernels=True)
     bytes_per_tour = algorithmic_bytes_per_tour(n, prof["completed_construction_steps"] / ptours,
                                                 prof["completed_ls_evaluations"] / ptours)
     tours_per_launch = colonies * ants
@@ -216,14 +266,16 @@ def main() -> int:
             "tours_per_launch": tours_per_launch,
             "launch_ms": launch_ms,
             "colony_update_ms": colony_ms,
-            "mean_ant_phase_ms_per_step": {k: v / args.steps for k, v in prof["ant_phase_milliseconds"].items()},
-            "mean_ant_subphase_megaclocks_per_step": {k: v / args.steps
-                                                      for k, v in prof["ant_subphase_megaclocks"].items()},
+            "mean_ant_phase_ms_per_step": {k: v / phase_steps for k, v in phases["ant_phase_milliseconds"].items()},
+            "mean_ant_subphase_megaclocks_per_step": {k: v / phase_steps
+                                                      for k, v in phases["ant_subphase_megaclocks"].items()},
         },
         "cpu_baseline": None,
     }
     if args.population_steps:
         line["config"]["gp_population"] = gp_population_rate(ext, args.population_steps)
+    if args.prior_budgets:
+        line["config"]["learned_prior"] = learned_prior_legs(ext, [int(b) for b in args.prior_budgets.split(",")])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(xys, keys, list(range(instances)), ants, args.steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

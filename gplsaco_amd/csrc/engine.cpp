@@ -328,14 +328,16 @@ struct Engine {
         checked(launch_colony_update(cp, colonies, stream), "colony update launch");
     }
     // Eager iterations with HIP events around each kernel on the engine
-    // stream; ms[0] += construction+LS, ms[1] += colony update.
-    void run_profiled(uint32_t colonies, uint64_t iterations, double* ms) {
+    // stream; ms[0] += construction+LS, ms[1] += colony update. With
+    // `clocks`, the ant kernel's clock-instrumented variant also fills the
+    // per-ant phase figures ms[2..9]; without, the production kernel runs.
+    void run_profiled(uint32_t colonies, uint64_t iterations, double* ms, bool clocks) {
         AntParams ap = ant_params(colonies);
         const ColonyParams cp = colony_params();
         const size_t ants_total = static_cast<size_t>(colonies) * config.ants;
         DeviceArray<uint64_t> ticks;
-        ticks.allocate(8 * ants_total);
-        ap.phase_ticks = ticks.data();
+        if (clocks) ticks.allocate(8 * ants_total);
+        ap.phase_ticks = clocks ? ticks.data() : nullptr;
         std::vector<uint64_t> host(8 * ants_total);
         int clock_khz = 0;
         checked(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
@@ -355,6 +357,7 @@ struct Engine {
             checked(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
             checked(hipEventElapsedTime(&b, ev[1], ev[2]), "event time");
             ms[0] += a; ms[1] += b;
+            if (!clocks) continue;
             ticks.download(host.data(), host.size(), stream);
             checked(hipStreamSynchronize(stream), "phase ticks");
             double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -440,6 +443,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
               uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
               uint64_t limit) {
     const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
+    const bool events = (flags & FACO_RUN_EVENTS) != 0;
     double kernel_ms[FACO_STAGE_COUNT] = {0};
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
@@ -456,7 +460,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     checked(hipStreamSynchronize(e.stream), "initialisation");
     const auto search_started = Clock::now();
     checked(hipEventRecord(e.ev_start, e.stream), "event");
-    if (profile) e.run_profiled(tasks, batches, kernel_ms);
+    if (profile || events) e.run_profiled(tasks, batches, kernel_ms, profile);
     else e.run(tasks, batches, use_graph);
     checked(hipEventRecord(e.ev_stop, e.stream), "event");
     checked(hipStreamSynchronize(e.stream), "search");

@@ -441,7 +441,7 @@ class FacoBatchEngine:
 
     def evaluate_faco_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, mne=8,
                                   preparation_mode="cached", *, checkpoint_iterations=(), profile_events_only=False,
-                                  use_graph=True, profile_kernels=False) -> dict:
+                                  use_graph=True, profile_kernels=False, time_kernels=False) -> dict:
         keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
         if keys.size != seeds.size:
             raise ValueError("FACO count entry needs complete task arrays")
@@ -457,14 +457,15 @@ class FacoBatchEngine:
         work = np.zeros((tasks, 2), dtype=np.uint64)
         result = Result()
         started = time.perf_counter()
-        flags = (1 if use_graph else 0) | (2 if profile_kernels else 0)
+        flags = (1 if use_graph else 0) | (2 if profile_kernels else 0) | (4 if time_kernels else 0)
         check(lib().faco_engine_evaluate_faco(self._handle, keys, seeds, tasks, limit, mne, flags,
                                               ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1)))
         out = _batch_output(result, tours, costs, preparation_mode, time.perf_counter() - started)
         out["colony_work"] = work
-        if profile_kernels:
+        if profile_kernels or time_kernels:
             out["kernel_milliseconds"] = {"construction_and_search": result.kernel_milliseconds[0],
                                           "reduction_and_pheromone": result.kernel_milliseconds[1]}
+        if profile_kernels:
             out["ant_phase_milliseconds"] = {"setup": result.kernel_milliseconds[2],
                                              "construction": result.kernel_milliseconds[3],
                                              "local_search": result.kernel_milliseconds[4]}

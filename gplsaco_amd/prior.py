@@ -9,13 +9,19 @@ edges of the best tours plain FACO finds on synthetic instances.
 """
 from __future__ import annotations
 
+import argparse
 import ctypes
+import json
+import time
+from pathlib import Path
 
 import numpy as np
 
 from ._native import check, lib
 
 PARAM_COUNT = 4801
+# Weights trained by `python -m gplsaco_amd.prior train` (k = 20 candidates).
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "data" / "edge_prior_k20.npz"
 
 
 class EdgePrior:
@@ -45,13 +51,19 @@ class EdgePrior:
             raise ValueError(f"the prior has {PARAM_COUNT} parameters")
         check(lib().faco_prior_set_params(self._handle, p))
 
-    def save(self, path) -> None:
-        np.savez(path, params=self.params(), spec=np.array([8, 64, 64, 1]))
+    def save(self, path, info=None) -> None:
+        """Parameters, layer widths and a JSON description of the training run."""
+        np.savez(path, params=self.params(), spec=np.array([8, 64, 64, 1]),
+                 info=np.array(json.dumps(info or {})))
 
     @classmethod
-    def load(cls, path) -> "EdgePrior":
+    def load(cls, path=DEFAULT_WEIGHTS) -> "EdgePrior":
         with np.load(path, allow_pickle=False) as data:
-            return cls(params=data["params"])
+            if data["spec"].tolist() != [8, 64, 64, 1]:
+                raise ValueError("not an 8-64-64-1 edge prior")
+            prior = cls(params=data["params"])
+            prior.info = json.loads(str(data["info"])) if "info" in data else {}
+            return prior
 
     def logits(self, features) -> np.ndarray:
         f = _features(features)
@@ -112,29 +124,152 @@ def tour_labels(primary, tour) -> np.ndarray:
     return ((primary == succ[:, None]) | (primary == pred[:, None])).astype(np.uint8).reshape(-1)
 
 
-def build_dataset(sizes, per_size: int, iterations: int, seed: int = 0):
-    """Features and labels from plain FACO's best tours on synthetic instances:
-    `per_size` instances (alternately uniform and clustered) of each size,
-    `iterations` FACO iterations each. Returns (features [E, 8], labels [E],
-    per-size summary)."""
-    from . import gp_faco_ext as ext
+def training_instances(count: int, seed: int = 0, n_range=(200, 1000), size_step: int = 100):
+    """(n, clustered, index) of `count` training instances: n uniform in
+    `n_range` (SplitMix64 words of `seed`) rounded to a multiple of
+    `size_step`, alternately uniform and clustered."""
+    from .instances import _key, _uniform_words
+    lo, hi = n_range
+    words = _uniform_words(_key(seed, 0, 0x51A5), count)
+    u = (words >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    sizes = np.clip(np.rint((lo + u * (hi - lo)) / size_step) * size_step, lo, hi).astype(int)
+    return [(int(sizes[i]), i % 2 == 1, i) for i in range(count)]
+
+
+def synthetic_instance(n, clustered, index, seed):
+    """(instance key, coordinates) of a generated instance."""
     from .instances import clustered_instance, instance_key, uniform_instance
+    gen = clustered_instance if clustered else uniform_instance
+    return instance_key(n, seed, index, clustered), gen(n, seed=seed, instance=index)
+
+
+def build_dataset(count: int, *, iterations: int = 200, primary_width: int = 20, seed: int = 0,
+                  n_range=(200, 1000), size_step: int = 100, log=None):
+    """Training edges of `count` synthetic instances (training_instances):
+    each instance is solved by plain FACO (one colony, `iterations`
+    iterations, default ants) on the GPU, and its candidate edges are
+    labelled by membership in the colony's best tour. Returns (features
+    [E, 8] float32, labels [E] uint8, per-size summary)."""
+    from . import gp_faco_ext as ext
+    groups = {}
+    for n, clustered, index in training_instances(count, seed, n_range, size_step):
+        groups.setdefault(n, []).append((clustered, index))
     feats, labs, summary = [], [], []
-    for n in sizes:
-        engine = ext.FacoBatchEngine(n, per_size, ext.FixedFacoSettings())
+    for n in sorted(groups):
+        items = groups[n]
+        settings = ext.FixedFacoSettings()
+        settings.primary_width = primary_width
+        engine = ext.FacoBatchEngine(n, len(items), settings)
         keys = []
-        for i in range(per_size):
-            clustered = i % 2 == 1
-            gen = clustered_instance if clustered else uniform_instance
-            key = instance_key(n, seed, i, clustered)
-            engine.register_problem(key, gen(n, seed=seed, instance=i))
+        for clustered, index in items:
+            key, xy = synthetic_instance(n, clustered, index, seed)
+            engine.register_problem(key, xy)
             keys.append(key)
         ants = engine.settings.ants
-        out = engine.evaluate_faco_evaluations(np.array(keys, dtype=np.uint64), np.full(per_size, seed, np.uint64),
+        out = engine.evaluate_faco_evaluations(np.array(keys, dtype=np.uint64), np.full(len(keys), seed, np.uint64),
                                                ants * iterations)
         for key, item in zip(keys, out["items"]):
             primary = engine.problem_data(key)["primary"]
             feats.append(engine.edge_features(key))
             labs.append(tour_labels(primary, item["tour"]))
-        summary.append({"n": n, "instances": per_size, "mean_cost": float(np.mean([i["cost"] for i in out["items"]]))})
+        summary.append({"n": n, "instances": len(items), "search_seconds": out["search_seconds"],
+                        "mean_cost": float(np.mean([i["cost"] for i in out["items"]]))})
+        if log:
+            log(f"dataset n={n}: {len(items)} instances, {out['search_seconds']:.2f} s")
+        del engine
     return np.concatenate(feats), np.concatenate(labs), summary
+
+
+def multipliers(logits, exponent: float = 1.0) -> np.ndarray:
+    """Heuristic multipliers sigmoid(logit) ** exponent (float32, > 0)."""
+    s = 1.0 / (1.0 + np.exp(-np.asarray(logits, dtype=np.float64)))
+    return np.maximum(s ** exponent, 1e-30).astype(np.float32)
+
+
+def equal_budget_comparison(prior: "EdgePrior", *, n: int = 1000, instances: int = 8, seeds: int = 4,
+                            iterations: int = 100, fractions=None, primary_width: int = 20, exponent: float = 1.0,
+                            seed: int = 7919):
+    """Learned-prior FACO against plain FACO on unseen instances (half
+    uniform, half clustered; instance seed `seed`, not a training seed).
+
+    Plain FACO runs `iterations` iterations per colony; the learned-prior
+    runs use each fraction of that budget (the same seeds, so every shorter
+    run is a prefix of the longer one). Returns plain's mean final tour
+    length, the learned mean at each fraction, the gap at the full budget
+    ((learned - plain) / plain) and the smallest fraction whose learned mean
+    reaches plain's mean (None when none does)."""
+    from . import gp_faco_ext as ext
+    fractions = sorted(fractions or [round(0.05 * k, 2) for k in range(4, 21)])
+    settings = ext.FixedFacoSettings()
+    settings.primary_width = primary_width
+    colonies = instances * seeds
+    engine = ext.FacoBatchEngine(n, colonies, settings)
+    keys = []
+    for i in range(instances):
+        key, xy = synthetic_instance(n, i % 2 == 1, i, seed)
+        engine.register_problem(key, xy)
+        keys.append(key)
+    task_keys = np.array([keys[c % instances] for c in range(colonies)], dtype=np.uint64)
+    task_seeds = np.array([1 + c // instances for c in range(colonies)], dtype=np.uint64)
+    ants = engine.settings.ants
+    mean = lambda out: float(np.mean([it["cost"] for it in out["items"]]))  # noqa: E731
+    plain = mean(engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * iterations))
+    for key in keys:
+        engine.set_edge_prior(key, multipliers(prior.logits(engine.edge_features(key)), exponent))
+    learned = {}
+    for f in fractions:
+        its = max(1, int(round(f * iterations)))
+        learned[f] = mean(engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * its))
+    reach = next((f for f in fractions if learned[f] <= plain), None)
+    full = learned[fractions[-1]]
+    return {"n": n, "instances": instances, "seeds": seeds, "iterations": iterations, "ants": ants,
+            "primary_width": primary_width, "exponent": exponent, "plain_mean": plain,
+            "learned_mean": {str(f): v for f, v in learned.items()}, "gap_at_equal_fe": (full - plain) / plain,
+            "fe_fraction_to_reach_plain": reach}
+
+
+def main(argv=None) -> int:
+    parser = argparse.ArgumentParser(prog="python -m gplsaco_amd.prior",
+                                     description="Train the learned edge prior on the GPU, or compare it with plain FACO.")
+    sub = parser.add_subparsers(dest="command", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--instances", type=int, default=2000)
+    t.add_argument("--iterations", type=int, default=300, help="FACO iterations per labelling run")
+    t.add_argument("--epochs", type=int, default=8)
+    t.add_argument("--batch", type=int, default=8192)
+    t.add_argument("--lr", type=float, default=5e-4)
+    t.add_argument("--pos-weight", type=float, default=1.0)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", default=str(DEFAULT_WEIGHTS))
+    c = sub.add_parser("compare")
+    c.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
+    c.add_argument("--iterations", type=int, default=300)
+    c.add_argument("--instances", type=int, default=16)
+    c.add_argument("--seeds", type=int, default=8)
+    c.add_argument("--exponent", type=float, default=1.0)
+    args = parser.parse_args(argv)
+    if args.command == "train":
+        started = time.perf_counter()
+        f, y, summary = build_dataset(args.instances, iterations=args.iterations, seed=args.seed,
+                                      log=lambda m: print(m, flush=True))
+        built = time.perf_counter()
+        prior = EdgePrior(seed=args.seed + 1)
+        losses = prior.train(f, y, batch=args.batch, epochs=args.epochs, lr=args.lr, pos_weight=args.pos_weight,
+                             seed=args.seed + 3)
+        info = {"instances": args.instances, "labelling_iterations": args.iterations, "edges": int(f.shape[0]),
+                "positive_rate": float(y.mean()), "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
+                "pos_weight": args.pos_weight, "seed": args.seed, "epoch_losses": [float(v) for v in losses],
+                "dataset_seconds": built - started, "training_seconds": time.perf_counter() - built,
+                "sizes": summary}
+        Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+        prior.save(args.out, info)
+        print(json.dumps({k: v for k, v in info.items() if k != "sizes"}), flush=True)
+    else:
+        prior = EdgePrior.load(args.weights)
+        print(json.dumps(equal_budget_comparison(prior, iterations=args.iterations, instances=args.instances,
+                                                 seeds=args.seeds, exponent=args.exponent)), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

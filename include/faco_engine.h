@@ -242,6 +242,8 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_
                                iterations; [5..9] = mean per-ant shader megaclocks of roulette
                                selection, relocation, MNE/checklist pushes, 2-opt scan and
                                2-opt apply, summed over iterations */
+#define FACO_RUN_EVENTS 4u  /* eager launches of the production kernels with HIP events around
+                               each: kernel_milliseconds[0] and [1] as for FACO_RUN_PROFILE */
 
 /* evaluate_faco_evaluations(keys, seeds, evaluation_limit_per_colony, mne)
  * (reference batch_engine.hpp:87-89, bindings.cpp:701-720): plain FACO on
