@@ -269,6 +269,8 @@ def main() -> int:
             "mean_ant_phase_ms_per_step": {k: v / phase_steps for k, v in phases["ant_phase_milliseconds"].items()},
             "mean_ant_subphase_megaclocks_per_step": {k: v / phase_steps
                                                       for k, v in phases["ant_subphase_megaclocks"].items()},
+            "mean_ant_hold_ms_per_step": phases["ant_hold_milliseconds"] / phase_steps,
+            "wave_utilisation": phases["wave_utilisation"] / phase_steps,
         },
         "cpu_baseline": None,
     }

@@ -336,9 +336,9 @@ struct Engine {
         const ColonyParams cp = colony_params();
         const size_t ants_total = static_cast<size_t>(colonies) * config.ants;
         DeviceArray<uint64_t> ticks;
-        if (clocks) ticks.allocate(8 * ants_total);
+        if (clocks) ticks.allocate(kPhaseSlots * ants_total);
         ap.phase_ticks = clocks ? ticks.data() : nullptr;
-        std::vector<uint64_t> host(8 * ants_total);
+        std::vector<uint64_t> host(clocks ? kPhaseSlots * ants_total : 0);
         int clock_khz = 0;
         checked(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
         const double ms_per_tick = clock_khz > 0 ? 1.0 / clock_khz : 1e-5;
@@ -346,6 +346,9 @@ struct Engine {
         for (auto& e : ev) checked(hipEventCreate(&e), "hipEventCreate");
         struct Destroy { hipEvent_t* e; ~Destroy() { for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]); } } d{ev};
         const uint32_t blocks = colonies * ap.blocks_per_colony;
+        // waves that can hold an ant at once: resident workgroups x waves
+        const double resident_waves = static_cast<double>(std::min<uint64_t>(
+            blocks, static_cast<uint64_t>(compute_units) * ant_search_blocks_per_cu(ap))) * ap.waves;
         for (uint64_t i = 0; i < iterations; ++i) {
             checked(hipEventRecord(ev[0], stream), "event");
             checked(launch_ant_search(ap, blocks, stream), "ant search launch");
@@ -360,13 +363,23 @@ struct Engine {
             if (!clocks) continue;
             ticks.download(host.data(), host.size(), stream);
             checked(hipStreamSynchronize(stream), "phase ticks");
-            double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (size_t k = 0; k < ants_total; ++k)
-                for (int j = 0; j < 8; ++j) phase[j] += static_cast<double>(host[8 * k + j]);
+            double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0}, held = 0;
+            uint64_t first = ~0ull, last = 0;
+            for (size_t k = 0; k < ants_total; ++k) {
+                const uint64_t* t = host.data() + kPhaseSlots * k;
+                for (int j = 0; j < 8; ++j) phase[j] += static_cast<double>(t[j]);
+                held += static_cast<double>(t[9] - t[8]);
+                first = std::min(first, t[8]);
+                last = std::max(last, t[9]);
+            }
             // mean per-ant milliseconds of setup, construction and local search
             for (int j = 0; j < 3; ++j) ms[2 + j] += phase[j] * ms_per_tick / ants_total;
             // mean per-ant shader clocks (millions) of the five sub-phases
             for (int j = 3; j < 8; ++j) ms[2 + j] += phase[j] / ants_total * 1e-6;
+            // mean per-ant milliseconds from taking the ant to its write-back
+            ms[10] += held * ms_per_tick / ants_total;
+            // share of the resident waves' time spent holding an ant
+            ms[11] += last > first ? held / (resident_waves * static_cast<double>(last - first)) : 0.0;
         }
     }
     // Run `iterations` batches; through a cached HIP graph of up to 16

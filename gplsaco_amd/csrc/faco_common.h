@@ -183,6 +183,8 @@ FACO_HD uint32_t pack_xy(uint32_t x, uint32_t y) { return x | (y << 16); }
 
 // ---------------------------------------------------------------------------
 // Kernel parameter blocks (passed by value as kernel arguments).
+constexpr uint32_t kPhaseSlots = 10;
+
 struct AntParams {
     uint32_t nbr_in_lds;                // neighbour rows staged in LDS (else read from HBM)
     uint32_t n, ants, pw, bw, lw, K;   // K = neighbour row width = max(pw, lw)
@@ -217,8 +219,9 @@ struct AntParams {
     uint32_t* diag_checklist;           // [A][5n] or null
     uint32_t* trace_selected;           // [A][n] or null
     double* trace_uniform;              // [A][n] or null
-    uint64_t* phase_ticks;              // [A][8] or null: setup, construction, LS (wall clock ticks);
-                                        // select, relocate, mne/push, LS scan, LS apply (shader clocks)
+    uint64_t* phase_ticks;              // [A][kPhaseSlots] or null: setup, construction, LS (wall clock
+                                        // ticks); select, relocate, mne/push, LS scan, LS apply (shader
+                                        // clocks); the ant's start and end (absolute wall clock ticks)
     const ControlState* control;        // [C] or null: start cities from the chosen region
     uint64_t* out_identity;             // [A][2] or null: tour identity of each ant
     uint32_t* out_zero;                 // [A] or null: position of city 0 | forward << 31
@@ -334,7 +337,8 @@ hipError_t launch_philox_check(uint64_t key, uint32_t batch, uint32_t ant, uint3
                                uint32_t* out, hipStream_t stream);
 // LDS bytes of one ant-search workgroup.
 inline uint32_t round16_host(uint32_t v) { return (v + 15u) & ~15u; }
-inline uint32_t ant_shared_bytes(uint32_t n, uint32_t nbr_lds_bytes) { return round16_host(4u * n) + nbr_lds_bytes; }
+// packed coordinates + the colony's parent links + neighbour rows (when staged in LDS)
+inline uint32_t ant_shared_bytes(uint32_t n, uint32_t nbr_lds_bytes) { return 2u * round16_host(4u * n) + nbr_lds_bytes; }
 inline uint32_t ant_wave_bytes(uint32_t n) { return 3u * round16_host(2u * n) + round16_host(n); }
 
 }  // namespace faco

@@ -127,15 +127,23 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     const uint32_t chunk_base = (p.explicit_mode ? blk : blk % p.blocks_per_colony) * p.chunk;
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
 
-    // Shared instance data: packed coordinates + neighbour rows.
+    // Shared data: packed coordinates, the colony's parent links (the
+    // parent tour is the same for every ant of the colony in an iteration;
+    // the explicit mode's per-ant parents stay in global memory) and the
+    // neighbour rows.
     uint32_t* s_xy = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* s_plinks = reinterpret_cast<uint32_t*>(lds + round16(4u * n));
     const uint16_t* s_nbr;
     {
         const uint32_t* gxy = p.coords + static_cast<size_t>(geom) * n;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_xy[i] = gxy[i];
+        if (!p.explicit_mode) {
+            const uint32_t* glinks = p.links + static_cast<size_t>(colony) * n;
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_plinks[i] = glinks[i];
+        }
         const uint16_t* g_nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
         if constexpr (NbrLds) {
-            uint16_t* l_nbr = reinterpret_cast<uint16_t*>(lds + round16(4u * n));
+            uint16_t* l_nbr = reinterpret_cast<uint16_t*>(lds + 2u * round16(4u * n));
             const uint4* src = reinterpret_cast<const uint4*>(g_nbr);
             uint4* dst = reinterpret_cast<uint4*>(l_nbr);
             const uint32_t chunks = p.nbr_lds_bytes / 16u;
@@ -148,7 +156,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     if (threadIdx.x == 0) next_slot = 0;
     __syncthreads();
 
-    unsigned char* mine = lds + round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u) + wave * p.per_wave_bytes;
+    unsigned char* mine = lds + 2u * round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u) + wave * p.per_wave_bytes;
     uint16_t* tour = reinterpret_cast<uint16_t*>(mine);
     uint16_t* pos = reinterpret_cast<uint16_t*>(mine + round16(2u * n));
     uint16_t* queue = reinterpret_cast<uint16_t*>(mine + 2u * round16(2u * n));
@@ -161,6 +169,11 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     const uint16_t* backup = p.backup + static_cast<size_t>(geom) * n * p.bw;
     const uint64_t limit = p.ls_limit;
     const bool in_row = lane < pw;
+    // Product-row loads are issued by every lane (lanes past the row re-read
+    // its last slot), not under an exec mask: with no branch around them the
+    // compiler counts the loads in flight exactly and waits only for the row
+    // it needs, not for the prefetches issued after it.
+    const uint32_t row_lane = in_row ? lane : (pw ? pw - 1u : 0u);
 
     for (;;) {
         uint32_t slot = 0;
@@ -175,7 +188,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 
         const size_t parent_base = static_cast<size_t>(p.per_ant_parent ? ant : colony) * n;
         const uint32_t* parent = p.parent + parent_base;
-        const uint32_t* links = p.links + parent_base;
+        const uint32_t* links = p.links + parent_base;   // explicit mode only; else s_plinks
         // Construction keeps the working tour as a doubly linked list
         // (predecessor | successor << 16 per city, over the tour/pos bytes)
         // plus the city at array position 0; the array itself is rebuilt
@@ -185,9 +198,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         uint32_t* lnk = reinterpret_cast<uint32_t*>(mine);
         uint32_t head = 0;
         if (linked) {
-            for (uint32_t i = lane; i < n; i += kWave) {
-                lnk[i] = links[i];
-                flags[i] = 0;
+            if (p.explicit_mode) {
+                for (uint32_t i = lane; i < n; i += kWave) { lnk[i] = links[i]; flags[i] = 0; }
+            } else {
+                for (uint32_t i = lane; i < n; i += kWave) { lnk[i] = s_plinks[i]; flags[i] = 0; }
             }
             head = uniform(parent[0]);
         } else {
@@ -247,28 +261,31 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         };
 
         // ------------------------------------------------------------ construction
-        // The product row and parent links of the next city are prefetched
-        // for both parent neighbours of the current city: the roulette
-        // mostly follows a parent edge, so the next step usually finds its
-        // row already in registers. A miss loads it; results are identical.
-        // pf_links: lane 0 holds the parent links of pf_node0, lane 1 those
-        // of pf_node1 (one vector load, read out only at the next step, so
-        // no wait is placed on the prefetches in this step)
-        uint32_t pf_node0 = n, pf_node1 = n, pf_links = 0;
-        double pf_row0 = 0.0, pf_row1 = 0.0;
-        // Inputs of the next step, loaded as soon as its city is known (right
+        // The product rows of both parent neighbours of the current city are
+        // prefetched at the start of a step: the roulette mostly follows a
+        // parent edge, so the next step usually finds its row already
+        // loaded. A miss loads the selected city's row right after the
+        // selection. The row a step uses is picked only when that step
+        // starts (row_src), so the loads' latency overlaps the relocation
+        // and the checklist pushes instead of ending the selection with a
+        // wait. Results are identical either way.
+        uint32_t pf_node0 = n, pf_node1 = n;
+        double pf_row0 = 0.0, pf_row1 = 0.0, miss_row = 0.0;
+        uint32_t row_src = 2;   // 0: pf_row0, 1: pf_row1, 2: miss_row
+        // Inputs of the next step, read as soon as its city is known (right
         // after selection, overlapping the relocation): candidate row,
         // unvisited flags (only the just selected city changes state before
-        // the next step, and a city is never its own candidate), product row
-        // and parent links.
+        // the next step, and a city is never its own candidate) and parent
+        // links (LDS).
         uint32_t cur_cand = n, cur_link_parent = 0;
         bool cur_free = false;
-        double cur_prod = 0.0;
         if (!p.explicit_mode) {
-            cur_cand = in_row ? s_nbr[current * K + lane] : n;
-            cur_free = in_row && cur_cand < n && !(flags[cur_cand] & kVisited);
-            cur_prod = in_row ? products[static_cast<size_t>(current) * pw + lane] : 0.0;
-            cur_link_parent = links[current];
+            const uint32_t c = s_nbr[current * K + row_lane];
+            const uint8_t fl = flags[c < n ? c : 0u];
+            cur_cand = in_row ? c : n;
+            cur_free = in_row && c < n && !(fl & kVisited);
+            miss_row = products[static_cast<size_t>(current) * pw + row_lane];
+            cur_link_parent = s_plinks[current];
         }
         // Roulette draws come in pools of 64 computed lane-parallel on the
         // vector unit: lane l holds the uniform of step u_base + l (one
@@ -286,13 +303,12 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 selected = uniform(visits[step]);
                 link = links[current];
             } else {
-                const double prod = cur_prod;
+                const double prod = row_src == 0 ? pf_row0 : (row_src == 1 ? pf_row1 : miss_row);
                 link = cur_link_parent;
                 pf_node0 = link & 0xffffu;
                 pf_node1 = link >> 16;
-                pf_row0 = in_row ? products[static_cast<size_t>(pf_node0) * pw + lane] : 0.0;
-                pf_row1 = in_row ? products[static_cast<size_t>(pf_node1) * pw + lane] : 0.0;
-                pf_links = lane < 2 ? links[lane == 0 ? pf_node0 : pf_node1] : 0u;
+                pf_row0 = products[static_cast<size_t>(pf_node0) * pw + row_lane];
+                pf_row1 = products[static_cast<size_t>(pf_node1) * pw + row_lane];
                 if (u_base == 0 || step - u_base >= kWave) {
                     u_base = step;
                     u_pool = step_uniform(key, batch, local_ant, step + lane);
@@ -353,14 +369,13 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 }
                 selected = uniform(selected);
                 if (selected < n) {
-                    cur_cand = in_row ? s_nbr[selected * K + lane] : n;
-                    cur_free = in_row && cur_cand < n && !(flags[cur_cand] & kVisited);
-                    if (selected == pf_node0) { cur_prod = pf_row0; cur_link_parent = read_lane(pf_links, 0); }
-                    else if (selected == pf_node1) { cur_prod = pf_row1; cur_link_parent = read_lane(pf_links, 1); }
-                    else {
-                        cur_prod = in_row ? products[static_cast<size_t>(selected) * pw + lane] : 0.0;
-                        cur_link_parent = links[selected];
-                    }
+                    const uint32_t c = s_nbr[selected * K + row_lane];
+                    const uint8_t fl = flags[c < n ? c : 0u];
+                    cur_cand = in_row ? c : n;
+                    cur_free = in_row && c < n && !(fl & kVisited);
+                    cur_link_parent = s_plinks[selected];
+                    row_src = selected == pf_node0 ? 0u : (selected == pf_node1 ? 1u : 2u);
+                    if (row_src == 2u) miss_row = products[static_cast<size_t>(selected) * pw + row_lane];
                 }
             }
             if (p.trace_selected && lane == 0) {
@@ -728,7 +743,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         const uint64_t tick3 = Profile ? wall_clock64() : 0;
         // ------------------------------------------------------------ write back
         uint32_t* out = p.out_tours + static_cast<size_t>(ant) * n;
-        for (uint32_t i = lane; i < n; i += kWave) out[i] = tour[i];
+        for (uint32_t i = lane; i < n; i += kWave) __builtin_nontemporal_store(static_cast<uint32_t>(tour[i]), out + i);
         if (p.diag_positions) {
             for (uint32_t i = lane; i < n; i += kWave) p.diag_positions[static_cast<size_t>(ant) * n + i] = pos[i];
         }
@@ -768,9 +783,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             info.start_node = start;
             p.out_info[ant] = info;
             if constexpr (Profile) {
-                uint64_t* t = p.phase_ticks + 8 * static_cast<size_t>(ant);
+                uint64_t* t = p.phase_ticks + kPhaseSlots * static_cast<size_t>(ant);
                 t[0] = tick1 - tick0; t[1] = tick2 - tick1; t[2] = tick3 - tick2;
                 for (int k = 0; k < 5; ++k) t[3 + k] = sub[k];
+                t[8] = tick0; t[9] = wall_clock64();
             }
         }
         wave_sync();

@@ -472,6 +472,11 @@ class FacoBatchEngine:
             km = result.kernel_milliseconds
             out["ant_subphase_megaclocks"] = {"select": km[5], "relocate": km[6], "mne_push": km[7],
                                               "ls_scan": km[8], "ls_apply": km[9]}
+            # per-ant milliseconds from taking the ant to its write-back, and the
+            # share of the resident waves' time spent holding an ant (both summed
+            # over iterations)
+            out["ant_hold_milliseconds"] = km[10]
+            out["wave_utilisation"] = km[11]
         return out
 
 

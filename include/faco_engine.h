@@ -241,7 +241,10 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_
                                and local-search milliseconds (device wall clock), summed over
                                iterations; [5..9] = mean per-ant shader megaclocks of roulette
                                selection, relocation, MNE/checklist pushes, 2-opt scan and
-                               2-opt apply, summed over iterations */
+                               2-opt apply, summed over iterations; [10] = mean per-ant
+                               milliseconds from taking the ant to its write-back and [11] =
+                               share of the resident waves' time spent holding an ant, both
+                               summed over iterations */
 #define FACO_RUN_EVENTS 4u  /* eager launches of the production kernels with HIP events around
                                each: kernel_milliseconds[0] and [1] as for FACO_RUN_PROFILE */
 
