@@ -104,10 +104,24 @@ LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_
     const size_t budget = max_lds_per_block();
     shape.nbr_lds_bytes = round16_host(2u * n * K);
     const uint32_t per_wave = ant_wave_bytes(n);
+    if (forced_waves) {
+        // a forced shape: any wave count 1..16 (waves take ants dynamically,
+        // so it need not divide the ants); bit 8 keeps neighbour rows global
+        const uint32_t w = forced_waves & 0xffu;
+        if (w < 1 || w > 16) throw std::invalid_argument("waves per workgroup must be in 1..16");
+        for (uint32_t lds_rows : {1u, 0u}) {
+            if (lds_rows && (forced_waves & 0x100u)) continue;
+            const size_t shared = ant_shared_bytes(n, lds_rows ? shape.nbr_lds_bytes : 0u);
+            if (shared + static_cast<size_t>(w) * per_wave <= budget) {
+                shape.waves = w; shape.nbr_in_lds = lds_rows;
+                return shape;
+            }
+        }
+        throw std::invalid_argument("forced shape exceeds the LDS budget");
+    }
     for (uint32_t lds_rows : {1u, 0u}) {
         const size_t shared = ant_shared_bytes(n, lds_rows ? shape.nbr_lds_bytes : 0u);
         for (uint32_t w : {16u, 8u, 4u, 2u, 1u}) {
-            if (forced_waves && w != forced_waves) continue;
             if (ants % w) continue;
             if (shared + static_cast<size_t>(w) * per_wave <= budget) {
                 shape.waves = w; shape.nbr_in_lds = lds_rows;
@@ -289,8 +303,7 @@ struct Engine {
         uint32_t bpc = 1;
         const uint64_t slots = static_cast<uint64_t>(compute_units) * ant_search_blocks_per_cu(p);
         while (static_cast<uint64_t>(colonies) * bpc < slots && config.ants % (2 * bpc * shape.waves) == 0) bpc *= 2;
-        if (forced_blocks_per_colony && config.ants % (forced_blocks_per_colony * shape.waves) == 0)
-            bpc = forced_blocks_per_colony;
+        if (forced_blocks_per_colony && config.ants % forced_blocks_per_colony == 0) bpc = forced_blocks_per_colony;
         p.blocks_per_colony = bpc;
         p.chunk = config.ants / bpc;
         p.ls_limit = config.ls_evaluation_limit;
