@@ -398,6 +398,9 @@ class FacoBatchEngine:
         return out
 
     def set_waves_per_block(self, waves: int, blocks_per_colony: int = 0) -> None:
+        """Force the ant kernel's shape: waves (ants in flight) per workgroup,
+        1..16, plus 0x100 to keep neighbour rows out of LDS; workgroups per
+        colony (0: automatic). waves = 0 restores the automatic shape."""
         check(lib().faco_engine_set_waves(self._handle, _unsigned(waves, "waves", 32),
                                           _unsigned(blocks_per_colony, "blocks_per_colony", 32)))
 

@@ -208,6 +208,10 @@ int faco_engine_settings(const faco_engine* engine, faco_settings* out);
  * choice, CMakeLists.txt:13-19); set_waves(0) restores the automatic shape. */
 int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint32_t* waves, uint32_t* nbr_in_lds,
                              uint32_t* lds_bytes, uint32_t* blocks_per_colony, uint32_t* blocks_per_cu);
+/* Force the ant kernel's launch shape: `waves` ants in flight per workgroup
+ * (1..16; bit 8 (0x100) keeps the neighbour rows in global memory instead
+ * of LDS) and `blocks_per_colony` workgroups per colony (must divide the
+ * ants; 0 = automatic). waves = 0 restores the automatic shape. */
 int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony);
 
 /* Local search of the ants: 0 = the reference's checklist 2-opt (default),
