@@ -166,6 +166,16 @@ class ControlTrace(ctypes.Structure):
     ]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("waves", ctypes.c_uint32),
+        ("nbr_in_lds", ctypes.c_uint32),
+        ("blocks_per_colony", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("ms_per_iteration", ctypes.c_double),
+    ]
+
+
 assert ctypes.sizeof(Settings) == 64
 assert ctypes.sizeof(Program) == 388
 assert ctypes.sizeof(Controller) == 1168
@@ -184,6 +194,10 @@ SIGNATURES = {
                                                    vp, vp, vp, vp]),
     "faco_engine_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Settings), ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.POINTER(vp)]),
+    "faco_engine_create_mixed": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Settings),
+                                                ctypes.POINTER(vp)]),
+    "faco_engine_autotune": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(TuneEntry),
+                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     "faco_engine_destroy": (None, [vp]),
     "faco_engine_settings": (ctypes.c_int, [vp, ctypes.POINTER(Settings)]),
     "faco_engine_launch_shape": (ctypes.c_int, [vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 5),

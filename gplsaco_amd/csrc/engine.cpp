@@ -149,6 +149,16 @@ struct Engine {
     LaunchShape shape;
     uint32_t forced_waves = 0, forced_blocks_per_colony = 0, xcd_remap = 1;
     uint32_t local_search = 0;  // 0: checklist 2-opt, 1: 2-opt + Or-opt
+    // Mixed-size batch: instances of any n up to `n` (which is then the
+    // stride of every per-city array; config.ants is the ant stride, the
+    // ants of the largest n). Each evaluation uploads every colony's shape
+    // and the largest-work-first block order.
+    bool mixed = false;
+    faco_settings base{};                  // settings as given (ants 0 = per-n default)
+    DeviceArray<ColonyShape> shapes;       // [C]
+    DeviceArray<uint32_t> order;           // [C]
+    bool shapes_active = false;
+    uint32_t min_ants = 0;                 // smallest ants of the current batch
     int compute_units = 1;
     double p_root = 0;
     // device state
@@ -180,8 +190,10 @@ struct Engine {
     uint32_t graph_colonies = 0, graph_iterations = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 
-    Engine(uint32_t dimension, uint32_t colonies, faco_settings settings, uint32_t pop)
+    Engine(uint32_t dimension, uint32_t colonies, faco_settings settings, uint32_t pop, bool mixed_sizes = false)
         : n(dimension), panel(colonies), population(pop), colonies_capacity(colonies * pop) {
+        mixed = mixed_sizes;
+        base = settings;
         require(colonies >= 1 && colonies <= 4096 && pop >= 1 && pop <= 128,
                 "panel size must be in 1..4096 and population in 1..128");
         config = normalized_settings(settings, n);
@@ -206,6 +218,7 @@ struct Engine {
         alloc(parent, C * n); alloc(links, C * n); alloc(epoch, C * n); alloc(global, C * n);
         alloc(trails, C * n * config.primary_width); alloc(products, C * n * config.primary_width);
         alloc(tours, A * n); alloc(info, A);
+        if (mixed) { alloc(shapes, C); alloc(order, C); }
     }
     ~Engine() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -228,6 +241,40 @@ struct Engine {
         alloc(b->masks, C); alloc(b->alternatives, C); alloc(b->regions, 128 * C); alloc(b->actions, C);
         ctl = std::move(b);
         resident.clear();  // the control geometry is uploaded with the next geometry upload
+    }
+
+    // Ants of an instance of `m` cities (the settings' per-size default
+    // when they give ants = 0).
+    uint32_t instance_ants(uint32_t m) const { return mixed ? normalized_settings(base, m).ants : config.ants; }
+
+    // Mixed batch: every colony's shape for an evaluation limit per colony
+    // (a whole number of its own ant batches), uploaded with the
+    // largest-work-first block order. Returns the most iterations any colony
+    // runs.
+    uint64_t prepare_shapes(const uint64_t* instance_keys, uint32_t tasks, uint64_t limit) {
+        std::vector<ColonyShape> host(tasks);
+        uint64_t most = 0;
+        uint32_t smallest = UINT32_MAX;
+        for (uint32_t c = 0; c < tasks; ++c) {
+            const uint32_t m = problem(instance_keys[c]).n, a = instance_ants(m);
+            require(limit % a == 0 && limit / a <= UINT32_MAX,
+                    "evaluation limit must be a whole number of ant batches for every colony");
+            host[c] = ColonyShape{m, a, static_cast<uint32_t>(limit / a), 0u};
+            most = std::max<uint64_t>(most, limit / a);
+            smallest = std::min(smallest, a);
+        }
+        std::vector<uint32_t> ord(tasks);
+        for (uint32_t c = 0; c < tasks; ++c) ord[c] = c;
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+            return static_cast<uint64_t>(host[x].n) * host[x].ants * host[x].batches >
+                   static_cast<uint64_t>(host[y].n) * host[y].ants * host[y].batches;
+        });
+        shapes.upload(host.data(), tasks, stream);
+        order.upload(ord.data(), tasks, stream);
+        if (smallest != min_ants) drop_graph();   // the workgroup split depends on it
+        min_ants = smallest;
+        shapes_active = true;
+        return most;
     }
 
     void set_waves(uint32_t waves, uint32_t blocks_per_colony) {
@@ -263,11 +310,14 @@ struct Engine {
         const uint32_t bw = config.backup_width, pw = config.primary_width;
         for (uint32_t g = 0; g < slots.size(); ++g) {
             const Prepared& p = problem(slots[g]);
-            geo.coords.upload(p.packed.data(), n, stream, static_cast<size_t>(g) * n);
+            // counts are the instance's own size (smaller than the stride n
+            // only in a mixed-size batch), offsets use the stride
+            const uint32_t m = p.n;
+            geo.coords.upload(p.packed.data(), m, stream, static_cast<size_t>(g) * n);
             geo.nbr.upload(p.nbr.data(), nbr_stride, stream, static_cast<size_t>(g) * nbr_stride);
-            if (bw) geo.backup.upload(p.backup16.data(), static_cast<size_t>(n) * bw, stream, static_cast<size_t>(g) * n * bw);
+            if (bw) geo.backup.upload(p.backup16.data(), static_cast<size_t>(m) * bw, stream, static_cast<size_t>(g) * n * bw);
             if (p.prior.empty()) {
-                geo.eta.upload(p.eta.data(), static_cast<size_t>(n) * pw, stream, static_cast<size_t>(g) * n * pw);
+                geo.eta.upload(p.eta.data(), static_cast<size_t>(m) * pw, stream, static_cast<size_t>(g) * n * pw);
             } else {
                 // learned edge prior: eta' = eta * multiplier, in FP64
                 std::vector<double> scaled(p.eta.size());
@@ -275,12 +325,12 @@ struct Engine {
                 geo.eta.upload(scaled.data(), scaled.size(), stream, static_cast<size_t>(g) * n * pw);
                 checked(hipStreamSynchronize(stream), "prior upload");
             }
-            geo.initial.upload(p.initial.data(), n, stream, static_cast<size_t>(g) * n);
+            geo.initial.upload(p.initial.data(), m, stream, static_cast<size_t>(g) * n);
             geo.initial_cost.upload(&p.initial_cost, 1, stream, g);
             if (ctl) {
-                const auto samples = sample_nodes(n, slots[g]);
+                const auto samples = sample_nodes(m, slots[g]);
                 ctl->samples.upload(samples.data(), kSamples, stream, static_cast<size_t>(g) * kSamples);
-                ctl->local_scale.upload(p.local_scale.data(), n, stream, static_cast<size_t>(g) * n);
+                ctl->local_scale.upload(p.local_scale.data(), m, stream, static_cast<size_t>(g) * n);
                 ctl->epsilon.upload(&p.scale_epsilon, 1, stream, g);
             }
         }
@@ -296,14 +346,16 @@ struct Engine {
         p.lw = config.ls_width; p.K = K; p.waves = shape.waves;
         p.nbr_lds_bytes = shape.nbr_lds_bytes; p.nbr_stride = nbr_stride;
         p.per_wave_bytes = ant_wave_bytes(n);
-        p.xcd_remap = xcd_remap;
+        p.xcd_remap = shapes_active ? 0u : xcd_remap;   // mixed: keep the largest-first order
         // Split each colony over just enough workgroups to fill every CU:
         // fewer, larger chunks let the waves of a workgroup balance uneven
         // ants among themselves.
         uint32_t bpc = 1;
         const uint64_t slots = static_cast<uint64_t>(compute_units) * ant_search_blocks_per_cu(p);
-        while (static_cast<uint64_t>(colonies) * bpc < slots && config.ants % (2 * bpc * shape.waves) == 0) bpc *= 2;
-        if (forced_blocks_per_colony && config.ants % forced_blocks_per_colony == 0) bpc = forced_blocks_per_colony;
+        // (a mixed batch splits every colony the same way: bpc divides its smallest ants)
+        const uint32_t ants_split = shapes_active ? min_ants : config.ants;
+        while (static_cast<uint64_t>(colonies) * bpc < slots && ants_split % (2 * bpc * shape.waves) == 0) bpc *= 2;
+        if (forced_blocks_per_colony && ants_split % forced_blocks_per_colony == 0) bpc = forced_blocks_per_colony;
         p.blocks_per_colony = bpc;
         p.chunk = config.ants / bpc;
         p.ls_limit = config.ls_evaluation_limit;
@@ -312,6 +364,7 @@ struct Engine {
         p.products = products.data(); p.parent = parent.data(); p.links = links.data();
         p.state = state.data(); p.keys = keys.data(); p.targets = nullptr;
         p.geometry = geometry.data(); p.out_tours = tours.data(); p.out_info = info.data();
+        if (shapes_active) { p.shapes = shapes.data(); p.order = order.data(); }
         (void)colonies;
         return p;
     }
@@ -324,6 +377,7 @@ struct Engine {
         p.tours = tours.data(); p.info = info.data(); p.state = state.data(); p.parent = parent.data();
         p.links = links.data(); p.epoch = epoch.data(); p.global = global.data(); p.trails = trails.data();
         p.products = products.data();
+        p.shapes = shapes_active ? shapes.data() : nullptr;
         return p;
     }
     void reset(uint32_t colonies, uint32_t mne) {
@@ -333,6 +387,7 @@ struct Engine {
         r.geometry = geometry.data(); r.initial = geo.initial.data(); r.initial_cost = geo.initial_cost.data();
         r.eta = geo.eta.data(); r.state = state.data(); r.parent = parent.data(); r.links = links.data();
         r.epoch = epoch.data(); r.global = global.data(); r.trails = trails.data(); r.products = products.data();
+        r.shapes = shapes_active ? shapes.data() : nullptr;
         checked(launch_colony_reset(r, colonies, stream), "colony reset launch");
     }
     void iteration(uint32_t colonies, const AntParams& ap, const ColonyParams& cp) {
@@ -449,11 +504,11 @@ std::vector<int32_t> integer_coordinates(const double* xy, uint32_t n) {
 }
 
 void fill_result(faco_result* r, const Engine& e, uint32_t colonies, uint64_t limit, uint64_t batches,
-                 uint64_t steps, uint64_t evals, double actual, double search, float gpu_ms) {
+                 uint64_t fe_per_colony, uint64_t steps, uint64_t evals, double actual, double search, float gpu_ms) {
     if (!r) return;
     std::memset(r, 0, sizeof(*r));
     r->evaluation_limit_per_colony = limit;
-    r->completed_tour_evaluations_per_colony = batches * e.config.ants;
+    r->completed_tour_evaluations_per_colony = fe_per_colony;
     r->total_tour_evaluations = r->completed_tour_evaluations_per_colony * colonies;
     r->launched_batches = r->completed_batches = batches;
     r->completed_construction_steps = steps;
@@ -482,6 +537,8 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     std::vector<uint64_t> keys(tasks);
     for (uint32_t c = 0; c < tasks; ++c) keys[c] = colony_key(instance_keys[c], seeds[c]);
     e.keys.upload(keys.data(), tasks, e.stream);
+    struct Inactive { Engine& e; ~Inactive() { e.shapes_active = false; } } inactive{e};
+    if (e.mixed) batches = e.prepare_shapes(instance_keys, tasks, limit);
     e.reset(tasks, mne);
     checked(hipStreamSynchronize(e.stream), "initialisation");
     const auto search_started = Clock::now();
@@ -497,13 +554,19 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     e.state.download(states.data(), tasks, e.stream);
     if (tours) e.global.download(tours, static_cast<size_t>(tasks) * e.n, e.stream);
     checked(hipStreamSynchronize(e.stream), "readback");
+    if (tours && e.mixed) {
+        // a smaller colony's row ends at its own n: the rest is marked
+        for (uint32_t c = 0; c < tasks; ++c)
+            for (uint32_t i = e.problem(instance_keys[c]).n; i < e.n; ++i) tours[static_cast<size_t>(c) * e.n + i] = UINT32_MAX;
+    }
     uint64_t steps = 0, evals = 0;
     for (uint32_t c = 0; c < tasks; ++c) {
         if (costs) costs[c] = states[c].global_cost;
         if (work) { work[2 * c] = states[c].work_construction; work[2 * c + 1] = states[c].work_ls; }
         steps += states[c].work_construction; evals += states[c].work_ls;
     }
-    fill_result(result, e, tasks, limit, batches, steps, evals, seconds_since(started), search, gpu_ms);
+    fill_result(result, e, tasks, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
+                seconds_since(started), search, gpu_ms);
     if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
 }
 
@@ -543,6 +606,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     require(lock.owns_lock(), "one evaluation per engine at a time");
     require(current_device() == e.device, "engine must be used on the device it was created on");
     require(spec.mode >= FACO_CONTROL_PROGRAM && spec.mode <= FACO_CONTROL_FACTORIAL, "unknown control mode");
+    require(!e.mixed, "controlled evaluations need a fixed-size engine");
     require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
     const uint32_t ants = e.config.ants;
     require(limit % ants == 0 && limit / ants <= UINT32_MAX, "evaluation limit must be a whole number of ant batches");
@@ -698,7 +762,8 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         }
         steps += st[c].work_construction; evals += st[c].work_ls;
     }
-    fill_result(result, e, C, limit, batches, steps, evals, seconds_since(started), search, gpu_ms);
+    fill_result(result, e, C, limit, batches, batches * e.config.ants, steps, evals, seconds_since(started), search,
+                gpu_ms);
 }
 
 }  // namespace
@@ -755,6 +820,16 @@ int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_setting
     });
 }
 
+int faco_engine_create_mixed(uint32_t max_dimension, uint32_t colonies, const faco_settings* settings,
+                             faco_engine** out) {
+    return guarded([&] {
+        require(out && settings, "null argument");
+        auto engine = std::make_unique<faco_engine>();
+        engine->impl = std::make_unique<Engine>(max_dimension, colonies, *settings, 1u, true);
+        *out = engine.release();
+    });
+}
+
 void faco_engine_destroy(faco_engine* engine) { delete engine; }
 
 int faco_engine_settings(const faco_engine* engine, faco_settings* out) {
@@ -763,6 +838,56 @@ int faco_engine_settings(const faco_engine* engine, faco_settings* out) {
 
 int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony) {
     return guarded([&] { require(engine, "null argument"); engine->impl->set_waves(waves, blocks_per_colony); });
+}
+
+int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks,
+                         uint32_t iterations, faco_tune_entry* table, uint32_t capacity, uint32_t* count) {
+    return guarded([&] {
+        require(engine && instance_keys && seeds, "null argument");
+        require(iterations >= 1 && iterations <= 1000, "autotune iterations must be in 1..1000");
+        Engine& e = *engine->impl;
+        require(e.population == 1, "population engines tune through a fixed panel");
+        // An equal evaluation budget for every colony: a whole number of
+        // iterations of each colony's ants.
+        uint64_t unit = 1;
+        for (uint32_t c = 0; c < tasks; ++c) {
+            const uint64_t a = e.instance_ants(e.problem(instance_keys[c]).n);
+            unit = unit / std::gcd(unit, a) * a;
+        }
+        const uint64_t limit = unit * iterations;
+        std::vector<faco_tune_entry> rows;
+        for (uint32_t w : {16u, 12u, 10u, 8u, 6u, 4u, 2u, 1u}) {
+            for (uint32_t lds : {1u, 0u}) {
+                const uint32_t forced = w | (lds ? 0u : 0x100u);
+                try {
+                    (void)choose_shape(e.n, e.K, e.config.ants, forced);
+                } catch (const std::invalid_argument&) {
+                    continue;   // does not fit the LDS budget
+                }
+                e.set_waves(forced, 0);
+                faco_result r{};
+                evaluate(e, instance_keys, seeds, tasks, limit / e.config.ants, 8, FACO_RUN_EVENTS, &r, nullptr,
+                         nullptr, nullptr, limit);
+                faco_tune_entry row{};
+                row.waves = e.shape.waves;
+                row.nbr_in_lds = e.shape.nbr_in_lds;
+                e.shapes_active = e.mixed;
+                if (e.mixed) (void)e.prepare_shapes(instance_keys, tasks, limit);
+                row.blocks_per_colony = e.ant_params(tasks).blocks_per_colony;
+                e.shapes_active = false;
+                row.ms_per_iteration = (r.kernel_milliseconds[0] + r.kernel_milliseconds[1]) /
+                                       static_cast<double>(std::max<uint64_t>(r.launched_batches, 1));
+                rows.push_back(row);
+            }
+        }
+        require(!rows.empty(), "no launch shape fits the LDS budget");
+        size_t best = 0;
+        for (size_t i = 1; i < rows.size(); ++i)
+            if (rows[i].ms_per_iteration < rows[best].ms_per_iteration) best = i;
+        e.set_waves(rows[best].waves | (rows[best].nbr_in_lds ? 0u : 0x100u), 0);
+        if (table) std::copy(rows.begin(), rows.begin() + std::min<size_t>(rows.size(), capacity), table);
+        if (count) *count = static_cast<uint32_t>(rows.size());
+    });
 }
 
 int faco_engine_set_local_search(faco_engine* engine, uint32_t mode) {
@@ -796,7 +921,15 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double
         Engine& e = *engine->impl;
         std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
         require(lock.owns_lock(), "registration and evaluation of one engine must not overlap");
-        require(n == e.n, "instance dimension does not match the engine");
+        if (e.mixed) {
+            require(n <= e.n, "instance larger than the mixed engine's maximum dimension");
+            const faco_settings own = normalized_settings(e.base, n);
+            require(own.primary_width == e.config.primary_width && own.backup_width == e.config.backup_width &&
+                    own.ls_width == e.config.ls_width,
+                    "a mixed-size batch needs the same candidate widths for every instance");
+        } else {
+            require(n == e.n, "instance dimension does not match the engine");
+        }
         auto coords = integer_coordinates(xy, n);
         const auto existing = e.registry.find(key);
         if (existing != e.registry.end()) {
@@ -815,7 +948,7 @@ int faco_engine_edge_features(faco_engine* engine, uint64_t key, float* features
         require(engine && features, "null argument");
         Engine& e = *engine->impl;
         const Prepared& p = e.problem(key);
-        const uint32_t n = e.n, pw = e.config.primary_width;
+        const uint32_t n = p.n, pw = e.config.primary_width;
         double mean = 0;
         for (uint32_t a = 0; a < n; ++a) mean += p.local_scale[a];
         const float mean_scale = static_cast<float>(std::max(mean / n, p.scale_epsilon));
@@ -842,7 +975,7 @@ int faco_engine_set_edge_prior(faco_engine* engine, uint64_t key, const float* m
         require(lock.owns_lock(), "the prior of an engine can not change during an evaluation");
         auto it = e.registry.find(key);
         require(it != e.registry.end(), "task instance has not been registered");
-        const size_t cells = static_cast<size_t>(e.n) * e.config.primary_width;
+        const size_t cells = static_cast<size_t>(it->second.n) * e.config.primary_width;
         if (multipliers) {
             for (size_t i = 0; i < cells; ++i)
                 require(std::isfinite(multipliers[i]) && multipliers[i] > 0.0f, "prior multipliers must be positive");
@@ -858,7 +991,7 @@ int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_
     return guarded([&] {
         require(engine && prior, "null argument");
         Engine& e = *engine->impl;
-        const size_t cells = static_cast<size_t>(e.n) * e.config.primary_width;
+        const size_t cells = static_cast<size_t>(e.problem(key).n) * e.config.primary_width;
         std::vector<float> features(cells * 8), logits(cells), mult(cells);
         if (faco_engine_edge_features(engine, key, features.data()) != FACO_OK) throw std::runtime_error(g_error);
         DeviceArray<float> f, s;
@@ -904,8 +1037,9 @@ int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys
         require(engine && instance_keys && seeds, "null argument");
         Engine& e = *engine->impl;
         require(e.population == 1, "population engines evaluate through the population entry point");
-        require(limit % e.config.ants == 0 && limit / e.config.ants <= UINT32_MAX,
-                "evaluation limit must be a whole number of ant batches");
+        if (!e.mixed)
+            require(limit % e.config.ants == 0 && limit / e.config.ants <= UINT32_MAX,
+                    "evaluation limit must be a whole number of ant batches");
         evaluate(e, instance_keys, seeds, tasks, limit / e.config.ants, mne, flags, result, tours,
                  costs, work, limit);
     });

@@ -185,6 +185,13 @@ FACO_HD uint32_t pack_xy(uint32_t x, uint32_t y) { return x | (y << 16); }
 // Kernel parameter blocks (passed by value as kernel arguments).
 constexpr uint32_t kPhaseSlots = 10;
 
+// Per-colony shape of a mixed-size batch: the colony's city count, ants and
+// its own iteration budget (a colony past its budget is left unchanged).
+// Null in fixed-size engines, where every colony has n and ants of the
+// parameter block. In a mixed batch the parameter block's n and ants are the
+// strides of every per-city and per-ant array (the batch maxima).
+struct ColonyShape { uint32_t n, ants, batches, pad; };
+
 struct AntParams {
     uint32_t nbr_in_lds;                // neighbour rows staged in LDS (else read from HBM)
     uint32_t n, ants, pw, bw, lw, K;   // K = neighbour row width = max(pw, lw)
@@ -225,6 +232,8 @@ struct AntParams {
     const ControlState* control;        // [C] or null: start cities from the chosen region
     uint64_t* out_identity;             // [A][2] or null: tour identity of each ant
     uint32_t* out_zero;                 // [A] or null: position of city 0 | forward << 31
+    const ColonyShape* shapes;          // [C] or null (fixed size)
+    const uint32_t* order;              // [C] or null: colony of each block group (largest work first)
 };
 
 struct ColonyParams {
@@ -244,6 +253,7 @@ struct ColonyParams {
     uint32_t* global;                   // [C][n]
     double* trails;                     // [C][n][pw]
     double* products;                   // [C][n][pw]
+    const ColonyShape* shapes;          // [C] or null (fixed size)
 };
 
 struct ResetParams {
@@ -256,6 +266,7 @@ struct ResetParams {
     ColonyState* state;
     uint32_t* parent; uint32_t* links; uint32_t* epoch; uint32_t* global;
     double* trails; double* products;
+    const ColonyShape* shapes;          // [C] or null (fixed size)
 };
 
 // Controlled loop (control_kernels.hip). One workgroup per colony.

@@ -115,16 +115,24 @@ template<bool NbrLds, bool Profile>
 __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     __shared__ uint32_t next_slot;
-    const uint32_t n = p.n;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = uniform(threadIdx.x >> 6);
     uint32_t blk = blockIdx.x;
     const uint32_t nblocks = gridDim.x;
     if (p.xcd_remap && (nblocks & 7u) == 0) blk = (blk & 7u) * (nblocks >> 3) + (blk >> 3);
     // One workgroup = one chunk of one colony's ants; its waves take ants
-    // from the chunk dynamically so a slow ant does not idle the others.
-    const uint32_t colony = p.explicit_mode ? 0u : blk / p.blocks_per_colony;
-    const uint32_t chunk_base = (p.explicit_mode ? blk : blk % p.blocks_per_colony) * p.chunk;
+    // from the chunk dynamically so a slow ant does not idle the others. In a
+    // mixed-size batch the block groups follow `order` (largest work first:
+    // the hardware dispatches workgroups in index order, so the longest
+    // colonies start first) and each colony has its own n and ants; `ns` is
+    // the stride of every per-city array.
+    const uint32_t group = p.explicit_mode ? 0u : blk / p.blocks_per_colony;
+    const uint32_t colony = p.order ? p.order[group] : group;
+    const uint32_t ns = p.n;
+    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
+    const uint32_t chunk = p.shapes ? p.shapes[colony].ants / p.blocks_per_colony : p.chunk;
+    if (p.shapes && p.state[colony].batch >= p.shapes[colony].batches) return;   // colony done
+    const uint32_t chunk_base = (p.explicit_mode ? blk : blk % p.blocks_per_colony) * chunk;
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
 
     // Shared data: packed coordinates, the colony's parent links (the
@@ -135,10 +143,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     uint32_t* s_plinks = reinterpret_cast<uint32_t*>(lds + round16(4u * n));
     const uint16_t* s_nbr;
     {
-        const uint32_t* gxy = p.coords + static_cast<size_t>(geom) * n;
+        const uint32_t* gxy = p.coords + static_cast<size_t>(geom) * ns;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_xy[i] = gxy[i];
         if (!p.explicit_mode) {
-            const uint32_t* glinks = p.links + static_cast<size_t>(colony) * n;
+            const uint32_t* glinks = p.links + static_cast<size_t>(colony) * ns;
             for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_plinks[i] = glinks[i];
         }
         const uint16_t* g_nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
@@ -146,7 +154,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             uint16_t* l_nbr = reinterpret_cast<uint16_t*>(lds + 2u * round16(4u * n));
             const uint4* src = reinterpret_cast<const uint4*>(g_nbr);
             uint4* dst = reinterpret_cast<uint4*>(l_nbr);
-            const uint32_t chunks = p.nbr_lds_bytes / 16u;
+            const uint32_t chunks = (2u * n * p.K + 15u) / 16u;   // this instance's rows
             for (uint32_t i = threadIdx.x; i < chunks; i += blockDim.x) dst[i] = src[i];
             s_nbr = l_nbr;
         } else {
@@ -165,8 +173,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     const uint32_t pw = p.pw, K = p.K, lw = p.lw;
     const uint64_t key = p.explicit_mode ? 0ull : p.keys[colony];
     const uint32_t batch = p.explicit_mode ? 0u : p.state[colony].batch;
-    const double* products = p.products + static_cast<size_t>(colony) * n * pw;
-    const uint16_t* backup = p.backup + static_cast<size_t>(geom) * n * p.bw;
+    const double* products = p.products + static_cast<size_t>(colony) * ns * pw;
+    const uint16_t* backup = p.backup + static_cast<size_t>(geom) * ns * p.bw;
     const uint64_t limit = p.ls_limit;
     const bool in_row = lane < pw;
     // Product-row loads are issued by every lane (lanes past the row re-read
@@ -179,14 +187,14 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(&next_slot, 1u);
         slot = uniform(slot);
-        if (slot >= p.chunk) break;
+        if (slot >= chunk) break;
         const uint32_t local_ant = chunk_base + slot;                // ant index within the colony
         const uint32_t ant = colony * p.ants + local_ant;            // global ant index
         const uint64_t tick0 = Profile ? wall_clock64() : 0;
         uint64_t sub[5] = {0, 0, 0, 0, 0};  // select, relocate, mne/push, LS scan, LS apply (shader clocks)
         uint64_t mark = 0;
 
-        const size_t parent_base = static_cast<size_t>(p.per_ant_parent ? ant : colony) * n;
+        const size_t parent_base = static_cast<size_t>(p.per_ant_parent ? ant : colony) * ns;
         const uint32_t* parent = p.parent + parent_base;
         const uint32_t* links = p.links + parent_base;   // explicit mode only; else s_plinks
         // Construction keeps the working tour as a doubly linked list
@@ -215,8 +223,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         wave_sync();
 
         const uint32_t target = p.explicit_mode ? p.targets[ant] : p.state[colony].target;
-        const uint32_t* visits = p.visits + static_cast<size_t>(ant) * n;
-        uint32_t* checklist_log = p.diag_checklist ? p.diag_checklist + static_cast<size_t>(ant) * 5u * n : nullptr;
+        const uint32_t* visits = p.visits + static_cast<size_t>(ant) * ns;
+        uint32_t* checklist_log = p.diag_checklist ? p.diag_checklist + static_cast<size_t>(ant) * 5u * ns : nullptr;
         int64_t cost = p.per_ant_parent ? p.parent_cost[ant] : p.state[colony].parent_cost;
         uint32_t current;
         if (p.explicit_mode) {
@@ -231,8 +239,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         current = uniform(current);
         const uint32_t start = current;
         if (p.trace_selected && lane == 0) {
-            p.trace_selected[static_cast<size_t>(ant) * n] = current;
-            p.trace_uniform[static_cast<size_t>(ant) * n] = 0.0;
+            p.trace_selected[static_cast<size_t>(ant) * ns] = current;
+            p.trace_uniform[static_cast<size_t>(ant) * ns] = 0.0;
         }
         if (lane == 0) flags[current] = kVisited;
         // array mode: position of `current`; linked mode: its links
@@ -379,8 +387,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 }
             }
             if (p.trace_selected && lane == 0) {
-                p.trace_selected[static_cast<size_t>(ant) * n + step] = selected;
-                p.trace_uniform[static_cast<size_t>(ant) * n + step] = u;
+                p.trace_selected[static_cast<size_t>(ant) * ns + step] = selected;
+                p.trace_uniform[static_cast<size_t>(ant) * ns + step] = u;
             }
             if constexpr (Profile) { const uint64_t t = clock64(); sub[0] += t - mark; mark = t; }
             if (selected >= n) { exhausted = 1; break; }
@@ -537,7 +545,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             wave_sync();
         }
         if (p.diag_construction) {
-            for (uint32_t i = lane; i < n; i += kWave) p.diag_construction[static_cast<size_t>(ant) * n + i] = tour[i];
+            for (uint32_t i = lane; i < n; i += kWave) p.diag_construction[static_cast<size_t>(ant) * ns + i] = tour[i];
         }
         const int64_t construction_cost = cost;
         const uint64_t tick2 = Profile ? wall_clock64() : 0;
@@ -742,10 +750,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 
         const uint64_t tick3 = Profile ? wall_clock64() : 0;
         // ------------------------------------------------------------ write back
-        uint32_t* out = p.out_tours + static_cast<size_t>(ant) * n;
+        uint32_t* out = p.out_tours + static_cast<size_t>(ant) * ns;
         for (uint32_t i = lane; i < n; i += kWave) __builtin_nontemporal_store(static_cast<uint32_t>(tour[i]), out + i);
         if (p.diag_positions) {
-            for (uint32_t i = lane; i < n; i += kWave) p.diag_positions[static_cast<size_t>(ant) * n + i] = pos[i];
+            for (uint32_t i = lane; i < n; i += kWave) p.diag_positions[static_cast<size_t>(ant) * ns + i] = pos[i];
         }
         if (p.out_identity) {
             // tour identity: XOR of the edge identities, then the canonical
@@ -803,9 +811,14 @@ __device__ __forceinline__ void bounds(ColonyState& s, uint32_t width, double re
 }
 
 __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) {
-    const uint32_t colony = blockIdx.x, n = p.n, tid = threadIdx.x;
+    // n: this colony's cities, ns: stride of the per-city arrays (they
+    // differ only in a mixed-size batch; so do ants and the ant stride)
+    const uint32_t colony = blockIdx.x, ns = p.n, tid = threadIdx.x;
+    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
+    const uint32_t ants = p.shapes ? p.shapes[colony].ants : p.ants;
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
     ColonyState* st = p.state + colony;
+    if (p.shapes && st->batch >= p.shapes[colony].batches) return;   // colony done
     const faco_ant_info* info = p.info + static_cast<size_t>(colony) * p.ants;
     __shared__ uint64_t best_key[256];
     __shared__ uint64_t work_c[256], work_l[256];
@@ -813,7 +826,7 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
     extern __shared__ uint32_t s_links[];
 
     uint64_t best = ~0ull, wc = 0, wl = 0;
-    for (uint32_t a = tid; a < p.ants; a += blockDim.x) {
+    for (uint32_t a = tid; a < ants; a += blockDim.x) {
         // final costs are positive and < 2^31: (cost, ant) orders ties by ant
         const uint64_t k = (static_cast<uint64_t>(info[a].final_cost) << 32) | a;
         best = k < best ? k : best;
@@ -849,11 +862,11 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
     }
     __syncthreads();
     const uint32_t improve = s.pad;
-    const uint32_t* best_tour = p.tours + (static_cast<size_t>(colony) * p.ants + s.iteration_best) * n;
-    uint32_t* epoch = p.epoch + static_cast<size_t>(colony) * n;
-    uint32_t* global = p.global + static_cast<size_t>(colony) * n;
-    uint32_t* parent = p.parent + static_cast<size_t>(colony) * n;
-    uint32_t* links = p.links + static_cast<size_t>(colony) * n;
+    const uint32_t* best_tour = p.tours + (static_cast<size_t>(colony) * p.ants + s.iteration_best) * ns;
+    uint32_t* epoch = p.epoch + static_cast<size_t>(colony) * ns;
+    uint32_t* global = p.global + static_cast<size_t>(colony) * ns;
+    uint32_t* parent = p.parent + static_cast<size_t>(colony) * ns;
+    uint32_t* links = p.links + static_cast<size_t>(colony) * ns;
     if (improve & 1u) for (uint32_t i = tid; i < n; i += blockDim.x) epoch[i] = best_tour[i];
     if (improve & 2u) for (uint32_t i = tid; i < n; i += blockDim.x) global[i] = best_tour[i];
     __syncthreads();
@@ -870,9 +883,9 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
     // Evaporate, clamp below, deposit on parent edges, clamp above, refresh
     // products (reference cuda/colony_state.cuh:210-229).
     const uint32_t cells = n * p.pw;
-    double* trails = p.trails + static_cast<size_t>(colony) * cells;
-    double* products = p.products + static_cast<size_t>(colony) * cells;
-    const double* eta = p.eta + static_cast<size_t>(geom) * cells;
+    double* trails = p.trails + static_cast<size_t>(colony) * ns * p.pw;
+    double* products = p.products + static_cast<size_t>(colony) * ns * p.pw;
+    const double* eta = p.eta + static_cast<size_t>(geom) * ns * p.pw;
     const uint16_t* nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
     const double minimum = s.minimum, maximum = s.maximum, retention = p.retention;
     const double increment = 1.0 / static_cast<double>(s.parent_cost);
@@ -904,9 +917,10 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
 // bounds from the initial cost, every trail at the upper bound.
 
 __global__ void __launch_bounds__(256) faco_colony_reset(const ResetParams p) {
-    const uint32_t colony = blockIdx.x, n = p.n, tid = threadIdx.x;
+    const uint32_t colony = blockIdx.x, ns = p.n, tid = threadIdx.x;
+    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
-    const uint32_t* init = p.initial + static_cast<size_t>(geom) * n;
+    const uint32_t* init = p.initial + static_cast<size_t>(geom) * ns;
     __shared__ ColonyState s;
     if (tid == 0) {
         ColonyState z = {};
@@ -917,7 +931,7 @@ __global__ void __launch_bounds__(256) faco_colony_reset(const ResetParams p) {
         s = z;
     }
     __syncthreads();
-    const size_t base = static_cast<size_t>(colony) * n;
+    const size_t base = static_cast<size_t>(colony) * ns;
     for (uint32_t i = tid; i < n; i += blockDim.x) {
         const uint32_t v = init[i];
         p.parent[base + i] = p.epoch[base + i] = p.global[base + i] = v;
@@ -928,10 +942,10 @@ __global__ void __launch_bounds__(256) faco_colony_reset(const ResetParams p) {
     __syncthreads();
     if (tid == 0) p.state[colony] = s;
     const uint32_t cells = n * p.pw;
-    const double* eta = p.eta + static_cast<size_t>(geom) * cells;
+    const double* eta = p.eta + static_cast<size_t>(geom) * ns * p.pw;
     for (uint32_t i = tid; i < cells; i += blockDim.x) {
-        p.trails[static_cast<size_t>(colony) * cells + i] = s.maximum;
-        p.products[static_cast<size_t>(colony) * cells + i] = s.maximum * eta[i];
+        p.trails[static_cast<size_t>(colony) * ns * p.pw + i] = s.maximum;
+        p.products[static_cast<size_t>(colony) * ns * p.pw + i] = s.maximum * eta[i];
     }
 }
 

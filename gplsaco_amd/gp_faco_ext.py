@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _native
 from ._native import (BaselinePolicy, ControlSpec, ControlState, ControlTrace, Controller, Program, Result, Settings,
-                      check, lib, ptr)
+                      TuneEntry, check, lib, ptr)
 
 numeric_backend = "euc2d_int32"
 campaign_interface_version = 1
@@ -372,9 +372,13 @@ class FacoBatchEngine:
         check(lib().faco_engine_set_local_search(self._handle, _local_search_mode(mode)))
 
     # Learned edge prior (this engine's extension; gplsaco_amd.prior).
+    def _instance_n(self, instance_key) -> int:
+        """City count of a registered instance (the engine's dimension)."""
+        return self.dimension
+
     def edge_features(self, instance_key) -> np.ndarray:
         """Features [n * primary_width, 8] of a registered instance's candidate edges."""
-        out = np.zeros((self.dimension * self.settings.primary_width, 8), dtype=np.float32)
+        out = np.zeros((self._instance_n(instance_key) * self.settings.primary_width, 8), dtype=np.float32)
         check(lib().faco_engine_edge_features(self._handle, _unsigned(instance_key, "instance_key"), out.reshape(-1)))
         return out
 
@@ -385,17 +389,35 @@ class FacoBatchEngine:
             check(lib().faco_engine_set_edge_prior(self._handle, key, None))
             return
         m = np.ascontiguousarray(multipliers, dtype=np.float32).reshape(-1)
-        if m.size != self.dimension * self.settings.primary_width:
+        if m.size != self._instance_n(key) * self.settings.primary_width:
             raise ValueError("multipliers must be [n, primary_width]")
         check(lib().faco_engine_set_edge_prior(self._handle, key, m.ctypes.data_as(ctypes.c_void_p)))
 
     def apply_edge_prior(self, prior, instance_key) -> np.ndarray:
         """Score the instance's candidate edges with `prior` (an EdgePrior) and use
         sigmoid(logit) as their heuristic multipliers; returns them."""
-        out = np.zeros((self.dimension, self.settings.primary_width), dtype=np.float32)
+        out = np.zeros((self._instance_n(instance_key), self.settings.primary_width), dtype=np.float32)
         check(lib().faco_engine_apply_edge_prior(self._handle, prior._handle, _unsigned(instance_key, "instance_key"),
                                                  out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+    def autotune(self, instance_keys, seeds, iterations: int = 4) -> dict:
+        """Time every ant-kernel launch shape that fits (ants per workgroup,
+        neighbour rows in LDS or not) for `iterations` iterations of plain
+        FACO on these tasks and keep the fastest. Results of evaluations do
+        not depend on the shape; only their speed does."""
+        keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
+        if keys.size != seeds.size:
+            raise ValueError("autotune needs complete task arrays")
+        table = (TuneEntry * 32)()
+        count = ctypes.c_uint32()
+        check(lib().faco_engine_autotune(self._handle, keys, seeds, keys.size, _unsigned(iterations, "iterations", 32),
+                                         table, 32, ctypes.byref(count)))
+        rows = [{"waves": t.waves, "neighbour_rows_in_lds": bool(t.nbr_in_lds),
+                 "blocks_per_colony": t.blocks_per_colony, "ms_per_iteration": t.ms_per_iteration}
+                for t in table[:min(count.value, 32)]]
+        best = min(rows, key=lambda r: r["ms_per_iteration"])
+        return {"table": rows, "best": best, "launch_shape": self.launch_shape(keys.size)}
 
     def set_waves_per_block(self, waves: int, blocks_per_colony: int = 0) -> None:
         """Force the ant kernel's shape: waves (ants in flight) per workgroup,
@@ -431,7 +453,7 @@ class FacoBatchEngine:
             raise ValueError("frozen preparation charges of one instance can not change")
 
     def problem_data(self, instance_key) -> dict:
-        n, s = self.dimension, self.settings
+        n, s = self._instance_n(instance_key), self.settings
         primary = np.zeros(n * s.primary_width, dtype=np.uint32)
         backup = np.zeros(max(1, n * s.backup_width), dtype=np.uint32)
         ls = np.zeros(n * s.ls_width, dtype=np.uint32)
@@ -691,6 +713,54 @@ class FacoBatchEngine:
                 item["factorial_policy"] = factorial_output(variant, b)
             outputs.append(item)
         return outputs
+
+
+class MixedFacoBatchEngine(FacoBatchEngine):
+    """Instances of different sizes (3 <= n <= max_dimension, equal candidate
+    widths) in one engine: every colony runs with its own n and ants (the
+    settings' per-size default when ants is 0) in the same kernel launches,
+    largest work first. No counterpart in the reference, which needs one n
+    per engine (cuda/batch_engine.cu:78-99). The evaluation limit must be a
+    whole number of every colony's ant batches; tours come back at each
+    colony's own n. Controlled evaluations need FacoBatchEngine."""
+
+    def __init__(self, max_dimension, colonies, settings):
+        if type(max_dimension) is not int or type(colonies) is not int:
+            raise ValueError("max_dimension/colonies must be integers")
+        if not isinstance(settings, FixedFacoSettings):
+            raise TypeError("settings must be FixedFacoSettings")
+        handle = ctypes.c_void_p()
+        check(lib().faco_engine_create_mixed(max_dimension, colonies, ctypes.byref(settings._native()),
+                                             ctypes.byref(handle)))
+        self._handle = handle
+        self.dimension, self.colonies, self.population_size = max_dimension, colonies, 1
+        s = Settings()
+        check(lib().faco_engine_settings(self._handle, ctypes.byref(s)))
+        self.settings = FixedFacoSettings._from_native(s)
+        self._charges = {}
+        self._sizes = {}
+
+    def _instance_n(self, instance_key) -> int:
+        key = _unsigned(instance_key, "instance_key")
+        if key not in self._sizes:
+            raise ValueError("task instance has not been registered")
+        return self._sizes[key]
+
+    def register_problem(self, instance_key, coordinates) -> dict:
+        out = super().register_problem(instance_key, coordinates)
+        self._sizes[_unsigned(instance_key, "instance_key")] = _coordinates(coordinates).shape[0]
+        return out
+
+    def evaluate_faco_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, mne=8,
+                                  preparation_mode="cached", **kwargs) -> dict:
+        out = super().evaluate_faco_evaluations(instance_keys, seeds, evaluation_limit_per_colony, mne,
+                                                preparation_mode, **kwargs)
+        for key, item in zip(_keys(instance_keys, "instance_keys"), out["items"]):
+            item["tour"] = item["tour"][: self._instance_n(int(key))]
+        return out
+
+    def evaluate_control_raw(self, *args, **kwargs):
+        raise ValueError("controlled evaluations need a fixed-size FacoBatchEngine")
 
 
 def _controller_stages(scores, mask, action) -> list:

@@ -195,6 +195,17 @@ int faco_prepare_common_initial(const double* xy, uint32_t n, const faco_setting
  * constraint_mode: 0 unrestricted. population_size in [1, 128]. */
 int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_settings* settings,
                        uint32_t constraint_mode, uint32_t population_size, faco_engine** out);
+/* Mixed-size batch engine: instances of any n up to max_dimension in one
+ * panel (all with the same candidate widths), every colony run with its own
+ * n and ants (the settings' per-size default when ants = 0) in the same
+ * launches; workgroups are dispatched largest work first. No counterpart in
+ * the reference, which needs one n per engine (cuda/batch_engine.cu:78-99;
+ * python/gp_faco/worker.py:41-53). faco_engine_evaluate_faco then needs an
+ * evaluation limit that is a whole number of every colony's ant batches;
+ * each row of `tours` holds the colony's n cities followed by UINT32_MAX.
+ * Controlled evaluations need a fixed-size engine. */
+int faco_engine_create_mixed(uint32_t max_dimension, uint32_t colonies, const faco_settings* settings,
+                             faco_engine** out);
 void faco_engine_destroy(faco_engine* engine);
 /* The normalised settings the engine runs with. */
 int faco_engine_settings(const faco_engine* engine, faco_settings* out);
@@ -213,6 +224,27 @@ int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint3
  * of LDS) and `blocks_per_colony` workgroups per colony (must divide the
  * ants; 0 = automatic). waves = 0 restores the automatic shape. */
 int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony);
+
+/* One timed launch shape of faco_engine_autotune. */
+typedef struct faco_tune_entry {
+    uint32_t waves;                 /* ants in flight per workgroup */
+    uint32_t nbr_in_lds;            /* neighbour rows staged in LDS */
+    uint32_t blocks_per_colony;     /* workgroups per colony chosen for the batch */
+    uint32_t pad;
+    double ms_per_iteration;        /* ant kernel + colony update, HIP events */
+} faco_tune_entry;
+
+/* Autotuner of the ant kernel's launch shape (no counterpart in the
+ * reference, where warps per block is a compile-time choice,
+ * CMakeLists.txt:13-19). Times `iterations` eager iterations of plain FACO
+ * on the given tasks for every shape that fits the LDS budget (16, 12, 10,
+ * 8, 6, 4, 2 or 1 ants per workgroup, neighbour rows in LDS or in global
+ * memory), keeps the fastest as with faco_engine_set_waves and writes the
+ * timings to `table` (up to `capacity` rows; `count` receives the number of
+ * shapes timed). Later evaluations give identical results for every shape;
+ * only their time changes. */
+int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks,
+                         uint32_t iterations, faco_tune_entry* table, uint32_t capacity, uint32_t* count);
 
 /* Local search of the ants: 0 = the reference's checklist 2-opt (default),
  * 1 = checklist 2-opt + Or-opt. Or-opt has no counterpart in the reference:
