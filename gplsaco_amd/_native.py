@@ -195,7 +195,7 @@ SIGNATURES = {
     "faco_engine_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Settings), ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.POINTER(vp)]),
     "faco_engine_create_mixed": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Settings),
-                                                ctypes.POINTER(vp)]),
+                                                ctypes.c_uint32, ctypes.POINTER(vp)]),
     "faco_engine_autotune": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(TuneEntry),
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     "faco_engine_destroy": (None, [vp]),

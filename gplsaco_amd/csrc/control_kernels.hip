@@ -177,11 +177,14 @@ __device__ __forceinline__ bool id_less(const uint64_t* a, const uint64_t* b) {
 // tour}, tracked costs = initial cost, active = epoch identity = its
 // identity, feedback cleared.
 __global__ void __launch_bounds__(256) faco_control_init(const ControlParams p) {
-    const uint32_t colony = blockIdx.x, n = p.n, tid = threadIdx.x;
+    // n: this colony's cities; ns: stride of the per-city arrays (they
+    // differ only in a mixed-size batch)
+    const uint32_t colony = blockIdx.x, ns = p.n, tid = threadIdx.x;
+    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
     const uint32_t geom = p.geometry[colony];
-    const uint32_t* init = p.initial + static_cast<size_t>(geom) * n;
-    uint32_t* tours = p.archive_tours + static_cast<size_t>(colony) * 2 * kArchive * n;
-    uint32_t* links = p.archive_links + static_cast<size_t>(colony) * 2 * kArchive * n;
+    const uint32_t* init = p.initial + static_cast<size_t>(geom) * ns;
+    uint32_t* tours = p.archive_tours + static_cast<size_t>(colony) * 2 * kArchive * ns;
+    uint32_t* links = p.archive_links + static_cast<size_t>(colony) * 2 * kArchive * ns;
     __shared__ uint64_t red[2][256];
     __shared__ uint32_t s_zero;
     uint64_t f = 0, g = 0;
@@ -220,16 +223,19 @@ __global__ void __launch_bounds__(256) faco_control_init(const ControlParams p) 
 // build_control_features + scoring + apply_control_action for one colony
 // (cuda/control_state.cuh:200-364, cuda/gp_score.cuh, baseline_policy.cuh).
 __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p) {
-    const uint32_t colony = blockIdx.x, colonies = gridDim.x, n = p.n, tid = threadIdx.x;
+    const uint32_t colony = blockIdx.x, colonies = gridDim.x, ns = p.n, tid = threadIdx.x;
+    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
+    const uint32_t ants = p.shapes ? p.shapes[colony].ants : p.ants;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     const uint32_t geom = p.geometry[colony], pw = p.pw, K = p.K;
     ControlState* ctl = p.control + colony;
     ColonyState* st = p.state + colony;
+    if (p.shapes && st->batch >= p.shapes[colony].batches) return;   // colony done
     const uint16_t* nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
-    const uint32_t* coords = p.coords + static_cast<size_t>(geom) * n;
-    uint32_t* parent = p.parent + static_cast<size_t>(colony) * n;
-    uint32_t* plinks = p.links + static_cast<size_t>(colony) * n;
-    const double* trails = p.trails + static_cast<size_t>(colony) * n * pw;
+    const uint32_t* coords = p.coords + static_cast<size_t>(geom) * ns;
+    uint32_t* parent = p.parent + static_cast<size_t>(colony) * ns;
+    uint32_t* plinks = p.links + static_cast<size_t>(colony) * ns;
+    const double* trails = p.trails + static_cast<size_t>(colony) * ns * pw;
 
     __shared__ ControlState c;
     __shared__ ColonyState s;
@@ -242,8 +248,8 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
     __shared__ float s_scores[32];
     if (tid == 0) { c = *ctl; s = *st; }
     __syncthreads();
-    const uint32_t* arch_tours = p.archive_tours + (static_cast<size_t>(colony) * 2 + c.parity) * kArchive * n;
-    const uint32_t* arch_links = p.archive_links + (static_cast<size_t>(colony) * 2 + c.parity) * kArchive * n;
+    const uint32_t* arch_tours = p.archive_tours + (static_cast<size_t>(colony) * 2 + c.parity) * kArchive * ns;
+    const uint32_t* arch_links = p.archive_links + (static_cast<size_t>(colony) * 2 + c.parity) * kArchive * ns;
     const uint32_t count = n < kRegion ? n : kRegion;
 
     // (1) archive alternative: the entry (other than the active tour) that
@@ -254,11 +260,11 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
         const uint32_t sample_count = n < kSamples ? n : kSamples;
         uint32_t alt = kArchive, largest = 0;
         for (uint32_t i = 0; i < c.archive_size; ++i) {
-            const uint32_t* entry = arch_tours + static_cast<size_t>(i) * n;
+            const uint32_t* entry = arch_tours + static_cast<size_t>(i) * ns;
             if (same_id(c.archive_id[i], c.active_id) &&
                 wave_canonical_compare(entry, c.archive_zero[i], parent, s.parent_zero, n, lane) == 0)
                 continue;
-            const uint32_t* elinks = arch_links + static_cast<size_t>(i) * n;
+            const uint32_t* elinks = arch_links + static_cast<size_t>(i) * ns;
             uint32_t miss = 0;
             if (lane < sample_count) {
                 const uint32_t v = samples[lane];
@@ -273,7 +279,7 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
                 if (c.archive_cost[i] == c.archive_cost[alt]) {
                     if (id_less(c.archive_id[i], c.archive_id[alt])) better_tie = true;
                     else if (same_id(c.archive_id[i], c.archive_id[alt]))
-                        better_tie = wave_canonical_compare(entry, c.archive_zero[i], arch_tours + static_cast<size_t>(alt) * n,
+                        better_tie = wave_canonical_compare(entry, c.archive_zero[i], arch_tours + static_cast<size_t>(alt) * ns,
                                                             c.archive_zero[alt], n, lane) < 0;
                 }
             }
@@ -297,7 +303,7 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
         const uint32_t mode = tid / 4, region = tid % 4;
         for (uint32_t i = 0; i < kRegion; ++i) s_nodes[tid][i] = 0;
         if (mode == 0 || alt < kArchive) {
-            const uint32_t* order = mode == 0 ? parent : arch_tours + static_cast<size_t>(alt) * n;
+            const uint32_t* order = mode == 0 ? parent : arch_tours + static_cast<size_t>(alt) * ns;
             build_region(order, nbr, K, pw, n, p.keys[colony], s.batch, mode, region, s_nodes[tid], s_sorted[tid]);
         }
     }
@@ -311,9 +317,9 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
         v[0] = v[1] = v[2] = v[3] = v[4] = 0;
         if ((mode == 0 || alt < kArchive) && l < count) {
             const uint32_t node = s_nodes[g][l];
-            const uint32_t link = mode == 0 ? plinks[node] : arch_links[static_cast<size_t>(alt) * n + node];
+            const uint32_t link = mode == 0 ? plinks[node] : arch_links[static_cast<size_t>(alt) * ns + node];
             const uint32_t prev = link & 0xffffu, next = link >> 16;
-            const double scale_raw = p.local_scale[static_cast<size_t>(geom) * n + node];
+            const double scale_raw = p.local_scale[static_cast<size_t>(geom) * ns + node];
             const double eps = p.epsilon[geom];
             const double scale = scale_raw < eps ? eps : scale_raw;
             const double near = 0.5 * static_cast<double>(euc2d(coords[node], coords[prev])) +
@@ -323,7 +329,7 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
             for (uint32_t k = 0; k < 2; ++k) {
                 const uint32_t w = neighbours[k];
                 for (uint32_t j = 0; j < c.archive_size; ++j) {
-                    const uint32_t al = arch_links[static_cast<size_t>(j) * n + node];
+                    const uint32_t al = arch_links[static_cast<size_t>(j) * ns + node];
                     v[1] += ((al & 0xffffu) == w || (al >> 16) == w) ? 0.0 : 1.0;
                 }
                 double trail = s.default_trail;
@@ -357,7 +363,7 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
     // (4) the 12 features of the 32 actions
     if (tid < 32) {
         const uint32_t a = tid, mode = a / 16, region = (a / 4) % 4, level = a % 4;
-        const double progress = static_cast<double>(static_cast<uint64_t>(s.batch) * p.ants) /
+        const double progress = static_cast<double>(static_cast<uint64_t>(s.batch) * ants) /
                                 static_cast<double>(p.evaluation_limit);
         const float* r = s_regional[mode * 4 + region];
         const float values[12] = {static_cast<float>(unit_clip(progress)),
@@ -440,17 +446,17 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
         if (c.restarts != ~0ull) ctl->restarts = c.restarts + 1;
     }
     __syncthreads();
-    uint32_t* epoch = p.epoch + static_cast<size_t>(colony) * n;
-    const uint32_t* src = arch_tours + static_cast<size_t>(alt) * n;
-    const uint32_t* src_links = arch_links + static_cast<size_t>(alt) * n;
+    uint32_t* epoch = p.epoch + static_cast<size_t>(colony) * ns;
+    const uint32_t* src = arch_tours + static_cast<size_t>(alt) * ns;
+    const uint32_t* src_links = arch_links + static_cast<size_t>(alt) * ns;
     for (uint32_t i = tid; i < n; i += blockDim.x) {
         parent[i] = epoch[i] = src[i];
         plinks[i] = src_links[i];
     }
     const uint32_t cells = n * pw;
-    double* tr = p.trails + static_cast<size_t>(colony) * cells;
-    double* pr = p.products + static_cast<size_t>(colony) * cells;
-    const double* eta = p.eta + static_cast<size_t>(geom) * cells;
+    double* tr = p.trails + static_cast<size_t>(colony) * ns * pw;
+    double* pr = p.products + static_cast<size_t>(colony) * ns * pw;
+    const double* eta = p.eta + static_cast<size_t>(geom) * ns * pw;
     const double maximum = s.maximum;
     for (uint32_t i = tid; i < cells; i += blockDim.x) {
         tr[i] = maximum;
@@ -465,12 +471,18 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
 // Candidates: archive entries 0..3 then the colony's ants.
 __global__ void __launch_bounds__(256) faco_control_update(const ControlParams p) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    const uint32_t colony = blockIdx.x, n = p.n, A = p.ants, tid = threadIdx.x;
+    // n, A: this colony's cities and ants; ns and p.ants: the strides
+    const uint32_t colony = blockIdx.x, ns = p.n, tid = threadIdx.x;
+    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
+    const uint32_t A = p.shapes ? p.shapes[colony].ants : p.ants;
     const uint32_t lane = tid & 63u, wave = tid >> 6, waves = blockDim.x >> 6;
     const uint32_t M = kArchive + A;
     ControlState* ctl = p.control + colony;
     const ColonyState* st = p.state + colony;
-    const size_t ant_base = static_cast<size_t>(colony) * A;
+    // a colony past its iteration budget was left idle by this iteration's
+    // colony update (which marks it so)
+    if (p.shapes && st->pad == kColonyIdle) return;
+    const size_t ant_base = static_cast<size_t>(colony) * p.ants;
     __shared__ ControlState c;
     __shared__ ColonyState s;
     __shared__ uint32_t s_chosen[kArchive], s_count, s_collision, s_pairs;
@@ -484,12 +496,12 @@ __global__ void __launch_bounds__(256) faco_control_update(const ControlParams p
     if (tid == 0) { c = *ctl; s = *st; s_collision = 0; s_pairs = 0; }
     __syncthreads();
     const uint32_t parity = c.parity;
-    const uint32_t* old_tours = p.archive_tours + (static_cast<size_t>(colony) * 2 + parity) * kArchive * n;
-    uint32_t* new_tours = p.archive_tours + (static_cast<size_t>(colony) * 2 + (parity ^ 1u)) * kArchive * n;
-    uint32_t* new_links = p.archive_links + (static_cast<size_t>(colony) * 2 + (parity ^ 1u)) * kArchive * n;
-    const uint32_t* ant_tours = p.tours + ant_base * n;
+    const uint32_t* old_tours = p.archive_tours + (static_cast<size_t>(colony) * 2 + parity) * kArchive * ns;
+    uint32_t* new_tours = p.archive_tours + (static_cast<size_t>(colony) * 2 + (parity ^ 1u)) * kArchive * ns;
+    uint32_t* new_links = p.archive_links + (static_cast<size_t>(colony) * 2 + (parity ^ 1u)) * kArchive * ns;
+    const uint32_t* ant_tours = p.tours + ant_base * ns;
     auto tour_of = [&](uint32_t k) -> const uint32_t* {
-        return k < kArchive ? old_tours + static_cast<size_t>(k) * n : ant_tours + static_cast<size_t>(k - kArchive) * n;
+        return k < kArchive ? old_tours + static_cast<size_t>(k) * ns : ant_tours + static_cast<size_t>(k - kArchive) * ns;
     };
     const double band = static_cast<double>(s.global_cost) * (1 + kBand);
     const bool improved = s.global_cost < c.tracked_global;
@@ -631,8 +643,8 @@ __global__ void __launch_bounds__(256) faco_control_update(const ControlParams p
     const uint32_t count = s_count;
     for (uint32_t slot = 0; slot < count; ++slot) {
         const uint32_t* src = tour_of(s_chosen[slot]);
-        uint32_t* dst = new_tours + static_cast<size_t>(slot) * n;
-        uint32_t* dl = new_links + static_cast<size_t>(slot) * n;
+        uint32_t* dst = new_tours + static_cast<size_t>(slot) * ns;
+        uint32_t* dl = new_links + static_cast<size_t>(slot) * ns;
         for (uint32_t i = tid; i < n; i += blockDim.x) {
             const uint32_t v = src[i];
             const uint32_t pred = src[i == 0 ? n - 1 : i - 1], succ = src[i + 1 == n ? 0 : i + 1];

@@ -606,10 +606,10 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     require(lock.owns_lock(), "one evaluation per engine at a time");
     require(current_device() == e.device, "engine must be used on the device it was created on");
     require(spec.mode >= FACO_CONTROL_PROGRAM && spec.mode <= FACO_CONTROL_FACTORIAL, "unknown control mode");
-    require(!e.mixed, "controlled evaluations need a fixed-size engine");
     require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
-    const uint32_t ants = e.config.ants;
-    require(limit % ants == 0 && limit / ants <= UINT32_MAX, "evaluation limit must be a whole number of ant batches");
+    const uint32_t ants = e.config.ants;   // the ant stride in a mixed-size batch
+    if (!e.mixed)
+        require(limit % ants == 0 && limit / ants <= UINT32_MAX, "evaluation limit must be a whole number of ant batches");
     const bool scored = spec.mode != FACO_CONTROL_BASELINE;
     const uint32_t deciders = scored ? spec.deciders : 1u;
     require(deciders >= 1 && deciders <= e.population, "decider count must be in 1..population_size");
@@ -637,23 +637,27 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     const uint32_t C = panel * deciders;
     require(C <= e.colonies_capacity, "colonies exceed the engine's capacity");
     for (uint32_t t = 0; t < panel; ++t) (void)e.problem(panel_keys[t]);
+    std::vector<uint64_t> keys(C), inst(C);
+    for (uint32_t c = 0; c < C; ++c) {
+        inst[c] = panel_keys[c % panel];
+        keys[c] = colony_key(panel_keys[c % panel], panel_seeds[c % panel]);
+    }
+    // a mixed-size batch runs until its longest colony is done; the others
+    // stop at their own budget
+    struct Inactive { Engine& e; ~Inactive() { e.shapes_active = false; } } inactive{e};
+    const uint64_t batches = e.mixed ? e.prepare_shapes(inst.data(), C, limit) : limit / ants;
     std::vector<uint32_t> traced;
     if (trace && trace->count) {
         require(trace->batches != nullptr, "trace batches missing");
         traced.assign(trace->batches, trace->batches + trace->count);
         for (size_t i = 0; i < traced.size(); ++i)
-            require(traced[i] < limit / ants && (i == 0 || traced[i] > traced[i - 1]),
+            require(traced[i] < batches && (i == 0 || traced[i] > traced[i - 1]),
                     "trace batches must be ascending and inside the budget");
     }
     e.ensure_control();
     auto& b = *e.ctl;
     // LDS of the archive update: per candidate cost, identity, start, leader, pair
     require(36ull * (kArchive + ants) <= max_lds_per_block(), "too many ants for the archive update");
-    std::vector<uint64_t> keys(C), inst(C);
-    for (uint32_t c = 0; c < C; ++c) {
-        inst[c] = panel_keys[c % panel];
-        keys[c] = colony_key(panel_keys[c % panel], panel_seeds[c % panel]);
-    }
     e.upload_geometry(inst.data(), C);
     e.keys.upload(keys.data(), C, e.stream);
     if (spec.mode == FACO_CONTROL_PROGRAM || spec.mode == FACO_CONTROL_FACTORIAL)
@@ -679,6 +683,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     cp.ant_identity = b.ant_identity.data(); cp.ant_zero = b.ant_zero.data();
     cp.features = b.features.data(); cp.scores = b.scores.data(); cp.masks = b.masks.data();
     cp.actions = b.actions.data(); cp.alternatives = b.alternatives.data(); cp.regions = b.regions.data();
+    cp.shapes = e.shapes_active ? e.shapes.data() : nullptr;
 
     AntParams ap = e.ant_params(C);
     ap.control = b.control.data();
@@ -695,7 +700,6 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     e.reset(C, 2u);
     checked(launch_control_init(cp, C, e.stream), "control init launch");
     checked(hipStreamSynchronize(e.stream), "initialisation");
-    const uint64_t batches = limit / ants;
     const auto search_started = Clock::now();
     checked(hipEventRecord(e.ev_start, e.stream), "event");
     if (!traced.empty()) {
@@ -750,6 +754,9 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     b.control.download(cs.data(), C, e.stream);
     if (tours) e.global.download(tours, static_cast<size_t>(C) * e.n, e.stream);
     checked(hipStreamSynchronize(e.stream), "readback");
+    if (tours && e.mixed)
+        for (uint32_t c = 0; c < C; ++c)
+            for (uint32_t i = e.problem(inst[c]).n; i < e.n; ++i) tours[static_cast<size_t>(c) * e.n + i] = UINT32_MAX;
     uint64_t steps = 0, evals = 0;
     for (uint32_t c = 0; c < C; ++c) {
         if (costs) costs[c] = st[c].global_cost;
@@ -762,8 +769,8 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         }
         steps += st[c].work_construction; evals += st[c].work_ls;
     }
-    fill_result(result, e, C, limit, batches, batches * e.config.ants, steps, evals, seconds_since(started), search,
-                gpu_ms);
+    fill_result(result, e, C, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
+                seconds_since(started), search, gpu_ms);
 }
 
 }  // namespace
@@ -821,11 +828,11 @@ int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_setting
 }
 
 int faco_engine_create_mixed(uint32_t max_dimension, uint32_t colonies, const faco_settings* settings,
-                             faco_engine** out) {
+                             uint32_t population_size, faco_engine** out) {
     return guarded([&] {
         require(out && settings, "null argument");
         auto engine = std::make_unique<faco_engine>();
-        engine->impl = std::make_unique<Engine>(max_dimension, colonies, *settings, 1u, true);
+        engine->impl = std::make_unique<Engine>(max_dimension, colonies, *settings, population_size, true);
         *out = engine.release();
     });
 }

@@ -191,6 +191,9 @@ constexpr uint32_t kPhaseSlots = 10;
 // parameter block. In a mixed batch the parameter block's n and ants are the
 // strides of every per-city and per-ant array (the batch maxima).
 struct ColonyShape { uint32_t n, ants, batches, pad; };
+// ColonyState::pad after a colony update that left a finished colony idle
+// (0 after every update that ran).
+constexpr uint32_t kColonyIdle = 1;
 
 struct AntParams {
     uint32_t nbr_in_lds;                // neighbour rows staged in LDS (else read from HBM)
@@ -312,6 +315,7 @@ struct ControlParams {
     int32_t* actions;                   // [C] or null
     uint32_t* alternatives;             // [C] or null
     uint32_t* regions;                  // [C][2][4][16] or null
+    const ColonyShape* shapes;          // [C] or null (fixed size); n and ants above are then strides
 };
 hipError_t launch_control_init(const ControlParams& p, uint32_t colonies, hipStream_t stream);
 hipError_t launch_control_decide(const ControlParams& p, uint32_t colonies, hipStream_t stream);

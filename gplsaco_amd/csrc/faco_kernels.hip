@@ -818,7 +818,10 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
     const uint32_t ants = p.shapes ? p.shapes[colony].ants : p.ants;
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
     ColonyState* st = p.state + colony;
-    if (p.shapes && st->batch >= p.shapes[colony].batches) return;   // colony done
+    if (p.shapes && st->batch >= p.shapes[colony].batches) {   // colony done: mark it idle
+        if (tid == 0) st->pad = kColonyIdle;
+        return;
+    }
     const faco_ant_info* info = p.info + static_cast<size_t>(colony) * p.ants;
     __shared__ uint64_t best_key[256];
     __shared__ uint64_t work_c[256], work_l[256];

@@ -719,21 +719,23 @@ class MixedFacoBatchEngine(FacoBatchEngine):
     """Instances of different sizes (3 <= n <= max_dimension, equal candidate
     widths) in one engine: every colony runs with its own n and ants (the
     settings' per-size default when ants is 0) in the same kernel launches,
-    largest work first. No counterpart in the reference, which needs one n
+    largest work first; plain FACO and the controlled (GP / baseline)
+    evaluations alike. No counterpart in the reference, which needs one n
     per engine (cuda/batch_engine.cu:78-99). The evaluation limit must be a
-    whole number of every colony's ant batches; tours come back at each
-    colony's own n. Controlled evaluations need FacoBatchEngine."""
+    whole number of every colony's ant batches (each colony stops at its own
+    budget); tours come back at each colony's own n."""
 
-    def __init__(self, max_dimension, colonies, settings):
+    def __init__(self, max_dimension, colonies, settings, population_size=1):
         if type(max_dimension) is not int or type(colonies) is not int:
             raise ValueError("max_dimension/colonies must be integers")
         if not isinstance(settings, FixedFacoSettings):
             raise TypeError("settings must be FixedFacoSettings")
+        population = _unsigned(population_size, "population_size", 32)
         handle = ctypes.c_void_p()
-        check(lib().faco_engine_create_mixed(max_dimension, colonies, ctypes.byref(settings._native()),
+        check(lib().faco_engine_create_mixed(max_dimension, colonies, ctypes.byref(settings._native()), population,
                                              ctypes.byref(handle)))
         self._handle = handle
-        self.dimension, self.colonies, self.population_size = max_dimension, colonies, 1
+        self.dimension, self.colonies, self.population_size = max_dimension, colonies, population
         s = Settings()
         check(lib().faco_engine_settings(self._handle, ctypes.byref(s)))
         self.settings = FixedFacoSettings._from_native(s)
@@ -759,8 +761,13 @@ class MixedFacoBatchEngine(FacoBatchEngine):
             item["tour"] = item["tour"][: self._instance_n(int(key))]
         return out
 
-    def evaluate_control_raw(self, *args, **kwargs):
-        raise ValueError("controlled evaluations need a fixed-size FacoBatchEngine")
+    def _control_output(self, raw, decision_iterations, kind) -> dict:
+        out = super()._control_output(raw, decision_iterations, kind)
+        for item in out["items"]:   # each row ends at its colony's own n
+            tour = item["tour"]
+            if 0xFFFFFFFF in tour:
+                item["tour"] = tour[: tour.index(0xFFFFFFFF)]
+        return out
 
 
 def _controller_stages(scores, mask, action) -> list:

@@ -200,12 +200,13 @@ int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_setting
  * n and ants (the settings' per-size default when ants = 0) in the same
  * launches; workgroups are dispatched largest work first. No counterpart in
  * the reference, which needs one n per engine (cuda/batch_engine.cu:78-99;
- * python/gp_faco/worker.py:41-53). faco_engine_evaluate_faco then needs an
- * evaluation limit that is a whole number of every colony's ant batches;
- * each row of `tours` holds the colony's n cities followed by UINT32_MAX.
- * Controlled evaluations need a fixed-size engine. */
+ * python/gp_faco/worker.py:41-53). population_size as for
+ * faco_engine_create. Evaluations then need an evaluation limit that is a
+ * whole number of every colony's ant batches; each colony stops at its own
+ * budget, and each row of `tours` holds the colony's n cities followed by
+ * UINT32_MAX. */
 int faco_engine_create_mixed(uint32_t max_dimension, uint32_t colonies, const faco_settings* settings,
-                             faco_engine** out);
+                             uint32_t population_size, faco_engine** out);
 void faco_engine_destroy(faco_engine* engine);
 /* The normalised settings the engine runs with. */
 int faco_engine_settings(const faco_engine* engine, faco_settings* out);

@@ -136,6 +136,47 @@ def test_mixed_engine_rejects_bad_inputs(ext):
     engine.register_problem(4, uniform_instance(500, seed=1))
     with pytest.raises(ValueError):   # 64 and 128 ants: 64 * 3 is not a whole number of 128-ant batches
         engine.evaluate_faco_evaluations(np.array([3, 4], np.uint64), np.array([0, 0], np.uint64), 192)
-    with pytest.raises(ValueError):
-        engine.evaluate_program_evaluations(np.array([3, 4], np.uint64), np.array([0, 0], np.uint64), 256,
-                                            {"feature_spec_id": 2, "ops": [], "constants": []})
+    from tests.test_gpu_control import RESTARTY
+    with pytest.raises(ValueError):   # the same whole-batch rule for controlled runs
+        engine.evaluate_program_evaluations(np.array([3, 4], np.uint64), np.array([0, 0], np.uint64), 192, RESTARTY)
+
+
+def test_mixed_program_population_matches_oracle(ext, oracle_mod):
+    """GP-controlled FACO on a mixed-size panel (the uneven-batch shape at a
+    small scale): colony d * panel + t equals the oracle's controlled run of
+    instance t alone under program d, with that instance's own ants and
+    iteration count (features, actions, restarts, archive, tours)."""
+    from tests.program_gen import random_program
+    from tests.test_gpu_control import RESTARTY, spec2, to_oracle_program
+    sizes = (130, 300, 220)
+    xys, keys = _panel(sizes)
+    limit = math.lcm(*[_default_ants(n) for n in sizes]) * 8
+    rng = np.random.default_rng(5)
+    programs = [RESTARTY, spec2(random_program(rng, depth=4))]
+    engine = ext.MixedFacoBatchEngine(max(sizes), len(sizes), ext.FixedFacoSettings(), population_size=2)
+    for k, xy in zip(keys, xys):
+        engine.register_problem(k, xy)
+    parsed = [ext.read_program(p) for p in programs]
+    seeds = np.array([2, 3, 4], dtype=np.uint64)
+    announce("mixed-control", sizes, limit)
+    raw = engine.evaluate_control_raw(np.array(keys, dtype=np.uint64), seeds, limit, "program", programs=parsed)
+    s = ext.FixedFacoSettings()
+    restarts = 0
+    for d, prog in enumerate(parsed):
+        for t, n in enumerate(sizes):
+            c = d * len(sizes) + t
+            ants = _default_ants(n)
+            want = oracle_mod.run_controlled(xys[t][None].astype(np.int32), [keys[t]], [int(seeds[t])],
+                                             _oracle_settings(oracle_mod, s, ants), limit, "program",
+                                             programs=[to_oracle_program(oracle_mod, prog)])
+            assert raw["tours"][c][:n].tolist() == want["tours"][0].tolist(), (d, t)
+            assert (raw["tours"][c][n:] == 0xFFFFFFFF).all()
+            assert int(raw["costs"][c]) == int(want["costs"][0]), (d, t)
+            np.testing.assert_array_equal(raw["work"][c], want["work"][0])
+            for key in ("archive_size", "stagnant_batches", "epoch_batches", "restarts"):
+                assert raw["states"][c][key] == want["states"][0][key], (d, t, key)
+            assert np.float32(raw["states"][c]["return_rate"]) == np.float32(want["states"][0]["return_rate"])
+            restarts += raw["states"][c]["restarts"]
+    assert restarts > 0, "the restart path was not exercised"
+    out = engine.evaluate_program_evaluations(np.array(keys, dtype=np.uint64), seeds, limit, programs[1])
+    assert [len(item["tour"]) for item in out["items"]] == list(sizes)
