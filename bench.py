@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -105,6 +106,51 @@ def gp_population_rate(ext, steps: int) -> dict:
             "mean_final_tour_length": float(raw["costs"].mean())}
 
 
+def uneven_batch_rate(ext, rounds: int) -> dict:
+    """The uneven batch: 64 instances with n uniform in [200, 2000] (32
+    uniform, 32 clustered) x 128 random single-tree GP programs (depth at
+    most 5, the reference's limit) x 1 seed =
+    8192 colonies in one mixed-size engine (each colony with its own n and
+    per-size ants: 64, 128 or 192), one launch per kernel per iteration,
+    HIP-graph replayed, launch shape picked by the autotuner. The budget is
+    `rounds` x 384 ant tours per colony (384 = a whole number of batches of
+    every ant count)."""
+    from gplsaco_amd.instances import _key, _uniform_words, clustered_instance, instance_key, uniform_instance
+    from tests.program_gen import random_program
+    count, individuals = 64, 128
+    words = _uniform_words(_key(3, 0, 0xC3), count)
+    sizes = [200 + int(w % np.uint64(1801)) for w in words]
+    xys, keys = [], []
+    for i, n in enumerate(sizes):
+        clustered = i % 2 == 1
+        xys.append((clustered_instance if clustered else uniform_instance)(n, seed=3, instance=i))
+        keys.append(instance_key(n, 3, i, clustered))
+    rng = np.random.default_rng(3)
+    programs = [ext.read_program({**random_program(rng, depth=5), "feature_spec_id": 2}) for _ in range(individuals)]
+    started = time.perf_counter()
+    engine = ext.MixedFacoBatchEngine(max(sizes), count, ext.FixedFacoSettings(), population_size=individuals)
+    for k, xy in zip(keys, xys):
+        engine.register_problem(k, xy)
+    prep = time.perf_counter() - started
+    task_keys = np.array(keys, dtype=np.uint64)
+    seeds = np.zeros(count, dtype=np.uint64)
+    # the autotuner times plain FACO on the panel (shape choice only)
+    tuned = engine.autotune(np.tile(task_keys, individuals), np.tile(seeds, individuals), iterations=1)
+    limit = 384 * rounds
+    raw = engine.evaluate_control_raw(task_keys, seeds, limit, "program", programs=programs, use_graph=True)
+    res = raw["result"]
+    ants = [64 * math.ceil(math.sqrt(n) / 16) if n > 256 else 64 for n in sizes]
+    return {"workload": f"GP-controlled FACO, {count} instances n in [200,2000] (half clustered) x {individuals} "
+                        f"random programs = {count * individuals} colonies, {limit} ant tours per colony",
+            "sizes": {"min": min(sizes), "max": max(sizes), "mean": float(np.mean(sizes))},
+            "ant_tours_per_s": res.total_tour_evaluations / res.search_seconds,
+            "tours": res.total_tour_evaluations, "seconds": res.search_seconds,
+            "launched_iterations": res.launched_batches, "preparation_seconds": prep,
+            "restarts": sum(st["restarts"] for st in raw["states"]),
+            "autotune": {"best": tuned["best"], "shapes_timed": len(tuned["table"])},
+            "ants_per_colony": {"64": ants.count(64), "128": ants.count(128), "192": ants.count(192)}}
+
+
 def learned_prior_legs(ext, budgets) -> dict:
     """Learned edge prior: (1) the committed trained network against plain
     FACO at equal FE on unseen TSP1000 instances (16: half uniform, half
@@ -160,6 +206,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--population-steps", type=int, default=50,
                     help="iterations of the GP-controlled population measurement (0 skips it)")
+    ap.add_argument("--uneven-rounds", type=int, default=4,
+                    help="budget of the uneven-size GP batch in units of 384 ant tours per colony (0 skips it)")
     ap.add_argument("--prior-budgets", default="100,300",
                     help="iteration budgets of the learned-prior comparison (empty skips it)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -276,6 +324,8 @@ def main() -> int:
     }
     if args.population_steps:
         line["config"]["gp_population"] = gp_population_rate(ext, args.population_steps)
+    if args.uneven_rounds:
+        line["config"]["uneven_batch"] = uneven_batch_rate(ext, args.uneven_rounds)
     if args.prior_budgets:
         line["config"]["learned_prior"] = learned_prior_legs(ext, [int(b) for b in args.prior_budgets.split(",")])
     if not args.no_cpu:

@@ -520,9 +520,11 @@ void fill_result(faco_result* r, const Engine& e, uint32_t colonies, uint64_t li
 }
 
 // Shared evaluation body for the batch engine and the fixed entry point.
+// `any_count`: the tuner may run any number of colonies up to the engine's
+// capacity (a population engine's whole shape), not just its panel.
 void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks, uint64_t batches,
               uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
-              uint64_t limit) {
+              uint64_t limit, bool any_count = false) {
     const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
     const bool events = (flags & FACO_RUN_EVENTS) != 0;
     double kernel_ms[FACO_STAGE_COUNT] = {0};
@@ -530,7 +532,8 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "one evaluation per engine at a time");
     require(current_device() == e.device, "engine must be used on the device it was created on");
-    require(tasks == e.panel, "task count must equal the engine's colonies");
+    require(tasks == e.panel || (any_count && tasks >= 1 && tasks <= e.colonies_capacity),
+            "task count must equal the engine's colonies");
     require(mne > 0, "mne_target must be positive");
     for (uint32_t c = 0; c < tasks; ++c) (void)e.problem(instance_keys[c]);
     e.upload_geometry(instance_keys, tasks);
@@ -853,7 +856,6 @@ int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, con
         require(engine && instance_keys && seeds, "null argument");
         require(iterations >= 1 && iterations <= 1000, "autotune iterations must be in 1..1000");
         Engine& e = *engine->impl;
-        require(e.population == 1, "population engines tune through a fixed panel");
         // An equal evaluation budget for every colony: a whole number of
         // iterations of each colony's ants.
         uint64_t unit = 1;
@@ -874,7 +876,7 @@ int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, con
                 e.set_waves(forced, 0);
                 faco_result r{};
                 evaluate(e, instance_keys, seeds, tasks, limit / e.config.ants, 8, FACO_RUN_EVENTS, &r, nullptr,
-                         nullptr, nullptr, limit);
+                         nullptr, nullptr, limit, true);
                 faco_tune_entry row{};
                 row.waves = e.shape.waves;
                 row.nbr_in_lds = e.shape.nbr_in_lds;

@@ -238,7 +238,9 @@ typedef struct faco_tune_entry {
 /* Autotuner of the ant kernel's launch shape (no counterpart in the
  * reference, where warps per block is a compile-time choice,
  * CMakeLists.txt:13-19). Times `iterations` eager iterations of plain FACO
- * on the given tasks for every shape that fits the LDS budget (16, 12, 10,
+ * on the given tasks (any number up to colonies x population_size, so a
+ * population engine is tuned at its full shape) for every shape that fits
+ * the LDS budget (16, 12, 10,
  * 8, 6, 4, 2 or 1 ants per workgroup, neighbour rows in LDS or in global
  * memory), keeps the fastest as with faco_engine_set_waves and writes the
  * timings to `table` (up to `capacity` rows; `count` receives the number of
