@@ -206,7 +206,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--population-steps", type=int, default=50,
                     help="iterations of the GP-controlled population measurement (0 skips it)")
-    ap.add_argument("--uneven-rounds", type=int, default=4,
+    ap.add_argument("--uneven-rounds", type=int, default=40,
                     help="budget of the uneven-size GP batch in units of 384 ant tours per colony (0 skips it)")
     ap.add_argument("--prior-budgets", default="100,300",
                     help="iteration budgets of the learned-prior comparison (empty skips it)")

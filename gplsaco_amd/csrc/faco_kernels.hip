@@ -111,8 +111,18 @@ __device__ __forceinline__ double row_prefix_sum(double v) {
     return acc;
 }
 
-template<bool NbrLds, bool Profile>
+template<bool NbrLds, bool Profile, bool Diag>
 __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
+    // Diagnostic features (explicit visits, per-ant parents, traces and
+    // logs) exist only in the Diag instantiation: the production kernel
+    // keeps none of their pointers or flags live.
+    const bool explicit_mode = Diag && p.explicit_mode;
+    const bool per_ant_parent = Diag && p.per_ant_parent;
+    uint32_t* const trace_selected = Diag ? p.trace_selected : nullptr;
+    double* const trace_uniform = Diag ? p.trace_uniform : nullptr;
+    uint32_t* const diag_construction = Diag ? p.diag_construction : nullptr;
+    uint32_t* const diag_positions = Diag ? p.diag_positions : nullptr;
+    uint32_t* const diag_checklist = Diag ? p.diag_checklist : nullptr;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     __shared__ uint32_t next_slot;
     const uint32_t lane = threadIdx.x & 63u;
@@ -126,13 +136,13 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     // the hardware dispatches workgroups in index order, so the longest
     // colonies start first) and each colony has its own n and ants; `ns` is
     // the stride of every per-city array.
-    const uint32_t group = p.explicit_mode ? 0u : blk / p.blocks_per_colony;
+    const uint32_t group = explicit_mode ? 0u : blk / p.blocks_per_colony;
     const uint32_t colony = p.order ? p.order[group] : group;
     const uint32_t ns = p.n;
     const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
     const uint32_t chunk = p.shapes ? p.shapes[colony].ants / p.blocks_per_colony : p.chunk;
     if (p.shapes && p.state[colony].batch >= p.shapes[colony].batches) return;   // colony done
-    const uint32_t chunk_base = (p.explicit_mode ? blk : blk % p.blocks_per_colony) * chunk;
+    const uint32_t chunk_base = (explicit_mode ? blk : blk % p.blocks_per_colony) * chunk;
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
 
     // Shared data: packed coordinates, the colony's parent links (the
@@ -145,7 +155,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     {
         const uint32_t* gxy = p.coords + static_cast<size_t>(geom) * ns;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_xy[i] = gxy[i];
-        if (!p.explicit_mode) {
+        if (!explicit_mode) {
             const uint32_t* glinks = p.links + static_cast<size_t>(colony) * ns;
             for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_plinks[i] = glinks[i];
         }
@@ -171,8 +181,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     uint8_t* flags = reinterpret_cast<uint8_t*>(mine + 3u * round16(2u * n));
 
     const uint32_t pw = p.pw, K = p.K, lw = p.lw;
-    const uint64_t key = p.explicit_mode ? 0ull : p.keys[colony];
-    const uint32_t batch = p.explicit_mode ? 0u : p.state[colony].batch;
+    const uint64_t key = explicit_mode ? 0ull : p.keys[colony];
+    const uint32_t batch = explicit_mode ? 0u : p.state[colony].batch;
     const double* products = p.products + static_cast<size_t>(colony) * ns * pw;
     const uint16_t* backup = p.backup + static_cast<size_t>(geom) * ns * p.bw;
     const uint64_t limit = p.ls_limit;
@@ -194,7 +204,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         uint64_t sub[5] = {0, 0, 0, 0, 0};  // select, relocate, mne/push, LS scan, LS apply (shader clocks)
         uint64_t mark = 0;
 
-        const size_t parent_base = static_cast<size_t>(p.per_ant_parent ? ant : colony) * ns;
+        const size_t parent_base = static_cast<size_t>(per_ant_parent ? ant : colony) * ns;
         const uint32_t* parent = p.parent + parent_base;
         const uint32_t* links = p.links + parent_base;   // explicit mode only; else s_plinks
         // Construction keeps the working tour as a doubly linked list
@@ -206,7 +216,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         uint32_t* lnk = reinterpret_cast<uint32_t*>(mine);
         uint32_t head = 0;
         if (linked) {
-            if (p.explicit_mode) {
+            if (explicit_mode) {
                 for (uint32_t i = lane; i < n; i += kWave) { lnk[i] = links[i]; flags[i] = 0; }
             } else {
                 for (uint32_t i = lane; i < n; i += kWave) { lnk[i] = s_plinks[i]; flags[i] = 0; }
@@ -222,12 +232,12 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         }
         wave_sync();
 
-        const uint32_t target = p.explicit_mode ? p.targets[ant] : p.state[colony].target;
+        const uint32_t target = explicit_mode ? p.targets[ant] : p.state[colony].target;
         const uint32_t* visits = p.visits + static_cast<size_t>(ant) * ns;
-        uint32_t* checklist_log = p.diag_checklist ? p.diag_checklist + static_cast<size_t>(ant) * 5u * ns : nullptr;
-        int64_t cost = p.per_ant_parent ? p.parent_cost[ant] : p.state[colony].parent_cost;
+        uint32_t* checklist_log = diag_checklist ? diag_checklist + static_cast<size_t>(ant) * 5u * ns : nullptr;
+        int64_t cost = per_ant_parent ? p.parent_cost[ant] : p.state[colony].parent_cost;
         uint32_t current;
-        if (p.explicit_mode) {
+        if (explicit_mode) {
             current = visits[0];
         } else if (p.control) {
             // controlled runs start in the region the colony's action chose
@@ -238,9 +248,9 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         }
         current = uniform(current);
         const uint32_t start = current;
-        if (p.trace_selected && lane == 0) {
-            p.trace_selected[static_cast<size_t>(ant) * ns] = current;
-            p.trace_uniform[static_cast<size_t>(ant) * ns] = 0.0;
+        if (trace_selected && lane == 0) {
+            trace_selected[static_cast<size_t>(ant) * ns] = current;
+            trace_uniform[static_cast<size_t>(ant) * ns] = 0.0;
         }
         if (lane == 0) flags[current] = kVisited;
         // array mode: position of `current`; linked mode: its links
@@ -287,7 +297,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         // links (LDS).
         uint32_t cur_cand = n, cur_link_parent = 0;
         bool cur_free = false;
-        if (!p.explicit_mode) {
+        if (!explicit_mode) {
             const uint32_t c = s_nbr[current * K + row_lane];
             const uint8_t fl = flags[c < n ? c : 0u];
             cur_cand = in_row ? c : n;
@@ -307,7 +317,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             uint32_t selected;
             double u = 0.0;
             uint32_t link;
-            if (p.explicit_mode) {
+            if (explicit_mode) {
                 selected = uniform(visits[step]);
                 link = links[current];
             } else {
@@ -319,7 +329,13 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 pf_row1 = products[static_cast<size_t>(pf_node1) * pw + row_lane];
                 if (u_base == 0 || step - u_base >= kWave) {
                     u_base = step;
-                    u_pool = step_uniform(key, batch, local_ant, step + lane);
+                    // The key passes through an empty asm so the compiler
+                    // derives the Philox round keys here, every 64 steps,
+                    // instead of keeping them live in scalar registers
+                    // across the whole construction loop.
+                    uint32_t k_lo = static_cast<uint32_t>(key), k_hi = static_cast<uint32_t>(key >> 32);
+                    asm volatile("" : "+s"(k_lo), "+s"(k_hi));
+                    u_pool = step_uniform((static_cast<uint64_t>(k_hi) << 32) | k_lo, batch, local_ant, step + lane);
                 }
                 u = read_lane(u_pool, step - u_base);
                 const uint32_t cand = cur_cand;
@@ -386,9 +402,9 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                     if (row_src == 2u) miss_row = products[static_cast<size_t>(selected) * pw + row_lane];
                 }
             }
-            if (p.trace_selected && lane == 0) {
-                p.trace_selected[static_cast<size_t>(ant) * ns + step] = selected;
-                p.trace_uniform[static_cast<size_t>(ant) * ns + step] = u;
+            if (trace_selected && lane == 0) {
+                trace_selected[static_cast<size_t>(ant) * ns + step] = selected;
+                trace_uniform[static_cast<size_t>(ant) * ns + step] = u;
             }
             if constexpr (Profile) { const uint64_t t = clock64(); sub[0] += t - mark; mark = t; }
             if (selected >= n) { exhausted = 1; break; }
@@ -544,8 +560,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             }
             wave_sync();
         }
-        if (p.diag_construction) {
-            for (uint32_t i = lane; i < n; i += kWave) p.diag_construction[static_cast<size_t>(ant) * ns + i] = tour[i];
+        if (diag_construction) {
+            for (uint32_t i = lane; i < n; i += kWave) diag_construction[static_cast<size_t>(ant) * ns + i] = tour[i];
         }
         const int64_t construction_cost = cost;
         const uint64_t tick2 = Profile ? wall_clock64() : 0;
@@ -752,8 +768,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         // ------------------------------------------------------------ write back
         uint32_t* out = p.out_tours + static_cast<size_t>(ant) * ns;
         for (uint32_t i = lane; i < n; i += kWave) __builtin_nontemporal_store(static_cast<uint32_t>(tour[i]), out + i);
-        if (p.diag_positions) {
-            for (uint32_t i = lane; i < n; i += kWave) p.diag_positions[static_cast<size_t>(ant) * ns + i] = pos[i];
+        if (diag_positions) {
+            for (uint32_t i = lane; i < n; i += kWave) diag_positions[static_cast<size_t>(ant) * ns + i] = pos[i];
         }
         if (p.out_identity) {
             // tour identity: XOR of the edge identities, then the canonical
@@ -1008,20 +1024,26 @@ int ant_search_blocks_per_cu(const AntParams& p) {
     const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     int blocks = 0;
     const hipError_t e = p.nbr_in_lds
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<true, false>, 64 * p.waves, lds)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<false, false>, 64 * p.waves, lds);
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<true, false, false>, 64 * p.waves, lds)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<false, false, false>, 64 * p.waves, lds);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
 hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t stream) {
     const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     const dim3 grid(blocks), block(64 * p.waves);
-    if (p.phase_ticks) {
-        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, true>), grid, block, lds, stream, p);
-        else hipLaunchKernelGGL((faco_ant_search<false, true>), grid, block, lds, stream, p);
+    const bool diag = p.explicit_mode || p.per_ant_parent || p.trace_selected || p.diag_construction ||
+                      p.diag_positions || p.diag_checklist;
+    if (diag) {
+        if (p.phase_ticks) return hipErrorInvalidValue;   // no clock profile of diagnostic runs
+        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, false, true>), grid, block, lds, stream, p);
+        else hipLaunchKernelGGL((faco_ant_search<false, false, true>), grid, block, lds, stream, p);
+    } else if (p.phase_ticks) {
+        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, true, false>), grid, block, lds, stream, p);
+        else hipLaunchKernelGGL((faco_ant_search<false, true, false>), grid, block, lds, stream, p);
     } else {
-        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, false>), grid, block, lds, stream, p);
-        else hipLaunchKernelGGL((faco_ant_search<false, false>), grid, block, lds, stream, p);
+        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, false, false>), grid, block, lds, stream, p);
+        else hipLaunchKernelGGL((faco_ant_search<false, false, false>), grid, block, lds, stream, p);
     }
     return hipGetLastError();
 }
