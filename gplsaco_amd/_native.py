@@ -203,6 +203,7 @@ SIGNATURES = {
     "faco_engine_launch_shape": (ctypes.c_int, [vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 5),
     "faco_engine_set_waves": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_uint32]),
     "faco_engine_set_local_search": (ctypes.c_int, [vp, ctypes.c_uint32]),
+    "faco_engine_set_roulette_margin": (ctypes.c_int, [vp, ctypes.c_double]),
     "faco_fixed_set_local_search": (ctypes.c_int, [vp, ctypes.c_uint32]),
     "faco_engine_register_problem": (ctypes.c_int, [vp, ctypes.c_uint64, f64p, ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),

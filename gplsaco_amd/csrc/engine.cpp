@@ -149,6 +149,7 @@ struct Engine {
     LaunchShape shape;
     uint32_t forced_waves = 0, forced_blocks_per_colony = 0, xcd_remap = 1;
     uint32_t local_search = 0;  // 0: checklist 2-opt, 1: 2-opt + Or-opt
+    double roulette_margin = 0x1.0p-46;  // fast roulette margin (relative to the row total)
     // Mixed-size batch: instances of any n up to `n` (which is then the
     // stride of every per-city array; config.ants is the ant stride, the
     // ants of the largest n). Each evaluation uploads every colony's shape
@@ -359,6 +360,7 @@ struct Engine {
         p.blocks_per_colony = bpc;
         p.chunk = config.ants / bpc;
         p.ls_limit = config.ls_evaluation_limit;
+        p.roulette_margin = roulette_margin;
         p.or_opt = local_search;
         p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
         p.products = products.data(); p.parent = parent.data(); p.links = links.data();
@@ -899,6 +901,15 @@ int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, con
     });
 }
 
+int faco_engine_set_roulette_margin(faco_engine* engine, double margin) {
+    return guarded([&] {
+        require(engine, "null argument");
+        require(margin >= 0x1.0p-46, "the roulette margin can only be widened (2^-46 or more)");
+        engine->impl->roulette_margin = margin;
+        engine->impl->drop_graph();
+    });
+}
+
 int faco_engine_set_local_search(faco_engine* engine, uint32_t mode) {
     return guarded([&] {
         require(engine, "null argument");
@@ -1216,6 +1227,7 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         p.chunk = chunk; p.blocks_per_colony = 1;
         p.nbr_lds_bytes = round16_host(2u * n * K); p.nbr_stride = stride; p.per_wave_bytes = ant_wave_bytes(n);
         p.xcd_remap = 0; p.explicit_mode = 1; p.per_ant_parent = 1; p.log_checklist = 1;
+        p.roulette_margin = 0x1.0p-46;
         require(local_search <= 1, "local search mode must be 0 or 1");
         p.or_opt = local_search;
         p.ls_limit = limit;

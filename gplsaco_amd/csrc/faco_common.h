@@ -210,6 +210,7 @@ struct AntParams {
     uint32_t log_checklist;             // append every checklist push to diag_checklist
     uint32_t or_opt;                    // local search adds Or-opt segment moves
     uint64_t ls_limit;
+    double roulette_margin;             // relative margin of the fast roulette (+inf: sequential sums always)
     const uint32_t* coords;             // [G][n] packed x | y << 16
     const uint16_t* nbr;                // [G][n][K] (d, id)-ordered nearest cities
     const uint16_t* backup;             // [G][n][bw]

@@ -111,6 +111,23 @@ __device__ __forceinline__ double row_prefix_sum(double v) {
     return acc;
 }
 
+// Prefix sums over the 16 lanes of each row in 4 doubling steps (lane j ends
+// with v[0] + ... + v[j] in a tree order, not left to right).
+template<int Shift> __device__ __forceinline__ double row_shifted(double v) {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(bits), 0x110 + Shift, 0xF, 0xF, true));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(bits >> 32), 0x110 + Shift, 0xF, 0xF, true));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double row_scan4(double v) {
+    double acc = v;
+    acc = acc + row_shifted<1>(acc);
+    acc = acc + row_shifted<2>(acc);
+    acc = acc + row_shifted<4>(acc);
+    acc = acc + row_shifted<8>(acc);
+    return acc;
+}
+
 template<bool NbrLds, bool Profile, bool Diag>
 __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     // Diagnostic features (explicit visits, per-ant parents, traces and
@@ -342,14 +359,39 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 const bool avail = cur_free;
                 const uint64_t amask = ballot(avail);
                 if (amask) {
-                    // In-order prefix sums of the available weights: lane j
-                    // holds ((w0 + w1) + ...) + wj over available slots <= j,
-                    // the exact left-to-right sums of select_next.
-                    double acc;
-                    double total;
+                    // Roulette of select_next: the first available slot j with
+                    // u * total < prefix_j, where total and the prefixes are
+                    // the exact left-to-right sums of the available weights.
+                    // Fast path (pw <= 16): tree-order prefix sums. Any order
+                    // of summing at most 16 non-negative doubles is within
+                    // 15 * 2^-53 * sum of the exact sum, so the tree-order
+                    // threshold and prefixes differ from the sequential ones
+                    // by less than 64 * 2^-53 * total; when no available
+                    // prefix lies within the margin (2^-46 * total unless a
+                    // test widens it) of the threshold, every
+                    // comparison has the same outcome as with the sequential
+                    // sums and so has the selection. Otherwise (and never in
+                    // practice) the sequential sums are formed.
+                    double acc = 0.0;
+                    double total = 0.0;
+                    uint32_t sel_lane = kWave;
                     if (pw <= 16) {
-                        acc = row_prefix_sum(avail ? prod : 0.0);
-                        total = read_lane(acc, 15);
+                        const double w = avail ? prod : 0.0;
+                        const double fast = row_scan4(w);
+                        const double fast_total = read_lane(fast, 15);
+                        if (fast_total > 0.0) {
+                            const double threshold = u * fast_total;
+                            const double diff = threshold - fast;
+                            const double margin = fast_total * p.roulette_margin;
+                            if (!ballot(avail && diff <= margin && diff >= -margin)) {
+                                const uint64_t hit = ballot(avail && threshold < fast);
+                                sel_lane = hit ? ctz64(hit) : last64(amask);
+                            }
+                        }
+                        if (sel_lane == kWave) {
+                            acc = row_prefix_sum(w);
+                            total = read_lane(acc, 15);
+                        }
                     } else {
                         acc = 0.0;
                         for (uint64_t m = amask; m; m &= m - 1) {
@@ -359,7 +401,9 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                         }
                         total = read_lane(acc, 63);
                     }
-                    if (total > 0.0) {
+                    if (sel_lane < kWave) {
+                        selected = read_lane(cand, sel_lane);
+                    } else if (total > 0.0) {
                         const double threshold = u * total;
                         const uint64_t hit = ballot(avail && threshold < acc);
                         selected = read_lane(cand, hit ? ctz64(hit) : last64(amask));

@@ -371,6 +371,11 @@ class FacoBatchEngine:
         """'2opt' (the reference's checklist 2-opt, default) or '2opt+oropt'."""
         check(lib().faco_engine_set_local_search(self._handle, _local_search_mode(mode)))
 
+    def set_roulette_margin(self, margin: float) -> None:
+        """Test hook: widen the fast roulette's margin (float('inf'): the exact
+        left-to-right sums at every step). Results must not change."""
+        check(lib().faco_engine_set_roulette_margin(self._handle, float(margin)))
+
     # Learned edge prior (this engine's extension; gplsaco_amd.prior).
     def _instance_n(self, instance_key) -> int:
         """City count of a registered instance (the engine's dimension)."""

@@ -279,3 +279,31 @@ def test_device_start_nodes_match_oracle(ext, oracle_mod):
         dev = ext.diagnostic_start_nodes(key, batch, ant, counts, on_device=True)
         want = np.array([oracle_mod.start_node(key, batch, ant, int(c)) for c in counts], dtype=np.uint32)
         np.testing.assert_array_equal(dev, want)
+
+
+@pytest.mark.parametrize("n,pw", [(300, 16), (120, 8), (200, 2)])
+def test_fast_roulette_equals_sequential_sums(ext, oracle_mod, n, pw):
+    """The roulette's tree-order fast path never changes a selection: with the
+    margin at +inf every step uses the exact left-to-right sums, and both runs
+    equal the oracle (which sums left to right)."""
+    xys, keys = _panel(n, 2, clustered=True)
+    s = _settings(ext, ants=32, primary_width=pw, backup_width=min(64, n - 1 - pw))
+    colonies = 4
+    task_keys = np.array([keys[c % 2] for c in range(colonies)], dtype=np.uint64)
+    task_seeds = np.array([11 + c // 2 for c in range(colonies)], dtype=np.uint64)
+    runs = []
+    for margin in (None, float("inf")):
+        engine = ext.FacoBatchEngine(n, colonies, s)
+        for k, xy in zip(keys, xys):
+            engine.register_problem(k, xy)
+        if margin is not None:
+            engine.set_roulette_margin(margin)
+        announce("roulette", n, pw, margin)
+        runs.append(engine.evaluate_faco_evaluations(task_keys, task_seeds, s.ants * 12, 8, use_graph=False))
+    want = oracle_mod.run_faco(np.stack(xys).astype(np.int32), keys, task_seeds, _oracle_settings(oracle_mod, s), 12, 8)
+    for run in runs:
+        for c in range(colonies):
+            assert run["items"][c]["tour"] == want["tours"][c].tolist(), c
+        np.testing.assert_array_equal(run["colony_work"], want["work"])
+    with pytest.raises(ValueError):
+        ext.FacoBatchEngine(n, colonies, s).set_roulette_margin(0.0)
