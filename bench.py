@@ -151,6 +151,60 @@ def uneven_batch_rate(ext, rounds: int) -> dict:
             "ants_per_colony": {"64": ants.count(64), "128": ants.count(128), "192": ants.count(192)}}
 
 
+def full_pipeline(ext, rounds: int) -> dict:
+    """The full pipeline at TSP1000 and TSP2000: 10 + 10 instances (half
+    clustered) x 128 random conditional three-tree GP controllers in one
+    mixed-size engine, the learned edge prior on (k = 20 candidates),
+    autotuned launch shape, `rounds` x 384 ant tours per colony; plain FACO
+    (same k, no prior, no control) on the same instances with 8 seeds and the
+    same budget for the gap."""
+    from gplsaco_amd import prior as P
+    from tests.program_gen import random_controller
+    sizes = [1000] * 10 + [2000] * 10
+    settings = ext.FixedFacoSettings()
+    settings.primary_width = 20
+    keys, xys = [], []
+    for i, n in enumerate(sizes):
+        key, xy = P.synthetic_instance(n, i % 2 == 1, i, 5)
+        keys.append(key)
+        xys.append(xy)
+    rng = np.random.default_rng(5)
+    controllers = [ext.read_controller(random_controller(rng, f"c{i}")) for i in range(128)]
+    limit = 384 * rounds
+    engine = ext.MixedFacoBatchEngine(2000, len(sizes), settings, population_size=len(controllers))
+    for k, xy in zip(keys, xys):
+        engine.register_problem(k, xy)
+    net = P.EdgePrior.load()
+    for k in keys:
+        engine.apply_edge_prior(net, k)
+    task_keys = np.array(keys, dtype=np.uint64)
+    seeds = np.zeros(len(sizes), dtype=np.uint64)
+    tuned = engine.autotune(np.tile(task_keys, len(controllers)), np.tile(seeds, len(controllers)), iterations=1)
+    raw = engine.evaluate_control_raw(task_keys, seeds, limit, "controller", controllers=controllers, use_graph=True)
+    res = raw["result"]
+    costs = raw["costs"].reshape(len(controllers), len(sizes))
+    plain_engine = ext.MixedFacoBatchEngine(2000, len(sizes) * 8, settings)
+    for k, xy in zip(keys, xys):
+        plain_engine.register_problem(k, xy)
+    plain = plain_engine.evaluate_faco_evaluations(np.tile(task_keys, 8), np.repeat(np.arange(8, dtype=np.uint64), len(sizes)),
+                                                   limit)
+    plain_costs = np.array([it["cost"] for it in plain["items"]]).reshape(8, len(sizes))
+    out = {"workload": f"GP-controlled FACO with the learned prior, 10 x TSP1000 + 10 x TSP2000 (half clustered) x "
+                       f"{len(controllers)} random conditional three-tree controllers = {len(sizes) * len(controllers)} "
+                       f"colonies, k = 20, {limit} ant tours per colony",
+           "ant_tours_per_s": res.total_tour_evaluations / res.search_seconds, "tours": res.total_tour_evaluations,
+           "seconds": res.search_seconds, "restarts": sum(st["restarts"] for st in raw["states"]),
+           "autotune": tuned["best"], "plain_faco_ant_tours_per_s": plain["total_tour_evaluations"] / plain["search_seconds"]}
+    for n in (1000, 2000):
+        cols = [i for i, m in enumerate(sizes) if m == n]
+        controlled, base = float(costs[:, cols].mean()), float(plain_costs[:, cols].mean())
+        best = float(costs[:, cols].mean(axis=1).min())
+        out[f"tsp{n}"] = {"mean_final_tour_length": controlled, "plain_faco_mean": base,
+                          "gap_vs_plain": (controlled - base) / base,
+                          "best_controller_gap_vs_plain": (best - base) / base}
+    return out
+
+
 def learned_prior_legs(ext, budgets) -> dict:
     """Learned edge prior: (1) the committed trained network against plain
     FACO at equal FE on unseen TSP1000 instances (16: half uniform, half
@@ -208,6 +262,8 @@ def main() -> int:
                     help="iterations of the GP-controlled population measurement (0 skips it)")
     ap.add_argument("--uneven-rounds", type=int, default=40,
                     help="budget of the uneven-size GP batch in units of 384 ant tours per colony (0 skips it)")
+    ap.add_argument("--pipeline-rounds", type=int, default=10,
+                    help="budget of the full-pipeline leg in units of 384 ant tours per colony (0 skips it)")
     ap.add_argument("--prior-budgets", default="100,300",
                     help="iteration budgets of the learned-prior comparison (empty skips it)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -326,6 +382,8 @@ def main() -> int:
         line["config"]["gp_population"] = gp_population_rate(ext, args.population_steps)
     if args.uneven_rounds:
         line["config"]["uneven_batch"] = uneven_batch_rate(ext, args.uneven_rounds)
+    if args.pipeline_rounds:
+        line["config"]["full_pipeline"] = full_pipeline(ext, args.pipeline_rounds)
     if args.prior_budgets:
         line["config"]["learned_prior"] = learned_prior_legs(ext, [int(b) for b in args.prior_budgets.split(",")])
     if not args.no_cpu:

@@ -41,3 +41,20 @@ def random_program(rng, depth=5, feature_spec_id=1):
         operand.append(arg)
     return {"ir_version": 1, "numeric_spec_id": 1, "feature_spec_id": feature_spec_id,
             "opcode": opcode, "operand": operand, "constant_bits": constants}
+
+
+def random_controller_tree(rng, role, depth=3):
+    """A feature-spec-2 tree for stage `role` of a conditional three-tree
+    controller: the restart stage (0) reads neither the MNE level nor the
+    regional features, the region stage (1) not the MNE level."""
+    while True:
+        p = {**random_program(rng, depth=depth), "feature_spec_id": 2}
+        bad = any(op == 0 and ((role == 0 and (f == 5 or f >= 8)) or (role == 1 and f == 5))
+                  for op, f in zip(p["opcode"], p["operand"]))
+        if not bad:
+            return p
+
+
+def random_controller(rng, name):
+    return {"controller_version": 1, "representation": "conditional_three", "controller_id": name,
+            "trees": [random_controller_tree(rng, role) for role in range(3)]}
