@@ -226,6 +226,13 @@ SIGNATURES = {
     "faco_diagnostic_start_nodes": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                                    ctypes.c_uint32, ctypes.c_uint32, u32p]),
     "faco_diagnostic_copy": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "faco_diagnostic_escape_rows": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p,
+                                                   ctypes.c_uint32, u32p, ctypes.c_uint64, ctypes.c_uint32, u32p,
+                                                   ctypes.c_uint32, vp, u32p, u8p, u32p]),
+    "faco_diagnostic_escape_transactions": (ctypes.c_int, [ctypes.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p, u64p,
+                                                           u32p, u32p, ctypes.c_uint32, u32p, u32p, u32p, u32p]),
+    "faco_diagnostic_escape_draws": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                    ctypes.c_uint32, ctypes.c_uint32, u32p]),
     "faco_diagnostic_philox": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_uint32, u32p]),
     "faco_score_cpu": (ctypes.c_int, [ctypes.POINTER(Program), f32p, u32p, ctypes.c_uint32, f32p, i32p]),

@@ -899,6 +899,58 @@ def diagnostic_philox(key, batch, ant, step, words=8) -> np.ndarray:
     return out
 
 
+def diagnostic_escape_rows(original, backup, n, from_cities, key, batch, events, ls=False, coordinates=None) -> dict:
+    """Escape slot views of rows (original [rows][width <= 20], backup
+    [rows][bw <= 64], padding = n) built on the device with the escape stream
+    of (key, batch, row, events[row], ls); with coordinates, ordered as LS rows
+    by (EUC_2D distance, id)."""
+    o = np.ascontiguousarray(original, dtype=np.uint32)
+    b = np.ascontiguousarray(backup, dtype=np.uint32)
+    rows, width = o.shape
+    bw = b.shape[1] if b.ndim == 2 else 0
+    f = np.ascontiguousarray(from_cities, dtype=np.uint32)
+    e = np.ascontiguousarray(events, dtype=np.uint32)
+    if b.shape[0] != rows or f.size != rows or e.size != rows:
+        raise ValueError("one backup row, city and event per row")
+    xy = None if coordinates is None else _coordinates(coordinates)
+    members = np.zeros((rows, width), dtype=np.uint32)
+    replaced = np.zeros((rows, width), dtype=np.uint8)
+    counts = np.zeros((rows, 2), dtype=np.uint32)
+    check(lib().faco_diagnostic_escape_rows(rows, width, o.reshape(-1), bw, b.reshape(-1) if bw else np.zeros(1, np.uint32),
+                                            n, f, _unsigned(key, "key"), _unsigned(batch, "batch", 32), e,
+                                            1 if ls else 0, None if xy is None else xy.ctypes.data_as(ctypes.c_void_p),
+                                            members.reshape(-1), replaced.reshape(-1), counts.reshape(-1)))
+    return {"members": members, "replaced": replaced, "valid": counts[:, 0], "replacements": counts[:, 1]}
+
+
+def diagnostic_escape_transactions(counts, removed, added, permits, cache_edges, cache_sizes, cache_epochs,
+                                   offsets, neighbors) -> dict:
+    """Escape edge transactions on the device: prepare each case's proposal
+    (removed/added [cases][3][2]) against the CSR graph and its cache
+    ([cases][64][2], size, epoch), then commit it."""
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    cases = c.size
+    off = np.ascontiguousarray(offsets, dtype=np.uint32)
+    nb = np.ascontiguousarray(neighbors, dtype=np.uint32)
+    proposals = np.zeros((cases, 11), dtype=np.uint32)
+    committed = np.zeros(cases, dtype=np.uint32)
+    after = np.zeros((cases, 64, 2), dtype=np.uint32)
+    size_after = np.zeros(cases, dtype=np.uint32)
+    check(lib().faco_diagnostic_escape_transactions(
+        cases, c, np.ascontiguousarray(removed, dtype=np.uint32).reshape(-1),
+        np.ascontiguousarray(added, dtype=np.uint32).reshape(-1), np.ascontiguousarray(permits, dtype=np.uint32),
+        np.ascontiguousarray(cache_edges, dtype=np.uint32).reshape(-1), np.ascontiguousarray(cache_sizes, dtype=np.uint32),
+        np.ascontiguousarray(cache_epochs, dtype=np.uint64), off, nb if nb.size else np.zeros(1, np.uint32),
+        off.size - 1, proposals.reshape(-1), committed, after.reshape(-1), size_after))
+    return {"proposals": proposals, "committed": committed, "cache_after": after, "size_after": size_after}
+
+
+def diagnostic_escape_draws(key, batch, ant, event, ls, count) -> np.ndarray:
+    out = np.zeros(count, dtype=np.uint32)
+    check(lib().faco_diagnostic_escape_draws(_unsigned(key, "key"), batch, ant, event, 1 if ls else 0, count, out))
+    return out
+
+
 def diagnostic_copy(nbytes: int, repeats: int = 1) -> float:
     """Average milliseconds of the calibration copy kernel over `nbytes`."""
     ms = ctypes.c_double()

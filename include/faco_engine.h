@@ -432,6 +432,42 @@ int faco_score_cpu(const faco_program* program, const float* features, const uin
 int faco_score_gpu(const faco_program* program, const float* features, const uint32_t* masks,
                    uint32_t colonies, float* scores, int32_t* actions);
 
+/* Escape primitives of the E3 candidate-graph exit (reference
+ * cuda/escape_device.cuh:8-128, cpp/include/gp_faco/escape_slots.hpp:18-71,
+ * escape_edges.hpp:13-122), one wave per row or transaction; diagnostic
+ * entry points that run them on the device.
+ *
+ * Rows: for row r (ant index r), the escape stream of (key, batch, r,
+ * events[r], ls) builds the slot view of original[r] (width <= 20 slots,
+ * valid prefix, padding = n) with backup[r] (bw <= 64, padding = n) for city
+ * from[r]; with xy ([n][2] integer coordinates, may be NULL) the view is then
+ * ordered by (EUC_2D distance to from[r], id) as an LS row. Outputs members
+ * and replaced flags [rows][width] and counts [rows][2] = (valid, replacements).
+ * The escape stream: Philox4x32-10 keyed by `key`, counter block
+ * (block, event, domain | ant, batch), domain 0x40000000 (construction) or
+ * 0x80000000 (LS rows), four words per block. */
+int faco_diagnostic_escape_rows(uint32_t rows, uint32_t width, const uint32_t* original, uint32_t bw,
+                                const uint32_t* backup, uint32_t n, const uint32_t* from, uint64_t key, uint32_t batch,
+                                const uint32_t* events, uint32_t ls, const double* xy, uint32_t* out_members,
+                                uint8_t* out_replaced, uint32_t* out_counts);
+/* Transactions: case c proposes counts[c] (1..3) added edges against as many
+ * removed ones (removed/added [cases][3][2]) for a cache of cache_sizes[c]
+ * edges (cache_edges [cases][64][2]) at epoch cache_epochs[c], with the CSR
+ * graph (offsets [dimension+1], neighbors sorted per row) and permission
+ * permits[c], then commits into the cache. out_proposals [cases][11] =
+ * status (0 allowed, 1 invalid edge, 2 outside graph, 3 capacity), count,
+ * expected size, expected epoch (low, high), edges [3][2]; out_committed,
+ * out_cache_after [cases][64][2] and out_size_after [cases]. */
+int faco_diagnostic_escape_transactions(uint32_t cases, const uint32_t* counts, const uint32_t* removed,
+                                        const uint32_t* added, const uint32_t* permits, const uint32_t* cache_edges,
+                                        const uint32_t* cache_sizes, const uint64_t* cache_epochs,
+                                        const uint32_t* offsets, const uint32_t* neighbors, uint32_t dimension,
+                                        uint32_t* out_proposals, uint32_t* out_committed, uint32_t* out_cache_after,
+                                        uint32_t* out_size_after);
+/* `count` (1..4096) words of the escape stream of (key, batch, ant < 2^30, event, ls). */
+int faco_diagnostic_escape_draws(uint64_t key, uint32_t batch, uint32_t ant, uint32_t event, uint32_t ls, uint32_t count,
+                                 uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1616,3 +1616,267 @@ int oracle_prior_features(u32 n, const std::int32_t* xy, u32 pw, const u32* prim
     });
 }
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Escape primitives (the E3 candidate-graph exit), restated from
+// cpp/include/gp_faco/escape_slots.hpp:18-71, escape_edges.hpp:13-122,
+// edge_constraints.hpp:15-33 and cuda/escape_device.cuh:8-27.
+namespace {
+
+constexpr u32 kEscCapacity = 64, kEscLsWidth = 20, kEscBackupWidth = 64;
+
+// EscapeRandom (cuda/escape_device.cuh:8-27) over the shared Philox: a word
+// stream per (batch, domain | ant, event) event, domain 0x40000000 for
+// construction gates and 0x80000000 for LS rows; counter block
+// (block, event, domain | ant, batch). Replaces cuRAND's stream (unpinned).
+struct EscStream {
+    u32 key[2], ant, batch, event, block = 0, index = 4, words[4] = {0, 0, 0, 0};
+    EscStream(u64 k, u32 b, u32 a, u32 e, bool ls) {
+        key[0] = static_cast<u32>(k); key[1] = static_cast<u32>(k >> 32);
+        batch = b; event = e; ant = (ls ? 0x80000000u : 0x40000000u) | a;
+    }
+    u32 draw() {
+        if (index == 4) {
+            const u32 ctr[4] = {block++, event, ant, batch};
+            philox4x32_10(ctr, key, words);
+            index = 0;
+        }
+        return words[index++];
+    }
+    u32 below(u32 count) {   // unbiased: reject words below 2^32 mod count
+        const u32 threshold = (0u - count) % count;
+        u32 v;
+        do { v = draw(); } while (v < threshold);
+        return v % count;
+    }
+};
+
+// The reference test's scripted draws (tests/cpp/test_escape_slots.cpp:13-17).
+struct ScriptDraws {
+    u32 rounding;
+    u32 draw() { return rounding; }
+    u32 below(u32 count) {
+        if (!count) throw std::invalid_argument("sampling an empty set");
+        return count - 1;
+    }
+};
+
+struct EscRow {
+    u32 members[kEscLsWidth] = {};
+    std::uint8_t replaced[kEscLsWidth] = {};
+    u32 valid = 0, replacements = 0;
+};
+
+// escape_slot_view (escape_slots.hpp:20-49).
+template<class Random>
+EscRow slot_view(const u32* original, u32 width, const u32* backup, u32 bw, u32 n, u32 from, Random& random) {
+    if (width > kEscLsWidth || bw > kEscBackupWidth) throw std::invalid_argument("escape row too wide");
+    EscRow row;
+    for (u32 j = 0; j < width; ++j) { row.members[j] = original[j]; row.valid += original[j] < n ? 1u : 0u; }
+    u32 wanted = row.valid / 4;
+    if (row.valid % 4) wanted += (random.draw() & 3u) < row.valid % 4 ? 1u : 0u;
+    if (!wanted) return row;
+    u32 eligible[kEscBackupWidth], count = 0;
+    for (u32 j = 0; j < bw; ++j) {
+        const u32 node = backup[j];
+        if (node >= n) break;
+        if (node == from) continue;
+        bool duplicate = false;
+        for (u32 k = 0; k < row.valid; ++k) duplicate |= original[k] == node;
+        for (u32 k = 0; k < count; ++k) duplicate |= eligible[k] == node;
+        if (!duplicate) eligible[count++] = node;
+    }
+    wanted = std::min(wanted, count);
+    u32 slots[kEscLsWidth];
+    for (u32 j = 0; j < row.valid; ++j) slots[j] = j;
+    for (u32 j = 0; j < wanted; ++j) {
+        const u32 pick_slot = j + random.below(row.valid - j);
+        std::swap(slots[pick_slot], slots[j]);
+        const u32 pick_node = j + random.below(count - j);
+        std::swap(eligible[pick_node], eligible[j]);
+        row.members[slots[j]] = eligible[j];
+        row.replaced[slots[j]] = 1;
+    }
+    row.replacements = wanted;
+    return row;
+}
+
+// sort_escape_ls_row (escape_slots.hpp:52-68): insertion sort of the valid
+// prefix by (distance, id); the replaced flags move with their members.
+void sort_row(EscRow& row, const double* distance_of) {
+    for (u32 j = 1; j < row.valid; ++j) {
+        const u32 node = row.members[j];
+        const std::uint8_t flag = row.replaced[j];
+        const double d = distance_of[node];
+        u32 at = j;
+        while (at) {
+            const u32 prior = row.members[at - 1];
+            const double previous = distance_of[prior];
+            if (previous < d || (previous == d && prior < node)) break;
+            row.members[at] = prior; row.replaced[at] = row.replaced[at - 1]; --at;
+        }
+        row.members[at] = node; row.replaced[at] = flag;
+    }
+}
+
+struct EscEdge { u32 a, b; };
+bool same_edge(EscEdge x, EscEdge y) { return (x.a == y.a && x.b == y.b) || (x.a == y.b && x.b == y.a); }
+
+// SparseGraphView::operator() (edge_constraints.hpp:15-28): binary search of b in a's CSR row.
+bool graph_has(const u32* offsets, const u32* neighbors, u32 dimension, u32 a, u32 b) {
+    if (a >= dimension || b >= dimension || a == b) return false;
+    u32 first = offsets[a], last = offsets[a + 1];
+    const u32 end = last;
+    while (first < last) {
+        const u32 middle = first + (last - first) / 2;
+        if (neighbors[middle] < b) first = middle + 1; else last = middle;
+    }
+    return first < end && neighbors[first] == b;
+}
+
+struct EscCache { EscEdge edges[kEscCapacity]; u32 size; u64 epoch; };
+bool cache_has(const EscCache& c, EscEdge e) {
+    for (u32 i = 0; i < c.size; ++i) if (same_edge(c.edges[i], e)) return true;
+    return false;
+}
+
+enum EscStatus : u32 { kAllowed = 0, kInvalidEdge = 1, kOutsideGraph = 2, kCapacity = 3 };
+struct EscProposal { EscEdge edges[3] = {}; u32 count = 0, expected_size = 0; u64 expected_epoch = 0; u32 status = kAllowed; };
+
+// prepare_escape_edges (escape_edges.hpp:45-72).
+EscProposal prepare_edges(u32 count, const EscEdge* removed, const EscEdge* added, const u32* offsets,
+                          const u32* neighbors, u32 dimension, const EscCache& cache, bool permit) {
+    EscProposal p;
+    p.expected_size = cache.size; p.expected_epoch = cache.epoch;
+    bool consumed[3] = {false, false, false};
+    for (u32 i = 0; i < count; ++i) {
+        const EscEdge e = added[i].a < added[i].b ? added[i] : EscEdge{added[i].b, added[i].a};
+        if (e.a >= dimension || e.b >= dimension || e.a == e.b) { p.status = kInvalidEdge; return p; }
+        bool unchanged = false;
+        for (u32 j = 0; j < count; ++j)
+            if (!consumed[j] && same_edge(e, removed[j])) { consumed[j] = true; unchanged = true; break; }
+        if (unchanged || graph_has(offsets, neighbors, dimension, e.a, e.b) || cache_has(cache, e)) continue;
+        if (!permit) { p.status = kOutsideGraph; return p; }
+        bool duplicate = false;
+        for (u32 j = 0; j < p.count; ++j) duplicate |= same_edge(e, p.edges[j]);
+        if (!duplicate) p.edges[p.count++] = e;
+    }
+    if (cache.size > kEscCapacity || p.count > kEscCapacity - cache.size) p.status = kCapacity;
+    return p;
+}
+
+// commit_escape_edges (escape_edges.hpp:74-84).
+bool commit_edges(EscCache& cache, const EscProposal& p) {
+    if (p.status != kAllowed || cache.epoch != p.expected_epoch || cache.size != p.expected_size ||
+        cache.size > kEscCapacity || p.count > kEscCapacity - cache.size) return false;
+    for (u32 i = 0; i < p.count; ++i) cache.edges[cache.size + i] = p.edges[i];
+    cache.size += p.count;
+    return true;
+}
+
+void load_cache(EscCache& c, const u32* edges, u32 size, u64 epoch) {
+    if (size > kEscCapacity) throw std::invalid_argument("escape cache holds at most 64 edges");
+    c.size = size; c.epoch = epoch;
+    for (u32 i = 0; i < kEscCapacity; ++i) c.edges[i] = i < size ? EscEdge{edges[2 * i], edges[2 * i + 1]} : EscEdge{0, 0};
+}
+
+void store_proposal(const EscProposal& p, u32* out) {
+    // status, count, expected size, expected epoch (low, high), edges [3][2]
+    out[0] = p.status; out[1] = p.count; out[2] = p.expected_size;
+    out[3] = static_cast<u32>(p.expected_epoch); out[4] = static_cast<u32>(p.expected_epoch >> 32);
+    for (u32 i = 0; i < 3; ++i) { out[5 + 2 * i] = p.edges[i].a; out[6 + 2 * i] = p.edges[i].b; }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Escape slot view of one row, optionally sorted as an LS row. `script`:
+// 0 = the escape stream of (key, batch, ant, event, ls); else scripted draws
+// with draw() = script - 1 and below(count) = count - 1. `distance_of` [n]
+// (may be null: no sort) gives the sort key of each node. out_counts = valid,
+// replacements.
+int oracle_escape_slot_view(const u32* original, u32 width, const u32* backup, u32 bw, u32 n, u32 from, u32 script,
+                            u64 key, u32 batch, u32 ant, u32 event, u32 ls, const double* distance_of,
+                            u32* out_members, std::uint8_t* out_replaced, u32* out_counts) {
+    return guarded([&] {
+        EscRow row;
+        if (script) { ScriptDraws d{script - 1}; row = slot_view(original, width, backup, bw, n, from, d); }
+        else { EscStream d(key, batch, ant, event, ls != 0); row = slot_view(original, width, backup, bw, n, from, d); }
+        if (distance_of) sort_row(row, distance_of);
+        for (u32 j = 0; j < width; ++j) { out_members[j] = row.members[j]; out_replaced[j] = row.replaced[j]; }
+        out_counts[0] = row.valid; out_counts[1] = row.replacements;
+    });
+}
+
+// `count` words of the escape stream of (key, batch, ant, event, ls).
+void oracle_escape_draws(u64 key, u32 batch, u32 ant, u32 event, u32 ls, u32 count, u32* out) {
+    EscStream d(key, batch, ant, event, ls != 0);
+    for (u32 i = 0; i < count; ++i) out[i] = d.draw();
+}
+
+// One escape transaction: prepare (removed/added [count][2]) against the
+// CSR graph and the cache (edges [size][2], epoch), then commit into a copy
+// of the cache. out_proposal [11] (see store_proposal); out_cache_after
+// [64][2] + out_size_after; returns the commit result in *committed.
+int oracle_escape_transaction(u32 count, const u32* removed, const u32* added, const u32* offsets, const u32* neighbors,
+                              u32 dimension, const u32* cache_edges, u32 cache_size, u64 cache_epoch, u32 permit,
+                              u32* out_proposal, u32* committed, u32* out_cache_after, u32* out_size_after) {
+    return guarded([&] {
+        if (count < 1 || count > 3) throw std::invalid_argument("escape transactions hold 1 to 3 edges");
+        EscEdge r[3], a[3];
+        for (u32 i = 0; i < count; ++i) { r[i] = {removed[2 * i], removed[2 * i + 1]}; a[i] = {added[2 * i], added[2 * i + 1]}; }
+        EscCache c;
+        load_cache(c, cache_edges, cache_size, cache_epoch);
+        const EscProposal p = prepare_edges(count, r, a, offsets, neighbors, dimension, c, permit != 0);
+        store_proposal(p, out_proposal);
+        *committed = commit_edges(c, p) ? 1u : 0u;
+        for (u32 i = 0; i < kEscCapacity; ++i) { out_cache_after[2 * i] = c.edges[i].a; out_cache_after[2 * i + 1] = c.edges[i].b; }
+        *out_size_after = c.size;
+    });
+}
+
+// commit_escape_edges of a stored proposal (out of store_proposal) into a
+// copy of a cache: the commit of a proposal prepared against another cache
+// state (expired by a move or by a reset).
+int oracle_escape_commit(const u32* proposal, const u32* cache_edges, u32 cache_size, u64 cache_epoch, u32* committed,
+                         u32* out_cache_after, u32* out_size_after) {
+    return guarded([&] {
+        EscCache c;
+        load_cache(c, cache_edges, cache_size, cache_epoch);
+        EscProposal p;
+        p.status = proposal[0]; p.count = proposal[1]; p.expected_size = proposal[2];
+        p.expected_epoch = static_cast<u64>(proposal[3]) | (static_cast<u64>(proposal[4]) << 32);
+        if (p.count > 3) throw std::invalid_argument("a proposal holds at most 3 edges");
+        for (u32 i = 0; i < 3; ++i) p.edges[i] = {proposal[5 + 2 * i], proposal[6 + 2 * i]};
+        *committed = commit_edges(c, p) ? 1u : 0u;
+        for (u32 i = 0; i < kEscCapacity; ++i) { out_cache_after[2 * i] = c.edges[i].a; out_cache_after[2 * i + 1] = c.edges[i].b; }
+        *out_size_after = c.size;
+    });
+}
+
+// prepare_escape_relocation (escape_edges.hpp:101-115) on an array tour.
+int oracle_escape_relocation(const u32* tour, const u32* positions, u32 n, u32 target, u32 node, const u32* offsets,
+                             const u32* neighbors, const u32* cache_edges, u32 cache_size, u64 cache_epoch, u32 permit,
+                             u32* out_proposal) {
+    return guarded([&] {
+        EscCache c;
+        load_cache(c, cache_edges, cache_size, cache_epoch);
+        EscProposal p;
+        p.expected_size = c.size; p.expected_epoch = c.epoch;
+        if (target >= n || node >= n || target == node) {
+            p.status = kInvalidEdge;
+        } else {
+            const u32 target_after = tour[(positions[target] + 1) % n];
+            if (target_after != node) {
+                const u32 before = tour[(positions[node] + n - 1) % n], after = tour[(positions[node] + 1) % n];
+                const EscEdge removed[3] = {{before, node}, {node, after}, {target, target_after}};
+                const EscEdge added[3] = {{before, after}, {target, node}, {node, target_after}};
+                p = prepare_edges(3, removed, added, offsets, neighbors, n, c, permit != 0);
+            }
+        }
+        store_proposal(p, out_proposal);
+    });
+}
+
+}  // extern "C"

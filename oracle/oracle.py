@@ -190,6 +190,17 @@ def lib():
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p]
         L.oracle_prior_features.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, f32p]
+        L.oracle_escape_slot_view.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, u32p,
+                                              u8p, u32p]
+        L.oracle_escape_draws.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_uint32, u32p]
+        L.oracle_escape_transaction.argtypes = [ctypes.c_uint32, u32p, u32p, u32p, u32p, ctypes.c_uint32, u32p,
+                                                ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p, u32p, u32p, u32p]
+        L.oracle_escape_relocation.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                               u32p, u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p]
+        L.oracle_escape_commit.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p]
         L.oracle_max_threads.restype = ctypes.c_int
         L.oracle_tour_ident.argtypes = [ctypes.c_uint32, u32p, u64p]
         L.oracle_canonical_compare.restype = ctypes.c_int
@@ -415,6 +426,121 @@ def prior_features(xy, primary, local_scale, epsilon):
     _check(L.oracle_prior_features(n, xy.reshape(-1), pw, primary.reshape(-1), scale, float(epsilon),
                                    out.reshape(-1)), L)
     return out
+
+
+# ----------------------------------------------------------------------------
+# Escape primitives (cpp/include/gp_faco/escape_slots.hpp, escape_edges.hpp).
+
+STATUS = {0: "allowed", 1: "invalid_edge", 2: "outside_graph", 3: "capacity"}
+
+
+def escape_slot_view(original, backup, n, from_city, *, script=None, key=0, batch=0, ant=0, event=0, ls=False,
+                     distance_of=None):
+    """One slot view. script=r: the reference test's scripted draws (draw() = r,
+    below(c) = c - 1); else the escape stream of (key, batch, ant, event, ls).
+    distance_of [n] (float64): sort as an LS row by (distance_of[node], node)."""
+    o = np.ascontiguousarray(original, dtype=np.uint32)
+    b = np.ascontiguousarray(backup, dtype=np.uint32)
+    members = np.zeros(o.size, dtype=np.uint32)
+    replaced = np.zeros(o.size, dtype=np.uint8)
+    counts = np.zeros(2, dtype=np.uint32)
+    dist = None if distance_of is None else np.ascontiguousarray(distance_of, dtype=np.float64)
+    L = lib()
+    _check(L.oracle_escape_slot_view(o, o.size, b if b.size else np.zeros(1, np.uint32), b.size, n, from_city,
+                                     0 if script is None else script + 1, key, batch, ant, event, 1 if ls else 0,
+                                     None if dist is None else dist.ctypes.data_as(ctypes.c_void_p), members, replaced,
+                                     counts), L)
+    return {"members": members, "replaced": replaced, "valid": int(counts[0]), "replacements": int(counts[1])}
+
+
+def escape_draws(key, batch, ant, event, ls, count):
+    out = np.zeros(count, dtype=np.uint32)
+    lib().oracle_escape_draws(key, batch, ant, event, 1 if ls else 0, count, out)
+    return out
+
+
+def _csr(graph_edges, n):
+    rows = [[] for _ in range(n)]
+    for a, b in graph_edges:
+        rows[a].append(b)
+        rows[b].append(a)
+    offsets = np.zeros(n + 1, dtype=np.uint32)
+    neighbors = []
+    for a in range(n):
+        r = sorted(set(rows[a]))
+        neighbors += r
+        offsets[a + 1] = offsets[a] + len(r)
+    return offsets, np.array(neighbors or [0], dtype=np.uint32)
+
+
+def csr(graph_edges, n):
+    """Undirected CSR (offsets [n+1], sorted neighbours) of an edge list."""
+    return _csr(graph_edges, n)
+
+
+def _proposal(p):
+    return {"status": STATUS[int(p[0])], "count": int(p[1]), "expected_size": int(p[2]),
+            "expected_epoch": int(p[3]) | (int(p[4]) << 32),
+            "edges": [(int(p[5 + 2 * i]), int(p[6 + 2 * i])) for i in range(int(p[1]))]}
+
+
+def escape_transaction(removed, added, csr_graph, n, cache_edges, cache_epoch, permit):
+    """prepare_escape_edges + commit_escape_edges on a copy of the cache."""
+    r = np.ascontiguousarray(removed, dtype=np.uint32).reshape(-1, 2)
+    a = np.ascontiguousarray(added, dtype=np.uint32).reshape(-1, 2)
+    cache = np.zeros((64, 2), dtype=np.uint32)
+    if len(cache_edges):
+        cache[: len(cache_edges)] = np.asarray(cache_edges, dtype=np.uint32)
+    offsets, neighbors = csr_graph
+    prop = np.zeros(11, dtype=np.uint32)
+    committed = np.zeros(1, dtype=np.uint32)
+    after = np.zeros((64, 2), dtype=np.uint32)
+    size_after = np.zeros(1, dtype=np.uint32)
+    L = lib()
+    _check(L.oracle_escape_transaction(r.shape[0], r.reshape(-1), a.reshape(-1), offsets, neighbors, n,
+                                       cache.reshape(-1), len(cache_edges), cache_epoch, 1 if permit else 0, prop,
+                                       committed, after.reshape(-1), size_after), L)
+    out = _proposal(prop)
+    out.update(committed=bool(committed[0]), cache_after=[tuple(e) for e in after[: int(size_after[0])].tolist()])
+    return out
+
+
+def escape_commit(proposal, cache_edges, cache_epoch):
+    """commit_escape_edges of a proposal dict (from escape_transaction) into a
+    copy of another cache state; returns (committed, cache edges after)."""
+    raw = np.zeros(11, dtype=np.uint32)
+    raw[0] = {v: k for k, v in STATUS.items()}[proposal["status"]]
+    raw[1], raw[2] = proposal["count"], proposal["expected_size"]
+    raw[3], raw[4] = proposal["expected_epoch"] & 0xFFFFFFFF, proposal["expected_epoch"] >> 32
+    for i, (a, b) in enumerate(proposal["edges"]):
+        raw[5 + 2 * i], raw[6 + 2 * i] = a, b
+    cache = np.zeros((64, 2), dtype=np.uint32)
+    if len(cache_edges):
+        cache[: len(cache_edges)] = np.asarray(cache_edges, dtype=np.uint32)
+    committed = np.zeros(1, dtype=np.uint32)
+    after = np.zeros((64, 2), dtype=np.uint32)
+    size_after = np.zeros(1, dtype=np.uint32)
+    L = lib()
+    _check(L.oracle_escape_commit(raw, cache.reshape(-1), len(cache_edges), cache_epoch, committed, after.reshape(-1),
+                                  size_after), L)
+    return bool(committed[0]), [tuple(e) for e in after[: int(size_after[0])].tolist()]
+
+
+def escape_relocation(tour, target, node, csr_graph, cache_edges, cache_epoch, permit):
+    """prepare_escape_relocation of moving `node` after `target` in `tour`."""
+    t = np.ascontiguousarray(tour, dtype=np.uint32)
+    n = t.size
+    pos = np.zeros(n, dtype=np.uint32)
+    pos[t] = np.arange(n, dtype=np.uint32)
+    cache = np.zeros((64, 2), dtype=np.uint32)
+    if len(cache_edges):
+        cache[: len(cache_edges)] = np.asarray(cache_edges, dtype=np.uint32)
+    offsets, neighbors = csr_graph
+    prop = np.zeros(11, dtype=np.uint32)
+    L = lib()
+    _check(L.oracle_escape_relocation(t, pos, n, target, node, offsets, neighbors, cache.reshape(-1), len(cache_edges),
+                                      cache_epoch, 1 if permit else 0, prop), L)
+    return _proposal(prop)
 
 
 def max_threads() -> int:
