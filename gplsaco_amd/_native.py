@@ -151,6 +151,15 @@ class ControlSpec(ctypes.Structure):
     ]
 
 
+class Checkpoints(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_void_p),
+        ("count", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+        ("tours", ctypes.c_void_p),
+    ]
+
+
 class ControlTrace(ctypes.Structure):
     _fields_ = [
         ("batches", ctypes.c_void_p),
@@ -210,7 +219,8 @@ SIGNATURES = {
     "faco_engine_preparation_profile": (ctypes.c_int, [vp, ctypes.c_uint64, f64p]),
     "faco_engine_problem_data": (ctypes.c_int, [vp, ctypes.c_uint64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int64)]),
     "faco_engine_evaluate_faco": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
-                                                 ctypes.c_uint32, ctypes.POINTER(Result), u32p, i64p, u64p]),
+                                                 ctypes.c_uint32, ctypes.POINTER(Result), u32p, i64p, u64p,
+                                                 ctypes.POINTER(Checkpoints)]),
     "faco_fixed_create": (ctypes.c_int, [f64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(Settings),
                                          ctypes.POINTER(vp)]),
     "faco_fixed_destroy": (None, [vp]),
@@ -258,7 +268,7 @@ SIGNATURES = {
     "faco_engine_evaluate_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64,
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,
-                                                    u64p, vp]),
+                                                    u64p, vp, ctypes.POINTER(Checkpoints)]),
 }
 
 _lib = None

@@ -521,12 +521,51 @@ void fill_result(faco_result* r, const Engine& e, uint32_t colonies, uint64_t li
     r->search_gpu_milliseconds = gpu_ms;
 }
 
+// Curve checkpoints (reference cuda/batch_engine.cu:635-653): the run stops
+// at each requested batch count, copies every colony's global-best tour on
+// the device, and reads all of them back once at the end.
+struct Checkpoints {
+    const faco_checkpoints* spec = nullptr;
+    DeviceArray<uint32_t> tours;
+    size_t nodes = 0, cursor = 0;
+
+    Checkpoints(const faco_checkpoints* s, uint64_t batches, size_t colony_nodes) : nodes(colony_nodes) {
+        if (!s || !s->count) return;
+        require(s->iterations != nullptr && s->tours != nullptr, "checkpoint arrays missing");
+        for (uint32_t i = 0; i < s->count; ++i)
+            require(s->iterations[i] <= batches && (i == 0 || s->iterations[i] > s->iterations[i - 1]),
+                    "checkpoint iterations must be ascending, distinct and inside the budget");
+        spec = s;
+        tours.allocate(static_cast<size_t>(s->count) * nodes);
+    }
+    // the next batch count at which the run must stop (or the end)
+    uint64_t next(uint64_t batches) const {
+        return spec && cursor < spec->count ? spec->iterations[cursor] : batches;
+    }
+    void record(const uint32_t* global, uint64_t done, hipStream_t stream) {
+        if (!spec || cursor >= spec->count || spec->iterations[cursor] != done) return;
+        checked(hipMemcpyAsync(tours.data() + cursor * nodes, global, nodes * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, stream), "checkpoint copy");
+        ++cursor;
+    }
+    // after the run's final synchronisation; `n_of(c)` is colony c's own size
+    template<class SizeOf> void read_back(hipStream_t stream, uint32_t stride, SizeOf n_of) {
+        if (!spec) return;
+        tours.download(spec->tours, tours.size(), stream);
+        checked(hipStreamSynchronize(stream), "checkpoint readback");
+        for (uint32_t k = 0; k < spec->count; ++k)
+            for (size_t c = 0; c < nodes / stride; ++c)
+                for (uint32_t i = n_of(static_cast<uint32_t>(c)); i < stride; ++i)
+                    spec->tours[k * nodes + c * stride + i] = UINT32_MAX;
+    }
+};
+
 // Shared evaluation body for the batch engine and the fixed entry point.
 // `any_count`: the tuner may run any number of colonies up to the engine's
 // capacity (a population engine's whole shape), not just its panel.
 void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks, uint64_t batches,
               uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
-              uint64_t limit, bool any_count = false) {
+              uint64_t limit, bool any_count = false, const faco_checkpoints* checkpoints = nullptr) {
     const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
     const bool events = (flags & FACO_RUN_EVENTS) != 0;
     double kernel_ms[FACO_STAGE_COUNT] = {0};
@@ -544,12 +583,19 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     e.keys.upload(keys.data(), tasks, e.stream);
     struct Inactive { Engine& e; ~Inactive() { e.shapes_active = false; } } inactive{e};
     if (e.mixed) batches = e.prepare_shapes(instance_keys, tasks, limit);
+    Checkpoints marks(checkpoints, batches, static_cast<size_t>(tasks) * e.n);
     e.reset(tasks, mne);
+    marks.record(e.global.data(), 0, e.stream);
     checked(hipStreamSynchronize(e.stream), "initialisation");
     const auto search_started = Clock::now();
     checked(hipEventRecord(e.ev_start, e.stream), "event");
-    if (profile || events) e.run_profiled(tasks, batches, kernel_ms, profile);
-    else e.run(tasks, batches, use_graph);
+    for (uint64_t done = 0; done < batches;) {
+        const uint64_t stop = std::max(marks.next(batches), done + 1);
+        if (profile || events) e.run_profiled(tasks, stop - done, kernel_ms, profile);
+        else e.run(tasks, stop - done, use_graph);
+        done = stop;
+        marks.record(e.global.data(), done, e.stream);
+    }
     checked(hipEventRecord(e.ev_stop, e.stream), "event");
     checked(hipStreamSynchronize(e.stream), "search");
     const double search = seconds_since(search_started);
@@ -564,6 +610,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
         for (uint32_t c = 0; c < tasks; ++c)
             for (uint32_t i = e.problem(instance_keys[c]).n; i < e.n; ++i) tours[static_cast<size_t>(c) * e.n + i] = UINT32_MAX;
     }
+    marks.read_back(e.stream, e.n, [&](uint32_t c) { return e.problem(instance_keys[c]).n; });
     uint64_t steps = 0, evals = 0;
     for (uint32_t c = 0; c < tasks; ++c) {
         if (costs) costs[c] = states[c].global_cost;
@@ -605,7 +652,7 @@ void validate_baseline(const faco_baseline_policy& p, uint32_t mask) {
 void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* panel_seeds, uint32_t panel,
                       uint64_t limit, const faco_control_spec& spec, uint32_t flags, const faco_control_trace* trace,
                       faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
-                      faco_control_state* states) {
+                      faco_control_state* states, const faco_checkpoints* checkpoints) {
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "one evaluation per engine at a time");
@@ -695,15 +742,47 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     ap.out_identity = b.ant_identity.data();
     ap.out_zero = b.ant_zero.data();
     const ColonyParams colony = e.colony_params();
+    // FACO_RUN_EVENTS: HIP events between the launches of every batch;
+    // kernel_ms [0] construction + LS, [1] colony update, [2] decision
+    // (features, regions, scoring, action and restart), [3] archive and
+    // feedback update, [4] initialisation, all summed over batches
+    const bool events = (flags & FACO_RUN_EVENTS) != 0 && traced.empty();
+    double kernel_ms[FACO_STAGE_COUNT] = {0};
+    hipEvent_t ev[5] = {};
+    struct Destroy { hipEvent_t* e; ~Destroy() { for (int i = 0; i < 5; ++i) if (e[i]) (void)hipEventDestroy(e[i]); } } destroy{ev};
+    if (events)
+        for (auto& x : ev) checked(hipEventCreate(&x), "hipEventCreate");
+    auto elapsed = [&](int a, int b) {
+        float ms = 0;
+        checked(hipEventElapsedTime(&ms, ev[a], ev[b]), "event time");
+        return static_cast<double>(ms);
+    };
     auto iteration = [&]() {
+        if (events) checked(hipEventRecord(ev[0], e.stream), "event");
         checked(launch_control_decide(cp, C, e.stream), "control decide launch");
+        if (events) checked(hipEventRecord(ev[1], e.stream), "event");
         checked(launch_ant_search(ap, C * ap.blocks_per_colony, e.stream), "ant search launch");
+        if (events) checked(hipEventRecord(ev[2], e.stream), "event");
         checked(launch_colony_update(colony, C, e.stream), "colony update launch");
+        if (events) checked(hipEventRecord(ev[3], e.stream), "event");
         checked(launch_control_update(cp, C, e.stream), "control update launch");
+        if (!events) return;
+        checked(hipEventRecord(ev[4], e.stream), "event");
+        checked(hipEventSynchronize(ev[4]), "event sync");
+        kernel_ms[2] += elapsed(0, 1); kernel_ms[0] += elapsed(1, 2);
+        kernel_ms[1] += elapsed(2, 3); kernel_ms[3] += elapsed(3, 4);
     };
 
+    Checkpoints marks(checkpoints, batches, static_cast<size_t>(C) * e.n);
+    if (events) checked(hipEventRecord(ev[0], e.stream), "event");
     e.reset(C, 2u);
     checked(launch_control_init(cp, C, e.stream), "control init launch");
+    if (events) {
+        checked(hipEventRecord(ev[1], e.stream), "event");
+        checked(hipEventSynchronize(ev[1]), "event sync");
+        kernel_ms[4] = elapsed(0, 1);
+    }
+    marks.record(e.global.data(), 0, e.stream);
     checked(hipStreamSynchronize(e.stream), "initialisation");
     const auto search_started = Clock::now();
     checked(hipEventRecord(e.ev_start, e.stream), "event");
@@ -712,6 +791,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         std::vector<ColonyState> st(C);
         for (uint64_t batch = 0; batch < batches; ++batch) {
             iteration();
+            marks.record(e.global.data(), batch + 1, e.stream);
             if (next < traced.size() && traced[next] == batch) {
                 const size_t k = next++;
                 if (trace->features) b.features.download(trace->features + k * 12 * 32 * C, 12 * 32 * C, e.stream);
@@ -730,23 +810,34 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
                     }
             }
         }
-    } else if ((flags & FACO_RUN_GRAPH) && batches >= 16) {
-        constexpr uint32_t kGraphIterations = 16;
-        e.drop_graph();
-        hipGraph_t g = nullptr;
-        hipGraphExec_t exec = nullptr;
-        checked(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        for (uint32_t i = 0; i < kGraphIterations; ++i) iteration();
-        checked(hipStreamEndCapture(e.stream, &g), "end capture");
-        checked(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "graph instantiate");
-        (void)hipGraphDestroy(g);
-        struct ExecGuard { hipGraphExec_t x; ~ExecGuard() { if (x) (void)hipGraphExecDestroy(x); } } guard{exec};
-        const uint64_t rounds = batches / kGraphIterations;
-        for (uint64_t r = 0; r < rounds; ++r) checked(hipGraphLaunch(exec, e.stream), "graph launch");
-        for (uint64_t i = rounds * kGraphIterations; i < batches; ++i) iteration();
-        checked(hipStreamSynchronize(e.stream), "search");
     } else {
-        for (uint64_t batch = 0; batch < batches; ++batch) iteration();
+        // segments between checkpoints; each replays a captured graph of 16
+        // batches where it can and runs the rest eagerly
+        constexpr uint32_t kGraphIterations = 16;
+        const bool graphs = (flags & FACO_RUN_GRAPH) != 0 && !events;
+        hipGraphExec_t exec = nullptr;
+        struct ExecGuard { hipGraphExec_t& x; ~ExecGuard() { if (x) (void)hipGraphExecDestroy(x); } } guard{exec};
+        for (uint64_t done = 0; done < batches;) {
+            const uint64_t stop = std::max(marks.next(batches), done + 1);
+            uint64_t left = stop - done;
+            if (graphs && left >= kGraphIterations) {
+                if (!exec) {
+                    e.drop_graph();
+                    hipGraph_t g = nullptr;
+                    checked(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                    for (uint32_t i = 0; i < kGraphIterations; ++i) iteration();
+                    checked(hipStreamEndCapture(e.stream, &g), "end capture");
+                    checked(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "graph instantiate");
+                    (void)hipGraphDestroy(g);
+                }
+                for (; left >= kGraphIterations; left -= kGraphIterations)
+                    checked(hipGraphLaunch(exec, e.stream), "graph launch");
+            }
+            for (; left; --left) iteration();
+            done = stop;
+            marks.record(e.global.data(), done, e.stream);
+        }
+        checked(hipStreamSynchronize(e.stream), "search");
     }
     checked(hipEventRecord(e.ev_stop, e.stream), "event");
     checked(hipStreamSynchronize(e.stream), "search");
@@ -762,6 +853,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     if (tours && e.mixed)
         for (uint32_t c = 0; c < C; ++c)
             for (uint32_t i = e.problem(inst[c]).n; i < e.n; ++i) tours[static_cast<size_t>(c) * e.n + i] = UINT32_MAX;
+    marks.read_back(e.stream, e.n, [&](uint32_t c) { return e.problem(inst[c]).n; });
     uint64_t steps = 0, evals = 0;
     for (uint32_t c = 0; c < C; ++c) {
         if (costs) costs[c] = st[c].global_cost;
@@ -776,6 +868,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     }
     fill_result(result, e, C, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
                 seconds_since(started), search, gpu_ms);
+    if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
 }
 
 }  // namespace
@@ -1052,7 +1145,8 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t key, uint32_t* primar
 
 int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
                               uint32_t tasks, uint64_t limit, uint32_t mne, uint32_t flags,
-                              faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work) {
+                              faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
+                              const faco_checkpoints* checkpoints) {
     return guarded([&] {
         require(engine && instance_keys && seeds, "null argument");
         Engine& e = *engine->impl;
@@ -1061,18 +1155,19 @@ int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys
             require(limit % e.config.ants == 0 && limit / e.config.ants <= UINT32_MAX,
                     "evaluation limit must be a whole number of ant batches");
         evaluate(e, instance_keys, seeds, tasks, limit / e.config.ants, mne, flags, result, tours,
-                 costs, work, limit);
+                 costs, work, limit, false, checkpoints);
     });
 }
 
 int faco_engine_evaluate_control(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
                                  uint32_t panel, uint64_t limit, const faco_control_spec* spec, uint32_t flags,
                                  const faco_control_trace* trace, faco_result* result, uint32_t* tours,
-                                 int64_t* costs, uint64_t* work, faco_control_state* states) {
+                                 int64_t* costs, uint64_t* work, faco_control_state* states,
+                                 const faco_checkpoints* checkpoints) {
     return guarded([&] {
         require(engine && instance_keys && seeds && spec, "null argument");
         evaluate_control(*engine->impl, instance_keys, seeds, panel, limit, *spec, flags, trace, result, tours, costs,
-                         work, states);
+                         work, states, checkpoints);
     });
 }
 

@@ -19,7 +19,8 @@ import time
 import numpy as np
 
 from . import _native
-from ._native import (BaselinePolicy, ControlSpec, ControlState, ControlTrace, Controller, Program, Result, Settings,
+from ._native import (BaselinePolicy, Checkpoints, ControlSpec, ControlState, ControlTrace, Controller, Program, Result,
+                      Settings,
                       TuneEntry, check, lib, ptr)
 
 numeric_backend = "euc2d_int32"
@@ -303,6 +304,52 @@ def cuda_device_info() -> dict:
             "compute_units": cus.value, "total_memory": mem.value, "max_lds_per_block": lds.value}
 
 
+class _CheckpointRequest:
+    """checkpoint_iterations of the count entry points (reference
+    cuda/batch_engine.cu:635-653, bindings.cpp:308-321): the global-best tour
+    of every colony after each listed number of completed batches. The engine
+    checks order and range; a row of a smaller mixed-size colony ends at its
+    own n."""
+
+    def __init__(self, points, colonies, dimension):
+        points = list(points)
+        if any(type(v) is not int or v < 0 or v >= 1 << 64 for v in points):
+            raise ValueError("checkpoint iterations must be non-negative integers")
+        self.points = points
+        self.iterations = np.ascontiguousarray(points, dtype=np.uint64)
+        self.tours = np.zeros((len(points), colonies, dimension), dtype=np.uint32)
+        self.struct = Checkpoints()
+        self.struct.iterations = self.iterations.ctypes.data_as(ctypes.c_void_p)
+        self.struct.count = len(points)
+        self.struct.tours = self.tours.ctypes.data_as(ctypes.c_void_p)
+
+    def argument(self):
+        return ctypes.byref(self.struct) if self.points else None
+
+    def output(self) -> list:
+        out = []
+        for k, iterations in enumerate(self.points):
+            tours = []
+            for row in self.tours[k]:
+                end = np.flatnonzero(row == 0xFFFFFFFF)
+                tours.append(row[: end[0] if end.size else row.size].tolist())
+            out.append({"iterations": iterations, "tours": tours})
+        return out
+
+
+def _production_stages(km, controlled: bool) -> dict:
+    """production_kernel_milliseconds under the reference's stage names
+    (cpp/include/gp_faco/profiling.hpp:11-19). Stages that run fused in one
+    kernel here report under the first name of their group and 0.0 under the
+    others: decision = features_and_regions (+ gp_scoring, action_and_restart),
+    colony update = reduction (+ pheromone), archive update =
+    archive_and_feedback (+ ant_fingerprints)."""
+    return {"initialization": km[4] if controlled else 0.0,
+            "features_and_regions": km[2] if controlled else 0.0, "gp_scoring": 0.0, "action_and_restart": 0.0,
+            "construction_and_ls": km[0], "reduction": km[1], "pheromone": 0.0, "ant_fingerprints": 0.0,
+            "archive_and_feedback": km[3] if controlled else 0.0}
+
+
 def _batch_output(result: Result, tours: np.ndarray, costs: np.ndarray, mode: str, elapsed: float) -> dict:
     items = [{"has_incumbent": True, "tour": tours[i].tolist(), "cost": float(costs[i]),
               "completed_seconds": elapsed} for i in range(tours.shape[0])]
@@ -477,21 +524,25 @@ class FacoBatchEngine:
             raise ValueError("FACO count entry needs complete task arrays")
         if preparation_mode not in ("cached", "end_to_end"):
             raise ValueError("unknown preparation mode")
-        if checkpoint_iterations:
-            raise ValueError("checkpoint iterations are not available in this engine")
         limit = _unsigned(evaluation_limit_per_colony, "evaluation_limit_per_colony")
         mne = _unsigned(mne, "mne", 32)
         tasks = keys.size
+        request = _CheckpointRequest(checkpoint_iterations, tasks, self.dimension)
         tours = np.zeros((tasks, self.dimension), dtype=np.uint32)
         costs = np.zeros(tasks, dtype=np.int64)
         work = np.zeros((tasks, 2), dtype=np.uint64)
         result = Result()
         started = time.perf_counter()
-        flags = (1 if use_graph else 0) | (2 if profile_kernels else 0) | (4 if time_kernels else 0)
+        flags = (1 if use_graph else 0) | (2 if profile_kernels else 0) | (4 if time_kernels or profile_events_only else 0)
         check(lib().faco_engine_evaluate_faco(self._handle, keys, seeds, tasks, limit, mne, flags,
-                                              ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1)))
+                                              ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1),
+                                              request.argument()))
         out = _batch_output(result, tours, costs, preparation_mode, time.perf_counter() - started)
         out["colony_work"] = work
+        if request.points:
+            out["checkpoints"] = request.output()
+        if profile_events_only:
+            out["production_kernel_milliseconds"] = _production_stages(result.kernel_milliseconds, False)
         if profile_kernels or time_kernels:
             out["kernel_milliseconds"] = {"construction_and_search": result.kernel_milliseconds[0],
                                           "reduction_and_pheromone": result.kernel_milliseconds[1]}
@@ -515,12 +566,16 @@ class FacoBatchEngine:
 
     def evaluate_control_raw(self, instance_keys, seeds, evaluation_limit_per_colony, mode, *, programs=(),
                              controllers=(), baseline=None, variant=3, experiment_mask=0xFFFFFFFF,
-                             trace_batches=(), use_graph=True, force_fingerprint_collisions=False) -> dict:
+                             trace_batches=(), use_graph=True, force_fingerprint_collisions=False,
+                             checkpoint_iterations=(), profile_events_only=False) -> dict:
         """Controlled run through faco_engine_evaluate_control; arrays out.
 
         mode: "program", "controller", "baseline" or "factorial"; colony
         c = d * panel + t runs task t under decider d. trace_batches
-        (ascending, 0-based) selects the batches whose decisions are kept."""
+        (ascending, 0-based) selects the batches whose decisions are kept;
+        checkpoint_iterations records the global-best tours after those
+        numbers of completed batches; profile_events_only times each launch
+        with HIP events (eager launches, same results)."""
         keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
         if keys.size != seeds.size:
             raise ValueError("controlled entry points need complete task arrays")
@@ -550,6 +605,7 @@ class FacoBatchEngine:
         work = np.zeros((C, 2), dtype=np.uint64)
         states = (ControlState * C)()
         result = Result()
+        request = _CheckpointRequest(checkpoint_iterations, C, self.dimension)
         batches = np.ascontiguousarray(sorted(trace_batches), dtype=np.uint32)
         k = batches.size
         traces = {}
@@ -568,11 +624,14 @@ class FacoBatchEngine:
                 setattr(trace, name, array.ctypes.data_as(ctypes.c_void_p))
         started = time.perf_counter()
         check(lib().faco_engine_evaluate_control(self._handle, keys, seeds, panel, limit, ctypes.byref(spec),
-                                                 1 if use_graph else 0, ctypes.byref(trace) if trace else None,
+                                                 (1 if use_graph else 0) | (4 if profile_events_only else 0),
+                                                 ctypes.byref(trace) if trace else None,
                                                  ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1),
-                                                 ctypes.addressof(states)))
+                                                 ctypes.addressof(states), request.argument()))
         del prog_arr, ctrl_arr
         return {"result": result, "tours": tours, "costs": costs, "work": work, "elapsed": time.perf_counter() - started,
+                "checkpoints": request.output() if request.points else None,
+                "profile_events_only": bool(profile_events_only),
                 "states": [{"archive_size": st.archive_size, "stagnant_batches": st.stagnant_batches,
                             "return_rate": st.return_rate, "ls_work": st.ls_work, "epoch_batches": st.epoch_batches,
                             "restarts": st.restarts} for st in states],
@@ -589,6 +648,10 @@ class FacoBatchEngine:
     def _control_output(self, raw, decision_iterations, kind) -> dict:
         out = _batch_output(raw["result"], raw["tours"], raw["costs"], "cached", raw["elapsed"])
         out["control_states"] = raw["states"]
+        if raw.get("checkpoints"):
+            out["checkpoints"] = raw["checkpoints"]
+        if raw.get("profile_events_only"):
+            out["production_kernel_milliseconds"] = _production_stages(raw["result"].kernel_milliseconds, True)
         if decision_iterations:
             rows = []
             C = raw["tours"].shape[0]
@@ -615,12 +678,14 @@ class FacoBatchEngine:
             raise ValueError("count entry needs an integer limit and mask")
         if preparation_mode not in ("cached", "end_to_end"):
             raise ValueError("count entry takes cached or end_to_end preparation")
-        if record_behavior or checkpoint_iterations or profile:
-            raise ValueError("behaviour rows, checkpoints and profiles are not available in this engine")
+        if record_behavior or profile:
+            raise ValueError("behaviour rows and profiles are not available in this engine")
         p = read_program(program)
         batches = self._decision_batches(decision_iterations, evaluation_limit_per_colony, self.settings.ants)
         raw = self.evaluate_control_raw(instance_keys, seeds, evaluation_limit_per_colony, "program", programs=[p],
-                                        experiment_mask=experiment_mask, trace_batches=batches, use_graph=use_graph)
+                                        experiment_mask=experiment_mask, trace_batches=batches, use_graph=use_graph,
+                                        checkpoint_iterations=checkpoint_iterations,
+                                        profile_events_only=profile_events_only)
         return self._control_output(raw, list(decision_iterations), "program")
 
     def evaluate_controller_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, controller,
@@ -638,13 +703,15 @@ class FacoBatchEngine:
                                       decision_iterations=(), use_graph=True) -> dict:
         if preparation_mode not in ("cached", "end_to_end"):
             raise ValueError("count entry takes cached or end_to_end preparation")
-        if record_behavior or checkpoint_iterations or profile:
-            raise ValueError("behaviour rows, checkpoints and profiles are not available in this engine")
+        if record_behavior or profile:
+            raise ValueError("behaviour rows and profiles are not available in this engine")
         b = read_baseline(policy)
         batches = self._decision_batches(decision_iterations, evaluation_limit_per_colony, self.settings.ants)
         raw = self.evaluate_control_raw(instance_keys, seeds, evaluation_limit_per_colony, "baseline", baseline=b,
                                         experiment_mask=_unsigned(experiment_mask, "experiment_mask", 32),
-                                        trace_batches=batches, use_graph=use_graph)
+                                        trace_batches=batches, use_graph=use_graph,
+                                        checkpoint_iterations=checkpoint_iterations,
+                                        profile_events_only=profile_events_only)
         out = self._control_output(raw, list(decision_iterations), "fixed")
         out["baseline_policy"] = baseline_output(b)
         return out
@@ -687,15 +754,17 @@ class FacoBatchEngine:
             if factorial_policy is not None:
                 raise ValueError("controller populations take no factorial policy")
             raw = self.evaluate_control_raw(keys, seeds, evaluation_limit_per_colony, "controller",
-                                            controllers=controllers, experiment_mask=mask, use_graph=use_graph)
+                                            controllers=controllers, experiment_mask=mask, use_graph=use_graph,
+                                            profile_events_only=profile_events_only)
         elif factorial_policy is not None:
             raw = self.evaluate_control_raw(keys, seeds, evaluation_limit_per_colony, "factorial",
                                             programs=[c.trees[0] for c in controllers], baseline=b, variant=variant,
-                                            experiment_mask=mask, use_graph=use_graph)
+                                            experiment_mask=mask, use_graph=use_graph,
+                                            profile_events_only=profile_events_only)
         else:
             raw = self.evaluate_control_raw(keys, seeds, evaluation_limit_per_colony, "program",
                                             programs=[c.trees[0] for c in controllers], experiment_mask=mask,
-                                            use_graph=use_graph)
+                                            use_graph=use_graph, profile_events_only=profile_events_only)
         count, panel = len(programs), keys.size
         r = raw["result"]
         outputs = []
@@ -716,6 +785,9 @@ class FacoBatchEngine:
                 entry["completed_seconds"] = raw["elapsed"] / count
             if factorial_policy is not None:
                 item["factorial_policy"] = factorial_output(variant, b)
+            if profile_events_only:   # one launch sequence for the population: each individual its equal share
+                item["production_kernel_milliseconds"] = {
+                    k: v / count for k, v in _production_stages(r.kernel_milliseconds, True).items()}
             outputs.append(item)
         return outputs
 

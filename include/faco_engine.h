@@ -295,15 +295,30 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_
 #define FACO_RUN_EVENTS 4u  /* eager launches of the production kernels with HIP events around
                                each: kernel_milliseconds[0] and [1] as for FACO_RUN_PROFILE */
 
+/* Curve checkpoints (reference cuda/batch_engine.cu:635-653, the
+ * checkpoint_iterations argument of bindings.cpp:701-757,818-853): the
+ * global-best tour of every colony after iterations[i] completed batches
+ * (0 = after initialisation), written to tours[i][colonies][n]. The
+ * iterations are strictly ascending and at most the run's batch count; a
+ * mixed-size colony that has finished its own budget keeps its final tour
+ * and its row ends in UINT32_MAX padding after its own n. */
+typedef struct faco_checkpoints {
+    const uint64_t* iterations;
+    uint32_t count;
+    uint32_t pad;
+    uint32_t* tours;
+} faco_checkpoints;
+
 /* evaluate_faco_evaluations(keys, seeds, evaluation_limit_per_colony, mne)
  * (reference batch_engine.hpp:87-89, bindings.cpp:701-720): plain FACO on
  * `tasks` colonies for evaluation_limit / ants batches. Outputs per colony
  * the global-best tour [tasks][n] and cost [tasks]; `work` (may be NULL)
- * receives [tasks][2] construction steps and LS evaluations. */
+ * receives [tasks][2] construction steps and LS evaluations; `checkpoints`
+ * (may be NULL) records the curve. */
 int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
                               uint32_t tasks, uint64_t evaluation_limit_per_colony, uint32_t mne_target,
                               uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs,
-                              uint64_t* work);
+                              uint64_t* work, const faco_checkpoints* checkpoints);
 
 /* Controlled FACO: the GP-controlled restart and perturbation loop.
  * One entry point for the reference's evaluate_program_evaluations,
@@ -318,12 +333,18 @@ int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys
  * (cuda/control_state.cuh). Outputs per colony: global tour [C][n], cost
  * [C], work [C][2] (may be NULL) and control summary [C] (may be NULL).
  * flags: FACO_RUN_GRAPH as for faco_engine_evaluate_faco (ignored while a
- * trace is taken). */
+ * trace is taken); FACO_RUN_EVENTS (reference profile_events_only,
+ * bindings.cpp:721-757): eager launches with HIP events between them,
+ * result->kernel_milliseconds [0] construction + LS, [1] colony update
+ * (reduction and pheromone), [2] decision (features, regions, scoring, action
+ * and restart), [3] archive and feedback update, [4] initialisation, summed
+ * over batches; results are unchanged. `checkpoints` (may be NULL) as for
+ * faco_engine_evaluate_faco, over the C colonies. */
 int faco_engine_evaluate_control(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
                                  uint32_t panel, uint64_t evaluation_limit_per_colony,
                                  const faco_control_spec* spec, uint32_t flags, const faco_control_trace* trace,
                                  faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
-                                 faco_control_state* states);
+                                 faco_control_state* states, const faco_checkpoints* checkpoints);
 
 /* score_controller_cuda (reference cpp/src/bindings.cpp:1010-1030,
  * cuda/gp_score.cuh:37-62) and its host twin: features FP32 [12][colonies][32],
