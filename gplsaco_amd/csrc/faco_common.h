@@ -156,9 +156,11 @@ FACO_HD void edge_identity(uint32_t a, uint32_t b, uint64_t& first, uint64_t& se
 }
 
 // EUC_2D on packed coordinates (x | y << 16), both in [0, 32767]:
-// s = dx^2 + dy^2 < 2^31, r = isqrt(s), d = r + (s > r^2 + r).
-// The float estimate of sqrt(s) is within 0.01 of the true root, so one
-// correction step each way gives the exact integer root.
+// d = nint(sqrt(s)) with s = dx^2 + dy^2 < 2^31, i.e. the r >= 0 with
+// r^2 - r < s <= r^2 + r (r = 0 only for s = 0).
+// Device: the float root of s is within 0.01 of sqrt(s), so r0 =
+// floor(f + 0.5) is within one of d and one comparison each way (both
+// against r0) gives d; for r0 = 0 (s = 0) the lower test wraps to false.
 // Every product below fits 24 bits of operand (|dx|, |dy| <= 32767 and
 // r <= 46341), so the device uses full-rate 24-bit multiplies.
 FACO_HD uint32_t euc2d(uint32_t a, uint32_t b) {
@@ -166,10 +168,9 @@ FACO_HD uint32_t euc2d(uint32_t a, uint32_t b) {
     const int dy = static_cast<int>(a >> 16) - static_cast<int>(b >> 16);
 #if defined(__HIP_DEVICE_COMPILE__)
     const uint32_t s = static_cast<uint32_t>(__mul24(dx, dx)) + static_cast<uint32_t>(__mul24(dy, dy));
-    uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_sqrtf(static_cast<float>(s)));
-    r -= (__umul24(r, r) > s) ? 1u : 0u;
-    r += (__umul24(r + 1, r + 1) <= s) ? 1u : 0u;
-    return r + ((s > __umul24(r, r) + r) ? 1u : 0u);
+    const uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_sqrtf(static_cast<float>(s)) + 0.5f);
+    const uint32_t t = __umul24(r, r);
+    return r + ((s > t + r) ? 1u : 0u) - ((t > s + r - 1u) ? 1u : 0u);
 #else
     const uint32_t s = static_cast<uint32_t>(dx * dx) + static_cast<uint32_t>(dy * dy);
     uint32_t r = static_cast<uint32_t>(__builtin_sqrtf(static_cast<float>(s)));

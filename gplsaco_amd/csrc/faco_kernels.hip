@@ -252,7 +252,6 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         const uint32_t target = explicit_mode ? p.targets[ant] : p.state[colony].target;
         const uint32_t* visits = p.visits + static_cast<size_t>(ant) * ns;
         uint32_t* checklist_log = diag_checklist ? diag_checklist + static_cast<size_t>(ant) * 5u * ns : nullptr;
-        int64_t cost = per_ant_parent ? p.parent_cost[ant] : p.state[colony].parent_cost;
         uint32_t current;
         if (explicit_mode) {
             current = visits[0];
@@ -276,7 +275,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         wave_sync();
 
         uint32_t steps = 0, mne = 0, relocations = 0, exhausted = 0;
-        uint32_t qhead = 0, qtail = 0, logged = 0;
+        // FIFO checklist: a circular buffer of n entries (a city is queued at
+        // most once at a time); qhead / qtail are its positions, qsize its
+        // length and logged the number of pushes so far.
+        uint32_t qhead = 0, qtail = 0, qsize = 0, logged = 0;
         const uint64_t tick1 = Profile ? wall_clock64() : 0;
 
         // Append `node` to the FIFO checklist unless it is already pending
@@ -286,10 +288,11 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             if (f & kQueued) return 0u;
             if (lane == 0) {
                 flags[node] = static_cast<uint8_t>(f | kQueued);
-                queue[qtail % n] = static_cast<uint16_t>(node);
+                queue[qtail] = static_cast<uint16_t>(node);
                 if (checklist_log) checklist_log[logged] = node;
             }
-            ++qtail;
+            qtail = qtail + 1 == n ? 0u : qtail + 1;
+            ++qsize;
             ++logged;
             wave_sync();
             return 1u;
@@ -471,18 +474,6 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             cur_link = current | (after << 16);   // selected's links when the move is the identity
             if (target_after != selected) {
                 ++relocations;
-                // Six edge lengths of the relocation, one per lane:
-                // + (prev,after) + (current,selected) + (selected,target_after)
-                // - (prev,selected) - (selected,after) - (current,target_after).
-                const uint32_t e0 = lane == 0 || lane == 3 ? old_prev : (lane == 1 || lane == 5 ? current : selected);
-                const uint32_t e1 = lane == 0 || lane == 4 ? after : (lane == 1 || lane == 3 ? selected : target_after);
-                const int32_t d = lane < 6 ? static_cast<int32_t>(euc2d(s_xy[e0], s_xy[e1])) : 0;
-                cost += static_cast<int64_t>(static_cast<int32_t>(read_lane(static_cast<uint32_t>(d), 0))
-                        + static_cast<int32_t>(read_lane(static_cast<uint32_t>(d), 1))
-                        + static_cast<int32_t>(read_lane(static_cast<uint32_t>(d), 2))
-                        - static_cast<int32_t>(read_lane(static_cast<uint32_t>(d), 3))
-                        - static_cast<int32_t>(read_lane(static_cast<uint32_t>(d), 4))
-                        - static_cast<int32_t>(read_lane(static_cast<uint32_t>(d), 5)));
                 if (linked) {
                     // Unlink `selected`, relink it between `current` and its
                     // successor; the array's first city moves on when it is
@@ -546,6 +537,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             if constexpr (Profile) sub[2] += clock64() - mark;
         }
 
+        // Length of the constructed tour: the sum of its edges, one per city
+        // and lane (the reference updates it per relocation; the integer sum
+        // is the same).
+        uint32_t edge_sum = 0;
         if (linked) {
             // Array position of every city = its distance from `head` along
             // predecessor links: list ranking by pointer jumping, each lane
@@ -560,7 +555,11 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 #pragma unroll
             for (uint32_t k = 0; k < kPer; ++k) {
                 const uint32_t v = lane + k * kWave;
-                if (k < chunks && v < n) e[k] = v == head ? (head << 16) : (1u | ((lnk[v] & 0xffffu) << 16));
+                if (k < chunks && v < n) {
+                    const uint32_t pred = lnk[v] & 0xffffu;
+                    edge_sum += euc2d(s_xy[pred], s_xy[v]);
+                    e[k] = v == head ? (head << 16) : (1u | (pred << 16));
+                }
             }
             wave_sync();
 #pragma unroll
@@ -603,7 +602,12 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 }
             }
             wave_sync();
+        } else {
+            for (uint32_t i = lane; i < n; i += kWave) edge_sum += euc2d(s_xy[tour[i]], s_xy[tour[i + 1 == n ? 0 : i + 1]]);
         }
+#pragma unroll
+        for (int offset = 32; offset; offset >>= 1) edge_sum += __shfl_xor(edge_sum, offset);
+        const int64_t cost = uniform(edge_sum);
         if (diag_construction) {
             for (uint32_t i = lane; i < n; i += kWave) diag_construction[static_cast<size_t>(ant) * ns + i] = tour[i];
         }
@@ -647,11 +651,12 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         };
         uint32_t processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0, limit_hit = 0;
         int64_t gain_sum = 0;
-        while (qhead != qtail && accepted < n) {
+        while (qsize != 0 && accepted < n) {
             if (evaluations >= limit) { limit_hit = 1; break; }
             if constexpr (Profile) mark = clock64();
-            const uint32_t a = queue[qhead % n];
-            ++qhead;
+            const uint32_t a = queue[qhead];
+            qhead = qhead + 1 == n ? 0u : qhead + 1;
+            --qsize;
             if (lane == 0) flags[a] = static_cast<uint8_t>(flags[a] & ~kQueued);
             ++processed;
             const uint32_t b = scan_lane ? s_nbr[a * K + cslot] : n;

@@ -31,9 +31,15 @@ def test_device_distances_exact(ext, oracle_mod):
     pairs = rng.integers(0, 32768, size=(20000, 4))
     edge = np.array([[0, 0, 32767, 32767], [0, 0, 0, 0], [5, 5, 5, 5], [0, 0, 3, 4], [0, 0, 1, 1],
                      [0, 0, 2, 2], [0, 0, 7, 24], [32767, 0, 0, 32767], [100, 100, 103, 104]])
-    # s = k^2 + k (rounds down) and k^2 + k + 1 (rounds up)
+    # perfect squares s = k^2
     k = rng.integers(1, 23000, size=2000)
-    pairs = np.concatenate([pairs, edge, np.stack([0 * k, 0 * k, k, 0 * k], 1)]).astype(np.int32)
+    # the rounding boundaries: s = R^2 + R rounds down to R (dx = m^2, dy = m,
+    # R = m^2) and s = R^2 + R + 1 up to R + 1 (dx = m^2 - 1, dy = m, R = m^2 - 1)
+    m = np.arange(1, 182)
+    zero = 0 * m
+    boundary = np.concatenate([np.stack([zero, zero, m * m, m], 1), np.stack([zero, zero, m * m - 1, m], 1),
+                               np.stack([32767 - m * m, 32767 - m, zero + 32767, zero + 32767], 1)])
+    pairs = np.concatenate([pairs, edge, np.stack([0 * k, 0 * k, k, 0 * k], 1), boundary]).astype(np.int32)
     announce("distances", pairs.shape[0])
     got = ext.diagnostic_distances(pairs)
     d = pairs[:, :2].astype(np.int64) - pairs[:, 2:].astype(np.int64)
