@@ -81,12 +81,13 @@ __device__ __forceinline__ uint32_t round16(uint32_t v) { return (v + 15u) & ~15
 // Wave-wide maximum of a u32 with DPP row operations (quad swaps, half-row
 // and row mirrors) and one scalar combine of the four 16-lane rows.
 __device__ __forceinline__ uint32_t dpp_max_u32(uint32_t v) {
-    int x = static_cast<int>(v);
+    // every lane has a source lane in these patterns, so the moves need no
+    // old value and fold into the max instructions
     uint32_t o;
-    o = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false)); v = o > v ? o : v; x = static_cast<int>(v);
-    o = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false)); v = o > v ? o : v; x = static_cast<int>(v);
-    o = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false)); v = o > v ? o : v; x = static_cast<int>(v);
-    o = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false)); v = o > v ? o : v;
+    o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0xB1, 0xF, 0xF, false)); v = o > v ? o : v;
+    o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x4E, 0xF, 0xF, false)); v = o > v ? o : v;
+    o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x141, 0xF, 0xF, false)); v = o > v ? o : v;
+    o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x140, 0xF, 0xF, false)); v = o > v ? o : v;
     const uint32_t r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32), r3 = read_lane(v, 48);
     const uint32_t a = r0 > r1 ? r0 : r1, b = r2 > r3 ? r2 : r3;
     return a > b ? a : b;
@@ -618,6 +619,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         // Lanes [0, lw) scan successor moves (kind 0), lanes [32, 32 + lw)
         // predecessor moves (kind 1); slot j of kind k has tie index k*lw + j.
         const uint32_t kind = lane >> 5, cslot = lane & 31u;
+        const uint32_t kind_mask = 0u - kind;   // all ones in kind 1 lanes
         const bool scan_lane = cslot < lw;
         const uint32_t tie_index = kind * lw + cslot;
         // Reverse the tour between cities u and v (2-opt flip on the array,
@@ -657,7 +659,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             const uint32_t a = queue[qhead];
             qhead = qhead + 1 == n ? 0u : qhead + 1;
             --qsize;
-            if (lane == 0) flags[a] = static_cast<uint8_t>(flags[a] & ~kQueued);
+            // only the queued bit matters after construction: clear the byte
+            if (lane == 0) flags[a] = 0;
             ++processed;
             const uint32_t b = scan_lane ? s_nbr[a * K + cslot] : n;
             const uint32_t pa = pos[a];
@@ -667,7 +670,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             const uint32_t a_next = tour[pa + 1 == n ? 0 : pa + 1];
             const uint32_t a_prev = tour[pa == 0 ? n - 1 : pa - 1];
             // kind 0 lanes hold d(a, succ a), kind 1 lanes d(pred a, a)
-            const uint32_t closing = kind ? a_prev : a_next;
+            const uint32_t closing = (a_prev & kind_mask) | (a_next & ~kind_mask);
             const uint32_t xclosing = s_xy[closing];
             const int32_t current_edge = static_cast<int32_t>(euc2d(xa, xclosing));
             const int32_t dab = scan_lane && b < n ? static_cast<int32_t>(euc2d(xa, xb)) : 0;
