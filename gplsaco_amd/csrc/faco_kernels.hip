@@ -129,7 +129,7 @@ __device__ __forceinline__ double row_scan4(double v) {
     return acc;
 }
 
-template<bool NbrLds, bool Profile, bool Diag>
+template<bool NbrLds, bool Profile, bool Diag, bool OrOpt>
 __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     // Diagnostic features (explicit visits, per-ant parents, traces and
     // logs) exist only in the Diag instantiation: the production kernel
@@ -620,6 +620,12 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         // predecessor moves (kind 1); slot j of kind k has tie index k*lw + j.
         const uint32_t kind = lane >> 5, cslot = lane & 31u;
         const uint32_t kind_mask = 0u - kind;   // all ones in kind 1 lanes
+        // Lane constants of the branch-free scan: an in-row column for every
+        // lane, the step to a neighbour's tour neighbour (+1 successor, -1
+        // predecessor) and where a step off either array end lands.
+        const uint32_t column = cslot < K ? cslot : K - 1u;
+        const uint32_t step_dir = kind ? 0xffffffffu : 1u;
+        const uint32_t wrap_to = kind ? n - 1u : 0u;
         const bool scan_lane = cslot < lw;
         const uint32_t tie_index = kind * lw + cslot;
         // Reverse the tour between cities u and v (2-opt flip on the array,
@@ -662,19 +668,25 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             // only the queued bit matters after construction: clear the byte
             if (lane == 0) flags[a] = 0;
             ++processed;
-            const uint32_t b = scan_lane ? s_nbr[a * K + cslot] : n;
+            // every lane reads (in-bounds addresses, no exec masks); lanes
+            // outside the scan or past the row end see b = n and count as
+            // failing, their other values are not used
+            const uint32_t row_b = s_nbr[a * K + column];
+            const uint32_t b = scan_lane ? row_b : n;
+            const bool real = b < n;
+            const uint32_t bi = real ? b : a;
             const uint32_t pa = pos[a];
             const uint32_t xa = s_xy[a];
-            const uint32_t xb = scan_lane && b < n ? s_xy[b] : 0u;
-            const uint32_t pb = scan_lane && b < n ? pos[b] : 0u;
+            const uint32_t xb = s_xy[bi];
+            const uint32_t pb = pos[bi];
             const uint32_t a_next = tour[pa + 1 == n ? 0 : pa + 1];
             const uint32_t a_prev = tour[pa == 0 ? n - 1 : pa - 1];
             // kind 0 lanes hold d(a, succ a), kind 1 lanes d(pred a, a)
             const uint32_t closing = (a_prev & kind_mask) | (a_next & ~kind_mask);
             const uint32_t xclosing = s_xy[closing];
             const int32_t current_edge = static_cast<int32_t>(euc2d(xa, xclosing));
-            const int32_t dab = scan_lane && b < n ? static_cast<int32_t>(euc2d(xa, xb)) : 0;
-            const bool bad = !scan_lane || b >= n || !(current_edge > dab);
+            const int32_t dab = static_cast<int32_t>(euc2d(xa, xb));
+            const bool bad = !real || !(current_edge > dab);
             // First failing slot per kind; lanes past lw count as failing.
             const uint64_t bad_mask = ballot(bad);
             const uint32_t bad0 = static_cast<uint32_t>(bad_mask), bad1 = static_cast<uint32_t>(bad_mask >> 32);
@@ -695,15 +707,13 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             if (used0 + used1 == 0) continue;
 
             const bool active = scan_lane && cslot < (kind ? used1 : used0);
-            uint32_t nb = 0;
-            int32_t gain = 0;
-            if (active) {
-                nb = kind ? tour[pb == 0 ? n - 1 : pb - 1] : tour[pb + 1 == n ? 0 : pb + 1];
-                const uint32_t xn = s_xy[nb];
-                gain = (nb == a || b == closing) ? 0
-                     : current_edge + static_cast<int32_t>(euc2d(xb, xn)) - dab
-                       - static_cast<int32_t>(euc2d(xclosing, xn));
-            }
+            uint32_t nidx = pb + step_dir;
+            nidx = nidx >= n ? wrap_to : nidx;
+            const uint32_t nb = tour[nidx];
+            const uint32_t xn = s_xy[nb];
+            const int32_t move_gain = current_edge + static_cast<int32_t>(euc2d(xb, xn)) - dab
+                                    - static_cast<int32_t>(euc2d(xclosing, xn));
+            const int32_t gain = (nb == a || b == closing) ? 0 : move_gain;
             // key: larger gain first, then lower tie index; 0 = no candidate.
             const uint32_t key_value = active ? ((static_cast<uint32_t>(gain + (1 << 20)) << 6) | (63u - tie_index)) : 0u;
             const uint32_t best = uniform(dpp_max_u32(key_value));
@@ -715,7 +725,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             // one lane per b, six moves per lane in (L, forward first) order.
             uint32_t or_best = 0u;
             uint32_t s2 = 0, s3 = 0, nx3 = 0, or_succ_b = 0;
-            if (p.or_opt && used1 > 0) {
+            if (OrOpt && used1 > 0) {
                 s2 = a_next;
                 s3 = tour[pa + 2 >= n ? pa + 2 - n : pa + 2];
                 nx3 = tour[pa + 3 >= n ? pa + 3 - n : pa + 3];
@@ -1076,8 +1086,8 @@ int ant_search_blocks_per_cu(const AntParams& p) {
     const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     int blocks = 0;
     const hipError_t e = p.nbr_in_lds
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<true, false, false>, 64 * p.waves, lds)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<false, false, false>, 64 * p.waves, lds);
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<true, false, false, false>, 64 * p.waves, lds)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<false, false, false, false>, 64 * p.waves, lds);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
@@ -1086,16 +1096,20 @@ hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t st
     const dim3 grid(blocks), block(64 * p.waves);
     const bool diag = p.explicit_mode || p.per_ant_parent || p.trace_selected || p.diag_construction ||
                       p.diag_positions || p.diag_checklist;
+    // one instantiation per feature set, so the production kernel of plain
+    // 2-opt carries neither the Or-opt moves nor the diagnostic outputs
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p); };
+    const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
     if (diag) {
         if (p.phase_ticks) return hipErrorInvalidValue;   // no clock profile of diagnostic runs
-        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, false, true>), grid, block, lds, stream, p);
-        else hipLaunchKernelGGL((faco_ant_search<false, false, true>), grid, block, lds, stream, p);
+        if (nl) oo ? go(faco_ant_search<true, false, true, true>) : go(faco_ant_search<true, false, true, false>);
+        else oo ? go(faco_ant_search<false, false, true, true>) : go(faco_ant_search<false, false, true, false>);
     } else if (p.phase_ticks) {
-        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, true, false>), grid, block, lds, stream, p);
-        else hipLaunchKernelGGL((faco_ant_search<false, true, false>), grid, block, lds, stream, p);
+        if (nl) oo ? go(faco_ant_search<true, true, false, true>) : go(faco_ant_search<true, true, false, false>);
+        else oo ? go(faco_ant_search<false, true, false, true>) : go(faco_ant_search<false, true, false, false>);
     } else {
-        if (p.nbr_in_lds) hipLaunchKernelGGL((faco_ant_search<true, false, false>), grid, block, lds, stream, p);
-        else hipLaunchKernelGGL((faco_ant_search<false, false, false>), grid, block, lds, stream, p);
+        if (nl) oo ? go(faco_ant_search<true, false, false, true>) : go(faco_ant_search<true, false, false, false>);
+        else oo ? go(faco_ant_search<false, false, false, true>) : go(faco_ant_search<false, false, false, false>);
     }
     return hipGetLastError();
 }
