@@ -359,7 +359,7 @@ struct Engine {
         if (forced_blocks_per_colony && ants_split % forced_blocks_per_colony == 0) bpc = forced_blocks_per_colony;
         p.blocks_per_colony = bpc;
         p.chunk = config.ants / bpc;
-        p.ls_limit = config.ls_evaluation_limit;
+        p.ls_limit = std::min<uint64_t>(config.ls_evaluation_limit, UINT32_MAX);   // see AntParams::ls_limit
         p.roulette_margin = roulette_margin;
         p.or_opt = local_search;
         p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
@@ -1325,7 +1325,7 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         p.roulette_margin = 0x1.0p-46;
         require(local_search <= 1, "local search mode must be 0 or 1");
         p.or_opt = local_search;
-        p.ls_limit = limit;
+        p.ls_limit = std::min<uint64_t>(limit, UINT32_MAX);
         p.coords = d_coords.data(); p.nbr = d_nbr.data(); p.parent = d_parents.data(); p.links = d_links.data();
         p.parent_cost = d_costs.data(); p.targets = d_targets.data(); p.visits = d_visits.data();
         p.out_tours = d_tours.data(); p.out_info = d_info.data(); p.diag_construction = d_ctours.data();

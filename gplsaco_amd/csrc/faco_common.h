@@ -210,6 +210,10 @@ struct AntParams {
     uint32_t per_ant_parent;            // parent/links/cost indexed by ant, not colony
     uint32_t log_checklist;             // append every checklist push to diag_checklist
     uint32_t or_opt;                    // local search adds Or-opt segment moves
+    // LS evaluation limit, clamped to 2^32 - 1 by the host: per-ant counts
+    // stay far below 2^32 (at most 2 * lw per processed city, fewer than 8n
+    // processed cities), so the clamped limit stops exactly where the
+    // 64-bit one does
     uint64_t ls_limit;
     double roulette_margin;             // relative margin of the fast roulette (+inf: sequential sums always)
     const uint32_t* coords;             // [G][n] packed x | y << 16

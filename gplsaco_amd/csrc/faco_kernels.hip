@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     const uint32_t batch = explicit_mode ? 0u : p.state[colony].batch;
     const double* products = p.products + static_cast<size_t>(colony) * ns * pw;
     const uint16_t* backup = p.backup + static_cast<size_t>(geom) * ns * p.bw;
-    const uint64_t limit = p.ls_limit;
+    const uint32_t limit = static_cast<uint32_t>(p.ls_limit);   // clamped to 32 bits by the host
     const bool in_row = lane < pw;
     // Product-row loads are issued by every lane (lanes past the row re-read
     // its last slot), not under an exec mask: with no branch around them the
@@ -694,13 +694,11 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             const uint32_t prefix1 = bad1 ? min(static_cast<uint32_t>(__builtin_ctz(bad1)), lw) : lw;
             // Budget: kind 0 first, then kind 1; the limit check precedes every
             // evaluation and a node cut short commits nothing.
-            const uint64_t remaining0 = limit - evaluations;
-            const uint32_t used0 = remaining0 < prefix0 ? static_cast<uint32_t>(remaining0) : prefix0;
+            const uint32_t used0 = min(limit - evaluations, prefix0);
             evaluations += used0;
             checks += used0 + (used0 < lw ? 1u : 0u);
             if (used0 < prefix0) { limit_hit = 1; break; }
-            const uint64_t remaining1 = limit - evaluations;
-            const uint32_t used1 = remaining1 < prefix1 ? static_cast<uint32_t>(remaining1) : prefix1;
+            const uint32_t used1 = min(limit - evaluations, prefix1);
             evaluations += used1;
             checks += used1 + (used1 < lw ? 1u : 0u);
             if (used1 < prefix1) { limit_hit = 1; break; }
@@ -775,8 +773,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 #pragma unroll
                 for (int offset = 32; offset; offset >>= 1) total += __shfl_xor(total, offset);
                 total = uniform(total);
-                const uint64_t remaining = limit - evaluations;
-                if (total > remaining) { evaluations = static_cast<uint32_t>(limit); limit_hit = 1; break; }
+                if (total > limit - evaluations) { evaluations = limit; limit_hit = 1; break; }
                 evaluations += total;
                 or_best = uniform(dpp_max_u32(lane_key));
             }
