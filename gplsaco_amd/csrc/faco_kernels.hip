@@ -624,9 +624,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         // lane, the step to a neighbour's tour neighbour (+1 successor, -1
         // predecessor) and where a step off either array end lands.
         const uint32_t column = cslot < K ? cslot : K - 1u;
+        const bool scan_lane = cslot < lw;
+        const uint32_t off_scan = scan_lane ? 0u : 0xffffffffu;   // all ones outside the scan
         const uint32_t step_dir = kind ? 0xffffffffu : 1u;
         const uint32_t wrap_to = kind ? n - 1u : 0u;
-        const bool scan_lane = cslot < lw;
         const uint32_t tie_index = kind * lw + cslot;
         // Reverse the tour between cities u and v (2-opt flip on the array,
         // reference cpu flip_without_cost): as disjoint pair swaps of the
@@ -659,7 +660,9 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         };
         uint32_t processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0, limit_hit = 0;
         int64_t gain_sum = 0;
-        while (qsize != 0 && accepted < n) {
+        // The reference's loop condition (queue not empty and fewer than n
+        // accepted moves) with the move count tested where it changes.
+        while (qsize != 0) {
             if (evaluations >= limit) { limit_hit = 1; break; }
             if constexpr (Profile) mark = clock64();
             const uint32_t a = queue[qhead];
@@ -672,7 +675,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             // outside the scan or past the row end see b = n and count as
             // failing, their other values are not used
             const uint32_t row_b = s_nbr[a * K + column];
-            const uint32_t b = scan_lane ? row_b : n;
+            const uint32_t b = min(row_b | off_scan, n);   // row entries are at most n
             const bool real = b < n;
             const uint32_t bi = real ? b : a;
             const uint32_t pa = pos[a];
@@ -690,21 +693,24 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             // First failing slot per kind; lanes past lw count as failing.
             const uint64_t bad_mask = ballot(bad);
             const uint32_t bad0 = static_cast<uint32_t>(bad_mask), bad1 = static_cast<uint32_t>(bad_mask >> 32);
-            const uint32_t prefix0 = bad0 ? min(static_cast<uint32_t>(__builtin_ctz(bad0)), lw) : lw;
-            const uint32_t prefix1 = bad1 ? min(static_cast<uint32_t>(__builtin_ctz(bad1)), lw) : lw;
+            // lanes past lw always fail, so the first failing slot is at most lw
+            // (the extra bit 32 covers lw = 32)
+            const uint32_t prefix0 = ctz64(static_cast<uint64_t>(bad0) | (1ull << 32));
+            const uint32_t prefix1 = ctz64(static_cast<uint64_t>(bad1) | (1ull << 32));
             // Budget: kind 0 first, then kind 1; the limit check precedes every
             // evaluation and a node cut short commits nothing.
             const uint32_t used0 = min(limit - evaluations, prefix0);
             evaluations += used0;
-            checks += used0 + (used0 < lw ? 1u : 0u);
+            checks += min(used0 + 1u, lw);   // the failing slot counts as checked
             if (used0 < prefix0) { limit_hit = 1; break; }
             const uint32_t used1 = min(limit - evaluations, prefix1);
             evaluations += used1;
-            checks += used1 + (used1 < lw ? 1u : 0u);
+            checks += min(used1 + 1u, lw);
             if (used1 < prefix1) { limit_hit = 1; break; }
             if (used0 + used1 == 0) continue;
 
-            const bool active = scan_lane && cslot < (kind ? used1 : used0);
+            // evaluated slots of this lane's kind (lanes past lw never qualify)
+            const bool active = cslot < ((used1 & kind_mask) | (used0 & ~kind_mask));
             uint32_t nidx = pb + step_dir;
             nidx = nidx >= n ? wrap_to : nidx;
             const uint32_t nb = tour[nidx];
@@ -801,6 +807,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
                 reactivations += push(c);
                 reactivations += push(d);
                 if constexpr (Profile) sub[4] += clock64() - mark;
+                if (accepted >= n) break;
                 continue;
             }
             if (best == 0u) continue;
@@ -821,6 +828,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             reactivations += push(move[2]);
             reactivations += push(move[3]);
             if constexpr (Profile) sub[4] += clock64() - mark;
+            if (accepted >= n) break;
         }
 
         const uint64_t tick3 = Profile ? wall_clock64() : 0;
