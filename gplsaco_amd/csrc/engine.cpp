@@ -362,6 +362,7 @@ struct Engine {
         p.ls_limit = std::min<uint64_t>(config.ls_evaluation_limit, UINT32_MAX);   // see AntParams::ls_limit
         p.roulette_margin = roulette_margin;
         p.or_opt = local_search;
+        p.filter_tours = 1;   // plain FACO: only the iteration best tour is read back
         p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
         p.products = products.data(); p.parent = parent.data(); p.links = links.data();
         p.state = state.data(); p.keys = keys.data(); p.targets = nullptr;
@@ -738,6 +739,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     cp.shapes = e.shapes_active ? e.shapes.data() : nullptr;
 
     AntParams ap = e.ant_params(C);
+    ap.filter_tours = 0;   // the archive update reads every ant's tour
     ap.control = b.control.data();
     ap.out_identity = b.ant_identity.data();
     ap.out_zero = b.ant_zero.data();

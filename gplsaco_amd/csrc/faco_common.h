@@ -31,6 +31,7 @@ struct ColonyState {
     uint64_t work_construction, work_ls;
     uint32_t target;       // MNE target of this batch's ants
     uint32_t parent_zero;  // position of city 0 in the parent | (succ(0) < pred(0)) << 31
+    uint64_t best_key;     // smallest (final cost << 32 | ant) of this batch's ants so far; all ones between batches
 };
 
 // Per-colony control state of the GP / baseline controlled loop (the
@@ -205,11 +206,13 @@ struct AntParams {
     uint32_t nbr_lds_bytes;             // neighbour rows in LDS (16-byte rounded)
     uint32_t nbr_stride;                // u16 elements per geometry (nbr_lds_bytes / 2)
     uint32_t per_wave_bytes;            // per-ant LDS block
-    uint32_t xcd_remap;                 // map blocks so one colony stays on one XCD
+    uint32_t xcd_remap;                 // keep colony c's workgroups on XCD c % 8 (its colony update's XCD)
     uint32_t explicit_mode;             // visits[] drive construction (diagnostic)
     uint32_t per_ant_parent;            // parent/links/cost indexed by ant, not colony
     uint32_t log_checklist;             // append every checklist push to diag_checklist
     uint32_t or_opt;                    // local search adds Or-opt segment moves
+    uint32_t filter_tours;              // write an ant's tour only while it is the batch's best so far
+                                        // (plain FACO reads back the iteration best alone)
     // LS evaluation limit, clamped to 2^32 - 1 by the host: per-ant counts
     // stay far below 2^32 (at most 2 * lw per processed city, fewer than 8n
     // processed cities), so the clamped limit stops exactly where the
@@ -223,7 +226,7 @@ struct AntParams {
     const uint32_t* parent;             // [C][n] (or [A][n])
     const uint32_t* links;              // [C][n] parent predecessor | successor << 16
     const int64_t* parent_cost;         // [A] when per_ant_parent
-    const ColonyState* state;           // [C] (parent cost and batch counter)
+    ColonyState* state;                 // [C] (parent cost, batch counter, best key of the batch)
     const uint64_t* keys;               // [C]
     const uint32_t* targets;            // [A] MNE targets
     const uint32_t* geometry;           // [C] colony -> geometry, or null (all 0)

@@ -147,7 +147,15 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     const uint32_t wave = uniform(threadIdx.x >> 6);
     uint32_t blk = blockIdx.x;
     const uint32_t nblocks = gridDim.x;
-    if (p.xcd_remap && (nblocks & 7u) == 0) blk = (blk & 7u) * (nblocks >> 3) + (blk >> 3);
+    // Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD
+    // b % 8), and the per-colony kernels (colony update, reset, control) run
+    // colony c on XCD c % 8. The remap keeps every workgroup of colony c on
+    // that same XCD, so the colony's pheromone products written by the
+    // update stay in the L2 the ant kernel reads them from.
+    if (p.xcd_remap && (nblocks & 7u) == 0 && ((nblocks / p.blocks_per_colony) & 7u) == 0) {
+        const uint32_t x = blk & 7u, k = blk >> 3;
+        blk = (x + 8u * (k / p.blocks_per_colony)) * p.blocks_per_colony + k % p.blocks_per_colony;
+    }
     // One workgroup = one chunk of one colony's ants; its waves take ants
     // from the chunk dynamically so a slow ant does not idle the others. In a
     // mixed-size batch the block groups follow `order` (largest work first:
@@ -833,8 +841,22 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 
         const uint64_t tick3 = Profile ? wall_clock64() : 0;
         // ------------------------------------------------------------ write back
+        const int64_t final_cost = cost - gain_sum;
+        // The batch's best ant (smallest (cost, ant) key) always finds its key
+        // below every key published before it, so it always writes; an ant
+        // whose key is above one already published cannot be the best and
+        // skips the write.
+        bool write_tour = true;
+        if (!Diag && p.filter_tours) {
+            const uint64_t my_key = (static_cast<uint64_t>(final_cost) << 32) | local_ant;
+            uint64_t before = 0;
+            if (lane == 0) before = atomicMin(&p.state[colony].best_key, my_key);
+            const uint32_t lo = uniform(static_cast<uint32_t>(before)), hi = uniform(static_cast<uint32_t>(before >> 32));
+            write_tour = my_key < ((static_cast<uint64_t>(hi) << 32) | lo);
+        }
         uint32_t* out = p.out_tours + static_cast<size_t>(ant) * ns;
-        for (uint32_t i = lane; i < n; i += kWave) __builtin_nontemporal_store(static_cast<uint32_t>(tour[i]), out + i);
+        if (write_tour)
+            for (uint32_t i = lane; i < n; i += kWave) __builtin_nontemporal_store(static_cast<uint32_t>(tour[i]), out + i);
         if (diag_positions) {
             for (uint32_t i = lane; i < n; i += kWave) diag_positions[static_cast<size_t>(ant) * ns + i] = pos[i];
         }
@@ -864,7 +886,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         if (lane == 0) {
             faco_ant_info info;
             info.construction_cost = construction_cost;
-            info.final_cost = cost - gain_sum;
+            info.final_cost = final_cost;
             info.mne = mne; info.steps = steps; info.nonidentity_relocations = relocations;
             info.legal_exhausted = exhausted;
             info.processed_nodes = processed; info.candidate_checks = checks;
@@ -945,6 +967,7 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
         s.source_is_epoch = s.source_uniform < p.epoch_probability ? 1u : 0u;
         s.parent_cost = s.source_is_epoch ? s.epoch_cost : cost;
         s.pad = (improve_epoch ? 1u : 0u) | (improve_global ? 2u : 0u);
+        s.best_key = ~0ull;   // the next batch's ants publish into an empty slot
     }
     __syncthreads();
     const uint32_t improve = s.pad;
@@ -1014,6 +1037,7 @@ __global__ void __launch_bounds__(256) faco_colony_reset(const ResetParams p) {
         bounds(z, p.pw, p.retention, p.p_best_root);
         z.default_trail = z.maximum;
         z.target = p.target;
+        z.best_key = ~0ull;
         s = z;
     }
     __syncthreads();
