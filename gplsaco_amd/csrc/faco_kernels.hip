@@ -88,9 +88,15 @@ __device__ __forceinline__ uint32_t dpp_max_u32(uint32_t v) {
     o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x4E, 0xF, 0xF, false)); v = o > v ? o : v;
     o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x141, 0xF, 0xF, false)); v = o > v ? o : v;
     o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x140, 0xF, 0xF, false)); v = o > v ? o : v;
-    const uint32_t r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32), r3 = read_lane(v, 48);
-    const uint32_t a = r0 > r1 ? r0 : r1, b = r2 > r3 ? r2 : r3;
-    return a > b ? a : b;
+    // every lane of a row now holds the row maximum; row broadcasts fold
+    // row 0 into row 1, row 2 into row 3, then row 1 into rows 2 and 3, so
+    // lane 63 ends with the wave maximum (lanes of rows outside a
+    // broadcast's row mask keep their value)
+    int x = static_cast<int>(v);
+    o = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x142, 0xA, 0xF, false)); v = o > v ? o : v;
+    x = static_cast<int>(v);
+    o = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x143, 0xC, 0xF, false)); v = o > v ? o : v;
+    return read_lane(v, 63);
 }
 
 // Profile: per-ant phase and sub-phase clocks into p.phase_ticks (the
