@@ -312,6 +312,31 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             wave_sync();
             return 1u;
         };
+        // The same for up to four distinct cities at once, pushed in lane
+        // order (lane i < count holds the i-th city; a caller passes count
+        // lanes of distinct cities): one flag read, then the fresh ones take
+        // consecutive queue slots in that order. Equal to pushing them one by
+        // one, since pushing one city leaves the others' flags unchanged.
+        auto push_distinct = [&](uint32_t node, uint32_t count) -> uint32_t {
+            const bool member = lane < count;
+            const uint8_t f = flags[node];
+            const bool fresh = member && !(f & kQueued);
+            const uint64_t fresh_mask = ballot(fresh);
+            if (fresh) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(fresh_mask), 0u);
+                const uint32_t slot = qtail + rank;
+                flags[node] = static_cast<uint8_t>(f | kQueued);
+                queue[slot >= n ? slot - n : slot] = static_cast<uint16_t>(node);
+                if (checklist_log) checklist_log[logged + rank] = node;
+            }
+            const uint32_t pushed = static_cast<uint32_t>(__builtin_popcountll(fresh_mask));
+            qtail += pushed;
+            qtail = qtail >= n ? qtail - n : qtail;
+            qsize += pushed;
+            logged += pushed;
+            wave_sync();
+            return pushed;
+        };
 
         // ------------------------------------------------------------ construction
         // The product rows of both parent neighbours of the current city are
@@ -544,9 +569,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             if constexpr (Profile) { const uint64_t t = clock64(); sub[1] += t - mark; mark = t; }
             if (selected != (link & 0xffffu) && selected != (link >> 16)) {
                 ++mne;
-                push(current);
-                push(selected);
-                push(old_prev);
+                // old_prev is current itself when selected already followed
+                // it in the tour: it is then pushed once, as current
+                const uint32_t node = lane == 0 ? current : (lane == 1 ? selected : old_prev);
+                push_distinct(node, old_prev == current ? 2u : 3u);
             }
             current = selected;
             if constexpr (Profile) sub[2] += clock64() - mark;
@@ -837,10 +863,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             flip_nodes(move[0], move[1]);
             ++accepted;
             gain_sum += best_gain;
-            reactivations += push(move[0]);
-            reactivations += push(move[1]);
-            reactivations += push(move[2]);
-            reactivations += push(move[3]);
+            // the four endpoints of an improving 2-opt move are distinct
+            reactivations += push_distinct(lane == 0 ? move[0] : (lane == 1 ? move[1] : (lane == 2 ? move[2] : move[3])), 4u);
             if constexpr (Profile) sub[4] += clock64() - mark;
             if (accepted >= n) break;
         }
