@@ -101,7 +101,10 @@ size_t max_lds_per_block() {
 // staged in LDS when at least one ant still fits beside them.
 LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_waves) {
     LaunchShape shape;
-    const size_t budget = max_lds_per_block();
+    // the kernels' static LDS (ant slot counter, the persistent kernel's
+    // colony state) comes on top of the dynamic share
+    constexpr size_t kStaticLds = 256;
+    const size_t budget = max_lds_per_block() - kStaticLds;
     shape.nbr_lds_bytes = round16_host(2u * n * K);
     const uint32_t per_wave = ant_wave_bytes(n);
     if (forced_waves) {
@@ -417,6 +420,22 @@ struct Engine {
         for (auto& e : ev) checked(hipEventCreate(&e), "hipEventCreate");
         struct Destroy { hipEvent_t* e; ~Destroy() { for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]); } } d{ev};
         const uint32_t blocks = colonies * ap.blocks_per_colony;
+        if (!clocks && persistent(ap)) {
+            // the production (persistent) kernel: kernel_milliseconds[0] is the
+            // whole launch, ant search and colony update together
+            for (uint64_t done = 0; done < iterations;) {
+                const uint64_t rounds = std::min(iterations - done, kRoundsPerLaunch);
+                checked(hipEventRecord(ev[0], stream), "event");
+                checked(launch_colony_run(ap, cp, colonies, static_cast<uint32_t>(rounds), stream), "colony run launch");
+                checked(hipEventRecord(ev[1], stream), "event");
+                checked(hipEventSynchronize(ev[1]), "event sync");
+                float a = 0;
+                checked(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
+                ms[0] += a;
+                done += rounds;
+            }
+            return;
+        }
         // waves that can hold an ant at once: resident workgroups x waves
         const double resident_waves = static_cast<double>(std::min<uint64_t>(
             blocks, static_cast<uint64_t>(compute_units) * ant_search_blocks_per_cu(ap))) * ap.waves;
@@ -453,11 +472,25 @@ struct Engine {
             ms[11] += last > first ? held / (resident_waves * static_cast<double>(last - first)) : 0.0;
         }
     }
-    // Run `iterations` batches; through a cached HIP graph of up to 16
-    // iterations when use_graph is set, eagerly otherwise.
+    // Plain FACO with one workgroup per colony runs as the persistent
+    // kernel: up to kRoundsPerLaunch batches per launch, each colony's
+    // workgroup doing its own colony update between batches.
+    static constexpr uint64_t kRoundsPerLaunch = 64;
+    static bool persistent(const AntParams& ap) { return ap.blocks_per_colony == 1; }
+    // Run `iterations` batches: persistent launches when the shape allows,
+    // else per batch an ant launch and a colony-update launch, through a
+    // cached HIP graph of 16 batches when use_graph is set.
     void run(uint32_t colonies, uint64_t iterations, bool use_graph) {
         const AntParams ap = ant_params(colonies);
         const ColonyParams cp = colony_params();
+        if (persistent(ap)) {
+            for (uint64_t done = 0; done < iterations;) {
+                const uint64_t rounds = std::min(iterations - done, kRoundsPerLaunch);
+                checked(launch_colony_run(ap, cp, colonies, static_cast<uint32_t>(rounds), stream), "colony run launch");
+                done += rounds;
+            }
+            return;
+        }
         constexpr uint32_t kGraphIterations = 16;
         if (use_graph && iterations >= kGraphIterations) {
             if (!graph || graph_colonies != colonies || graph_iterations != kGraphIterations) {

@@ -355,6 +355,10 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t str
 hipError_t launch_start_check(uint64_t key, uint32_t batch, uint32_t ant, const uint32_t* counts, uint32_t n,
                               uint32_t* out, hipStream_t stream);
 hipError_t launch_colony_update(const ColonyParams& p, uint32_t colonies, hipStream_t stream);
+// `rounds` batches of plain FACO in one launch, each colony's workgroup
+// running its ants and then its colony update (blocks_per_colony must be 1).
+hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_t colonies, uint32_t rounds,
+                             hipStream_t stream);
 hipError_t launch_colony_reset(const ResetParams& p, uint32_t colonies, hipStream_t stream);
 hipError_t launch_distance_check(const int32_t* pairs, uint32_t count, int64_t* out, hipStream_t stream);
 hipError_t launch_philox_check(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step, uint32_t words,

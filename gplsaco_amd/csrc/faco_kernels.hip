@@ -135,8 +135,140 @@ __device__ __forceinline__ double row_scan4(double v) {
     return acc;
 }
 
-template<bool NbrLds, bool Profile, bool Diag, bool OrOpt>
-__global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
+// MMAS bounds from the epoch cost (reference cuda/colony_state.cuh:66-70).
+__device__ __forceinline__ void bounds(ColonyState& s, uint32_t width, double retention, double p) {
+    s.maximum = 1.0 / (static_cast<double>(s.epoch_cost) * (1.0 - retention));
+    const double lo = s.maximum * (1.0 - p) / ((width - 1.0) * p);
+    s.minimum = s.maximum < lo ? s.maximum : lo;
+}
+
+// Colony update of one colony by the calling workgroup (threads tid of
+// `threads`, a multiple of 64): work totals, iteration best (lowest ant on
+// ties), epoch/global update, MMAS bounds, epoch-source coin, next parent
+// and its links, and the fused evaporate/deposit/clamp/product update
+// (reference cuda/colony_state.cuh:148-229). `red` holds 3 words per wave,
+// `s` is the workgroup's copy of the colony state and `s_links` [n] receives
+// the new parent links (LDS).
+__device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t colony, uint32_t tid, uint32_t threads,
+                                              uint64_t* red, ColonyState& s, uint32_t* s_links) {
+    const uint32_t ns = p.n;
+    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
+    const uint32_t ants = p.shapes ? p.shapes[colony].ants : p.ants;
+    const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
+    ColonyState* st = p.state + colony;
+    const faco_ant_info* info = p.info + static_cast<size_t>(colony) * p.ants;
+    uint64_t best = ~0ull, wc = 0, wl = 0;
+    for (uint32_t a = tid; a < ants; a += threads) {
+        // final costs are positive and < 2^31: (cost, ant) orders ties by ant
+        const uint64_t k = (static_cast<uint64_t>(info[a].final_cost) << 32) | a;
+        best = k < best ? k : best;
+        wc += info[a].steps;
+        wl += info[a].move_evaluations;
+    }
+#pragma unroll
+    for (int offset = 32; offset; offset >>= 1) {
+        const uint64_t o = __shfl_xor(best, offset);
+        best = o < best ? o : best;
+        wc += __shfl_xor(wc, offset);
+        wl += __shfl_xor(wl, offset);
+    }
+    if ((tid & 63u) == 0) { red[3 * (tid >> 6)] = best; red[3 * (tid >> 6) + 1] = wc; red[3 * (tid >> 6) + 2] = wl; }
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t key = ~0ull, total_c = 0, total_l = 0;
+        for (uint32_t w = 0; w < threads / 64u; ++w) {
+            key = red[3 * w] < key ? red[3 * w] : key;
+            total_c += red[3 * w + 1];
+            total_l += red[3 * w + 2];
+        }
+        s = *st;
+        s.work_construction += total_c;
+        s.work_ls += total_l;
+        const uint32_t ib = static_cast<uint32_t>(key);
+        const int64_t cost = static_cast<int64_t>(key >> 32);
+        s.iteration_best = ib;
+        const bool improve_epoch = cost < s.epoch_cost, improve_global = cost < s.global_cost;
+        if (improve_epoch) s.epoch_cost = cost;
+        if (improve_global) s.global_cost = cost;
+        bounds(s, p.pw, p.retention, p.p_best_root);
+        Stream coin(p.keys[colony], s.batch, 0xffffffffu, 0u);
+        s.source_uniform = coin.uniform53();
+        s.source_is_epoch = s.source_uniform < p.epoch_probability ? 1u : 0u;
+        s.parent_cost = s.source_is_epoch ? s.epoch_cost : cost;
+        s.pad = (improve_epoch ? 1u : 0u) | (improve_global ? 2u : 0u);
+        s.best_key = ~0ull;   // the next batch's ants publish into an empty slot
+    }
+    __syncthreads();
+    const uint32_t improve = s.pad;
+    const uint32_t* best_tour = p.tours + (static_cast<size_t>(colony) * p.ants + s.iteration_best) * ns;
+    uint32_t* epoch = p.epoch + static_cast<size_t>(colony) * ns;
+    uint32_t* global = p.global + static_cast<size_t>(colony) * ns;
+    uint32_t* parent = p.parent + static_cast<size_t>(colony) * ns;
+    uint32_t* links = p.links + static_cast<size_t>(colony) * ns;
+    if (improve & 1u) for (uint32_t i = tid; i < n; i += threads) epoch[i] = best_tour[i];
+    if (improve & 2u) for (uint32_t i = tid; i < n; i += threads) global[i] = best_tour[i];
+    __syncthreads();
+    const uint32_t* source = s.source_is_epoch ? epoch : best_tour;
+    for (uint32_t i = tid; i < n; i += threads) {
+        const uint32_t v = source[i];
+        const uint32_t pred = source[i == 0 ? n - 1 : i - 1], succ = source[i + 1 == n ? 0 : i + 1];
+        parent[i] = v;
+        s_links[v] = pred | (succ << 16);
+        links[v] = pred | (succ << 16);
+        if (v == 0) s.parent_zero = i | (succ < pred ? 0x80000000u : 0u);
+    }
+    __syncthreads();
+    // Evaporate, clamp below, deposit on parent edges, clamp above, refresh
+    // products (reference cuda/colony_state.cuh:210-229).
+    const uint32_t cells = n * p.pw;
+    double* trails = p.trails + static_cast<size_t>(colony) * ns * p.pw;
+    double* products = p.products + static_cast<size_t>(colony) * ns * p.pw;
+    const double* eta = p.eta + static_cast<size_t>(geom) * ns * p.pw;
+    const uint16_t* nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
+    const double minimum = s.minimum, maximum = s.maximum, retention = p.retention;
+    const double increment = 1.0 / static_cast<double>(s.parent_cost);
+    for (uint32_t i = tid; i < cells; i += threads) {
+        const uint32_t from = i / p.pw, j = i - from * p.pw;
+        const uint32_t to = nbr[from * p.K + j];
+        double v = trails[i] * retention;
+        v = v > minimum ? v : minimum;
+        const uint32_t link = s_links[from];
+        if (to == (link & 0xffffu) || to == (link >> 16)) {
+            const double raised = v + increment;
+            v = raised < maximum ? raised : maximum;
+        }
+        trails[i] = v;
+        products[i] = v * eta[i];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double d = s.default_trail * retention;
+        s.default_trail = d > minimum ? d : minimum;
+        s.batch += 1;
+        s.pad = 0;
+        *st = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) {
+    const uint32_t colony = blockIdx.x, tid = threadIdx.x;
+    if (p.shapes && p.state[colony].batch >= p.shapes[colony].batches) {   // colony done: mark it idle
+        if (tid == 0) p.state[colony].pad = kColonyIdle;
+        return;
+    }
+    __shared__ uint64_t red[3 * 4];
+    __shared__ ColonyState s;
+    extern __shared__ uint32_t s_links[];
+    update_colony(p, colony, tid, blockDim.x, red, s, s_links);
+}
+
+// ant_search: the body of the ant kernel. Persistent: the workgroup runs
+// `rounds` whole batches of its colony, each followed by the colony update
+// (update_colony) in the same workgroup, keeping the colony state in
+// `s_state` (LDS); otherwise one batch and the update is a separate kernel.
+template<bool NbrLds, bool Profile, bool Diag, bool OrOpt, bool Persistent>
+__device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParams& cp, uint32_t rounds,
+                                           ColonyState* s_state) {
     // Diagnostic features (explicit visits, per-ant parents, traces and
     // logs) exist only in the Diag instantiation: the production kernel
     // keeps none of their pointers or flags live.
@@ -173,7 +305,7 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     const uint32_t ns = p.n;
     const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
     const uint32_t chunk = p.shapes ? p.shapes[colony].ants / p.blocks_per_colony : p.chunk;
-    if (p.shapes && p.state[colony].batch >= p.shapes[colony].batches) return;   // colony done
+    if (!Persistent && p.shapes && p.state[colony].batch >= p.shapes[colony].batches) return;   // colony done
     const uint32_t chunk_base = (explicit_mode ? blk : blk % p.blocks_per_colony) * chunk;
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
 
@@ -203,7 +335,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             s_nbr = g_nbr;
         }
     }
-    if (threadIdx.x == 0) next_slot = 0;
+    if (threadIdx.x == 0) {
+        next_slot = 0;
+        if constexpr (Persistent) *s_state = p.state[colony];
+    }
     __syncthreads();
 
     unsigned char* mine = lds + 2u * round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u) + wave * p.per_wave_bytes;
@@ -214,7 +349,6 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 
     const uint32_t pw = p.pw, K = p.K, lw = p.lw;
     const uint64_t key = explicit_mode ? 0ull : p.keys[colony];
-    const uint32_t batch = explicit_mode ? 0u : p.state[colony].batch;
     const double* products = p.products + static_cast<size_t>(colony) * ns * pw;
     const uint16_t* backup = p.backup + static_cast<size_t>(geom) * ns * p.bw;
     const uint32_t limit = static_cast<uint32_t>(p.ls_limit);   // clamped to 32 bits by the host
@@ -225,6 +359,11 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
     // it needs, not for the prefetches issued after it.
     const uint32_t row_lane = in_row ? lane : (pw ? pw - 1u : 0u);
 
+    for (uint32_t round = 0;; ++round) {
+    if constexpr (Persistent) {
+        if (round == rounds || (p.shapes && s_state->batch >= p.shapes[colony].batches)) break;
+    }
+    const uint32_t batch = explicit_mode ? 0u : (Persistent ? s_state->batch : p.state[colony].batch);
     for (;;) {
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(&next_slot, 1u);
@@ -264,7 +403,8 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         }
         wave_sync();
 
-        const uint32_t target = explicit_mode ? p.targets[ant] : p.state[colony].target;
+        const uint32_t target = explicit_mode ? p.targets[ant]
+                              : (Persistent ? s_state->target : p.state[colony].target);
         const uint32_t* visits = p.visits + static_cast<size_t>(ant) * ns;
         uint32_t* checklist_log = diag_checklist ? diag_checklist + static_cast<size_t>(ant) * 5u * ns : nullptr;
         uint32_t current;
@@ -885,8 +1025,15 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
             write_tour = my_key < ((static_cast<uint64_t>(hi) << 32) | lo);
         }
         uint32_t* out = p.out_tours + static_cast<size_t>(ant) * ns;
-        if (write_tour)
-            for (uint32_t i = lane; i < n; i += kWave) __builtin_nontemporal_store(static_cast<uint32_t>(tour[i]), out + i);
+        // (the persistent kernel's own update reads the tour back: a plain
+        // store keeps it in this CU's caches)
+        if (write_tour) {
+            if constexpr (Persistent) {
+                for (uint32_t i = lane; i < n; i += kWave) out[i] = tour[i];
+            } else {
+                for (uint32_t i = lane; i < n; i += kWave) __builtin_nontemporal_store(static_cast<uint32_t>(tour[i]), out + i);
+            }
+        }
         if (diag_positions) {
             for (uint32_t i = lane; i < n; i += kWave) diag_positions[static_cast<size_t>(ant) * ns + i] = pos[i];
         }
@@ -934,121 +1081,35 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
         }
         wave_sync();
     }
+    if constexpr (!Persistent) {
+        break;
+    } else {
+        // every ant of the batch is done: the colony update, in place
+        __syncthreads();
+        update_colony(cp, colony, threadIdx.x, blockDim.x,
+                      reinterpret_cast<uint64_t*>(lds + 2u * round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u)),
+                      *s_state, s_plinks);
+        if (threadIdx.x == 0) next_slot = 0;
+        __syncthreads();
+    }
+    }
+}
+
+template<bool NbrLds, bool Profile, bool Diag, bool OrOpt>
+__global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
+    ant_search<NbrLds, Profile, Diag, OrOpt, false>(p, ColonyParams{}, 1u, nullptr);
+}
+
+// Plain FACO over `rounds` batches in one launch: one workgroup per colony
+// runs its ants and its colony update batch after batch, so a colony never
+// waits for the others between batches.
+template<bool NbrLds, bool OrOpt>
+__global__ void __launch_bounds__(1024) faco_colony_run(const AntParams p, const ColonyParams cp, uint32_t rounds) {
+    __shared__ ColonyState s_state;
+    ant_search<NbrLds, false, false, OrOpt, true>(p, cp, rounds, &s_state);
 }
 
 // ---------------------------------------------------------------------------
-
-// MMAS bounds from the epoch cost (reference cuda/colony_state.cuh:66-70).
-__device__ __forceinline__ void bounds(ColonyState& s, uint32_t width, double retention, double p) {
-    s.maximum = 1.0 / (static_cast<double>(s.epoch_cost) * (1.0 - retention));
-    const double lo = s.maximum * (1.0 - p) / ((width - 1.0) * p);
-    s.minimum = s.maximum < lo ? s.maximum : lo;
-}
-
-__global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) {
-    // n: this colony's cities, ns: stride of the per-city arrays (they
-    // differ only in a mixed-size batch; so do ants and the ant stride)
-    const uint32_t colony = blockIdx.x, ns = p.n, tid = threadIdx.x;
-    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
-    const uint32_t ants = p.shapes ? p.shapes[colony].ants : p.ants;
-    const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
-    ColonyState* st = p.state + colony;
-    if (p.shapes && st->batch >= p.shapes[colony].batches) {   // colony done: mark it idle
-        if (tid == 0) st->pad = kColonyIdle;
-        return;
-    }
-    const faco_ant_info* info = p.info + static_cast<size_t>(colony) * p.ants;
-    __shared__ uint64_t best_key[256];
-    __shared__ uint64_t work_c[256], work_l[256];
-    __shared__ ColonyState s;
-    extern __shared__ uint32_t s_links[];
-
-    uint64_t best = ~0ull, wc = 0, wl = 0;
-    for (uint32_t a = tid; a < ants; a += blockDim.x) {
-        // final costs are positive and < 2^31: (cost, ant) orders ties by ant
-        const uint64_t k = (static_cast<uint64_t>(info[a].final_cost) << 32) | a;
-        best = k < best ? k : best;
-        wc += info[a].steps;
-        wl += info[a].move_evaluations;
-    }
-    best_key[tid] = best; work_c[tid] = wc; work_l[tid] = wl;
-    __syncthreads();
-    for (uint32_t stride = blockDim.x / 2; stride; stride >>= 1) {
-        if (tid < stride) {
-            best_key[tid] = best_key[tid + stride] < best_key[tid] ? best_key[tid + stride] : best_key[tid];
-            work_c[tid] += work_c[tid + stride];
-            work_l[tid] += work_l[tid + stride];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        s = *st;
-        s.work_construction += work_c[0];
-        s.work_ls += work_l[0];
-        const uint32_t ib = static_cast<uint32_t>(best_key[0]);
-        const int64_t cost = static_cast<int64_t>(best_key[0] >> 32);
-        s.iteration_best = ib;
-        const bool improve_epoch = cost < s.epoch_cost, improve_global = cost < s.global_cost;
-        if (improve_epoch) s.epoch_cost = cost;
-        if (improve_global) s.global_cost = cost;
-        bounds(s, p.pw, p.retention, p.p_best_root);
-        Stream coin(p.keys[colony], s.batch, 0xffffffffu, 0u);
-        s.source_uniform = coin.uniform53();
-        s.source_is_epoch = s.source_uniform < p.epoch_probability ? 1u : 0u;
-        s.parent_cost = s.source_is_epoch ? s.epoch_cost : cost;
-        s.pad = (improve_epoch ? 1u : 0u) | (improve_global ? 2u : 0u);
-        s.best_key = ~0ull;   // the next batch's ants publish into an empty slot
-    }
-    __syncthreads();
-    const uint32_t improve = s.pad;
-    const uint32_t* best_tour = p.tours + (static_cast<size_t>(colony) * p.ants + s.iteration_best) * ns;
-    uint32_t* epoch = p.epoch + static_cast<size_t>(colony) * ns;
-    uint32_t* global = p.global + static_cast<size_t>(colony) * ns;
-    uint32_t* parent = p.parent + static_cast<size_t>(colony) * ns;
-    uint32_t* links = p.links + static_cast<size_t>(colony) * ns;
-    if (improve & 1u) for (uint32_t i = tid; i < n; i += blockDim.x) epoch[i] = best_tour[i];
-    if (improve & 2u) for (uint32_t i = tid; i < n; i += blockDim.x) global[i] = best_tour[i];
-    __syncthreads();
-    const uint32_t* source = s.source_is_epoch ? epoch : best_tour;
-    for (uint32_t i = tid; i < n; i += blockDim.x) {
-        const uint32_t v = source[i];
-        const uint32_t pred = source[i == 0 ? n - 1 : i - 1], succ = source[i + 1 == n ? 0 : i + 1];
-        parent[i] = v;
-        s_links[v] = pred | (succ << 16);
-        links[v] = pred | (succ << 16);
-        if (v == 0) s.parent_zero = i | (succ < pred ? 0x80000000u : 0u);
-    }
-    __syncthreads();
-    // Evaporate, clamp below, deposit on parent edges, clamp above, refresh
-    // products (reference cuda/colony_state.cuh:210-229).
-    const uint32_t cells = n * p.pw;
-    double* trails = p.trails + static_cast<size_t>(colony) * ns * p.pw;
-    double* products = p.products + static_cast<size_t>(colony) * ns * p.pw;
-    const double* eta = p.eta + static_cast<size_t>(geom) * ns * p.pw;
-    const uint16_t* nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
-    const double minimum = s.minimum, maximum = s.maximum, retention = p.retention;
-    const double increment = 1.0 / static_cast<double>(s.parent_cost);
-    for (uint32_t i = tid; i < cells; i += blockDim.x) {
-        const uint32_t from = i / p.pw, j = i - from * p.pw;
-        const uint32_t to = nbr[from * p.K + j];
-        double v = trails[i] * retention;
-        v = v > minimum ? v : minimum;
-        const uint32_t link = s_links[from];
-        if (to == (link & 0xffffu) || to == (link >> 16)) {
-            const double raised = v + increment;
-            v = raised < maximum ? raised : maximum;
-        }
-        trails[i] = v;
-        products[i] = v * eta[i];
-    }
-    if (tid == 0) {
-        const double d = s.default_trail * retention;
-        s.default_trail = d > minimum ? d : minimum;
-        s.batch += 1;
-        s.pad = 0;
-        *st = s;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Colony reset (reference reset_colony + initialize_products,
@@ -1170,6 +1231,20 @@ hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t st
         if (nl) oo ? go(faco_ant_search<true, false, false, true>) : go(faco_ant_search<true, false, false, false>);
         else oo ? go(faco_ant_search<false, false, false, true>) : go(faco_ant_search<false, false, false, false>);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_t colonies, uint32_t rounds,
+                             hipStream_t stream) {
+    if (p.blocks_per_colony != 1 || p.phase_ticks || p.explicit_mode || p.per_ant_parent || p.trace_selected ||
+        p.diag_construction || p.diag_positions || p.diag_checklist || p.control || p.out_identity)
+        return hipErrorInvalidValue;   // one workgroup per colony, production plain FACO only
+    const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
+    const dim3 grid(colonies), block(64 * p.waves);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p, cp, rounds); };
+    const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
+    if (nl) oo ? go(faco_colony_run<true, true>) : go(faco_colony_run<true, false>);
+    else oo ? go(faco_colony_run<false, true>) : go(faco_colony_run<false, false>);
     return hipGetLastError();
 }
 
