@@ -10,11 +10,13 @@ ants, then selection and the pheromone update. The same run with one seed
 reported beside it.
 Inputs (instances, candidate rows, pheromone) are resident in HBM before the
 timed region; the timed region is the iteration loop between two stream
-synchronisations, replayed through a HIP graph.
+synchronisations: the persistent kernel faco_colony_run, one workgroup per
+colony running its ants and its colony update, up to 64 iterations per
+launch.
 
-Prints one JSON line with the metric, the roofline of the construction+LS
-kernel (HIP-event average launch time, algorithmic bytes per launch) and the
-CPU baseline (the oracle's OpenMP FACO on the host cores, a bounded sample).
+Prints one JSON line with the metric, the roofline of that kernel (HIP-event
+time per iteration, algorithmic bytes per iteration) and the CPU baseline
+(the oracle's OpenMP FACO on the host cores, a bounded sample).
 """
 from __future__ import annotations
 
@@ -34,12 +36,16 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes_per_tour(n: int, steps_per_tour: float, evals_per_tour: float) -> float:
-    """Bytes one ant tour must move through the construction+LS kernel:
-    parent read + tour write (4n each), per construction step 16 candidate ids
-    (4 B) and 16 trail*heuristic products (8 B), per LS move evaluation a
-    candidate id, its distance and two distance lookups (16 B)."""
-    return 8.0 * n + 192.0 * steps_per_tour + 16.0 * evals_per_tour
+def algorithmic_bytes_per_tour(n: int, steps_per_tour: float, evals_per_tour: float, pw: int = 16,
+                               ants: int = 128) -> float:
+    """Bytes one ant tour must move through the persistent kernel (ant search
+    and colony update): parent read + tour write (4n each), per construction
+    step 16 candidate ids (4 B) and 16 trail*heuristic products (8 B), per LS
+    move evaluation a candidate id, its distance and two distance lookups
+    (16 B), plus the ant's share of its colony's update (trails read and
+    written, heuristic read, products written: 32 B per candidate entry of
+    the n x pw table, over the colony's ants)."""
+    return 8.0 * n + 192.0 * steps_per_tour + 16.0 * evals_per_tour + 32.0 * n * pw / ants
 
 
 def cpu_baseline(xys, keys, seeds, ants, iterations, budget_seconds=20.0):
@@ -64,17 +70,18 @@ def cpu_baseline(xys, keys, seeds, ants, iterations, budget_seconds=20.0):
 
 
 def measured_traffic(n: int, colonies: int, ants: int):
-    """Calibrated HBM bytes per faco_ant_search launch of this workload from
-    the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
-    dispatch, each scaled by what the counter reads for the calibration copy
-    kernel of known size), or None when no summary matches the shape."""
-    path = ROOT / "profiles" / "faco_ant_search_tsp1000_traffic.json"
+    """Calibrated HBM bytes per iteration of faco_colony_run on this workload
+    from the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
+    dispatch over its iterations, each scaled by what the counter reads for
+    the calibration copy kernel of known size), or None when no summary
+    matches the shape."""
+    path = ROOT / "profiles" / "faco_colony_run_tsp1000_traffic.json"
     if not path.exists():
         return None
     data = json.loads(path.read_text())
     if (data.get("n"), data.get("colonies"), data.get("ants")) != (n, colonies, ants):
         return None
-    return data["fetch_bytes_per_launch"] * data["fetch_scale"] + data["write_bytes_per_launch"] * data["write_scale"]
+    return data["bytes_per_iteration"]
 
 
 def gp_population_rate(ext, steps: int) -> dict:
@@ -302,21 +309,24 @@ def main() -> int:
     steps_per_tour = timed["completed_construction_steps"] / tours
     evals_per_tour = timed["completed_ls_evaluations"] / tours
 
-    # Same iterations again, eagerly, with HIP events around every launch of
-    # the production kernels on the engine's stream: average launch times.
+    # Same iterations again with HIP events around every launch of the
+    # production kernel (the persistent faco_colony_run: ant search and
+    # colony update, up to 64 iterations per launch) on the engine's stream.
     prof = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=False,
                                             time_kernels=True)
     ptours = prof["total_tour_evaluations"]
-    launch_ms = prof["kernel_milliseconds"]["construction_and_search"] / args.steps
-    colony_ms = prof["kernel_milliseconds"]["reduction_and_pheromone"] / args.steps
+    iteration_ms = (prof["kernel_milliseconds"]["construction_and_search"]
+                    + prof["kernel_milliseconds"]["reduction_and_pheromone"]) / args.steps
+    launches = -(-args.steps // 64)
     # A few iterations of the clock-instrumented kernel variant: per-ant phases.
     phase_steps = min(args.steps, 10)
     phases = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * phase_steps, use_graph=False,
                                               profile_kernels=True)
     bytes_per_tour = algorithmic_bytes_per_tour(n, prof["completed_construction_steps"] / ptours,
-                                                prof["completed_ls_evaluations"] / ptours)
-    tours_per_launch = colonies * ants
-    achieved = bytes_per_tour * tours_per_launch / (launch_ms * 1e-3) / 1e9
+                                                prof["completed_ls_evaluations"] / ptours,
+                                                engine.settings.primary_width, ants)
+    tours_per_iteration = colonies * ants
+    achieved = bytes_per_tour * tours_per_iteration / (iteration_ms * 1e-3) / 1e9
     mean_cost = float(np.mean([item["cost"] for item in timed["items"]]))
 
     # The same workload with Or-opt added to the local search.
@@ -360,16 +370,18 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "faco_ant_search",
+            "kernel": "faco_colony_run",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(n, colonies, ants),
+            "per": "iteration (launches run up to 64 iterations each; achieved and traffic are per iteration)",
             "bytes_per_tour": bytes_per_tour,
-            "tours_per_launch": tours_per_launch,
-            "launch_ms": launch_ms,
-            "colony_update_ms": colony_ms,
+            "tours_per_iteration": tours_per_iteration,
+            "iteration_ms": iteration_ms,
+            "launches": launches,
+            "mean_launch_ms": iteration_ms * args.steps / launches,
             "mean_ant_phase_ms_per_step": {k: v / phase_steps for k, v in phases["ant_phase_milliseconds"].items()},
             "mean_ant_subphase_megaclocks_per_step": {k: v / phase_steps
                                                       for k, v in phases["ant_subphase_megaclocks"].items()},
