@@ -340,8 +340,10 @@ def main() -> int:
     small = ext.FacoBatchEngine(n, instances, settings)
     for k, xy in zip(keys, xys):
         small.register_problem(k, xy)
-    small_run = small.evaluate_faco_evaluations(np.array(keys, dtype=np.uint64),
-                                                np.zeros(instances, dtype=np.uint64), ants * args.steps)
+    small_keys, small_seeds = np.array(keys, dtype=np.uint64), np.zeros(instances, dtype=np.uint64)
+    if args.warmup:   # as for the main run: graph capture and first-touch costs stay out of the timing
+        small.evaluate_faco_evaluations(small_keys, small_seeds, ants * max(args.warmup, 16))
+    small_run = small.evaluate_faco_evaluations(small_keys, small_seeds, ants * args.steps)
     small_value = small_run["total_tour_evaluations"] / small_run["search_seconds"]
 
     line = {
