@@ -11,7 +11,7 @@ reported beside it.
 Inputs (instances, candidate rows, pheromone) are resident in HBM before the
 timed region; the timed region is the iteration loop between two stream
 synchronisations: the persistent kernel faco_colony_run, one workgroup per
-colony running its ants and its colony update, up to 64 iterations per
+colony running its ants and its colony update, up to 1024 iterations per
 launch.
 
 Prints one JSON line with the metric, the roofline of that kernel (HIP-event
@@ -311,13 +311,13 @@ def main() -> int:
 
     # Same iterations again with HIP events around every launch of the
     # production kernel (the persistent faco_colony_run: ant search and
-    # colony update, up to 64 iterations per launch) on the engine's stream.
+    # colony update, up to 1024 iterations per launch) on the engine's stream.
     prof = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=False,
                                             time_kernels=True)
     ptours = prof["total_tour_evaluations"]
     iteration_ms = (prof["kernel_milliseconds"]["construction_and_search"]
                     + prof["kernel_milliseconds"]["reduction_and_pheromone"]) / args.steps
-    launches = -(-args.steps // 64)
+    launches = -(-args.steps // 1024)
     # A few iterations of the clock-instrumented kernel variant: per-ant phases.
     phase_steps = min(args.steps, 10)
     phases = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * phase_steps, use_graph=False,
@@ -378,7 +378,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(n, colonies, ants),
-            "per": "iteration (launches run up to 64 iterations each; achieved and traffic are per iteration)",
+            "per": "iteration (launches run up to 1024 iterations each; achieved and traffic are per iteration)",
             "bytes_per_tour": bytes_per_tour,
             "tours_per_iteration": tours_per_iteration,
             "iteration_ms": iteration_ms,

@@ -475,7 +475,7 @@ struct Engine {
     // Plain FACO with one workgroup per colony runs as the persistent
     // kernel: up to kRoundsPerLaunch batches per launch, each colony's
     // workgroup doing its own colony update between batches.
-    static constexpr uint64_t kRoundsPerLaunch = 64;
+    static constexpr uint64_t kRoundsPerLaunch = 1024;
     static bool persistent(const AntParams& ap) { return ap.blocks_per_colony == 1; }
     // Run `iterations` batches: persistent launches when the shape allows,
     // else per batch an ant launch and a colony-update launch, through a
