@@ -75,6 +75,7 @@ private:
 struct DeviceGeometry {
     DeviceArray<uint32_t> coords;     // [G][n]
     DeviceArray<uint16_t> nbr;        // [G][nbr_stride]
+    DeviceArray<uint16_t> nbr_dist;   // [G][nbr_stride] distance of each nbr entry
     DeviceArray<uint16_t> backup;     // [G][n][bw]
     DeviceArray<double> eta;          // [G][n][pw]
     DeviceArray<uint32_t> initial;    // [G][n]
@@ -216,7 +217,7 @@ struct Engine {
         checked(hipEventCreate(&ev_stop), "hipEventCreate");
         const size_t C = colonies_capacity, A = C * config.ants, G = panel;
         auto alloc = [&](auto& buffer, size_t count) { buffer.allocate(count); allocated += buffer.bytes(); };
-        alloc(geo.coords, G * n); alloc(geo.nbr, G * nbr_stride); alloc(geo.backup, G * n * std::max(config.backup_width, 1u));
+        alloc(geo.coords, G * n); alloc(geo.nbr, G * nbr_stride); alloc(geo.nbr_dist, G * nbr_stride); alloc(geo.backup, G * n * std::max(config.backup_width, 1u));
         alloc(geo.eta, G * n * config.primary_width); alloc(geo.initial, G * n); alloc(geo.initial_cost, G);
         alloc(geometry, C); alloc(keys, C); alloc(state, C);
         alloc(parent, C * n); alloc(links, C * n); alloc(epoch, C * n); alloc(global, C * n);
@@ -319,6 +320,7 @@ struct Engine {
             const uint32_t m = p.n;
             geo.coords.upload(p.packed.data(), m, stream, static_cast<size_t>(g) * n);
             geo.nbr.upload(p.nbr.data(), nbr_stride, stream, static_cast<size_t>(g) * nbr_stride);
+            geo.nbr_dist.upload(p.nbr_dist.data(), nbr_stride, stream, static_cast<size_t>(g) * nbr_stride);
             if (bw) geo.backup.upload(p.backup16.data(), static_cast<size_t>(m) * bw, stream, static_cast<size_t>(g) * n * bw);
             if (p.prior.empty()) {
                 geo.eta.upload(p.eta.data(), static_cast<size_t>(m) * pw, stream, static_cast<size_t>(g) * n * pw);
@@ -366,7 +368,7 @@ struct Engine {
         p.roulette_margin = roulette_margin;
         p.or_opt = local_search;
         p.filter_tours = 1;   // plain FACO: only the iteration best tour is read back
-        p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.backup = geo.backup.data();
+        p.coords = geo.coords.data(); p.nbr = geo.nbr.data(); p.nbr_dist = geo.nbr_dist.data(); p.backup = geo.backup.data();
         p.products = products.data(); p.parent = parent.data(); p.links = links.data();
         p.state = state.data(); p.keys = keys.data(); p.targets = nullptr;
         p.geometry = geometry.data(); p.out_tours = tours.data(); p.out_info = info.data();
@@ -1325,8 +1327,11 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         const uint32_t K = lw, stride = round16_host(2u * n * K) / 2u;
         std::vector<uint32_t> packed(n);
         for (uint32_t i = 0; i < n; ++i) packed[i] = pack_xy(coords[2 * i], coords[2 * i + 1]);
-        std::vector<uint16_t> nbr(stride, 0);
-        for (size_t i = 0; i < static_cast<size_t>(n) * lw; ++i) nbr[i] = static_cast<uint16_t>(ls[i]);
+        std::vector<uint16_t> nbr(stride, 0), nbr_dist(stride, 0);
+        for (size_t i = 0; i < static_cast<size_t>(n) * lw; ++i) {
+            nbr[i] = static_cast<uint16_t>(ls[i]);
+            nbr_dist[i] = static_cast<uint16_t>(euc2d(packed[i / lw], packed[ls[i]]));
+        }
         std::vector<uint32_t> links(static_cast<size_t>(ants) * n);
         std::vector<int64_t> parent_costs(ants);
         for (uint32_t a = 0; a < ants; ++a) {
@@ -1341,13 +1346,14 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         }
         const size_t cells = static_cast<size_t>(ants) * n;
         DeviceArray<uint32_t> d_coords, d_parents, d_links, d_visits, d_targets, d_tours, d_ctours, d_pos, d_check;
-        DeviceArray<uint16_t> d_nbr;
+        DeviceArray<uint16_t> d_nbr, d_nbr_dist;
         DeviceArray<int64_t> d_costs;
         DeviceArray<faco_ant_info> d_info;
-        d_coords.allocate(n); d_nbr.allocate(stride); d_parents.allocate(cells); d_links.allocate(cells);
+        d_coords.allocate(n); d_nbr.allocate(stride); d_nbr_dist.allocate(stride); d_parents.allocate(cells); d_links.allocate(cells);
         d_visits.allocate(cells); d_targets.allocate(ants); d_tours.allocate(cells); d_ctours.allocate(cells);
         d_pos.allocate(cells); d_check.allocate(cells * 5); d_costs.allocate(ants); d_info.allocate(ants);
         d_coords.upload(packed.data(), n, stream); d_nbr.upload(nbr.data(), stride, stream);
+        d_nbr_dist.upload(nbr_dist.data(), stride, stream);
         d_parents.upload(parents, cells, stream); d_links.upload(links.data(), cells, stream);
         d_visits.upload(visits, cells, stream); d_targets.upload(mne_targets, ants, stream);
         d_costs.upload(parent_costs.data(), ants, stream);
@@ -1361,7 +1367,8 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         require(local_search <= 1, "local search mode must be 0 or 1");
         p.or_opt = local_search;
         p.ls_limit = std::min<uint64_t>(limit, UINT32_MAX);
-        p.coords = d_coords.data(); p.nbr = d_nbr.data(); p.parent = d_parents.data(); p.links = d_links.data();
+        p.coords = d_coords.data(); p.nbr = d_nbr.data(); p.nbr_dist = d_nbr_dist.data();
+        p.parent = d_parents.data(); p.links = d_links.data();
         p.parent_cost = d_costs.data(); p.targets = d_targets.data(); p.visits = d_visits.data();
         p.out_tours = d_tours.data(); p.out_info = d_info.data(); p.diag_construction = d_ctours.data();
         p.diag_positions = d_pos.data(); p.diag_checklist = d_check.data();

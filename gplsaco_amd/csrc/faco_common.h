@@ -221,6 +221,7 @@ struct AntParams {
     double roulette_margin;             // relative margin of the fast roulette (+inf: sequential sums always)
     const uint32_t* coords;             // [G][n] packed x | y << 16
     const uint16_t* nbr;                // [G][n][K] (d, id)-ordered nearest cities
+    const uint16_t* nbr_dist;           // [G][n][K] EUC_2D distance of each nbr entry
     const uint16_t* backup;             // [G][n][bw]
     const double* products;             // [C][n][pw] trail * heuristic
     const uint32_t* parent;             // [C][n] (or [A][n])

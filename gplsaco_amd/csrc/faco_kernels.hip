@@ -806,6 +806,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         const uint32_t column = cslot < K ? cslot : K - 1u;
         const bool scan_lane = cslot < lw;
         const uint32_t off_scan = scan_lane ? 0u : 0xffffffffu;   // all ones outside the scan
+        const uint16_t* g_nd = p.nbr_dist + static_cast<size_t>(geom) * p.nbr_stride;
         const uint32_t step_dir = kind ? 0xffffffffu : 1u;
         const uint32_t wrap_to = kind ? n - 1u : 0u;
         const uint32_t tie_index = kind * lw + cslot;
@@ -868,7 +869,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             const uint32_t closing = (a_prev & kind_mask) | (a_next & ~kind_mask);
             const uint32_t xclosing = s_xy[closing];
             const int32_t current_edge = static_cast<int32_t>(euc2d(xa, xclosing));
-            const int32_t dab = static_cast<int32_t>(euc2d(xa, xb));
+            const int32_t dab = g_nd[a * K + column];
             const bool bad = !real || !(current_edge > dab);
             // First failing slot per kind; lanes past lw count as failing.
             const uint64_t bad_mask = ballot(bad);

@@ -194,6 +194,9 @@ Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& c,
             p.nbr[static_cast<size_t>(a) * K + j] = static_cast<uint16_t>(v);
         }
     }
+    p.nbr_dist.assign(nbr_stride, 0);
+    for (size_t i = 0; i < static_cast<size_t>(n) * K; ++i)
+        p.nbr_dist[i] = static_cast<uint16_t>(euc2d(p.packed[i / K], p.packed[p.nbr[i]]));
     p.backup16.assign(p.backup.begin(), p.backup.end());
     const auto t_candidates = Clock::now();
     p.profile[0] = std::chrono::duration<double>(t_candidates - started).count();

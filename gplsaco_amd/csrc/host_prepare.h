@@ -18,6 +18,7 @@ struct Prepared {
     std::vector<uint32_t> packed;      // x | y << 16
     std::vector<uint32_t> primary, backup, ls;   // [n][width]
     std::vector<uint16_t> nbr;         // [n][K] rows (first max(pw, lw) nearest), padded to the stride
+    std::vector<uint16_t> nbr_dist;    // EUC_2D distance of each nbr entry (same layout)
     std::vector<uint16_t> backup16;    // [n][bw]
     std::vector<double> eta;           // [n][pw] heuristic 1/d^beta (1 when d == 0)
     std::vector<double> local_scale;   // mean of the 8 nearest distances
