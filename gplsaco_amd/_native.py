@@ -175,6 +175,16 @@ class ControlTrace(ctypes.Structure):
     ]
 
 
+class Fork(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("enabled", ctypes.c_uint32),
+        ("region", ctypes.c_uint32),
+        ("mne", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [
         ("waves", ctypes.c_uint32),
@@ -190,6 +200,7 @@ assert ctypes.sizeof(Program) == 388
 assert ctypes.sizeof(Controller) == 1168
 assert ctypes.sizeof(BaselinePolicy) == 48
 assert ctypes.sizeof(ControlState) == 40
+assert ctypes.sizeof(Fork) == 24
 
 # (name, restype, argtypes) of every exported entry point of include/faco_engine.h
 SIGNATURES = {
@@ -269,6 +280,18 @@ SIGNATURES = {
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,
                                                     u64p, vp, ctypes.POINTER(Checkpoints)]),
+    "faco_state_from_bytes": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "faco_state_to_bytes": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "faco_state_describe": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "faco_state_select_colony": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
+    "faco_state_destroy": (None, [vp]),
+    "faco_engine_capture_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
+                                                   ctypes.POINTER(ControlSpec), ctypes.c_uint32,
+                                                   ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,
+                                                   u64p, vp, ctypes.POINTER(ctypes.c_void_p)]),
+    "faco_engine_continue_control": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.POINTER(ControlSpec),
+                                                    ctypes.POINTER(Fork), ctypes.c_uint32, ctypes.POINTER(ControlTrace),
+                                                    ctypes.POINTER(Result), u32p, i64p, u64p, vp]),
 }
 
 _lib = None

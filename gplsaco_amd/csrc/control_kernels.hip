@@ -405,6 +405,9 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
             const faco_controller& ctrl = p.controllers[decider];
             action = static_cast<uint32_t>(controller_decide(ctrl, s_feat, 1, 0, mask, a, lane < 32 ? s_scores : nullptr));
         }
+        // a state fork replaces the first batch's decision (reference
+        // cuda/batch_engine.cu:922-923); the mask stays the policy's
+        if (p.forced_action >= 0) action = static_cast<uint32_t>(p.forced_action);
         if (lane == 0) { s_action = action; s_mask = mask; }
     }
     __syncthreads();
@@ -415,6 +418,14 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
         if (p.scores) p.scores[static_cast<size_t>(colony) * 32 + tid] = s_scores[tid];
         if (p.regions) for (uint32_t k = 0; k < 4; ++k)
             p.regions[static_cast<size_t>(colony) * 128 + tid * 4 + k] = s_nodes[(tid * 4 + k) / 16][(tid * 4 + k) % 16];
+    }
+    if (p.views_only) {
+        if (tid == 0) {
+            if (p.masks) p.masks[colony] = s_mask;
+            if (p.actions) p.actions[colony] = static_cast<int32_t>(action);
+            if (p.alternatives) p.alternatives[colony] = alt;
+        }
+        return;
     }
     if (tid < kRegion) ctl->start_nodes[tid] = s_nodes[(action / 16) * 4 + (action / 4) % 4][tid];
     if (tid == 0) {

@@ -5,6 +5,7 @@
 // Boundary semantics follow the reference's native module gp_faco_ext
 // (cpp/src/bindings.cpp:500-1046) and engine host code
 // (cuda/batch_engine.cu:78-1192, cuda/fixed_faco.cu:44-177).
+#include "counted_state.h"
 #include "faco_common.h"
 #include "gp_program.h"
 #include "host_prepare.h"
@@ -13,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -187,6 +189,7 @@ struct Engine {
         DeviceArray<float> features, scores;        // [12][C][32], [C][32]
         DeviceArray<uint32_t> masks, alternatives, regions;  // [C], [C], [C][128]
         DeviceArray<int32_t> actions;               // [C]
+        DeviceArray<uint64_t> source_keys;          // [C] keys before a state fork
     };
     std::unique_ptr<ControlBuffers> ctl;
     bool resident_control = false;
@@ -244,6 +247,7 @@ struct Engine {
         alloc(b->programs, population); alloc(b->controllers, population);
         alloc(b->features, 12 * 32 * C); alloc(b->scores, 32 * C);
         alloc(b->masks, C); alloc(b->alternatives, C); alloc(b->regions, 128 * C); alloc(b->actions, C);
+        alloc(b->source_keys, C);
         ctl = std::move(b);
         resident.clear();  // the control geometry is uploaded with the next geometry upload
     }
@@ -526,6 +530,7 @@ using namespace faco;
 
 struct faco_engine { std::unique_ptr<Engine> impl; };
 struct faco_fixed { std::unique_ptr<Engine> impl; uint64_t key; };
+struct faco_state { faco::CountedState s; };
 
 namespace {
 
@@ -685,10 +690,103 @@ void validate_baseline(const faco_baseline_policy& p, uint32_t mask) {
 // Controlled evaluation: the panel replicated per decider, per batch
 // decide -> ants -> colony update -> control update (reference
 // cuda/batch_engine.cu:880-1060).
+// A snapshot taken after `capture_after` evaluations per colony, or one
+// resumed (with an optional paired fork of its first decision).
+struct StateIO {
+    CountedState* capture = nullptr;
+    uint64_t capture_after = 0;
+    const CountedState* resume = nullptr;
+    faco_fork fork{0, 0, 0, 2, 0};   // seed, enabled, region, mne
+};
+
+// Snapshot of a controlled run (CountedState container). The named buffers
+// are every device buffer that carries algorithm state from one batch to the
+// next, then the decision views of the next batch (features [12][C][32],
+// scores, masks, actions, alternatives, start regions [C][2][4][16]); all
+// are colony-major except the feature planes. "xy" (first) holds the
+// instances' integer coordinates [C][n][2] for the restore check.
+struct StateSlot {
+    const char* name;
+    uint64_t element, planes;
+    size_t count;       // elements over all C colonies
+    void* device;
+    bool state;         // restored on resume; the views are recomputed
+};
+
+std::vector<StateSlot> state_slots(Engine& e, uint32_t C) {
+    auto& b = *e.ctl;
+    const size_t n = e.n, cells = static_cast<size_t>(C) * n * e.config.primary_width;
+    const size_t archive = static_cast<size_t>(C) * 2 * kArchive * n;
+    return {{"keys", sizeof(uint64_t), 1, C, e.keys.data(), true},
+            {"colony_state", sizeof(ColonyState), 1, C, e.state.data(), true},
+            {"control_state", sizeof(ControlState), 1, C, b.control.data(), true},
+            {"parent", 4, 1, C * n, e.parent.data(), true},
+            {"links", 4, 1, C * n, e.links.data(), true},
+            {"epoch", 4, 1, C * n, e.epoch.data(), true},
+            {"global", 4, 1, C * n, e.global.data(), true},
+            {"trails", 8, 1, cells, e.trails.data(), true},
+            {"products", 8, 1, cells, e.products.data(), true},
+            {"archive_tours", 4, 1, archive, b.archive_tours.data(), true},
+            {"archive_links", 4, 1, archive, b.archive_links.data(), true},
+            {"features", 4, 12, static_cast<size_t>(12) * 32 * C, b.features.data(), false},
+            {"scores", 4, 1, static_cast<size_t>(32) * C, b.scores.data(), false},
+            {"masks", 4, 1, C, b.masks.data(), false},
+            {"actions", 4, 1, C, b.actions.data(), false},
+            {"alternatives", 4, 1, C, b.alternatives.data(), false},
+            {"regions", 4, 1, static_cast<size_t>(128) * C, b.regions.data(), false}};
+}
+
+std::vector<uint8_t> state_coordinates(const Engine& e, const std::vector<uint64_t>& inst) {
+    std::vector<uint8_t> out;
+    for (uint64_t key : inst) {
+        const auto& xy = e.problem(key).xy;
+        const auto* first = reinterpret_cast<const uint8_t*>(xy.data());
+        out.insert(out.end(), first, first + xy.size() * sizeof(int32_t));
+    }
+    return out;
+}
+
+// Restore (reference cuda/batch_engine.cu:231-261 and 829-856): the whole
+// buffer set is checked against this engine's layout and the registered
+// coordinates before anything is uploaded; the global tours and the host
+// incumbents must cost what the state says.
+void restore_state(Engine& e, const CountedState& saved, uint32_t C, const std::vector<uint64_t>& inst) {
+    const auto slots = state_slots(e, C);
+    require(saved.buffers.size() == slots.size() + 1, "snapshot buffer set incomplete or extra");
+    const StateBuffer& xy = saved.buffers.front();
+    require(xy.name == "xy" && xy.element_bytes == sizeof(int32_t) && xy.planes == 1,
+            "snapshot buffer name, layout or ABI does not match the engine");
+    require(xy.bytes == state_coordinates(e, inst), "snapshot coordinates differ from the registered instances");
+    for (size_t i = 0; i < slots.size(); ++i) {
+        const StateBuffer& source = saved.buffers[i + 1];
+        require(source.name == slots[i].name && source.element_bytes == slots[i].element &&
+                source.planes == slots[i].planes && source.bytes.size() == slots[i].count * slots[i].element,
+                "snapshot buffer name, layout or ABI does not match the engine");
+    }
+    const auto& states = saved.buffer("colony_state").bytes;
+    const auto& global = saved.buffer("global").bytes;
+    for (uint32_t c = 0; c < C; ++c) {
+        const Prepared& p = e.problem(inst[c]);
+        ColonyState st;
+        std::memcpy(&st, states.data() + static_cast<size_t>(c) * sizeof(ColonyState), sizeof st);
+        std::vector<uint32_t> tour(e.n);
+        std::memcpy(tour.data(), global.data() + static_cast<size_t>(c) * e.n * 4, static_cast<size_t>(e.n) * 4);
+        require(is_permutation(tour.data(), e.n) && tour_cost(p, tour.data()) == st.global_cost,
+                "snapshot global tour does not cost what its colony state says");
+        const StateIncumbent& item = saved.incumbents[c];
+        require(static_cast<double>(tour_cost(p, item.tour.data())) == item.cost,
+                "snapshot host incumbent cost changed");
+    }
+    for (size_t i = 0; i < slots.size(); ++i)
+        if (slots[i].state)
+            checked(hipMemcpyAsync(slots[i].device, saved.buffers[i + 1].bytes.data(), slots[i].count * slots[i].element,
+                                   hipMemcpyHostToDevice, e.stream), "snapshot upload");
+}
+
 void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* panel_seeds, uint32_t panel,
                       uint64_t limit, const faco_control_spec& spec, uint32_t flags, const faco_control_trace* trace,
                       faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
-                      faco_control_state* states, const faco_checkpoints* checkpoints) {
+                      faco_control_state* states, const faco_checkpoints* checkpoints, const StateIO* io = nullptr) {
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "one evaluation per engine at a time");
@@ -725,6 +823,39 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     const uint32_t C = panel * deciders;
     require(C <= e.colonies_capacity, "colonies exceed the engine's capacity");
     for (uint32_t t = 0; t < panel; ++t) (void)e.problem(panel_keys[t]);
+    // state snapshots (reference cuda/batch_engine.cu:536-554): plain
+    // program or baseline count runs; the progress feature keeps the
+    // source run's budget and a resumed run keeps its batch numbers
+    const bool capturing = io && io->capture, resuming = io && io->resume;
+    const faco_fork fork = io ? io->fork : faco_fork{0, 0, 0, 2, 0};
+    uint64_t progress = limit, run_limit = limit;
+    if (capturing || resuming) {
+        require(C <= 128, "a state snapshot holds at most 128 colonies");
+        require(!e.mixed && deciders == 1 && (spec.mode == FACO_CONTROL_PROGRAM || spec.mode == FACO_CONTROL_BASELINE) &&
+                !spec.force_fingerprint_collisions && !(flags & FACO_RUN_EVENTS) && !(checkpoints && checkpoints->count),
+                "state snapshots take plain program or baseline count runs, without profiling, forced collisions "
+                "or checkpoints");
+    }
+    if (capturing) {
+        require(limit > 0 && io->capture_after <= limit && io->capture_after % ants == 0,
+                "the capture point must be a whole number of ant batches inside the source budget");
+        run_limit = io->capture_after;
+    }
+    if (resuming) {
+        const CountedState& saved = *io->resume;
+        saved.validate();
+        require(saved.dimension == e.n && saved.colonies == C && same_settings(saved.settings, e.config),
+                "snapshot shape or solver settings differ from the engine");
+        require(limit <= saved.progress_evaluation_limit - saved.completed_batches * ants,
+                "a continuation cannot pass the source progress budget");
+        progress = saved.progress_evaluation_limit;
+    }
+    if (fork.enabled)
+        require(resuming && spec.mode == FACO_CONTROL_BASELINE && limit > 0 && fork.region < 4 &&
+                (fork.mne == 2 || fork.mne == 16),
+                "a paired fork starts from a snapshot, runs a baseline continuation and takes MNE 2 or 16 and a region 0..3");
+    else
+        require(fork.seed == 0 && fork.region == 0 && fork.mne == 2, "fork fields set without an enabled fork");
     std::vector<uint64_t> keys(C), inst(C);
     for (uint32_t c = 0; c < C; ++c) {
         inst[c] = panel_keys[c % panel];
@@ -733,7 +864,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     // a mixed-size batch runs until its longest colony is done; the others
     // stop at their own budget
     struct Inactive { Engine& e; ~Inactive() { e.shapes_active = false; } } inactive{e};
-    const uint64_t batches = e.mixed ? e.prepare_shapes(inst.data(), C, limit) : limit / ants;
+    const uint64_t batches = e.mixed ? e.prepare_shapes(inst.data(), C, limit) : run_limit / ants;
     std::vector<uint32_t> traced;
     if (trace && trace->count) {
         require(trace->batches != nullptr, "trace batches missing");
@@ -747,7 +878,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     // LDS of the archive update: per candidate cost, identity, start, leader, pair
     require(36ull * (kArchive + ants) <= max_lds_per_block(), "too many ants for the archive update");
     e.upload_geometry(inst.data(), C);
-    e.keys.upload(keys.data(), C, e.stream);
+    if (!resuming) e.keys.upload(keys.data(), C, e.stream);
     if (spec.mode == FACO_CONTROL_PROGRAM || spec.mode == FACO_CONTROL_FACTORIAL)
         b.programs.upload(spec.programs, deciders, e.stream);
     if (spec.mode == FACO_CONTROL_CONTROLLER) b.controllers.upload(spec.controllers, deciders, e.stream);
@@ -756,7 +887,8 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     cp.n = e.n; cp.ants = ants; cp.pw = e.config.primary_width; cp.K = e.K; cp.nbr_stride = e.nbr_stride;
     cp.panel = panel; cp.mode = spec.mode; cp.variant = spec.factorial_variant;
     cp.experiment_mask = spec.experiment_mask; cp.force_collisions = spec.force_fingerprint_collisions ? 1u : 0u;
-    cp.evaluation_limit = limit; cp.ls_limit = e.config.ls_evaluation_limit;
+    cp.forced_action = -1;
+    cp.evaluation_limit = progress; cp.ls_limit = e.config.ls_evaluation_limit;
     cp.retention = e.config.retention; cp.p_best_root = e.p_root; cp.baseline = spec.baseline;
     cp.coords = e.geo.coords.data(); cp.nbr = e.geo.nbr.data(); cp.eta = e.geo.eta.data();
     cp.local_scale = b.local_scale.data(); cp.epsilon = b.epsilon.data(); cp.samples = b.samples.data();
@@ -794,9 +926,18 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         checked(hipEventElapsedTime(&ms, ev[a], ev[b]), "event time");
         return static_cast<double>(ms);
     };
+    // a paired fork's first decision takes the forced action, with the start
+    // regions drawn under the source keys (reference batch_engine.cu:883-923:
+    // that batch keeps the snapshot's decision views); the ants and every
+    // later batch use the forked keys
+    ControlParams forced = cp;
+    forced.forced_action = static_cast<int32_t>(fork.region * 4 + (fork.mne == 2 ? 0 : 3));
+    forced.keys = b.source_keys.data();
+    uint64_t launched = 0;
     auto iteration = [&]() {
+        const bool first_fork = fork.enabled && launched++ == 0;
         if (events) checked(hipEventRecord(ev[0], e.stream), "event");
-        checked(launch_control_decide(cp, C, e.stream), "control decide launch");
+        checked(launch_control_decide(first_fork ? forced : cp, C, e.stream), "control decide launch");
         if (events) checked(hipEventRecord(ev[1], e.stream), "event");
         checked(launch_ant_search(ap, C * ap.blocks_per_colony, e.stream), "ant search launch");
         if (events) checked(hipEventRecord(ev[2], e.stream), "event");
@@ -812,8 +953,24 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
 
     Checkpoints marks(checkpoints, batches, static_cast<size_t>(C) * e.n);
     if (events) checked(hipEventRecord(ev[0], e.stream), "event");
-    e.reset(C, 2u);
-    checked(launch_control_init(cp, C, e.stream), "control init launch");
+    std::vector<ColonyState> resumed(resuming ? C : 0);   // work counters of the snapshot
+    std::vector<uint64_t> source_keys, forked_keys;
+    if (resuming) {
+        restore_state(e, *io->resume, C, inst);
+        std::memcpy(resumed.data(), io->resume->buffer("colony_state").bytes.data(), C * sizeof(ColonyState));
+        if (fork.enabled) {
+            // paired forks of one snapshot and seed share their keys
+            // (reference batch_engine.cu:839-845)
+            source_keys.resize(C);
+            std::memcpy(source_keys.data(), io->resume->buffer("keys").bytes.data(), C * sizeof(uint64_t));
+            for (uint64_t key : source_keys) forked_keys.push_back(mixed(key ^ mixed(fork.seed + 0xa0761d6478bd642fULL)));
+            b.source_keys.upload(source_keys.data(), C, e.stream);
+            e.keys.upload(forked_keys.data(), C, e.stream);
+        }
+    } else {
+        e.reset(C, 2u);
+        checked(launch_control_init(cp, C, e.stream), "control init launch");
+    }
     if (events) {
         checked(hipEventRecord(ev[1], e.stream), "event");
         checked(hipEventSynchronize(ev[1]), "event sync");
@@ -854,7 +1011,9 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         const bool graphs = (flags & FACO_RUN_GRAPH) != 0 && !events;
         hipGraphExec_t exec = nullptr;
         struct ExecGuard { hipGraphExec_t& x; ~ExecGuard() { if (x) (void)hipGraphExecDestroy(x); } } guard{exec};
-        for (uint64_t done = 0; done < batches;) {
+        uint64_t done = 0;
+        if (fork.enabled && batches) { iteration(); done = 1; }   // eager: the graph replays later batches
+        while (done < batches) {
             const uint64_t stop = std::max(marks.next(batches), done + 1);
             uint64_t left = stop - done;
             if (graphs && left >= kGraphIterations) {
@@ -893,19 +1052,164 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     marks.read_back(e.stream, e.n, [&](uint32_t c) { return e.problem(inst[c]).n; });
     uint64_t steps = 0, evals = 0;
     for (uint32_t c = 0; c < C; ++c) {
+        // a resumed run reports its own work, not the snapshot's
+        const uint64_t own_steps = st[c].work_construction - (resuming ? resumed[c].work_construction : 0);
+        const uint64_t own_evals = st[c].work_ls - (resuming ? resumed[c].work_ls : 0);
         if (costs) costs[c] = st[c].global_cost;
-        if (work) { work[2 * c] = st[c].work_construction; work[2 * c + 1] = st[c].work_ls; }
+        if (work) { work[2 * c] = own_steps; work[2 * c + 1] = own_evals; }
         if (states) {
             faco_control_state& o = states[c];
             o.archive_size = cs[c].archive_size; o.pad = 0;
             o.stagnant_batches = cs[c].stagnant; o.epoch_batches = cs[c].epoch_batches;
             o.restarts = cs[c].restarts; o.return_rate = cs[c].return_rate; o.ls_work = cs[c].ls_work;
         }
-        steps += st[c].work_construction; evals += st[c].work_ls;
+        steps += own_steps; evals += own_evals;
+    }
+    if (capturing) {
+        // the next decision's views under the source keys (reference
+        // batch_engine.cu:207-229), then every buffer of the state
+        ControlParams views = cp;
+        views.views_only = 1;
+        checked(launch_control_decide(views, C, e.stream), "decision views launch");
+        CountedState& out = *io->capture;
+        out = CountedState{};
+        out.dimension = e.n; out.colonies = C; out.settings = e.config;
+        out.experiment_mask = spec.experiment_mask;
+        out.completed_batches = batches; out.progress_evaluation_limit = progress;
+        for (uint32_t c = 0; c < C; ++c) out.tasks.emplace_back(panel_keys[c], panel_seeds[c]);
+        const auto slots = state_slots(e, C);
+        out.buffers.resize(slots.size() + 1);
+        out.buffers[0].name = "xy"; out.buffers[0].element_bytes = sizeof(int32_t);
+        out.buffers[0].bytes = state_coordinates(e, inst);
+        for (size_t i = 0; i < slots.size(); ++i) {
+            StateBuffer& target = out.buffers[i + 1];
+            target.name = slots[i].name; target.element_bytes = slots[i].element; target.planes = slots[i].planes;
+            target.bytes.resize(slots[i].count * slots[i].element);
+            checked(hipMemcpyAsync(target.bytes.data(), slots[i].device, target.bytes.size(), hipMemcpyDeviceToHost,
+                                   e.stream), "snapshot download");
+        }
+        checked(hipStreamSynchronize(e.stream), "snapshot download");
+        const auto& global = out.buffer("global").bytes;
+        for (uint32_t c = 0; c < C; ++c) {
+            StateIncumbent item;
+            item.cost = static_cast<double>(st[c].global_cost);
+            item.tour.resize(e.n);
+            std::memcpy(item.tour.data(), global.data() + static_cast<size_t>(c) * e.n * 4, static_cast<size_t>(e.n) * 4);
+            out.incumbents.push_back(std::move(item));
+        }
+        out.validate();
     }
     fill_result(result, e, C, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
                 seconds_since(started), search, gpu_ms);
     if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+}
+
+// JSON view of a snapshot: the fields of the reference's describe()
+// (cpp/src/bindings.cpp:422-463), read from this engine's buffer ABI.
+struct Json {
+    std::string text;
+    void raw(const std::string& v) { text += v; }
+    void number(uint64_t v) { text += std::to_string(v); }
+    void real(double v) {
+        char buffer[40];
+        std::snprintf(buffer, sizeof buffer, "%.17g", v);
+        std::string value(buffer);
+        if (value.find_first_of(".en") == std::string::npos) value += ".0";
+        text += value;
+    }
+    void key(const char* name) {
+        if (!text.empty() && text.back() != '{' && text.back() != '[') text += ',';
+        text += '"'; text += name; text += "\":";
+    }
+    void item() { if (!text.empty() && text.back() != '[' && text.back() != '{') text += ','; }
+};
+
+template<class T> T state_record(const CountedState& s, const char* name, uint32_t colony) {
+    const StateBuffer& b = s.buffer(name);
+    require(b.element_bytes == sizeof(T) && b.planes == 1 && b.bytes.size() >= (colony + 1ull) * sizeof(T),
+            "snapshot field does not match the native ABI");
+    T value;
+    std::memcpy(&value, b.bytes.data() + static_cast<size_t>(colony) * sizeof(T), sizeof(T));
+    return value;
+}
+
+std::string describe_state(const CountedState& s) {
+    s.validate();
+    const StateBuffer& regions = s.buffer("regions");
+    require(regions.element_bytes == 4 && regions.bytes.size() == static_cast<size_t>(s.colonies) * 128 * 4,
+            "snapshot field does not match the native ABI");
+    Json j;
+    j.raw("{");
+    j.key("state_spec_id"); j.number(2);
+    j.key("native_abi_bound"); j.raw("true");
+    j.key("dimension"); j.number(s.dimension);
+    j.key("colonies"); j.number(s.colonies);
+    j.key("completed_batches"); j.number(s.completed_batches);
+    j.key("completed_evaluations_per_colony"); j.number(s.completed_batches * s.settings.ants);
+    j.key("progress_evaluation_limit"); j.number(s.progress_evaluation_limit);
+    j.key("experiment_mask"); j.number(s.experiment_mask);
+    const faco_settings& c = s.settings;
+    j.key("settings"); j.raw("{");
+    j.key("ants"); j.number(c.ants); j.key("primary_width"); j.number(c.primary_width);
+    j.key("backup_width"); j.number(c.backup_width); j.key("ls_width"); j.number(c.ls_width);
+    j.key("beta"); j.real(c.beta); j.key("retention"); j.real(c.retention); j.key("p_best"); j.real(c.p_best);
+    j.key("epoch_source_probability"); j.real(c.epoch_source_probability);
+    j.key("ls_evaluation_limit"); j.number(c.ls_evaluation_limit);
+    j.key("initial_ls_evaluation_limit"); j.number(c.initial_ls_evaluation_limit);
+    j.raw("}");
+    j.key("random_keys"); j.raw("[");
+    for (uint32_t i = 0; i < s.colonies; ++i) { j.item(); j.number(state_record<uint64_t>(s, "keys", i)); }
+    j.raw("]");
+    j.key("tasks"); j.raw("[");
+    for (const auto& t : s.tasks) {
+        j.item(); j.raw("{");
+        j.key("instance_key"); j.number(t.first); j.key("seed"); j.number(t.second);
+        j.raw("}");
+    }
+    j.raw("]");
+    j.key("members"); j.raw("[");
+    const uint32_t count = s.dimension < kRegion ? s.dimension : kRegion;
+    for (uint32_t i = 0; i < s.colonies; ++i) {
+        const auto control = state_record<ControlState>(s, "control_state", i);
+        j.item(); j.raw("{");
+        j.key("incumbent_cost"); j.real(s.incumbents[i].cost);
+        j.key("return_rate"); j.real(control.return_rate);
+        j.key("ls_work"); j.real(control.ls_work);
+        j.key("stagnant_batches"); j.number(control.stagnant);
+        j.key("epoch_batches"); j.number(control.epoch_batches);
+        j.key("restarts"); j.number(control.restarts);
+        j.key("archive_size"); j.number(control.archive_size);
+        j.key("keep_regions"); j.raw("[");
+        for (uint32_t r = 0; r < 4; ++r) {
+            j.item(); j.raw("[");
+            for (uint32_t k = 0; k < count; ++k) {
+                uint32_t node;
+                std::memcpy(&node, regions.bytes.data() + ((static_cast<size_t>(i) * 128 + r * 16 + k) * 4), 4);
+                j.item(); j.number(node);
+            }
+            j.raw("]");
+        }
+        j.raw("]}");
+    }
+    j.raw("]");
+    j.key("buffers"); j.raw("[");
+    for (const auto& b : s.buffers) {
+        j.item(); j.raw("{");
+        j.key("name"); j.raw("\"" + b.name + "\"");
+        j.key("bytes"); j.number(b.bytes.size());
+        j.key("element_bytes"); j.number(b.element_bytes);
+        j.key("planes"); j.number(b.planes);
+        j.raw("}");
+    }
+    j.raw("]}");
+    return j.text;
+}
+
+void copy_out(const std::vector<uint8_t>& data, uint8_t* out, size_t capacity, size_t* size) {
+    if (size) *size = data.size();
+    if (!out) return;
+    require(capacity >= data.size(), "output buffer too small");
+    std::memcpy(out, data.data(), data.size());
 }
 
 }  // namespace
@@ -1205,6 +1509,85 @@ int faco_engine_evaluate_control(faco_engine* engine, const uint64_t* instance_k
         require(engine && instance_keys && seeds && spec, "null argument");
         evaluate_control(*engine->impl, instance_keys, seeds, panel, limit, *spec, flags, trace, result, tours, costs,
                          work, states, checkpoints);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// CountedState snapshots (reference cpp/src/bindings.cpp:541-556, 880-934).
+int faco_state_from_bytes(const uint8_t* bytes, size_t size, faco_state** out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        auto state = std::make_unique<faco_state>();
+        state->s = CountedState::deserialize(bytes, size);
+        *out = state.release();
+    });
+}
+
+int faco_state_to_bytes(const faco_state* state, uint8_t* out, size_t capacity, size_t* size) {
+    return guarded([&] {
+        require(state != nullptr, "null argument");
+        state->s.validate();
+        if (!out) { if (size) *size = state->s.serialized_size(); return; }
+        copy_out(state->s.serialize(), out, capacity, size);
+    });
+}
+
+int faco_state_describe(const faco_state* state, char* out, size_t capacity, size_t* size) {
+    return guarded([&] {
+        require(state != nullptr, "null argument");
+        const std::string text = describe_state(state->s);
+        if (size) *size = text.size() + 1;
+        if (!out) return;
+        require(capacity > text.size(), "output buffer too small");
+        std::memcpy(out, text.c_str(), text.size() + 1);
+    });
+}
+
+int faco_state_select_colony(const faco_state* state, uint32_t colony, faco_state** out) {
+    return guarded([&] {
+        require(state != nullptr && out != nullptr, "null argument");
+        auto selected = std::make_unique<faco_state>();
+        selected->s = state->s.select_colony(colony);
+        *out = selected.release();
+    });
+}
+
+void faco_state_destroy(faco_state* state) { delete state; }
+
+int faco_engine_capture_control(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                                uint32_t tasks, uint64_t source_limit, uint64_t capture_after,
+                                const faco_control_spec* spec, uint32_t flags, const faco_control_trace* trace,
+                                faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
+                                faco_control_state* states, faco_state** out) {
+    return guarded([&] {
+        require(engine && instance_keys && seeds && spec && out, "null argument");
+        auto state = std::make_unique<faco_state>();
+        StateIO io;
+        io.capture = &state->s;
+        io.capture_after = capture_after;
+        evaluate_control(*engine->impl, instance_keys, seeds, tasks, source_limit, *spec, flags, trace, result, tours,
+                         costs, work, states, nullptr, &io);
+        *out = state.release();
+    });
+}
+
+int faco_engine_continue_control(faco_engine* engine, const faco_state* state, uint64_t additional,
+                                 const faco_control_spec* spec, const faco_fork* fork, uint32_t flags,
+                                 const faco_control_trace* trace, faco_result* result, uint32_t* tours,
+                                 int64_t* costs, uint64_t* work, faco_control_state* states) {
+    return guarded([&] {
+        require(engine && state && spec, "null argument");
+        const CountedState& saved = state->s;
+        saved.validate();
+        std::vector<uint64_t> keys, seeds;
+        for (const auto& t : saved.tasks) { keys.push_back(t.first); seeds.push_back(t.second); }
+        faco_control_spec resumed = *spec;
+        resumed.experiment_mask = saved.experiment_mask;   // the snapshot's mask holds for the continuation
+        StateIO io;
+        io.resume = &saved;
+        if (fork) io.fork = *fork;
+        evaluate_control(*engine->impl, keys.data(), seeds.data(), saved.colonies, additional, resumed, flags, trace,
+                         result, tours, costs, work, states, nullptr, &io);
     });
 }
 

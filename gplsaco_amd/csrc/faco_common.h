@@ -290,6 +290,9 @@ struct ControlParams {
     uint32_t variant;                   // factorial variant
     uint32_t experiment_mask;
     uint32_t force_collisions;
+    int32_t forced_action;              // >= 0: every colony takes this action (a state fork's first batch)
+    uint32_t views_only;                 // decide: write features, scores, masks, actions, alternatives and
+                                        // regions only; colony and control state stay unchanged
     uint64_t evaluation_limit;          // FE per colony (progress feature)
     uint64_t ls_limit;                  // LS evaluation limit (LS work feature)
     double retention, p_best_root;

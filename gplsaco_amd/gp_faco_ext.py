@@ -13,14 +13,15 @@ Differences from the reference, by design of this engine:
 from __future__ import annotations
 
 import ctypes
+import json
 import struct
 import time
 
 import numpy as np
 
 from . import _native
-from ._native import (BaselinePolicy, Checkpoints, ControlSpec, ControlState, ControlTrace, Controller, Program, Result,
-                      Settings,
+from ._native import (BaselinePolicy, Checkpoints, ControlSpec, ControlState, ControlTrace, Controller, Fork, Program,
+                      Result, Settings,
                       TuneEntry, check, lib, ptr)
 
 numeric_backend = "euc2d_int32"
@@ -378,6 +379,59 @@ def _batch_output(result: Result, tours: np.ndarray, costs: np.ndarray, mode: st
     }
 
 
+class CountedState:
+    """Snapshot of a controlled run after a whole number of batches
+    (reference cpp/include/gp_faco/counted_state.hpp:1-48, bindings.cpp:541-556).
+
+    The bytes are the reference's container (magic "GFACOST2", 64-bit
+    little-endian fields, named buffers); the buffers carry this library's
+    device state, so a snapshot resumes only in an engine of this library."""
+
+    def __init__(self, handle):
+        self._handle = handle
+        self._colonies = None
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle:
+            lib().faco_state_destroy(handle)
+            self._handle = None
+
+    @staticmethod
+    def from_bytes(data) -> "CountedState":
+        if not isinstance(data, (bytes, bytearray)):
+            raise TypeError("snapshot bytes must be bytes")
+        handle = ctypes.c_void_p()
+        check(lib().faco_state_from_bytes(bytes(data), len(data), ctypes.byref(handle)))
+        return CountedState(handle)
+
+    def to_bytes(self) -> bytes:
+        size = ctypes.c_size_t()
+        check(lib().faco_state_to_bytes(self._handle, None, 0, ctypes.byref(size)))
+        buffer = ctypes.create_string_buffer(max(size.value, 1))
+        check(lib().faco_state_to_bytes(self._handle, buffer, size.value, ctypes.byref(size)))
+        return buffer.raw[:size.value]
+
+    def describe(self) -> dict:
+        size = ctypes.c_size_t()
+        check(lib().faco_state_describe(self._handle, None, 0, ctypes.byref(size)))
+        buffer = ctypes.create_string_buffer(size.value)
+        check(lib().faco_state_describe(self._handle, buffer, size.value, ctypes.byref(size)))
+        return json.loads(buffer.value.decode())
+
+    @property
+    def colonies(self) -> int:
+        if self._colonies is None:
+            self._colonies = self.describe()["colonies"]
+        return self._colonies
+
+    def select_colony(self, colony) -> "CountedState":
+        index = _unsigned(colony, "colony", 32)
+        handle = ctypes.c_void_p()
+        check(lib().faco_state_select_colony(self._handle, index, ctypes.byref(handle)))
+        return CountedState(handle)
+
+
 class FacoBatchEngine:
     """Fixed-shape multi-colony engine (reference batch_engine.hpp:68-145)."""
 
@@ -580,6 +634,21 @@ class FacoBatchEngine:
         if keys.size != seeds.size:
             raise ValueError("controlled entry points need complete task arrays")
         limit = _unsigned(evaluation_limit_per_colony, "evaluation_limit_per_colony")
+
+        def call(spec, flags, trace, result, tours, costs, work, states, checkpoints):
+            return lib().faco_engine_evaluate_control(self._handle, keys, seeds, keys.size, limit, spec, flags, trace,
+                                                      result, tours, costs, work, states, checkpoints)
+        return self._control_run(keys.size, mode, call, programs=programs, controllers=controllers,
+                                 baseline=baseline, variant=variant, experiment_mask=experiment_mask,
+                                 trace_batches=trace_batches, use_graph=use_graph,
+                                 force_fingerprint_collisions=force_fingerprint_collisions,
+                                 checkpoint_iterations=checkpoint_iterations, profile_events_only=profile_events_only)
+
+    def _control_run(self, panel, mode, call, *, programs=(), controllers=(), baseline=None, variant=3,
+                     experiment_mask=0xFFFFFFFF, trace_batches=(), use_graph=True, force_fingerprint_collisions=False,
+                     checkpoint_iterations=(), profile_events_only=False) -> dict:
+        """One controlled native call over `panel` tasks: `call` receives the
+        spec, flags, trace and output pointers of faco_engine_evaluate_control."""
         spec = ControlSpec()
         spec.mode = {"program": 1, "controller": 2, "baseline": 3, "factorial": 4}[mode]
         spec.experiment_mask = _unsigned(experiment_mask, "experiment_mask", 32)
@@ -598,7 +667,6 @@ class FacoBatchEngine:
             spec.deciders = 1
         if baseline is not None:
             spec.baseline = baseline
-        panel = keys.size
         C = panel * spec.deciders
         tours = np.zeros((C, self.dimension), dtype=np.uint32)
         costs = np.zeros(C, dtype=np.int64)
@@ -623,11 +691,9 @@ class FacoBatchEngine:
             for name, array in traces.items():
                 setattr(trace, name, array.ctypes.data_as(ctypes.c_void_p))
         started = time.perf_counter()
-        check(lib().faco_engine_evaluate_control(self._handle, keys, seeds, panel, limit, ctypes.byref(spec),
-                                                 (1 if use_graph else 0) | (4 if profile_events_only else 0),
-                                                 ctypes.byref(trace) if trace else None,
-                                                 ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1),
-                                                 ctypes.addressof(states), request.argument()))
+        check(call(ctypes.byref(spec), (1 if use_graph else 0) | (4 if profile_events_only else 0),
+                   ctypes.byref(trace) if trace else None, ctypes.byref(result), tours.reshape(-1), costs,
+                   work.reshape(-1), ctypes.addressof(states), request.argument()))
         del prog_arr, ctrl_arr
         return {"result": result, "tours": tours, "costs": costs, "work": work, "elapsed": time.perf_counter() - started,
                 "checkpoints": request.output() if request.points else None,
@@ -715,6 +781,93 @@ class FacoBatchEngine:
         out = self._control_output(raw, list(decision_iterations), "fixed")
         out["baseline_policy"] = baseline_output(b)
         return out
+
+    # ------------------------------------------------------------------
+    # State snapshots and paired forks (reference bindings.cpp:880-934,
+    # batch_engine.cu:465-490)
+
+    @staticmethod
+    def _state_decider(kind, argument) -> dict:
+        if kind == "program":
+            return {"programs": [read_program(argument)]}
+        return {"baseline": read_baseline(argument)}
+
+    def _capture_state(self, kind, instance_keys, seeds, source_evaluations, capture_after, argument,
+                       experiment_mask, decision_iterations=()) -> dict:
+        keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
+        if keys.size != seeds.size:
+            raise ValueError("a capture needs complete task arrays")
+        source = _unsigned(source_evaluations, "source_evaluation_limit_per_colony")
+        after = _unsigned(capture_after, "capture_after_evaluations")
+        mask = _unsigned(experiment_mask, "experiment_mask", 32)
+        decider = self._state_decider(kind, argument)
+        handle = ctypes.c_void_p()
+
+        def call(spec, flags, trace, result, tours, costs, work, states, checkpoints):
+            return lib().faco_engine_capture_control(self._handle, keys, seeds, keys.size, source, after, spec, flags,
+                                                     trace, result, tours, costs, work, states, ctypes.byref(handle))
+        batches = self._decision_batches(decision_iterations, after, self.settings.ants)
+        raw = self._control_run(keys.size, kind, call, experiment_mask=mask, trace_batches=batches, **decider)
+        state = CountedState(handle)
+        evaluation = self._control_output(raw, list(decision_iterations), "program" if kind == "program" else "fixed")
+        if kind == "baseline":
+            evaluation["baseline_policy"] = baseline_output(decider["baseline"])
+        return {"evaluation": evaluation, "state": state}
+
+    def _continue_state(self, kind, state, additional_evaluations, argument, fork=None, decision_iterations=()) -> dict:
+        if not isinstance(state, CountedState):
+            raise TypeError("state must be a CountedState")
+        additional = _unsigned(additional_evaluations, "additional_evaluations_per_colony")
+        decider = self._state_decider(kind, argument)
+
+        def call(spec, flags, trace, result, tours, costs, work, states, checkpoints):
+            return lib().faco_engine_continue_control(self._handle, state._handle, additional, spec,
+                                                      ctypes.byref(fork) if fork is not None else None, flags, trace,
+                                                      result, tours, costs, work, states)
+        batches = self._decision_batches(decision_iterations, additional, self.settings.ants)
+        raw = self._control_run(state.colonies, kind, call, trace_batches=batches, **decider)
+        out = self._control_output(raw, list(decision_iterations), "program" if kind == "program" else "fixed")
+        if kind == "baseline":
+            out["baseline_policy"] = baseline_output(decider["baseline"])
+        return out
+
+    def capture_program_state(self, instance_keys, seeds, source_evaluation_limit_per_colony,
+                              capture_after_evaluations, program, experiment_mask=0xFFFFFFFF, *,
+                              decision_iterations=()) -> dict:
+        """Run a program-controlled panel for capture_after_evaluations per
+        colony (the progress feature still measured against the source
+        budget) and return {"evaluation": ..., "state": CountedState}.
+        decision_iterations (1-based batches of this call) keeps those
+        decisions as in evaluate_program_evaluations."""
+        return self._capture_state("program", instance_keys, seeds, source_evaluation_limit_per_colony,
+                                   capture_after_evaluations, program, experiment_mask, decision_iterations)
+
+    def capture_baseline_state(self, instance_keys, seeds, source_evaluation_limit_per_colony,
+                               capture_after_evaluations, policy, experiment_mask=0xFFFFFFFF, *,
+                               decision_iterations=()) -> dict:
+        return self._capture_state("baseline", instance_keys, seeds, source_evaluation_limit_per_colony,
+                                   capture_after_evaluations, policy, experiment_mask, decision_iterations)
+
+    def continue_program_state(self, state, additional_evaluations_per_colony, program, *,
+                               decision_iterations=()) -> dict:
+        """Resume a snapshot for additional evaluations per colony; capture +
+        continuation equals the uninterrupted run. decision_iterations count
+        the batches of this call from 1."""
+        return self._continue_state("program", state, additional_evaluations_per_colony, program,
+                                    decision_iterations=decision_iterations)
+
+    def continue_baseline_state(self, state, additional_evaluations_per_colony, policy, fork_seed=None, region=0,
+                                mne=2, *, decision_iterations=()) -> dict:
+        """Resume under a baseline policy; with fork_seed the first decision
+        is forced to region * 4 + (0 for MNE 2, 3 for MNE 16) and the keys
+        are re-derived from the seed (a paired intervention)."""
+        fork = Fork()
+        fork.enabled = 0 if fork_seed is None else 1
+        fork.seed = 0 if fork_seed is None else _unsigned(fork_seed, "fork_seed")
+        fork.region = _unsigned(region, "region", 32)
+        fork.mne = _unsigned(mne, "mne", 32)
+        return self._continue_state("baseline", state, additional_evaluations_per_colony, policy, fork,
+                                    decision_iterations)
 
     def evaluate_factorial_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, program, policy,
                                        preparation_mode="cached", experiment_mask=0xFFFFFFFF, *,

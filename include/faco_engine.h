@@ -346,6 +346,72 @@ int faco_engine_evaluate_control(faco_engine* engine, const uint64_t* instance_k
                                  faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
                                  faco_control_state* states, const faco_checkpoints* checkpoints);
 
+/* ---------------------------------------------------------------------
+ * CountedState snapshots of controlled runs (reference
+ * cpp/include/gp_faco/counted_state.hpp, cpp/src/counted_state.cpp,
+ * batch_engine.hpp:119-131, bindings.cpp:541-556 and 880-934). The
+ * container format is the reference's: little-endian 64-bit fields, magic
+ * "GFACOST2" (an older "GFACOST1" file has 8 trailing bytes that are
+ * skipped), settings, completed batches, the source progress budget,
+ * (instance key, seed) tasks, host incumbents and named buffers
+ * (name, element bytes, planes, bytes). The buffers hold this library's
+ * device ABI, so a snapshot resumes only in an engine of this library with
+ * the same dimension, colony count, settings and registered coordinates. */
+typedef struct faco_state faco_state;
+
+/* ForkIntervention (reference counted_state.hpp:40-44): a resumed baseline
+ * run whose first decision is forced to action region * 4 + (mne == 2 ? 0
+ * : 3) with the snapshot's start regions, under keys re-derived from `seed`.
+ * A disabled fork must leave seed 0, region 0 and mne 2. */
+typedef struct faco_fork {
+    uint64_t seed;
+    uint32_t enabled;
+    uint32_t region;   /* 0..3 */
+    uint32_t mne;      /* 2 or 16 */
+    uint32_t pad;
+} faco_fork;
+
+/* CountedState.from_bytes: parse and validate (truncated, extended or
+ * malformed containers fail with FACO_INVALID_ARGUMENT). */
+int faco_state_from_bytes(const uint8_t* bytes, size_t size, faco_state** out);
+/* CountedState.to_bytes: with out NULL only *size is set. */
+int faco_state_to_bytes(const faco_state* state, uint8_t* out, size_t capacity, size_t* size);
+/* CountedState.describe as a JSON object (NUL-terminated; *size counts the
+ * NUL): state_spec_id, dimension, colonies, completed batches and
+ * evaluations, progress budget, mask, settings, random keys, tasks, per
+ * colony members (incumbent cost, feedback, archive size, the four keep
+ * regions of the next decision) and the buffer list. */
+int faco_state_describe(const faco_state* state, char* out, size_t capacity, size_t* size);
+/* CountedState.select_colony: a one-colony snapshot, every buffer cut to the
+ * colony's slice of each plane. */
+int faco_state_select_colony(const faco_state* state, uint32_t colony, faco_state** out);
+void faco_state_destroy(faco_state* state);
+
+/* capture_program_state / capture_baseline_state (reference
+ * batch_engine.cu:465-477): a PROGRAM or BASELINE run (one decider) of the
+ * `tasks` colonies that stops after capture_after evaluations per colony (a
+ * whole number of batches, at most source_limit) with the progress feature
+ * measured against source_limit, and returns its state. The outputs are
+ * those of faco_engine_evaluate_control for the run up to the capture. */
+int faco_engine_capture_control(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                                uint32_t tasks, uint64_t source_limit, uint64_t capture_after,
+                                const faco_control_spec* spec, uint32_t flags, const faco_control_trace* trace,
+                                faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
+                                faco_control_state* states, faco_state** out);
+/* continue_program_state / continue_baseline_state (reference
+ * batch_engine.cu:479-490, 829-856): restores the snapshot (after checking
+ * its buffers, coordinates and costs) and runs `additional` more
+ * evaluations per colony, at most what is left of the source budget; the
+ * batch numbers, keys and progress feature continue the source run, so a
+ * capture plus its continuation equals the uninterrupted run bit for bit.
+ * The snapshot's experiment mask overrides spec->experiment_mask. `fork`
+ * (NULL = none) applies to baseline continuations. Work counters report the
+ * continuation only. */
+int faco_engine_continue_control(faco_engine* engine, const faco_state* state, uint64_t additional,
+                                 const faco_control_spec* spec, const faco_fork* fork, uint32_t flags,
+                                 const faco_control_trace* trace, faco_result* result, uint32_t* tours,
+                                 int64_t* costs, uint64_t* work, faco_control_state* states);
+
 /* score_controller_cuda (reference cpp/src/bindings.cpp:1010-1030,
  * cuda/gp_score.cuh:37-62) and its host twin: features FP32 [12][colonies][32],
  * masks [colonies] -> scores [colonies][32] (kind 1: restart, region and

@@ -1441,7 +1441,14 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           u32* global_tours, std::int64_t* global_costs, u64* work, ControlOut* states,
                           float* trace_features, float* trace_scores, u32* trace_masks, std::int32_t* trace_actions,
                           u32* trace_alternatives, u32* trace_regions, std::int64_t* trace_costs,
-                          double* elapsed_search_seconds, u32 local_search) {
+                          double* elapsed_search_seconds, u32 local_search, u32 stop_batch, u32 fork_batch,
+                          u64 fork_seed, std::int32_t fork_action) {
+    // stop_batch (0 = the whole budget) ends the run early with the progress
+    // feature still measured against evaluation_limit. At batch fork_batch
+    // (UINT32_MAX = none) a paired fork (reference cuda/batch_engine.cu:839-845,
+    // 883-923) replaces the decision by fork_action, the decision itself
+    // still computed under the colony's keys so far, and from that batch's
+    // ants on every colony runs under mixed(key ^ mixed(fork_seed + golden)).
     return guarded([&] {
         if (!instances || !colonies || colonies % instances) throw std::invalid_argument("invalid panel");
         if (mode < 1 || mode > 4) throw std::invalid_argument("unknown control mode");
@@ -1460,7 +1467,8 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
         }
         const Settings s = problems[0].s;
         if (evaluation_limit % s.ants) throw std::invalid_argument("evaluation limit must be whole ant batches");
-        const u32 batches = static_cast<u32>(evaluation_limit / s.ants);
+        const u32 batches = stop_batch ? std::min(stop_batch, static_cast<u32>(evaluation_limit / s.ants))
+                                       : static_cast<u32>(evaluation_limit / s.ants);
         std::vector<ColonyRun> runs(colonies);
         std::vector<Control> controls(colonies);
         std::vector<u64> keys(colonies);
@@ -1500,6 +1508,10 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                     d.action = score_program(programs[who < deciders ? who : 0], d.features, d.mask, d.scores);
                 } else {
                     d.action = score_controller(controllers[who < deciders ? who : 0], d.features, d.mask, d.scores);
+                }
+                if (batch == fork_batch) {
+                    d.action = static_cast<u32>(fork_action);
+                    keys[c] = mixed(keys[c] ^ mixed(fork_seed + 0xa0761d6478bd642fULL));
                 }
                 if (d.action >= 32) throw std::runtime_error("no legal control action");
                 targets[c] = 2u << (d.action % 4);

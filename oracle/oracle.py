@@ -237,7 +237,8 @@ def lib():
                                             ctypes.POINTER(Baseline), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                             ctypes.c_int,
                                             u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
-                                            ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]
+                                            ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
         L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.c_uint32]
@@ -776,8 +777,12 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
-                   trace=False, force_collisions=False, or_opt=False):
-    """Controlled FACO: colony c uses instance c % instances and decider c // instances."""
+                   trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None):
+    """Controlled FACO: colony c uses instance c % instances and decider c // instances.
+
+    stop_batch (0 = all) ends the run after that many batches, the progress
+    feature still over evaluation_limit; fork = (batch, seed, action) forces
+    that batch's action and re-keys every colony from its ants on."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -790,6 +795,8 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
     elapsed = ctypes.c_double()
     norm = prepare(xy_instances[0], settings)["settings"]
     batches = evaluation_limit // norm.ants
+    if stop_batch:
+        batches = min(batches, stop_batch)
     traces = {}
     if trace:
         traces = {"features": np.zeros((batches, 12, colonies, 32), dtype=np.float32),
@@ -818,7 +825,8 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         work.reshape(-1),
         ctypes.addressof(states), ptr(traces.get("features")), ptr(traces.get("scores")),
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
-        ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt))), L)
+        ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt)),
+        stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
                        "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,

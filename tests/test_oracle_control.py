@@ -356,3 +356,56 @@ def test_hold_only_mask_never_restarts(panel):
     with pytest.raises(ValueError):
         O.run_controlled(xy, keys, np.array([1, 2], dtype=np.uint64), s, 8 * 10, "program",
                          programs=[RESTARTY], experiment_mask=0xFFFF0000)
+
+
+# ---------------------------------------------------------------------------
+# stopping early and paired forks (the oracle side of state snapshots,
+# reference cuda/batch_engine.cu:839-845 and 883-923)
+
+def _same_prefix(a, b, batches):
+    for name in ("features", "scores", "masks", "actions", "alternatives", "regions", "costs"):
+        x, y = a["trace_" + name][:batches], b["trace_" + name][:batches]
+        assert (x.view(np.uint8) == y.view(np.uint8)).all(), name
+
+
+def test_stop_batch_is_the_prefix_of_the_full_run(panel):
+    xy, keys = panel
+    seeds = np.array([4, 9], dtype=np.uint64)
+    s = O.default_settings(ants=8)
+    full = O.run_controlled(xy, keys, seeds, s, 8 * 9, "program", programs=[RESTARTY], trace=True)
+    for stop in (1, 5):
+        part = O.run_controlled(xy, keys, seeds, s, 8 * 9, "program", programs=[RESTARTY], trace=True,
+                                stop_batch=stop)
+        assert part["trace_actions"].shape[0] == stop
+        _same_prefix(part, full, stop)
+        assert part["costs"].tolist() == full["trace_costs"][stop - 1, :, 1].tolist()
+    # the progress feature (feature 0) runs over the whole budget, not the stop
+    assert abs(float(part["trace_features"][4, 0, 0, 0]) - 4 / 9) < 1e-6
+
+
+@pytest.mark.parametrize("region,level", [(0, 0), (2, 3)])
+def test_fork_forces_one_decision_and_rekeys_the_ants(panel, region, level):
+    xy, keys = panel
+    seeds = np.array([4, 9], dtype=np.uint64)
+    s = O.default_settings(ants=8)
+    base = policy_of(kind=0, mne_level=1, max_mne_level=1, region=1)
+    full = O.run_controlled(xy, keys, seeds, s, 8 * 9, "baseline", baseline=base, trace=True)
+    cut, action = 4, region * 4 + level
+    fork = O.run_controlled(xy, keys, seeds, s, 8 * 9, "baseline", baseline=base, trace=True, stop_batch=7,
+                            fork=(cut, 53, action))
+    _same_prefix(fork, full, cut)
+    # the forced batch's decision inputs are those of the unforked run: the
+    # regions and features are drawn before the keys change
+    for name in ("features", "masks", "alternatives", "regions"):
+        assert (fork["trace_" + name][cut].view(np.uint8) == full["trace_" + name][cut].view(np.uint8)).all(), name
+    assert fork["trace_actions"][cut].tolist() == [action] * 2
+    assert full["trace_actions"][cut].tolist() == [1 * 4 + 1] * 2
+    assert fork["trace_actions"][cut + 1:].tolist() == full["trace_actions"][cut + 1:7].tolist()
+    # later regions come from the new keys
+    assert not (fork["trace_regions"][cut + 1:7] == full["trace_regions"][cut + 1:7]).all()
+    again = O.run_controlled(xy, keys, seeds, s, 8 * 9, "baseline", baseline=base, stop_batch=7, fork=(cut, 53, action))
+    assert again["tours"].tolist() == fork["tours"].tolist() and again["work"].tolist() == fork["work"].tolist()
+    other = O.run_controlled(xy, keys, seeds, s, 8 * 9, "baseline", baseline=base, trace=True, stop_batch=7,
+                             fork=(cut, 54, action))
+    assert (other["trace_features"][cut].view(np.uint32) == fork["trace_features"][cut].view(np.uint32)).all()
+    assert not (other["trace_regions"][cut + 1:] == fork["trace_regions"][cut + 1:]).all()
