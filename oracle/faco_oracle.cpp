@@ -79,16 +79,28 @@ void philox4x32_10(const u32 in[4], const u32 key_in[2], u32 out[4]) {
 // (batch << 32) | ant and the step selects a 128-bit counter block. The
 // stream yields 32-bit words in block order, four per block; the 128-bit
 // counter (step, 0, ant, batch) increments with carry.
+//
+// For statistical checks only, g_alternate_stream replaces every stream by
+// an unrelated generator (SplitMix64 over a per-(key, batch, ant, step) base,
+// high 32 bits of each output): the same algorithm driven by independent
+// uniforms, so its tour-length distribution can be compared with Philox's.
+bool g_alternate_stream = false;
+u64 mixed(u64 value);
+
 struct Stream {
     u32 ctr[4];
     u32 key[2];
     u32 block[4];
     int index = 4;
+    u64 alternate = 0;
     Stream(u64 seed, u32 batch, u32 ant, u32 step) {
         ctr[0] = step; ctr[1] = 0; ctr[2] = ant; ctr[3] = batch;
         key[0] = static_cast<u32>(seed); key[1] = static_cast<u32>(seed >> 32);
+        if (g_alternate_stream)
+            alternate = mixed(seed ^ mixed((static_cast<u64>(batch) << 32 | ant) ^ mixed(step + 0x632be59bd9b4e019ULL)));
     }
     u32 next() {
+        if (g_alternate_stream) return static_cast<u32>(mixed(alternate++) >> 32);
         if (index == 4) {
             philox4x32_10(ctr, key, block);
             index = 0;
@@ -1147,6 +1159,9 @@ std::int64_t oracle_euc2d(std::int32_t x1, std::int32_t y1, std::int32_t x2, std
 }
 
 void oracle_philox4x32_10(const u32* counter, const u32* key, u32* out) { philox4x32_10(counter, key, out); }
+
+// 1: every later run draws from the alternate generator; 0: Philox again.
+void oracle_set_alternate_stream(int on) { g_alternate_stream = on != 0; }
 
 double oracle_uniform53(u64 key, u32 batch, u32 ant, u32 step) {
     Stream s(key, batch, ant, step);

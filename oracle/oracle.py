@@ -166,6 +166,8 @@ def lib():
         L.oracle_euc2d.restype = ctypes.c_int64
         L.oracle_euc2d.argtypes = [ctypes.c_int32] * 4
         L.oracle_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.oracle_set_alternate_stream.argtypes = [ctypes.c_int]
+        L.oracle_set_alternate_stream.restype = None
         L.oracle_uniform53.restype = ctypes.c_double
         L.oracle_uniform53.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.oracle_start_node.restype = ctypes.c_uint32
@@ -377,6 +379,20 @@ def select_next(xy, primary_row, backup_row, products, visited, current, u):
                                 np.ascontiguousarray(visited, dtype=np.uint8), current, u,
                                 ctypes.byref(chosen), ctypes.byref(stage)), L)
     return chosen.value, stage.value
+
+
+class alternate_stream:
+    """Context manager: oracle runs inside draw every uniform from an
+    unrelated generator (SplitMix64) instead of Philox, for statistical
+    comparisons of the algorithm under independent random streams."""
+
+    def __enter__(self):
+        lib().oracle_set_alternate_stream(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_alternate_stream(0)
+        return False
 
 
 def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False,

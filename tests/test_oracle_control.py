@@ -409,3 +409,18 @@ def test_fork_forces_one_decision_and_rekeys_the_ants(panel, region, level):
                              fork=(cut, 54, action))
     assert (other["trace_features"][cut].view(np.uint32) == fork["trace_features"][cut].view(np.uint32)).all()
     assert not (other["trace_regions"][cut + 1:] == fork["trace_regions"][cut + 1:]).all()
+
+
+def test_alternate_stream_is_unrelated_and_restored():
+    xy = instances(60, 1, seed=8)
+    s = O.default_settings(ants=8)
+    seeds = np.array([3, 4], dtype=np.uint64)
+    before = O.run_faco(xy, [9], seeds, s, 5, 8)
+    with O.alternate_stream():
+        other = O.run_faco(xy, [9], seeds, s, 5, 8)
+        again = O.run_faco(xy, [9], seeds, s, 5, 8)
+        assert O.uniform53(9, 0, 0, 0) != 0.0
+    after = O.run_faco(xy, [9], seeds, s, 5, 8)
+    assert before["tours"].tolist() == after["tours"].tolist()
+    assert other["tours"].tolist() == again["tours"].tolist()
+    assert other["work"].tolist() != before["work"].tolist()
