@@ -280,6 +280,8 @@ SIGNATURES = {
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,
                                                     u64p, vp, ctypes.POINTER(Checkpoints)]),
+    "faco_engine_register_problems_device": (ctypes.c_int, [vp, u64p, f64p, vp, ctypes.c_uint32,
+                                                            ctypes.POINTER(ctypes.c_double)]),
     "faco_state_from_bytes": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "faco_state_to_bytes": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "faco_state_describe": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),

@@ -1104,6 +1104,135 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
 }
 
+// Device preparation (the O(n^2) steps of prepare_problem on the GPU):
+// candidate keys and nearest-neighbour tours from prepare_kernels.hip, rows
+// on the host from the keys, then the initial checklist 2-opt of every
+// instance in one explicit-mode launch of the ant kernel (one workgroup per
+// instance, the checklist holding every city in tour order). Instances are
+// grouped by their candidate count (equal unless an instance has fewer than
+// 9 cities). Each instance's preparation time is its share of the total.
+double register_problems_device(Engine& e, const uint64_t* keys, const double* xy, const uint32_t* sizes,
+                                uint32_t count) {
+    const auto started = Clock::now();
+    std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+    require(lock.owns_lock(), "registration and evaluation of one engine must not overlap");
+    require(current_device() == e.device, "engine must be used on the device it was created on");
+    std::map<uint32_t, std::vector<std::pair<uint64_t, Prepared>>> groups;   // candidate count -> instances
+    size_t offset = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t n = sizes ? sizes[i] : e.n;
+        if (e.mixed) {
+            require(n >= 3 && n <= e.n, "instance larger than the mixed engine's maximum dimension");
+            const faco_settings own = normalized_settings(e.base, n);
+            require(own.primary_width == e.config.primary_width && own.backup_width == e.config.backup_width &&
+                    own.ls_width == e.config.ls_width,
+                    "a mixed-size batch needs the same candidate widths for every instance");
+        } else {
+            require(n == e.n, "instance dimension does not match the engine");
+        }
+        auto coords = integer_coordinates(xy + 2 * offset, n);
+        offset += n;
+        const auto existing = e.registry.find(keys[i]);
+        if (existing != e.registry.end()) {
+            require(existing->second.xy == coords, "same instance key registered with different coordinates");
+            continue;
+        }
+        bool repeated = false;
+        for (const auto& group : groups)
+            for (const auto& item : group.second)
+                if (item.first == keys[i]) {
+                    require(item.second.xy == coords, "same instance key registered with different coordinates");
+                    repeated = true;
+                }
+        if (repeated) continue;
+        groups[candidate_count(e.config, n)].emplace_back(keys[i], prepare_begin(coords, e.config));
+    }
+    const uint32_t K = e.K, lw = e.config.ls_width;
+    for (auto& [needed, items] : groups) {
+        const uint32_t G = static_cast<uint32_t>(items.size());
+        uint32_t stride = 0;
+        for (const auto& item : items) stride = std::max(stride, item.second.n);
+        require(prepare_knn_lds(stride, needed) <= max_lds_per_block(), "instance too large for device preparation");
+        std::vector<uint32_t> packed(static_cast<size_t>(G) * stride, 0), n_of(G);
+        for (uint32_t g = 0; g < G; ++g) {
+            const Prepared& p = items[g].second;
+            n_of[g] = p.n;
+            std::copy(p.packed.begin(), p.packed.end(), packed.begin() + static_cast<size_t>(g) * stride);
+        }
+        const size_t cells = static_cast<size_t>(G) * stride;
+        DeviceArray<uint32_t> d_coords, d_sizes, d_keys, d_nn;
+        d_coords.allocate(cells); d_sizes.allocate(G); d_keys.allocate(cells * needed); d_nn.allocate(cells);
+        d_coords.upload(packed.data(), cells, e.stream);
+        d_sizes.upload(n_of.data(), G, e.stream);
+        checked(launch_prepare_knn(d_coords.data(), d_sizes.data(), G, stride, needed, d_keys.data(), e.stream),
+                "candidate selection launch");
+        checked(launch_prepare_nn(d_coords.data(), d_sizes.data(), G, stride, d_nn.data(), e.stream),
+                "nearest-neighbour launch");
+        std::vector<uint32_t> key_rows(cells * needed), nn(cells);
+        d_keys.download(key_rows.data(), cells * needed, e.stream);
+        d_nn.download(nn.data(), cells, e.stream);
+        checked(hipStreamSynchronize(e.stream), "device preparation");
+        // rows on the host, then the initial 2-opt of every instance on the device
+        std::vector<uint16_t> nbr(static_cast<size_t>(G) * e.nbr_stride), nbr_dist(static_cast<size_t>(G) * e.nbr_stride);
+        std::vector<uint32_t> links(cells, 0), zeros(G, 0);
+        std::vector<int64_t> nn_costs(G);
+        for (uint32_t g = 0; g < G; ++g) {
+            Prepared& p = items[g].second;
+            prepare_rows(p, e.config, K, e.nbr_stride, key_rows.data() + static_cast<size_t>(g) * stride * needed, needed);
+            std::copy(p.nbr.begin(), p.nbr.end(), nbr.begin() + static_cast<size_t>(g) * e.nbr_stride);
+            std::copy(p.nbr_dist.begin(), p.nbr_dist.end(), nbr_dist.begin() + static_cast<size_t>(g) * e.nbr_stride);
+            const uint32_t* t = nn.data() + static_cast<size_t>(g) * stride;
+            require(is_permutation(t, p.n), "device nearest-neighbour tour is not a permutation");
+            for (uint32_t i = 0; i < p.n; ++i)
+                links[static_cast<size_t>(g) * stride + t[i]] = t[i == 0 ? p.n - 1 : i - 1] | (t[i + 1 == p.n ? 0 : i + 1] << 16);
+            nn_costs[g] = tour_cost(p, t);
+        }
+        DeviceArray<uint16_t> d_nbr, d_nbr_dist;
+        DeviceArray<uint32_t> d_links, d_targets, d_out;
+        DeviceArray<int64_t> d_costs;
+        DeviceArray<faco_ant_info> d_info;
+        d_nbr.allocate(nbr.size()); d_nbr_dist.allocate(nbr_dist.size()); d_links.allocate(cells);
+        d_targets.allocate(G); d_out.allocate(cells); d_costs.allocate(G); d_info.allocate(G);
+        d_nbr.upload(nbr.data(), nbr.size(), e.stream); d_nbr_dist.upload(nbr_dist.data(), nbr_dist.size(), e.stream);
+        d_links.upload(links.data(), cells, e.stream); d_targets.upload(zeros.data(), G, e.stream);
+        d_costs.upload(nn_costs.data(), G, e.stream);
+        AntParams ap{};
+        ap.n = stride; ap.ants = G; ap.pw = 0; ap.bw = 0; ap.lw = lw; ap.K = K; ap.waves = 1; ap.chunk = 1;
+        ap.blocks_per_colony = 1; ap.nbr_stride = e.nbr_stride; ap.nbr_in_lds = 1;
+        ap.nbr_lds_bytes = round16_host(2u * stride * K); ap.per_wave_bytes = ant_wave_bytes(stride);
+        ap.explicit_mode = 1; ap.per_ant_parent = 1; ap.full_checklist = 1; ap.block_n = d_sizes.data();
+        ap.ls_limit = std::min<uint64_t>(e.config.initial_ls_evaluation_limit, UINT32_MAX);
+        ap.roulette_margin = 0x1.0p-46;
+        ap.coords = d_coords.data(); ap.nbr = d_nbr.data(); ap.nbr_dist = d_nbr_dist.data();
+        ap.parent = d_nn.data(); ap.links = d_links.data(); ap.visits = d_nn.data(); ap.targets = d_targets.data();
+        ap.parent_cost = d_costs.data(); ap.out_tours = d_out.data(); ap.out_info = d_info.data();
+        if (ant_shared_bytes(stride, ap.nbr_lds_bytes) + ap.per_wave_bytes > max_lds_per_block()) {
+            ap.nbr_in_lds = 0;
+            require(ant_shared_bytes(stride, 0) + ap.per_wave_bytes <= max_lds_per_block(),
+                    "instance too large for the LDS budget");
+        }
+        checked(launch_ant_search(ap, G, e.stream), "initial local search launch");
+        std::vector<uint32_t> improved(cells);
+        d_out.download(improved.data(), cells, e.stream);
+        checked(hipStreamSynchronize(e.stream), "initial local search");
+        for (uint32_t g = 0; g < G; ++g) {
+            Prepared& p = items[g].second;
+            const uint32_t* t = improved.data() + static_cast<size_t>(g) * stride;
+            require(is_permutation(t, p.n), "device 2-opt tour is not a permutation");
+            prepare_finish(p, std::vector<uint32_t>(t, t + p.n));
+        }
+    }
+    const double total = seconds_since(started);
+    uint32_t fresh = 0;
+    for (const auto& group : groups) fresh += static_cast<uint32_t>(group.second.size());
+    for (auto& group : groups)
+        for (auto& item : group.second) {
+            item.second.preparation_seconds = total / fresh;
+            e.registry.emplace(item.first, std::move(item.second));
+        }
+    return total;
+}
+
 // JSON view of a snapshot: the fields of the reference's describe()
 // (cpp/src/bindings.cpp:422-463), read from this engine's buffer ABI.
 struct Json {
@@ -1394,6 +1523,15 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double
         const Prepared& p = e.registry.at(key);
         if (cheap_seconds) *cheap_seconds = p.cheap_seconds;
         if (preparation_seconds) *preparation_seconds = p.preparation_seconds;
+    });
+}
+
+int faco_engine_register_problems_device(faco_engine* engine, const uint64_t* keys, const double* xy,
+                                         const uint32_t* sizes, uint32_t count, double* seconds) {
+    return guarded([&] {
+        require(engine && keys && xy, "null argument");
+        const double total = register_problems_device(*engine->impl, keys, xy, sizes, count);
+        if (seconds) *seconds = total;
     });
 }
 

@@ -247,6 +247,11 @@ struct AntParams {
     uint32_t* out_zero;                 // [A] or null: position of city 0 | forward << 31
     const ColonyShape* shapes;          // [C] or null (fixed size)
     const uint32_t* order;              // [C] or null: colony of each block group (largest work first)
+    // explicit mode only (device preparation): block b runs instance b
+    // (geometry slot b) of block_n[b] cities, and the checklist starts with
+    // every city in tour order instead of the construction's pushes
+    const uint32_t* block_n;            // [blocks] or null
+    uint32_t full_checklist;
 };
 
 struct ColonyParams {
@@ -330,6 +335,16 @@ struct ControlParams {
     uint32_t* regions;                  // [C][2][4][16] or null
     const ColonyShape* shapes;          // [C] or null (fixed size); n and ants above are then strides
 };
+// Device preparation (prepare_kernels.hip): instances g = 0..count-1 of
+// sizes[g] cities at stride `stride` in packed coordinates [count][stride].
+// keys [count][stride][needed]: every city's `needed` nearest other cities
+// as (distance << 16) | city, ascending; tours [count][stride]: the
+// nearest-neighbour tour from city 0 (lowest id on ties).
+size_t prepare_knn_lds(uint32_t stride, uint32_t needed);
+hipError_t launch_prepare_knn(const uint32_t* coords, const uint32_t* sizes, uint32_t count, uint32_t stride,
+                              uint32_t needed, uint32_t* keys, hipStream_t stream);
+hipError_t launch_prepare_nn(const uint32_t* coords, const uint32_t* sizes, uint32_t count, uint32_t stride,
+                             uint32_t* tours, hipStream_t stream);
 hipError_t launch_control_init(const ControlParams& p, uint32_t colonies, hipStream_t stream);
 hipError_t launch_control_decide(const ControlParams& p, uint32_t colonies, hipStream_t stream);
 hipError_t launch_control_update(const ControlParams& p, uint32_t colonies, hipStream_t stream);

@@ -303,11 +303,12 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     const uint32_t group = explicit_mode ? 0u : blk / p.blocks_per_colony;
     const uint32_t colony = p.order ? p.order[group] : group;
     const uint32_t ns = p.n;
-    const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
+    const uint32_t* const block_n = Diag ? p.block_n : nullptr;
+    const uint32_t n = block_n ? block_n[blk] : (p.shapes ? p.shapes[colony].n : ns);
     const uint32_t chunk = p.shapes ? p.shapes[colony].ants / p.blocks_per_colony : p.chunk;
     if (!Persistent && p.shapes && p.state[colony].batch >= p.shapes[colony].batches) return;   // colony done
     const uint32_t chunk_base = (explicit_mode ? blk : blk % p.blocks_per_colony) * chunk;
-    const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
+    const uint32_t geom = block_n ? blk : (p.geometry ? p.geometry[colony] : 0u);
 
     // Shared data: packed coordinates, the colony's parent links (the
     // parent tour is the same for every ant of the colony in an iteration;
@@ -793,6 +794,18 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             for (uint32_t i = lane; i < n; i += kWave) diag_construction[static_cast<size_t>(ant) * ns + i] = tour[i];
         }
         const int64_t construction_cost = cost;
+        if (Diag && p.full_checklist) {
+            // every city in tour order, as the reference's initial 2-opt of
+            // the prepared tour starts (cpp/src/prepared_problem.cpp:112-113,
+            // cpp/src/faco_cpu.cpp:148-160); the circular queue is full
+            for (uint32_t i = lane; i < n; i += kWave) {
+                const uint32_t v = tour[i];
+                queue[i] = static_cast<uint16_t>(v);
+                flags[v] = kQueued;
+            }
+            qhead = 0; qtail = 0; qsize = n;
+            wave_sync();
+        }
         const uint64_t tick2 = Profile ? wall_clock64() : 0;
 
         // ------------------------------------------------------------ checklist 2-opt

@@ -124,7 +124,11 @@ int64_t tour_cost(const Prepared& p, const uint32_t* tour) {
     return sum;
 }
 
-Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& c, uint32_t K, uint32_t nbr_stride) {
+uint32_t candidate_count(const faco_settings& c, uint32_t n) {
+    return std::max({c.primary_width + c.backup_width, c.ls_width, std::min<uint32_t>(8, n - 1)});
+}
+
+Prepared prepare_begin(const std::vector<int32_t>& xy, const faco_settings& c) {
     const auto started = Clock::now();
     Prepared p;
     const uint32_t n = static_cast<uint32_t>(xy.size() / 2);
@@ -136,71 +140,66 @@ Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& c,
     p.cheap.resize(n);
     for (uint32_t i = 0; i < n; ++i) p.cheap[i] = i;
     p.cheap_cost = tour_cost(p, p.cheap.data());
+    // scale floor: 1e-12 times the larger coordinate range, at least 1
+    // (cpp/src/prepared_problem.cpp:68-73)
+    double min_x = xy[0], max_x = xy[0], min_y = xy[1], max_y = xy[1];
+    for (uint32_t i = 0; i < n; ++i) {
+        min_x = std::min<double>(min_x, xy[2 * i]); max_x = std::max<double>(max_x, xy[2 * i]);
+        min_y = std::min<double>(min_y, xy[2 * i + 1]); max_y = std::max<double>(max_y, xy[2 * i + 1]);
+    }
+    p.scale_epsilon = 1e-12 * std::max({max_x - min_x, max_y - min_y, 1.0});
     p.cheap_seconds = since(started);
+    return p;
+}
 
-    const uint32_t pw = c.primary_width, bw = c.backup_width, lw = c.ls_width;
+void candidate_keys(const Prepared& p, uint32_t needed, uint32_t begin, uint32_t end, uint32_t* keys) {
+    std::vector<uint32_t> row(p.n - 1);
+    for (uint32_t a = begin; a < end; ++a) {
+        uint32_t k = 0;
+        for (uint32_t b = 0; b < p.n; ++b)
+            if (b != a) row[k++] = (static_cast<uint32_t>(euc2d(p.packed[a], p.packed[b])) << 16) | b;
+        std::partial_sort(row.begin(), row.begin() + needed, row.end());
+        std::copy(row.begin(), row.begin() + needed, keys + static_cast<size_t>(a) * needed);
+    }
+}
+
+void prepare_rows(Prepared& p, const faco_settings& c, uint32_t K, uint32_t nbr_stride, const uint32_t* keys,
+                  uint32_t needed) {
+    const uint32_t n = p.n, pw = c.primary_width, bw = c.backup_width, lw = c.ls_width;
     const uint32_t scale_width = std::min<uint32_t>(8, n - 1);
-    const uint32_t needed = std::max({pw + bw, lw, scale_width});
     p.primary.assign(static_cast<size_t>(n) * pw, 0);
     p.backup.assign(static_cast<size_t>(n) * bw, 0);
     p.ls.assign(static_cast<size_t>(n) * lw, 0);
     p.local_scale.assign(n, 0);
     p.eta.assign(static_cast<size_t>(n) * pw, 0);
-    // scale floor: 1e-12 times the larger coordinate range, at least 1
-    // (cpp/src/prepared_problem.cpp:68-73)
-    {
-        double min_x = xy[0], max_x = xy[0], min_y = xy[1], max_y = xy[1];
-        for (uint32_t i = 0; i < n; ++i) {
-            min_x = std::min<double>(min_x, xy[2 * i]); max_x = std::max<double>(max_x, xy[2 * i]);
-            min_y = std::min<double>(min_y, xy[2 * i + 1]); max_y = std::max<double>(max_y, xy[2 * i + 1]);
-        }
-        p.scale_epsilon = 1e-12 * std::max({max_x - min_x, max_y - min_y, 1.0});
-    }
-    auto rows = [&](uint32_t begin, uint32_t end) {
-        std::vector<uint64_t> keys(n - 1);
-        for (uint32_t a = begin; a < end; ++a) {
-            uint32_t k = 0;
-            for (uint32_t b = 0; b < n; ++b)
-                if (b != a) keys[k++] = (static_cast<uint64_t>(euc2d(p.packed[a], p.packed[b])) << 32) | b;
-            std::partial_sort(keys.begin(), keys.begin() + needed, keys.end());
-            double scale = 0;
-            for (uint32_t j = 0; j < scale_width; ++j) scale += static_cast<double>(keys[j] >> 32) / scale_width;
-            p.local_scale[a] = scale;
-            for (uint32_t j = 0; j < pw; ++j) {
-                p.primary[static_cast<size_t>(a) * pw + j] = static_cast<uint32_t>(keys[j]);
-                const double d = static_cast<double>(keys[j] >> 32);
-                p.eta[static_cast<size_t>(a) * pw + j] =
-                    d > 0 ? (c.beta == 1.0 ? 1.0 / d : 1.0 / std::pow(d, c.beta)) : 1.0;
-            }
-            for (uint32_t j = 0; j < bw; ++j) p.backup[static_cast<size_t>(a) * bw + j] = static_cast<uint32_t>(keys[pw + j]);
-            for (uint32_t j = 0; j < lw; ++j) p.ls[static_cast<size_t>(a) * lw + j] = static_cast<uint32_t>(keys[j]);
-        }
-    };
-    const uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), n / 64 + 1));
-    if (threads == 1) rows(0, n);
-    else {
-        std::vector<std::thread> pool;
-        for (uint32_t t = 0; t < threads; ++t)
-            pool.emplace_back(rows, static_cast<uint32_t>(static_cast<uint64_t>(n) * t / threads),
-                              static_cast<uint32_t>(static_cast<uint64_t>(n) * (t + 1) / threads));
-        for (auto& th : pool) th.join();
-    }
-    p.nbr.assign(nbr_stride, 0);
     for (uint32_t a = 0; a < n; ++a) {
-        // rows are prefixes of one (distance, id) order: primary = first pw,
-        // LS = first lw, so one row of K = max(pw, lw) serves both.
-        for (uint32_t j = 0; j < K; ++j) {
-            const uint32_t v = j < pw ? p.primary[static_cast<size_t>(a) * pw + j] : p.ls[static_cast<size_t>(a) * lw + j];
-            p.nbr[static_cast<size_t>(a) * K + j] = static_cast<uint16_t>(v);
+        const uint32_t* row = keys + static_cast<size_t>(a) * needed;
+        double scale = 0;
+        for (uint32_t j = 0; j < scale_width; ++j) scale += static_cast<double>(row[j] >> 16) / scale_width;
+        p.local_scale[a] = scale;
+        for (uint32_t j = 0; j < pw; ++j) {
+            p.primary[static_cast<size_t>(a) * pw + j] = row[j] & 0xffffu;
+            const double d = static_cast<double>(row[j] >> 16);
+            p.eta[static_cast<size_t>(a) * pw + j] = d > 0 ? (c.beta == 1.0 ? 1.0 / d : 1.0 / std::pow(d, c.beta)) : 1.0;
         }
+        for (uint32_t j = 0; j < bw; ++j) p.backup[static_cast<size_t>(a) * bw + j] = row[pw + j] & 0xffffu;
+        for (uint32_t j = 0; j < lw; ++j) p.ls[static_cast<size_t>(a) * lw + j] = row[j] & 0xffffu;
     }
+    // rows are prefixes of one (distance, id) order: primary = first pw,
+    // LS = first lw, so one row of K = max(pw, lw) serves both
+    p.nbr.assign(nbr_stride, 0);
     p.nbr_dist.assign(nbr_stride, 0);
-    for (size_t i = 0; i < static_cast<size_t>(n) * K; ++i)
-        p.nbr_dist[i] = static_cast<uint16_t>(euc2d(p.packed[i / K], p.packed[p.nbr[i]]));
+    for (uint32_t a = 0; a < n; ++a)
+        for (uint32_t j = 0; j < K; ++j) {
+            const uint32_t key = keys[static_cast<size_t>(a) * needed + j];
+            p.nbr[static_cast<size_t>(a) * K + j] = static_cast<uint16_t>(key & 0xffffu);
+            p.nbr_dist[static_cast<size_t>(a) * K + j] = static_cast<uint16_t>(key >> 16);
+        }
     p.backup16.assign(p.backup.begin(), p.backup.end());
-    const auto t_candidates = Clock::now();
-    p.profile[0] = std::chrono::duration<double>(t_candidates - started).count();
+}
 
+std::vector<uint32_t> nearest_neighbour_tour(const Prepared& p) {
+    const uint32_t n = p.n;
     std::vector<uint32_t> nn;
     nn.reserve(n);
     std::vector<uint8_t> seen(n, 0);
@@ -217,15 +216,41 @@ Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& c,
         }
         current = next;
     }
+    return nn;
+}
+
+void prepare_finish(Prepared& p, std::vector<uint32_t> tour) {
+    p.initial = std::move(tour);
+    p.initial_cost = tour_cost(p, p.initial.data());
+    if (p.cheap_cost <= p.initial_cost) { p.initial = p.cheap; p.initial_cost = p.cheap_cost; }
+}
+
+Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& c, uint32_t K, uint32_t nbr_stride) {
+    const auto started = Clock::now();
+    Prepared p = prepare_begin(xy, c);
+    const uint32_t n = p.n, needed = candidate_count(c, n);
+    std::vector<uint32_t> keys(static_cast<size_t>(n) * needed);
+    const uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), n / 64 + 1));
+    if (threads == 1) candidate_keys(p, needed, 0, n, keys.data());
+    else {
+        std::vector<std::thread> pool;
+        for (uint32_t t = 0; t < threads; ++t)
+            pool.emplace_back(candidate_keys, std::cref(p), needed,
+                              static_cast<uint32_t>(static_cast<uint64_t>(n) * t / threads),
+                              static_cast<uint32_t>(static_cast<uint64_t>(n) * (t + 1) / threads), keys.data());
+        for (auto& th : pool) th.join();
+    }
+    prepare_rows(p, c, K, nbr_stride, keys.data(), needed);
+    const auto t_candidates = Clock::now();
+    p.profile[0] = std::chrono::duration<double>(t_candidates - started).count();
+    const std::vector<uint32_t> nn = nearest_neighbour_tour(p);
     const auto t_nn = Clock::now();
     p.profile[1] = std::chrono::duration<double>(t_nn - t_candidates).count();
     HostTour tour(p, nn);
     tour.two_opt(c.initial_ls_evaluation_limit);
     const auto t_ls = Clock::now();
     p.profile[2] = std::chrono::duration<double>(t_ls - t_nn).count();
-    p.initial = tour.order;
-    p.initial_cost = tour_cost(p, p.initial.data());
-    if (p.cheap_cost <= p.initial_cost) { p.initial = p.cheap; p.initial_cost = p.cheap_cost; }
+    prepare_finish(p, tour.order);
     p.profile[3] = since(t_ls);
     p.preparation_seconds = since(started);
     return p;

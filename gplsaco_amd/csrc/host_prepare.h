@@ -34,6 +34,20 @@ struct Prepared {
 faco_settings normalized_settings(faco_settings settings, uint32_t n);
 Prepared prepare_problem(const std::vector<int32_t>& xy, const faco_settings& settings, uint32_t K,
                          uint32_t nbr_stride);
+// The stages of prepare_problem, shared with the device preparation:
+// coordinates, cheap tour and scale floor; the candidate keys of every city
+// ((distance << 16) | city of its `needed` nearest others, ascending: the
+// reference's (distance, id) order); rows, heuristic and local scales from
+// those keys; the nearest-neighbour tour from city 0 (lowest id on ties);
+// and the initial tour after its checklist 2-opt (the cheap tour when that
+// is no longer).
+uint32_t candidate_count(const faco_settings& settings, uint32_t n);
+Prepared prepare_begin(const std::vector<int32_t>& xy, const faco_settings& settings);
+void candidate_keys(const Prepared& p, uint32_t needed, uint32_t begin, uint32_t end, uint32_t* keys);
+void prepare_rows(Prepared& p, const faco_settings& settings, uint32_t K, uint32_t nbr_stride, const uint32_t* keys,
+                  uint32_t needed);
+std::vector<uint32_t> nearest_neighbour_tour(const Prepared& p);
+void prepare_finish(Prepared& p, std::vector<uint32_t> tour);
 bool is_permutation(const uint32_t* tour, uint32_t n);
 int64_t tour_cost(const Prepared& p, const uint32_t* tour);
 // The 64 sampled cities of an instance for the control features ([64],

@@ -542,6 +542,21 @@ class FacoBatchEngine:
                                                  ctypes.byref(prep)))
         return {"cheap_seconds": cheap.value, "preparation_seconds": prep.value}
 
+    def register_problems_device(self, instance_keys, coordinates) -> dict:
+        """Register many instances with the preparation on the GPU (the same
+        rows, initial tour and cost as register_problem). coordinates: one
+        FP64 [n, 2] array per key. Returns the wall time of the call."""
+        keys = [_unsigned(k, "instance_key") for k in instance_keys]
+        arrays = [_coordinates(xy) for xy in coordinates]
+        if len(keys) != len(arrays) or not keys:
+            raise ValueError("one coordinate array per instance key")
+        sizes = np.array([a.shape[0] for a in arrays], dtype=np.uint32)
+        xy = np.ascontiguousarray(np.concatenate(arrays), dtype=np.float64)
+        seconds = ctypes.c_double()
+        check(lib().faco_engine_register_problems_device(self._handle, np.array(keys, dtype=np.uint64), xy.reshape(-1),
+                                                         ptr(sizes), len(keys), ctypes.byref(seconds)))
+        return {"seconds": seconds.value, "instances": len(keys)}
+
     def preparation_profile(self, instance_key) -> dict:
         out = np.zeros(4, dtype=np.float64)
         check(lib().faco_engine_preparation_profile(self._handle, _unsigned(instance_key, "instance_key"), out))
@@ -981,6 +996,12 @@ class MixedFacoBatchEngine(FacoBatchEngine):
     def register_problem(self, instance_key, coordinates) -> dict:
         out = super().register_problem(instance_key, coordinates)
         self._sizes[_unsigned(instance_key, "instance_key")] = _coordinates(coordinates).shape[0]
+        return out
+
+    def register_problems_device(self, instance_keys, coordinates) -> dict:
+        out = super().register_problems_device(instance_keys, coordinates)
+        for key, xy in zip(instance_keys, coordinates):
+            self._sizes[_unsigned(key, "instance_key")] = xy.shape[0]
         return out
 
     def evaluate_faco_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, mne=8,

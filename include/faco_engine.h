@@ -272,6 +272,17 @@ int faco_engine_set_local_search(faco_engine* engine, uint32_t mode);
  * k-NN rows, NN tour, initial checklist 2-opt. */
 int faco_engine_register_problem(faco_engine* engine, uint64_t instance_key, const double* xy,
                                  uint32_t n, double* cheap_seconds, double* preparation_seconds);
+/* Device-side preparation of `count` instances (reference
+ * prepared_problem.cpp:58-126, the step register_problem runs on the host):
+ * the same k-NN rows, nearest-neighbour tour and initial checklist 2-opt,
+ * bit for bit, with the O(n^2) candidate selection, the nearest-neighbour
+ * walk and the 2-opt on the GPU. Instance i has key keys[i], sizes[i] cities
+ * (sizes NULL: the engine's dimension) and coordinates xy concatenated in
+ * order. Keys already registered must carry the same coordinates and are
+ * skipped. *seconds (may be NULL): wall time of the whole call; each new
+ * instance's preparation_seconds is its share. */
+int faco_engine_register_problems_device(faco_engine* engine, const uint64_t* keys, const double* xy,
+                                         const uint32_t* sizes, uint32_t count, double* seconds);
 /* preparation_profile: candidates_and_scales, nearest_neighbor, initial_ls,
  * finalization seconds (reference bindings.cpp:639-650). */
 int faco_engine_preparation_profile(faco_engine* engine, uint64_t instance_key, double out[4]);
