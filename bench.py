@@ -134,11 +134,20 @@ def uneven_batch_rate(ext, rounds: int) -> dict:
         keys.append(instance_key(n, 3, i, clustered))
     rng = np.random.default_rng(3)
     programs = [ext.read_program({**random_program(rng, depth=5), "feature_spec_id": 2}) for _ in range(individuals)]
-    started = time.perf_counter()
     engine = ext.MixedFacoBatchEngine(max(sizes), count, ext.FixedFacoSettings(), population_size=individuals)
-    for k, xy in zip(keys, xys):
-        engine.register_problem(k, xy)
+    # preparation (candidate rows, nearest-neighbour tour, initial 2-opt) of
+    # the 64 instances on the device, and the same on the host for comparison
+    started = time.perf_counter()
+    engine.register_problems_device(keys, xys)
     prep = time.perf_counter() - started
+    host_engine = ext.MixedFacoBatchEngine(max(sizes), count, ext.FixedFacoSettings())
+    started = time.perf_counter()
+    for k, xy in zip(keys, xys):
+        host_engine.register_problem(k, xy)
+    host_prep = time.perf_counter() - started
+    same = all(np.array_equal(engine.problem_data(k)["initial_tour"], host_engine.problem_data(k)["initial_tour"])
+               for k in keys)
+    del host_engine
     task_keys = np.array(keys, dtype=np.uint64)
     seeds = np.zeros(count, dtype=np.uint64)
     # the autotuner times plain FACO on the panel (shape choice only)
@@ -153,6 +162,8 @@ def uneven_batch_rate(ext, rounds: int) -> dict:
             "ant_tours_per_s": res.total_tour_evaluations / res.search_seconds,
             "tours": res.total_tour_evaluations, "seconds": res.search_seconds,
             "launched_iterations": res.launched_batches, "preparation_seconds": prep,
+            "preparation": {"device_seconds": prep, "host_seconds": host_prep, "host_threads": os.cpu_count(),
+                            "identical_initial_tours": same},
             "restarts": sum(st["restarts"] for st in raw["states"]),
             "autotune": {"best": tuned["best"], "shapes_timed": len(tuned["table"])},
             "ants_per_colony": {"64": ants.count(64), "128": ants.count(128), "192": ants.count(192)}}

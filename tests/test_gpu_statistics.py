@@ -18,6 +18,8 @@ to Philox) stands in for any other generator:
   independent 30-sample means of the same distribution fall outside each
   other's 95% CI about one time in six.)
 """
+import os
+
 import numpy as np
 import pytest
 from scipy import stats
@@ -45,7 +47,7 @@ def _alternate_costs(oracle_mod, xy, seeds, ants, iterations, widths=None, or_op
     kw = {"primary_width": widths[0], "ls_width": widths[1]} if widths else {}
     with O.alternate_stream():
         out = O.run_faco(xy[None].astype(np.int32), [1], seeds, O.default_settings(ants=ants, **kw), iterations, 8,
-                         or_opt=or_opt)
+                         threads=min(16, os.cpu_count() or 1), or_opt=or_opt)
     return out["costs"].astype(float)
 
 
