@@ -275,7 +275,7 @@ SIGNATURES = {
     "faco_prior_mma_check": (ctypes.c_int, [f32p, f32p, f32p]),
     "faco_engine_edge_features": (ctypes.c_int, [vp, ctypes.c_uint64, f32p]),
     "faco_engine_set_edge_prior": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
-    "faco_engine_apply_edge_prior": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp]),
+    "faco_engine_apply_edge_prior": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, ctypes.c_double]),
     "faco_engine_evaluate_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64,
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,

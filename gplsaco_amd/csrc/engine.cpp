@@ -1579,9 +1579,11 @@ int faco_engine_set_edge_prior(faco_engine* engine, uint64_t key, const float* m
     });
 }
 
-int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_t key, float* multipliers) {
+int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_t key, float* multipliers,
+                                 double exponent) {
     return guarded([&] {
         require(engine && prior, "null argument");
+        require(std::isfinite(exponent) && exponent > 0, "prior exponent must be positive and finite");
         Engine& e = *engine->impl;
         const size_t cells = static_cast<size_t>(e.problem(key).n) * e.config.primary_width;
         std::vector<float> features(cells * 8), logits(cells), mult(cells);
@@ -1593,8 +1595,11 @@ int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_
                                    e.stream), "prior inference launch");
         s.download(logits.data(), cells, e.stream);
         checked(hipStreamSynchronize(e.stream), "prior inference");
-        for (size_t i = 0; i < cells; ++i)
-            mult[i] = static_cast<float>(1.0 / (1.0 + std::exp(-static_cast<double>(logits[i]))));
+        for (size_t i = 0; i < cells; ++i) {
+            const double sigmoid = 1.0 / (1.0 + std::exp(-static_cast<double>(logits[i])));
+            mult[i] = static_cast<float>(exponent == 1.0 ? sigmoid : (exponent == 2.0 ? sigmoid * sigmoid
+                                                                                        : std::pow(sigmoid, exponent)));
+        }
         for (size_t i = 0; i < cells; ++i) mult[i] = mult[i] > 0.0f ? mult[i] : 1e-30f;
         if (faco_engine_set_edge_prior(engine, key, mult.data()) != FACO_OK) throw std::runtime_error(g_error);
         if (multipliers) std::copy(mult.begin(), mult.end(), multipliers);

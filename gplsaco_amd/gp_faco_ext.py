@@ -499,12 +499,15 @@ class FacoBatchEngine:
             raise ValueError("multipliers must be [n, primary_width]")
         check(lib().faco_engine_set_edge_prior(self._handle, key, m.ctypes.data_as(ctypes.c_void_p)))
 
-    def apply_edge_prior(self, prior, instance_key) -> np.ndarray:
+    def apply_edge_prior(self, prior, instance_key, exponent=None) -> np.ndarray:
         """Score the instance's candidate edges with `prior` (an EdgePrior) and use
-        sigmoid(logit) as their heuristic multipliers; returns them."""
+        sigmoid(logit) ** exponent as their heuristic multipliers (exponent
+        None: the prior's default, 2); returns them."""
+        from .prior import PRIOR_EXPONENT
+        exponent = PRIOR_EXPONENT if exponent is None else float(exponent)
         out = np.zeros((self._instance_n(instance_key), self.settings.primary_width), dtype=np.float32)
         check(lib().faco_engine_apply_edge_prior(self._handle, prior._handle, _unsigned(instance_key, "instance_key"),
-                                                 out.ctypes.data_as(ctypes.c_void_p)))
+                                                 out.ctypes.data_as(ctypes.c_void_p), exponent))
         return out
 
     def autotune(self, instance_keys, seeds, iterations: int = 4) -> dict:

@@ -465,9 +465,12 @@ int faco_engine_edge_features(faco_engine* engine, uint64_t instance_key, float*
 /* heuristic multipliers [n][primary_width] of a registered instance
  * (NULL restores plain FACO) */
 int faco_engine_set_edge_prior(faco_engine* engine, uint64_t instance_key, const float* multipliers);
-/* features -> logits on the device -> sigmoid; sets and returns the multipliers */
+/* features -> logits on the device -> sigmoid(logit) ** exponent (FP64,
+ * then FP32, floored at 1e-30); sets and returns the multipliers. The
+ * committed network is used with exponent 2 (chosen on validation
+ * instances). */
 int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_t instance_key,
-                                 float* multipliers);
+                                 float* multipliers, double exponent);
 
 /* FixedFacoGpu(coordinates, instance_key, settings).run_iterations(seed,
  * batches, mne_target) (reference cpp/include/gp_faco/fixed_faco_gpu.hpp:44-60,

@@ -185,9 +185,13 @@ def test_apply_edge_prior_uses_sigmoid_of_logits(ext, oracle_mod, prior_mod):
     engine.register_problem(9, xy)
     prior = prior_mod.EdgePrior(seed=21)
     announce("apply-prior", n)
-    m = engine.apply_edge_prior(prior, 9)
     logits = prior.logits(engine.edge_features(9))
-    want_m = (1.0 / (1.0 + np.exp(-logits.astype(np.float64)))).astype(np.float32).reshape(n, -1)
+    sigmoid = 1.0 / (1.0 + np.exp(-logits.astype(np.float64)))
+    m2 = engine.apply_edge_prior(prior, 9)          # the default exponent, 2
+    np.testing.assert_array_equal(m2, (sigmoid * sigmoid).astype(np.float32).reshape(n, -1))
+    np.testing.assert_array_equal(m2, prior_mod.multipliers(logits).reshape(n, -1))
+    m = engine.apply_edge_prior(prior, 9, exponent=1.0)
+    want_m = sigmoid.astype(np.float32).reshape(n, -1)
     np.testing.assert_array_equal(m, want_m)
     got = engine.evaluate_faco_evaluations(np.array([9, 9], np.uint64), np.array([1, 2], np.uint64), s.ants * 5, 8,
                                            use_graph=False)
