@@ -1882,6 +1882,45 @@ int oracle_escape_commit(const u32* proposal, const u32* cache_edges, u32 cache_
     });
 }
 
+// Hard mode (edge_constraints.hpp:47-90): a move is allowed when every
+// added edge either cancels a removed one (multiset: an edge removed and
+// added back is no new edge) or is an edge of the graph.
+int oracle_hard_edges_allowed(u32 count, const u32* removed, const u32* added, const u32* offsets,
+                              const u32* neighbors, u32 n) {
+    if (count > 3) return -1;
+    bool consumed[3] = {false, false, false};
+    for (u32 i = 0; i < count; ++i) {
+        const EscEdge e{added[2 * i], added[2 * i + 1]};
+        bool existing = false;
+        for (u32 j = 0; j < count; ++j)
+            if (!consumed[j] && same_edge(e, EscEdge{removed[2 * j], removed[2 * j + 1]})) {
+                consumed[j] = true; existing = true; break;
+            }
+        if (!existing && !graph_has(offsets, neighbors, n, e.a, e.b)) return 0;
+    }
+    return 1;
+}
+
+// relocation_allowed: `node` moved to follow `target` on an array tour.
+int oracle_hard_relocation_allowed(const u32* tour, const u32* positions, u32 n, u32 target, u32 node,
+                                   const u32* offsets, const u32* neighbors) {
+    if (target >= n || node >= n || target == node) return 0;
+    const u32 target_after = tour[(positions[target] + 1) % n];
+    if (target_after == node) return 1;
+    const u32 before = tour[(positions[node] + n - 1) % n], after = tour[(positions[node] + 1) % n];
+    const u32 removed[6] = {before, node, node, after, target, target_after};
+    const u32 added[6] = {before, after, target, node, node, target_after};
+    return oracle_hard_edges_allowed(3, removed, added, offsets, neighbors, n);
+}
+
+// two_opt_allowed: remove (a, a_neighbor), (b, b_neighbor); add (a, b), (a_neighbor, b_neighbor).
+int oracle_hard_two_opt_allowed(u32 a, u32 a_neighbor, u32 b, u32 b_neighbor, const u32* offsets,
+                                const u32* neighbors, u32 n) {
+    const u32 removed[4] = {a, a_neighbor, b, b_neighbor};
+    const u32 added[4] = {a, b, a_neighbor, b_neighbor};
+    return oracle_hard_edges_allowed(2, removed, added, offsets, neighbors, n);
+}
+
 // prepare_escape_relocation (escape_edges.hpp:101-115) on an array tour.
 int oracle_escape_relocation(const u32* tour, const u32* positions, u32 n, u32 target, u32 node, const u32* offsets,
                              const u32* neighbors, const u32* cache_edges, u32 cache_size, u64 cache_epoch, u32 permit,

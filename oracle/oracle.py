@@ -200,6 +200,9 @@ def lib():
                                           ctypes.c_uint32, ctypes.c_uint32, u32p]
         L.oracle_escape_transaction.argtypes = [ctypes.c_uint32, u32p, u32p, u32p, u32p, ctypes.c_uint32, u32p,
                                                 ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p, u32p, u32p, u32p]
+        L.oracle_hard_relocation_allowed.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                     u32p, u32p]
+        L.oracle_hard_two_opt_allowed.argtypes = [ctypes.c_uint32] * 4 + [u32p, u32p, ctypes.c_uint32]
         L.oracle_escape_relocation.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                                u32p, u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p]
         L.oracle_escape_commit.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p]
@@ -493,6 +496,22 @@ def _csr(graph_edges, n):
 def csr(graph_edges, n):
     """Undirected CSR (offsets [n+1], sorted neighbours) of an edge list."""
     return _csr(graph_edges, n)
+
+
+def hard_relocation_allowed(tour, target, node, csr_graph):
+    """Hard-mode relocation predicate: `node` moved to follow `target`."""
+    tour = np.ascontiguousarray(tour, dtype=np.uint32)
+    n = tour.size
+    positions = np.zeros(n, dtype=np.uint32)
+    positions[tour] = np.arange(n, dtype=np.uint32)
+    offsets, neighbors = csr_graph
+    return bool(lib().oracle_hard_relocation_allowed(tour, positions, n, target, node, offsets, neighbors))
+
+
+def hard_two_opt_allowed(a, a_neighbor, b, b_neighbor, csr_graph, n):
+    """Hard-mode 2-opt predicate: (a, a_neighbor), (b, b_neighbor) -> (a, b), (a_neighbor, b_neighbor)."""
+    offsets, neighbors = csr_graph
+    return bool(lib().oracle_hard_two_opt_allowed(a, a_neighbor, b, b_neighbor, offsets, neighbors, n))
 
 
 def _proposal(p):
