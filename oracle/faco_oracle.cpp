@@ -176,8 +176,15 @@ struct Problem {
     std::vector<float> prior;  // learned heuristic multipliers [n][pw] (this engine's extension), empty: none
     std::vector<u32> cheap, initial;
     i64 cheap_cost = 0, initial_cost = 0;
+    // Hard mode: the undirected CSR graph every tour edge must belong to
+    // (null: unrestricted)
+    const u32* graph_offsets = nullptr;
+    const u32* graph_neighbors = nullptr;
     i64 d(u32 a, u32 b) const { return euc2d(xy, a, b); }
+    bool hard() const { return graph_offsets != nullptr; }
 };
+
+bool graph_has(const u32* offsets, const u32* neighbors, u32 dimension, u32 a, u32 b);
 
 // Array tour with inverse positions: restates CpuTour (cpp/src/faco_cpu.cpp:46-217).
 struct Tour {
@@ -244,6 +251,36 @@ struct Tour {
     }
 };
 
+// Hard-mode moves (edge_constraints.hpp:47-90): every added edge that does
+// not cancel a removed one (as a multiset) must be a graph edge.
+bool hard_edges_allowed(const Problem& p, u32 count, const u32 (*removed)[2], const u32 (*added)[2]) {
+    bool consumed[3] = {false, false, false};
+    for (u32 i = 0; i < count; ++i) {
+        bool existing = false;
+        for (u32 j = 0; j < count; ++j) {
+            const bool same = (added[i][0] == removed[j][0] && added[i][1] == removed[j][1]) ||
+                              (added[i][0] == removed[j][1] && added[i][1] == removed[j][0]);
+            if (!consumed[j] && same) { consumed[j] = true; existing = true; break; }
+        }
+        if (!existing && !graph_has(p.graph_offsets, p.graph_neighbors, p.n, added[i][0], added[i][1])) return false;
+    }
+    return true;
+}
+bool hard_relocation_allowed(const Problem& p, const Tour& t, u32 target, u32 node) {
+    if (target >= p.n || node >= p.n || target == node) return false;
+    const u32 target_after = t.succ(target);
+    if (target_after == node) return true;
+    const u32 before = t.pred(node), after = t.succ(node);
+    const u32 removed[3][2] = {{before, node}, {node, after}, {target, target_after}};
+    const u32 added[3][2] = {{before, after}, {target, node}, {node, target_after}};
+    return hard_edges_allowed(p, 3, removed, added);
+}
+bool hard_two_opt_allowed(const Problem& p, u32 a, u32 a_neighbor, u32 b, u32 b_neighbor) {
+    const u32 removed[2][2] = {{a, a_neighbor}, {b, b_neighbor}};
+    const u32 added[2][2] = {{a, b}, {a_neighbor, b_neighbor}};
+    return hard_edges_allowed(p, 2, removed, added);
+}
+
 struct LsStats {
     u64 processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0;
     u32 limit_reached = 0;
@@ -309,8 +346,11 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
                 ++passed[kind];
                 const u32 nb = kind == 0 ? t.succ(b) : t.pred(b);
                 const u32 closing = kind == 0 ? a_next : a_prev;
+                // Hard mode: a move that leaves the graph never wins
+                // (cuda/faco_device.cuh:383-394); it still counts as evaluated
                 const i64 gain = (nb == a || b == closing) ? 0
-                    : current + t.p->d(b, nb) - ab - t.p->d(closing, nb);
+                    : (!t.p->hard() || hard_two_opt_allowed(*t.p, a, closing, b, nb))
+                    ? current + t.p->d(b, nb) - ab - t.p->d(closing, nb) : std::numeric_limits<i64>::min();
                 if (gain > best) {
                     best = gain;
                     if (kind == 0) { move[0] = a_next; move[1] = nb; move[2] = a; move[3] = b; }
@@ -384,10 +424,15 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
 // ---------------------------------------------------------------------------
 // select_next (cpp/src/faco_cpu.cpp:247-294) for one construction step.
 // Returns the chosen node; stage: 0 primary, 1 zero-weight, 2 backup,
-// 3 global fallback, 4 exhausted.
+// 3 global fallback, 4 exhausted. In Hard mode (cuda/faco_choices.cuh:46-53)
+// a city is available only when moving it after `current` in the working
+// tour `work` keeps every edge in the graph.
 u32 select_next(const Problem& p, u32 current, const double* products,
-                const std::vector<std::uint8_t>& visited, double u, int* stage) {
+                const std::vector<std::uint8_t>& visited, double u, int* stage, const Tour* work = nullptr) {
     const u32 pw = p.s.primary_width, bw = p.s.backup_width, n = p.n;
+    auto free_city = [&](u32 v) {
+        return v < n && !visited[v] && (!p.hard() || hard_relocation_allowed(p, *work, current, v));
+    };
     const u32* primary = p.primary.data() + static_cast<size_t>(current) * pw;
     double sum = 0;
     u32 avail[64];
@@ -395,7 +440,7 @@ u32 select_next(const Problem& p, u32 current, const double* products,
     u32 count = 0;
     for (u32 j = 0; j < pw; ++j) {
         const u32 v = primary[j];
-        if (v < n && !visited[v]) { sum += products[j]; avail[count] = v; prefix[count] = sum; ++count; }
+        if (free_city(v)) { sum += products[j]; avail[count] = v; prefix[count] = sum; ++count; }
     }
     if (count) {
         if (sum == 0) {
@@ -411,12 +456,12 @@ u32 select_next(const Problem& p, u32 current, const double* products,
     }
     const u32* backup = p.backup.data() + static_cast<size_t>(current) * bw;
     for (u32 j = 0; j < bw; ++j) {
-        if (backup[j] < n && !visited[backup[j]]) { *stage = 2; return backup[j]; }
+        if (free_city(backup[j])) { *stage = 2; return backup[j]; }
     }
     u32 chosen = current;
     i64 best = std::numeric_limits<i64>::max();
     for (u32 v = 0; v < n; ++v) {
-        if (visited[v]) continue;
+        if (!free_city(v)) continue;
         const i64 value = p.d(current, v);
         if (value < best) { best = value; chosen = v; }
     }
@@ -452,7 +497,7 @@ AntResult run_ant(const Problem& p, const u32* parent, u32 start, u32 target, u6
     visited[current] = 1;
     while (r.c.mne < target && r.c.steps + 1 < n) {
         const u32 step = r.c.steps + 1;
-        const u32 selected = choose(current, step, visited);
+        const u32 selected = choose(current, step, visited, work);
         if (selected >= n || visited[selected]) { r.c.exhausted = 1; break; }
         const u32 previous = work.pred(selected);
         visited[selected] = 1;
@@ -622,11 +667,12 @@ AntResult stochastic_ant(const Problem& p, const ColonyRun& r, u64 key, u32 batc
     const u32 pw = p.s.primary_width;
     const u32 start = start_node(key, batch, ant, p.n);
     return run_ant(p, r.parent.data(), start, target, p.s.ls_evaluation_limit,
-        [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited) {
+        [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited, const Tour& work) {
             Stream stream(key, batch, ant, step);
             const double u = stream.uniform53();
             int stage = 0;
-            return select_next(p, current, r.products.data() + static_cast<size_t>(current) * pw, visited, u, &stage);
+            return select_next(p, current, r.products.data() + static_cast<size_t>(current) * pw, visited, u, &stage,
+                               &work);
         });
 }
 
@@ -1122,11 +1168,12 @@ AntResult controlled_ant(const Problem& p, const ColonyRun& r, u64 key, u32 batc
     const u32 pw = p.s.primary_width;
     const u32 start = start_nodes[start_node(key, batch, ant, start_count)];
     return run_ant(p, r.parent.data(), start, target, p.s.ls_evaluation_limit,
-        [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited) {
+        [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited, const Tour& work) {
             Stream stream(key, batch, ant, step);
             const double u = stream.uniform53();
             int stage = 0;
-            return select_next(p, current, r.products.data() + static_cast<size_t>(current) * pw, visited, u, &stage);
+            return select_next(p, current, r.products.data() + static_cast<size_t>(current) * pw, visited, u, &stage,
+                               &work);
         });
 }
 
@@ -1211,7 +1258,7 @@ int oracle_construct_ls(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, co
         p.ls.assign(ls, ls + static_cast<size_t>(n) * lw);
         u32 cursor = 1;
         auto r = run_ant(p, parent, visit[0], mne_target, limit,
-            [&](u32, u32, const std::vector<std::uint8_t>&) { return visit[cursor++]; });
+            [&](u32, u32, const std::vector<std::uint8_t>&, const Tour&) { return visit[cursor++]; });
         std::copy(r.construction_tour.begin(), r.construction_tour.end(), construction_tour);
         std::copy(r.tour.begin(), r.tour.end(), tour);
         std::copy(r.positions.begin(), r.positions.end(), positions);
@@ -1257,6 +1304,26 @@ int oracle_select_next(u32 n, const std::int32_t* xy, u32 pw, u32 bw, const u32*
     });
 }
 
+// select_next in Hard mode: availability also needs the relocation of the
+// candidate after `current` in `tour` to stay inside the CSR graph.
+int oracle_select_next_hard(u32 n, const std::int32_t* xy, u32 pw, u32 bw, const u32* primary_row,
+                            const u32* backup_row, const double* products, const std::uint8_t* visited, u32 current,
+                            double u, const u32* tour, const u32* offsets, const u32* neighbors, u32* chosen,
+                            int* stage) {
+    return guarded([&] {
+        Problem p; p.n = n; p.xy = xy; p.s.primary_width = pw; p.s.backup_width = bw;
+        p.graph_offsets = offsets; p.graph_neighbors = neighbors;
+        p.primary.assign(static_cast<size_t>(n) * pw, n);
+        p.backup.assign(static_cast<size_t>(n) * bw, n);
+        std::copy(primary_row, primary_row + pw, p.primary.begin() + static_cast<size_t>(current) * pw);
+        std::copy(backup_row, backup_row + bw, p.backup.begin() + static_cast<size_t>(current) * bw);
+        Tour t;
+        t.assign(tour, n, &p);
+        std::vector<std::uint8_t> v(visited, visited + n);
+        *chosen = select_next(p, current, products, v, u, stage, &t);
+    });
+}
+
 // Full fixed-iteration run of `colonies` independent colonies (instance i %
 // instances, seed seeds[i]) for `batches` batches. Outputs per colony the
 // global tour, global cost and work counters. Optional per-batch trace for
@@ -1268,9 +1335,12 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
                     u32* global_tours, std::int64_t* global_costs, u64* work /* [colonies][2] */,
                     u32* trace_parent, double* trace_trails, std::int64_t* trace_costs /*[batches][3]*/,
                     double* elapsed_search_seconds, u32 local_search,
-                    const float* multipliers /* [instances][n * pw] or null */) {
+                    const float* multipliers /* [instances][n * pw] or null */,
+                    const u32* graph_offsets /* [instances][n + 1] or null: Hard mode */,
+                    const u32* graph_neighbors /* [instances][neighbor_stride] */, u32 neighbor_stride) {
     return guarded([&] {
         if (!instances || !colonies) throw std::invalid_argument("empty panel");
+        if (graph_offsets && local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
         std::vector<Problem> problems(instances);
         for (u32 g = 0; g < instances; ++g) {
             check_coordinates(n, xy_all + static_cast<size_t>(g) * 2 * n);
@@ -1278,6 +1348,14 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             problems[g].s = normalized(*settings, n);
             prepare(problems[g]);
             problems[g].or_opt = local_search != 0;
+            if (graph_offsets) {
+                problems[g].graph_offsets = graph_offsets + static_cast<size_t>(g) * (n + 1);
+                problems[g].graph_neighbors = graph_neighbors + static_cast<size_t>(g) * neighbor_stride;
+                for (u32 i = 0; i < n; ++i)
+                    if (!graph_has(problems[g].graph_offsets, problems[g].graph_neighbors, n, problems[g].initial[i],
+                                   problems[g].initial[(i + 1) % n]))
+                        throw std::invalid_argument("the graph must hold the initial tour");
+            }
             if (multipliers) {
                 const size_t cells = static_cast<size_t>(n) * problems[g].s.primary_width;
                 problems[g].prior.assign(multipliers + g * cells, multipliers + (g + 1) * cells);

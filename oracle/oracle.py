@@ -190,7 +190,8 @@ def lib():
         L.oracle_run_faco.argtypes = [ctypes.c_uint32, ctypes.c_uint32, i32p, u64p, ctypes.c_uint32, u64p,
                                       ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p]
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
         L.oracle_prior_features.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, f32p]
         L.oracle_escape_slot_view.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
@@ -202,6 +203,9 @@ def lib():
                                                 ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p, u32p, u32p, u32p]
         L.oracle_hard_relocation_allowed.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                      u32p, u32p]
+        L.oracle_select_next_hard.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, f64p,
+                                              u8p, ctypes.c_uint32, ctypes.c_double, u32p, u32p, u32p,
+                                              ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int)]
         L.oracle_hard_two_opt_allowed.argtypes = [ctypes.c_uint32] * 4 + [u32p, u32p, ctypes.c_uint32]
         L.oracle_escape_relocation.argtypes = [u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                                u32p, u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, u32p]
@@ -399,10 +403,12 @@ class alternate_stream:
 
 
 def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False,
-             or_opt=False, multipliers=None):
+             or_opt=False, multipliers=None, graphs=None):
     """Fixed-iteration FACO over colonies (colony i uses instance i % instances).
     `multipliers` [instances, n, pw] (float32): learned heuristic multipliers,
-    eta' = eta * multiplier in FP64."""
+    eta' = eta * multiplier in FP64. `graphs`: one undirected CSR
+    (offsets, neighbors) per instance (csr()) for the Hard mode, in which
+    every move must keep the tour inside its graph."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -422,13 +428,23 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
         mult = np.ascontiguousarray(multipliers, dtype=np.float32)
         if mult.shape[0] != instances:
             raise ValueError("one multiplier table per instance")
+    g_offsets = g_neighbors = None
+    stride = 0
+    if graphs is not None:
+        if len(graphs) != instances:
+            raise ValueError("one graph per instance")
+        stride = max(len(nb) for _, nb in graphs)
+        g_offsets = np.ascontiguousarray(np.stack([np.asarray(o, dtype=np.uint32) for o, _ in graphs]))
+        g_neighbors = np.zeros((instances, stride), dtype=np.uint32)
+        for g, (_, nb) in enumerate(graphs):
+            g_neighbors[g, :len(nb)] = nb
     L = lib()
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None  # noqa: E731
     _check(L.oracle_run_faco(n, instances, xy_instances.reshape(-1), keys, colonies, seeds,
                              ctypes.byref(settings), batches, mne_target, threads, tours.reshape(-1), costs,
                              work.reshape(-1), ptr(trace_parent), ptr(trace_trails), ptr(trace_costs),
                              ctypes.byref(elapsed), int(bool(or_opt)),
-                             ptr(None if multipliers is None else mult)), L)
+                             ptr(None if multipliers is None else mult), ptr(g_offsets), ptr(g_neighbors), stride), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value}
     if trace:
         out.update(trace_parent=trace_parent, trace_trails=trace_trails, trace_costs=trace_costs)
@@ -506,6 +522,21 @@ def hard_relocation_allowed(tour, target, node, csr_graph):
     positions[tour] = np.arange(n, dtype=np.uint32)
     offsets, neighbors = csr_graph
     return bool(lib().oracle_hard_relocation_allowed(tour, positions, n, target, node, offsets, neighbors))
+
+
+def select_next_hard(xy, primary_row, backup_row, products, visited, current, u, tour, csr_graph):
+    """One Hard-mode roulette decision; returns (chosen, stage) with stage 0
+    primary, 1 zero-weight, 2 backup, 3 global fallback, 4 exhausted."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32)
+    n = xy.shape[0]
+    chosen, stage = ctypes.c_uint32(), ctypes.c_int()
+    offsets, neighbors = csr_graph
+    _check(lib().oracle_select_next_hard(
+        n, xy.reshape(-1), len(primary_row), len(backup_row), np.ascontiguousarray(primary_row, dtype=np.uint32),
+        np.ascontiguousarray(backup_row, dtype=np.uint32), np.ascontiguousarray(products, dtype=np.float64),
+        np.ascontiguousarray(visited, dtype=np.uint8), current, u, np.ascontiguousarray(tour, dtype=np.uint32),
+        offsets, neighbors, ctypes.byref(chosen), ctypes.byref(stage)), lib())
+    return chosen.value, stage.value
 
 
 def hard_two_opt_allowed(a, a_neighbor, b, b_neighbor, csr_graph, n):

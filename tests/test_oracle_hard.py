@@ -11,8 +11,14 @@ when the graph holds every edge of the tour after it. The graphs: the
 tour's own edges, those plus every new edge of the move, and the latter
 less each new edge in turn. Plus the reference's regression cases (a chord
 that covers only one of the new edges) and the undirected CSR rules
-(symmetry, no truncation of degrees, self-loops rejected by the engine's
-graph builder, out-of-range cities never members).
+(symmetry, no truncation of degrees, out-of-range cities never members).
+
+The oracle's Hard-mode runs (cuda/faco_choices.cuh:46-53: a city is
+available only when its relocation stays in the graph; faco_device.cuh:383-394:
+a 2-opt move leaving the graph never wins): selection exhausts instead of
+leaving the graph (the reference's 6-city regression), a run on the
+complete graph equals plain FACO bit for bit, and every tour of a run on the
+graph of primary candidates plus the initial tour stays inside that graph.
 """
 import itertools
 
@@ -108,3 +114,47 @@ def test_undirected_graph_rules():
     tour = np.array(order, dtype=np.uint32)
     assert not O.hard_two_opt_allowed(0, 1, 6, 2, (offsets, neighbors), 6)
     assert not O.hard_relocation_allowed(tour, 0, 0, (offsets, neighbors))
+
+
+def test_select_next_exhausts_instead_of_leaving_the_graph():
+    # the reference's regression: the graph is the 6-city tour itself; every
+    # candidate's relocation after city 0 adds an edge outside it
+    order = [0, 1, 2, 3, 4, 5]
+    ring = O.csr(sorted(cycle_edges(order)), 6)
+    xy = np.array([[0, 0], [1, 0], [2, 0], [3, 0], [4, 0], [5, 0]], dtype=np.int32)
+    visited = np.array([1, 1, 0, 0, 0, 0], dtype=np.uint8)
+    chosen, stage = O.select_next_hard(xy, [2, 3], [4, 5], [1.0, 1.0], visited, 0, 0.5, order, ring)
+    assert stage == 4 and chosen == 0
+    # with the chord (0, 2), (1, 3) the relocation of 2 after 0 stays inside
+    chordal = O.csr(sorted(cycle_edges(order) | {(0, 2), (1, 3)}), 6)
+    chosen, stage = O.select_next_hard(xy, [2, 3], [4, 5], [1.0, 1.0], visited, 0, 0.5, order, chordal)
+    assert (chosen, stage) == (2, 0)
+
+
+def _instances(n, count, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 10001, size=(count, n, 2)).astype(np.int32)
+
+
+def test_hard_runs_stay_in_the_graph_and_equal_plain_faco_on_the_complete_graph():
+    n, count, batches = 60, 2, 6
+    xys = _instances(n, count, 17)
+    s = O.default_settings(ants=16)
+    keys, seeds = np.array([5, 6], dtype=np.uint64), np.array([1, 2, 3, 4], dtype=np.uint64)
+    plain = O.run_faco(xys, keys, seeds, s, batches, 8)
+    complete = [O.csr([(a, b) for a in range(n) for b in range(a + 1, n)], n) for _ in range(count)]
+    same = O.run_faco(xys, keys, seeds, s, batches, 8, graphs=complete)
+    assert same["tours"].tolist() == plain["tours"].tolist() and same["work"].tolist() == plain["work"].tolist()
+    # the graph of the reference's from_candidates: primary candidates plus the initial tour
+    graphs, members = [], []
+    for g in range(count):
+        prep = O.prepare(xys[g], s)
+        tour = prep["initial_tour"].tolist()
+        edges = {tuple(sorted((a, int(b)))) for a in range(n) for b in prep["primary"][a]}
+        edges |= cycle_edges(tour)
+        graphs.append(O.csr(sorted(edges), n))
+        members.append(edges)
+    hard = O.run_faco(xys, keys, seeds, s, batches, 8, graphs=graphs)
+    for c in range(seeds.size):
+        assert cycle_edges(hard["tours"][c].tolist()) <= members[c % count], c
+    assert hard["work"].tolist() != plain["work"].tolist()     # the restriction changes the search
