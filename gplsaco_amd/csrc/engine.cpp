@@ -1246,6 +1246,20 @@ struct Json {
         if (value.find_first_of(".en") == std::string::npos) value += ".0";
         text += value;
     }
+    void string(const std::string& v) {   // JSON string with escapes (names come from the file)
+        text += '"';
+        for (const unsigned char c : v) {
+            if (c == '"' || c == '\\') { text += '\\'; text += static_cast<char>(c); }
+            else if (c < 0x20 || c >= 0x7f) {
+                char buffer[8];
+                std::snprintf(buffer, sizeof buffer, "\\u%04x", c);
+                text += buffer;
+            } else {
+                text += static_cast<char>(c);
+            }
+        }
+        text += '"';
+    }
     void key(const char* name) {
         if (!text.empty() && text.back() != '{' && text.back() != '[') text += ',';
         text += '"'; text += name; text += "\":";
@@ -1324,7 +1338,7 @@ std::string describe_state(const CountedState& s) {
     j.key("buffers"); j.raw("[");
     for (const auto& b : s.buffers) {
         j.item(); j.raw("{");
-        j.key("name"); j.raw("\"" + b.name + "\"");
+        j.key("name"); j.string(b.name);
         j.key("bytes"); j.number(b.bytes.size());
         j.key("element_bytes"); j.number(b.element_bytes);
         j.key("planes"); j.number(b.planes);
