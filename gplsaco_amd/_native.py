@@ -280,6 +280,9 @@ SIGNATURES = {
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,
                                                     u64p, vp, ctypes.POINTER(Checkpoints)]),
+    "faco_engine_register_graph_problem": (ctypes.c_int, [vp, ctypes.c_uint64, f64p, ctypes.c_uint32, vp, ctypes.c_uint64,
+                                                          u32p, u32p, vp, u32p, ctypes.POINTER(ctypes.c_double),
+                                                          ctypes.POINTER(ctypes.c_double)]),
     "faco_engine_register_problems_device": (ctypes.c_int, [vp, u64p, f64p, vp, ctypes.c_uint32,
                                                             ctypes.POINTER(ctypes.c_double)]),
     "faco_state_from_bytes": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),

@@ -161,6 +161,9 @@ struct Engine {
     // ants of the largest n). Each evaluation uploads every colony's shape
     // and the largest-work-first block order.
     bool mixed = false;
+    bool hard = false;                     // constraint mode Hard: every tour stays in its instance's graph
+    uint32_t graph_stride = 0;             // neighbour capacity per slot: 2 n (primary_width + 1)
+    DeviceArray<uint32_t> graph_offsets, graph_neighbors;   // [G][n + 1], [G][graph_stride]
     faco_settings base{};                  // settings as given (ants 0 = per-n default)
     DeviceArray<ColonyShape> shapes;       // [C]
     DeviceArray<uint32_t> order;           // [C]
@@ -337,6 +340,12 @@ struct Engine {
             }
             geo.initial.upload(p.initial.data(), m, stream, static_cast<size_t>(g) * n);
             geo.initial_cost.upload(&p.initial_cost, 1, stream, g);
+            if (hard) {
+                require(!p.graph_offsets.empty(), "a Hard-mode instance needs its graph (register_graph_problem)");
+                graph_offsets.upload(p.graph_offsets.data(), m + 1, stream, static_cast<size_t>(g) * (n + 1));
+                graph_neighbors.upload(p.graph_neighbors.data(), p.graph_neighbors.size(), stream,
+                                       static_cast<size_t>(g) * graph_stride);
+            }
             if (ctl) {
                 const auto samples = sample_nodes(m, slots[g]);
                 ctl->samples.upload(samples.data(), kSamples, stream, static_cast<size_t>(g) * kSamples);
@@ -377,6 +386,10 @@ struct Engine {
         p.state = state.data(); p.keys = keys.data(); p.targets = nullptr;
         p.geometry = geometry.data(); p.out_tours = tours.data(); p.out_info = info.data();
         if (shapes_active) { p.shapes = shapes.data(); p.order = order.data(); }
+        if (hard) {
+            p.graph_offsets = graph_offsets.data(); p.graph_neighbors = graph_neighbors.data();
+            p.graph_stride = graph_stride;
+        }
         (void)colonies;
         return p;
     }
@@ -482,7 +495,7 @@ struct Engine {
     // kernel: up to kRoundsPerLaunch batches per launch, each colony's
     // workgroup doing its own colony update between batches.
     static constexpr uint64_t kRoundsPerLaunch = 1024;
-    static bool persistent(const AntParams& ap) { return ap.blocks_per_colony == 1; }
+    static bool persistent(const AntParams& ap) { return ap.blocks_per_colony == 1 && !ap.graph_offsets; }
     // Run `iterations` batches: persistent launches when the shape allows,
     // else per batch an ant launch and a colony-update launch, through a
     // cached HIP graph of 16 batches when use_graph is set.
@@ -792,6 +805,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     require(lock.owns_lock(), "one evaluation per engine at a time");
     require(current_device() == e.device, "engine must be used on the device it was created on");
     require(spec.mode >= FACO_CONTROL_PROGRAM && spec.mode <= FACO_CONTROL_FACTORIAL, "unknown control mode");
+    require(!e.hard, "a Hard-mode engine runs plain FACO evaluations only");
     require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
     const uint32_t ants = e.config.ants;   // the ant stride in a mixed-size batch
     if (!e.mixed)
@@ -1402,9 +1416,18 @@ int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_setting
                        uint32_t constraint_mode, uint32_t population_size, faco_engine** out) {
     return guarded([&] {
         require(out && settings, "null argument");
-        require(constraint_mode == 0, "only the unrestricted constraint mode is available");
+        require(constraint_mode <= 1, "constraint mode must be 0 (unrestricted) or 1 (hard); escape is not available");
+        require(constraint_mode == 0 || population_size == 1, "a Hard-mode engine holds one population member");
         auto engine = std::make_unique<faco_engine>();
         engine->impl = std::make_unique<Engine>(dimension, colonies, *settings, population_size);
+        Engine& e = *engine->impl;
+        if (constraint_mode == 1) {
+            e.hard = true;
+            e.graph_stride = 2u * e.n * (e.config.primary_width + 1u);
+            e.graph_offsets.allocate(static_cast<size_t>(e.panel) * (e.n + 1));
+            e.graph_neighbors.allocate(static_cast<size_t>(e.panel) * e.graph_stride);
+            e.allocated += e.graph_offsets.bytes() + e.graph_neighbors.bytes();
+        }
         *out = engine.release();
     });
 }
@@ -1491,6 +1514,7 @@ int faco_engine_set_local_search(faco_engine* engine, uint32_t mode) {
     return guarded([&] {
         require(engine, "null argument");
         require(mode <= 1, "local search mode must be 0 (2-opt) or 1 (2-opt + Or-opt)");
+        require(mode == 0 || !engine->impl->hard, "a Hard-mode engine uses the plain 2-opt");
         engine->impl->local_search = mode;
         engine->impl->drop_graph();
     });
@@ -1518,6 +1542,7 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double
         Engine& e = *engine->impl;
         std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
         require(lock.owns_lock(), "registration and evaluation of one engine must not overlap");
+        require(!e.hard, "a Hard-mode engine registers instances with their graph (register_graph_problem)");
         if (e.mixed) {
             require(n <= e.n, "instance larger than the mixed engine's maximum dimension");
             const faco_settings own = normalized_settings(e.base, n);
@@ -1537,6 +1562,72 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double
         const Prepared& p = e.registry.at(key);
         if (cheap_seconds) *cheap_seconds = p.cheap_seconds;
         if (preparation_seconds) *preparation_seconds = p.preparation_seconds;
+    });
+}
+
+// register_graph_problem (reference batch_engine.cu:296-314,
+// prepared_problem.cpp:128-166): the common preparation, then the
+// candidate graph. This engine keeps one row layout (primary and LS rows as
+// prefixes of one (distance, id) order), so the spec's rows must be the
+// prepared rows; the graph itself is free (undirected, no self-loops or
+// repeated edges, holding the initial tour and every primary and LS row
+// edge, at most n (primary_width + 1) edges).
+int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const double* xy, uint32_t n,
+                                       const uint32_t* edges, uint64_t edge_count, const uint32_t* initial_tour,
+                                       const uint32_t* primary, const uint32_t* backup, const uint32_t* ls,
+                                       double* cheap_seconds, double* preparation_seconds) {
+    return guarded([&] {
+        require(engine && xy && initial_tour && primary && ls && (edges || !edge_count), "null argument");
+        Engine& e = *engine->impl;
+        std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+        require(lock.owns_lock(), "registration and evaluation of one engine must not overlap");
+        require(e.hard && n == e.n, "graph registration needs a Hard-mode engine of this dimension");
+        auto coords = integer_coordinates(xy, n);
+        Prepared p = prepare_problem(coords, e.config, e.K, e.nbr_stride);
+        const auto started = Clock::now();
+        require(std::equal(p.initial.begin(), p.initial.end(), initial_tour),
+                "the graph spec changed the common initial tour");
+        const uint32_t pw = e.config.primary_width, bw = e.config.backup_width, lw = e.config.ls_width;
+        require(std::equal(p.primary.begin(), p.primary.end(), primary) &&
+                std::equal(p.ls.begin(), p.ls.end(), ls) && (bw == 0 || (backup && std::equal(p.backup.begin(), p.backup.end(), backup))),
+                "graph rows must be the prepared candidate rows in this engine");
+        require(edge_count <= static_cast<uint64_t>(n) * (pw + 1u), "the graph exceeds n (primary_width + 1) edges");
+        std::vector<std::vector<uint32_t>> rows(n);
+        for (uint64_t i = 0; i < edge_count; ++i) {
+            const uint32_t a = edges[2 * i], b = edges[2 * i + 1];
+            require(a < n && b < n && a != b, "graph edge out of range or a self-loop");
+            rows[a].push_back(b);
+            rows[b].push_back(a);
+        }
+        p.graph_offsets.assign(1, 0);
+        for (auto& row : rows) {
+            std::sort(row.begin(), row.end());
+            require(std::adjacent_find(row.begin(), row.end()) == row.end(), "graph holds a repeated edge");
+            p.graph_neighbors.insert(p.graph_neighbors.end(), row.begin(), row.end());
+            p.graph_offsets.push_back(static_cast<uint32_t>(p.graph_neighbors.size()));
+        }
+        auto member = [&](uint32_t a, uint32_t b) {
+            return std::binary_search(p.graph_neighbors.begin() + p.graph_offsets[a],
+                                      p.graph_neighbors.begin() + p.graph_offsets[a + 1], b);
+        };
+        for (uint32_t i = 0; i < n; ++i)
+            require(member(p.initial[i], p.initial[(i + 1) % n]), "the graph misses an edge of the initial tour");
+        for (uint32_t a = 0; a < n; ++a) {
+            for (uint32_t j = 0; j < pw; ++j) require(member(a, p.primary[a * pw + j]), "a primary row edge is not in the graph");
+            for (uint32_t j = 0; j < lw; ++j) require(member(a, p.ls[a * lw + j]), "an LS row edge is not in the graph");
+        }
+        p.preparation_seconds += seconds_since(started);
+        const auto existing = e.registry.find(key);
+        if (existing != e.registry.end()) {
+            require(existing->second.xy == p.xy && existing->second.graph_offsets == p.graph_offsets &&
+                    existing->second.graph_neighbors == p.graph_neighbors,
+                    "same instance key registered with different coordinates or graph");
+        } else {
+            e.registry.emplace(key, std::move(p));
+        }
+        const Prepared& q = e.registry.at(key);
+        if (cheap_seconds) *cheap_seconds = q.cheap_seconds;
+        if (preparation_seconds) *preparation_seconds = q.preparation_seconds;
     });
 }
 

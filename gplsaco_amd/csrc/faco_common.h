@@ -252,6 +252,11 @@ struct AntParams {
     // every city in tour order instead of the construction's pushes
     const uint32_t* block_n;            // [blocks] or null
     uint32_t full_checklist;
+    // Hard mode: undirected CSR graph of every geometry slot, offsets
+    // [G][n + 1] and sorted neighbours [G][graph_stride]; null: unrestricted
+    const uint32_t* graph_offsets;
+    const uint32_t* graph_neighbors;
+    uint32_t graph_stride;
 };
 
 struct ColonyParams {

@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
 // `rounds` whole batches of its colony, each followed by the colony update
 // (update_colony) in the same workgroup, keeping the colony state in
 // `s_state` (LDS); otherwise one batch and the update is a separate kernel.
-template<bool NbrLds, bool Profile, bool Diag, bool OrOpt, bool Persistent>
+template<bool NbrLds, bool Profile, bool Diag, bool OrOpt, bool Persistent, bool Hard = false>
 __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParams& cp, uint32_t rounds,
                                            ColonyState* s_state) {
     // Diagnostic features (explicit visits, per-ant parents, traces and
@@ -353,6 +353,36 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     const double* products = p.products + static_cast<size_t>(colony) * ns * pw;
     const uint16_t* backup = p.backup + static_cast<size_t>(geom) * ns * p.bw;
     const uint32_t limit = static_cast<uint32_t>(p.ls_limit);   // clamped to 32 bits by the host
+    // Hard mode (reference cpp/include/gp_faco/edge_constraints.hpp:15-90):
+    // the instance's undirected CSR graph; a move is allowed when every
+    // added edge that does not cancel a removed one (as a multiset) is a
+    // graph edge
+    const uint32_t* g_off = Hard ? p.graph_offsets + static_cast<size_t>(geom) * (ns + 1) : nullptr;
+    const uint32_t* g_adj = Hard ? p.graph_neighbors + static_cast<size_t>(geom) * p.graph_stride : nullptr;
+    auto in_graph = [&](uint32_t a, uint32_t b) -> bool {
+        if (a >= n || b >= n || a == b) return false;
+        uint32_t lo = g_off[a], hi = g_off[a + 1];
+        const uint32_t end = hi;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (g_adj[mid] < b) lo = mid + 1; else hi = mid;
+        }
+        return lo < end && g_adj[lo] == b;
+    };
+    auto edges_allowed = [&](uint32_t count, const uint32_t (&removed)[3][2], const uint32_t (&added)[3][2]) -> bool {
+        bool consumed[3] = {false, false, false};
+        bool ok = true;
+        for (uint32_t i = 0; i < count; ++i) {
+            bool existing = false;
+            for (uint32_t j = 0; j < count; ++j) {
+                const bool same = (added[i][0] == removed[j][0] && added[i][1] == removed[j][1]) ||
+                                  (added[i][0] == removed[j][1] && added[i][1] == removed[j][0]);
+                if (!existing && !consumed[j] && same) { consumed[j] = true; existing = true; }
+            }
+            if (!existing && !in_graph(added[i][0], added[i][1])) ok = false;
+        }
+        return ok;
+    };
     const bool in_row = lane < pw;
     // Product-row loads are issued by every lane (lanes past the row re-read
     // its last slot), not under an exec mask: with no branch around them the
@@ -479,6 +509,26 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             return pushed;
         };
 
+        // Hard mode: may city c move to follow `cur`, whose successor in the
+        // working tour is cur_after (reference relocation_allowed,
+        // edge_constraints.hpp:70-82; availability, faco_choices.cuh:46-53)?
+        auto relocation_ok = [&](uint32_t c, uint32_t cur, uint32_t cur_after) -> bool {
+            if (c >= n || c == cur) return false;
+            if (cur_after == c) return true;
+            uint32_t before, after;
+            if (linked) {
+                const uint32_t l = lnk[c];
+                before = l & 0xffffu; after = l >> 16;
+            } else {
+                const uint32_t pc_c = pos[c];
+                before = tour[pc_c == 0 ? n - 1 : pc_c - 1];
+                after = tour[pc_c + 1 == n ? 0 : pc_c + 1];
+            }
+            const uint32_t removed[3][2] = {{before, c}, {c, after}, {cur, cur_after}};
+            const uint32_t added[3][2] = {{before, after}, {cur, c}, {c, cur_after}};
+            return edges_allowed(3, removed, added);
+        };
+
         // ------------------------------------------------------------ construction
         // The product rows of both parent neighbours of the current city are
         // prefetched at the start of a step: the roulette mostly follows a
@@ -540,7 +590,9 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                 }
                 u = read_lane(u_pool, step - u_base);
                 const uint32_t cand = cur_cand;
-                const bool avail = cur_free;
+                uint32_t cur_after = 0;
+                if constexpr (Hard) cur_after = linked ? (cur_link >> 16) : tour[pc + 1 == n ? 0 : pc + 1];
+                const bool avail = cur_free && (!Hard || relocation_ok(cand, current, cur_after));
                 const uint64_t amask = ballot(avail);
                 if (amask) {
                     // Roulette of select_next: the first available slot j with
@@ -602,7 +654,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     for (uint32_t base = 0; base < p.bw && selected == n; base += kWave) {
                         const uint32_t j = base + lane;
                         const uint32_t b = j < p.bw ? backup[static_cast<size_t>(current) * p.bw + j] : n;
-                        const uint64_t bm = ballot(b < n && !(flags[b] & kVisited));
+                        const uint64_t bm = ballot(b < n && !(flags[b] & kVisited) &&
+                                                   (!Hard || relocation_ok(b, current, cur_after)));
                         if (bm) selected = read_lane(b, ctz64(bm));
                     }
                     if (selected == n) {
@@ -610,7 +663,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                         const uint32_t cxy = s_xy[current];
                         uint64_t best = ~0ull;
                         for (uint32_t v = lane; v < n; v += kWave) {
-                            if (!(flags[v] & kVisited)) {
+                            if (!(flags[v] & kVisited) && (!Hard || relocation_ok(v, current, cur_after))) {
                                 const uint64_t kv = (static_cast<uint64_t>(euc2d(cxy, s_xy[v])) << 32) | v;
                                 best = kv < best ? kv : best;
                             }
@@ -913,7 +966,18 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                                     - static_cast<int32_t>(euc2d(xclosing, xn));
             const int32_t gain = (nb == a || b == closing) ? 0 : move_gain;
             // key: larger gain first, then lower tie index; 0 = no candidate.
-            const uint32_t key_value = active ? ((static_cast<uint32_t>(gain + (1 << 20)) << 6) | (63u - tie_index)) : 0u;
+            // Hard mode: a 2-opt move that leaves the graph never wins
+            // (reference cuda/faco_device.cuh:383-394)
+            bool move_allowed = true;
+            if constexpr (Hard) {
+                if (active && nb != a && b != closing) {
+                    const uint32_t removed[3][2] = {{a, closing}, {b, nb}, {0u, 0u}};
+                    const uint32_t added[3][2] = {{a, b}, {closing, nb}, {0u, 0u}};
+                    move_allowed = edges_allowed(2, removed, added);
+                }
+            }
+            const uint32_t key_value = active && move_allowed
+                ? ((static_cast<uint32_t>(gain + (1 << 20)) << 6) | (63u - tie_index)) : 0u;
             const uint32_t best = uniform(dpp_max_u32(key_value));
             int32_t best_gain = best ? static_cast<int32_t>(best >> 6) - (1 << 20) : -1;
 
@@ -1117,6 +1181,12 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 // Plain FACO over `rounds` batches in one launch: one workgroup per colony
 // runs its ants and its colony update batch after batch, so a colony never
 // waits for the others between batches.
+// Hard mode (graph-constrained construction and 2-opt), per batch.
+template<bool NbrLds>
+__global__ void __launch_bounds__(1024) faco_ant_search_hard(const AntParams p) {
+    ant_search<NbrLds, false, false, false, false, true>(p, ColonyParams{}, 1u, nullptr);
+}
+
 template<bool NbrLds, bool OrOpt>
 __global__ void __launch_bounds__(1024) faco_colony_run(const AntParams p, const ColonyParams cp, uint32_t rounds) {
     __shared__ ColonyState s_state;
@@ -1234,6 +1304,11 @@ hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t st
     // 2-opt carries neither the Or-opt moves nor the diagnostic outputs
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p); };
     const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
+    if (p.graph_offsets) {
+        if (diag || p.phase_ticks || oo) return hipErrorInvalidValue;   // Hard mode: plain 2-opt, no diagnostics
+        nl ? go(faco_ant_search_hard<true>) : go(faco_ant_search_hard<false>);
+        return hipGetLastError();
+    }
     if (diag) {
         if (p.phase_ticks) return hipErrorInvalidValue;   // no clock profile of diagnostic runs
         if (nl) oo ? go(faco_ant_search<true, false, true, true>) : go(faco_ant_search<true, false, true, false>);
@@ -1251,6 +1326,7 @@ hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t st
 hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_t colonies, uint32_t rounds,
                              hipStream_t stream) {
     if (p.blocks_per_colony != 1 || p.phase_ticks || p.explicit_mode || p.per_ant_parent || p.trace_selected ||
+        p.graph_offsets ||
         p.diag_construction || p.diag_positions || p.diag_checklist || p.control || p.out_identity)
         return hipErrorInvalidValue;   // one workgroup per colony, production plain FACO only
     const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;

@@ -29,6 +29,9 @@ struct Prepared {
     double profile[4] = {0, 0, 0, 0};  // candidates, nearest neighbour, initial LS, finalisation
     uint32_t s_lw = 0;                 // LS row width of `ls`
     std::vector<float> prior;          // [n][pw] heuristic multipliers of a learned edge prior, or empty
+    // Hard mode: the registered undirected graph as CSR (offsets [n + 1],
+    // sorted neighbours), empty for an unrestricted instance
+    std::vector<uint32_t> graph_offsets, graph_neighbors;
 };
 
 faco_settings normalized_settings(faco_settings settings, uint32_t n);

@@ -441,13 +441,14 @@ class FacoBatchEngine:
         population = _unsigned(population_size, "population_size", 32)
         if constraint_mode not in ("unrestricted", "hard", "escape"):
             raise ValueError("unknown constraint mode")
-        if constraint_mode != "unrestricted":
-            raise ValueError("only the unrestricted constraint mode is available in this engine")
+        if constraint_mode == "escape":
+            raise ValueError("the escape constraint mode is not available in this engine")
         if not isinstance(settings, FixedFacoSettings):
             raise TypeError("settings must be FixedFacoSettings")
         handle = ctypes.c_void_p()
-        check(lib().faco_engine_create(dimension, colonies, ctypes.byref(settings._native()), 0, population,
-                                       ctypes.byref(handle)))
+        check(lib().faco_engine_create(dimension, colonies, ctypes.byref(settings._native()),
+                                       1 if constraint_mode == "hard" else 0, population, ctypes.byref(handle)))
+        self.constraint_mode = constraint_mode
         self._handle = handle
         self.dimension, self.colonies, self.population_size = dimension, colonies, population
         s = Settings()
@@ -543,6 +544,42 @@ class FacoBatchEngine:
         cheap, prep = ctypes.c_double(), ctypes.c_double()
         check(lib().faco_engine_register_problem(self._handle, key, xy.reshape(-1), xy.shape[0], ctypes.byref(cheap),
                                                  ctypes.byref(prep)))
+        return {"cheap_seconds": cheap.value, "preparation_seconds": prep.value}
+
+    def register_graph_problem(self, instance_key, coordinates, dictionary) -> dict:
+        """Hard-mode registration (reference bindings.cpp:574-622): the spec
+        holds graph_spec_id 1, common_initial_tour, edges [[a, b], ...] and the
+        primary, backup and ls rows ([n][width] lists, right-padded with n).
+        The rows and the initial tour must be the prepared ones."""
+        xy = _coordinates(coordinates)
+        fields = ("graph_spec_id", "common_initial_tour", "edges", "primary", "backup", "ls")
+        if not isinstance(dictionary, dict) or set(dictionary) != set(fields):
+            raise ValueError("graph registration fields missing or extra")
+        if type(dictionary["graph_spec_id"]) is not int or dictionary["graph_spec_id"] != 1:
+            raise ValueError("unknown graph spec version")
+        n, s = xy.shape[0], self.settings
+
+        def rows(name, width):
+            value = np.asarray(dictionary[name], dtype=np.int64)
+            if value.shape != (n, width) or (value.size and (value.min() < 0 or value.max() > n)):
+                raise ValueError(f"graph rows '{name}' must be [n][{width}] cities (padding n)")
+            return np.ascontiguousarray(value, dtype=np.uint32).reshape(-1)
+        tour = np.asarray(dictionary["common_initial_tour"], dtype=np.int64)
+        if tour.shape != (n,) or tour.min() < 0 or tour.max() >= n:
+            raise ValueError("common initial tour must hold n cities")
+        edges = np.asarray(dictionary["edges"], dtype=np.int64).reshape(-1, 2) if len(dictionary["edges"]) else \
+            np.zeros((0, 2), dtype=np.int64)
+        if edges.size and (edges.min() < 0 or edges.max() >= n):
+            raise ValueError("graph edge out of range")
+        edges = np.ascontiguousarray(edges, dtype=np.uint32)
+        primary, ls = rows("primary", s.primary_width), rows("ls", s.ls_width)
+        backup = rows("backup", s.backup_width) if s.backup_width else None
+        cheap, prep = ctypes.c_double(), ctypes.c_double()
+        check(lib().faco_engine_register_graph_problem(
+            self._handle, _unsigned(instance_key, "instance_key"), xy.reshape(-1), n,
+            edges.ctypes.data_as(ctypes.c_void_p), edges.shape[0], np.ascontiguousarray(tour, dtype=np.uint32), primary,
+            backup.ctypes.data_as(ctypes.c_void_p) if backup is not None else None, ls, ctypes.byref(cheap),
+            ctypes.byref(prep)))
         return {"cheap_seconds": cheap.value, "preparation_seconds": prep.value}
 
     def register_problems_device(self, instance_keys, coordinates) -> dict:

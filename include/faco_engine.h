@@ -192,7 +192,8 @@ int faco_prepare_common_initial(const double* xy, uint32_t n, const faco_setting
 
 /* FacoBatchEngine(dimension, colonies, settings, constraint_mode, population_size)
  * (reference cpp/include/gp_faco/batch_engine.hpp:68-70; bindings.cpp:557-573).
- * constraint_mode: 0 unrestricted. population_size in [1, 128]. */
+ * constraint_mode: 0 unrestricted, 1 Hard (population 1; instances come
+ * through faco_engine_register_graph_problem). population_size in [1, 128]. */
 int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_settings* settings,
                        uint32_t constraint_mode, uint32_t population_size, faco_engine** out);
 /* Mixed-size batch engine: instances of any n up to max_dimension in one
@@ -272,6 +273,20 @@ int faco_engine_set_local_search(faco_engine* engine, uint32_t mode);
  * k-NN rows, NN tour, initial checklist 2-opt. */
 int faco_engine_register_problem(faco_engine* engine, uint64_t instance_key, const double* xy,
                                  uint32_t n, double* cheap_seconds, double* preparation_seconds);
+/* register_graph_problem(instance_key, coordinates, spec) for a Hard-mode
+ * engine (constraint_mode 1; reference batch_engine.cu:296-314,
+ * prepared_problem.cpp:128-166, bindings.cpp:574-622): the common
+ * preparation plus the candidate graph, edges [edge_count][2] (undirected,
+ * no self-loops or repeats, at most n * (primary_width + 1), holding the
+ * initial tour and every primary and LS row edge). The spec's initial tour
+ * and rows ([n][width], as problem_data returns them) must equal the
+ * prepared ones: this engine keeps one row layout. In Hard mode every
+ * relocation and 2-opt move keeps the tour inside the graph (construction
+ * exhausts instead of leaving it); plain 2-opt, plain FACO evaluations. */
+int faco_engine_register_graph_problem(faco_engine* engine, uint64_t instance_key, const double* xy, uint32_t n,
+                                       const uint32_t* edges, uint64_t edge_count, const uint32_t* initial_tour,
+                                       const uint32_t* primary, const uint32_t* backup, const uint32_t* ls,
+                                       double* cheap_seconds, double* preparation_seconds);
 /* Device-side preparation of `count` instances (reference
  * prepared_problem.cpp:58-126, the step register_problem runs on the host):
  * the same k-NN rows, nearest-neighbour tour and initial checklist 2-opt,

@@ -1,0 +1,98 @@
+"""Hard mode on the device (constraint_mode "hard", register_graph_problem)
+against the oracle's Hard runs (oracle.run_faco(..., graphs=...), itself
+pinned in tests/test_oracle_hard.py): the same tours, costs and work counters
+bit for bit, every tour inside its graph, on the linked-list (n <= 1024) and
+the array (n > 1024) construction paths. The graph is the reference's usual
+candidate graph: the primary and LS rows plus the initial tour
+(SparseUndirectedGraph::from_candidates, cpp/src/sparse_graph.cpp:27-40),
+optionally with a few random extra edges.
+"""
+import numpy as np
+import pytest
+
+from gplsaco_amd.instances import clustered_instance, uniform_instance
+from tests.conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def _cycle(tour):
+    return {tuple(sorted((int(tour[i]), int(tour[(i + 1) % len(tour)])))) for i in range(len(tour))}
+
+
+def _spec(data, extra_edges):
+    n = data["primary"].shape[0]
+    edges = set(_cycle(data["initial_tour"])) | set(extra_edges)
+    for name in ("primary", "ls"):
+        for a in range(n):
+            for b in data[name][a]:
+                if b < n:
+                    edges.add(tuple(sorted((a, int(b)))))
+    return {"graph_spec_id": 1, "common_initial_tour": data["initial_tour"].tolist(),
+            "edges": [list(e) for e in sorted(edges)], "primary": data["primary"].tolist(),
+            "backup": data["backup"].tolist(), "ls": data["ls"].tolist()}, edges
+
+
+@pytest.mark.parametrize("n,extra", [(200, 0), (300, 150), (1100, 0)])
+def test_hard_mode_matches_the_oracle_and_stays_in_the_graph(ext, oracle_mod, n, extra):
+    O = oracle_mod
+    count, batches = 2, 5
+    xys = [(clustered_instance if i % 2 else uniform_instance)(n, seed=41, instance=i) for i in range(count)]
+    keys = [300 + i for i in range(count)]
+    s = ext.FixedFacoSettings()
+    s.ants = 32
+    plain = ext.FacoBatchEngine(n, 4, s)
+    hard = ext.FacoBatchEngine(n, 4, s, constraint_mode="hard")
+    rng = np.random.default_rng(n)
+    graphs, members = [], []
+    for k, xy in zip(keys, xys):
+        plain.register_problem(k, xy)
+        pairs = {tuple(sorted(map(int, rng.choice(n, 2, replace=False)))) for _ in range(extra)}
+        spec, edges = _spec(plain.problem_data(k), pairs)
+        hard.register_graph_problem(k, xy, spec)
+        graphs.append(O.csr(sorted(edges), n))
+        members.append(edges)
+    task_keys = np.array([keys[c % count] for c in range(4)], dtype=np.uint64)
+    seeds = np.array([1, 1, 2, 2], dtype=np.uint64)
+    announce("hard", n, extra)
+    got = hard.evaluate_faco_evaluations(task_keys, seeds, s.ants * batches)
+    ref = plain.evaluate_faco_evaluations(task_keys, seeds, s.ants * batches)
+    settings = O.default_settings(**{f: getattr(s, f) for f in (
+        "ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
+        "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")})
+    want = O.run_faco(np.stack(xys).astype(np.int32), keys, seeds, settings, batches, 8, graphs=graphs)
+    for c in range(4):
+        tour = got["items"][c]["tour"]
+        assert tour == want["tours"][c].tolist(), c
+        assert got["items"][c]["cost"] == float(want["costs"][c])
+        assert _cycle(tour) <= members[c % count], c
+    np.testing.assert_array_equal(got["colony_work"], want["work"])
+    assert got["colony_work"].tolist() != ref["colony_work"].tolist()   # the graph changes the search
+
+
+def test_hard_registration_rules(ext):
+    n = 120
+    xy = uniform_instance(n, seed=43)
+    s = ext.FixedFacoSettings()
+    plain = ext.FacoBatchEngine(n, 2, s)
+    plain.register_problem(1, xy)
+    data = plain.problem_data(1)
+    spec, _ = _spec(data, [])
+    hard = ext.FacoBatchEngine(n, 2, s, constraint_mode="hard")
+    with pytest.raises(ValueError):
+        hard.register_problem(1, xy)                      # Hard instances come with their graph
+    missing = dict(spec, edges=spec["edges"][1:])          # an initial-tour or row edge dropped
+    with pytest.raises(ValueError):
+        hard.register_graph_problem(1, xy, missing)
+    other = dict(spec, common_initial_tour=spec["common_initial_tour"][::-1])
+    with pytest.raises(ValueError):
+        hard.register_graph_problem(1, xy, other)
+    looped = dict(spec, edges=spec["edges"] + [[3, 3]])
+    with pytest.raises(ValueError):
+        hard.register_graph_problem(1, xy, looped)
+    hard.register_graph_problem(1, xy, spec)
+    hard.register_graph_problem(1, xy, spec)              # the same registration again
+    with pytest.raises(ValueError):
+        hard.set_local_search("2opt+oropt")
+    with pytest.raises(ValueError):
+        ext.FacoBatchEngine(n, 2, s, constraint_mode="escape")
