@@ -805,7 +805,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     require(lock.owns_lock(), "one evaluation per engine at a time");
     require(current_device() == e.device, "engine must be used on the device it was created on");
     require(spec.mode >= FACO_CONTROL_PROGRAM && spec.mode <= FACO_CONTROL_FACTORIAL, "unknown control mode");
-    require(!e.hard, "a Hard-mode engine runs plain FACO evaluations only");
+    require(!e.hard || !e.local_search, "a Hard-mode engine uses the plain 2-opt");
     require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
     const uint32_t ants = e.config.ants;   // the ant stride in a mixed-size batch
     if (!e.mixed)
@@ -845,6 +845,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     uint64_t progress = limit, run_limit = limit;
     if (capturing || resuming) {
         require(C <= 128, "a state snapshot holds at most 128 colonies");
+        require(!e.hard, "state snapshots take unrestricted engines (reference batch_engine.cu:543-548)");
         require(!e.mixed && deciders == 1 && (spec.mode == FACO_CONTROL_PROGRAM || spec.mode == FACO_CONTROL_BASELINE) &&
                 !spec.force_fingerprint_collisions && !(flags & FACO_RUN_EVENTS) && !(checkpoints && checkpoints->count),
                 "state snapshots take plain program or baseline count runs, without profiling, forced collisions "

@@ -1535,7 +1535,9 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           float* trace_features, float* trace_scores, u32* trace_masks, std::int32_t* trace_actions,
                           u32* trace_alternatives, u32* trace_regions, std::int64_t* trace_costs,
                           double* elapsed_search_seconds, u32 local_search, u32 stop_batch, u32 fork_batch,
-                          u64 fork_seed, std::int32_t fork_action) {
+                          u64 fork_seed, std::int32_t fork_action,
+                          const u32* graph_offsets /* [instances][n + 1] or null: Hard mode */,
+                          const u32* graph_neighbors, u32 neighbor_stride) {
     // stop_batch (0 = the whole budget) ends the run early with the progress
     // feature still measured against evaluation_limit. At batch fork_batch
     // (UINT32_MAX = none) a paired fork (reference cuda/batch_engine.cu:839-845,
@@ -1556,6 +1558,11 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
             problems[g].s = normalized(*settings, n);
             prepare(problems[g]);
             problems[g].or_opt = local_search != 0;
+            if (graph_offsets) {
+                if (local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
+                problems[g].graph_offsets = graph_offsets + static_cast<size_t>(g) * (n + 1);
+                problems[g].graph_neighbors = graph_neighbors + static_cast<size_t>(g) * neighbor_stride;
+            }
             sample_nodes(n, instance_keys[g], samples[g].data());
         }
         const Settings s = problems[0].s;

@@ -247,7 +247,8 @@ def lib():
                                             ctypes.c_int,
                                             u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
-                                            ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
+                                            ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_uint32]
         L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.c_uint32]
@@ -386,6 +387,21 @@ def select_next(xy, primary_row, backup_row, products, visited, current, u):
                                 np.ascontiguousarray(visited, dtype=np.uint8), current, u,
                                 ctypes.byref(chosen), ctypes.byref(stage)), L)
     return chosen.value, stage.value
+
+
+def _graph_arrays(graphs, instances):
+    """(offsets [instances][n+1], neighbours [instances][stride], stride) of
+    one CSR graph per instance, or (None, None, 0)."""
+    if graphs is None:
+        return None, None, 0
+    if len(graphs) != instances:
+        raise ValueError("one graph per instance")
+    stride = max(len(nb) for _, nb in graphs)
+    offsets = np.ascontiguousarray(np.stack([np.asarray(o, dtype=np.uint32) for o, _ in graphs]))
+    neighbors = np.zeros((instances, stride), dtype=np.uint32)
+    for g, (_, nb) in enumerate(graphs):
+        neighbors[g, :len(nb)] = nb
+    return offsets, neighbors, stride
 
 
 class alternate_stream:
@@ -843,7 +859,7 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
-                   trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None):
+                   trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None, graphs=None):
     """Controlled FACO: colony c uses instance c % instances and decider c // instances.
 
     stop_batch (0 = all) ends the run after that many batches, the progress
@@ -882,6 +898,7 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         ctrl_arr = (Controller * len(controllers))(*controllers)
         deciders = len(controllers)
     policy = baseline if baseline is not None else Baseline()
+    g_off, g_nb, g_stride = _graph_arrays(graphs, instances)   # kept alive through the call
     L = lib()
     _check(L.oracle_run_controlled(
         n, instances, xy_instances.reshape(-1), keys, colonies, seeds, ctypes.byref(settings), evaluation_limit,
@@ -892,7 +909,8 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         ctypes.addressof(states), ptr(traces.get("features")), ptr(traces.get("scores")),
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
         ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt)),
-        stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1), L)
+        stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1,
+        ptr(g_off), ptr(g_nb), g_stride), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
                        "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,

@@ -5,7 +5,9 @@ bit for bit, every tour inside its graph, on the linked-list (n <= 1024) and
 the array (n > 1024) construction paths. The graph is the reference's usual
 candidate graph: the primary and LS rows plus the initial tour
 (SparseUndirectedGraph::from_candidates, cpp/src/sparse_graph.cpp:27-40),
-optionally with a few random extra edges.
+optionally with a few random extra edges. Controlled runs (a GP program,
+a rule baseline) in Hard mode equal the oracle's controlled Hard runs too
+(tours, work, every action, restarts).
 """
 import numpy as np
 import pytest
@@ -68,6 +70,45 @@ def test_hard_mode_matches_the_oracle_and_stays_in_the_graph(ext, oracle_mod, n,
         assert _cycle(tour) <= members[c % count], c
     np.testing.assert_array_equal(got["colony_work"], want["work"])
     assert got["colony_work"].tolist() != ref["colony_work"].tolist()   # the graph changes the search
+
+
+@pytest.mark.parametrize("mode", ["program", "baseline"])
+def test_controlled_hard_runs_match_the_oracle(ext, oracle_mod, mode):
+    O = oracle_mod
+    n, count, batches = 180, 2, 12
+    xys = [(clustered_instance if i % 2 else uniform_instance)(n, seed=47, instance=i) for i in range(count)]
+    keys = [400 + i for i in range(count)]
+    s = ext.FixedFacoSettings()
+    s.ants = 16
+    plain = ext.FacoBatchEngine(n, 4, s)
+    hard = ext.FacoBatchEngine(n, 4, s, constraint_mode="hard")
+    graphs = []
+    for k, xy in zip(keys, xys):
+        plain.register_problem(k, xy)
+        spec, edges = _spec(plain.problem_data(k), [])
+        hard.register_graph_problem(k, xy, spec)
+        graphs.append(O.csr(sorted(edges), n))
+    task_keys = np.array(keys * 2, dtype=np.uint64)
+    seeds = np.array([3, 3, 4, 4], dtype=np.uint64)
+    program = {"ir_version": 1, "numeric_spec_id": 1, "feature_spec_id": 2, "opcode": [0, 0, 2],
+               "operand": [4, 1, 0], "constant_bits": []}          # restarts readily
+    baseline = ext.read_baseline({"policy_spec_id": 1, "kind": "rule", "mne_level": 0, "max_mne_level": 3,
+                                  "region": 1, "restart_mode": "none", "restart_period": 0, "restart_probability": 0.0,
+                                  "stagnation_step": 2, "restart_stagnation": 5, "restart_cooldown": 3})
+    announce("hard-control", mode)
+    kw = {"programs": [ext.read_program(program)]} if mode == "program" else {"baseline": baseline}
+    raw = hard.evaluate_control_raw(task_keys, seeds, s.ants * batches, mode, trace_batches=range(batches), **kw)
+    settings = O.default_settings(**{f: getattr(s, f) for f in (
+        "ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
+        "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")})
+    okw = ({"programs": [O.Program.from_buffer_copy(bytes(kw["programs"][0]))]} if mode == "program"
+           else {"baseline": O.Baseline.from_buffer_copy(bytes(baseline))})
+    want = O.run_controlled(np.stack(xys).astype(np.int32), np.array(keys, dtype=np.uint64), seeds, settings,
+                            s.ants * batches, mode, trace=True, graphs=graphs, **okw)
+    assert raw["tours"].tolist() == want["tours"].tolist()
+    np.testing.assert_array_equal(raw["work"], want["work"])
+    np.testing.assert_array_equal(raw["trace_actions"], want["trace_actions"])
+    assert [st["restarts"] for st in raw["states"]] == [st["restarts"] for st in want["states"]]
 
 
 def test_hard_registration_rules(ext):
