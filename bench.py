@@ -113,6 +113,45 @@ def gp_population_rate(ext, steps: int) -> dict:
             "mean_final_tour_length": float(raw["costs"].mean())}
 
 
+def hard_mode_rate(ext, iterations: int) -> dict:
+    """Hard mode (every relocation and 2-opt move kept inside a candidate
+    graph): the headline panel (32 TSP1000 instances x 8 seeds, 128 ants,
+    2-opt) with each instance's graph = primary and LS rows plus the initial
+    tour, per-batch launches; beside it the same panel unrestricted."""
+    from gplsaco_amd.instances import instance_key, uniform_instance
+    n, count, seeds_per = 1000, 32, 8
+    xys = [uniform_instance(n, seed=0, instance=i) for i in range(count)]
+    keys = [instance_key(n, 0, i, False) for i in range(count)]
+    plain = ext.FacoBatchEngine(n, count * seeds_per, ext.FixedFacoSettings())
+    hard = ext.FacoBatchEngine(n, count * seeds_per, ext.FixedFacoSettings(), constraint_mode="hard")
+    edges_total = 0
+    for k, xy in zip(keys, xys):
+        plain.register_problem(k, xy)
+        data = plain.problem_data(k)
+        tour = data["initial_tour"]
+        edges = {tuple(sorted((int(tour[i]), int(tour[(i + 1) % n])))) for i in range(n)}
+        for name in ("primary", "ls"):
+            for a in range(n):
+                edges.update(tuple(sorted((a, int(b)))) for b in data[name][a])
+        edges_total += len(edges)
+        hard.register_graph_problem(k, xy, {"graph_spec_id": 1, "common_initial_tour": tour.tolist(),
+                                            "edges": sorted(edges), "primary": data["primary"].tolist(),
+                                            "backup": data["backup"].tolist(), "ls": data["ls"].tolist()})
+    task_keys = np.array([keys[c % count] for c in range(count * seeds_per)], dtype=np.uint64)
+    seeds = np.array([c // count for c in range(count * seeds_per)], dtype=np.uint64)
+    ants = hard.settings.ants
+    hard.evaluate_faco_evaluations(task_keys, seeds, ants * 4)      # warm-up (graph capture)
+    out = hard.evaluate_faco_evaluations(task_keys, seeds, ants * iterations)
+    ref = plain.evaluate_faco_evaluations(task_keys, seeds, ants * iterations)
+    mean = lambda o: float(np.mean([i["cost"] for i in o["items"]]))  # noqa: E731
+    return {"workload": f"Hard mode, {count} x TSP1000 x {seeds_per} seeds, {ants} ants, {iterations} iterations, "
+                        f"graph = primary + LS rows + initial tour ({edges_total / count:.0f} edges per instance)",
+            "ant_tours_per_s": out["total_tour_evaluations"] / out["search_seconds"],
+            "unrestricted_ant_tours_per_s": ref["total_tour_evaluations"] / ref["search_seconds"],
+            "mean_final_tour_length": mean(out), "unrestricted_mean_final_tour_length": mean(ref),
+            "ls_evaluations_per_tour": out["completed_ls_evaluations"] / out["total_tour_evaluations"]}
+
+
 def uneven_batch_rate(ext, rounds: int) -> dict:
     """The uneven batch: 64 instances with n uniform in [200, 2000] (32
     uniform, 32 clustered) x 128 random single-tree GP programs (depth at
@@ -278,6 +317,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--population-steps", type=int, default=50,
                     help="iterations of the GP-controlled population measurement (0 skips it)")
+    ap.add_argument("--hard-iterations", type=int, default=20,
+                    help="iterations of the Hard-mode leg (0 skips it)")
     ap.add_argument("--uneven-rounds", type=int, default=40,
                     help="budget of the uneven-size GP batch in units of 384 ant tours per colony (0 skips it)")
     ap.add_argument("--pipeline-rounds", type=int, default=10,
@@ -405,6 +446,8 @@ def main() -> int:
     }
     if args.population_steps:
         line["config"]["gp_population"] = gp_population_rate(ext, args.population_steps)
+    if args.hard_iterations:
+        line["config"]["hard_mode"] = hard_mode_rate(ext, args.hard_iterations)
     if args.uneven_rounds:
         line["config"]["uneven_batch"] = uneven_batch_rate(ext, args.uneven_rounds)
     if args.pipeline_rounds:
