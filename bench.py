@@ -89,7 +89,7 @@ def gp_population_rate(ext, steps: int) -> dict:
     random feature-spec-2 programs (512 colonies x 128 ants), one launch per
     iteration stage, `steps` iterations replayed through a HIP graph."""
     from gplsaco_amd.instances import instance_key, uniform_instance
-    from tests.program_gen import random_program
+    from gplsaco_amd.programs import random_program
     n, instances, individuals = 500, 4, 128
     rng = np.random.default_rng(0)
     programs = [{**random_program(rng, depth=5), "feature_spec_id": 2} for _ in range(individuals)]
@@ -162,7 +162,7 @@ def uneven_batch_rate(ext, rounds: int) -> dict:
     `rounds` x 384 ant tours per colony (384 = a whole number of batches of
     every ant count)."""
     from gplsaco_amd.instances import _key, _uniform_words, clustered_instance, instance_key, uniform_instance
-    from tests.program_gen import random_program
+    from gplsaco_amd.programs import random_program
     count, individuals = 64, 128
     words = _uniform_words(_key(3, 0, 0xC3), count)
     sizes = [200 + int(w % np.uint64(1801)) for w in words]
@@ -216,7 +216,7 @@ def full_pipeline(ext, rounds: int) -> dict:
     (same k, no prior, no control) on the same instances with 8 seeds and the
     same budget for the gap."""
     from gplsaco_amd import prior as P
-    from tests.program_gen import random_controller
+    from gplsaco_amd.programs import random_controller
     sizes = [1000] * 10 + [2000] * 10
     settings = ext.FixedFacoSettings()
     settings.primary_width = 20

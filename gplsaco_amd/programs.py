@@ -1,4 +1,7 @@
-"""Random IR v1 programs (postfix, depth-bounded) for scoring tests."""
+"""Random GP deciders for synthetic workloads: IR v1 programs (postfix,
+depth-bounded, at most 63 nodes and a stack of 6, constants in [-2, 2]: the
+limits of the reference's validate_program, cpp/src/program_cpu.cpp:26-62)
+and conditional three-tree controllers (python/gp_faco/controller.py:61-82)."""
 import struct
 
 import numpy as np

@@ -1537,7 +1537,8 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           double* elapsed_search_seconds, u32 local_search, u32 stop_batch, u32 fork_batch,
                           u64 fork_seed, std::int32_t fork_action,
                           const u32* graph_offsets /* [instances][n + 1] or null: Hard mode */,
-                          const u32* graph_neighbors, u32 neighbor_stride) {
+                          const u32* graph_neighbors, u32 neighbor_stride,
+                          const float* multipliers /* [instances][n * pw] or null */) {
     // stop_batch (0 = the whole budget) ends the run early with the progress
     // feature still measured against evaluation_limit. At batch fork_batch
     // (UINT32_MAX = none) a paired fork (reference cuda/batch_engine.cu:839-845,
@@ -1562,6 +1563,10 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                 if (local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
                 problems[g].graph_offsets = graph_offsets + static_cast<size_t>(g) * (n + 1);
                 problems[g].graph_neighbors = graph_neighbors + static_cast<size_t>(g) * neighbor_stride;
+            }
+            if (multipliers) {
+                const size_t cells = static_cast<size_t>(n) * problems[g].s.primary_width;
+                problems[g].prior.assign(multipliers + g * cells, multipliers + (g + 1) * cells);
             }
             sample_nodes(n, instance_keys[g], samples[g].data());
         }
@@ -1666,6 +1671,93 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                                        k.fb.restarts, k.fb.return_rate, k.fb.ls_work};
             }
         }
+    });
+}
+
+// Explicit-state access to one batch's reduce/select + pheromone update
+// (colony_select_and_update) and to one restart transaction
+// (control_restart), so that their rules can be checked against hand-made
+// cases. The instance is prepared as in a run (rows, heuristic); the colony
+// state comes in and goes out as plain arrays:
+//   costs [3] = epoch, global, parent cost; tours [3][n] = parent, epoch,
+//   global tour; trails [n][pw]; default_trail; bounds [2] = min, max (out);
+//   products [n][pw] (out).
+int oracle_colony_select(u32 n, const std::int32_t* xy, const Settings* settings, u64 key, u32 batch, u32 ants,
+                         const u32* ant_tours, const std::int64_t* ant_costs, std::int64_t* costs, u32* tours,
+                         double* trails, double* default_trail, double* bounds, double* products,
+                         u32* iteration_best, u32* source_is_epoch, double* source_uniform) {
+    return guarded([&] {
+        check_coordinates(n, xy);
+        if (!ants) throw std::invalid_argument("a batch needs at least one ant");
+        Problem p;
+        p.n = n; p.xy = xy; p.s = normalized(*settings, n);
+        prepare(p);
+        ColonyRun r;
+        colony_reset(p, r);
+        const size_t cells = static_cast<size_t>(n) * p.s.primary_width;
+        r.epoch_cost = costs[0]; r.global_cost = costs[1]; r.parent_cost = costs[2];
+        r.parent.assign(tours, tours + n);
+        r.epoch.assign(tours + n, tours + 2 * n);
+        r.global.assign(tours + 2 * n, tours + 3 * n);
+        r.trails.assign(trails, trails + cells);
+        r.default_trail = *default_trail;
+        std::vector<AntResult> results(ants);
+        for (u32 a = 0; a < ants; ++a) {
+            results[a].tour.assign(ant_tours + static_cast<size_t>(a) * n, ant_tours + static_cast<size_t>(a + 1) * n);
+            results[a].final_cost = ant_costs[a];
+        }
+        colony_select_and_update(p, r, results, key, batch);
+        costs[0] = r.epoch_cost; costs[1] = r.global_cost; costs[2] = r.parent_cost;
+        std::copy(r.parent.begin(), r.parent.end(), tours);
+        std::copy(r.epoch.begin(), r.epoch.end(), tours + n);
+        std::copy(r.global.begin(), r.global.end(), tours + 2 * n);
+        std::copy(r.trails.begin(), r.trails.end(), trails);
+        std::copy(r.products.begin(), r.products.end(), products);
+        *default_trail = r.default_trail;
+        bounds[0] = r.minimum; bounds[1] = r.maximum;
+        *iteration_best = r.iteration_best;
+        *source_is_epoch = r.source_is_epoch;
+        *source_uniform = r.source_uniform;
+    });
+}
+
+// archive [count][n] tours with costs; identities out [2][2] = active and
+// epoch identity after the transaction.
+int oracle_control_restart(u32 n, const std::int32_t* xy, const Settings* settings, u32 count, const u32* archive,
+                           const std::int64_t* archive_costs, u32 alternative, int initializing, std::int64_t* costs,
+                           u32* tours, double* trails, double* default_trail, double* bounds, double* products,
+                           Feedback* feedback, u64* identities) {
+    return guarded([&] {
+        check_coordinates(n, xy);
+        if (!count || count > kArchive || alternative >= count) throw std::invalid_argument("archive slot out of range");
+        Problem p;
+        p.n = n; p.xy = xy; p.s = normalized(*settings, n);
+        prepare(p);
+        ColonyRun r;
+        colony_reset(p, r);
+        const size_t cells = static_cast<size_t>(n) * p.s.primary_width;
+        r.epoch_cost = costs[0]; r.global_cost = costs[1]; r.parent_cost = costs[2];
+        r.parent.assign(tours, tours + n);
+        r.epoch.assign(tours + n, tours + 2 * n);
+        r.global.assign(tours + 2 * n, tours + 3 * n);
+        r.trails.assign(trails, trails + cells);
+        r.default_trail = *default_trail;
+        Control ctl;
+        ctl.fb = *feedback;
+        for (u32 i = 0; i < count; ++i)
+            ctl.archive.push_back(make_rec(archive + static_cast<size_t>(i) * n, n, archive_costs[i]));
+        control_restart(p, r, ctl, alternative, initializing != 0);
+        costs[0] = r.epoch_cost; costs[1] = r.global_cost; costs[2] = r.parent_cost;
+        std::copy(r.parent.begin(), r.parent.end(), tours);
+        std::copy(r.epoch.begin(), r.epoch.end(), tours + n);
+        std::copy(r.global.begin(), r.global.end(), tours + 2 * n);
+        std::copy(r.trails.begin(), r.trails.end(), trails);
+        std::copy(r.products.begin(), r.products.end(), products);
+        *default_trail = r.default_trail;
+        bounds[0] = r.minimum; bounds[1] = r.maximum;
+        *feedback = ctl.fb;
+        identities[0] = ctl.active.first; identities[1] = ctl.active.second;
+        identities[2] = ctl.epoch_id.first; identities[3] = ctl.epoch_id.second;
     });
 }
 

@@ -248,7 +248,14 @@ def lib():
                                             u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_uint32]
+                                            ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.oracle_colony_select.argtypes = [ctypes.c_uint32, i32p, ctypes.POINTER(Settings), ctypes.c_uint64,
+                                           ctypes.c_uint32, ctypes.c_uint32, u32p, i64p, i64p, u32p, f64p, f64p,
+                                           f64p, f64p, ctypes.POINTER(ctypes.c_uint32),
+                                           ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
+        L.oracle_control_restart.argtypes = [ctypes.c_uint32, i32p, ctypes.POINTER(Settings), ctypes.c_uint32, u32p,
+                                             i64p, ctypes.c_uint32, ctypes.c_int, i64p, u32p, f64p, f64p, f64p, f64p,
+                                             ctypes.POINTER(Feedback), u64p]
         L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.c_uint32]
@@ -859,10 +866,12 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
-                   trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None, graphs=None):
+                   trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None, graphs=None,
+                   multipliers=None):
     """Controlled FACO: colony c uses instance c % instances and decider c // instances.
 
-    stop_batch (0 = all) ends the run after that many batches, the progress
+    multipliers [instances, n, pw] (float32): learned heuristic multipliers,
+    as for run_faco. stop_batch (0 = all) ends the run after that many batches, the progress
     feature still over evaluation_limit; fork = (batch, seed, action) forces
     that batch's action and re-keys every colony from its ants on."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
@@ -899,6 +908,11 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         deciders = len(controllers)
     policy = baseline if baseline is not None else Baseline()
     g_off, g_nb, g_stride = _graph_arrays(graphs, instances)   # kept alive through the call
+    mult = None
+    if multipliers is not None:
+        mult = np.ascontiguousarray(multipliers, dtype=np.float32)
+        if mult.shape[0] != instances:
+            raise ValueError("one multiplier table per instance")
     L = lib()
     _check(L.oracle_run_controlled(
         n, instances, xy_instances.reshape(-1), keys, colonies, seeds, ctypes.byref(settings), evaluation_limit,
@@ -910,13 +924,72 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
         ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt)),
         stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1,
-        ptr(g_off), ptr(g_nb), g_stride), L)
+        ptr(g_off), ptr(g_nb), g_stride, ptr(mult)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
                        "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,
                        "ls_work": s.ls_work} for s in states]}
     out.update({"trace_" + k: v for k, v in traces.items()})
     return out
+
+
+def _colony_arrays(n, pw, costs, tours, trails):
+    costs = np.array(costs, dtype=np.int64)
+    tours = np.ascontiguousarray(np.asarray(tours, dtype=np.uint32).reshape(3, n)).copy()
+    trails = np.ascontiguousarray(np.asarray(trails, dtype=np.float64).reshape(n * pw)).copy()
+    return costs, tours, trails
+
+
+def colony_select(xy, settings, key, batch, ant_tours, ant_costs, costs, tours, trails, default_trail):
+    """One batch's reduce/select and pheromone update on explicit colony
+    state. costs = (epoch, global, parent), tours = [parent, epoch, global]
+    ([3][n]), trails [n][pw]. Returns the state after the batch with the
+    bounds, products, iteration best, source coin and its uniform."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
+    n = xy.size // 2
+    norm = prepare(xy, settings)["settings"]
+    ant_tours = np.ascontiguousarray(ant_tours, dtype=np.uint32)
+    ant_costs = np.ascontiguousarray(ant_costs, dtype=np.int64)
+    costs, tours, trails = _colony_arrays(n, norm.primary_width, costs, tours, trails)
+    default = np.array([default_trail], dtype=np.float64)
+    bounds = np.zeros(2, dtype=np.float64)
+    products = np.zeros(n * norm.primary_width, dtype=np.float64)
+    best, epoch_source, uniform = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+    L = lib()
+    _check(L.oracle_colony_select(n, xy, ctypes.byref(settings), key, batch, ant_costs.size, ant_tours.reshape(-1),
+                                  ant_costs, costs, tours, trails, default, bounds, products, ctypes.byref(best),
+                                  ctypes.byref(epoch_source), ctypes.byref(uniform)), L)
+    return {"costs": costs, "parent": tours[0], "epoch": tours[1], "global": tours[2],
+            "trails": trails.reshape(n, norm.primary_width), "default_trail": float(default[0]),
+            "minimum": float(bounds[0]), "maximum": float(bounds[1]),
+            "products": products.reshape(n, norm.primary_width), "iteration_best": best.value,
+            "source_is_epoch": bool(epoch_source.value), "source_uniform": uniform.value}
+
+
+def control_restart(xy, settings, archive, archive_costs, alternative, initializing, costs, tours, trails,
+                    default_trail, feedback: Feedback):
+    """One restart transaction on explicit colony state (same layout as
+    colony_select) with an archive [count][n]; `feedback` is updated in place."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
+    n = xy.size // 2
+    norm = prepare(xy, settings)["settings"]
+    archive = np.ascontiguousarray(archive, dtype=np.uint32)
+    archive_costs = np.ascontiguousarray(archive_costs, dtype=np.int64)
+    costs, tours, trails = _colony_arrays(n, norm.primary_width, costs, tours, trails)
+    default = np.array([default_trail], dtype=np.float64)
+    bounds = np.zeros(2, dtype=np.float64)
+    products = np.zeros(n * norm.primary_width, dtype=np.float64)
+    identities = np.zeros(4, dtype=np.uint64)
+    L = lib()
+    _check(L.oracle_control_restart(n, xy, ctypes.byref(settings), archive_costs.size, archive.reshape(-1),
+                                    archive_costs, alternative, int(bool(initializing)), costs, tours, trails,
+                                    default, bounds, products, ctypes.byref(feedback), identities), L)
+    return {"costs": costs, "parent": tours[0], "epoch": tours[1], "global": tours[2],
+            "trails": trails.reshape(n, norm.primary_width), "default_trail": float(default[0]),
+            "minimum": float(bounds[0]), "maximum": float(bounds[1]),
+            "products": products.reshape(n, norm.primary_width),
+            "active_identity": (int(identities[0]), int(identities[1])),
+            "epoch_identity": (int(identities[2]), int(identities[3]))}
 
 
 # Reference checker, control layer.

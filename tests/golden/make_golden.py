@@ -26,7 +26,7 @@ ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
 
 from oracle import oracle as O  # noqa: E402
-from tests.program_gen import random_program  # noqa: E402
+from gplsaco_amd.programs import random_program  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 U64 = (1 << 64) - 1

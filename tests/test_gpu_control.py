@@ -13,7 +13,7 @@ import pytest
 
 from gplsaco_amd.instances import clustered_instance, uniform_instance
 from tests.conftest import announce
-from tests.program_gen import random_program
+from gplsaco_amd.programs import random_program
 
 pytestmark = pytest.mark.gpu
 

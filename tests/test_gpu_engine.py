@@ -268,7 +268,7 @@ def test_score_gpu_matches_score_cpu(ext):
     features = rng.random((12, 7, 32), dtype=np.float32)
     masks = rng.integers(1, 0xFFFFFFFF, size=7, dtype=np.uint32)
     masks[:3] = [1, 1 << 31, 0xFFFFFFFF]
-    from tests.program_gen import random_program
+    from gplsaco_amd.programs import random_program
     for i in range(200):
         program = random_program(rng, depth=5)
         a = ext.score_cpu(program, features, masks)

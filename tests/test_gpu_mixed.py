@@ -146,7 +146,7 @@ def test_mixed_program_population_matches_oracle(ext, oracle_mod):
     small scale): colony d * panel + t equals the oracle's controlled run of
     instance t alone under program d, with that instance's own ants and
     iteration count (features, actions, restarts, archive, tours)."""
-    from tests.program_gen import random_program
+    from gplsaco_amd.programs import random_program
     from tests.test_gpu_control import RESTARTY, spec2, to_oracle_program
     sizes = (130, 300, 220)
     xys, keys = _panel(sizes)

@@ -19,7 +19,7 @@ import pytest
 
 from gplsaco_amd import _native, gp_faco_ext as ext
 from oracle import oracle as O
-from tests.program_gen import random_program
+from gplsaco_amd.programs import random_program
 
 ROOT = Path(__file__).resolve().parents[1]
 HEADER = ROOT / "include" / "faco_engine.h"
