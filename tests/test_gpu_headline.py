@@ -62,7 +62,7 @@ def test_headline_shape_persistent_kernel_matches_oracle(ext, oracle_mod, local_
     shape = engine.launch_shape(colonies)
     # the production shape: one workgroup per colony (the persistent kernel),
     # 16 ants in flight, neighbour rows staged in LDS
-    assert (shape["blocks_per_colony"], shape["waves_per_block"], shape["nbr_in_lds"]) == (1, 16, 1), shape
+    assert (shape["blocks_per_colony"], shape["waves_per_block"], shape["neighbour_rows_in_lds"]) == (1, 16, True), shape
     keys = np.array([300 + c % instances for c in range(colonies)], dtype=np.uint64)
     seeds = np.array([c // instances for c in range(colonies)], dtype=np.uint64)
     announce("headline", local_search, colonies, batches)
