@@ -270,6 +270,8 @@ SIGNATURES = {
     "faco_prior_logits": (ctypes.c_int, [vp, f32p, ctypes.c_uint32, f32p]),
     "faco_prior_gradients": (ctypes.c_int, [vp, f32p, u8p, ctypes.c_uint32, ctypes.c_float, f32p,
                                             ctypes.POINTER(ctypes.c_float)]),
+    "faco_prior_gradients_noalloc": (ctypes.c_int, [vp, f32p, u8p, ctypes.c_uint32, ctypes.c_float, f32p,
+                                                    ctypes.POINTER(ctypes.c_float)]),
     "faco_prior_train": (ctypes.c_int, [vp, f32p, u8p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_float, ctypes.c_float, ctypes.c_uint64, f32p]),
     "faco_prior_mma_check": (ctypes.c_int, [f32p, f32p, f32p]),

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -74,6 +75,49 @@ struct PriorImpl {
 };
 
 const float* prior_device_params(const faco_prior* prior) { return reinterpret_cast<const PriorImpl*>(prior)->params.ptr; }
+
+// Scratch of faco_prior_gradients_noalloc: one minibatch of at most
+// FACO_PRIOR_NOALLOC_EDGES edges (the training loop's batch size). Layout in
+// floats, every region 256-byte aligned: the per-wave gradient partials
+// [rows][P + 1], the reduced gradient and loss [P + 1], the staged features
+// [edges][8] and the staged labels (bytes). Each GPU the library runs on
+// holds its own copy until the process ends.
+namespace noalloc {
+constexpr uint32_t kEdges = FACO_PRIOR_NOALLOC_EDGES;
+constexpr uint32_t kTileEdges = 32, kWaves = 2;   // prior_grad_rows: one row per wave of 2-wave blocks
+constexpr uint32_t kRows = ((kEdges + kTileEdges - 1) / kTileEdges + kWaves - 1) / kWaves * kWaves;
+constexpr uint32_t kRow = 4801 + 1;
+constexpr size_t align64(size_t floats) { return (floats + 63) / 64 * 64; }
+constexpr size_t kPartial = 0;
+constexpr size_t kGrad = kPartial + align64(static_cast<size_t>(kRows) * kRow);
+constexpr size_t kFeatures = kGrad + align64(kRow);
+constexpr size_t kLabels = kFeatures + align64(static_cast<size_t>(kEdges) * 8);
+constexpr size_t kFloats = kLabels + align64(kEdges / 4);
+constexpr int kMaxDevices = 64;
+}  // namespace noalloc
+
+static __device__ __align__(256) float scratch[noalloc::kFloats];
+
+namespace {
+// Device address of `scratch` per device, looked up once with that device
+// current. In host code the array's name is not a device address: only
+// these pointers are passed to kernels and copies.
+std::mutex g_scratch_mutex;
+float* g_scratch[noalloc::kMaxDevices] = {};
+
+float* device_scratch() {
+    int device = 0;
+    checked(hipGetDevice(&device), "hipGetDevice");
+    require(device >= 0 && device < noalloc::kMaxDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void* address = nullptr;
+        checked(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch[device] = static_cast<float*>(address);
+    }
+    return g_scratch[device];
+}
+}  // namespace
 
 }  // namespace faco
 
@@ -155,6 +199,36 @@ int faco_prior_gradients(faco_prior* prior, const float* features, const uint8_t
         faco::checked(hipStreamSynchronize(p->stream), "prior gradients");
         std::memcpy(grads, g.data(), faco::prior_param_count() * sizeof(float));
         if (loss) *loss = g.back();
+    });
+}
+
+int faco_prior_gradients_noalloc(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges,
+                                 float pos_weight, float* grads, float* loss) {
+    return faco::guarded([&] {
+        faco::require(prior && features && labels && grads && edges, "null argument or no edges");
+        faco::require(edges <= FACO_PRIOR_NOALLOC_EDGES,
+                      "faco_prior_gradients_noalloc takes at most FACO_PRIOR_NOALLOC_EDGES (8192) edges");
+        faco::require(std::isfinite(pos_weight) && pos_weight > 0, "positive weight must be positive");
+        const uint32_t rows = faco::prior_grad_rows(edges), P = faco::prior_param_count();
+        faco::require(rows <= faco::noalloc::kRows && P + 1 == faco::noalloc::kRow, "scratch layout out of date");
+        auto* p = reinterpret_cast<PriorImpl*>(prior);
+        float* base = faco::device_scratch();
+        float* partial = base + faco::noalloc::kPartial;
+        float* grad = base + faco::noalloc::kGrad;
+        float* staged = base + faco::noalloc::kFeatures;
+        auto* y = reinterpret_cast<uint8_t*>(base + faco::noalloc::kLabels);
+        // every partial row the reduction reads and every gradient entry is
+        // written by this call's kernels, so nothing needs clearing
+        faco::checked(hipMemcpyAsync(staged, features, static_cast<size_t>(edges) * 8 * sizeof(float),
+                                     hipMemcpyHostToDevice, p->stream), "upload");
+        faco::checked(hipMemcpyAsync(y, labels, edges, hipMemcpyHostToDevice, p->stream), "upload");
+        faco::checked(faco::launch_prior_grad(p->params.ptr, staged, y, nullptr, 0, edges, pos_weight,
+                                              1.0f / static_cast<float>(edges), partial, p->stream), "gradient launch");
+        faco::checked(faco::launch_prior_reduce(partial, rows, grad, p->params.ptr, p->m.ptr, p->v.ptr, false,
+                                                0, 0, 0, 0, 1, nullptr, p->stream), "reduce launch");
+        faco::checked(hipMemcpyAsync(grads, grad, P * sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");
+        if (loss) faco::checked(hipMemcpyAsync(loss, grad + P, sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");
+        faco::checked(hipStreamSynchronize(p->stream), "prior gradients");
     });
 }
 

@@ -468,6 +468,19 @@ int faco_prior_logits(faco_prior* prior, const float* features, uint32_t edges, 
 /* full-batch gradient [param_count] and loss of the weighted BCE (mean over edges) */
 int faco_prior_gradients(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges,
                          float pos_weight, float* grads, float* loss);
+/* faco_prior_gradients without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference (which
+ * has no learned prior). Same kernels and results, bit for bit, as
+ * faco_prior_gradients; its gradient partials and staged inputs live in a
+ * fixed device array of the library (about 5.2 MB per GPU, held until the
+ * process ends), so it takes at most FACO_PRIOR_NOALLOC_EDGES edges
+ * (FACO_INVALID_ARGUMENT beyond) and calls no hipMalloc or hipFree. Calls
+ * that overlap on one GPU share that array and race: the caller serialises
+ * them. */
+#define FACO_PRIOR_NOALLOC_EDGES 8192u
+int faco_prior_gradients_noalloc(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges,
+                                 float pos_weight, float* grads, float* loss);
 /* `epochs` passes of Adam (lr, betas 0.9/0.999, eps 1e-8) over minibatches of
  * `batch` edges in a shuffled order; mean batch loss per epoch */
 int faco_prior_train(faco_prior* prior, const float* features, const uint8_t* labels, uint32_t edges,
