@@ -16,8 +16,7 @@ chosen through ROCR_VISIBLE_DEVICES before the engine library starts):
 
 A session holds an exclusive lock file per GPU, refuses a device that is
 not gfx950, is busy or is used by another process, and then keeps one
-engine per (dimension) panel and per (dimension, population) like the
-reference's session.
+engine per panel shape (dimension, population size).
 """
 from __future__ import annotations
 
@@ -119,6 +118,37 @@ def check_idle(gpu: dict, *, own_pid: int | None = None) -> None:
         raise RuntimeError(f"GPU {gpu['uuid']} has {gpu['vram_used']} bytes of VRAM in use")
 
 
+def decider_kind(controller) -> str:
+    """Which evaluation a session panel runs for `controller`: "faco" (None:
+    plain FACO with MNE 8, or {"uniform_mne": m}), "controller" (a
+    controller dict), "program" (an IR program dict) or "baseline" (a
+    baseline policy dict)."""
+    if controller is None or "uniform_mne" in controller:
+        return "faco"
+    if "controller_version" in controller:
+        return "controller"
+    return "program" if "opcode" in controller else "baseline"
+
+
+class _Panel:
+    """One engine of a session and the problem names registered in it."""
+
+    def __init__(self, engine):
+        self.engine = engine
+        self.names = set()
+
+    def register(self, problems, names) -> float:
+        """Registers each name not yet in the engine; returns the seconds spent."""
+        started = time.perf_counter()
+        for name in names:
+            if name in self.names:
+                continue
+            problem = problems[name]
+            self.engine.register_problem(problem.numeric_id, np.asarray(problem.coordinates, dtype=np.float64))
+            self.names.add(name)
+        return time.perf_counter() - started
+
+
 class GpuSession:
     """Resident FACO solver on one GPU (reference gpu_session.py:17-176).
 
@@ -126,42 +156,20 @@ class GpuSession:
     this engine's numeric backend ("euc2d_int32"). The GPU is chosen with
     ROCR_VISIBLE_DEVICES before the engine library starts; a process whose
     library already runs keeps its device, which must then be the requested
-    one."""
+    one. Engines are kept per (dimension, population size) and reused by
+    later panels of that shape; changing the colony count drops them."""
 
     def __init__(self, gpu_uuid, backend="euc2d_int32", *, colonies=32, lock_directory=None, sysfs="/sys",
                  idle_wait_seconds=0.0):
-        if not 1 <= colonies <= 128:
-            raise ValueError("the GPU panel must hold 1..128 colonies")
+        _check_colonies(colonies)
         self.gpu_uuid, self.backend, self.colonies = gpu_uuid, backend, colonies
         directory = Path(lock_directory) if lock_directory else Path(os.environ.get("TMPDIR", "/tmp"))
         directory.mkdir(parents=True, exist_ok=True)
         self.lease = (directory / f"worker-{gpu_uuid}.lock").open("a")
         try:
             fcntl.flock(self.lease, fcntl.LOCK_EX | fcntl.LOCK_NB)
-            # like the reference worker (worker.py:465-494), wait for idle
-            # readings while holding the lock, up to idle_wait_seconds
-            deadline = time.monotonic() + idle_wait_seconds
-            while True:
-                matches = [g for g in list_gpus(sysfs) if g["uuid"] == gpu_uuid]
-                if len(matches) != 1:
-                    raise RuntimeError(f"no GPU {gpu_uuid} in the KFD topology")
-                gpu = matches[0]
-                try:
-                    check_idle(gpu)
-                    break
-                except RuntimeError:
-                    if gpu["gfx_target_version"] != GFX950 or time.monotonic() >= deadline:
-                        raise
-                    time.sleep(0.2)
-            from . import _native
-            if _native._lib is None:
-                os.environ["ROCR_VISIBLE_DEVICES"] = gpu_uuid
-            from . import gp_faco_ext as native
-            if native.numeric_backend != backend:
-                raise ValueError("one process loads one numeric backend")
-            info = native.cuda_device_info()
-            if info.get("arch", "").split(":")[0] != gpu["gfx"]:
-                raise RuntimeError("the visible HIP device is not the requested GPU")
+            gpu = wait_until_idle(gpu_uuid, sysfs=sysfs, wait_seconds=idle_wait_seconds)
+            native, info = _load_engine_library(gpu, backend)
         except BaseException:
             self.lease.close()
             raise
@@ -169,87 +177,118 @@ class GpuSession:
         driver = _read(Path(sysfs) / "module" / "amdgpu" / "version")
         self.device = {"gpu_uuid": gpu_uuid, "model": gpu["product"] or info.get("name"), "driver": driver,
                        "build": "libfaco_hip", "gfx": gpu["gfx"], "compute_units": gpu["compute_units"]}
-        self.engines, self.registered = {}, {}
-        self.population_engines, self.population_registered = {}, {}
+        self._panels = {}
 
-    def _engine(self, n, population=1):
-        settings = self.native.FixedFacoSettings()
-        settings.ants = faco_ants(n)
-        return self.native.FacoBatchEngine(n, self.colonies, settings, population_size=population)
+    def _panel(self, n, population=1) -> _Panel:
+        shape = (n, population)
+        if shape not in self._panels:
+            settings = self.native.FixedFacoSettings()
+            settings.ants = faco_ants(n)
+            self._panels[shape] = _Panel(self.native.FacoBatchEngine(n, self.colonies, settings,
+                                                                     population_size=population))
+        return self._panels[shape]
+
+    def _tasks(self, problems, pairs):
+        """Dimension, instance keys and seeds of a complete panel of one size."""
+        sizes = {problems[name].dimension for name, _ in pairs}
+        if len(sizes) != 1:
+            raise ValueError("a panel cannot mix dimensions")
+        keys = np.fromiter((problems[name].numeric_id for name, _ in pairs), dtype=np.uint64, count=len(pairs))
+        seeds = np.fromiter((seed for _, seed in pairs), dtype=np.uint64, count=len(pairs))
+        return sizes.pop(), keys, seeds
 
     def solve(self, problems, pairs, iterations, controller, *, checkpoints=(), decision_iterations=()):
         """One panel: pairs = [(problem name, seed)] * colonies; controller
         None (plain FACO, MNE 8), {"uniform_mne": m}, a controller, a
-        program or a baseline policy dict (gpu_session.py:62-129)."""
+        program or a baseline policy dict (reference gpu_session.py:62-129).
+        Returns the engine's result and the registration / solve seconds and
+        device bytes."""
         if len(pairs) != self.colonies:
             raise ValueError("FACO needs one complete fixed panel")
-        n = problems[pairs[0][0]].dimension
-        if n not in self.engines:
-            self.engines[n] = self._engine(n)
-            self.registered[n] = set()
-        engine = self.engines[n]
-        before = time.perf_counter()
-        for name, _ in pairs:
-            problem = problems[name]
-            if problem.dimension != n:
-                raise ValueError("a panel cannot mix dimensions")
-            if name not in self.registered[n]:
-                engine.register_problem(problem.numeric_id, np.asarray(problem.coordinates, dtype=np.float64))
-                self.registered[n].add(name)
-        registration_seconds = time.perf_counter() - before
-        keys = np.asarray([problems[name].numeric_id for name, _ in pairs], dtype=np.uint64)
-        seeds = np.asarray([seed for _, seed in pairs], dtype=np.uint64)
+        n, keys, seeds = self._tasks(problems, pairs)
+        panel = self._panel(n)
+        registration_seconds = panel.register(problems, [name for name, _ in pairs])
         budget = faco_ants(n) * iterations
-        observation = {"decision_iterations": list(decision_iterations)} if decision_iterations else {}
-        before = time.perf_counter()
-        if controller is None or "uniform_mne" in controller:
-            result = engine.evaluate_faco_evaluations(keys, seeds, budget,
-                                                      8 if controller is None else controller["uniform_mne"],
-                                                      "cached", checkpoint_iterations=list(checkpoints))
-        elif "controller_version" in controller:
-            result = engine.evaluate_controller_evaluations(keys, seeds, budget, controller, **observation)
-        elif "opcode" in controller:
-            result = engine.evaluate_program_evaluations(keys, seeds, budget, controller, "cached",
-                                                         checkpoint_iterations=list(checkpoints), **observation)
+        kind = decider_kind(controller)
+        extra = {"decision_iterations": list(decision_iterations)} if decision_iterations else {}
+        if kind != "controller":
+            extra["checkpoint_iterations"] = list(checkpoints)
+        engine = panel.engine
+        started = time.perf_counter()
+        if kind == "faco":
+            mne = 8 if controller is None else controller["uniform_mne"]
+            result = engine.evaluate_faco_evaluations(keys, seeds, budget, mne, "cached", **extra)
+        elif kind == "controller":
+            result = engine.evaluate_controller_evaluations(keys, seeds, budget, controller, **extra)
+        elif kind == "program":
+            result = engine.evaluate_program_evaluations(keys, seeds, budget, controller, "cached", **extra)
         else:
-            result = engine.evaluate_baseline_evaluations(keys, seeds, budget, controller, "cached",
-                                                          checkpoint_iterations=list(checkpoints), **observation)
-        return result, {"registration_seconds": registration_seconds, "solve_seconds": time.perf_counter() - before,
+            result = engine.evaluate_baseline_evaluations(keys, seeds, budget, controller, "cached", **extra)
+        return result, {"registration_seconds": registration_seconds, "solve_seconds": time.perf_counter() - started,
                         "device_bytes": result["allocated_device_bytes"]}
 
-    def set_colonies(self, colonies):
-        if not 1 <= colonies <= 128:
-            raise ValueError("the GPU panel must hold 1..128 colonies")
-        if colonies != self.colonies:
-            self.engines.clear(); self.registered.clear()
-            self.population_engines.clear(); self.population_registered.clear()
-            self.colonies = colonies
-
     def solve_population(self, problems, pairs, iterations, programs, *, experiment_mask=0xFFFFFFFF):
-        """The panel under 1..128 programs in one engine (gpu_session.py:139-171)."""
+        """The panel under 1..128 programs in one engine (reference
+        gpu_session.py:139-171): colony d * panel + t is task t under program d."""
         if len(pairs) != self.colonies or not 1 <= len(programs) <= 128:
             raise ValueError("a population shard needs a complete panel and 1..128 programs")
-        n = problems[pairs[0][0]].dimension
-        key = n, len(programs)
-        before = time.perf_counter()
-        if key not in self.population_engines:
-            self.population_engines[key] = self._engine(n, len(programs))
-            self.population_registered[key] = set()
-        engine = self.population_engines[key]
-        for name in dict.fromkeys(name for name, _ in pairs):
-            if name not in self.population_registered[key]:
-                problem = problems[name]
-                engine.register_problem(problem.numeric_id, np.asarray(problem.coordinates, dtype=np.float64))
-                self.population_registered[key].add(name)
-        registration = time.perf_counter() - before
-        keys = np.asarray([problems[name].numeric_id for name, _ in pairs], dtype=np.uint64)
-        seeds = np.asarray([seed for _, seed in pairs], dtype=np.uint64)
-        before = time.perf_counter()
-        result = engine.evaluate_population_evaluations(keys, seeds, faco_ants(n) * iterations, programs,
-                                                        experiment_mask)
-        return result, {"registration_seconds": registration, "solve_seconds": time.perf_counter() - before}
+        n, keys, seeds = self._tasks(problems, pairs)
+        started = time.perf_counter()
+        panel = self._panel(n, len(programs))
+        panel.register(problems, [name for name, _ in pairs])
+        registration_seconds = time.perf_counter() - started
+        started = time.perf_counter()
+        result = panel.engine.evaluate_population_evaluations(keys, seeds, faco_ants(n) * iterations, programs,
+                                                              experiment_mask)
+        return result, {"registration_seconds": registration_seconds, "solve_seconds": time.perf_counter() - started}
+
+    def set_colonies(self, colonies):
+        """A new panel size: the engines of the old size are dropped."""
+        _check_colonies(colonies)
+        if colonies != self.colonies:
+            self._panels = {}
+            self.colonies = colonies
 
     def close(self):
-        self.engines.clear()
-        self.population_engines.clear()
+        self._panels = {}
         self.lease.close()
+
+
+def _check_colonies(colonies):
+    if not 1 <= colonies <= 128:
+        raise ValueError("the GPU panel must hold 1..128 colonies")
+
+
+def wait_until_idle(gpu_uuid, *, sysfs="/sys", wait_seconds=0.0, poll_seconds=0.2) -> dict:
+    """The device record of gpu_uuid once check_idle passes, polling for up to
+    wait_seconds (the reference worker waits for idle readings while holding
+    its lock, worker.py:465-494). A device that is not gfx950 fails at once."""
+    deadline = time.monotonic() + wait_seconds
+    while True:
+        matches = [g for g in list_gpus(sysfs) if g["uuid"] == gpu_uuid]
+        if len(matches) != 1:
+            raise RuntimeError(f"no GPU {gpu_uuid} in the KFD topology")
+        gpu = matches[0]
+        try:
+            check_idle(gpu)
+            return gpu
+        except RuntimeError:
+            if gpu["gfx_target_version"] != GFX950 or time.monotonic() >= deadline:
+                raise
+        time.sleep(poll_seconds)
+
+
+def _load_engine_library(gpu, backend):
+    """Binds the process to `gpu` (ROCR_VISIBLE_DEVICES, when the engine
+    library has not started yet) and loads it; checks the numeric backend and
+    that the visible HIP device is that GPU."""
+    from . import _native
+    if _native._lib is None:
+        os.environ["ROCR_VISIBLE_DEVICES"] = gpu["uuid"]
+    from . import gp_faco_ext as native
+    if native.numeric_backend != backend:
+        raise ValueError("one process loads one numeric backend")
+    info = native.cuda_device_info()
+    if info.get("arch", "").split(":")[0] != gpu["gfx"]:
+        raise RuntimeError("the visible HIP device is not the requested GPU")
+    return native, info
