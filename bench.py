@@ -3,16 +3,22 @@
 Workload: 32 uniform random Euclidean TSP1000 instances (integer coordinates
 in [0,10000]^2, EUC_2D) x 8 seeds = 256 colonies, 128 ants per colony, MNE 8,
 16/64/20 candidate widths, 2-opt checklist local search with the 100000
-evaluation limit, 100 iterations from the prepared initial tour. A step is one
-FACO iteration of every colony: construction + local search for all 32768
-ants, then selection and the pheromone update. The same run with one seed
-(32 colonies, the batch shape of the reference's own TSP1000 profile row) is
-reported beside it.
+evaluation limit: a 100-iteration run from the prepared initial tour (the
+length of the reference's TSP1000 profile row). A step is one FACO iteration
+of every colony: construction + local search for all 32768 ants, then
+selection and the pheromone update. The timed steps are the LAST --steps
+iterations of that run (the steady state: with --steps 20, iterations
+81-100); the iterations before them are the warm-up (at least --warmup),
+run through the per-batch kernels, which give bit-identical colony states,
+so that the production kernel's only dispatch is the timed window. The same
+window with one seed (32 colonies, the batch shape of the reference's own
+TSP1000 profile row) is reported beside it.
 Inputs (instances, candidate rows, pheromone) are resident in HBM before the
-timed region; the timed region is the iteration loop between two stream
-synchronisations: the persistent kernel faco_colony_run, one workgroup per
-colony running its ants and its colony update, up to 1024 iterations per
-launch.
+timed region; the timed region is the continuation's iteration loop between
+two stream synchronisations: the persistent kernel faco_colony_run, one
+workgroup per colony running its ants and its colony update, up to 1024
+iterations per launch, with HIP events around each launch on the engine's
+stream.
 
 Prints one JSON line with the metric, the roofline of that kernel (HIP-event
 time per iteration, algorithmic bytes per iteration) and the CPU baseline
@@ -48,40 +54,100 @@ def algorithmic_bytes_per_tour(n: int, steps_per_tour: float, evals_per_tour: fl
     return 8.0 * n + 192.0 * steps_per_tour + 16.0 * evals_per_tour + 32.0 * n * pw / ants
 
 
-def cpu_baseline(xys, keys, seeds, ants, iterations, budget_seconds=20.0):
+def _cpu_window(xy, keys, seeds, ants, warm, steps, threads, budget_seconds):
+    """Oracle FACO over colonies c = (instance c % I, seed c // I) for warm +
+    steps iterations, timing the last `steps` only; colonies are cut so the
+    timed window takes about `budget_seconds`."""
+    from oracle import oracle
+    settings = oracle.default_settings(ants=ants)
+    instances = xy.shape[0]
+    probe_colonies = min(len(seeds), max(threads, 1))
+    probe = oracle.run_faco(xy, keys, seeds[:probe_colonies], settings, min(steps, 5), 8, threads=threads)
+    per_colony = probe["search_seconds"] / probe_colonies * steps / min(steps, 5)
+    colonies = int(max(1, min(len(seeds), budget_seconds / max(per_colony, 1e-9))))
+    if colonies >= instances:
+        colonies -= colonies % instances
+    run = oracle.run_faco(xy[:min(colonies, instances)], keys[:min(colonies, instances)], seeds[:colonies], settings,
+                          warm + steps, 8, threads=threads, timed_from=warm)
+    tours = colonies * ants * steps
+    return tours / run["search_seconds"], colonies, tours, run["search_seconds"]
+
+
+def cpu_baseline(xys, keys, seeds, ants, warm, steps, budget_seconds=10.0):
     """The oracle's C++ FACO (OpenMP over colony x ant) on the host cores, on
-    the GPU workload's own instances and iteration count; colonies are cut
-    to keep the sample near `budget_seconds`."""
+    the GPU workload's own colonies (colony c: instance c % instances, seed
+    seeds[c]), timed on the same window of the run (iterations warm+1 ..
+    warm+steps); a bounded sample of the colonies, and the same window on one
+    core."""
     from oracle import oracle
     threads = oracle.max_threads()
-    settings = oracle.default_settings(ants=ants)
     xy = np.stack(xys).astype(np.int32)
-    probe_colonies = min(len(seeds), threads)
-    probe = oracle.run_faco(xy[:probe_colonies], keys[:probe_colonies], seeds[:probe_colonies], settings,
-                            min(iterations, 10), 8, threads=threads)
-    per_colony = probe["search_seconds"] / probe_colonies * iterations / min(iterations, 10)
-    colonies = int(max(1, min(len(seeds), budget_seconds / max(per_colony, 1e-6))))
-    run = oracle.run_faco(xy[:colonies], keys[:colonies], seeds[:colonies], settings, iterations, 8, threads=threads)
-    tours = colonies * ants * iterations
-    return {"value": tours / run["search_seconds"], "unit": "ant-tours/s", "cores": threads, "kind": "port",
-            "sample": f"oracle C++ FACO (OpenMP over colony x ant, -O3), the same TSP1000 instances, {colonies} "
-                      f"colonies x {ants} ants x {iterations} iterations = {tours} tours in "
-                      f"{run['search_seconds']:.2f} s of search; host nproc={os.cpu_count()}"}
+    keys = np.asarray(keys, dtype=np.uint64)
+    seeds = np.asarray(seeds, dtype=np.uint64)
+    value, colonies, tours, secs = _cpu_window(xy, keys, seeds, ants, warm, steps, threads, budget_seconds)
+    one, colonies1, tours1, secs1 = _cpu_window(xy, keys, seeds, ants, warm, steps, 1, budget_seconds / 5)
+    return {"value": value, "unit": "ant-tours/s", "cores": threads, "kind": "port",
+            "sample": f"oracle C++ FACO (OpenMP over colony x ant, -O3), the same TSP1000 instances and run: "
+                      f"{colonies} colonies x {ants} ants, iterations {warm + 1}-{warm + steps} timed "
+                      f"({tours} tours in {secs:.2f} s of search) after {warm} untimed; host nproc={os.cpu_count()}",
+            "single_core": {"value": one, "cores": 1,
+                            "sample": f"{colonies1} colonies, the same window: {tours1} tours in {secs1:.2f} s"}}
 
 
-def measured_traffic(n: int, colonies: int, ants: int):
-    """Calibrated HBM bytes per iteration of faco_colony_run on this workload
-    from the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
-    dispatch over its iterations, each scaled by what the counter reads for
-    the calibration copy kernel of known size), or None when no summary
-    matches the shape."""
-    path = ROOT / "profiles" / "faco_colony_run_tsp1000_traffic.json"
+def config1_cpu_path(ext) -> dict:
+    """The CPU-runnable configuration: plain FACO on one uniform TSP100
+    (seed 0), 32 ants, 1000 iterations, 2-opt + Or-opt, 20-nearest
+    candidates, run by the oracle's C++ FACO on one host core; the same run
+    on the GPU (FixedFacoGpu) must end at the same tour."""
+    from gplsaco_amd.instances import uniform_instance
+    from oracle import oracle
+    xy = uniform_instance(100, seed=0, instance=0)
+    settings = ext.FixedFacoSettings()
+    settings.ants, settings.primary_width, settings.ls_width = 32, 20, 20
+    fields = ("primary_width", "backup_width", "ls_width", "beta", "retention", "p_best", "epoch_source_probability",
+              "ls_evaluation_limit", "initial_ls_evaluation_limit")
+    osettings = oracle.default_settings(ants=32, **{k: getattr(settings, k) for k in fields})
+    cpu = oracle.run_faco(xy[None].astype(np.int32), [0], [0], osettings, 1000, 8, threads=1, or_opt=True)
+    gpu = ext.FixedFacoGpu(xy, 0, settings)
+    gpu.set_local_search("2opt+oropt")
+    started = time.perf_counter()
+    got = gpu.run_iterations(0, 1000, 8)
+    gpu_seconds = time.perf_counter() - started
+    return {"workload": "plain FACO, uniform TSP100 (seed 0), 32 ants, 1000 iterations, 2-opt + Or-opt, 20-NN",
+            "cpu_one_core_ant_tours_per_s": 32000 / cpu["search_seconds"], "cpu_seconds": cpu["search_seconds"],
+            "final_tour_length": int(cpu["costs"][0]), "gpu_final_tour_length": int(got["cost"]),
+            "gpu_equals_cpu": got["tour"] == cpu["tours"][0].tolist(),
+            "gpu_ant_tours_per_s": 32000 / gpu_seconds, "gpu_seconds": gpu_seconds}
+
+
+def committed_counters(n: int, colonies: int, ants: int, first: int, last: int):
+    """The committed rocprofv3 --pmc summary of faco_colony_run on this
+    workload and window (iterations first..last), or None when none matches:
+    calibrated HBM bytes per iteration (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for the calibration copy
+    kernel of known size) and the issue counters."""
+    path = ROOT / "profiles" / "faco_colony_run_tsp1000_counters.json"
     if not path.exists():
         return None
     data = json.loads(path.read_text())
-    if (data.get("n"), data.get("colonies"), data.get("ants")) != (n, colonies, ants):
+    if (data.get("n"), data.get("colonies"), data.get("ants"), data.get("first_iteration"),
+            data.get("last_iteration")) != (n, colonies, ants, first, last):
         return None
-    return data["bytes_per_iteration"]
+    return data
+
+
+def steady_window(engine, keys, seeds, ants, warm, steps, **timing):
+    """`warm` iterations from the prepared tour through the per-batch kernels
+    (each colony split over two workgroups: results bit-identical to the
+    persistent kernel), then the continuation of `steps` iterations in the
+    engine's automatic (production) shape; returns the continuation."""
+    if not warm:
+        return engine.evaluate_faco_evaluations(keys, seeds, ants * steps, **timing)
+    waves = engine.launch_shape(keys.size)["waves_per_block"]
+    engine.set_waves_per_block(waves, 2)
+    engine.evaluate_faco_evaluations(keys, seeds, ants * warm)
+    engine.set_waves_per_block(0)
+    return engine.evaluate_faco_evaluations(keys, seeds, ants * steps, continue_run=True, **timing)
 
 
 def gp_population_rate(ext, steps: int) -> dict:
@@ -309,8 +375,11 @@ def learned_prior_legs(ext, budgets) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20, help="timed iterations: the last ones of the run")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed iterations before them (at least)")
+    ap.add_argument("--run-length", type=int, default=100,
+                    help="iterations of the run whose last --steps are timed (warm-up = run length - steps, "
+                         "at least --warmup)")
     ap.add_argument("--n", type=int, default=1000)
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--seeds", type=int, default=8)
@@ -351,51 +420,47 @@ def main() -> int:
     task_seeds = np.array([c // instances for c in range(colonies)], dtype=np.uint64)
     print(f"# prepared {instances} x TSP{n} in {prep_seconds:.2f} s; shape {engine.launch_shape()}",
           file=sys.stderr, flush=True)
-    if args.warmup:
-        engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.warmup)
-    timed = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=True)
-    print(f"# timed {args.steps} steps in {timed['search_seconds']:.3f} s", file=sys.stderr, flush=True)
+    warm = max(args.warmup, args.run_length - args.steps)
+    first, last = warm + 1, warm + args.steps
+    # The timed window: the continuation of the warm-up, HIP events around
+    # every launch of the production kernel (the persistent faco_colony_run:
+    # ant search and colony update, up to 1024 iterations per launch) on the
+    # engine's stream; its wall clock is the bench value.
+    timed = steady_window(engine, task_keys, task_seeds, ants, warm, args.steps, time_kernels=True)
+    print(f"# timed iterations {first}-{last} in {timed['search_seconds']:.3f} s", file=sys.stderr, flush=True)
     tours = timed["total_tour_evaluations"]
     seconds = timed["search_seconds"]
     value = tours / seconds
     steps_per_tour = timed["completed_construction_steps"] / tours
     evals_per_tour = timed["completed_ls_evaluations"] / tours
-
-    # Same iterations again with HIP events around every launch of the
-    # production kernel (the persistent faco_colony_run: ant search and
-    # colony update, up to 1024 iterations per launch) on the engine's stream.
-    prof = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=False,
-                                            time_kernels=True)
-    ptours = prof["total_tour_evaluations"]
-    iteration_ms = (prof["kernel_milliseconds"]["construction_and_search"]
-                    + prof["kernel_milliseconds"]["reduction_and_pheromone"]) / args.steps
     launches = -(-args.steps // 1024)
-    # A few iterations of the clock-instrumented kernel variant: per-ant phases.
+    iteration_ms = (timed["kernel_milliseconds"]["construction_and_search"]
+                    + timed["kernel_milliseconds"]["reduction_and_pheromone"]) / args.steps
+    # The last iterations of the window again with the clock-instrumented
+    # kernel variant (per-batch launches): per-ant phases.
     phase_steps = min(args.steps, 10)
-    phases = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * phase_steps, use_graph=False,
-                                              profile_kernels=True)
-    bytes_per_tour = algorithmic_bytes_per_tour(n, prof["completed_construction_steps"] / ptours,
-                                                prof["completed_ls_evaluations"] / ptours,
-                                                engine.settings.primary_width, ants)
+    phases = steady_window(engine, task_keys, task_seeds, ants, last - phase_steps, phase_steps, use_graph=False,
+                           profile_kernels=True)
+    bytes_per_tour = algorithmic_bytes_per_tour(n, steps_per_tour, evals_per_tour, engine.settings.primary_width, ants)
     tours_per_iteration = colonies * ants
     achieved = bytes_per_tour * tours_per_iteration / (iteration_ms * 1e-3) / 1e9
     mean_cost = float(np.mean([item["cost"] for item in timed["items"]]))
+    counters = committed_counters(n, colonies, ants, first, last)
 
-    # The same workload with Or-opt added to the local search.
+    # The same window with Or-opt added to the local search.
     engine.set_local_search("2opt+oropt")
-    oropt = engine.evaluate_faco_evaluations(task_keys, task_seeds, ants * args.steps, use_graph=True)
+    oropt = steady_window(engine, task_keys, task_seeds, ants, warm, args.steps)
     engine.set_local_search("2opt")
     oropt_value = oropt["total_tour_evaluations"] / oropt["search_seconds"]
     oropt_cost = float(np.mean([item["cost"] for item in oropt["items"]]))
 
-    # The reference's profile shape: 32 colonies (one seed per instance).
+    # The reference's profile shape: 32 colonies (one seed per instance),
+    # the same window.
     small = ext.FacoBatchEngine(n, instances, settings)
     for k, xy in zip(keys, xys):
         small.register_problem(k, xy)
     small_keys, small_seeds = np.array(keys, dtype=np.uint64), np.zeros(instances, dtype=np.uint64)
-    if args.warmup:   # as for the main run: graph capture and first-touch costs stay out of the timing
-        small.evaluate_faco_evaluations(small_keys, small_seeds, ants * max(args.warmup, 16))
-    small_run = small.evaluate_faco_evaluations(small_keys, small_seeds, ants * args.steps)
+    small_run = steady_window(small, small_keys, small_seeds, ants, warm, args.steps)
     small_value = small_run["total_tour_evaluations"] / small_run["search_seconds"]
 
     line = {
@@ -414,8 +479,9 @@ def main() -> int:
         "config": {
             "workload": f"plain FACO (no GP, fixed MNE 8), {instances} uniform random EUC_2D TSP{n} instances x "
                         f"{args.seeds} seeds = {colonies} colonies, {ants} ants each, 16/64/20 candidates, "
-                        f"checklist 2-opt, {args.steps} iterations",
+                        f"checklist 2-opt; iterations {first}-{last} of a {last}-iteration run timed",
             "n": n, "colonies": colonies, "ants": ants, "mne": 8, "iterations": args.steps,
+            "timed_iterations": [first, last], "warmup_iterations": warm,
             "value_at_32_colonies": small_value,
             "value_with_or_opt": oropt_value, "mean_final_tour_length_with_or_opt": oropt_cost,
             "ls_evaluations_per_tour_with_or_opt": oropt["completed_ls_evaluations"] / oropt["total_tour_evaluations"],
@@ -429,7 +495,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(n, colonies, ants),
+            "traffic": counters["bytes_per_iteration"] if counters else None,
+            "limiter": "instruction issue (VALU and SALU), not HBM: see issue",
+            "issue": counters["issue"] if counters else None,
             "per": "iteration (launches run up to 1024 iterations each; achieved and traffic are per iteration)",
             "bytes_per_tour": bytes_per_tour,
             "tours_per_iteration": tours_per_iteration,
@@ -455,8 +523,10 @@ def main() -> int:
     if args.prior_budgets:
         line["config"]["learned_prior"] = learned_prior_legs(ext, [int(b) for b in args.prior_budgets.split(",")])
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(xys, keys, list(range(instances)), ants, args.steps)
+        line["cpu_baseline"] = cpu_baseline(xys, keys, task_seeds, ants, warm, args.steps)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+        line["gpu_over_one_core"] = value / line["cpu_baseline"]["single_core"]["value"]
+        line["config"]["config1_cpu_path"] = config1_cpu_path(ext)
     if args.copy_mib:
         copy_ms = ext.diagnostic_copy(args.copy_mib << 20, 4)
         line["calibration_copy"] = {"bytes_read": args.copy_mib << 20, "bytes_written": args.copy_mib << 20,

@@ -196,6 +196,12 @@ struct Engine {
     };
     std::unique_ptr<ControlBuffers> ctl;
     bool resident_control = false;
+    // The last plain evaluation, which FACO_RUN_CONTINUE resumes: its colony
+    // keys and MNE target (empty: nothing to resume). Anything that changes
+    // what the next batch would compute from the colony state forgets it.
+    std::vector<uint64_t> continuable;
+    uint32_t continuable_mne = 0;
+    void forget_run() { continuable.clear(); continuable_mne = 0; }
     // graph cache
     hipGraphExec_t graph = nullptr;
     uint32_t graph_colonies = 0, graph_iterations = 0;
@@ -621,7 +627,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
               uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
               uint64_t limit, bool any_count = false, const faco_checkpoints* checkpoints = nullptr) {
     const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
-    const bool events = (flags & FACO_RUN_EVENTS) != 0;
+    const bool events = (flags & FACO_RUN_EVENTS) != 0, resume = (flags & FACO_RUN_CONTINUE) != 0;
     double kernel_ms[FACO_STAGE_COUNT] = {0};
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
@@ -634,11 +640,19 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     e.upload_geometry(instance_keys, tasks);
     std::vector<uint64_t> keys(tasks);
     for (uint32_t c = 0; c < tasks; ++c) keys[c] = colony_key(instance_keys[c], seeds[c]);
+    if (resume)
+        require(!e.mixed && !(checkpoints && checkpoints->count) && keys == e.continuable && mne == e.continuable_mne,
+                "a continuation resumes the engine's last plain evaluation of the same tasks and MNE target "
+                "(fixed-size engines, no checkpoints)");
+    e.forget_run();
     e.keys.upload(keys.data(), tasks, e.stream);
     struct Inactive { Engine& e; ~Inactive() { e.shapes_active = false; } } inactive{e};
     if (e.mixed) batches = e.prepare_shapes(instance_keys, tasks, limit);
     Checkpoints marks(checkpoints, batches, static_cast<size_t>(tasks) * e.n);
-    e.reset(tasks, mne);
+    // a continuation keeps every colony's state and reports only its own work
+    std::vector<ColonyState> before(resume ? tasks : 0);
+    if (resume) e.state.download(before.data(), tasks, e.stream);
+    else e.reset(tasks, mne);
     marks.record(e.global.data(), 0, e.stream);
     checked(hipStreamSynchronize(e.stream), "initialisation");
     const auto search_started = Clock::now();
@@ -667,13 +681,16 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     marks.read_back(e.stream, e.n, [&](uint32_t c) { return e.problem(instance_keys[c]).n; });
     uint64_t steps = 0, evals = 0;
     for (uint32_t c = 0; c < tasks; ++c) {
+        const uint64_t own_steps = states[c].work_construction - (resume ? before[c].work_construction : 0);
+        const uint64_t own_evals = states[c].work_ls - (resume ? before[c].work_ls : 0);
         if (costs) costs[c] = states[c].global_cost;
-        if (work) { work[2 * c] = states[c].work_construction; work[2 * c + 1] = states[c].work_ls; }
-        steps += states[c].work_construction; evals += states[c].work_ls;
+        if (work) { work[2 * c] = own_steps; work[2 * c + 1] = own_evals; }
+        steps += own_steps; evals += own_evals;
     }
     fill_result(result, e, tasks, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
                 seconds_since(started), search, gpu_ms);
     if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+    if (!e.mixed) { e.continuable = std::move(keys); e.continuable_mne = mne; }
 }
 
 // validate_baseline (cpp/include/gp_faco/baseline_policy.hpp:36-56).
@@ -804,6 +821,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "one evaluation per engine at a time");
     require(current_device() == e.device, "engine must be used on the device it was created on");
+    e.forget_run();
     require(spec.mode >= FACO_CONTROL_PROGRAM && spec.mode <= FACO_CONTROL_FACTORIAL, "unknown control mode");
     require(!e.hard || !e.local_search, "a Hard-mode engine uses the plain 2-opt");
     require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
@@ -1131,6 +1149,7 @@ double register_problems_device(Engine& e, const uint64_t* keys, const double* x
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "registration and evaluation of one engine must not overlap");
+    e.forget_run();   // the preparation kernels reuse the ant buffers
     require(current_device() == e.device, "engine must be used on the device it was created on");
     std::map<uint32_t, std::vector<std::pair<uint64_t, Prepared>>> groups;   // candidate count -> instances
     size_t offset = 0;
@@ -1517,6 +1536,7 @@ int faco_engine_set_local_search(faco_engine* engine, uint32_t mode) {
         require(mode <= 1, "local search mode must be 0 (2-opt) or 1 (2-opt + Or-opt)");
         require(mode == 0 || !engine->impl->hard, "a Hard-mode engine uses the plain 2-opt");
         engine->impl->local_search = mode;
+        engine->impl->forget_run();
         engine->impl->drop_graph();
     });
 }
@@ -1682,6 +1702,7 @@ int faco_engine_set_edge_prior(faco_engine* engine, uint64_t key, const float* m
             it->second.prior.clear();
         }
         e.resident.clear();
+        e.forget_run();
     });
 }
 
@@ -1893,6 +1914,7 @@ int faco_fixed_run(faco_fixed* fixed, uint64_t seed, uint32_t batches, uint32_t 
         } else {
             std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
             require(lock.owns_lock(), "one evaluation per engine at a time");
+            e.forget_run();
             const uint32_t n = e.n, ants = e.config.ants, pw = e.config.primary_width;
             e.upload_geometry(&keyv, 1);
             const uint64_t key = colony_key(keyv, seed);

@@ -627,7 +627,14 @@ class FacoBatchEngine:
 
     def evaluate_faco_evaluations(self, instance_keys, seeds, evaluation_limit_per_colony, mne=8,
                                   preparation_mode="cached", *, checkpoint_iterations=(), profile_events_only=False,
-                                  use_graph=True, profile_kernels=False, time_kernels=False) -> dict:
+                                  use_graph=True, profile_kernels=False, time_kernels=False,
+                                  continue_run=False) -> dict:
+        """Plain FACO for evaluation_limit_per_colony tours per colony
+        (reference bindings.cpp:701-720). continue_run (an engine extension):
+        resume the previous plain evaluation of the same tasks and MNE target
+        from its last batch instead of resetting the colonies, so that a run
+        and its continuation equal one longer run; work and evaluations then
+        count the continuation only."""
         keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
         if keys.size != seeds.size:
             raise ValueError("FACO count entry needs complete task arrays")
@@ -642,7 +649,8 @@ class FacoBatchEngine:
         work = np.zeros((tasks, 2), dtype=np.uint64)
         result = Result()
         started = time.perf_counter()
-        flags = (1 if use_graph else 0) | (2 if profile_kernels else 0) | (4 if time_kernels or profile_events_only else 0)
+        flags = (1 if use_graph else 0) | (2 if profile_kernels else 0) | (4 if time_kernels or profile_events_only else 0) \
+            | (8 if continue_run else 0)
         check(lib().faco_engine_evaluate_faco(self._handle, keys, seeds, tasks, limit, mne, flags,
                                               ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1),
                                               request.argument()))

@@ -320,6 +320,14 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_
                                summed over iterations */
 #define FACO_RUN_EVENTS 4u  /* eager launches of the production kernels with HIP events around
                                each: kernel_milliseconds[0] and [1] as for FACO_RUN_PROFILE */
+#define FACO_RUN_CONTINUE 8u /* resume the engine's last plain evaluation (same tasks and MNE
+                               target, fixed-size engine, no checkpoints) from its last batch
+                               instead of resetting the colonies: a run of a batches followed by
+                               a continuation of b batches equals one run of a + b batches bit
+                               for bit. Work counters and evaluations report the continuation
+                               only. No counterpart in the reference (whose plain entry point
+                               always starts from the prepared tour); a steady-state window of a
+                               long run is measured this way */
 
 /* Curve checkpoints (reference cuda/batch_engine.cu:635-653, the
  * checkpoint_iterations argument of bindings.cpp:701-757,818-853): the

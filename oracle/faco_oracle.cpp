@@ -1337,7 +1337,8 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
                     double* elapsed_search_seconds, u32 local_search,
                     const float* multipliers /* [instances][n * pw] or null */,
                     const u32* graph_offsets /* [instances][n + 1] or null: Hard mode */,
-                    const u32* graph_neighbors /* [instances][neighbor_stride] */, u32 neighbor_stride) {
+                    const u32* graph_neighbors /* [instances][neighbor_stride] */, u32 neighbor_stride,
+                    u32 timed_from /* elapsed_search_seconds covers batches timed_from.. only */) {
     return guarded([&] {
         if (!instances || !colonies) throw std::invalid_argument("empty panel");
         if (graph_offsets && local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
@@ -1371,9 +1372,12 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
         std::vector<std::vector<AntResult>> results(colonies, std::vector<AntResult>(s.ants));
 #ifdef _OPENMP
         if (threads > 0) omp_set_num_threads(threads);
-        const double started = omp_get_wtime();
+        double started = omp_get_wtime();
 #endif
         for (u32 batch = 0; batch < batches; ++batch) {
+#ifdef _OPENMP
+            if (batch == timed_from) started = omp_get_wtime();
+#endif
             const long long total = static_cast<long long>(colonies) * s.ants;
 #pragma omp parallel for schedule(dynamic, 1)
             for (long long i = 0; i < total; ++i) {

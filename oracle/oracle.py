@@ -191,7 +191,7 @@ def lib():
                                       ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p,
-                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
         L.oracle_prior_features.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, f32p]
         L.oracle_escape_slot_view.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
@@ -426,8 +426,9 @@ class alternate_stream:
 
 
 def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False,
-             or_opt=False, multipliers=None, graphs=None):
+             or_opt=False, multipliers=None, graphs=None, timed_from=0):
     """Fixed-iteration FACO over colonies (colony i uses instance i % instances).
+    search_seconds times batches timed_from..batches-1 only (a window of the run).
     `multipliers` [instances, n, pw] (float32): learned heuristic multipliers,
     eta' = eta * multiplier in FP64. `graphs`: one undirected CSR
     (offsets, neighbors) per instance (csr()) for the Hard mode, in which
@@ -467,7 +468,8 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
                              ctypes.byref(settings), batches, mne_target, threads, tours.reshape(-1), costs,
                              work.reshape(-1), ptr(trace_parent), ptr(trace_trails), ptr(trace_costs),
                              ctypes.byref(elapsed), int(bool(or_opt)),
-                             ptr(None if multipliers is None else mult), ptr(g_offsets), ptr(g_neighbors), stride), L)
+                             ptr(None if multipliers is None else mult), ptr(g_offsets), ptr(g_neighbors), stride,
+                             int(timed_from)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value}
     if trace:
         out.update(trace_parent=trace_parent, trace_trails=trace_trails, trace_costs=trace_costs)
