@@ -138,14 +138,14 @@ def committed_counters(n: int, colonies: int, ants: int, first: int, last: int):
 
 def steady_window(engine, keys, seeds, ants, warm, steps, **timing):
     """`warm` iterations from the prepared tour through the per-batch kernels
-    (each colony split over two workgroups: results bit-identical to the
-    persistent kernel), then the continuation of `steps` iterations in the
+    (each colony split over two workgroups, eager launches: results
+    bit-identical to the persistent kernel), then the continuation of `steps` iterations in the
     engine's automatic (production) shape; returns the continuation."""
     if not warm:
         return engine.evaluate_faco_evaluations(keys, seeds, ants * steps, **timing)
     waves = engine.launch_shape(keys.size)["waves_per_block"]
     engine.set_waves_per_block(waves, 2)
-    engine.evaluate_faco_evaluations(keys, seeds, ants * warm)
+    engine.evaluate_faco_evaluations(keys, seeds, ants * warm, use_graph=False)
     engine.set_waves_per_block(0)
     return engine.evaluate_faco_evaluations(keys, seeds, ants * steps, continue_run=True, **timing)
 

@@ -87,3 +87,35 @@ def test_persistent_mixed_sizes_stop_at_their_own_budget(ext, oracle_mod):
         want = oracle_mod.run_faco(xys[t][None].astype(np.int32), [200 + t], [int(seeds[c])],
                                    _oracle_settings(oracle_mod, ext, ants), limit // ants, 8)
         assert persistent["items"][c]["tour"] == want["tours"][0].tolist(), c
+
+
+@pytest.mark.parametrize("split", [1, 2])
+def test_continuation_equals_one_longer_run(ext, split):
+    """A run of a batches continued for b more (FACO_RUN_CONTINUE) ends
+    exactly where one run of a + b batches ends; its work counters are the
+    continuation's own; only the previous plain run of the same tasks can be
+    continued."""
+    n, instances, seeds_per = 220, 3, 4
+    colonies = instances * seeds_per
+    xys = [uniform_instance(n, seed=9, instance=i) for i in range(instances)]
+    engine = ext.FacoBatchEngine(n, colonies, ext.FixedFacoSettings())
+    for i, xy in enumerate(xys):
+        engine.register_problem(500 + i, xy)
+    ants = engine.settings.ants
+    keys = np.array([500 + c % instances for c in range(colonies)], dtype=np.uint64)
+    seeds = np.array([c // instances for c in range(colonies)], dtype=np.uint64)
+    engine.set_waves_per_block(engine.launch_shape(colonies)["waves_per_block"], split)
+    announce("continuation", split)
+    whole = engine.evaluate_faco_evaluations(keys, seeds, ants * 7)
+    head = engine.evaluate_faco_evaluations(keys, seeds, ants * 3)
+    tail = engine.evaluate_faco_evaluations(keys, seeds, ants * 4, continue_run=True)
+    assert [i["tour"] for i in tail["items"]] == [i["tour"] for i in whole["items"]]
+    assert [i["cost"] for i in tail["items"]] == [i["cost"] for i in whole["items"]]
+    np.testing.assert_array_equal(head["colony_work"] + tail["colony_work"], whole["colony_work"])
+    assert tail["total_tour_evaluations"] == colonies * ants * 4
+    with pytest.raises(ValueError):   # other seeds: not the previous run
+        engine.evaluate_faco_evaluations(keys, seeds + 1, ants, continue_run=True)
+    engine.evaluate_faco_evaluations(keys, seeds, ants)
+    engine.set_local_search("2opt+oropt")   # a changed search forgets the run
+    with pytest.raises(ValueError):
+        engine.evaluate_faco_evaluations(keys, seeds, ants, continue_run=True)
