@@ -134,6 +134,56 @@ class ControlState(ctypes.Structure):
     ]
 
 
+class DeadlineSpec(ctypes.Structure):
+    """faco_deadline_spec: a time-budget evaluation."""
+
+    _fields_ = [
+        ("budget_seconds", ctypes.c_double),
+        ("mne_target", ctypes.c_uint32),
+        ("preparation_mode", ctypes.c_uint32),
+        ("program", ctypes.c_void_p),
+        ("experiment_mask", ctypes.c_uint32),
+        ("fixed_batches", ctypes.c_uint32),
+        ("fixed_elapsed_ratio", ctypes.c_double),
+        ("profile", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
+class DeadlineResult(ctypes.Structure):
+    _fields_ = [
+        ("budget_seconds", ctypes.c_double),
+        ("elapsed_seconds", ctypes.c_double),
+        ("actual_seconds", ctypes.c_double),
+        ("charged_seconds", ctypes.c_double),
+        ("last_batch_completed_seconds", ctypes.c_double),
+        ("overrun_seconds", ctypes.c_double),
+        ("launched_batches", ctypes.c_uint64),
+        ("completed_batches", ctypes.c_uint64),
+        ("discarded_batches", ctypes.c_uint64),
+        ("completed_construction_steps", ctypes.c_uint64),
+        ("completed_ls_evaluations", ctypes.c_uint64),
+        ("allocated_device_bytes", ctypes.c_uint64),
+        ("preparation_completed", ctypes.c_uint32),
+        ("profiled_batches", ctypes.c_uint32),
+        ("setup_seconds", ctypes.c_double),
+        ("upload_seconds", ctypes.c_double),
+        ("initialization_gpu_milliseconds", ctypes.c_double),
+    ]
+
+
+class BatchProfile(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_uint32),
+        ("committed", ctypes.c_uint32),
+        ("wall_seconds", ctypes.c_double),
+        ("download_seconds", ctypes.c_double),
+        ("verification_seconds", ctypes.c_double),
+        ("collection_seconds", ctypes.c_double),
+        ("gpu_milliseconds", ctypes.c_double * 9),
+    ]
+
+
 CONTROL_PROGRAM, CONTROL_CONTROLLER, CONTROL_BASELINE, CONTROL_FACTORIAL = 1, 2, 3, 4
 
 
@@ -278,6 +328,10 @@ SIGNATURES = {
     "faco_engine_edge_features": (ctypes.c_int, [vp, ctypes.c_uint64, f32p]),
     "faco_engine_set_edge_prior": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
     "faco_engine_apply_edge_prior": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, ctypes.c_double]),
+    "faco_engine_evaluate_deadline": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.POINTER(DeadlineSpec),
+                                                     ctypes.POINTER(DeadlineResult), u32p, f64p, u8p, f64p, vp, vp,
+                                                     ctypes.c_uint32, vp]),
+    "faco_engine_set_preparation_charges": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
     "faco_engine_evaluate_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64,
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,

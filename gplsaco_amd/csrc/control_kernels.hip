@@ -363,8 +363,9 @@ __global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p
     // (4) the 12 features of the 32 actions
     if (tid < 32) {
         const uint32_t a = tid, mode = a / 16, region = (a / 4) % 4, level = a % 4;
-        const double progress = static_cast<double>(static_cast<uint64_t>(s.batch) * ants) /
-                                static_cast<double>(p.evaluation_limit);
+        const double progress = p.fixed_progress >= 0
+            ? p.fixed_progress
+            : static_cast<double>(static_cast<uint64_t>(s.batch) * ants) / static_cast<double>(p.evaluation_limit);
         const float* r = s_regional[mode * 4 + region];
         const float values[12] = {static_cast<float>(unit_clip(progress)),
             static_cast<float>(c.stagnant < 32 ? c.stagnant / 32.0 : 1), c.return_rate, c.ls_work,

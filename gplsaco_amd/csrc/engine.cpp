@@ -201,6 +201,9 @@ struct Engine {
     // what the next batch would compute from the colony state forgets it.
     std::vector<uint64_t> continuable;
     uint32_t continuable_mne = 0;
+    // frozen preparation charges per instance key (cheap, preparation
+    // seconds) billed by cached_charged time-budget evaluations
+    std::map<uint64_t, std::pair<double, double>> charges;
     void forget_run() { continuable.clear(); continuable_mne = 0; }
     // graph cache
     hipGraphExec_t graph = nullptr;
@@ -813,6 +816,46 @@ void restore_state(Engine& e, const CountedState& saved, uint32_t C, const std::
                                    hipMemcpyHostToDevice, e.stream), "snapshot upload");
 }
 
+// Launch parameters of the control kernels for a panel of `panel` tasks
+// under `spec`, the progress feature measured against `progress` FE.
+ControlParams control_params(Engine& e, uint32_t panel, const faco_control_spec& spec, uint64_t progress) {
+    auto& b = *e.ctl;
+    ControlParams cp{};
+    cp.n = e.n; cp.ants = e.config.ants; cp.pw = e.config.primary_width; cp.K = e.K; cp.nbr_stride = e.nbr_stride;
+    cp.panel = panel; cp.mode = spec.mode; cp.variant = spec.factorial_variant;
+    cp.experiment_mask = spec.experiment_mask; cp.force_collisions = spec.force_fingerprint_collisions ? 1u : 0u;
+    cp.forced_action = -1;
+    cp.evaluation_limit = progress; cp.ls_limit = e.config.ls_evaluation_limit;
+    cp.retention = e.config.retention; cp.p_best_root = e.p_root; cp.baseline = spec.baseline;
+    cp.coords = e.geo.coords.data(); cp.nbr = e.geo.nbr.data(); cp.eta = e.geo.eta.data();
+    cp.local_scale = b.local_scale.data(); cp.epsilon = b.epsilon.data(); cp.samples = b.samples.data();
+    cp.initial = e.geo.initial.data(); cp.initial_cost = e.geo.initial_cost.data();
+    cp.geometry = e.geometry.data(); cp.keys = e.keys.data();
+    cp.programs = b.programs.data(); cp.controllers = b.controllers.data();
+    cp.state = e.state.data(); cp.control = b.control.data();
+    cp.parent = e.parent.data(); cp.links = e.links.data(); cp.epoch = e.epoch.data();
+    cp.trails = e.trails.data(); cp.products = e.products.data();
+    cp.archive_tours = b.archive_tours.data(); cp.archive_links = b.archive_links.data();
+    cp.tours = e.tours.data(); cp.info = e.info.data();
+    cp.ant_identity = b.ant_identity.data(); cp.ant_zero = b.ant_zero.data();
+    cp.features = b.features.data(); cp.scores = b.scores.data(); cp.masks = b.masks.data();
+    cp.actions = b.actions.data(); cp.alternatives = b.alternatives.data(); cp.regions = b.regions.data();
+    cp.shapes = e.shapes_active ? e.shapes.data() : nullptr;
+    return cp;
+}
+
+// The ant kernel of a controlled run: every ant's tour and identity are
+// written (the archive update reads them) and ants start in the region the
+// decision chose.
+AntParams controlled_ant_params(Engine& e, uint32_t colonies) {
+    AntParams ap = e.ant_params(colonies);
+    ap.filter_tours = 0;
+    ap.control = e.ctl->control.data();
+    ap.out_identity = e.ctl->ant_identity.data();
+    ap.out_zero = e.ctl->ant_zero.data();
+    return ap;
+}
+
 void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* panel_seeds, uint32_t panel,
                       uint64_t limit, const faco_control_spec& spec, uint32_t flags, const faco_control_trace* trace,
                       faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
@@ -916,33 +959,8 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         b.programs.upload(spec.programs, deciders, e.stream);
     if (spec.mode == FACO_CONTROL_CONTROLLER) b.controllers.upload(spec.controllers, deciders, e.stream);
 
-    ControlParams cp{};
-    cp.n = e.n; cp.ants = ants; cp.pw = e.config.primary_width; cp.K = e.K; cp.nbr_stride = e.nbr_stride;
-    cp.panel = panel; cp.mode = spec.mode; cp.variant = spec.factorial_variant;
-    cp.experiment_mask = spec.experiment_mask; cp.force_collisions = spec.force_fingerprint_collisions ? 1u : 0u;
-    cp.forced_action = -1;
-    cp.evaluation_limit = progress; cp.ls_limit = e.config.ls_evaluation_limit;
-    cp.retention = e.config.retention; cp.p_best_root = e.p_root; cp.baseline = spec.baseline;
-    cp.coords = e.geo.coords.data(); cp.nbr = e.geo.nbr.data(); cp.eta = e.geo.eta.data();
-    cp.local_scale = b.local_scale.data(); cp.epsilon = b.epsilon.data(); cp.samples = b.samples.data();
-    cp.initial = e.geo.initial.data(); cp.initial_cost = e.geo.initial_cost.data();
-    cp.geometry = e.geometry.data(); cp.keys = e.keys.data();
-    cp.programs = b.programs.data(); cp.controllers = b.controllers.data();
-    cp.state = e.state.data(); cp.control = b.control.data();
-    cp.parent = e.parent.data(); cp.links = e.links.data(); cp.epoch = e.epoch.data();
-    cp.trails = e.trails.data(); cp.products = e.products.data();
-    cp.archive_tours = b.archive_tours.data(); cp.archive_links = b.archive_links.data();
-    cp.tours = e.tours.data(); cp.info = e.info.data();
-    cp.ant_identity = b.ant_identity.data(); cp.ant_zero = b.ant_zero.data();
-    cp.features = b.features.data(); cp.scores = b.scores.data(); cp.masks = b.masks.data();
-    cp.actions = b.actions.data(); cp.alternatives = b.alternatives.data(); cp.regions = b.regions.data();
-    cp.shapes = e.shapes_active ? e.shapes.data() : nullptr;
-
-    AntParams ap = e.ant_params(C);
-    ap.filter_tours = 0;   // the archive update reads every ant's tour
-    ap.control = b.control.data();
-    ap.out_identity = b.ant_identity.data();
-    ap.out_zero = b.ant_zero.data();
+    ControlParams cp = control_params(e, panel, spec, progress);
+    const AntParams ap = controlled_ant_params(e, C);
     const ColonyParams colony = e.colony_params();
     // FACO_RUN_EVENTS: HIP events between the launches of every batch;
     // kernel_ms [0] construction + LS, [1] colony update, [2] decision
@@ -1135,6 +1153,250 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     fill_result(result, e, C, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
                 seconds_since(started), search, gpu_ms);
     if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+}
+
+// Time-budget evaluation (reference cpp/include/gp_faco/deadline.hpp and
+// the budget path of cuda/batch_engine.cu:492-1192): the clock starts with
+// the call; charges (cached_charged preparation) add to it. An incumbent is
+// accepted only when offered no later than the budget and strictly shorter
+// than the one held.
+struct Deadline {
+    Clock::time_point started;
+    double budget, charged = 0;
+    double elapsed() const { return seconds_since(started) + charged; }
+    bool can_start() const { return elapsed() < budget; }
+    bool expired() const { return elapsed() > budget; }
+};
+
+struct Incumbent {
+    std::vector<uint32_t> tour;
+    double cost = 0, completed = 0;
+    bool present = false;
+};
+
+void evaluate_deadline(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks,
+                       const faco_deadline_spec& spec, faco_deadline_result* result, uint32_t* tours, double* costs,
+                       uint8_t* has_incumbent, double* completed_seconds, faco_control_state* states, double* ratios,
+                       uint32_t ratio_capacity, faco_batch_profile* profile) {
+    Deadline clock{Clock::now(), spec.budget_seconds, 0};
+    std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+    require(lock.owns_lock(), "one evaluation per engine at a time");
+    require(current_device() == e.device, "engine must be used on the device it was created on");
+    require(!e.mixed && !e.hard && e.population == 1,
+            "time-budget evaluations take a fixed-size, unrestricted engine of population 1");
+    require(tasks == e.panel, "task count must equal the engine's colonies");
+    require(std::isfinite(spec.budget_seconds) && spec.budget_seconds >= 0, "budget seconds must be finite and >= 0");
+    require(spec.preparation_mode <= 1, "unknown preparation mode");
+    require(spec.fixed_batches <= 10000, "diagnostic batch count must be in 1..10000");
+    const bool fixed_ratio = spec.fixed_elapsed_ratio >= 0;
+    require(std::isfinite(spec.fixed_elapsed_ratio) && (!fixed_ratio ||
+            (spec.fixed_batches > 0 && spec.fixed_elapsed_ratio <= 1)),
+            "a fixed progress feature lies in [0, 1] and needs a fixed batch count");
+    require(!spec.profile || (spec.fixed_batches > 0 && fixed_ratio),
+            "a profile needs a fixed batch count and a fixed progress feature");
+    const bool controlled = spec.program != nullptr;
+    if (controlled) {
+        validate_program(*spec.program);
+        require(spec.program->feature_spec_id == 1, "time-budget programs read progress feature spec 1");
+        require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
+    } else {
+        require(spec.mne_target > 0, "mne_target must be positive");
+    }
+    for (uint32_t c = 0; c < tasks; ++c) (void)e.problem(instance_keys[c]);
+    e.forget_run();
+    const uint32_t n = e.n;
+    std::vector<Incumbent> held(tasks);
+    faco_deadline_result r{};
+    r.budget_seconds = spec.budget_seconds;
+    r.allocated_device_bytes = e.allocated;
+    auto offer = [&](const std::vector<uint32_t>& ids, const uint32_t* tour, double cost) {
+        const double completed = clock.elapsed();
+        if (completed > clock.budget) return;
+        for (uint32_t c : ids) {
+            Incumbent& h = held[c];
+            if (h.present && !(cost < h.cost)) continue;
+            h.tour.assign(tour, tour + n);
+            h.cost = cost; h.completed = completed; h.present = true;
+        }
+    };
+    if (states) std::fill(states, states + tasks, faco_control_state{});
+    auto finish = [&]() {
+        r.actual_seconds = seconds_since(clock.started);
+        r.charged_seconds = clock.charged;
+        r.elapsed_seconds = r.actual_seconds + r.charged_seconds;
+        r.overrun_seconds = std::max(0.0, r.elapsed_seconds - clock.budget);
+        for (uint32_t c = 0; c < tasks; ++c) {
+            const Incumbent& h = held[c];
+            if (has_incumbent) has_incumbent[c] = h.present ? 1 : 0;
+            if (costs) costs[c] = h.present ? h.cost : 0.0;
+            if (completed_seconds) completed_seconds[c] = h.present ? h.completed : 0.0;
+            if (tours) {
+                if (h.present) std::copy(h.tour.begin(), h.tour.end(), tours + static_cast<size_t>(c) * n);
+                else std::fill(tours + static_cast<size_t>(c) * n, tours + static_cast<size_t>(c + 1) * n, UINT32_MAX);
+            }
+        }
+        if (result) *result = r;
+    };
+    // colonies grouped by instance, in key order
+    std::map<uint64_t, std::vector<uint32_t>> groups;
+    for (uint32_t c = 0; c < tasks; ++c) groups[instance_keys[c]].push_back(c);
+    auto charge_of = [&](uint64_t key, bool cheap) {
+        const auto frozen = e.charges.find(key);
+        const Prepared& p = e.problem(key);
+        if (frozen != e.charges.end()) return cheap ? frozen->second.first : frozen->second.second;
+        return cheap ? p.cheap_seconds : p.preparation_seconds;
+    };
+    // preparation: every instance's cheap tour first, then its prepared
+    // initial tour (end_to_end prepares again inside the budget; the result
+    // is the registered preparation, which is deterministic)
+    const auto setup = Clock::now();
+    for (const auto& [key, ids] : groups) {
+        if (!clock.can_start()) return finish();
+        const Prepared& p = e.problem(key);
+        if (spec.preparation_mode == 0) {
+            clock.charged += charge_of(key, true);
+            offer(ids, p.cheap.data(), static_cast<double>(p.cheap_cost));
+        } else {
+            const Prepared cheap = prepare_begin(p.xy, e.config);
+            offer(ids, cheap.cheap.data(), static_cast<double>(cheap.cheap_cost));
+        }
+        if (clock.expired()) return finish();
+    }
+    for (const auto& [key, ids] : groups) {
+        if (!clock.can_start()) return finish();
+        const Prepared& p = e.problem(key);
+        if (spec.preparation_mode == 0) {
+            clock.charged += charge_of(key, false);
+            offer(ids, p.initial.data(), static_cast<double>(p.initial_cost));
+        } else {
+            const Prepared fresh = prepare_problem(p.xy, e.config, e.K, e.nbr_stride);
+            require(fresh.initial == p.initial, "end-to-end preparation differs from the registered one");
+            offer(ids, fresh.initial.data(), static_cast<double>(fresh.initial_cost));
+        }
+        if (clock.expired()) return finish();
+    }
+    // upload and initialise the colonies
+    if (controlled) e.ensure_control();
+    e.upload_geometry(instance_keys, tasks);
+    std::vector<uint64_t> keys(tasks);
+    for (uint32_t c = 0; c < tasks; ++c) keys[c] = colony_key(instance_keys[c], seeds[c]);
+    e.keys.upload(keys.data(), tasks, e.stream);
+    r.setup_seconds = seconds_since(setup);
+    hipEvent_t ev[6] = {};
+    struct Destroy { hipEvent_t* x; ~Destroy() { for (int i = 0; i < 6; ++i) if (x[i]) (void)hipEventDestroy(x[i]); } } destroy{ev};
+    if (spec.profile)
+        for (auto& x : ev) checked(hipEventCreate(&x), "hipEventCreate");
+    auto ms = [&](int a, int b) {
+        float v = 0;
+        checked(hipEventElapsedTime(&v, ev[a], ev[b]), "event time");
+        return static_cast<double>(v);
+    };
+    faco_control_spec cspec{};
+    cspec.mode = FACO_CONTROL_PROGRAM; cspec.deciders = 1; cspec.programs = spec.program;
+    cspec.experiment_mask = spec.experiment_mask;
+    if (spec.profile) checked(hipEventRecord(ev[0], e.stream), "event");
+    ControlParams cp{};
+    if (controlled) {
+        e.ctl->programs.upload(spec.program, 1, e.stream);
+        cp = control_params(e, tasks, cspec, 1);
+        e.reset(tasks, 2u);
+        checked(launch_control_init(cp, tasks, e.stream), "control init launch");
+    } else {
+        e.reset(tasks, spec.mne_target);
+    }
+    if (spec.profile) {
+        checked(hipEventRecord(ev[1], e.stream), "event");
+        checked(hipEventSynchronize(ev[1]), "event sync");
+        r.initialization_gpu_milliseconds = ms(0, 1);
+    }
+    std::vector<ControlState> cs(controlled ? tasks : 0), committed_cs;
+    if (controlled) e.ctl->control.download(cs.data(), tasks, e.stream);
+    checked(hipStreamSynchronize(e.stream), "initialisation");
+    auto write_states = [&]() {
+        if (!states) return;
+        for (uint32_t c = 0; c < tasks; ++c) {
+            faco_control_state& o = states[c];
+            o = faco_control_state{};
+            if (committed_cs.empty()) continue;
+            const ControlState& k = committed_cs[c];
+            o.archive_size = k.archive_size; o.stagnant_batches = k.stagnant; o.epoch_batches = k.epoch_batches;
+            o.restarts = k.restarts; o.return_rate = k.return_rate; o.ls_work = k.ls_work;
+        }
+    };
+    if (clock.expired()) { write_states(); return finish(); }
+    r.preparation_completed = 1;
+    committed_cs = cs;   // the initial control states until a batch completes
+    // per batch (eager launches, one synchronisation): decision, ants,
+    // colony update, archive update; then the host checks every colony's
+    // global tour and cost before offering it
+    const AntParams ap = controlled ? controlled_ant_params(e, tasks) : e.ant_params(tasks);
+    const ColonyParams colony = e.colony_params();
+    std::vector<ColonyState> st(tasks);
+    std::vector<uint32_t> global(static_cast<size_t>(tasks) * n);
+    for (uint32_t batch = 0; clock.can_start(); ++batch) {
+        if (spec.fixed_batches && batch >= spec.fixed_batches) break;
+        ++r.launched_batches;
+        const auto batch_started = Clock::now();
+        faco_batch_profile row{};
+        row.batch = batch;
+        if (spec.profile) checked(hipEventRecord(ev[0], e.stream), "event");
+        if (controlled) {
+            const double ratio = fixed_ratio ? spec.fixed_elapsed_ratio : clock.elapsed() / clock.budget;
+            if (ratios && batch < ratio_capacity) ratios[batch] = ratio;
+            cp.fixed_progress = ratio;
+            checked(launch_control_decide(cp, tasks, e.stream), "control decide launch");
+        }
+        if (spec.profile) checked(hipEventRecord(ev[1], e.stream), "event");
+        checked(launch_ant_search(ap, tasks * ap.blocks_per_colony, e.stream), "ant search launch");
+        if (spec.profile) checked(hipEventRecord(ev[2], e.stream), "event");
+        checked(launch_colony_update(colony, tasks, e.stream), "colony update launch");
+        if (spec.profile) checked(hipEventRecord(ev[3], e.stream), "event");
+        if (controlled) checked(launch_control_update(cp, tasks, e.stream), "control update launch");
+        if (spec.profile) checked(hipEventRecord(ev[4], e.stream), "event");
+        checked(hipStreamSynchronize(e.stream), "batch");
+        const auto download_started = Clock::now();
+        e.state.download(st.data(), tasks, e.stream);
+        e.global.download(global.data(), global.size(), e.stream);
+        if (controlled) e.ctl->control.download(cs.data(), tasks, e.stream);
+        checked(hipStreamSynchronize(e.stream), "batch readback");
+        row.download_seconds = seconds_since(download_started);
+        const auto verify_started = Clock::now();
+        for (uint32_t c = 0; c < tasks; ++c) {
+            const uint32_t* t = global.data() + static_cast<size_t>(c) * n;
+            if (!is_permutation(t, n) || tour_cost(e.problem(instance_keys[c]), t) != st[c].global_cost)
+                throw std::runtime_error("device incumbent is not a tour of the cost its colony reports");
+        }
+        row.verification_seconds = seconds_since(verify_started);
+        if (spec.profile) {
+            const auto collection_started = Clock::now();
+            row.gpu_milliseconds[1] = ms(0, 1);   // features_and_regions (decision with scoring and restart)
+            row.gpu_milliseconds[4] = ms(1, 2);   // construction_and_ls
+            row.gpu_milliseconds[5] = ms(2, 3);   // reduction (with the pheromone update)
+            row.gpu_milliseconds[8] = ms(3, 4);   // archive_and_feedback (with the fingerprints)
+            row.collection_seconds = seconds_since(collection_started);
+        }
+        const double completed = clock.elapsed();
+        r.last_batch_completed_seconds = completed;
+        row.wall_seconds = seconds_since(batch_started);
+        if (completed > clock.budget) {
+            ++r.discarded_batches;
+            if (profile) profile[r.profiled_batches++] = row;
+            break;
+        }
+        for (uint32_t c = 0; c < tasks; ++c) offer({c}, global.data() + static_cast<size_t>(c) * n,
+                                                   static_cast<double>(st[c].global_cost));
+        ++r.completed_batches;
+        row.committed = 1;
+        if (profile) profile[r.profiled_batches++] = row;
+        r.completed_construction_steps = r.completed_ls_evaluations = 0;
+        for (uint32_t c = 0; c < tasks; ++c) {
+            r.completed_construction_steps += st[c].work_construction;
+            r.completed_ls_evaluations += st[c].work_ls;
+        }
+        if (controlled) committed_cs = cs;
+    }
+    write_states();
+    finish();
 }
 
 // Device preparation (the O(n^2) steps of prepare_problem on the GPU):
@@ -1767,6 +2029,33 @@ int faco_engine_evaluate_faco(faco_engine* engine, const uint64_t* instance_keys
                     "evaluation limit must be a whole number of ant batches");
         evaluate(e, instance_keys, seeds, tasks, limit / e.config.ants, mne, flags, result, tours,
                  costs, work, limit, false, checkpoints);
+    });
+}
+
+int faco_engine_evaluate_deadline(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                                  uint32_t tasks, const faco_deadline_spec* spec, faco_deadline_result* result,
+                                  uint32_t* tours, double* costs, uint8_t* has_incumbent, double* completed_seconds,
+                                  faco_control_state* states, double* ratios, uint32_t ratio_capacity,
+                                  faco_batch_profile* profile) {
+    return guarded([&] {
+        require(engine && instance_keys && seeds && spec, "null argument");
+        evaluate_deadline(*engine->impl, instance_keys, seeds, tasks, *spec, result, tours, costs, has_incumbent,
+                          completed_seconds, states, ratios, ratio_capacity, profile);
+    });
+}
+
+int faco_engine_set_preparation_charges(faco_engine* engine, uint64_t key, double cheap, double preparation) {
+    return guarded([&] {
+        require(engine, "null argument");
+        Engine& e = *engine->impl;
+        std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+        require(lock.owns_lock(), "charges of an engine can not change during an evaluation");
+        (void)e.problem(key);
+        require(std::isfinite(cheap) && cheap >= 0 && std::isfinite(preparation) && preparation >= 0,
+                "frozen charges must be finite non-negative numbers");
+        const auto [entry, inserted] = e.charges.emplace(key, std::make_pair(cheap, preparation));
+        require(inserted || entry->second == std::make_pair(cheap, preparation),
+                "frozen preparation charges of one instance can not change");
     });
 }
 

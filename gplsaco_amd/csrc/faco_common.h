@@ -304,6 +304,7 @@ struct ControlParams {
     uint32_t views_only;                 // decide: write features, scores, masks, actions, alternatives and
                                         // regions only; colony and control state stay unchanged
     uint64_t evaluation_limit;          // FE per colony (progress feature)
+    double fixed_progress = -1.0;       // >= 0: the progress feature of this batch (time budgets); else FE-based
     uint64_t ls_limit;                  // LS evaluation limit (LS work feature)
     double retention, p_best_root;
     faco_baseline_policy baseline;

@@ -20,6 +20,7 @@ import time
 import numpy as np
 
 from . import _native
+from ._native import BatchProfile, DeadlineResult, DeadlineSpec
 from ._native import (BaselinePolicy, Checkpoints, ControlSpec, ControlState, ControlTrace, Controller, Fork, Program,
                       Result, Settings,
                       TuneEntry, check, lib, ptr)
@@ -379,6 +380,12 @@ def _batch_output(result: Result, tours: np.ndarray, costs: np.ndarray, mode: st
     }
 
 
+def _control_state_dict(st: ControlState) -> dict:
+    """One entry of control_states (reference bindings.cpp:394-407)."""
+    return {"archive_size": st.archive_size, "stagnant_batches": st.stagnant_batches, "return_rate": st.return_rate,
+            "ls_work": st.ls_work, "epoch_batches": st.epoch_batches, "restarts": st.restarts}
+
+
 class CountedState:
     """Snapshot of a controlled run after a whole number of batches
     (reference cpp/include/gp_faco/counted_state.hpp:1-48, bindings.cpp:541-556).
@@ -608,10 +615,9 @@ class FacoBatchEngine:
         for v in (cheap_seconds, preparation_seconds):
             if type(v) not in (int, float) or not np.isfinite(v) or v < 0:
                 raise ValueError("frozen charges must be finite non-negative numbers")
-        self.preparation_profile(key)  # raises when not registered
-        old = self._charges.setdefault(key, (float(cheap_seconds), float(preparation_seconds)))
-        if old != (float(cheap_seconds), float(preparation_seconds)):
-            raise ValueError("frozen preparation charges of one instance can not change")
+        check(lib().faco_engine_set_preparation_charges(self._handle, key, float(cheap_seconds),
+                                                        float(preparation_seconds)))
+        self._charges[key] = (float(cheap_seconds), float(preparation_seconds))
 
     def problem_data(self, instance_key) -> dict:
         n, s = self._instance_n(instance_key), self.settings
@@ -677,6 +683,115 @@ class FacoBatchEngine:
             out["wave_utilisation"] = km[11]
         return out
 
+
+    # ------------------------------------------------------------------
+    # Time-budget evaluations (reference bindings.cpp:662-700, 937-971;
+    # cpp/include/gp_faco/deadline.hpp)
+
+    def _deadline(self, instance_keys, seeds, budget_seconds, mode, *, mne=0, program=None,
+                  experiment_mask=0xFFFFFFFF, fixed_batches=0, fixed_ratio=-1.0, profile=False) -> dict:
+        keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
+        if keys.size != seeds.size or type(budget_seconds) not in (int, float):
+            raise ValueError("task arrays or budget type invalid")
+        if mode not in ("cached_charged", "end_to_end"):
+            raise ValueError("unknown preparation mode")
+        spec = DeadlineSpec()
+        spec.budget_seconds = float(budget_seconds)
+        spec.mne_target = mne
+        spec.preparation_mode = 0 if mode == "cached_charged" else 1
+        spec.experiment_mask = experiment_mask
+        spec.fixed_batches = fixed_batches
+        spec.fixed_elapsed_ratio = fixed_ratio
+        spec.profile = 1 if profile else 0
+        if program is not None:
+            spec.program = ctypes.addressof(program)
+        tasks, n = keys.size, self.dimension
+        tours = np.zeros((tasks, n), dtype=np.uint32)
+        costs = np.zeros(tasks, dtype=np.float64)
+        has = np.zeros(tasks, dtype=np.uint8)
+        completed = np.zeros(tasks, dtype=np.float64)
+        states = (ControlState * tasks)()
+        capacity = fixed_batches or 1 << 16
+        ratios = np.zeros(capacity, dtype=np.float64)
+        rows = (BatchProfile * max(fixed_batches, 1))()
+        r = DeadlineResult()
+        check(lib().faco_engine_evaluate_deadline(
+            self._handle, keys, seeds, tasks, ctypes.byref(spec), ctypes.byref(r), tours.reshape(-1), costs, has,
+            completed, ctypes.addressof(states) if program is not None else None,
+            ratios.ctypes.data_as(ctypes.c_void_p), capacity, ctypes.addressof(rows) if profile else None))
+        items = [{"has_incumbent": bool(has[c]), "tour": tours[c].tolist() if has[c] else [],
+                  "cost": float(costs[c]) if has[c] else None, "completed_seconds": float(completed[c])}
+                 for c in range(tasks)]
+        out = {"items": items, "budget_seconds": r.budget_seconds, "elapsed_seconds": r.elapsed_seconds,
+               "actual_seconds": r.actual_seconds, "charged_seconds": r.charged_seconds,
+               "last_batch_completed_seconds": r.last_batch_completed_seconds, "overrun_seconds": r.overrun_seconds,
+               "launched_batches": r.launched_batches, "completed_batches": r.completed_batches,
+               "discarded_batches": r.discarded_batches,
+               "completed_construction_steps": r.completed_construction_steps,
+               "completed_ls_evaluations": r.completed_ls_evaluations,
+               "allocated_device_bytes": r.allocated_device_bytes,
+               "preparation_completed": bool(r.preparation_completed), "preparation_mode": mode,
+               "progress_features": ratios[:min(r.launched_batches, capacity)].tolist()}
+        if program is not None:
+            out["control_states"] = [_control_state_dict(x) for x in states]
+        if profile:
+            stages = ("initialization", "features_and_regions", "gp_scoring", "action_and_restart",
+                      "construction_and_ls", "reduction", "pheromone", "ant_fingerprints", "archive_and_feedback")
+            out["profile"] = {
+                "profile_version": 1, "setup_seconds": r.setup_seconds, "host_pack_seconds": 0.0,
+                "upload_call_seconds": r.upload_seconds,
+                "initialization_gpu_milliseconds": r.initialization_gpu_milliseconds,
+                "initialization_includes_uploads": False, "initialization_download_seconds": 0.0,
+                "diagnostic_device_bytes": 0,
+                "batches": [{"batch": row.batch, "committed": bool(row.committed), "wall_seconds": row.wall_seconds,
+                             "download_seconds": row.download_seconds,
+                             "verification_seconds": row.verification_seconds,
+                             "collection_seconds": row.collection_seconds,
+                             "gpu_milliseconds": dict(zip(stages, row.gpu_milliseconds)),
+                             "ant_phase_cycles": []}
+                            for row in rows[:r.profiled_batches]]}
+        return out
+
+    def evaluate(self, instance_keys, seeds, budget_seconds, mne_target, preparation_mode="cached_charged") -> dict:
+        """Plain FACO under a time budget (reference bindings.cpp:662-680):
+        every colony's incumbent is the best tour offered within the budget
+        (cheap tour, prepared tour, then the global best after each batch
+        that completes in time)."""
+        if type(mne_target) is not int:
+            raise ValueError("task arrays or budget/MNE type invalid")
+        mne = _unsigned(mne_target, "mne_target", 32)
+        return self._deadline(instance_keys, seeds, budget_seconds, preparation_mode, mne=mne)
+
+    def evaluate_program(self, instance_keys, seeds, budget_seconds, program, preparation_mode="cached_charged",
+                         experiment_mask=0xFFFFFFFF) -> dict:
+        """A feature-spec-1 program controls the colonies under a time budget
+        (reference bindings.cpp:681-700): its progress feature is the
+        elapsed share of the budget at each decision."""
+        if type(experiment_mask) is not int:
+            raise ValueError("task arrays or budget/mask type invalid")
+        p = read_program(program)
+        mask = _unsigned(experiment_mask, "experiment_mask", 32)
+        return self._deadline(instance_keys, seeds, budget_seconds, preparation_mode, program=p, experiment_mask=mask)
+
+    def run_program_diagnostic(self, instance_keys, seeds, program, batches, elapsed_ratio=0.5, profile=True,
+                               experiment_mask=0xFFFFFFFF, preparation_mode="cached_charged",
+                               budget_seconds=120.0) -> dict:
+        """Fixed-batch program run for diagnostics, not fitness (reference
+        bindings.cpp:937-971): `batches` batches (1..10000) with a constant
+        progress feature, per-batch HIP-event and host timings when
+        `profile`, inside a time budget."""
+        if type(batches) is not int or type(profile) is not bool or type(elapsed_ratio) not in (int, float) \
+                or type(experiment_mask) is not int or type(budget_seconds) not in (int, float):
+            raise ValueError("diagnostic arrays or timing argument types invalid")
+        if not 1 <= batches <= 10000:
+            raise ValueError("diagnostic batch count must be in 1..10000")
+        p = read_program(program)
+        mask = _unsigned(experiment_mask, "experiment_mask", 32)
+        out = self._deadline(instance_keys, seeds, budget_seconds, preparation_mode, program=p, experiment_mask=mask,
+                             fixed_batches=batches, fixed_ratio=float(elapsed_ratio), profile=profile)
+        out["scope"] = "fixed_batch_diagnostic_not_fitness"
+        out.setdefault("profile", {"profile_version": 1, "batches": []})
+        return out
 
     # ------------------------------------------------------------------
     # Controlled evaluations (reference bindings.cpp:721-879)
@@ -761,9 +876,7 @@ class FacoBatchEngine:
         return {"result": result, "tours": tours, "costs": costs, "work": work, "elapsed": time.perf_counter() - started,
                 "checkpoints": request.output() if request.points else None,
                 "profile_events_only": bool(profile_events_only),
-                "states": [{"archive_size": st.archive_size, "stagnant_batches": st.stagnant_batches,
-                            "return_rate": st.return_rate, "ls_work": st.ls_work, "epoch_batches": st.epoch_batches,
-                            "restarts": st.restarts} for st in states],
+                "states": [_control_state_dict(st) for st in states],
                 **{"trace_" + name: array for name, array in traces.items()}}
 
     @staticmethod

@@ -381,6 +381,83 @@ int faco_engine_evaluate_control(faco_engine* engine, const uint64_t* instance_k
                                  faco_control_state* states, const faco_checkpoints* checkpoints);
 
 /* ---------------------------------------------------------------------
+ * Time-budget evaluations (reference cpp/include/gp_faco/deadline.hpp,
+ * cuda/batch_engine.cu:340-362,492-1192 with a budget in seconds;
+ * bindings.cpp:662-700 evaluate / evaluate_program and :937-971
+ * run_program_diagnostic). The clock runs from the call's start; in
+ * preparation mode 0 ("cached_charged") each instance's registered (or
+ * frozen, faco_engine_set_preparation_charges) cheap and preparation
+ * seconds are charged to it instead of preparing again, in mode 1
+ * ("end_to_end") the preparation is redone inside the budget. Every
+ * colony's incumbent is first the cheap tour, then the prepared initial
+ * tour, then its global best after each whole batch that completes within
+ * the budget (a batch still running at the deadline is discarded); a
+ * later offer replaces the incumbent only when strictly shorter. */
+typedef struct faco_deadline_spec {
+    double budget_seconds;         /* finite, >= 0 */
+    uint32_t mne_target;           /* plain FACO (program == NULL), > 0 */
+    uint32_t preparation_mode;     /* 0 cached_charged, 1 end_to_end */
+    const faco_program* program;   /* NULL: plain FACO (evaluate); else program control (evaluate_program),
+                                      feature spec 1: the progress feature is elapsed / budget */
+    uint32_t experiment_mask;      /* program control: legal actions */
+    uint32_t fixed_batches;        /* 0: batches until the deadline; 1..10000: at most this many
+                                      (run_program_diagnostic) */
+    double fixed_elapsed_ratio;    /* < 0: elapsed / budget; in [0, 1]: this constant progress feature
+                                      (needs fixed_batches) */
+    uint32_t profile;              /* per-batch profile rows (needs fixed_batches and a fixed ratio) */
+    uint32_t pad;
+} faco_deadline_spec;
+
+typedef struct faco_deadline_result {
+    double budget_seconds;
+    double elapsed_seconds;              /* actual + charged */
+    double actual_seconds;
+    double charged_seconds;
+    double last_batch_completed_seconds;
+    double overrun_seconds;              /* max(0, elapsed - budget) */
+    uint64_t launched_batches, completed_batches, discarded_batches;
+    uint64_t completed_construction_steps, completed_ls_evaluations;
+    uint64_t allocated_device_bytes;
+    uint32_t preparation_completed;
+    uint32_t profiled_batches;           /* rows written to `profile` */
+    double setup_seconds, upload_seconds, initialization_gpu_milliseconds;
+} faco_deadline_result;
+
+/* One batch of a profiled diagnostic run (reference profiling.hpp:26-33):
+ * host wall, download and verification seconds; HIP-event milliseconds of
+ * the stages in the reference's order (initialization, features_and_regions,
+ * gp_scoring, action_and_restart, construction_and_ls, reduction, pheromone,
+ * ant_fingerprints, archive_and_feedback; fused stages report under their
+ * first name). */
+typedef struct faco_batch_profile {
+    uint32_t batch, committed;
+    double wall_seconds, download_seconds, verification_seconds, collection_seconds;
+    double gpu_milliseconds[9];
+} faco_batch_profile;
+
+/* tours [tasks][n] and costs, has_incumbent, completed_seconds [tasks]
+ * receive the incumbents (a colony without one keeps has_incumbent 0);
+ * states [tasks] (may be NULL) the control summary of the last completed
+ * batch of a program run; ratios [ratio_capacity] (may be NULL) the
+ * progress feature each launched batch decided with; profile [fixed_batches]
+ * (may be NULL) the per-batch rows. A fixed-size, unrestricted,
+ * population-1 engine (the reference's time-budget path takes no Hard
+ * mode and no population). */
+int faco_engine_evaluate_deadline(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds,
+                                  uint32_t tasks, const faco_deadline_spec* spec, faco_deadline_result* result,
+                                  uint32_t* tours, double* costs, uint8_t* has_incumbent, double* completed_seconds,
+                                  faco_control_state* states, double* ratios, uint32_t ratio_capacity,
+                                  faco_batch_profile* profile);
+
+/* set_preparation_charges(instance_key, cheap_seconds, preparation_seconds)
+ * (reference batch_engine.cu:318-330, bindings.cpp:651-661): frozen
+ * charges a cached_charged time-budget evaluation bills for the instance
+ * instead of its measured registration times; finite, non-negative and,
+ * once set, unchangeable. */
+int faco_engine_set_preparation_charges(faco_engine* engine, uint64_t instance_key, double cheap_seconds,
+                                        double preparation_seconds);
+
+/* ---------------------------------------------------------------------
  * CountedState snapshots of controlled runs (reference
  * cpp/include/gp_faco/counted_state.hpp, cpp/src/counted_state.cpp,
  * batch_engine.hpp:119-131, bindings.cpp:541-556 and 880-934). The

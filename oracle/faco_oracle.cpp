@@ -1542,7 +1542,9 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           u64 fork_seed, std::int32_t fork_action,
                           const u32* graph_offsets /* [instances][n + 1] or null: Hard mode */,
                           const u32* graph_neighbors, u32 neighbor_stride,
-                          const float* multipliers /* [instances][n * pw] or null */) {
+                          const float* multipliers /* [instances][n * pw] or null */,
+                          const double* progress /* [batches]: the progress feature of each batch, or null:
+                                                    completed FE / evaluation_limit */) {
     // stop_batch (0 = the whole budget) ends the run early with the progress
     // feature still measured against evaluation_limit. At batch fork_batch
     // (UINT32_MAX = none) a paired fork (reference cuda/batch_engine.cu:839-845,
@@ -1596,8 +1598,9 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
         const double started = omp_get_wtime();
 #endif
         for (u32 batch = 0; batch < batches; ++batch) {
-            const double elapsed = static_cast<double>(static_cast<u64>(batch) * s.ants) /
-                                   static_cast<double>(evaluation_limit);
+            const double elapsed = progress ? progress[batch]
+                                            : static_cast<double>(static_cast<u64>(batch) * s.ants) /
+                                                  static_cast<double>(evaluation_limit);
             for (u32 c = 0; c < colonies; ++c) {
                 const Problem& p = problems[c % instances];
                 Decision& d = decisions[c];

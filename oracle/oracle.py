@@ -248,7 +248,7 @@ def lib():
                                             u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+                                            ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_colony_select.argtypes = [ctypes.c_uint32, i32p, ctypes.POINTER(Settings), ctypes.c_uint64,
                                            ctypes.c_uint32, ctypes.c_uint32, u32p, i64p, i64p, u32p, f64p, f64p,
                                            f64p, f64p, ctypes.POINTER(ctypes.c_uint32),
@@ -869,11 +869,13 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
                    trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None, graphs=None,
-                   multipliers=None):
+                   multipliers=None, progress=None):
     """Controlled FACO: colony c uses instance c % instances and decider c // instances.
 
     multipliers [instances, n, pw] (float32): learned heuristic multipliers,
-    as for run_faco. stop_batch (0 = all) ends the run after that many batches, the progress
+    as for run_faco. progress [batches] (float64): the progress feature of
+    each batch (a time-budget run's elapsed share) instead of completed FE
+    over evaluation_limit. stop_batch (0 = all) ends the run after that many batches, the progress
     feature still over evaluation_limit; fork = (batch, seed, action) forces
     that batch's action and re-keys every colony from its ants on."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
@@ -910,6 +912,11 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         deciders = len(controllers)
     policy = baseline if baseline is not None else Baseline()
     g_off, g_nb, g_stride = _graph_arrays(graphs, instances)   # kept alive through the call
+    prog = None
+    if progress is not None:
+        prog = np.ascontiguousarray(progress, dtype=np.float64)
+        if prog.size < batches:
+            raise ValueError("one progress value per batch")
     mult = None
     if multipliers is not None:
         mult = np.ascontiguousarray(multipliers, dtype=np.float32)
@@ -926,7 +933,7 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
         ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt)),
         stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1,
-        ptr(g_off), ptr(g_nb), g_stride, ptr(mult)), L)
+        ptr(g_off), ptr(g_nb), g_stride, ptr(mult), ptr(prog)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
                        "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,

@@ -424,3 +424,26 @@ def test_alternate_stream_is_unrelated_and_restored():
     assert before["tours"].tolist() == after["tours"].tolist()
     assert other["tours"].tolist() == again["tours"].tolist()
     assert other["work"].tolist() != before["work"].tolist()
+
+
+def test_progress_override_reproduces_the_count_feature():
+    """The explicit per-batch progress feature (time-budget runs) equals the
+    default run when it carries the default's values, completed FE over the
+    evaluation limit, and changes the run when it does not."""
+    rng = np.random.default_rng(61)
+    n, ants, batches = 45, 8, 12
+    xy = rng.integers(0, 10001, size=(2, n, 2)).astype(np.int32)
+    s = O.default_settings(ants=ants)
+    program = O.make_program([0, 0, 2], [0, 1, 0], feature_spec_id=1)   # progress + stagnation
+    seeds = np.array([3, 4], dtype=np.uint64)
+    limit = ants * batches
+    base = O.run_controlled(xy, [11, 12], seeds, s, limit, "program", programs=[program], trace=True)
+    same = O.run_controlled(xy, [11, 12], seeds, s, limit, "program", programs=[program], trace=True,
+                            progress=[b * ants / limit for b in range(batches)])
+    np.testing.assert_array_equal(base["tours"], same["tours"])
+    np.testing.assert_array_equal(base["trace_features"], same["trace_features"])
+    other = O.run_controlled(xy, [11, 12], seeds, s, limit, "program", programs=[program], trace=True,
+                             progress=[1.0] * batches)
+    assert (other["trace_features"][:, 0] == 1.0).all()
+    assert not np.array_equal(other["trace_actions"], base["trace_actions"]) or \
+        not np.array_equal(other["trace_features"], base["trace_features"])
