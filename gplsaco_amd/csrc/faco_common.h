@@ -186,6 +186,10 @@ FACO_HD uint32_t pack_xy(uint32_t x, uint32_t y) { return x | (y << 16); }
 // ---------------------------------------------------------------------------
 // Kernel parameter blocks (passed by value as kernel arguments).
 constexpr uint32_t kPhaseSlots = 10;
+// Largest instance whose construction runs on the linked-list tour (the
+// fallback list ranking keeps kMaxLinked / 64 entries per lane in
+// registers); such workgroups also stage the parent positions in LDS.
+constexpr uint32_t kMaxLinked = 1024;
 
 // Per-colony shape of a mixed-size batch: the colony's city count, ants and
 // its own iteration budget (a colony past its budget is left unchanged).
@@ -390,8 +394,12 @@ hipError_t launch_philox_check(uint64_t key, uint32_t batch, uint32_t ant, uint3
                                uint32_t* out, hipStream_t stream);
 // LDS bytes of one ant-search workgroup.
 inline uint32_t round16_host(uint32_t v) { return (v + 15u) & ~15u; }
-// packed coordinates + the colony's parent links + neighbour rows (when staged in LDS)
-inline uint32_t ant_shared_bytes(uint32_t n, uint32_t nbr_lds_bytes) { return 2u * round16_host(4u * n) + nbr_lds_bytes; }
+// packed coordinates + the colony's parent links + neighbour rows (when
+// staged in LDS) + the parent positions of every city (u16, linked-list
+// construction only: n <= kMaxLinked)
+inline uint32_t ant_shared_bytes(uint32_t n, uint32_t nbr_lds_bytes) {
+    return 2u * round16_host(4u * n) + nbr_lds_bytes + (n <= kMaxLinked ? round16_host(2u * n) : 0u);
+}
 inline uint32_t ant_wave_bytes(uint32_t n) { return 3u * round16_host(2u * n) + round16_host(n); }
 
 }  // namespace faco

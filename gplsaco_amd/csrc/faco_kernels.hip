@@ -71,9 +71,6 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 }
 
 constexpr uint8_t kVisited = 1, kQueued = 2;
-// Largest instance whose construction runs on the linked-list tour (the
-// list ranking keeps kMaxLinked / 64 entries per lane in registers).
-constexpr uint32_t kMaxLinked = 1024;
 
 
 __device__ __forceinline__ uint32_t round16(uint32_t v) { return (v + 15u) & ~15u; }
@@ -148,9 +145,10 @@ __device__ __forceinline__ void bounds(ColonyState& s, uint32_t width, double re
 // and its links, and the fused evaporate/deposit/clamp/product update
 // (reference cuda/colony_state.cuh:148-229). `red` holds 3 words per wave,
 // `s` is the workgroup's copy of the colony state and `s_links` [n] receives
-// the new parent links (LDS).
+// the new parent links (LDS); `s_ppos` [n] (LDS, may be null) the parent
+// position of every city.
 __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t colony, uint32_t tid, uint32_t threads,
-                                              uint64_t* red, ColonyState& s, uint32_t* s_links) {
+                                              uint64_t* red, ColonyState& s, uint32_t* s_links, uint16_t* s_ppos) {
     const uint32_t ns = p.n;
     const uint32_t n = p.shapes ? p.shapes[colony].n : ns;
     const uint32_t ants = p.shapes ? p.shapes[colony].ants : p.ants;
@@ -215,6 +213,7 @@ __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t co
         parent[i] = v;
         s_links[v] = pred | (succ << 16);
         links[v] = pred | (succ << 16);
+        if (s_ppos) s_ppos[v] = static_cast<uint16_t>(i);
         if (v == 0) s.parent_zero = i | (succ < pred ? 0x80000000u : 0u);
     }
     __syncthreads();
@@ -259,7 +258,7 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
     __shared__ uint64_t red[3 * 4];
     __shared__ ColonyState s;
     extern __shared__ uint32_t s_links[];
-    update_colony(p, colony, tid, blockDim.x, red, s, s_links);
+    update_colony(p, colony, tid, blockDim.x, red, s, s_links, nullptr);
 }
 
 // ant_search: the body of the ant kernel. Persistent: the workgroup runs
@@ -312,10 +311,15 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
 
     // Shared data: packed coordinates, the colony's parent links (the
     // parent tour is the same for every ant of the colony in an iteration;
-    // the explicit mode's per-ant parents stay in global memory) and the
-    // neighbour rows.
+    // the explicit mode's per-ant parents stay in global memory), the
+    // neighbour rows and, for linked-list construction (a stride of at
+    // most kMaxLinked cities), the parent position of every city.
+    const bool with_ppos = ns <= kMaxLinked;
     uint32_t* s_xy = reinterpret_cast<uint32_t*>(lds);
     uint32_t* s_plinks = reinterpret_cast<uint32_t*>(lds + round16(4u * n));
+    uint16_t* s_ppos = reinterpret_cast<uint16_t*>(lds + 2u * round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u));
+    const uint32_t shared_bytes = 2u * round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u) +
+                                  (with_ppos ? round16(2u * n) : 0u);
     const uint16_t* s_nbr;
     {
         const uint32_t* gxy = p.coords + static_cast<size_t>(geom) * ns;
@@ -323,6 +327,10 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         if (!explicit_mode) {
             const uint32_t* glinks = p.links + static_cast<size_t>(colony) * ns;
             for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_plinks[i] = glinks[i];
+            if (with_ppos) {
+                const uint32_t* gparent = p.parent + static_cast<size_t>(colony) * ns;
+                for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_ppos[gparent[i]] = static_cast<uint16_t>(i);
+            }
         }
         const uint16_t* g_nbr = p.nbr + static_cast<size_t>(geom) * p.nbr_stride;
         if constexpr (NbrLds) {
@@ -342,7 +350,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     }
     __syncthreads();
 
-    unsigned char* mine = lds + 2u * round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u) + wave * p.per_wave_bytes;
+    unsigned char* mine = lds + shared_bytes + wave * p.per_wave_bytes;
     uint16_t* tour = reinterpret_cast<uint16_t*>(mine);
     uint16_t* pos = reinterpret_cast<uint16_t*>(mine + round16(2u * n));
     uint16_t* queue = reinterpret_cast<uint16_t*>(mine + 2u * round16(2u * n));
@@ -455,6 +463,13 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             trace_uniform[static_cast<size_t>(ant) * ns] = 0.0;
         }
         if (lane == 0) flags[current] = kVisited;
+        // The visit order (start, then each selected city) is recorded at the
+        // top of the checklist ring, entry i at queue[n - 1 - i], while the
+        // construction's pushes fill it from the bottom; an entry is written
+        // only while it lies above every push so far, and the record is used
+        // only if the two never met (logged + steps < n at the end).
+        const bool chain_rec = linked && with_ppos && !explicit_mode && !per_ant_parent;
+        if (chain_rec && lane == 0) queue[n - 1] = static_cast<uint16_t>(current);
         // array mode: position of `current`; linked mode: its links
         uint32_t pc = linked ? 0u : pos[current];
         uint32_t cur_link = linked ? uniform(lnk[current]) : 0u;
@@ -691,6 +706,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             if (selected >= n) { exhausted = 1; break; }
 
             ++steps;
+            if (chain_rec && lane == 0 && logged + steps < n) queue[n - 1 - steps] = static_cast<uint16_t>(selected);
             uint32_t ps = 0, old_prev, after, target_after;
             if (linked) {
                 const uint32_t sel_link = lnk[selected];
@@ -777,66 +793,129 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         // is the same).
         uint32_t edge_sum = 0;
         if (linked) {
-            // Array position of every city = its distance from `head` along
-            // predecessor links: list ranking by pointer jumping, each lane
-            // holding (distance | link << 16) for cities lane + 64k. An
-            // entry is one packed word, so every update keeps
-            // distance(v) = hops from v to link(v); reading an entry already
-            // advanced in this round only jumps further, and the ranks are
-            // exact once every link reaches `head`.
+            // Edge lengths from the final links (each city and its predecessor).
             constexpr uint32_t kPer = kMaxLinked / kWave;
-            uint32_t e[kPer];
             const uint32_t chunks = (n + kWave - 1) / kWave;
 #pragma unroll
             for (uint32_t k = 0; k < kPer; ++k) {
                 const uint32_t v = lane + k * kWave;
-                if (k < chunks && v < n) {
-                    const uint32_t pred = lnk[v] & 0xffffu;
-                    edge_sum += euc2d(s_xy[pred], s_xy[v]);
-                    e[k] = v == head ? (head << 16) : (1u | (pred << 16));
-                }
+                if (k < chunks && v < n) edge_sum += euc2d(s_xy[lnk[v] & 0xffffu], s_xy[v]);
             }
             wave_sync();
-#pragma unroll
-            for (uint32_t k = 0; k < kPer; ++k) {
-                const uint32_t v = lane + k * kWave;
-                if (k < chunks && v < n) lnk[v] = e[k];
-            }
-            wave_sync();
-            for (uint32_t round = 0; round < 17; ++round) {
-                bool pending = false;
-#pragma unroll
-                for (uint32_t k0 = 0; k0 < kPer; k0 += 8) {
-                    if (k0 >= chunks) break;
-                    uint32_t t[8];
-#pragma unroll
-                    for (uint32_t k = k0; k < k0 + 8; ++k) {
-                        const uint32_t v = lane + k * kWave;
-                        t[k - k0] = (k < chunks && v < n) ? lnk[e[k] >> 16] : 0u;
+            if (chain_rec && logged + steps < n) {
+                // The working tour is the visit chain [start, s1, .., sk]
+                // followed by the unvisited cities in the parent's cyclic order
+                // from start on (a relocation keeps the chain contiguous and
+                // the others in their relative order), and the array is that
+                // cycle read from `head`. One pass over the parent positions
+                // places every city: from head's position when head is
+                // unvisited, else from start's, shifted by head's chain index;
+                // the whole chain lands where start lies in the parent.
+                const uint32_t chain_len = steps + 1;
+                uint32_t shift = 0, from;
+                if (flags[head] & kVisited) {
+                    for (uint32_t i0 = 0; i0 < chain_len; i0 += kWave) {
+                        const uint32_t i = i0 + lane;
+                        const uint64_t hit = ballot(i < chain_len && queue[n - 1 - (i < chain_len ? i : 0u)] == head);
+                        if (hit) { shift = i0 + ctz64(hit); break; }
                     }
-#pragma unroll
-                    for (uint32_t k = k0; k < k0 + 8; ++k) {
-                        const uint32_t v = lane + k * kWave;
-                        if (k < chunks && v < n) {
-                            e[k] = ((e[k] & 0xffffu) + (t[k - k0] & 0xffffu)) | (t[k - k0] & 0xffff0000u);
-                            lnk[v] = e[k];
-                            pending |= (e[k] >> 16) != head;
+                    from = s_ppos[start];
+                } else {
+                    from = s_ppos[head];
+                }
+                from = uniform(from);
+                uint32_t placed = 0, extra = 0;   // cities placed so far; chain_len once start is passed
+                for (uint32_t t0 = 0; t0 < n; t0 += kWave) {
+                    const uint32_t t = t0 + lane;
+                    const bool valid = t < n;
+                    uint32_t at = from + (valid ? t : 0u);
+                    at = at >= n ? at - n : at;
+                    const uint32_t c = parent[at];
+                    const bool free = valid && !(flags[c] & kVisited);
+                    const uint64_t fm = ballot(free);
+                    const uint64_t sm = ballot(valid && c == start);
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(fm >> 32),
+                                          __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(fm), 0u));
+                    uint32_t lane_extra = extra;
+                    if (sm) {
+                        const uint32_t ls = ctz64(sm);
+                        const uint32_t below = static_cast<uint32_t>(__builtin_popcountll(fm & ((1ull << ls) - 1ull)));
+                        const uint32_t first = placed + below + n - shift;   // chain position of start, shifted
+                        for (uint32_t i = lane; i < chain_len; i += kWave) {
+                            const uint32_t cc = queue[n - 1 - i];
+                            uint32_t d = first + i;
+                            d = d >= n ? d - n : d;
+                            tour[d] = static_cast<uint16_t>(cc);
+                            pos[cc] = static_cast<uint16_t>(d);
                         }
+                        lane_extra = lane > ls ? chain_len : extra;
+                        extra = chain_len;
+                    }
+                    if (free) {
+                        uint32_t d = placed + rank + lane_extra + n - shift;
+                        d = d >= n ? d - n : d;
+                        tour[d] = static_cast<uint16_t>(c);
+                        pos[c] = static_cast<uint16_t>(d);
+                    }
+                    placed += static_cast<uint32_t>(__builtin_popcountll(fm));
+                }
+                wave_sync();
+            } else {
+                // Array position of every city = its distance from `head`
+                // along predecessor links: list ranking by pointer jumping,
+                // each lane holding (distance | link << 16) for cities
+                // lane + 64k. An entry is one packed word, so every update
+                // keeps distance(v) = hops from v to link(v); reading an entry
+                // already advanced in this round only jumps further, and the
+                // ranks are exact once every link reaches `head`.
+                uint32_t e[kPer];
+#pragma unroll
+                for (uint32_t k = 0; k < kPer; ++k) {
+                    const uint32_t v = lane + k * kWave;
+                    if (k < chunks && v < n) e[k] = v == head ? (head << 16) : (1u | ((lnk[v] & 0xffffu) << 16));
+                }
+                wave_sync();
+#pragma unroll
+                for (uint32_t k = 0; k < kPer; ++k) {
+                    const uint32_t v = lane + k * kWave;
+                    if (k < chunks && v < n) lnk[v] = e[k];
+                }
+                wave_sync();
+                for (uint32_t round = 0; round < 17; ++round) {
+                    bool pending = false;
+#pragma unroll
+                    for (uint32_t k0 = 0; k0 < kPer; k0 += 8) {
+                        if (k0 >= chunks) break;
+                        uint32_t t[8];
+#pragma unroll
+                        for (uint32_t k = k0; k < k0 + 8; ++k) {
+                            const uint32_t v = lane + k * kWave;
+                            t[k - k0] = (k < chunks && v < n) ? lnk[e[k] >> 16] : 0u;
+                        }
+#pragma unroll
+                        for (uint32_t k = k0; k < k0 + 8; ++k) {
+                            const uint32_t v = lane + k * kWave;
+                            if (k < chunks && v < n) {
+                                e[k] = ((e[k] & 0xffffu) + (t[k - k0] & 0xffffu)) | (t[k - k0] & 0xffff0000u);
+                                lnk[v] = e[k];
+                                pending |= (e[k] >> 16) != head;
+                            }
+                        }
+                    }
+                    wave_sync();
+                    if (!ballot(pending)) break;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kPer; ++k) {
+                    const uint32_t v = lane + k * kWave;
+                    if (k < chunks && v < n) {
+                        const uint32_t d = e[k] & 0xffffu;
+                        tour[d] = static_cast<uint16_t>(v);
+                        pos[v] = static_cast<uint16_t>(d);
                     }
                 }
                 wave_sync();
-                if (!ballot(pending)) break;
             }
-#pragma unroll
-            for (uint32_t k = 0; k < kPer; ++k) {
-                const uint32_t v = lane + k * kWave;
-                if (k < chunks && v < n) {
-                    const uint32_t d = e[k] & 0xffffu;
-                    tour[d] = static_cast<uint16_t>(v);
-                    pos[v] = static_cast<uint16_t>(d);
-                }
-            }
-            wave_sync();
         } else {
             for (uint32_t i = lane; i < n; i += kWave) edge_sum += euc2d(s_xy[tour[i]], s_xy[tour[i + 1 == n ? 0 : i + 1]]);
         }
@@ -1164,9 +1243,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     } else {
         // every ant of the batch is done: the colony update, in place
         __syncthreads();
-        update_colony(cp, colony, threadIdx.x, blockDim.x,
-                      reinterpret_cast<uint64_t*>(lds + 2u * round16(4u * n) + (NbrLds ? p.nbr_lds_bytes : 0u)),
-                      *s_state, s_plinks);
+        update_colony(cp, colony, threadIdx.x, blockDim.x, reinterpret_cast<uint64_t*>(lds + shared_bytes),
+                      *s_state, s_plinks, with_ppos ? s_ppos : nullptr);
         if (threadIdx.x == 0) next_slot = 0;
         __syncthreads();
     }
