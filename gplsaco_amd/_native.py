@@ -59,6 +59,8 @@ class AntInfo(ctypes.Structure):
         ("evaluation_limit_reached", ctypes.c_uint32),
         ("checklist_size", ctypes.c_uint32),
         ("start_node", ctypes.c_uint32),
+        ("constraint_rejections", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
     ]
 
 
@@ -82,6 +84,7 @@ class Result(ctypes.Structure):
         ("search_seconds", ctypes.c_double),
         ("search_gpu_milliseconds", ctypes.c_double),
         ("kernel_milliseconds", ctypes.c_double * STAGE_COUNT),
+        ("completed_constraint_rejections", ctypes.c_uint64),
     ]
 
 
@@ -332,6 +335,7 @@ SIGNATURES = {
                                                      ctypes.POINTER(DeadlineResult), u32p, f64p, u8p, f64p, vp, vp,
                                                      ctypes.c_uint32, vp]),
     "faco_engine_set_preparation_charges": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
+    "faco_engine_constraint_rejections": (ctypes.c_int, [vp, u64p, ctypes.c_uint32]),
     "faco_engine_evaluate_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64,
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,

@@ -201,6 +201,8 @@ struct Engine {
     // what the next batch would compute from the colony state forgets it.
     std::vector<uint64_t> continuable;
     uint32_t continuable_mne = 0;
+    // Hard-mode 2-opt rejections of each colony in the last count evaluation
+    std::vector<uint64_t> last_rejections;
     // frozen preparation charges per instance key (cheap, preparation
     // seconds) billed by cached_charged time-budget evaluations
     std::map<uint64_t, std::pair<double, double>> charges;
@@ -629,6 +631,7 @@ struct Checkpoints {
 void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks, uint64_t batches,
               uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
               uint64_t limit, bool any_count = false, const faco_checkpoints* checkpoints = nullptr) {
+    e.last_rejections.clear();
     const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
     const bool events = (flags & FACO_RUN_EVENTS) != 0, resume = (flags & FACO_RUN_CONTINUE) != 0;
     double kernel_ms[FACO_STAGE_COUNT] = {0};
@@ -682,17 +685,23 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
             for (uint32_t i = e.problem(instance_keys[c]).n; i < e.n; ++i) tours[static_cast<size_t>(c) * e.n + i] = UINT32_MAX;
     }
     marks.read_back(e.stream, e.n, [&](uint32_t c) { return e.problem(instance_keys[c]).n; });
-    uint64_t steps = 0, evals = 0;
+    uint64_t steps = 0, evals = 0, rejections = 0;
+    e.last_rejections.assign(tasks, 0);
     for (uint32_t c = 0; c < tasks; ++c) {
         const uint64_t own_steps = states[c].work_construction - (resume ? before[c].work_construction : 0);
         const uint64_t own_evals = states[c].work_ls - (resume ? before[c].work_ls : 0);
         if (costs) costs[c] = states[c].global_cost;
         if (work) { work[2 * c] = own_steps; work[2 * c + 1] = own_evals; }
         steps += own_steps; evals += own_evals;
+        e.last_rejections[c] = states[c].work_constraints - (resume ? before[c].work_constraints : 0);
+        rejections += e.last_rejections[c];
     }
     fill_result(result, e, tasks, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
                 seconds_since(started), search, gpu_ms);
-    if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+    if (result) {
+        std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+        result->completed_constraint_rejections = rejections;
+    }
     if (!e.mixed) { e.continuable = std::move(keys); e.continuable_mne = mne; }
 }
 
@@ -860,6 +869,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
                       uint64_t limit, const faco_control_spec& spec, uint32_t flags, const faco_control_trace* trace,
                       faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
                       faco_control_state* states, const faco_checkpoints* checkpoints, const StateIO* io = nullptr) {
+    e.last_rejections.clear();
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "one evaluation per engine at a time");
@@ -1101,8 +1111,11 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         for (uint32_t c = 0; c < C; ++c)
             for (uint32_t i = e.problem(inst[c]).n; i < e.n; ++i) tours[static_cast<size_t>(c) * e.n + i] = UINT32_MAX;
     marks.read_back(e.stream, e.n, [&](uint32_t c) { return e.problem(inst[c]).n; });
-    uint64_t steps = 0, evals = 0;
+    uint64_t steps = 0, evals = 0, rejections = 0;
+    e.last_rejections.assign(C, 0);
     for (uint32_t c = 0; c < C; ++c) {
+        e.last_rejections[c] = st[c].work_constraints - (resuming ? resumed[c].work_constraints : 0);
+        rejections += e.last_rejections[c];
         // a resumed run reports its own work, not the snapshot's
         const uint64_t own_steps = st[c].work_construction - (resuming ? resumed[c].work_construction : 0);
         const uint64_t own_evals = st[c].work_ls - (resuming ? resumed[c].work_ls : 0);
@@ -1152,7 +1165,10 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     }
     fill_result(result, e, C, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
                 seconds_since(started), search, gpu_ms);
-    if (result) std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+    if (result) {
+        std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
+        result->completed_constraint_rejections = rejections;
+    }
 }
 
 // Time-budget evaluation (reference cpp/include/gp_faco/deadline.hpp and
@@ -1182,6 +1198,8 @@ void evaluate_deadline(Engine& e, const uint64_t* instance_keys, const uint64_t*
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "one evaluation per engine at a time");
     require(current_device() == e.device, "engine must be used on the device it was created on");
+    // a Hard engine takes evaluation counts only: a cheap tour off the graph
+    // must not be published (reference cuda/batch_engine.cu:539-540)
     require(!e.mixed && !e.hard && e.population == 1,
             "time-budget evaluations take a fixed-size, unrestricted engine of population 1");
     require(tasks == e.panel, "task count must equal the engine's colonies");
@@ -2041,6 +2059,17 @@ int faco_engine_evaluate_deadline(faco_engine* engine, const uint64_t* instance_
         require(engine && instance_keys && seeds && spec, "null argument");
         evaluate_deadline(*engine->impl, instance_keys, seeds, tasks, *spec, result, tours, costs, has_incumbent,
                           completed_seconds, states, ratios, ratio_capacity, profile);
+    });
+}
+
+int faco_engine_constraint_rejections(faco_engine* engine, uint64_t* out, uint32_t count) {
+    return guarded([&] {
+        require(engine && out, "null argument");
+        Engine& e = *engine->impl;
+        std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+        require(lock.owns_lock(), "rejections can not be read during an evaluation");
+        require(count == e.last_rejections.size(), "one entry per task of the last count evaluation");
+        std::copy(e.last_rejections.begin(), e.last_rejections.end(), out);
     });
 }
 

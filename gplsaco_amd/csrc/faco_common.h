@@ -28,7 +28,7 @@ struct ColonyState {
     double minimum, maximum, default_trail, source_uniform;
     uint32_t iteration_best, source_is_epoch;
     uint32_t batch, pad;
-    uint64_t work_construction, work_ls;
+    uint64_t work_construction, work_ls, work_constraints;
     uint32_t target;       // MNE target of this batch's ants
     uint32_t parent_zero;  // position of city 0 in the parent | (succ(0) < pred(0)) << 31
     uint64_t best_key;     // smallest (final cost << 32 | ant) of this batch's ants so far; all ones between batches

@@ -143,7 +143,7 @@ __device__ __forceinline__ void bounds(ColonyState& s, uint32_t width, double re
 // `threads`, a multiple of 64): work totals, iteration best (lowest ant on
 // ties), epoch/global update, MMAS bounds, epoch-source coin, next parent
 // and its links, and the fused evaporate/deposit/clamp/product update
-// (reference cuda/colony_state.cuh:148-229). `red` holds 3 words per wave,
+// (reference cuda/colony_state.cuh:148-229). `red` holds 4 words per wave,
 // `s` is the workgroup's copy of the colony state and `s_links` [n] receives
 // the new parent links (LDS); `s_ppos` [n] (LDS, may be null) the parent
 // position of every city.
@@ -155,13 +155,14 @@ __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t co
     const uint32_t geom = p.geometry ? p.geometry[colony] : 0u;
     ColonyState* st = p.state + colony;
     const faco_ant_info* info = p.info + static_cast<size_t>(colony) * p.ants;
-    uint64_t best = ~0ull, wc = 0, wl = 0;
+    uint64_t best = ~0ull, wc = 0, wl = 0, wr = 0;
     for (uint32_t a = tid; a < ants; a += threads) {
         // final costs are positive and < 2^31: (cost, ant) orders ties by ant
         const uint64_t k = (static_cast<uint64_t>(info[a].final_cost) << 32) | a;
         best = k < best ? k : best;
         wc += info[a].steps;
         wl += info[a].move_evaluations;
+        wr += info[a].constraint_rejections;
     }
 #pragma unroll
     for (int offset = 32; offset; offset >>= 1) {
@@ -169,19 +170,26 @@ __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t co
         best = o < best ? o : best;
         wc += __shfl_xor(wc, offset);
         wl += __shfl_xor(wl, offset);
+        wr += __shfl_xor(wr, offset);
     }
-    if ((tid & 63u) == 0) { red[3 * (tid >> 6)] = best; red[3 * (tid >> 6) + 1] = wc; red[3 * (tid >> 6) + 2] = wl; }
+    if ((tid & 63u) == 0) {
+        uint64_t* r = red + 4 * (tid >> 6);
+        r[0] = best; r[1] = wc; r[2] = wl; r[3] = wr;
+    }
     __syncthreads();
     if (tid == 0) {
-        uint64_t key = ~0ull, total_c = 0, total_l = 0;
+        uint64_t key = ~0ull, total_c = 0, total_l = 0, total_r = 0;
         for (uint32_t w = 0; w < threads / 64u; ++w) {
-            key = red[3 * w] < key ? red[3 * w] : key;
-            total_c += red[3 * w + 1];
-            total_l += red[3 * w + 2];
+            const uint64_t* r = red + 4 * w;
+            key = r[0] < key ? r[0] : key;
+            total_c += r[1];
+            total_l += r[2];
+            total_r += r[3];
         }
         s = *st;
         s.work_construction += total_c;
         s.work_ls += total_l;
+        s.work_constraints += total_r;
         const uint32_t ib = static_cast<uint32_t>(key);
         const int64_t cost = static_cast<int64_t>(key >> 32);
         s.iteration_best = ib;
@@ -255,7 +263,7 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
         if (tid == 0) p.state[colony].pad = kColonyIdle;
         return;
     }
-    __shared__ uint64_t red[3 * 4];
+    __shared__ uint64_t red[4 * 4];
     __shared__ ColonyState s;
     extern __shared__ uint32_t s_links[];
     update_colony(p, colony, tid, blockDim.x, red, s, s_links, nullptr);
@@ -985,6 +993,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             else flip_nodes(x1, y1);
         };
         uint32_t processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0, limit_hit = 0;
+        uint32_t rejections = 0;
         int64_t gain_sum = 0;
         // The reference's loop condition (queue not empty and fewer than n
         // accepted moves) with the move count tested where it changes.
@@ -1054,6 +1063,9 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     const uint32_t added[3][2] = {{a, b}, {closing, nb}, {0u, 0u}};
                     move_allowed = edges_allowed(2, removed, added);
                 }
+                // every evaluated move that would leave the graph counts
+                // (reference cpp/src/faco_cpu.cpp:180-183)
+                rejections += static_cast<uint32_t>(__builtin_popcountll(ballot(active && !move_allowed)));
             }
             const uint32_t key_value = active && move_allowed
                 ? ((static_cast<uint32_t>(gain + (1 << 20)) << 6) | (63u - tie_index)) : 0u;
@@ -1228,6 +1240,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             info.reactivations = reactivations; info.evaluation_limit_reached = limit_hit;
             info.checklist_size = logged;
             info.start_node = start;
+            info.constraint_rejections = rejections;
+            info.pad = 0;
             p.out_info[ant] = info;
             if constexpr (Profile) {
                 uint64_t* t = p.phase_ticks + kPhaseSlots * static_cast<size_t>(ant);

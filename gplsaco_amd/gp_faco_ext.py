@@ -462,6 +462,7 @@ class FacoBatchEngine:
         check(lib().faco_engine_settings(self._handle, ctypes.byref(s)))
         self.settings = FixedFacoSettings._from_native(s)
         self._charges = {}
+        self._graph_edges = {}   # Hard mode: undirected edge count of each registered graph
 
     def __del__(self):
         handle = getattr(self, "_handle", None)
@@ -582,12 +583,33 @@ class FacoBatchEngine:
         primary, ls = rows("primary", s.primary_width), rows("ls", s.ls_width)
         backup = rows("backup", s.backup_width) if s.backup_width else None
         cheap, prep = ctypes.c_double(), ctypes.c_double()
+        key = _unsigned(instance_key, "instance_key")
         check(lib().faco_engine_register_graph_problem(
-            self._handle, _unsigned(instance_key, "instance_key"), xy.reshape(-1), n,
+            self._handle, key, xy.reshape(-1), n,
             edges.ctypes.data_as(ctypes.c_void_p), edges.shape[0], np.ascontiguousarray(tour, dtype=np.uint32), primary,
             backup.ctypes.data_as(ctypes.c_void_p) if backup is not None else None, ls, ctypes.byref(cheap),
             ctypes.byref(prep)))
+        self._graph_edges[key] = int(edges.shape[0])
         return {"cheap_seconds": cheap.value, "preparation_seconds": prep.value}
+
+    def _constraint_output(self, out: dict, keys, rejections: int) -> dict:
+        """The Hard-mode keys of a result (reference bindings.cpp:364-371):
+        the graph's undirected edge count per colony and the 2-opt moves
+        rejected for leaving the graph."""
+        if self.constraint_mode == "hard":
+            out.update({"constraint_mode": "hard", "graph_spec_id": 1,
+                        "graph_edges_per_colony": [self._graph_edges[int(k)] for k in keys],
+                        "completed_constraint_rejections": int(rejections),
+                        "preparation_scope": "engine_only; external candidate graph cached",
+                        "reserved_escape_device_bytes": 0})
+        return out
+
+    def colony_constraint_rejections(self, tasks) -> np.ndarray:
+        """Hard mode: the 2-opt moves each task's colony rejected in the last
+        count evaluation (its own batches for a continuation), in task order."""
+        out = np.zeros(_unsigned(tasks, "tasks", 32), dtype=np.uint64)
+        check(lib().faco_engine_constraint_rejections(self._handle, out, out.size))
+        return out
 
     def register_problems_device(self, instance_keys, coordinates) -> dict:
         """Register many instances with the preparation on the GPU (the same
@@ -662,6 +684,7 @@ class FacoBatchEngine:
                                               request.argument()))
         out = _batch_output(result, tours, costs, preparation_mode, time.perf_counter() - started)
         out["colony_work"] = work
+        self._constraint_output(out, keys, result.completed_constraint_rejections)
         if request.points:
             out["checkpoints"] = request.output()
         if profile_events_only:
@@ -816,17 +839,19 @@ class FacoBatchEngine:
         def call(spec, flags, trace, result, tours, costs, work, states, checkpoints):
             return lib().faco_engine_evaluate_control(self._handle, keys, seeds, keys.size, limit, spec, flags, trace,
                                                       result, tours, costs, work, states, checkpoints)
-        return self._control_run(keys.size, mode, call, programs=programs, controllers=controllers,
+        return self._control_run(keys, mode, call, programs=programs, controllers=controllers,
                                  baseline=baseline, variant=variant, experiment_mask=experiment_mask,
                                  trace_batches=trace_batches, use_graph=use_graph,
                                  force_fingerprint_collisions=force_fingerprint_collisions,
                                  checkpoint_iterations=checkpoint_iterations, profile_events_only=profile_events_only)
 
-    def _control_run(self, panel, mode, call, *, programs=(), controllers=(), baseline=None, variant=3,
+    def _control_run(self, panel_keys, mode, call, *, programs=(), controllers=(), baseline=None, variant=3,
                      experiment_mask=0xFFFFFFFF, trace_batches=(), use_graph=True, force_fingerprint_collisions=False,
                      checkpoint_iterations=(), profile_events_only=False) -> dict:
-        """One controlled native call over `panel` tasks: `call` receives the
-        spec, flags, trace and output pointers of faco_engine_evaluate_control."""
+        """One controlled native call over the tasks of `panel_keys` (their
+        instance keys): `call` receives the spec, flags, trace and output
+        pointers of faco_engine_evaluate_control."""
+        panel = len(panel_keys)
         spec = ControlSpec()
         spec.mode = {"program": 1, "controller": 2, "baseline": 3, "factorial": 4}[mode]
         spec.experiment_mask = _unsigned(experiment_mask, "experiment_mask", 32)
@@ -873,7 +898,11 @@ class FacoBatchEngine:
                    ctypes.byref(trace) if trace else None, ctypes.byref(result), tours.reshape(-1), costs,
                    work.reshape(-1), ctypes.addressof(states), request.argument()))
         del prog_arr, ctrl_arr
+        rejections = None
+        if self.constraint_mode == "hard":
+            rejections = self.colony_constraint_rejections(C)
         return {"result": result, "tours": tours, "costs": costs, "work": work, "elapsed": time.perf_counter() - started,
+                "keys": np.resize(panel_keys, C), "rejections": rejections,
                 "checkpoints": request.output() if request.points else None,
                 "profile_events_only": bool(profile_events_only),
                 "states": [_control_state_dict(st) for st in states],
@@ -890,6 +919,7 @@ class FacoBatchEngine:
     def _control_output(self, raw, decision_iterations, kind) -> dict:
         out = _batch_output(raw["result"], raw["tours"], raw["costs"], "cached", raw["elapsed"])
         out["control_states"] = raw["states"]
+        self._constraint_output(out, raw["keys"], raw["result"].completed_constraint_rejections)
         if raw.get("checkpoints"):
             out["checkpoints"] = raw["checkpoints"]
         if raw.get("profile_events_only"):
@@ -983,7 +1013,7 @@ class FacoBatchEngine:
             return lib().faco_engine_capture_control(self._handle, keys, seeds, keys.size, source, after, spec, flags,
                                                      trace, result, tours, costs, work, states, ctypes.byref(handle))
         batches = self._decision_batches(decision_iterations, after, self.settings.ants)
-        raw = self._control_run(keys.size, kind, call, experiment_mask=mask, trace_batches=batches, **decider)
+        raw = self._control_run(keys, kind, call, experiment_mask=mask, trace_batches=batches, **decider)
         state = CountedState(handle)
         evaluation = self._control_output(raw, list(decision_iterations), "program" if kind == "program" else "fixed")
         if kind == "baseline":
@@ -1001,7 +1031,9 @@ class FacoBatchEngine:
                                                       ctypes.byref(fork) if fork is not None else None, flags, trace,
                                                       result, tours, costs, work, states)
         batches = self._decision_batches(decision_iterations, additional, self.settings.ants)
-        raw = self._control_run(state.colonies, kind, call, trace_batches=batches, **decider)
+        tasks = [t["instance_key"] for t in state.describe()["tasks"]]
+        raw = self._control_run(np.resize(np.array(tasks, dtype=np.uint64), state.colonies), kind, call,
+                                trace_batches=batches, **decider)
         out = self._control_output(raw, list(decision_iterations), "program" if kind == "program" else "fixed")
         if kind == "baseline":
             out["baseline_policy"] = baseline_output(decider["baseline"])
@@ -1110,6 +1142,8 @@ class FacoBatchEngine:
                 "completed_ls_evaluations": int(work[:, 1].sum()),
                 "control_states": raw["states"][sl],
             })
+            self._constraint_output(item, raw["keys"][sl],
+                                    int(raw["rejections"][sl].sum()) if raw["rejections"] is not None else 0)
             for entry in item["items"]:
                 entry["completed_seconds"] = raw["elapsed"] / count
             if factorial_policy is not None:
@@ -1145,7 +1179,9 @@ class MixedFacoBatchEngine(FacoBatchEngine):
         s = Settings()
         check(lib().faco_engine_settings(self._handle, ctypes.byref(s)))
         self.settings = FixedFacoSettings._from_native(s)
+        self.constraint_mode = "unrestricted"
         self._charges = {}
+        self._graph_edges = {}
         self._sizes = {}
 
     def _instance_n(self, instance_key) -> int:

@@ -62,6 +62,8 @@ typedef struct faco_ant_info {
     uint32_t evaluation_limit_reached;
     uint32_t checklist_size;
     uint32_t start_node;
+    uint32_t constraint_rejections; /* Hard mode: evaluated 2-opt moves that would leave the graph */
+    uint32_t pad;
 } faco_ant_info;
 
 /* Batch result summary, the scalar fields of the reference's batch_output
@@ -80,6 +82,8 @@ typedef struct faco_result {
     double search_seconds;       /* host wall clock of the iteration loop */
     double search_gpu_milliseconds; /* device time of the iteration loop (HIP events) */
     double kernel_milliseconds[FACO_STAGE_COUNT]; /* filled when profiling is on */
+    uint64_t completed_constraint_rejections;     /* Hard mode: evaluated 2-opt moves that would leave the graph
+                                                     (reference faco_cpu.cpp:180-183) */
 } faco_result;
 
 /* Program IR v1 record (reference cpp/include/gp_faco/program.hpp:10-19). */
@@ -448,6 +452,14 @@ int faco_engine_evaluate_deadline(faco_engine* engine, const uint64_t* instance_
                                   uint32_t* tours, double* costs, uint8_t* has_incumbent, double* completed_seconds,
                                   faco_control_state* states, double* ratios, uint32_t ratio_capacity,
                                   faco_batch_profile* profile);
+
+/* Per-task Hard-mode 2-opt rejections of the last count evaluation
+ * (evaluate_faco* or a controlled count run; a continuation counts its own
+ * batches), in task order: the per-colony split of
+ * faco_result.completed_constraint_rejections (reference
+ * cuda/batch_engine.cu:432, colony_work[index][2]). `count` must equal the
+ * last evaluation's task count. */
+int faco_engine_constraint_rejections(faco_engine* engine, uint64_t* out, uint32_t count);
 
 /* set_preparation_charges(instance_key, cheap_seconds, preparation_seconds)
  * (reference batch_engine.cu:318-330, bindings.cpp:651-661): frozen

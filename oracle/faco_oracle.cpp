@@ -283,6 +283,7 @@ bool hard_two_opt_allowed(const Problem& p, u32 a, u32 a_neighbor, u32 b, u32 b_
 
 struct LsStats {
     u64 processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0;
+    u64 rejections = 0;  // Hard mode: evaluated moves that would leave the graph (faco_cpu.cpp:180-183)
     u32 limit_reached = 0;
 };
 
@@ -348,9 +349,11 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
                 const u32 closing = kind == 0 ? a_next : a_prev;
                 // Hard mode: a move that leaves the graph never wins
                 // (cuda/faco_device.cuh:383-394); it still counts as evaluated
-                const i64 gain = (nb == a || b == closing) ? 0
-                    : (!t.p->hard() || hard_two_opt_allowed(*t.p, a, closing, b, nb))
-                    ? current + t.p->d(b, nb) - ab - t.p->d(closing, nb) : std::numeric_limits<i64>::min();
+                const bool identity = nb == a || b == closing;
+                const bool allowed = identity || !t.p->hard() || hard_two_opt_allowed(*t.p, a, closing, b, nb);
+                st.rejections += allowed ? 0 : 1;
+                const i64 gain = identity ? 0
+                    : allowed ? current + t.p->d(b, nb) - ab - t.p->d(closing, nb) : std::numeric_limits<i64>::min();
                 if (gain > best) {
                     best = gain;
                     if (kind == 0) { move[0] = a_next; move[1] = nb; move[2] = a; move[3] = b; }
@@ -600,7 +603,7 @@ struct ColonyRun {
     double minimum = 0, maximum = 0, default_trail = 0, source_uniform = 0;
     u32 iteration_best = 0, source_is_epoch = 0;
     std::vector<double> trails, products, eta;
-    u64 construction_steps = 0, ls_evaluations = 0;
+    u64 construction_steps = 0, ls_evaluations = 0, constraint_rejections = 0;
 };
 
 void colony_reset(const Problem& p, ColonyRun& r) {
@@ -618,7 +621,7 @@ void colony_reset(const Problem& p, ColonyRun& r) {
         if (!p.prior.empty()) r.eta[i] *= static_cast<double>(p.prior[i]);
         r.products[i] = r.trails[i] * r.eta[i];
     }
-    r.construction_steps = r.ls_evaluations = 0;
+    r.construction_steps = r.ls_evaluations = r.constraint_rejections = 0;
     r.source_uniform = 0; r.source_is_epoch = 0; r.iteration_best = 0;
 }
 
@@ -630,6 +633,7 @@ void colony_select_and_update(const Problem& p, ColonyRun& r, const std::vector<
     for (u32 a = 0; a < ants.size(); ++a) {
         r.construction_steps += ants[a].c.steps;
         r.ls_evaluations += ants[a].ls.evaluations;
+        r.constraint_rejections += ants[a].ls.rejections;
         if (ants[a].final_cost < ants[best].final_cost) best = a;
     }
     r.iteration_best = best;
@@ -1289,6 +1293,27 @@ int oracle_two_opt(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, u32* to
     });
 }
 
+// The Hard-mode checklist 2-opt (plain 2-opt, graph given as an undirected
+// CSR) with its count of rejected moves (faco_cpu.cpp:180-183).
+int oracle_two_opt_hard(u32 n, const std::int32_t* xy, const u32* ls, u32 lw, u32* tour, const u32* checklist_in,
+                        u32 checklist_size, u64 limit, const u32* graph_offsets, const u32* graph_neighbors,
+                        std::int64_t* cost, u64* evaluations, u64* rejections) {
+    return guarded([&] {
+        check_coordinates(n, xy);
+        Problem p; p.n = n; p.xy = xy;
+        p.graph_offsets = graph_offsets; p.graph_neighbors = graph_neighbors;
+        Tour t; t.assign(tour, n, &p);
+        std::vector<u32> checklist(checklist_in, checklist_in + checklist_size);
+        std::vector<std::uint8_t> queued(n, 0);
+        for (u32 v : checklist) queued[v] = 1;
+        const LsStats st = two_opt(t, ls, lw, checklist, queued, limit, false);
+        std::copy(t.order.begin(), t.order.end(), tour);
+        *cost = t.recomputed();
+        *evaluations = st.evaluations;
+        *rejections = st.rejections;
+    });
+}
+
 // One roulette decision (select_next) with explicit inputs.
 int oracle_select_next(u32 n, const std::int32_t* xy, u32 pw, u32 bw, const u32* primary_row,
                        const u32* backup_row, const double* products, const std::uint8_t* visited,
@@ -1338,7 +1363,8 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
                     const float* multipliers /* [instances][n * pw] or null */,
                     const u32* graph_offsets /* [instances][n + 1] or null: Hard mode */,
                     const u32* graph_neighbors /* [instances][neighbor_stride] */, u32 neighbor_stride,
-                    u32 timed_from /* elapsed_search_seconds covers batches timed_from.. only */) {
+                    u32 timed_from /* elapsed_search_seconds covers batches timed_from.. only */,
+                    u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */) {
     return guarded([&] {
         if (!instances || !colonies) throw std::invalid_argument("empty panel");
         if (graph_offsets && local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
@@ -1406,6 +1432,7 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             global_costs[c] = runs[c].global_cost;
             work[2 * c] = runs[c].construction_steps;
             work[2 * c + 1] = runs[c].ls_evaluations;
+            if (rejections) rejections[c] = runs[c].constraint_rejections;
         }
     });
 }
@@ -1544,7 +1571,8 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           const u32* graph_neighbors, u32 neighbor_stride,
                           const float* multipliers /* [instances][n * pw] or null */,
                           const double* progress /* [batches]: the progress feature of each batch, or null:
-                                                    completed FE / evaluation_limit */) {
+                                                    completed FE / evaluation_limit */,
+                          u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */) {
     // stop_batch (0 = the whole budget) ends the run early with the progress
     // feature still measured against evaluation_limit. At batch fork_batch
     // (UINT32_MAX = none) a paired fork (reference cuda/batch_engine.cu:839-845,
@@ -1672,6 +1700,7 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
             global_costs[c] = runs[c].global_cost;
             work[2 * c] = runs[c].construction_steps;
             work[2 * c + 1] = runs[c].ls_evaluations;
+            if (rejections) rejections[c] = runs[c].constraint_rejections;
             if (states) {
                 const Control& k = controls[c];
                 states[c] = ControlOut{static_cast<u32>(k.archive.size()), 0, k.fb.stagnant, k.fb.epoch_batches,

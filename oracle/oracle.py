@@ -191,7 +191,8 @@ def lib():
                                       ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p,
-                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_void_p]
         L.oracle_prior_features.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, f32p]
         L.oracle_escape_slot_view.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
@@ -248,7 +249,8 @@ def lib():
                                             u32p, i64p, u64p, ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+                                            ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]
         L.oracle_colony_select.argtypes = [ctypes.c_uint32, i32p, ctypes.POINTER(Settings), ctypes.c_uint64,
                                            ctypes.c_uint32, ctypes.c_uint32, u32p, i64p, i64p, u32p, f64p, f64p,
                                            f64p, f64p, ctypes.POINTER(ctypes.c_uint32),
@@ -259,6 +261,9 @@ def lib():
         L.oracle_two_opt.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.c_uint32]
+        L.oracle_two_opt_hard.argtypes = [ctypes.c_uint32, i32p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
+                                          ctypes.c_uint64, u32p, u32p, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -381,6 +386,26 @@ def two_opt(xy, ls, tour, checklist, limit, or_opt=False):
     return tour, cost.value, evals.value
 
 
+def two_opt_hard(xy, ls, tour, checklist, limit, graph):
+    """Hard-mode checklist 2-opt inside `graph` (csr()): (tour, cost,
+    evaluations, rejected moves)."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
+    n = xy.size // 2
+    ls = np.ascontiguousarray(ls, dtype=np.uint32)
+    tour = np.ascontiguousarray(tour, dtype=np.uint32).copy()
+    checklist = np.ascontiguousarray(checklist, dtype=np.uint32)
+    offsets = np.ascontiguousarray(graph[0], dtype=np.uint32)
+    neighbors = np.ascontiguousarray(graph[1], dtype=np.uint32)
+    if not neighbors.size:
+        neighbors = np.zeros(1, dtype=np.uint32)
+    cost, evals, rejected = ctypes.c_int64(), ctypes.c_uint64(), ctypes.c_uint64()
+    L = lib()
+    _check(L.oracle_two_opt_hard(n, xy, ls.reshape(-1), ls.shape[1], tour, checklist, checklist.size, limit,
+                                 offsets, neighbors, ctypes.byref(cost), ctypes.byref(evals),
+                                 ctypes.byref(rejected)), L)
+    return tour, cost.value, evals.value, rejected.value
+
+
 def select_next(xy, primary_row, backup_row, products, visited, current, u):
     xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
     n = xy.size // 2
@@ -441,6 +466,7 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
     tours = np.zeros((colonies, n), dtype=np.uint32)
     costs = np.zeros(colonies, dtype=np.int64)
     work = np.zeros((colonies, 2), dtype=np.uint64)
+    rejections = np.zeros(colonies, dtype=np.uint64)
     elapsed = ctypes.c_double()
     trace_parent = trace_trails = trace_costs = None
     if trace:
@@ -469,8 +495,9 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
                              work.reshape(-1), ptr(trace_parent), ptr(trace_trails), ptr(trace_costs),
                              ctypes.byref(elapsed), int(bool(or_opt)),
                              ptr(None if multipliers is None else mult), ptr(g_offsets), ptr(g_neighbors), stride,
-                             int(timed_from)), L)
-    out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value}
+                             int(timed_from), ptr(rejections)), L)
+    out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
+           "constraint_rejections": rejections}
     if trace:
         out.update(trace_parent=trace_parent, trace_trails=trace_trails, trace_costs=trace_costs)
     return out
@@ -660,6 +687,11 @@ def ref():
         R.ref_select_next.argtypes = [ctypes.c_uint32, f64p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, f64p,
                                       u8p, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32),
                                       ctypes.POINTER(ctypes.c_int)]
+        R.ref_two_opt_hard.argtypes = [ctypes.c_uint32, f64p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32,
+                                       ctypes.c_uint64, u32p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), u64p]
+        R.ref_select_next_hard.argtypes = [ctypes.c_uint32, f64p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, f64p,
+                                           u8p, ctypes.c_uint32, ctypes.c_double, u32p, u32p, ctypes.c_uint64,
+                                           ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int)]
         R.ref_trail_limits.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         R.ref_pheromone_run.argtypes = [ctypes.c_uint32, f64p, u32p, ctypes.c_uint32, ctypes.c_double,
@@ -715,6 +747,46 @@ def ref_select_next(matrix, primary_row, backup_row, products, visited, current,
                              np.ascontiguousarray(products, dtype=np.float64),
                              np.ascontiguousarray(visited, dtype=np.uint8), current, u,
                              ctypes.byref(chosen), ctypes.byref(stage)), R)
+    return chosen.value, stage.value
+
+
+def _edge_array(edges):
+    e = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
+    return (e if e.size else np.zeros((1, 2), dtype=np.uint32)), len(edges)
+
+
+def ref_two_opt_hard(matrix, ls, tour, checklist, limit, edges):
+    """The reference's checklist_two_opt inside the graph of `edges` (its
+    SparseUndirectedGraph): (tour, cost, counters) with counters processed,
+    checks, evaluations, accepted, reactivations, constraint rejections,
+    limit reached."""
+    matrix = np.ascontiguousarray(matrix, dtype=np.float64)
+    n = matrix.shape[0]
+    tour = np.ascontiguousarray(tour, dtype=np.uint32).copy()
+    checklist = np.ascontiguousarray(checklist, dtype=np.uint32)
+    ls = np.ascontiguousarray(ls, dtype=np.uint32)
+    e, count = _edge_array(edges)
+    cost, counters = ctypes.c_double(), np.zeros(7, dtype=np.uint64)
+    R = ref()
+    _check(R.ref_two_opt_hard(n, matrix.reshape(-1), ls.reshape(-1), ls.shape[1], tour,
+                              checklist if checklist.size else np.zeros(1, np.uint32), checklist.size, limit,
+                              e.reshape(-1), count, ctypes.byref(cost), counters), R)
+    return tour, cost.value, counters.tolist()
+
+
+def ref_select_next_hard(matrix, primary_row, backup_row, products, visited, current, u, tour, edges):
+    matrix = np.ascontiguousarray(matrix, dtype=np.float64)
+    chosen, stage = ctypes.c_uint32(), ctypes.c_int()
+    primary_row = np.ascontiguousarray(primary_row, dtype=np.uint32)
+    backup_row = np.ascontiguousarray(backup_row, dtype=np.uint32)
+    e, count = _edge_array(edges)
+    R = ref()
+    _check(R.ref_select_next_hard(matrix.shape[0], matrix.reshape(-1), primary_row.size, backup_row.size,
+                                  primary_row, backup_row if backup_row.size else np.zeros(1, np.uint32),
+                                  np.ascontiguousarray(products, dtype=np.float64),
+                                  np.ascontiguousarray(visited, dtype=np.uint8), current, u,
+                                  np.ascontiguousarray(tour, dtype=np.uint32), e.reshape(-1), count,
+                                  ctypes.byref(chosen), ctypes.byref(stage)), R)
     return chosen.value, stage.value
 
 
@@ -886,6 +958,7 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
     tours = np.zeros((colonies, n), dtype=np.uint32)
     costs = np.zeros(colonies, dtype=np.int64)
     work = np.zeros((colonies, 2), dtype=np.uint64)
+    rejections = np.zeros(colonies, dtype=np.uint64)
     states = (ControlOut * colonies)()
     elapsed = ctypes.c_double()
     norm = prepare(xy_instances[0], settings)["settings"]
@@ -933,8 +1006,9 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
         ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt)),
         stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1,
-        ptr(g_off), ptr(g_nb), g_stride, ptr(mult), ptr(prog)), L)
+        ptr(g_off), ptr(g_nb), g_stride, ptr(mult), ptr(prog), ptr(rejections)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
+           "constraint_rejections": rejections,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
                        "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,
                        "ls_work": s.ls_work} for s in states]}

@@ -13,6 +13,7 @@
 #include "gp_faco/faco_cpu.hpp"
 #include "gp_faco/factorial_policy.hpp"
 #include "gp_faco/program.hpp"
+#include "gp_faco/sparse_graph.hpp"
 
 #include <algorithm>
 #include <cstdint>
@@ -89,6 +90,56 @@ int ref_two_opt(Node n, const double* matrix, const std::uint32_t* ls, Node lw, 
         std::copy(t.order().begin(), t.order().end(), tour);
         *cost = t.recomputed_cost();
         *evaluations = stats.move_evaluations;
+    });
+}
+
+// Hard mode on the reference's own predicate path: checklist_two_opt with
+// its SparseUndirectedGraph (built from the edge list) as EdgeAllowed
+// (cpp/src/faco_cpu.cpp:180-183). counters: processed, checks, evaluations,
+// accepted, reactivations, constraint rejections, limit reached.
+int ref_two_opt_hard(Node n, const double* matrix, const std::uint32_t* ls, Node lw, std::uint32_t* tour,
+                     const std::uint32_t* checklist_in, Node checklist_size, std::uint64_t limit,
+                     const std::uint32_t* edges, std::uint64_t edge_count, double* cost, std::uint64_t* counters) {
+    return guarded([&] {
+        const auto distance = matrix_distance(matrix, n);
+        std::vector<gp_faco::Edge> list(edge_count);
+        for (std::uint64_t i = 0; i < edge_count; ++i) list[i] = {edges[2 * i], edges[2 * i + 1]};
+        const gp_faco::SparseUndirectedGraph graph(n, list);
+        gp_faco::CpuTour t(std::vector<Node>(tour, tour + n), distance);
+        std::vector<Node> checklist(checklist_in, checklist_in + checklist_size);
+        gp_faco::DistanceOrderedCandidates candidates(rows_of(ls, n, lw), distance);
+        const auto stats = t.checklist_two_opt(candidates, checklist, limit, graph.view());
+        std::copy(t.order().begin(), t.order().end(), tour);
+        *cost = t.recomputed_cost();
+        const std::uint64_t values[7] = {stats.processed_nodes, stats.candidate_checks, stats.move_evaluations,
+            stats.accepted_moves, stats.reactivations, stats.constraint_rejections, stats.evaluation_limit_reached};
+        std::copy(values, values + 7, counters);
+    });
+}
+
+// Hard-mode select_next: a city is available only when its relocation after
+// `current` keeps `tour` inside the graph, the reference's node predicate
+// (cuda/faco_choices.cuh:46-53 over edge_constraints.hpp relocation_allowed).
+int ref_select_next_hard(Node n, const double* matrix, Node pw, Node bw, const std::uint32_t* primary_row,
+                         const std::uint32_t* backup_row, const double* products, const std::uint8_t* visited,
+                         Node current, double uniform, const std::uint32_t* tour, const std::uint32_t* edges,
+                         std::uint64_t edge_count, std::uint32_t* chosen, int* stage) {
+    return guarded([&] {
+        const auto distance = matrix_distance(matrix, n);
+        std::vector<gp_faco::Edge> list(edge_count);
+        for (std::uint64_t i = 0; i < edge_count; ++i) list[i] = {edges[2 * i], edges[2 * i + 1]};
+        const gp_faco::SparseUndirectedGraph graph(n, list);
+        std::vector<Node> positions(n);
+        for (Node i = 0; i < n; ++i) positions[tour[i]] = i;
+        const auto view = graph.view();
+        const auto allowed = [&](Node node) {
+            return gp_faco::relocation_allowed(tour, positions.data(), n, current, node, view);
+        };
+        const auto result = gp_faco::select_next(current, std::vector<Node>(primary_row, primary_row + pw),
+            std::vector<double>(products, products + pw), std::vector<Node>(backup_row, backup_row + bw),
+            std::vector<std::uint8_t>(visited, visited + n), distance, uniform, allowed);
+        *chosen = result.node;
+        *stage = static_cast<int>(result.stage);
     });
 }
 

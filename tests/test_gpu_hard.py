@@ -70,6 +70,12 @@ def test_hard_mode_matches_the_oracle_and_stays_in_the_graph(ext, oracle_mod, n,
         assert _cycle(tour) <= members[c % count], c
     np.testing.assert_array_equal(got["colony_work"], want["work"])
     assert got["colony_work"].tolist() != ref["colony_work"].tolist()   # the graph changes the search
+    # the reference's Hard result keys; rejections per colony and in total equal the oracle's
+    assert got["constraint_mode"] == "hard" and got["graph_spec_id"] == 1
+    assert got["graph_edges_per_colony"] == [len(members[c % count]) for c in range(4)]
+    assert got["reserved_escape_device_bytes"] == 0 and "constraint_mode" not in ref
+    np.testing.assert_array_equal(hard.colony_constraint_rejections(4), want["constraint_rejections"])
+    assert got["completed_constraint_rejections"] == int(want["constraint_rejections"].sum()) > 0
 
 
 @pytest.mark.parametrize("mode", ["program", "baseline"])
@@ -109,6 +115,8 @@ def test_controlled_hard_runs_match_the_oracle(ext, oracle_mod, mode):
     np.testing.assert_array_equal(raw["work"], want["work"])
     np.testing.assert_array_equal(raw["trace_actions"], want["trace_actions"])
     assert [st["restarts"] for st in raw["states"]] == [st["restarts"] for st in want["states"]]
+    np.testing.assert_array_equal(raw["rejections"], want["constraint_rejections"])
+    assert raw["result"].completed_constraint_rejections == int(want["constraint_rejections"].sum()) > 0
 
 
 def test_hard_registration_rules(ext):
@@ -137,3 +145,25 @@ def test_hard_registration_rules(ext):
         hard.set_local_search("2opt+oropt")
     with pytest.raises(ValueError):
         ext.FacoBatchEngine(n, 2, s, constraint_mode="escape")
+
+
+
+def test_hard_engines_take_evaluation_counts_only(ext):
+    """The reference's Hard engine refuses time budgets, whose cheap tour may
+    leave the graph (cuda/batch_engine.cu:539-540)."""
+    n = 150
+    xy = uniform_instance(n, seed=53)
+    s = ext.FixedFacoSettings()
+    s.ants = 16
+    plain = ext.FacoBatchEngine(n, 2, s)
+    plain.register_problem(7, xy)
+    spec, _ = _spec(plain.problem_data(7), [])
+    hard = ext.FacoBatchEngine(n, 2, s, constraint_mode="hard")
+    hard.register_graph_problem(7, xy, spec)
+    keys, seeds = np.array([7, 7], dtype=np.uint64), np.array([1, 2], dtype=np.uint64)
+    announce("hard-deadline")
+    with pytest.raises(ValueError):
+        hard.evaluate(keys, seeds, 0.2, 8)
+    out = hard.evaluate_faco_evaluations(keys, seeds, s.ants * 3)
+    assert out["constraint_mode"] == "hard" and out["completed_constraint_rejections"] == int(
+        hard.colony_constraint_rejections(2).sum())

@@ -9,6 +9,7 @@
 - input validation raises ValueError/TypeError like the reference module.
 """
 import ctypes
+import subprocess
 import json
 import re
 import struct
@@ -41,11 +42,35 @@ def test_library_exports_every_declared_symbol():
     ext.lib()  # every ctypes signature binds
 
 
-def test_record_layouts():
+def test_record_layouts(tmp_path):
     assert ctypes.sizeof(_native.Settings) == 64        # FixedFacoSettings
     assert ctypes.sizeof(_native.Program) == 388        # Program IR v1
-    assert ctypes.sizeof(_native.AntInfo) == 64
-    assert ctypes.sizeof(_native.Result) == 8 * 8 + 8 * 3 + 8 * _native.STAGE_COUNT
+    assert ctypes.sizeof(_native.Controller) == 1168    # ControllerProgram
+    assert ctypes.sizeof(_native.BaselinePolicy) == 48  # BaselinePolicy
+    # every ctypes mirror has the size and field offsets the C compiler gives the header's record
+    mirrors = {"faco_settings": _native.Settings, "faco_ant_info": _native.AntInfo, "faco_result": _native.Result,
+               "faco_program": _native.Program, "faco_controller": _native.Controller,
+               "faco_baseline_policy": _native.BaselinePolicy, "faco_control_state": _native.ControlState,
+               "faco_control_spec": _native.ControlSpec, "faco_deadline_spec": _native.DeadlineSpec,
+               "faco_deadline_result": _native.DeadlineResult, "faco_batch_profile": _native.BatchProfile,
+               "faco_fork": _native.Fork, "faco_tune_entry": _native.TuneEntry}
+    lines = ['#include <cstddef>', '#include <cstdio>', f'#include "{ROOT / "include" / "faco_engine.h"}"',
+             'int main() {']
+    for c_name, mirror in mirrors.items():
+        lines.append(f'    std::printf("{c_name} %zu\\n", sizeof({c_name}));')
+        for field, _ in mirror._fields_:
+            lines.append(f'    std::printf("{c_name}.{field} %zu\\n", offsetof({c_name}, {field}));')
+    lines.append('    return 0;\n}')
+    source = tmp_path / "layout.cpp"
+    source.write_text("\n".join(lines))
+    binary = tmp_path / "layout"
+    subprocess.run(["g++", "-std=c++17", str(source), "-o", str(binary)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(binary)], check=True, capture_output=True,
+                                                               text=True).stdout.splitlines())
+    for c_name, mirror in mirrors.items():
+        assert int(got[c_name]) == ctypes.sizeof(mirror), c_name
+        for field, _ in mirror._fields_:
+            assert int(got[f"{c_name}.{field}"]) == getattr(mirror, field).offset, (c_name, field)
 
 
 @pytest.mark.parametrize("n,grid,widths", [(5, 10000, (16, 64, 20)), (60, 9, (16, 64, 20)),

@@ -19,10 +19,17 @@ a 2-opt move leaving the graph never wins): selection exhausts instead of
 leaving the graph (the reference's 6-city regression), a run on the
 complete graph equals plain FACO bit for bit, and every tour of a run on the
 graph of primary candidates plus the initial tour stays inside that graph.
+
+Rejected moves (faco_cpu.cpp:180-183): the count of one city's scan equals
+the evaluated moves whose tour after the move leaves the graph (edge sets
+again). Where the reference checker oracle/_ref is built, the Hard 2-opt
+(tour, cost, evaluations, rejections) and the Hard selection equal the
+reference core's, given its own SparseUndirectedGraph as the predicate.
 """
 import itertools
 
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 
@@ -158,3 +165,123 @@ def test_hard_runs_stay_in_the_graph_and_equal_plain_faco_on_the_complete_graph(
     for c in range(seeds.size):
         assert cycle_edges(hard["tours"][c].tolist()) <= members[c % count], c
     assert hard["work"].tolist() != plain["work"].tolist()     # the restriction changes the search
+    # a rejection is an evaluated move, and the complete graph rejects nothing
+    assert same["constraint_rejections"].tolist() == [0] * seeds.size
+    assert all(0 < hard["constraint_rejections"][c] <= hard["work"][c][1] for c in range(seeds.size))
+
+
+def _dist(xy, a, b):
+    dx, dy = float(xy[a][0] - xy[b][0]), float(xy[a][1] - xy[b][1])
+    return int((dx * dx + dy * dy) ** 0.5 + 0.5)
+
+
+def test_hard_two_opt_rejections_are_the_moves_that_leave_the_graph():
+    """One city's 2-opt scan (the budget ends the run after it): the moves it
+    evaluates are the LS neighbours b nearer to a than a's successor (kind 0)
+    or predecessor (kind 1), in row order; the oracle's rejection count is the
+    number of them whose tour after the move (carried out on a list) has an
+    edge outside the graph."""
+    rng = np.random.default_rng(23)
+    cases = rejected_total = 0
+    for trial in range(60):
+        n = int(rng.integers(8, 30))
+        xy = rng.integers(0, 1000, size=(n, 2)).astype(np.int32)
+        lw = min(8, n - 1)
+        ls = np.array([[b for b in sorted(range(n), key=lambda b: (_dist(xy, a, b), b)) if b != a][:lw]
+                       for a in range(n)], dtype=np.uint32)
+        tour = [0, *rng.permutation(np.arange(1, n)).tolist()]
+        a = int(rng.integers(0, n))
+        i = tour.index(a)
+        edges = cycle_edges(tour) | {tuple(sorted((u, int(v)))) for u in range(n) for v in ls[u]
+                                     if rng.random() < 0.5}
+        graph = O.csr(sorted(edges), n)
+        evaluated = rejected = 0
+        for kind in (0, 1):
+            current = _dist(xy, a, tour[(i + 1) % n]) if kind == 0 else _dist(xy, tour[i - 1], a)
+            for b in ls[a].tolist():
+                if not current > _dist(xy, a, b):
+                    break
+                evaluated += 1
+                rejected += not cycle_edges(two_opt_after(tour, a, b, kind)) <= edges
+        if evaluated == 0:
+            continue
+        _, _, evals, got = O.two_opt_hard(xy, ls, tour, [a], evaluated, graph)
+        assert (evals, got) == (evaluated, rejected), (trial, n, a)
+        cases += 1
+        rejected_total += rejected
+    assert cases > 30 and rejected_total > 0
+
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference checker oracle/_ref not built here")
+
+
+def _random_graph(rng, n, tour, ls, share):
+    """The tour's edges, a share of the LS row edges and a few random chords."""
+    edges = set(cycle_edges(tour))
+    for a in range(n):
+        for b in ls[a]:
+            if rng.random() < share:
+                edges.add(tuple(sorted((a, int(b)))))
+    for _ in range(n // 4):
+        a, b = (int(v) for v in rng.choice(n, 2, replace=False))
+        edges.add(tuple(sorted((a, b))))
+    return sorted(edges)
+
+
+@needs_ref
+def test_hard_two_opt_against_reference_core():
+    """The oracle's Hard 2-opt equals the reference's checklist_two_opt given
+    its own SparseUndirectedGraph as the edge predicate: tour, cost, move
+    evaluations and constraint rejections, over random sparse graphs, tours,
+    checklists and budgets (cut short and unbounded)."""
+    rng = np.random.default_rng(31)
+    rejected = limited = 0
+    for trial in range(150):
+        n = int(rng.integers(6, 60))
+        xy = rng.integers(0, 2000 if trial % 3 else 12, size=(n, 2)).astype(np.int32)
+        M = O.distance_matrix(xy)
+        lw = min(int(rng.integers(3, 12)), n - 1)
+        ls = np.array([sorted((b for b in range(n) if b != a), key=lambda b: (M[a, b], b))[:lw] for a in range(n)],
+                      dtype=np.uint32)
+        tour = rng.permutation(n).tolist()
+        edges = _random_graph(rng, n, tour, ls, float(rng.choice([0.0, 0.3, 0.7])))
+        checklist = rng.permutation(n)[:int(rng.integers(1, n + 1))]
+        limit = int(rng.choice([3, 17, 2**64 - 1]))
+        got_tour, got_cost, evals, rejections = O.two_opt_hard(xy, ls, tour, checklist, limit, O.csr(edges, n))
+        ref_tour, ref_cost, counters = O.ref_two_opt_hard(M.astype(np.float64), ls, tour, checklist, limit, edges)
+        assert got_tour.tolist() == ref_tour.tolist(), trial
+        assert got_cost == ref_cost and evals == counters[2] and rejections == counters[5], trial
+        assert cycle_edges(got_tour.tolist()) <= set(edges) | cycle_edges(tour)
+        rejected += rejections
+        limited += counters[6]
+    assert rejected > 100 and limited > 10
+
+
+@needs_ref
+def test_hard_select_next_against_reference_core():
+    """The oracle's Hard select_next equals the reference's select_next with
+    the relocation predicate as its node filter (every stage, exhaustion
+    included)."""
+    rng = np.random.default_rng(37)
+    stages = set()
+    for k in range(400):
+        n = int(rng.integers(5, 24))
+        xy = rng.integers(0, 60, size=(n, 2)).astype(np.int32)
+        M = O.distance_matrix(xy)
+        tour = rng.permutation(n).tolist()
+        cur = int(rng.integers(0, n))
+        order = sorted((b for b in range(n) if b != cur), key=lambda b: (M[cur, b], b))
+        pw = min(int(rng.integers(2, 8)), n - 1)
+        bw = min(int(rng.integers(0, 5)), n - 1 - pw)
+        ls = np.array([sorted((b for b in range(n) if b != a), key=lambda b: (M[a, b], b))[:pw] for a in range(n)])
+        edges = _random_graph(rng, n, tour, ls, float(rng.choice([0.0, 0.2, 0.6, 1.0])))
+        visited = (rng.random(n) < rng.random()).astype(np.uint8)
+        visited[cur] = 1
+        products = rng.random(pw) * (k % 5 != 0)
+        u = float(rng.random())
+        got = O.select_next_hard(xy, order[:pw], order[pw:pw + bw], products, visited, cur, u, tour, O.csr(edges, n))
+        want = O.ref_select_next_hard(M.astype(np.float64), order[:pw], order[pw:pw + bw], products, visited, cur, u,
+                                      tour, edges)
+        assert got == want, k
+        stages.add(got[1])
+    assert stages == {0, 1, 2, 3, 4}
