@@ -85,6 +85,7 @@ class Result(ctypes.Structure):
         ("search_gpu_milliseconds", ctypes.c_double),
         ("kernel_milliseconds", ctypes.c_double * STAGE_COUNT),
         ("completed_constraint_rejections", ctypes.c_uint64),
+        ("persistent_launches", ctypes.c_uint64),
     ]
 
 

@@ -193,6 +193,7 @@ struct Engine {
         DeviceArray<uint32_t> masks, alternatives, regions;  // [C], [C], [C][128]
         DeviceArray<int32_t> actions;               // [C]
         DeviceArray<uint64_t> source_keys;          // [C] keys before a state fork
+        DeviceArray<ControlParams> params;          // [1] the persistent controlled kernel's parameters
     };
     std::unique_ptr<ControlBuffers> ctl;
     bool resident_control = false;
@@ -203,6 +204,8 @@ struct Engine {
     uint32_t continuable_mne = 0;
     // Hard-mode 2-opt rejections of each colony in the last count evaluation
     std::vector<uint64_t> last_rejections;
+    // launches of the persistent kernels so far (results report the delta)
+    uint64_t persistent_launch_count = 0;
     // frozen preparation charges per instance key (cheap, preparation
     // seconds) billed by cached_charged time-budget evaluations
     std::map<uint64_t, std::pair<double, double>> charges;
@@ -262,6 +265,7 @@ struct Engine {
         alloc(b->features, 12 * 32 * C); alloc(b->scores, 32 * C);
         alloc(b->masks, C); alloc(b->alternatives, C); alloc(b->regions, 128 * C); alloc(b->actions, C);
         alloc(b->source_keys, C);
+        alloc(b->params, 1);
         ctl = std::move(b);
         resident.clear();  // the control geometry is uploaded with the next geometry upload
     }
@@ -457,6 +461,7 @@ struct Engine {
                 const uint64_t rounds = std::min(iterations - done, kRoundsPerLaunch);
                 checked(hipEventRecord(ev[0], stream), "event");
                 checked(launch_colony_run(ap, cp, colonies, static_cast<uint32_t>(rounds), stream), "colony run launch");
+                ++persistent_launch_count;
                 checked(hipEventRecord(ev[1], stream), "event");
                 checked(hipEventSynchronize(ev[1]), "event sync");
                 float a = 0;
@@ -517,6 +522,7 @@ struct Engine {
             for (uint64_t done = 0; done < iterations;) {
                 const uint64_t rounds = std::min(iterations - done, kRoundsPerLaunch);
                 checked(launch_colony_run(ap, cp, colonies, static_cast<uint32_t>(rounds), stream), "colony run launch");
+                ++persistent_launch_count;
                 done += rounds;
             }
             return;
@@ -632,6 +638,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
               uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
               uint64_t limit, bool any_count = false, const faco_checkpoints* checkpoints = nullptr) {
     e.last_rejections.clear();
+    const uint64_t launches_before = e.persistent_launch_count;
     const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
     const bool events = (flags & FACO_RUN_EVENTS) != 0, resume = (flags & FACO_RUN_CONTINUE) != 0;
     double kernel_ms[FACO_STAGE_COUNT] = {0};
@@ -701,6 +708,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
     if (result) {
         std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
         result->completed_constraint_rejections = rejections;
+        result->persistent_launches = e.persistent_launch_count - launches_before;
     }
     if (!e.mixed) { e.continuable = std::move(keys); e.continuable_mne = mne; }
 }
@@ -853,6 +861,14 @@ ControlParams control_params(Engine& e, uint32_t panel, const faco_control_spec&
     return cp;
 }
 
+// Controlled batches run as one persistent launch when every colony is one
+// workgroup of 2..12 waves (the fused kernel's launch bounds), the engine is
+// unrestricted and the control bodies' LDS fits beside the ants' shared data.
+bool fused_control(const AntParams& ap) {
+    return ap.blocks_per_colony == 1 && !ap.graph_offsets && ap.waves >= 2 && ap.waves <= 12 &&
+           control_run_lds_bytes(ap) + 256u <= max_lds_per_block();
+}
+
 // The ant kernel of a controlled run: every ant's tour and identity are
 // written (the archive update reads them) and ants start in the region the
 // decision chose.
@@ -870,6 +886,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
                       faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
                       faco_control_state* states, const faco_checkpoints* checkpoints, const StateIO* io = nullptr) {
     e.last_rejections.clear();
+    const uint64_t launches_before = e.persistent_launch_count;
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
     require(lock.owns_lock(), "one evaluation per engine at a time");
@@ -1066,10 +1083,15 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
             }
         }
     } else {
-        // segments between checkpoints; each replays a captured graph of 16
-        // batches where it can and runs the rest eagerly
+        // segments between checkpoints. With one workgroup per colony each
+        // segment is one persistent launch (decision, ants, colony update and
+        // archive update per batch in the colony's workgroup); otherwise it
+        // replays a captured graph of 16 batches where it can and runs the
+        // rest eagerly
         constexpr uint32_t kGraphIterations = 16;
         const bool graphs = (flags & FACO_RUN_GRAPH) != 0 && !events;
+        const bool fused = graphs && fused_control(ap);
+        if (fused) b.params.upload(&cp, 1, e.stream);
         hipGraphExec_t exec = nullptr;
         struct ExecGuard { hipGraphExec_t& x; ~ExecGuard() { if (x) (void)hipGraphExecDestroy(x); } } guard{exec};
         uint64_t done = 0;
@@ -1077,6 +1099,13 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         while (done < batches) {
             const uint64_t stop = std::max(marks.next(batches), done + 1);
             uint64_t left = stop - done;
+            for (; fused && left;) {
+                const uint64_t rounds = std::min(left, Engine::kRoundsPerLaunch);
+                checked(launch_control_run(ap, colony, cp, b.params.data(), C, static_cast<uint32_t>(rounds), e.stream),
+                        "control run launch");
+                ++e.persistent_launch_count;
+                left -= rounds;
+            }
             if (graphs && left >= kGraphIterations) {
                 if (!exec) {
                     e.drop_graph();
@@ -1168,6 +1197,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     if (result) {
         std::copy(kernel_ms, kernel_ms + FACO_STAGE_COUNT, result->kernel_milliseconds);
         result->completed_constraint_rejections = rejections;
+        result->persistent_launches = e.persistent_launch_count - launches_before;
     }
 }
 

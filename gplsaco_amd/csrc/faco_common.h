@@ -34,6 +34,13 @@ struct ColonyState {
     uint64_t best_key;     // smallest (final cost << 32 | ant) of this batch's ants so far; all ones between batches
 };
 
+// MMAS bounds from the epoch cost (reference cuda/colony_state.cuh:66-70).
+FACO_HD void bounds(ColonyState& s, uint32_t width, double retention, double p) {
+    s.maximum = 1.0 / (static_cast<double>(s.epoch_cost) * (1.0 - retention));
+    const double lo = s.maximum * (1.0 - p) / ((width - 1.0) * p);
+    s.minimum = s.maximum < lo ? s.maximum : lo;
+}
+
 // Per-colony control state of the GP / baseline controlled loop (the
 // reference's ControllerState, cpp/include/gp_faco/control_ops.hpp:160-173,
 // plus this batch's decision). Archive tours live in a two-buffer ring
@@ -389,6 +396,14 @@ hipError_t launch_colony_update(const ColonyParams& p, uint32_t colonies, hipStr
 hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_t colonies, uint32_t rounds,
                              hipStream_t stream);
 hipError_t launch_colony_reset(const ResetParams& p, uint32_t colonies, hipStream_t stream);
+// `rounds` batches of GP-controlled FACO in one launch: per colony one
+// workgroup runs decision, ants, colony update and archive update batch
+// after batch (blocks_per_colony must be 1; no traces, forks or diagnostics).
+// `device_ctl` is a copy of `ctl` in device memory (the kernel reads it).
+hipError_t launch_control_run(const AntParams& p, const ColonyParams& cp, const ControlParams& ctl,
+                              const ControlParams* device_ctl, uint32_t colonies, uint32_t rounds,
+                              hipStream_t stream);
+uint32_t control_run_lds_bytes(const AntParams& p);
 hipError_t launch_distance_check(const int32_t* pairs, uint32_t count, int64_t* out, hipStream_t stream);
 hipError_t launch_philox_check(uint64_t key, uint32_t batch, uint32_t ant, uint32_t step, uint32_t words,
                                uint32_t* out, hipStream_t stream);

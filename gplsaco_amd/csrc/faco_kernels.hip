@@ -19,6 +19,7 @@
 //   evaporate/deposit/clamp/product pheromone update over the colony's
 //   [n][primary] entries (each entry owned by one thread, no atomics).
 //   Semantics: reference cuda/colony_state.cuh:148-229.
+#include "control_device.h"
 #include "faco_common.h"
 
 namespace faco {
@@ -130,13 +131,6 @@ __device__ __forceinline__ double row_scan4(double v) {
     acc = acc + row_shifted<4>(acc);
     acc = acc + row_shifted<8>(acc);
     return acc;
-}
-
-// MMAS bounds from the epoch cost (reference cuda/colony_state.cuh:66-70).
-__device__ __forceinline__ void bounds(ColonyState& s, uint32_t width, double retention, double p) {
-    s.maximum = 1.0 / (static_cast<double>(s.epoch_cost) * (1.0 - retention));
-    const double lo = s.maximum * (1.0 - p) / ((width - 1.0) * p);
-    s.minimum = s.maximum < lo ? s.maximum : lo;
 }
 
 // Colony update of one colony by the calling workgroup (threads tid of
@@ -273,9 +267,14 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
 // `rounds` whole batches of its colony, each followed by the colony update
 // (update_colony) in the same workgroup, keeping the colony state in
 // `s_state` (LDS); otherwise one batch and the update is a separate kernel.
-template<bool NbrLds, bool Profile, bool Diag, bool OrOpt, bool Persistent, bool Hard = false>
+// Controlled (persistent only): each batch is preceded by the control
+// decision and followed by the archive / feedback update of `ctl`, both in
+// the same workgroup, on the LDS the ants leave free between batches.
+template<bool NbrLds, bool Profile, bool Diag, bool OrOpt, bool Persistent, bool Hard = false,
+         bool Controlled = false>
 __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParams& cp, uint32_t rounds,
-                                           ColonyState* s_state) {
+                                           ColonyState* s_state, const ControlParams* ctl = nullptr) {
+    static_assert(!Controlled || Persistent, "the fused control runs in the persistent kernel");
     // Diagnostic features (explicit visits, per-ant parents, traces and
     // logs) exist only in the Diag instantiation: the production kernel
     // keeps none of their pointers or flags live.
@@ -409,6 +408,15 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     for (uint32_t round = 0;; ++round) {
     if constexpr (Persistent) {
         if (round == rounds || (p.shapes && s_state->batch >= p.shapes[colony].batches)) break;
+    }
+    if constexpr (Controlled) {
+        // this batch's decision: features, scoring, action, restart (which
+        // also refreshes the LDS parent links and positions)
+        control_decide(*ctl, colony, nblocks, threadIdx.x, blockDim.x, *reinterpret_cast<DecideShared*>(lds + shared_bytes),
+                       s_plinks, with_ppos ? s_ppos : nullptr);
+        __syncthreads();
+        if (threadIdx.x == 0) *s_state = p.state[colony];
+        __syncthreads();
     }
     const uint32_t batch = explicit_mode ? 0u : (Persistent ? s_state->batch : p.state[colony].batch);
     for (;;) {
@@ -1259,6 +1267,11 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         __syncthreads();
         update_colony(cp, colony, threadIdx.x, blockDim.x, reinterpret_cast<uint64_t*>(lds + shared_bytes),
                       *s_state, s_plinks, with_ppos ? s_ppos : nullptr);
+        if constexpr (Controlled) {
+            __syncthreads();
+            control_update(*ctl, colony, threadIdx.x, blockDim.x, *reinterpret_cast<UpdateShared*>(lds + shared_bytes),
+                           lds + shared_bytes + round16(static_cast<uint32_t>(sizeof(UpdateShared))));
+        }
         if (threadIdx.x == 0) next_slot = 0;
         __syncthreads();
     }
@@ -1283,6 +1296,19 @@ template<bool NbrLds, bool OrOpt>
 __global__ void __launch_bounds__(1024) faco_colony_run(const AntParams p, const ColonyParams cp, uint32_t rounds) {
     __shared__ ColonyState s_state;
     ant_search<NbrLds, false, false, OrOpt, true>(p, cp, rounds, &s_state);
+}
+
+// GP-controlled FACO over `rounds` batches in one launch: one workgroup per
+// colony runs decision, ants, colony update and archive update batch after
+// batch (the per-batch kernels' order, cuda/batch_engine.cu:880-1060). At
+// most 12 waves per workgroup: three waves per SIMD leave the ant loop and
+// the control bodies 168 registers. The parameters of the control bodies
+// are read from device memory (`ctl`).
+template<bool NbrLds, bool OrOpt>
+__global__ void __launch_bounds__(768) faco_control_run(const AntParams p, const ColonyParams cp,
+                                                         const ControlParams* ctl, uint32_t rounds) {
+    __shared__ ColonyState s_state;
+    ant_search<NbrLds, false, false, OrOpt, true, false, true>(p, cp, rounds, &s_state, ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1427,6 +1453,33 @@ hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_
     const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
     if (nl) oo ? go(faco_colony_run<true, true>) : go(faco_colony_run<true, false>);
     else oo ? go(faco_colony_run<false, true>) : go(faco_colony_run<false, false>);
+    return hipGetLastError();
+}
+
+// LDS of the persistent controlled kernel: the ants' share, or the
+// control bodies' if larger (they reuse the ants' per-wave space).
+uint32_t control_run_lds_bytes(const AntParams& p) {
+    const uint32_t shared = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u);
+    const uint32_t ants = p.waves * p.per_wave_bytes;
+    const uint32_t decide = static_cast<uint32_t>(sizeof(DecideShared));
+    const uint32_t update = round16_host(static_cast<uint32_t>(sizeof(UpdateShared))) +
+                            static_cast<uint32_t>(control_candidate_bytes(p.ants));
+    return shared + std::max(ants, std::max(decide, update));
+}
+
+hipError_t launch_control_run(const AntParams& p, const ColonyParams& cp, const ControlParams& ctl,
+                              const ControlParams* device_ctl, uint32_t colonies, uint32_t rounds,
+                              hipStream_t stream) {
+    if (p.blocks_per_colony != 1 || p.waves < 2 || p.waves > 12 || p.phase_ticks || p.explicit_mode || p.per_ant_parent ||
+        p.trace_selected || p.graph_offsets || p.diag_construction || p.diag_positions || p.diag_checklist ||
+        !p.control || !p.out_identity || p.filter_tours || ctl.views_only || ctl.forced_action >= 0)
+        return hipErrorInvalidValue;   // one workgroup per colony, production controlled FACO only
+    const uint32_t lds = control_run_lds_bytes(p);
+    const dim3 grid(colonies), block(64 * p.waves);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p, cp, device_ctl, rounds); };
+    const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
+    if (nl) oo ? go(faco_control_run<true, true>) : go(faco_control_run<true, false>);
+    else oo ? go(faco_control_run<false, true>) : go(faco_control_run<false, false>);
     return hipGetLastError();
 }
 

@@ -377,6 +377,7 @@ def _batch_output(result: Result, tours: np.ndarray, costs: np.ndarray, mode: st
         "preparation_mode": mode,
         "search_seconds": result.search_seconds,
         "search_gpu_milliseconds": result.search_gpu_milliseconds,
+        "persistent_launches": result.persistent_launches,
     }
 
 

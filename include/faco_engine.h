@@ -84,6 +84,7 @@ typedef struct faco_result {
     double kernel_milliseconds[FACO_STAGE_COUNT]; /* filled when profiling is on */
     uint64_t completed_constraint_rejections;     /* Hard mode: evaluated 2-opt moves that would leave the graph
                                                      (reference faco_cpu.cpp:180-183) */
+    uint64_t persistent_launches; /* launches of the persistent kernels (many batches per colony workgroup) */
 } faco_result;
 
 /* Program IR v1 record (reference cpp/include/gp_faco/program.hpp:10-19). */
