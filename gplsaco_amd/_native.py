@@ -272,6 +272,9 @@ SIGNATURES = {
                                                 ctypes.c_uint32, ctypes.POINTER(vp)]),
     "faco_engine_autotune": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(TuneEntry),
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    "faco_engine_autotune_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.POINTER(ControlSpec),
+                                                    ctypes.c_uint32, ctypes.POINTER(TuneEntry), ctypes.c_uint32,
+                                                    ctypes.POINTER(ctypes.c_uint32)]),
     "faco_engine_destroy": (None, [vp]),
     "faco_engine_settings": (ctypes.c_int, [vp, ctypes.POINTER(Settings)]),
     "faco_engine_launch_shape": (ctypes.c_int, [vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 5),

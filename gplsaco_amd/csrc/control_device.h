@@ -61,64 +61,99 @@ __device__ int wave_canonical_compare(const uint32_t* a, uint32_t za, const uint
 }
 
 // ControlRandom (control_ops.hpp:81-89).
+// The rejection threshold 2^64 mod bound is below bound < 2^32, so a word
+// of 2^32 or more always passes and the threshold is computed only for the
+// rare smaller words; below 2^16 the remainder is taken in 32-bit steps,
+// word = hi 2^32 + lo: ((hi mod b)(2^32 mod b) + lo mod b) mod b.
 struct ControlRandom {
     uint64_t key, counter;
     __device__ uint32_t bounded(uint32_t bound) {
-        const uint64_t threshold = (0ULL - static_cast<uint64_t>(bound)) % bound;
         uint64_t word;
-        do { word = mixed(key + counter++); } while (word < threshold);
-        return static_cast<uint32_t>(word % bound);
+        for (;;) {
+            word = mixed(key + counter++);
+            if (word >> 32) break;
+            if (word >= (0ULL - static_cast<uint64_t>(bound)) % bound) break;
+        }
+        if (bound > 0xffffu) return static_cast<uint32_t>(word % bound);
+        const uint32_t hi = static_cast<uint32_t>(word >> 32), lo = static_cast<uint32_t>(word);
+        const uint32_t wrap = (0u - bound) % bound;   // 2^32 mod bound
+        return ((hi % bound) * wrap + lo % bound) % bound;
     }
 };
-
-// rank-th city outside nodes[0..count) (control_ops.hpp:92-102); `sorted`
-// holds the same cities in ascending order.
-__device__ __forceinline__ uint32_t remaining_node(uint32_t rank, const uint32_t* sorted, uint32_t count) {
-    uint32_t v = rank;
-    for (uint32_t i = 0; i < count; ++i) v += sorted[i] <= v ? 1u : 0u;
-    return v;
-}
-
-__device__ __forceinline__ void insert_sorted(uint32_t* sorted, uint32_t count, uint32_t v) {
-    uint32_t j = count;
-    while (j && sorted[j - 1] > v) { sorted[j] = sorted[j - 1]; --j; }
-    sorted[j] = v;
-}
 
 __device__ __forceinline__ bool has_node(const uint32_t* nodes, uint32_t count, uint32_t v) {
     for (uint32_t i = 0; i < count; ++i) if (nodes[i] == v) return true;
     return false;
 }
 
-// build_region (control_ops.hpp:120-146) by one thread into nodes[16].
-__device__ void build_region(const uint32_t* order, const uint16_t* nbr, uint32_t K, uint32_t pw, uint32_t n,
-                             uint64_t key, uint32_t batch, uint32_t mode, uint32_t region, uint32_t* nodes,
-                             uint32_t* sorted) {
+// Candidate-row reads for the control bodies: `count` entries of a u16 row
+// (global memory) in chunks of 16 whose loads are issued together, so a
+// search costs one memory round trip per chunk instead of one per entry.
+constexpr uint32_t kRowChunk = 16;
+__device__ __forceinline__ void row_chunk(const uint16_t* row, uint32_t base, uint32_t count, uint32_t (&v)[kRowChunk]) {
+#pragma unroll
+    for (uint32_t j = 0; j < kRowChunk; ++j) v[j] = base + j < count ? row[base + j] : 0xffffffffu;
+}
+
+// build_region (control_ops.hpp:120-146) by one wave into nodes[kRegion].
+// The serial rule's state is spread over the lanes: lane i holds the
+// region's i-th city (`mine`) and the i-th smallest of them (`sorted`), so
+// a membership test is one ballot, a sorted insertion one ballot and one
+// lane shift, and the rank-th remaining city a scan of `used` lane reads.
+// The draws are the serial rule's, in its order; every lane runs the same
+// control flow.
+__device__ void wave_build_region(const uint32_t* order, const uint16_t* nbr, uint32_t K, uint32_t pw, uint32_t n,
+                                  uint64_t key, uint32_t batch, uint32_t mode, uint32_t region, uint32_t lane,
+                                  uint32_t* nodes) {
     const uint32_t count = n < kRegion ? n : kRegion;
     ControlRandom random{mixed(key ^ 0xbb67ae8584caa73bULL ^ mixed(static_cast<uint64_t>(batch) ^ 0x3c6ef372fe94f82bULL) ^
                                mixed(static_cast<uint64_t>(mode * 4 + region + 1) ^ 0xa54ff53a5f1d36f1ULL)), 0};
-    uint32_t used = 0;
+    uint32_t mine = 0, sorted = 0xffffffffu, used = 0;
+    // lane i's value of a per-lane register (i uniform)
+    auto at_lane = [](uint32_t v, uint32_t i) {
+        return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(i)));
+    };
+    auto member = [&](uint32_t v) { return __ballot(lane < used && mine == v) != 0; };
+    auto add = [&](uint32_t v) {
+        // sorted insertion after every smaller city (the cities are
+        // distinct); lanes 0..15 hold the list, so the shift by one lane
+        // stays inside the first DPP row
+        const uint32_t at = static_cast<uint32_t>(__popcll(__ballot(lane < used && sorted <= v)));
+        const uint32_t before = static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(sorted), 0x111 /* row_shr:1 */, 0xf, 0xf, false));
+        if (lane > at && lane <= used) sorted = before;
+        if (lane == at) sorted = v;
+        if (lane == used) mine = v;
+        ++used;
+    };
+    auto city = [&](uint32_t i) { return at_lane(mine, i); };
     if (region < 2) {
         const uint32_t start = random.bounded(n);
-        for (uint32_t i = 0; i < count; ++i) {
-            const uint32_t k = start + i;
-            nodes[i] = order[k >= n ? k - n : k];
-        }
-        return;
-    }
-    if (region == 2) {
-        const uint32_t first = random.bounded(n);
-        nodes[used] = first; insert_sorted(sorted, used, first); ++used;
-        for (uint32_t head = 0; head < used && used < count; ++head)
-            for (uint32_t j = 0; j < pw && used < count; ++j) {
-                const uint32_t v = nbr[static_cast<size_t>(nodes[head]) * K + j];
-                if (v < n && !has_node(nodes, used, v)) { nodes[used] = v; insert_sorted(sorted, used, v); ++used; }
+        const uint32_t k = start + lane;
+        if (lane < count) mine = order[k >= n ? k - n : k];
+        used = count;
+    } else {
+        if (region == 2) {
+            add(random.bounded(n));
+            for (uint32_t head = 0; head < used && used < count; ++head) {
+                const uint16_t* row = nbr + static_cast<size_t>(city(head)) * K;
+                for (uint32_t base = 0; base < pw && used < count; base += 64) {
+                    const uint32_t entry = base + lane < pw ? row[base + lane] : 0xffffffffu;
+                    for (uint32_t j = 0; j < 64 && base + j < pw && used < count; ++j) {
+                        const uint32_t v = at_lane(entry, j);
+                        if (v < n && !member(v)) add(v);
+                    }
+                }
             }
+        }
+        while (used < count) {
+            // the rank-th city outside the region (control_ops.hpp:92-102)
+            uint32_t v = random.bounded(n - used);
+            for (uint32_t i = 0; i < used; ++i) v += at_lane(sorted, i) <= v ? 1u : 0u;
+            add(v);
+        }
     }
-    while (used < count) {
-        const uint32_t v = remaining_node(random.bounded(n - used), sorted, used);
-        nodes[used] = v; insert_sorted(sorted, used, v); ++used;
-    }
+    if (lane < kRegion) nodes[lane] = lane < count ? mine : 0u;
 }
 
 // baseline_action (baseline_policy.hpp:59-79).
@@ -166,7 +201,7 @@ struct DecideShared {
     ColonyState s;
     uint32_t alt, mask, action, pad;
     float gap[2], diff, pad2;
-    uint32_t nodes[8][kRegion], sorted[8][kRegion];
+    uint32_t nodes[8][kRegion];
     double contrib[128][5];
     float regional[8][4];
     float feat[12 * 32];
@@ -178,15 +213,9 @@ struct DecideShared {
 struct UpdateShared {
     ControlState c;
     ColonyState s;
-    uint32_t chosen[kArchive], count, collision, pairs, pad;
+    uint32_t chosen[kArchive], count, collision, pairs, returned;
     uint64_t best[256];
 };
-
-// LDS bytes of control_update's candidate arrays: cost, identity, then
-// canonical start, group leader and pair list.
-__host__ __device__ constexpr size_t control_candidate_bytes(uint32_t ants) {
-    return 36u * static_cast<size_t>(kArchive + ants);
-}
 
 // ---------------------------------------------------------------------------
 // build_control_features + scoring + apply_control_action for one colony
@@ -218,7 +247,6 @@ __device__ __forceinline__ void control_decide(const ControlParams& p, uint32_t 
     float (&s_gap)[2] = sh.gap;
     float& s_diff = sh.diff;
     auto& s_nodes = sh.nodes;
-    auto& s_sorted = sh.sorted;
     auto& s_contrib = sh.contrib;
     auto& s_regional = sh.regional;
     float* s_feat = sh.feat;
@@ -275,13 +303,15 @@ __device__ __forceinline__ void control_decide(const ControlParams& p, uint32_t 
     __syncthreads();
     const uint32_t alt = s_alt;
 
-    // (2) start regions: one thread per (mode, region)
-    if (tid < 8) {
-        const uint32_t mode = tid / 4, region = tid % 4;
-        for (uint32_t i = 0; i < kRegion; ++i) s_nodes[tid][i] = 0;
+    // (2) start regions: one wave per (mode, region), the waves taking
+    // them in turn
+    for (uint32_t g = wave; g < 8; g += threads / 64) {
+        const uint32_t mode = g / 4, region = g % 4;
         if (mode == 0 || alt < kArchive) {
             const uint32_t* order = mode == 0 ? parent : arch_tours + static_cast<size_t>(alt) * ns;
-            build_region(order, nbr, K, pw, n, p.keys[colony], s.batch, mode, region, s_nodes[tid], s_sorted[tid]);
+            wave_build_region(order, nbr, K, pw, n, p.keys[colony], s.batch, mode, region, lane, s_nodes[g]);
+        } else if (lane < kRegion) {
+            s_nodes[g][lane] = 0;
         }
     }
     __syncthreads();
@@ -302,16 +332,27 @@ __device__ __forceinline__ void control_decide(const ControlParams& p, uint32_t 
             const double near = 0.5 * static_cast<double>(euc2d(coords[node], coords[prev])) +
                                 0.5 * static_cast<double>(euc2d(coords[node], coords[next]));
             v[0] = unit_clip((near / scale - 1) / 3);
+            // slot of each neighbour in the node's primary row (pw: none);
+            // the row is read in chunks whose loads are issued together
             const uint32_t neighbours[2] = {prev, next};
+            uint32_t slot[2] = {pw, pw};
+            const uint16_t* row = nbr + static_cast<size_t>(node) * K;
+            for (uint32_t base = 0; base < pw && (slot[0] == pw || slot[1] == pw); base += kRowChunk) {
+                uint32_t chunk[kRowChunk];
+                row_chunk(row, base, pw, chunk);
+#pragma unroll
+                for (uint32_t j = 0; j < kRowChunk; ++j)
+                    for (uint32_t k = 0; k < 2; ++k)
+                        if (chunk[j] == neighbours[k] && slot[k] == pw && base + j < pw) slot[k] = base + j;
+            }
             for (uint32_t k = 0; k < 2; ++k) {
                 const uint32_t w = neighbours[k];
+                // integer-valued terms: exact in any order
                 for (uint32_t j = 0; j < c.archive_size; ++j) {
                     const uint32_t al = arch_links[static_cast<size_t>(j) * ns + node];
                     v[1] += ((al & 0xffffu) == w || (al >> 16) == w) ? 0.0 : 1.0;
                 }
-                double trail = s.default_trail;
-                for (uint32_t j = 0; j < pw; ++j)
-                    if (nbr[static_cast<size_t>(node) * K + j] == w) { trail = trails[static_cast<size_t>(node) * pw + j]; break; }
+                const double trail = slot[k] < pw ? trails[static_cast<size_t>(node) * pw + slot[k]] : s.default_trail;
                 v[2 + k] = s.minimum == s.maximum ? 1.0 : unit_clip((trail - s.minimum) / (s.maximum - s.minimum));
                 v[4] += has_node(s_nodes[g], count, w) ? 0.0 : 1.0;
             }
@@ -490,7 +531,8 @@ __device__ __forceinline__ void control_update(const ControlParams& p, uint32_t 
     uint32_t* zero = reinterpret_cast<uint32_t*>(lds + 24u * M);
     uint32_t* leader = zero + M;
     uint32_t* pairs = leader + M;          // candidates to check against their leader
-    if (tid == 0) { c = *ctl; s = *st; s_collision = 0; s_pairs = 0; }
+    uint32_t* parent_tour = pairs + M;     // ants whose tour is the batch's parent tour
+    if (tid == 0) { c = *ctl; s = *st; s_collision = 0; s_pairs = 0; sh.returned = 0; }
     __syncthreads();
     const uint32_t parity = c.parity;
     const uint32_t* old_tours = p.archive_tours + (static_cast<size_t>(colony) * 2 + parity) * kArchive * ns;
@@ -519,29 +561,57 @@ __device__ __forceinline__ void control_update(const ControlParams& p, uint32_t 
             z = p.ant_zero[ant_base + a];
         }
         if (ck >= 0 && static_cast<double>(ck) > band) ck = -1;
-        cost[k] = ck; ident[2 * k] = i0; ident[2 * k + 1] = i1; zero[k] = z;
+        cost[k] = ck; ident[2 * k] = i0; ident[2 * k + 1] = i1;
+        zero[k] = z & ~kParentTour;
+        parent_tour[k] = (z & kParentTour) ? 1u : 0u;
     }
     __syncthreads();
-    // group leader: the first candidate with the same (cost, identity)
+    // group leader: the first candidate with the same (cost, identity).
+    // Every member is compared with its leader below, except where
+    // equality is already known: ants that both hold the batch's parent
+    // tour, and a parent-tour ant after the group's first parent-tour
+    // member (`rep`), whose own comparison covers it.
+    // (The costs are read eight at a time, the loads of a chunk together.)
     for (uint32_t k = tid; k < M; k += threads) {
-        uint32_t lead = k;
-        if (cost[k] >= 0)
-            for (uint32_t j = 0; j < k; ++j)
-                if (cost[j] == cost[k] && ident[2 * j] == ident[2 * k] && ident[2 * j + 1] == ident[2 * k + 1]) { lead = j; break; }
+        uint32_t lead = k, rep = kNone;
+        const int64_t ck = cost[k];
+        if (ck >= 0) {
+            const uint64_t i0 = ident[2 * k], i1 = ident[2 * k + 1];
+            bool done = false;
+            for (uint32_t base = 0; base < k && !done; base += 8) {
+                int64_t cj[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) cj[u] = base + u < k ? cost[base + u] : -1;
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const uint32_t j = base + u;
+                    if (done || cj[u] != ck || ident[2 * j] != i0 || ident[2 * j + 1] != i1) continue;
+                    if (lead == k) {
+                        lead = j;
+                        done = !parent_tour[k] || parent_tour[j];
+                    } else if (parent_tour[j]) {
+                        rep = j;
+                        done = true;
+                    }
+                }
+            }
+        }
         leader[k] = lead;
-        if (lead != k) pairs[atomicAdd(&s_pairs, 1u)] = k;
+        if (lead != k && (!parent_tour[k] || (!parent_tour[lead] && rep == kNone)))
+            pairs[atomicAdd(&s_pairs, 1u)] = k;
     }
     __syncthreads();
     // every group member must be its leader's tour (else: identity collision)
     {
-        // (members x positions) flattened over the workgroup, four loads
+        // (members x positions) flattened over the workgroup, eight load
         // pairs in flight per thread; (4 + ants) * n < 2^32
+        constexpr uint32_t kFlight = 8;
         const uint32_t P = s_pairs, span = n - 1, total = P * span, step = threads;
         bool bad = false;
-        for (uint32_t base = tid; base < total; base += 4 * step) {
-            uint32_t va[4], vb[4];
+        for (uint32_t base = tid; base < total; base += kFlight * step) {
+            uint32_t va[kFlight], vb[kFlight];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
+            for (uint32_t u = 0; u < kFlight; ++u) {
                 const uint32_t k = base + u * step;
                 va[u] = vb[u] = 0;
                 if (k < total) {
@@ -551,7 +621,7 @@ __device__ __forceinline__ void control_update(const ControlParams& p, uint32_t 
                 }
             }
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) bad |= va[u] != vb[u];
+            for (uint32_t u = 0; u < kFlight; ++u) bad |= va[u] != vb[u];
         }
         if (bad) s_collision = 1;
     }
@@ -635,31 +705,34 @@ __device__ __forceinline__ void control_update(const ControlParams& p, uint32_t 
     }
     __syncthreads();
 
-    // write the next archive: tours, links, canonical starts
+    // write the next archive (tours, links; every slot's positions over the
+    // workgroup at once) and stage each ant's LS work term for the feedback
     const uint32_t count = s_count;
-    for (uint32_t slot = 0; slot < count; ++slot) {
+    for (uint32_t k = tid; k < count * n; k += threads) {
+        const uint32_t slot = k / n, i = k - slot * n;
         const uint32_t* src = tour_of(s_chosen[slot]);
-        uint32_t* dst = new_tours + static_cast<size_t>(slot) * ns;
-        uint32_t* dl = new_links + static_cast<size_t>(slot) * ns;
-        for (uint32_t i = tid; i < n; i += threads) {
-            const uint32_t v = src[i];
-            const uint32_t pred = src[i == 0 ? n - 1 : i - 1], succ = src[i + 1 == n ? 0 : i + 1];
-            dst[i] = v;
-            dl[v] = pred | (succ << 16);
-        }
+        const uint32_t v = src[i];
+        const uint32_t pred = src[i == 0 ? n - 1 : i - 1], succ = src[i + 1 == n ? 0 : i + 1];
+        new_tours[static_cast<size_t>(slot) * ns + i] = v;
+        new_links[static_cast<size_t>(slot) * ns + v] = pred | (succ << 16);
+    }
+    double* work_terms = reinterpret_cast<double*>(cost);   // the costs are no longer read
+    for (uint32_t a = tid; a < A; a += threads) {
+        work_terms[a] = p.ls_limit ? static_cast<double>(p.info[ant_base + a].move_evaluations) /
+                                         static_cast<double>(p.ls_limit) : 0.0;
+        const uint32_t k = kArchive + a;
+        if (ident[2 * k] == c.active_id[0] && ident[2 * k + 1] == c.active_id[1]) atomicAdd(&sh.returned, 1u);
     }
     __syncthreads();
     if (tid == 0) {
         ControlState out = c;
-        // feedback: ants that returned to the active tour, mean LS work,
-        // both summed in ant order (cuda/control_state.cuh:166-172)
-        double returned = 0, work = 0;
-        for (uint32_t a = 0; a < A; ++a) {
-            const uint32_t k = kArchive + a;
-            returned += (ident[2 * k] == c.active_id[0] && ident[2 * k + 1] == c.active_id[1]) ? 1.0 : 0.0;
-            if (p.ls_limit)
-                work += static_cast<double>(p.info[ant_base + a].move_evaluations) / static_cast<double>(p.ls_limit);
-        }
+        // feedback: ants that returned to the active tour (an integer
+        // count, exact in any order) and mean LS work, summed in ant order
+        // (cuda/control_state.cuh:166-172)
+        const double returned = static_cast<double>(sh.returned);
+        double work = 0;
+        if (p.ls_limit)
+            for (uint32_t a = 0; a < A; ++a) work += work_terms[a];
         if (A) {
             const float r = static_cast<float>(unit_clip(returned / A));
             const float w = static_cast<float>(unit_clip(work / A));

@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) faco_control_init(const ControlParams p) 
 
 // ---------------------------------------------------------------------------
 // Per-batch control kernels: one workgroup per colony.
-__global__ void __launch_bounds__(128) faco_control_decide(const ControlParams p) {
+__global__ void __launch_bounds__(512) faco_control_decide(const ControlParams p) {
     __shared__ DecideShared sh;
     control_decide(p, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x, sh, nullptr, nullptr);
 }
@@ -89,7 +89,7 @@ hipError_t launch_control_init(const ControlParams& p, uint32_t colonies, hipStr
 }
 
 hipError_t launch_control_decide(const ControlParams& p, uint32_t colonies, hipStream_t stream) {
-    hipLaunchKernelGGL(faco_control_decide, dim3(colonies), dim3(128), 0, stream, p);
+    hipLaunchKernelGGL(faco_control_decide, dim3(colonies), dim3(512), 0, stream, p);
     return hipGetLastError();
 }
 

@@ -206,6 +206,9 @@ struct Engine {
     std::vector<uint64_t> last_rejections;
     // launches of the persistent kernels so far (results report the delta)
     uint64_t persistent_launch_count = 0;
+    // controlled runs may use the persistent controlled kernel (the
+    // controlled autotuner keeps whichever form was faster)
+    bool fuse_control = true;
     // frozen preparation charges per instance key (cheap, preparation
     // seconds) billed by cached_charged time-budget evaluations
     std::map<uint64_t, std::pair<double, double>> charges;
@@ -862,10 +865,10 @@ ControlParams control_params(Engine& e, uint32_t panel, const faco_control_spec&
 }
 
 // Controlled batches run as one persistent launch when every colony is one
-// workgroup of 2..12 waves (the fused kernel's launch bounds), the engine is
+// workgroup of 2..16 waves, the engine is
 // unrestricted and the control bodies' LDS fits beside the ants' shared data.
 bool fused_control(const AntParams& ap) {
-    return ap.blocks_per_colony == 1 && !ap.graph_offsets && ap.waves >= 2 && ap.waves <= 12 &&
+    return ap.blocks_per_colony == 1 && !ap.graph_offsets && ap.waves >= 2 && ap.waves <= 16 &&
            control_run_lds_bytes(ap) + 256u <= max_lds_per_block();
 }
 
@@ -978,8 +981,8 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     }
     e.ensure_control();
     auto& b = *e.ctl;
-    // LDS of the archive update: per candidate cost, identity, start, leader, pair
-    require(36ull * (kArchive + ants) <= max_lds_per_block(), "too many ants for the archive update");
+    // LDS of the archive update: per candidate cost, identity, start, leader, pair, parent flag
+    require(control_candidate_bytes(ants) + 4096u <= max_lds_per_block(), "too many ants for the archive update");
     e.upload_geometry(inst.data(), C);
     if (!resuming) e.keys.upload(keys.data(), C, e.stream);
     if (spec.mode == FACO_CONTROL_PROGRAM || spec.mode == FACO_CONTROL_FACTORIAL)
@@ -1090,7 +1093,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
         // rest eagerly
         constexpr uint32_t kGraphIterations = 16;
         const bool graphs = (flags & FACO_RUN_GRAPH) != 0 && !events;
-        const bool fused = graphs && fused_control(ap);
+        const bool fused = graphs && e.fuse_control && fused_control(ap);
         if (fused) b.params.upload(&cp, 1, e.stream);
         hipGraphExec_t exec = nullptr;
         struct ExecGuard { hipGraphExec_t& x; ~ExecGuard() { if (x) (void)hipGraphExecDestroy(x); } } guard{exec};
@@ -1779,7 +1782,11 @@ int faco_engine_settings(const faco_engine* engine, faco_settings* out) {
 }
 
 int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony) {
-    return guarded([&] { require(engine, "null argument"); engine->impl->set_waves(waves, blocks_per_colony); });
+    return guarded([&] {
+        require(engine, "null argument");
+        engine->impl->set_waves(waves, blocks_per_colony);
+        engine->impl->fuse_control = true;   // a forced shape runs the persistent controlled kernel where it can
+    });
 }
 
 int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks,
@@ -1826,6 +1833,68 @@ int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, con
         for (size_t i = 1; i < rows.size(); ++i)
             if (rows[i].ms_per_iteration < rows[best].ms_per_iteration) best = i;
         e.set_waves(rows[best].waves | (rows[best].nbr_in_lds ? 0u : 0x100u), 0);
+        if (table) std::copy(rows.begin(), rows.begin() + std::min<size_t>(rows.size(), capacity), table);
+        if (count) *count = static_cast<uint32_t>(rows.size());
+    });
+}
+
+int faco_engine_autotune_control(faco_engine* engine, const uint64_t* panel_keys, const uint64_t* panel_seeds,
+                                 uint32_t panel, const faco_control_spec* spec, uint32_t iterations,
+                                 faco_tune_entry* table, uint32_t capacity, uint32_t* count) {
+    return guarded([&] {
+        require(engine && panel_keys && panel_seeds && spec, "null argument");
+        require(iterations >= 1 && iterations <= 1000, "autotune iterations must be in 1..1000");
+        Engine& e = *engine->impl;
+        // a whole number of batches of every colony's ants
+        uint64_t unit = 1;
+        for (uint32_t t = 0; t < panel; ++t) {
+            const uint64_t a = e.instance_ants(e.problem(panel_keys[t]).n);
+            unit = unit / std::gcd(unit, a) * a;
+        }
+        const uint64_t limit = unit * iterations;
+        const uint32_t deciders = spec->mode == FACO_CONTROL_BASELINE ? 1u : spec->deciders;
+        std::vector<faco_tune_entry> rows;
+        const uint32_t C = panel * deciders;
+        std::vector<uint64_t> inst(C);
+        for (uint32_t c = 0; c < C; ++c) inst[c] = panel_keys[c % panel];
+        for (uint32_t w : {16u, 12u, 10u, 8u, 6u, 4u, 2u}) {
+            for (uint32_t lds : {1u, 0u}) {
+                const uint32_t forced = w | (lds ? 0u : 0x100u);
+                try {
+                    (void)choose_shape(e.n, e.K, e.config.ants, forced);
+                } catch (const std::invalid_argument&) {
+                    continue;   // does not fit the LDS budget
+                }
+                e.set_waves(forced, 0);
+                e.shapes_active = e.mixed;
+                if (e.mixed) (void)e.prepare_shapes(inst.data(), C, limit);
+                const AntParams ap = e.ant_params(C);   // the launch shape only (no control buffers yet)
+                e.shapes_active = false;
+                // the persistent controlled kernel where the shape allows it,
+                // and the per-batch launches
+                for (const bool fuse : {true, false}) {
+                    if (fuse && !fused_control(ap)) continue;
+                    e.fuse_control = fuse;
+                    faco_result r{};
+                    evaluate_control(e, panel_keys, panel_seeds, panel, limit, *spec, FACO_RUN_GRAPH, nullptr, &r,
+                                     nullptr, nullptr, nullptr, nullptr, nullptr);
+                    faco_tune_entry row{};
+                    row.waves = e.shape.waves;
+                    row.nbr_in_lds = e.shape.nbr_in_lds;
+                    row.blocks_per_colony = ap.blocks_per_colony;
+                    row.pad = fuse ? 1u : 0u;
+                    row.ms_per_iteration = r.search_gpu_milliseconds /
+                                           static_cast<double>(std::max<uint64_t>(r.launched_batches, 1));
+                    rows.push_back(row);
+                }
+            }
+        }
+        require(!rows.empty(), "no launch shape fits the LDS budget");
+        size_t best = 0;
+        for (size_t i = 1; i < rows.size(); ++i)
+            if (rows[i].ms_per_iteration < rows[best].ms_per_iteration) best = i;
+        e.set_waves(rows[best].waves | (rows[best].nbr_in_lds ? 0u : 0x100u), 0);
+        e.fuse_control = rows[best].pad != 0;
         if (table) std::copy(rows.begin(), rows.begin() + std::min<size_t>(rows.size(), capacity), table);
         if (count) *count = static_cast<uint32_t>(rows.size());
     });

@@ -207,6 +207,13 @@ struct ColonyShape { uint32_t n, ants, batches, pad; };
 // ColonyState::pad after a colony update that left a finished colony idle
 // (0 after every update that ran).
 constexpr uint32_t kColonyIdle = 1;
+// bit of an ant's canonical start (out_zero / ant_zero): its tour is the
+// batch's parent tour (cities < 2^16, so the position needs bits 0..15)
+constexpr uint32_t kParentTour = 1u << 30;
+// LDS bytes of the archive update's candidate arrays (kArchive + ants
+// candidates): cost, identity, then canonical start, group leader, pair
+// list and parent-tour flag.
+constexpr size_t control_candidate_bytes(uint32_t ants) { return 40u * static_cast<size_t>(kArchive + ants); }
 
 struct AntParams {
     uint32_t nbr_in_lds;                // neighbour rows staged in LDS (else read from HBM)
@@ -255,7 +262,7 @@ struct AntParams {
                                         // clocks); the ant's start and end (absolute wall clock ticks)
     const ControlState* control;        // [C] or null: start cities from the chosen region
     uint64_t* out_identity;             // [A][2] or null: tour identity of each ant
-    uint32_t* out_zero;                 // [A] or null: position of city 0 | forward << 31
+    uint32_t* out_zero;                 // [A] or null: position of city 0 | forward << 31 | kParentTour
     const ColonyShape* shapes;          // [C] or null (fixed size)
     const uint32_t* order;              // [C] or null: colony of each block group (largest work first)
     // explicit mode only (device preparation): block b runs instance b

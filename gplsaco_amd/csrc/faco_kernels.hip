@@ -1229,12 +1229,23 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                 f ^= __shfl_xor(f, offset);
                 g ^= __shfl_xor(g, offset);
             }
+            // equality with the batch's parent tour as a cycle (every city
+            // keeps its two parent neighbours): the archive update takes it
+            // as proof that two such ants hold the same tour
+            bool moved = false;
+            for (uint32_t v = lane; v < n; v += kWave) {
+                const uint32_t pv = pos[v];
+                const uint32_t succ = tour[pv + 1 == n ? 0 : pv + 1], pred = tour[pv == 0 ? n - 1 : pv - 1];
+                const uint32_t link = s_plinks[v], lp = link & 0xffffu, ls = link >> 16;
+                moved |= !((pred == lp && succ == ls) || (pred == ls && succ == lp));
+            }
+            const bool parent_tour = ballot(moved) == 0;
             if (lane == 0) {
                 const uint32_t z = pos[0];
                 const uint32_t next = tour[z + 1 == n ? 0 : z + 1], prev = tour[z == 0 ? n - 1 : z - 1];
                 p.out_identity[2 * static_cast<size_t>(ant)] = f;
                 p.out_identity[2 * static_cast<size_t>(ant) + 1] = g;
-                p.out_zero[ant] = z | (next < prev ? 0x80000000u : 0u);
+                p.out_zero[ant] = z | (next < prev ? 0x80000000u : 0u) | (parent_tour ? kParentTour : 0u);
             }
         }
         if (lane == 0) {
@@ -1300,13 +1311,14 @@ __global__ void __launch_bounds__(1024) faco_colony_run(const AntParams p, const
 
 // GP-controlled FACO over `rounds` batches in one launch: one workgroup per
 // colony runs decision, ants, colony update and archive update batch after
-// batch (the per-batch kernels' order, cuda/batch_engine.cu:880-1060). At
-// most 12 waves per workgroup: three waves per SIMD leave the ant loop and
-// the control bodies 168 registers. The parameters of the control bodies
-// are read from device memory (`ctl`).
-template<bool NbrLds, bool OrOpt>
-__global__ void __launch_bounds__(768) faco_control_run(const AntParams p, const ColonyParams cp,
-                                                         const ControlParams* ctl, uint32_t rounds) {
+// batch (the per-batch kernels' order, cuda/batch_engine.cu:880-1060).
+// Two instantiations by workgroup size: up to 12 waves (three per SIMD
+// leave the ant loop and the control bodies 168 registers) and up to 16
+// (128 registers; the control bodies spill some values). The parameters of
+// the control bodies are read from device memory (`ctl`).
+template<bool NbrLds, bool OrOpt, int MaxThreads>
+__global__ void __launch_bounds__(MaxThreads) faco_control_run(const AntParams p, const ColonyParams cp,
+                                                                const ControlParams* ctl, uint32_t rounds) {
     __shared__ ColonyState s_state;
     ant_search<NbrLds, false, false, OrOpt, true, false, true>(p, cp, rounds, &s_state, ctl);
 }
@@ -1470,7 +1482,7 @@ uint32_t control_run_lds_bytes(const AntParams& p) {
 hipError_t launch_control_run(const AntParams& p, const ColonyParams& cp, const ControlParams& ctl,
                               const ControlParams* device_ctl, uint32_t colonies, uint32_t rounds,
                               hipStream_t stream) {
-    if (p.blocks_per_colony != 1 || p.waves < 2 || p.waves > 12 || p.phase_ticks || p.explicit_mode || p.per_ant_parent ||
+    if (p.blocks_per_colony != 1 || p.waves < 2 || p.waves > 16 || p.phase_ticks || p.explicit_mode || p.per_ant_parent ||
         p.trace_selected || p.graph_offsets || p.diag_construction || p.diag_positions || p.diag_checklist ||
         !p.control || !p.out_identity || p.filter_tours || ctl.views_only || ctl.forced_action >= 0)
         return hipErrorInvalidValue;   // one workgroup per colony, production controlled FACO only
@@ -1478,8 +1490,13 @@ hipError_t launch_control_run(const AntParams& p, const ColonyParams& cp, const 
     const dim3 grid(colonies), block(64 * p.waves);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p, cp, device_ctl, rounds); };
     const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
-    if (nl) oo ? go(faco_control_run<true, true>) : go(faco_control_run<true, false>);
-    else oo ? go(faco_control_run<false, true>) : go(faco_control_run<false, false>);
+    if (p.waves <= 12) {
+        if (nl) oo ? go(faco_control_run<true, true, 768>) : go(faco_control_run<true, false, 768>);
+        else oo ? go(faco_control_run<false, true, 768>) : go(faco_control_run<false, false, 768>);
+    } else {
+        if (nl) oo ? go(faco_control_run<true, true, 1024>) : go(faco_control_run<true, false, 1024>);
+        else oo ? go(faco_control_run<false, true, 1024>) : go(faco_control_run<false, false, 1024>);
+    }
     return hipGetLastError();
 }
 

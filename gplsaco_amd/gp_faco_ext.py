@@ -381,6 +381,30 @@ def _batch_output(result: Result, tours: np.ndarray, costs: np.ndarray, mode: st
     }
 
 
+def _control_spec(mode, programs, controllers, baseline, variant, experiment_mask, force_collisions):
+    """The faco_control_spec of a controlled run and the decider arrays it
+    points to (keep them alive through the call)."""
+    spec = ControlSpec()
+    spec.mode = {"program": 1, "controller": 2, "baseline": 3, "factorial": 4}[mode]
+    spec.experiment_mask = _unsigned(experiment_mask, "experiment_mask", 32)
+    spec.factorial_variant = variant
+    spec.force_fingerprint_collisions = 1 if force_collisions else 0
+    prog_arr = ctrl_arr = None
+    if mode in ("program", "factorial"):
+        prog_arr = (Program * len(programs))(*programs)
+        spec.programs = ctypes.addressof(prog_arr)
+        spec.deciders = len(programs)
+    if mode == "controller":
+        ctrl_arr = (Controller * len(controllers))(*controllers)
+        spec.controllers = ctypes.addressof(ctrl_arr)
+        spec.deciders = len(controllers)
+    if mode == "baseline":
+        spec.deciders = 1
+    if baseline is not None:
+        spec.baseline = baseline
+    return spec, prog_arr, ctrl_arr
+
+
 def _control_state_dict(st: ControlState) -> dict:
     """One entry of control_states (reference bindings.cpp:394-407)."""
     return {"archive_size": st.archive_size, "stagnant_batches": st.stagnant_batches, "return_rate": st.return_rate,
@@ -537,6 +561,30 @@ class FacoBatchEngine:
                 for t in table[:min(count.value, 32)]]
         best = min(rows, key=lambda r: r["ms_per_iteration"])
         return {"table": rows, "best": best, "launch_shape": self.launch_shape(keys.size)}
+
+    def autotune_control(self, instance_keys, seeds, mode, *, programs=(), controllers=(), baseline=None,
+                         variant=3, experiment_mask=0xFFFFFFFF, iterations: int = 2) -> dict:
+        """The autotuner on a controlled run itself: every launch shape that
+        fits is timed on `iterations` batches of this controlled run over the
+        panel (colony d * panel + t under decider d, as in
+        evaluate_control_raw); the fastest is kept. "persistent" marks the
+        shapes that run the persistent controlled kernel."""
+        keys, seeds = _keys(instance_keys, "instance_keys"), _keys(seeds, "seeds")
+        if keys.size != seeds.size:
+            raise ValueError("autotune needs complete task arrays")
+        spec, prog_arr, ctrl_arr = _control_spec(mode, programs, controllers, baseline, variant, experiment_mask, False)
+        table = (TuneEntry * 32)()
+        count = ctypes.c_uint32()
+        check(lib().faco_engine_autotune_control(self._handle, keys, seeds, keys.size, ctypes.byref(spec),
+                                                 _unsigned(iterations, "iterations", 32), table, 32,
+                                                 ctypes.byref(count)))
+        del prog_arr, ctrl_arr
+        rows = [{"waves": t.waves, "neighbour_rows_in_lds": bool(t.nbr_in_lds),
+                 "blocks_per_colony": t.blocks_per_colony, "persistent": bool(t.pad),
+                 "ms_per_iteration": t.ms_per_iteration}
+                for t in table[:min(count.value, 32)]]
+        best = min(rows, key=lambda r: r["ms_per_iteration"])
+        return {"table": rows, "best": best}
 
     def set_waves_per_block(self, waves: int, blocks_per_colony: int = 0) -> None:
         """Force the ant kernel's shape: waves (ants in flight) per workgroup,
@@ -853,24 +901,8 @@ class FacoBatchEngine:
         instance keys): `call` receives the spec, flags, trace and output
         pointers of faco_engine_evaluate_control."""
         panel = len(panel_keys)
-        spec = ControlSpec()
-        spec.mode = {"program": 1, "controller": 2, "baseline": 3, "factorial": 4}[mode]
-        spec.experiment_mask = _unsigned(experiment_mask, "experiment_mask", 32)
-        spec.factorial_variant = variant
-        spec.force_fingerprint_collisions = 1 if force_fingerprint_collisions else 0
-        prog_arr = ctrl_arr = None
-        if mode in ("program", "factorial"):
-            prog_arr = (Program * len(programs))(*programs)
-            spec.programs = ctypes.addressof(prog_arr)
-            spec.deciders = len(programs)
-        if mode == "controller":
-            ctrl_arr = (Controller * len(controllers))(*controllers)
-            spec.controllers = ctypes.addressof(ctrl_arr)
-            spec.deciders = len(controllers)
-        if mode == "baseline":
-            spec.deciders = 1
-        if baseline is not None:
-            spec.baseline = baseline
+        spec, prog_arr, ctrl_arr = _control_spec(mode, programs, controllers, baseline, variant, experiment_mask,
+                                                 force_fingerprint_collisions)
         C = panel * spec.deciders
         tours = np.zeros((C, self.dimension), dtype=np.uint32)
         costs = np.zeros(C, dtype=np.int64)

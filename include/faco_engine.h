@@ -245,7 +245,7 @@ typedef struct faco_tune_entry {
     uint32_t waves;                 /* ants in flight per workgroup */
     uint32_t nbr_in_lds;            /* neighbour rows staged in LDS */
     uint32_t blocks_per_colony;     /* workgroups per colony chosen for the batch */
-    uint32_t pad;
+    uint32_t pad;                   /* autotune_control: 1 when the shape runs the persistent controlled kernel */
     double ms_per_iteration;        /* ant kernel + colony update, HIP events */
 } faco_tune_entry;
 
@@ -262,6 +262,18 @@ typedef struct faco_tune_entry {
  * only their time changes. */
 int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, const uint64_t* seeds, uint32_t tasks,
                          uint32_t iterations, faco_tune_entry* table, uint32_t capacity, uint32_t* count);
+
+/* The autotuner on the controlled workload itself: every launch shape that
+ * fits is timed on `iterations` batches of the controlled run `spec` over
+ * the panel (search_gpu_milliseconds per batch), once as the persistent
+ * controlled kernel where the shape allows it (entry pad = 1) and once as
+ * graph-replayed per-batch launches; the fastest shape and form are kept
+ * for the engine's later controlled runs (faco_engine_set_waves restores
+ * the persistent form). No counterpart in the reference (its block shape is
+ * a compile-time choice); results never depend on the shape or form. */
+int faco_engine_autotune_control(faco_engine* engine, const uint64_t* panel_keys, const uint64_t* panel_seeds,
+                                 uint32_t panel, const faco_control_spec* spec, uint32_t iterations,
+                                 faco_tune_entry* table, uint32_t capacity, uint32_t* count);
 
 /* Local search of the ants: 0 = the reference's checklist 2-opt (default),
  * 1 = checklist 2-opt + Or-opt. Or-opt has no counterpart in the reference:

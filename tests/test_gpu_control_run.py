@@ -63,7 +63,7 @@ def _both(e, keys, seeds, limit, mode, **kw):
 
 
 @pytest.mark.parametrize("n,ants,batches,waves", [(200, 32, 30, 8), (90, 16, 40, 2), (1100, 64, 5, 12),
-                                                  (300, 32, 12, 6 | 0x100)])
+                                                  (300, 32, 12, 6 | 0x100), (400, 64, 10, 16)])
 def test_program_population(ext, oracle_mod, n, ants, batches, waves):
     O = oracle_mod
     count = 2
@@ -207,3 +207,29 @@ def test_population_entry_point_uses_it(ext):
         assert want["persistent_launches"] == 1
         assert [(i["tour"], i["cost"]) for i in got["items"]] == [(i["tour"], i["cost"]) for i in want["items"]]
         assert got["control_states"] == want["control_states"]
+
+
+def test_controlled_autotune_keeps_results(ext):
+    """The controlled autotuner on a fresh engine times every shape both as
+    the persistent kernel and as per-batch launches, keeps the fastest, and
+    the run after it equals the same run on the default shape."""
+    n, count, ants = 160, 2, 32
+    xys, keys = panel(n, count)
+    s = settings(ext, ants=ants)
+    rng = np.random.default_rng(12)
+    programs = [ext.read_program(p) for p in (RESTARTY, spec2(random_program(rng, depth=4)))]
+    k = np.array(keys, dtype=np.uint64)
+    seeds = np.array([1, 2], dtype=np.uint64)
+    tuned = ext.FacoBatchEngine(n, count, s, population_size=2)
+    plain = ext.FacoBatchEngine(n, count, s, population_size=2)
+    for key, xy in zip(keys, xys):
+        tuned.register_problem(key, xy)
+        plain.register_problem(key, xy)
+    announce("control-run-autotune")
+    out = tuned.autotune_control(k, seeds, "program", programs=programs, iterations=2)
+    assert any(r["persistent"] for r in out["table"]) and any(not r["persistent"] for r in out["table"])
+    assert out["best"] == min(out["table"], key=lambda r: r["ms_per_iteration"])
+    got = tuned.evaluate_control_raw(k, seeds, ants * 12, "program", programs=programs)
+    assert (got["result"].persistent_launches > 0) == out["best"]["persistent"]
+    want = plain.evaluate_control_raw(k, seeds, ants * 12, "program", programs=programs, use_graph=False)
+    _same(got, want, 4)
