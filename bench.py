@@ -152,8 +152,9 @@ def steady_window(engine, keys, seeds, ants, warm, steps, **timing):
 
 def gp_population_rate(ext, steps: int) -> dict:
     """The GP-controlled population shape: 4 uniform TSP500 instances x 128
-    random feature-spec-2 programs (512 colonies x 128 ants), one launch per
-    iteration stage, `steps` iterations replayed through a HIP graph."""
+    random feature-spec-2 programs (512 colonies x 128 ants), `steps`
+    iterations, launch shape and form (persistent controlled kernel or
+    graph-replayed per-batch launches) picked by the controlled autotuner."""
     from gplsaco_amd.instances import instance_key, uniform_instance
     from gplsaco_amd.programs import random_program
     n, instances, individuals = 500, 4, 128
@@ -168,7 +169,7 @@ def gp_population_rate(ext, steps: int) -> dict:
     task_keys = np.array(keys, dtype=np.uint64)
     seeds = np.zeros(instances, dtype=np.uint64)
     parsed = [ext.read_program(p) for p in programs]
-    engine.evaluate_control_raw(task_keys, seeds, ants * 2, "program", programs=parsed)
+    tuned = engine.autotune_control(task_keys, seeds, "program", programs=parsed, iterations=8)
     raw = engine.evaluate_control_raw(task_keys, seeds, ants * steps, "program", programs=parsed, use_graph=True)
     tours = raw["result"].total_tour_evaluations
     restarts = sum(st["restarts"] for st in raw["states"])
@@ -176,6 +177,8 @@ def gp_population_rate(ext, steps: int) -> dict:
                         f"{instances * individuals} colonies x {ants} ants, {steps} iterations",
             "ant_tours_per_s": tours / raw["result"].search_seconds, "tours": tours,
             "seconds": raw["result"].search_seconds, "restarts": restarts,
+            "persistent_launches": raw["result"].persistent_launches,
+            "autotune": {"best": tuned["best"], "shapes_timed": len(tuned["table"])},
             "mean_final_tour_length": float(raw["costs"].mean())}
 
 
@@ -223,10 +226,12 @@ def uneven_batch_rate(ext, rounds: int) -> dict:
     uniform, 32 clustered) x 128 random single-tree GP programs (depth at
     most 5, the reference's limit) x 1 seed =
     8192 colonies in one mixed-size engine (each colony with its own n and
-    per-size ants: 64, 128 or 192), one launch per kernel per iteration,
-    HIP-graph replayed, launch shape picked by the autotuner. The budget is
-    `rounds` x 384 ant tours per colony (384 = a whole number of batches of
-    every ant count)."""
+    per-size ants: 64, 128 or 192), the whole run in one launch of the
+    persistent controlled kernel, waves per workgroup picked by the
+    controlled autotuner (which also times the graph-replayed per-batch
+    form). The budget is `rounds` x 384 ant tours per colony (384 = a whole
+    number of batches of every ant count; 167 rounds = 501 iterations of the
+    128-ant colonies)."""
     from gplsaco_amd.instances import _key, _uniform_words, clustered_instance, instance_key, uniform_instance
     from gplsaco_amd.programs import random_program
     count, individuals = 64, 128
@@ -255,22 +260,29 @@ def uneven_batch_rate(ext, rounds: int) -> dict:
     del host_engine
     task_keys = np.array(keys, dtype=np.uint64)
     seeds = np.zeros(count, dtype=np.uint64)
-    # the autotuner times plain FACO on the panel (shape choice only)
-    tuned = engine.autotune(np.tile(task_keys, individuals), np.tile(seeds, individuals), iterations=1)
+    # the controlled autotuner on this run; the run itself is one persistent
+    # launch at the best wave count
+    tuned = engine.autotune_control(task_keys, seeds, "program", programs=programs, iterations=2)
+    best = tuned["best"]
+    engine.set_waves_per_block(best["waves"] | (0 if best["neighbour_rows_in_lds"] else 0x100), 1)
     limit = 384 * rounds
     raw = engine.evaluate_control_raw(task_keys, seeds, limit, "program", programs=programs, use_graph=True)
     res = raw["result"]
+    per_batch = [r for r in tuned["table"] if not r["persistent"]]
     ants = [64 * math.ceil(math.sqrt(n) / 16) if n > 256 else 64 for n in sizes]
     return {"workload": f"GP-controlled FACO, {count} instances n in [200,2000] (half clustered) x {individuals} "
                         f"random programs = {count * individuals} colonies, {limit} ant tours per colony",
             "sizes": {"min": min(sizes), "max": max(sizes), "mean": float(np.mean(sizes))},
             "ant_tours_per_s": res.total_tour_evaluations / res.search_seconds,
             "tours": res.total_tour_evaluations, "seconds": res.search_seconds,
-            "launched_iterations": res.launched_batches, "preparation_seconds": prep,
+            "launched_iterations": res.launched_batches, "persistent_launches": res.persistent_launches,
+            "iterations_per_colony": {str(a): limit // a for a in (64, 128, 192)},
+            "preparation_seconds": prep,
             "preparation": {"device_seconds": prep, "host_seconds": host_prep, "host_threads": os.cpu_count(),
                             "identical_initial_tours": same},
             "restarts": sum(st["restarts"] for st in raw["states"]),
-            "autotune": {"best": tuned["best"], "shapes_timed": len(tuned["table"])},
+            "autotune": {"best": best, "shapes_timed": len(tuned["table"]),
+                         "best_per_batch_form": min(per_batch, key=lambda r: r["ms_per_iteration"]) if per_batch else None},
             "ants_per_colony": {"64": ants.count(64), "128": ants.count(128), "192": ants.count(192)}}
 
 
@@ -278,9 +290,11 @@ def full_pipeline(ext, rounds: int) -> dict:
     """The full pipeline at TSP1000 and TSP2000: 10 + 10 instances (half
     clustered) x 128 random conditional three-tree GP controllers in one
     mixed-size engine, the learned edge prior on (k = 20 candidates),
-    autotuned launch shape, `rounds` x 384 ant tours per colony; plain FACO
-    (same k, no prior, no control) on the same instances with 8 seeds and the
-    same budget for the gap."""
+    launch shape and form picked by the controlled autotuner, `rounds` x 384
+    ant tours per colony (500 rounds = 1000 iterations of the TSP2000
+    colonies' 192 ants, 1500 of the TSP1000 colonies' 128); plain FACO (same
+    k, no prior, no control) on the same instances with 8 seeds and the same
+    budget for the gap."""
     from gplsaco_amd import prior as P
     from gplsaco_amd.programs import random_controller
     sizes = [1000] * 10 + [2000] * 10
@@ -302,7 +316,7 @@ def full_pipeline(ext, rounds: int) -> dict:
         engine.apply_edge_prior(net, k)
     task_keys = np.array(keys, dtype=np.uint64)
     seeds = np.zeros(len(sizes), dtype=np.uint64)
-    tuned = engine.autotune(np.tile(task_keys, len(controllers)), np.tile(seeds, len(controllers)), iterations=1)
+    tuned = engine.autotune_control(task_keys, seeds, "controller", controllers=controllers, iterations=2)
     raw = engine.evaluate_control_raw(task_keys, seeds, limit, "controller", controllers=controllers, use_graph=True)
     res = raw["result"]
     costs = raw["costs"].reshape(len(controllers), len(sizes))
@@ -317,7 +331,9 @@ def full_pipeline(ext, rounds: int) -> dict:
                        f"colonies, k = 20, {limit} ant tours per colony",
            "ant_tours_per_s": res.total_tour_evaluations / res.search_seconds, "tours": res.total_tour_evaluations,
            "seconds": res.search_seconds, "restarts": sum(st["restarts"] for st in raw["states"]),
-           "autotune": tuned["best"], "plain_faco_ant_tours_per_s": plain["total_tour_evaluations"] / plain["search_seconds"]}
+           "autotune": tuned["best"], "persistent_launches": res.persistent_launches,
+           "iterations_per_colony": {"tsp1000": limit // 128, "tsp2000": limit // 192},
+           "plain_faco_ant_tours_per_s": plain["total_tour_evaluations"] / plain["search_seconds"]}
     for n in (1000, 2000):
         cols = [i for i, m in enumerate(sizes) if m == n]
         controlled, base = float(costs[:, cols].mean()), float(plain_costs[:, cols].mean())
@@ -384,13 +400,13 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--seeds", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--population-steps", type=int, default=50,
+    ap.add_argument("--population-steps", type=int, default=1000,
                     help="iterations of the GP-controlled population measurement (0 skips it)")
     ap.add_argument("--hard-iterations", type=int, default=20,
                     help="iterations of the Hard-mode leg (0 skips it)")
-    ap.add_argument("--uneven-rounds", type=int, default=40,
+    ap.add_argument("--uneven-rounds", type=int, default=167,
                     help="budget of the uneven-size GP batch in units of 384 ant tours per colony (0 skips it)")
-    ap.add_argument("--pipeline-rounds", type=int, default=10,
+    ap.add_argument("--pipeline-rounds", type=int, default=500,
                     help="budget of the full-pipeline leg in units of 384 ant tours per colony (0 skips it)")
     ap.add_argument("--prior-budgets", default="100,300",
                     help="iteration budgets of the learned-prior comparison (empty skips it)")
