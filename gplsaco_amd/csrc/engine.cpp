@@ -146,7 +146,7 @@ struct PreparedSlot { uint64_t key; const Prepared* problem; };
 struct Engine {
     uint32_t n, panel, population, colonies_capacity;
     faco_settings config;
-    uint32_t K, nbr_stride;
+    uint32_t K, nbr_stride, ls_offset = 0;
     int device;
     std::mutex mutex;
     std::map<uint64_t, Prepared> registry;
@@ -218,7 +218,11 @@ struct Engine {
     uint32_t graph_colonies = 0, graph_iterations = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 
-    Engine(uint32_t dimension, uint32_t colonies, faco_settings settings, uint32_t pop, bool mixed_sizes = false)
+    // split_rows: every neighbour row holds the primary row then the LS row
+    // (K = pw + lw, for graph specs whose rows are independent); otherwise
+    // both are prefixes of one (distance, id) order (K = max(pw, lw)).
+    Engine(uint32_t dimension, uint32_t colonies, faco_settings settings, uint32_t pop, bool mixed_sizes = false,
+           bool split_rows = false)
         : n(dimension), panel(colonies), population(pop), colonies_capacity(colonies * pop) {
         mixed = mixed_sizes;
         base = settings;
@@ -229,7 +233,8 @@ struct Engine {
         require(config.ls_width <= kMaxLs, "ls_width above 32 is not supported");
         require(config.backup_width <= kMaxBackup, "backup_width above 256 is not supported");
         require(n <= 65535, "dimension above 65535 is not supported");
-        K = std::max(config.primary_width, config.ls_width);
+        K = split_rows ? config.primary_width + config.ls_width : std::max(config.primary_width, config.ls_width);
+        ls_offset = split_rows ? config.primary_width : 0u;
         nbr_stride = round16_host(2u * n * K) / 2u;
         device = current_device();
         checked(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device), "CU count");
@@ -380,7 +385,7 @@ struct Engine {
         AntParams p{};
         p.nbr_in_lds = shape.nbr_in_lds;
         p.n = n; p.ants = config.ants; p.pw = config.primary_width; p.bw = config.backup_width;
-        p.lw = config.ls_width; p.K = K; p.waves = shape.waves;
+        p.lw = config.ls_width; p.K = K; p.ls_offset = ls_offset; p.waves = shape.waves;
         p.nbr_lds_bytes = shape.nbr_lds_bytes; p.nbr_stride = nbr_stride;
         p.per_wave_bytes = ant_wave_bytes(n);
         p.xcd_remap = shapes_active ? 0u : xcd_remap;   // mixed: keep the largest-first order
@@ -1752,7 +1757,8 @@ int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_setting
         require(constraint_mode <= 1, "constraint mode must be 0 (unrestricted) or 1 (hard); escape is not available");
         require(constraint_mode == 0 || population_size == 1, "a Hard-mode engine holds one population member");
         auto engine = std::make_unique<faco_engine>();
-        engine->impl = std::make_unique<Engine>(dimension, colonies, *settings, population_size);
+        engine->impl = std::make_unique<Engine>(dimension, colonies, *settings, population_size, false,
+                                                constraint_mode == 1);
         Engine& e = *engine->impl;
         if (constraint_mode == 1) {
             e.hard = true;
@@ -1966,12 +1972,13 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double
 }
 
 // register_graph_problem (reference batch_engine.cu:296-314,
-// prepared_problem.cpp:128-166): the common preparation, then the
-// candidate graph. This engine keeps one row layout (primary and LS rows as
-// prefixes of one (distance, id) order), so the spec's rows must be the
-// prepared rows; the graph itself is free (undirected, no self-loops or
-// repeated edges, holding the initial tour and every primary and LS row
-// edge, at most n (primary_width + 1) edges).
+// prepared_problem.cpp:128-166): the common preparation (cheap tour, kNN
+// rows, initial tour), then the candidate graph and its own rows. The graph
+// is undirected, without self-loops or repeated edges, holds the initial
+// tour and has at most n (primary_width + 1) edges; the primary and LS rows
+// are graph neighbours, LS rows ordered by (distance, city), backup rows
+// disjoint from the primary ones; every row is right-padded with n. A Hard
+// engine keeps split neighbour rows (primary row, then LS row) for them.
 int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const double* xy, uint32_t n,
                                        const uint32_t* edges, uint64_t edge_count, const uint32_t* initial_tour,
                                        const uint32_t* primary, const uint32_t* backup, const uint32_t* ls,
@@ -1982,25 +1989,24 @@ int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const 
         std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
         require(lock.owns_lock(), "registration and evaluation of one engine must not overlap");
         require(e.hard && n == e.n, "graph registration needs a Hard-mode engine of this dimension");
+        const uint32_t pw = e.config.primary_width, bw = e.config.backup_width, lw = e.config.ls_width;
+        require(bw == 0 || backup, "null argument");
         auto coords = integer_coordinates(xy, n);
-        Prepared p = prepare_problem(coords, e.config, e.K, e.nbr_stride);
+        const uint32_t k_prep = std::max(pw, lw);
+        Prepared p = prepare_problem(coords, e.config, k_prep, round16_host(2u * n * k_prep) / 2u);
         const auto started = Clock::now();
         require(std::equal(p.initial.begin(), p.initial.end(), initial_tour),
                 "the graph spec changed the common initial tour");
-        const uint32_t pw = e.config.primary_width, bw = e.config.backup_width, lw = e.config.ls_width;
-        require(std::equal(p.primary.begin(), p.primary.end(), primary) &&
-                std::equal(p.ls.begin(), p.ls.end(), ls) && (bw == 0 || (backup && std::equal(p.backup.begin(), p.backup.end(), backup))),
-                "graph rows must be the prepared candidate rows in this engine");
         require(edge_count <= static_cast<uint64_t>(n) * (pw + 1u), "the graph exceeds n (primary_width + 1) edges");
-        std::vector<std::vector<uint32_t>> rows(n);
+        std::vector<std::vector<uint32_t>> adjacency(n);
         for (uint64_t i = 0; i < edge_count; ++i) {
             const uint32_t a = edges[2 * i], b = edges[2 * i + 1];
             require(a < n && b < n && a != b, "graph edge out of range or a self-loop");
-            rows[a].push_back(b);
-            rows[b].push_back(a);
+            adjacency[a].push_back(b);
+            adjacency[b].push_back(a);
         }
         p.graph_offsets.assign(1, 0);
-        for (auto& row : rows) {
+        for (auto& row : adjacency) {
             std::sort(row.begin(), row.end());
             require(std::adjacent_find(row.begin(), row.end()) == row.end(), "graph holds a repeated edge");
             p.graph_neighbors.insert(p.graph_neighbors.end(), row.begin(), row.end());
@@ -2012,16 +2018,68 @@ int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const 
         };
         for (uint32_t i = 0; i < n; ++i)
             require(member(p.initial[i], p.initial[(i + 1) % n]), "the graph misses an edge of the initial tour");
+        auto d = [&](uint32_t a, uint32_t b) { return euc2d(p.packed[a], p.packed[b]); };
+        // fixed-width rows, cities before padding, no repeats or self-loops;
+        // primary and LS members in the graph; LS rows in (distance, city) order
+        auto check_rows = [&](const uint32_t* rows, uint32_t width, bool in_graph, bool ordered) {
+            for (uint32_t a = 0; a < n; ++a) {
+                const uint32_t* row = rows + static_cast<size_t>(a) * width;
+                bool padded = false;
+                int64_t last_d = -1;
+                uint32_t last_b = 0;
+                for (uint32_t j = 0; j < width; ++j) {
+                    const uint32_t b = row[j];
+                    if (b == n) { padded = true; continue; }
+                    require(!padded && b < n && b != a && std::find(row, row + j, b) == row + j,
+                            "graph rows hold an unknown or repeated city, a self-loop or a city after padding");
+                    require(!in_graph || member(a, b), "a primary or LS row city is not in the graph");
+                    const int64_t db = d(a, b);
+                    require(!ordered || last_d < db || (last_d == db && last_b < b),
+                            "LS rows must be ordered by distance, then city");
+                    last_d = db; last_b = b;
+                }
+            }
+        };
+        check_rows(primary, pw, true, false);
+        if (bw) check_rows(backup, bw, false, false);
+        check_rows(ls, lw, true, true);
+        for (uint32_t a = 0; a < n; ++a)
+            for (uint32_t j = 0; j < bw; ++j) {
+                const uint32_t b = backup[static_cast<size_t>(a) * bw + j];
+                require(b == n || std::find(primary + static_cast<size_t>(a) * pw, primary + static_cast<size_t>(a + 1) * pw, b) ==
+                                      primary + static_cast<size_t>(a + 1) * pw,
+                        "a backup row repeats a primary row city");
+            }
+        // the spec's rows replace the kNN rows: split neighbour rows, their
+        // distances, backup rows and the heuristic of the primary slots
+        // (padding slots are never available; their weight is 0)
+        p.primary.assign(primary, primary + static_cast<size_t>(n) * pw);
+        p.ls.assign(ls, ls + static_cast<size_t>(n) * lw);
+        p.backup.assign(static_cast<size_t>(n) * bw, n);
+        if (bw) std::copy(backup, backup + static_cast<size_t>(n) * bw, p.backup.begin());
+        p.backup16.assign(p.backup.begin(), p.backup.end());
+        p.nbr.assign(e.nbr_stride, static_cast<uint16_t>(n));
+        p.nbr_dist.assign(e.nbr_stride, 0);
         for (uint32_t a = 0; a < n; ++a) {
-            for (uint32_t j = 0; j < pw; ++j) require(member(a, p.primary[a * pw + j]), "a primary row edge is not in the graph");
-            for (uint32_t j = 0; j < lw; ++j) require(member(a, p.ls[a * lw + j]), "an LS row edge is not in the graph");
+            for (uint32_t j = 0; j < pw + lw; ++j) {
+                const uint32_t b = j < pw ? primary[static_cast<size_t>(a) * pw + j] : ls[static_cast<size_t>(a) * lw + j - pw];
+                p.nbr[static_cast<size_t>(a) * e.K + j] = static_cast<uint16_t>(b);
+                p.nbr_dist[static_cast<size_t>(a) * e.K + j] = b < n ? static_cast<uint16_t>(d(a, b)) : 0;
+            }
+            for (uint32_t j = 0; j < pw; ++j) {
+                const uint32_t b = primary[static_cast<size_t>(a) * pw + j];
+                const double db = b < n ? static_cast<double>(d(a, b)) : 0.0;
+                p.eta[static_cast<size_t>(a) * pw + j] =
+                    b >= n ? 0.0 : db > 0 ? (e.config.beta == 1.0 ? 1.0 / db : 1.0 / std::pow(db, e.config.beta)) : 1.0;
+            }
         }
         p.preparation_seconds += seconds_since(started);
         const auto existing = e.registry.find(key);
         if (existing != e.registry.end()) {
-            require(existing->second.xy == p.xy && existing->second.graph_offsets == p.graph_offsets &&
-                    existing->second.graph_neighbors == p.graph_neighbors,
-                    "same instance key registered with different coordinates or graph");
+            const Prepared& q = existing->second;
+            require(q.xy == p.xy && q.graph_offsets == p.graph_offsets && q.graph_neighbors == p.graph_neighbors &&
+                    q.primary == p.primary && q.backup == p.backup && q.ls == p.ls,
+                    "same instance key registered with different coordinates, graph or rows");
         } else {
             e.registry.emplace(key, std::move(p));
         }
@@ -2046,6 +2104,8 @@ int faco_engine_edge_features(faco_engine* engine, uint64_t key, float* features
         Engine& e = *engine->impl;
         const Prepared& p = e.problem(key);
         const uint32_t n = p.n, pw = e.config.primary_width;
+        require(std::find(p.primary.begin(), p.primary.end(), n) == p.primary.end(),
+                "edge features need primary rows without padding");
         double mean = 0;
         for (uint32_t a = 0; a < n; ++a) mean += p.local_scale[a];
         const float mean_scale = static_cast<float>(std::max(mean / n, p.scale_epsilon));

@@ -217,7 +217,8 @@ constexpr size_t control_candidate_bytes(uint32_t ants) { return 40u * static_ca
 
 struct AntParams {
     uint32_t nbr_in_lds;                // neighbour rows staged in LDS (else read from HBM)
-    uint32_t n, ants, pw, bw, lw, K;   // K = neighbour row width = max(pw, lw)
+    uint32_t n, ants, pw, bw, lw, K;   // K = neighbour row width: max(pw, lw), or pw + lw for split rows
+    uint32_t ls_offset;                 // LS row within a neighbour row: 0 (prefix of the primary order) or pw
     uint32_t waves;                     // waves per workgroup
     uint32_t chunk;                     // ants of one colony handled by one workgroup
     uint32_t blocks_per_colony;         // workgroups per colony (explicit mode: unused)

@@ -964,7 +964,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         // Lane constants of the branch-free scan: an in-row column for every
         // lane, the step to a neighbour's tour neighbour (+1 successor, -1
         // predecessor) and where a step off either array end lands.
-        const uint32_t column = cslot < K ? cslot : K - 1u;
+        // (every lane reads inside the LS row, which starts at ls_offset)
+        const uint32_t column = p.ls_offset + (cslot < lw ? cslot : lw - 1u);
         const bool scan_lane = cslot < lw;
         const uint32_t off_scan = scan_lane ? 0u : 0xffffffffu;   // all ones outside the scan
         const uint16_t* g_nd = p.nbr_dist + static_cast<size_t>(geom) * p.nbr_stride;

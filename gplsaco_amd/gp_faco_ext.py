@@ -606,8 +606,9 @@ class FacoBatchEngine:
     def register_graph_problem(self, instance_key, coordinates, dictionary) -> dict:
         """Hard-mode registration (reference bindings.cpp:574-622): the spec
         holds graph_spec_id 1, common_initial_tour, edges [[a, b], ...] and the
-        primary, backup and ls rows ([n][width] lists, right-padded with n).
-        The rows and the initial tour must be the prepared ones."""
+        primary, backup and ls rows ([n][width] lists, right-padded with n),
+        which replace the kNN rows; the initial tour must be the prepared
+        one."""
         xy = _coordinates(coordinates)
         fields = ("graph_spec_id", "common_initial_tour", "edges", "primary", "backup", "ls")
         if not isinstance(dictionary, dict) or set(dictionary) != set(fields):

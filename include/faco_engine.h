@@ -295,11 +295,12 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t instance_key, con
  * prepared_problem.cpp:128-166, bindings.cpp:574-622): the common
  * preparation plus the candidate graph, edges [edge_count][2] (undirected,
  * no self-loops or repeats, at most n * (primary_width + 1), holding the
- * initial tour and every primary and LS row edge). The spec's initial tour
- * and rows ([n][width], as problem_data returns them) must equal the
- * prepared ones: this engine keeps one row layout. In Hard mode every
- * relocation and 2-opt move keeps the tour inside the graph (construction
- * exhausts instead of leaving it); plain 2-opt, plain FACO evaluations. */
+ * initial tour), and the spec's own rows: primary [n][primary_width] and
+ * LS [n][ls_width] members of the graph (LS rows ordered by distance, then
+ * city), backup [n][backup_width] disjoint from the primary row; every row
+ * right-padded with n. The spec's initial tour must be the prepared one.
+ * In Hard mode every relocation and 2-opt move keeps the tour inside the
+ * graph (construction exhausts instead of leaving it); plain 2-opt. */
 int faco_engine_register_graph_problem(faco_engine* engine, uint64_t instance_key, const double* xy, uint32_t n,
                                        const uint32_t* edges, uint64_t edge_count, const uint32_t* initial_tour,
                                        const uint32_t* primary, const uint32_t* backup, const uint32_t* ls,

@@ -340,6 +340,7 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
             for (u32 j = 0; j < lw; ++j) {
                 const u32 b = ls[static_cast<size_t>(a) * lw + j];
                 ++st.checks;
+                if (b >= N) break;   // right padding of an LS row (faco_device.cuh:320)
                 const i64 ab = t.p->d(a, b);
                 if (!(current > ab)) break;
                 if (st.evaluations == limit) { interrupted = true; break; }
@@ -617,7 +618,7 @@ void colony_reset(const Problem& p, ColonyRun& r) {
     r.products.resize(static_cast<size_t>(n) * pw);
     for (u32 a = 0; a < n; ++a) for (u32 j = 0; j < pw; ++j) {
         const size_t i = static_cast<size_t>(a) * pw + j;
-        r.eta[i] = heuristic(p.d(a, p.primary[i]), p.s.beta);
+        r.eta[i] = p.primary[i] < n ? heuristic(p.d(a, p.primary[i]), p.s.beta) : 0.0;   // padding: never available
         if (!p.prior.empty()) r.eta[i] *= static_cast<double>(p.prior[i]);
         r.products[i] = r.trails[i] * r.eta[i];
     }
@@ -1354,6 +1355,19 @@ int oracle_select_next_hard(u32 n, const std::int32_t* xy, u32 pw, u32 bw, const
 // global tour, global cost and work counters. Optional per-batch trace for
 // colony 0: parent after each batch [batches][n] and trails after each batch
 // [batches][n*pw]. Threads: OpenMP over (colony, ant) within a batch.
+// A graph spec's own candidate rows (prepared_problem.cpp:128-166), right
+// padded with n: primary [n][pw], backup [n][bw] and LS [n][lw] one after
+// the other, replacing the prepared kNN rows (the initial tour stays).
+size_t row_cells(const Problem& p) {
+    return static_cast<size_t>(p.n) * (p.s.primary_width + p.s.backup_width + p.s.ls_width);
+}
+void apply_rows(Problem& p, const u32* rows) {
+    const size_t n = p.n, pw = p.s.primary_width, bw = p.s.backup_width, lw = p.s.ls_width;
+    p.primary.assign(rows, rows + n * pw);
+    p.backup.assign(rows + n * pw, rows + n * (pw + bw));
+    p.ls.assign(rows + n * (pw + bw), rows + n * (pw + bw + lw));
+}
+
 int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64* instance_keys,
                     u32 colonies, const u64* seeds, const Settings* settings, u32 batches,
                     u32 mne_target, int threads,
@@ -1364,7 +1378,8 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
                     const u32* graph_offsets /* [instances][n + 1] or null: Hard mode */,
                     const u32* graph_neighbors /* [instances][neighbor_stride] */, u32 neighbor_stride,
                     u32 timed_from /* elapsed_search_seconds covers batches timed_from.. only */,
-                    u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */) {
+                    u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */,
+                    const u32* rows /* [instances][n (pw + bw + lw)] or null: a graph spec's own rows */) {
     return guarded([&] {
         if (!instances || !colonies) throw std::invalid_argument("empty panel");
         if (graph_offsets && local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
@@ -1374,6 +1389,7 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             problems[g].n = n; problems[g].xy = xy_all + static_cast<size_t>(g) * 2 * n;
             problems[g].s = normalized(*settings, n);
             prepare(problems[g]);
+            if (rows) apply_rows(problems[g], rows + static_cast<size_t>(g) * row_cells(problems[g]));
             problems[g].or_opt = local_search != 0;
             if (graph_offsets) {
                 problems[g].graph_offsets = graph_offsets + static_cast<size_t>(g) * (n + 1);
@@ -1572,7 +1588,8 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           const float* multipliers /* [instances][n * pw] or null */,
                           const double* progress /* [batches]: the progress feature of each batch, or null:
                                                     completed FE / evaluation_limit */,
-                          u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */) {
+                          u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */,
+                          const u32* rows /* [instances][n (pw + bw + lw)] or null: a graph spec's own rows */) {
     // stop_batch (0 = the whole budget) ends the run early with the progress
     // feature still measured against evaluation_limit. At batch fork_batch
     // (UINT32_MAX = none) a paired fork (reference cuda/batch_engine.cu:839-845,
@@ -1592,6 +1609,7 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
             problems[g].n = n; problems[g].xy = xy_all + static_cast<size_t>(g) * 2 * n;
             problems[g].s = normalized(*settings, n);
             prepare(problems[g]);
+            if (rows) apply_rows(problems[g], rows + static_cast<size_t>(g) * row_cells(problems[g]));
             problems[g].or_opt = local_search != 0;
             if (graph_offsets) {
                 if (local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");

@@ -192,7 +192,7 @@ def lib():
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
-                                      ctypes.c_void_p]
+                                      ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_prior_features.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, f32p]
         L.oracle_escape_slot_view.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
@@ -250,7 +250,7 @@ def lib():
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_void_p]
+                                            ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_colony_select.argtypes = [ctypes.c_uint32, i32p, ctypes.POINTER(Settings), ctypes.c_uint64,
                                            ctypes.c_uint32, ctypes.c_uint32, u32p, i64p, i64p, u32p, f64p, f64p,
                                            f64p, f64p, ctypes.POINTER(ctypes.c_uint32),
@@ -450,14 +450,27 @@ class alternate_stream:
         return False
 
 
+def _spec_rows(rows, instances):
+    """One (primary, backup, ls) triple per instance ([n][width] each, right
+    padded with n) as the oracle's contiguous block, or None."""
+    if rows is None:
+        return None
+    if len(rows) != instances:
+        raise ValueError("one set of rows per instance")
+    return np.ascontiguousarray(np.stack([np.concatenate([np.asarray(r, dtype=np.uint32).reshape(-1) for r in triple])
+                                          for triple in rows]))
+
+
 def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False,
-             or_opt=False, multipliers=None, graphs=None, timed_from=0):
+             or_opt=False, multipliers=None, graphs=None, timed_from=0, rows=None):
     """Fixed-iteration FACO over colonies (colony i uses instance i % instances).
     search_seconds times batches timed_from..batches-1 only (a window of the run).
     `multipliers` [instances, n, pw] (float32): learned heuristic multipliers,
     eta' = eta * multiplier in FP64. `graphs`: one undirected CSR
     (offsets, neighbors) per instance (csr()) for the Hard mode, in which
-    every move must keep the tour inside its graph."""
+    every move must keep the tour inside its graph; `rows`: one (primary,
+    backup, ls) triple per instance replacing the prepared rows, as a graph
+    spec's own rows do."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -488,6 +501,7 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
         g_neighbors = np.zeros((instances, stride), dtype=np.uint32)
         for g, (_, nb) in enumerate(graphs):
             g_neighbors[g, :len(nb)] = nb
+    spec_rows = _spec_rows(rows, instances)   # kept alive through the call
     L = lib()
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None  # noqa: E731
     _check(L.oracle_run_faco(n, instances, xy_instances.reshape(-1), keys, colonies, seeds,
@@ -495,7 +509,7 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
                              work.reshape(-1), ptr(trace_parent), ptr(trace_trails), ptr(trace_costs),
                              ctypes.byref(elapsed), int(bool(or_opt)),
                              ptr(None if multipliers is None else mult), ptr(g_offsets), ptr(g_neighbors), stride,
-                             int(timed_from), ptr(rejections)), L)
+                             int(timed_from), ptr(rejections), ptr(spec_rows)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
            "constraint_rejections": rejections}
     if trace:
@@ -941,7 +955,7 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
                    trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None, graphs=None,
-                   multipliers=None, progress=None):
+                   multipliers=None, progress=None, rows=None):
     """Controlled FACO: colony c uses instance c % instances and decider c // instances.
 
     multipliers [instances, n, pw] (float32): learned heuristic multipliers,
@@ -995,6 +1009,7 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         mult = np.ascontiguousarray(multipliers, dtype=np.float32)
         if mult.shape[0] != instances:
             raise ValueError("one multiplier table per instance")
+    spec_rows = _spec_rows(rows, instances)
     L = lib()
     _check(L.oracle_run_controlled(
         n, instances, xy_instances.reshape(-1), keys, colonies, seeds, ctypes.byref(settings), evaluation_limit,
@@ -1006,7 +1021,7 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
         ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt)),
         stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1,
-        ptr(g_off), ptr(g_nb), g_stride, ptr(mult), ptr(prog), ptr(rejections)), L)
+        ptr(g_off), ptr(g_nb), g_stride, ptr(mult), ptr(prog), ptr(rejections), ptr(spec_rows)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
            "constraint_rejections": rejections,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,

@@ -35,10 +35,13 @@ gp_faco::DistanceFunction matrix_distance(const double* matrix, Node n) {
     return [matrix, n](Node a, Node b) { return matrix[static_cast<std::size_t>(a) * n + b]; };
 }
 
+// Fixed-width rows as the reference's CandidateRows; entries >= n (a graph
+// spec's right padding) end a row.
 gp_faco::CandidateRows rows_of(const std::uint32_t* flat, Node n, Node width) {
     gp_faco::CandidateRows rows(n);
-    for (Node a = 0; a < n; ++a) rows[a].assign(flat + static_cast<std::size_t>(a) * width,
-                                                flat + static_cast<std::size_t>(a + 1) * width);
+    for (Node a = 0; a < n; ++a)
+        for (Node j = 0; j < width && flat[static_cast<std::size_t>(a) * width + j] < n; ++j)
+            rows[a].push_back(flat[static_cast<std::size_t>(a) * width + j]);
     return rows;
 }
 }  // namespace

@@ -167,3 +167,132 @@ def test_hard_engines_take_evaluation_counts_only(ext):
     out = hard.evaluate_faco_evaluations(keys, seeds, s.ants * 3)
     assert out["constraint_mode"] == "hard" and out["completed_constraint_rejections"] == int(
         hard.colony_constraint_rejections(2).sum())
+
+
+def _independent_spec(O, data, xy, rng, pw, bw, lw):
+    """A graph spec with rows of its own (reference prepared_problem.cpp:128-166):
+    the graph is the initial tour, a random half of each city's 12 nearest
+    edges and a few random chords; primary rows are graph neighbours in a
+    random order, LS rows graph neighbours by (distance, city), backup rows
+    random other cities; every row right-padded with n."""
+    n = xy.shape[0]
+    M = O.distance_matrix(xy.astype(np.int64))
+    tour = [int(v) for v in data["initial_tour"]]
+    edges = set(_cycle(tour))
+    for a in range(n):
+        near = [b for b in np.argsort(M[a], kind="stable") if b != a][:12]
+        for b in near:
+            if rng.random() < 0.5:
+                edges.add(tuple(sorted((a, int(b)))))
+    for _ in range(n // 8):
+        a, b = (int(v) for v in rng.choice(n, 2, replace=False))
+        edges.add(tuple(sorted((a, b))))
+    nbrs = [[] for _ in range(n)]
+    for a, b in edges:
+        nbrs[a].append(b)
+        nbrs[b].append(a)
+    primary = np.full((n, pw), n, dtype=np.int64)
+    ls = np.full((n, lw), n, dtype=np.int64)
+    backup = np.full((n, bw), n, dtype=np.int64)
+    for a in range(n):
+        chosen = list(rng.permutation(nbrs[a]))[:pw]
+        primary[a, :len(chosen)] = chosen
+        by_distance = sorted(nbrs[a], key=lambda b: (M[a, b], b))[:lw]
+        ls[a, :len(by_distance)] = by_distance
+        others = [int(b) for b in rng.permutation(n) if b != a and b not in chosen][:bw - int(rng.integers(0, 3))]
+        backup[a, :len(others)] = others
+    assert len(edges) <= n * (pw + 1)
+    spec = {"graph_spec_id": 1, "common_initial_tour": tour, "edges": [list(e) for e in sorted(edges)],
+            "primary": primary.tolist(), "backup": backup.tolist(), "ls": ls.tolist()}
+    return spec, edges, (primary, backup, ls)
+
+
+@pytest.mark.parametrize("n", [200, 1100])
+def test_hard_mode_with_independent_rows_matches_the_oracle(ext, oracle_mod, n):
+    """General Hard-mode graphs: the spec's primary, backup and LS rows
+    replace the kNN rows (split neighbour rows on the device, padding
+    included); plain and GP-controlled runs equal the oracle given the same
+    rows, every tour inside its graph."""
+    O = oracle_mod
+    count, batches = 2, 5
+    xys = [(clustered_instance if i % 2 else uniform_instance)(n, seed=59, instance=i) for i in range(count)]
+    keys = [700 + i for i in range(count)]
+    s = ext.FixedFacoSettings()
+    s.ants = 32
+    s.backup_width = 8
+    plain = ext.FacoBatchEngine(n, 4, s)
+    hard = ext.FacoBatchEngine(n, 4, s, constraint_mode="hard")
+    rng = np.random.default_rng(n + 1)
+    graphs, members, rows = [], [], []
+    for k, xy in zip(keys, xys):
+        plain.register_problem(k, xy)
+        spec, edges, triple = _independent_spec(O, plain.problem_data(k), xy, rng, s.primary_width, s.backup_width,
+                                                s.ls_width)
+        hard.register_graph_problem(k, xy, spec)
+        data = hard.problem_data(k)
+        assert data["primary"].tolist() == triple[0].tolist() and data["ls"].tolist() == triple[2].tolist()
+        graphs.append(O.csr(sorted(edges), n))
+        members.append(edges)
+        rows.append(triple)
+    task_keys = np.array([keys[c % count] for c in range(4)], dtype=np.uint64)
+    seeds = np.array([1, 1, 2, 2], dtype=np.uint64)
+    settings = O.default_settings(**{f: getattr(s, f) for f in (
+        "ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
+        "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")})
+    announce("hard-rows", n)
+    got = hard.evaluate_faco_evaluations(task_keys, seeds, s.ants * batches)
+    want = O.run_faco(np.stack(xys).astype(np.int32), keys, seeds, settings, batches, 8, graphs=graphs, rows=rows)
+    for c in range(4):
+        tour = got["items"][c]["tour"]
+        assert tour == want["tours"][c].tolist(), c
+        assert got["items"][c]["cost"] == float(want["costs"][c])
+        assert _cycle(tour) <= members[c % count], c
+    np.testing.assert_array_equal(got["colony_work"], want["work"])
+    np.testing.assert_array_equal(hard.colony_constraint_rejections(4), want["constraint_rejections"])
+    program = {"ir_version": 1, "numeric_spec_id": 1, "feature_spec_id": 2, "opcode": [0, 0, 2],
+               "operand": [4, 1, 0], "constant_bits": []}
+    raw = hard.evaluate_control_raw(task_keys, seeds, s.ants * batches * 2, "program",
+                                    programs=[ext.read_program(program)])
+    wantc = O.run_controlled(np.stack(xys).astype(np.int32), np.array(keys, dtype=np.uint64), seeds, settings,
+                             s.ants * batches * 2, "program", graphs=graphs, rows=rows,
+                             programs=[O.Program.from_buffer_copy(bytes(ext.read_program(program)))])
+    assert raw["tours"].tolist() == wantc["tours"].tolist()
+    np.testing.assert_array_equal(raw["work"], wantc["work"])
+    assert [st["restarts"] for st in raw["states"]] == [st["restarts"] for st in wantc["states"]]
+
+
+def test_independent_row_rules(ext, oracle_mod):
+    """The reference's row checks (prepared_problem.cpp:139-162): fixed
+    widths, cities before padding, no repeats, primary and LS members in the
+    graph, LS rows ordered by (distance, city), backup rows disjoint from
+    primary rows; the same key again with other rows is refused."""
+    n = 90
+    xy = uniform_instance(n, seed=61)
+    s = ext.FixedFacoSettings()
+    s.backup_width = 4
+    plain = ext.FacoBatchEngine(n, 2, s)
+    plain.register_problem(1, xy)
+    spec, _, (primary, backup, ls) = _independent_spec(oracle_mod, plain.problem_data(1), xy,
+                                                       np.random.default_rng(3), s.primary_width, s.backup_width,
+                                                       s.ls_width)
+    hard = ext.FacoBatchEngine(n, 2, s, constraint_mode="hard")
+
+    def variant(name, rows):
+        return dict(spec, **{name: rows.tolist()})
+    a = int(np.flatnonzero((ls < n).sum(axis=1) >= 2)[0])
+    swapped = ls.copy()
+    swapped[a, [0, 1]] = swapped[a, [1, 0]]
+    gap = primary.copy()
+    gap[a, 0] = n                                      # padding before a city
+    stranger = primary.copy()
+    outside = [b for b in range(n) if b != a and [min(a, b), max(a, b)] not in spec["edges"]][0]
+    stranger[a, 0] = outside                           # a primary city outside the graph
+    echo = backup.copy()
+    echo[a, 0] = primary[a, 0]                         # a backup city repeating a primary one
+    for bad in (variant("ls", swapped), variant("primary", gap), variant("primary", stranger),
+                variant("backup", echo)):
+        with pytest.raises(ValueError):
+            hard.register_graph_problem(1, xy, bad)
+    hard.register_graph_problem(1, xy, spec)
+    with pytest.raises(ValueError):
+        hard.register_graph_problem(1, xy, variant("backup", np.roll(backup, 1, axis=1)))

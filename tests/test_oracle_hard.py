@@ -285,3 +285,51 @@ def test_hard_select_next_against_reference_core():
         assert got == want, k
         stages.add(got[1])
     assert stages == {0, 1, 2, 3, 4}
+
+
+@needs_ref
+def test_padded_ls_rows_against_reference_core():
+    """A graph spec's LS rows are right-padded with n: the oracle's Hard 2-opt
+    over padded rows equals the reference core over the same rows without
+    the padding (a padded slot ends the row, faco_device.cuh:320)."""
+    rng = np.random.default_rng(41)
+    shorter = 0
+    for trial in range(80):
+        n = int(rng.integers(8, 50))
+        xy = rng.integers(0, 2000, size=(n, 2)).astype(np.int32)
+        M = O.distance_matrix(xy)
+        tour = rng.permutation(n).tolist()
+        knn = np.array([sorted((b for b in range(n) if b != a), key=lambda b: (M[a, b], b))[:6] for a in range(n)])
+        edges = _random_graph(rng, n, tour, knn, 0.5)
+        nbrs = [[] for _ in range(n)]
+        for a, b in edges:
+            nbrs[a].append(b)
+            nbrs[b].append(a)
+        lw = 8
+        ls = np.full((n, lw), n, dtype=np.uint32)
+        for a in range(n):
+            row = sorted(nbrs[a], key=lambda b: (M[a, b], b))[:lw]
+            ls[a, :len(row)] = row
+            shorter += len(row) < lw
+        checklist = rng.permutation(n)[:int(rng.integers(1, n + 1))]
+        limit = int(rng.choice([5, 2**64 - 1]))
+        got_tour, got_cost, evals, rejections = O.two_opt_hard(xy, ls, tour, checklist, limit, O.csr(edges, n))
+        ref_tour, ref_cost, counters = O.ref_two_opt_hard(M.astype(np.float64), ls, tour, checklist, limit, edges)
+        assert got_tour.tolist() == ref_tour.tolist(), trial
+        assert (got_cost, evals, rejections) == (ref_cost, counters[2], counters[5]), trial
+    assert shorter > 100
+
+
+def test_spec_rows_equal_to_the_prepared_ones_change_nothing():
+    n, count, batches = 50, 2, 4
+    xys = _instances(n, count, 29)
+    s = O.default_settings(ants=16, backup_width=6)
+    keys, seeds = np.array([8, 9], dtype=np.uint64), np.array([1, 2, 3, 4], dtype=np.uint64)
+    complete = [O.csr([(a, b) for a in range(n) for b in range(a + 1, n)], n) for _ in range(count)]
+    plain = O.run_faco(xys, keys, seeds, s, batches, 8, graphs=complete)
+    rows = []
+    for g in range(count):
+        prep = O.prepare(xys[g], s)
+        rows.append((prep["primary"], prep["backup"], prep["ls"]))
+    same = O.run_faco(xys, keys, seeds, s, batches, 8, graphs=complete, rows=rows)
+    assert same["tours"].tolist() == plain["tours"].tolist() and same["work"].tolist() == plain["work"].tolist()
