@@ -1976,7 +1976,7 @@ int faco_engine_register_problem(faco_engine* engine, uint64_t key, const double
 // rows, initial tour), then the candidate graph and its own rows. The graph
 // is undirected, without self-loops or repeated edges, holds the initial
 // tour and has at most n (primary_width + 1) edges; the primary and LS rows
-// are graph neighbours, LS rows ordered by (distance, city), backup rows
+// are graph neighbours, LS rows ordered by distance, backup rows
 // disjoint from the primary ones; every row is right-padded with n. A Hard
 // engine keeps split neighbour rows (primary row, then LS row) for them.
 int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const double* xy, uint32_t n,
@@ -2020,13 +2020,14 @@ int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const 
             require(member(p.initial[i], p.initial[(i + 1) % n]), "the graph misses an edge of the initial tour");
         auto d = [&](uint32_t a, uint32_t b) { return euc2d(p.packed[a], p.packed[b]); };
         // fixed-width rows, cities before padding, no repeats or self-loops;
-        // primary and LS members in the graph; LS rows in (distance, city) order
+        // primary and LS members in the graph; LS rows by non-decreasing
+        // distance (a row ordered by exact distance, then city, stays
+        // non-decreasing once distances are rounded, ties in either order)
         auto check_rows = [&](const uint32_t* rows, uint32_t width, bool in_graph, bool ordered) {
             for (uint32_t a = 0; a < n; ++a) {
                 const uint32_t* row = rows + static_cast<size_t>(a) * width;
                 bool padded = false;
                 int64_t last_d = -1;
-                uint32_t last_b = 0;
                 for (uint32_t j = 0; j < width; ++j) {
                     const uint32_t b = row[j];
                     if (b == n) { padded = true; continue; }
@@ -2034,9 +2035,8 @@ int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const 
                             "graph rows hold an unknown or repeated city, a self-loop or a city after padding");
                     require(!in_graph || member(a, b), "a primary or LS row city is not in the graph");
                     const int64_t db = d(a, b);
-                    require(!ordered || last_d < db || (last_d == db && last_b < b),
-                            "LS rows must be ordered by distance, then city");
-                    last_d = db; last_b = b;
+                    require(!ordered || last_d <= db, "LS rows must be ordered by distance");
+                    last_d = db;
                 }
             }
         };

@@ -264,8 +264,10 @@ def test_hard_mode_with_independent_rows_matches_the_oracle(ext, oracle_mod, n):
 def test_independent_row_rules(ext, oracle_mod):
     """The reference's row checks (prepared_problem.cpp:139-162): fixed
     widths, cities before padding, no repeats, primary and LS members in the
-    graph, LS rows ordered by (distance, city), backup rows disjoint from
-    primary rows; the same key again with other rows is refused."""
+    graph, LS rows ordered by distance, backup rows disjoint from primary
+    rows; the same key again with other rows is refused. Rows hold EUC_2D
+    distances, so two LS cities of one rounded distance may come in either
+    order (an exact-distance order stays valid after rounding)."""
     n = 90
     xy = uniform_instance(n, seed=61)
     s = ext.FixedFacoSettings()
@@ -279,7 +281,9 @@ def test_independent_row_rules(ext, oracle_mod):
 
     def variant(name, rows):
         return dict(spec, **{name: rows.tolist()})
-    a = int(np.flatnonzero((ls < n).sum(axis=1) >= 2)[0])
+    def euc(i, j):
+        return int(np.floor(np.hypot(*(xy[i] - xy[j])) + 0.5))
+    a = next(i for i in range(n) if ls[i, 1] < n and euc(i, ls[i, 0]) < euc(i, ls[i, 1]))
     swapped = ls.copy()
     swapped[a, [0, 1]] = swapped[a, [1, 0]]
     gap = primary.copy()
@@ -296,3 +300,47 @@ def test_independent_row_rules(ext, oracle_mod):
     hard.register_graph_problem(1, xy, spec)
     with pytest.raises(ValueError):
         hard.register_graph_problem(1, xy, variant("backup", np.roll(backup, 1, axis=1)))
+
+
+def test_matched_prior_graphs_run_in_hard_mode_like_the_oracle(ext, oracle_mod):
+    """Graphs matched from two candidate priors (candidate_prior.match_graphs,
+    reference graph_matching.py:37-169) register in a Hard engine as they
+    come (engine_graph_spec) and run like the oracle given the same graph and
+    rows: LS rows ordered by exact distance, primary rows by prior rank,
+    native backup tails and uniform backup slots, padding where a node has
+    fewer graph neighbours than the widths."""
+    from types import SimpleNamespace
+
+    from gplsaco_amd.candidate_prior import GraphSettings, engine_graph_spec, match_graphs
+    from tests.test_candidate_prior import matched_priors
+    O = oracle_mod
+    n, batches = 200, 4
+    xy = clustered_instance(n, seed=67, instance=0)
+    s = ext.FixedFacoSettings()
+    s.ants = 32
+    s.backup_width = 8
+    plain = ext.FacoBatchEngine(n, 2, s)
+    plain.register_problem(1, xy)
+    tour = tuple(int(v) for v in plain.problem_data(1)["initial_tour"])
+    problem = SimpleNamespace(instance_id="clustered-200", numeric_id=1, coordinates=tuple(map(tuple, xy.tolist())))
+    graphs = match_graphs(problem, tour, matched_priors(problem, 24, 11),
+                          GraphSettings(primary_width=s.primary_width, backup_width=8, ls_width=s.ls_width,
+                                        uniform_backup_slots=4))
+    settings = O.default_settings(**{f: getattr(s, f) for f in (
+        "ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
+        "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")})
+    for kind, graph in graphs.items():
+        hard = ext.FacoBatchEngine(n, 2, s, constraint_mode="hard")
+        hard.register_graph_problem(1, xy, engine_graph_spec(graph))
+        rows = tuple(np.asarray(graph[name], dtype=np.int64) for name in ("primary", "backup", "ls"))
+        edges = {tuple(e) for e in graph["edges"]}
+        seeds = np.array([3, 4], dtype=np.uint64)
+        announce("hard-matched", kind)
+        got = hard.evaluate_faco_evaluations(np.array([1, 1], dtype=np.uint64), seeds, s.ants * batches)
+        want = O.run_faco(xy[None].astype(np.int32), [1], seeds, settings, batches, 8,
+                          graphs=[O.csr(sorted(edges), n)], rows=[rows])
+        for c in range(2):
+            assert got["items"][c]["tour"] == want["tours"][c].tolist(), (kind, c)
+            assert _cycle(got["items"][c]["tour"]) <= edges
+        np.testing.assert_array_equal(got["colony_work"], want["work"])
+        np.testing.assert_array_equal(hard.colony_constraint_rejections(2), want["constraint_rejections"])
