@@ -180,6 +180,9 @@ struct Problem {
     // (null: unrestricted)
     const u32* graph_offsets = nullptr;
     const u32* graph_neighbors = nullptr;
+    // Escape mode on the same graph: 1 with its escape gates, 2 with them
+    // disabled (which must run exactly as Hard mode)
+    u32 escape = 0;
     i64 d(u32 a, u32 b) const { return euc2d(xy, a, b); }
     bool hard() const { return graph_offsets != nullptr; }
 };
@@ -281,11 +284,216 @@ bool hard_two_opt_allowed(const Problem& p, u32 a, u32 a_neighbor, u32 b, u32 b_
     return hard_edges_allowed(p, 2, removed, added);
 }
 
+// ---------------------------------------------------------------------------
+// Escape primitives (the E3 candidate-graph exit), restated from
+// cpp/include/gp_faco/escape_slots.hpp:18-71, escape_edges.hpp:13-122,
+// edge_constraints.hpp:15-33 and cuda/escape_device.cuh:8-27.
+
+constexpr u32 kEscCapacity = 64, kEscLsWidth = 20, kEscBackupWidth = 64;
+
+// EscapeRandom (cuda/escape_device.cuh:8-27) over the shared Philox: a word
+// stream per (batch, domain | ant, event) event, domain 0x40000000 for
+// construction gates and 0x80000000 for LS rows; counter block
+// (block, event, domain | ant, batch). Replaces cuRAND's stream (unpinned).
+struct EscStream {
+    u32 key[2], ant, batch, event, block = 0, index = 4, words[4] = {0, 0, 0, 0};
+    EscStream(u64 k, u32 b, u32 a, u32 e, bool ls) {
+        key[0] = static_cast<u32>(k); key[1] = static_cast<u32>(k >> 32);
+        batch = b; event = e; ant = (ls ? 0x80000000u : 0x40000000u) | a;
+    }
+    u32 draw() {
+        if (index == 4) {
+            const u32 ctr[4] = {block++, event, ant, batch};
+            philox4x32_10(ctr, key, words);
+            index = 0;
+        }
+        return words[index++];
+    }
+    u32 below(u32 count) {   // unbiased: reject words below 2^32 mod count
+        const u32 threshold = (0u - count) % count;
+        u32 v;
+        do { v = draw(); } while (v < threshold);
+        return v % count;
+    }
+};
+
+// The reference test's scripted draws (tests/cpp/test_escape_slots.cpp:13-17).
+struct ScriptDraws {
+    u32 rounding;
+    u32 draw() { return rounding; }
+    u32 below(u32 count) {
+        if (!count) throw std::invalid_argument("sampling an empty set");
+        return count - 1;
+    }
+};
+
+struct EscRow {
+    u32 members[kEscLsWidth] = {};
+    std::uint8_t replaced[kEscLsWidth] = {};
+    u32 valid = 0, replacements = 0;
+};
+
+// escape_slot_view (escape_slots.hpp:20-49).
+template<class Random>
+EscRow slot_view(const u32* original, u32 width, const u32* backup, u32 bw, u32 n, u32 from, Random& random) {
+    if (width > kEscLsWidth || bw > kEscBackupWidth) throw std::invalid_argument("escape row too wide");
+    EscRow row;
+    for (u32 j = 0; j < width; ++j) { row.members[j] = original[j]; row.valid += original[j] < n ? 1u : 0u; }
+    u32 wanted = row.valid / 4;
+    if (row.valid % 4) wanted += (random.draw() & 3u) < row.valid % 4 ? 1u : 0u;
+    if (!wanted) return row;
+    u32 eligible[kEscBackupWidth], count = 0;
+    for (u32 j = 0; j < bw; ++j) {
+        const u32 node = backup[j];
+        if (node >= n) break;
+        if (node == from) continue;
+        bool duplicate = false;
+        for (u32 k = 0; k < row.valid; ++k) duplicate |= original[k] == node;
+        for (u32 k = 0; k < count; ++k) duplicate |= eligible[k] == node;
+        if (!duplicate) eligible[count++] = node;
+    }
+    wanted = std::min(wanted, count);
+    u32 slots[kEscLsWidth];
+    for (u32 j = 0; j < row.valid; ++j) slots[j] = j;
+    for (u32 j = 0; j < wanted; ++j) {
+        const u32 pick_slot = j + random.below(row.valid - j);
+        std::swap(slots[pick_slot], slots[j]);
+        const u32 pick_node = j + random.below(count - j);
+        std::swap(eligible[pick_node], eligible[j]);
+        row.members[slots[j]] = eligible[j];
+        row.replaced[slots[j]] = 1;
+    }
+    row.replacements = wanted;
+    return row;
+}
+
+// sort_escape_ls_row (escape_slots.hpp:52-68): insertion sort of the valid
+// prefix by (distance, id); the replaced flags move with their members.
+void sort_row(EscRow& row, const double* distance_of) {
+    for (u32 j = 1; j < row.valid; ++j) {
+        const u32 node = row.members[j];
+        const std::uint8_t flag = row.replaced[j];
+        const double d = distance_of[node];
+        u32 at = j;
+        while (at) {
+            const u32 prior = row.members[at - 1];
+            const double previous = distance_of[prior];
+            if (previous < d || (previous == d && prior < node)) break;
+            row.members[at] = prior; row.replaced[at] = row.replaced[at - 1]; --at;
+        }
+        row.members[at] = node; row.replaced[at] = flag;
+    }
+}
+
+struct EscEdge { u32 a, b; };
+bool same_edge(EscEdge x, EscEdge y) { return (x.a == y.a && x.b == y.b) || (x.a == y.b && x.b == y.a); }
+
+// SparseGraphView::operator() (edge_constraints.hpp:15-28): binary search of b in a's CSR row.
+bool graph_has(const u32* offsets, const u32* neighbors, u32 dimension, u32 a, u32 b) {
+    if (a >= dimension || b >= dimension || a == b) return false;
+    u32 first = offsets[a], last = offsets[a + 1];
+    const u32 end = last;
+    while (first < last) {
+        const u32 middle = first + (last - first) / 2;
+        if (neighbors[middle] < b) first = middle + 1; else last = middle;
+    }
+    return first < end && neighbors[first] == b;
+}
+
+struct EscCache { EscEdge edges[kEscCapacity]; u32 size; u64 epoch; };
+bool cache_has(const EscCache& c, EscEdge e) {
+    for (u32 i = 0; i < c.size; ++i) if (same_edge(c.edges[i], e)) return true;
+    return false;
+}
+
+enum EscStatus : u32 { kAllowed = 0, kInvalidEdge = 1, kOutsideGraph = 2, kCapacity = 3 };
+struct EscProposal { EscEdge edges[3] = {}; u32 count = 0, expected_size = 0; u64 expected_epoch = 0; u32 status = kAllowed; };
+
+// prepare_escape_edges (escape_edges.hpp:45-72).
+EscProposal prepare_edges(u32 count, const EscEdge* removed, const EscEdge* added, const u32* offsets,
+                          const u32* neighbors, u32 dimension, const EscCache& cache, bool permit) {
+    EscProposal p;
+    p.expected_size = cache.size; p.expected_epoch = cache.epoch;
+    bool consumed[3] = {false, false, false};
+    for (u32 i = 0; i < count; ++i) {
+        const EscEdge e = added[i].a < added[i].b ? added[i] : EscEdge{added[i].b, added[i].a};
+        if (e.a >= dimension || e.b >= dimension || e.a == e.b) { p.status = kInvalidEdge; return p; }
+        bool unchanged = false;
+        for (u32 j = 0; j < count; ++j)
+            if (!consumed[j] && same_edge(e, removed[j])) { consumed[j] = true; unchanged = true; break; }
+        if (unchanged || graph_has(offsets, neighbors, dimension, e.a, e.b) || cache_has(cache, e)) continue;
+        if (!permit) { p.status = kOutsideGraph; return p; }
+        bool duplicate = false;
+        for (u32 j = 0; j < p.count; ++j) duplicate |= same_edge(e, p.edges[j]);
+        if (!duplicate) p.edges[p.count++] = e;
+    }
+    if (cache.size > kEscCapacity || p.count > kEscCapacity - cache.size) p.status = kCapacity;
+    return p;
+}
+
+// commit_escape_edges (escape_edges.hpp:74-84).
+bool commit_edges(EscCache& cache, const EscProposal& p) {
+    if (p.status != kAllowed || cache.epoch != p.expected_epoch || cache.size != p.expected_size ||
+        cache.size > kEscCapacity || p.count > kEscCapacity - cache.size) return false;
+    for (u32 i = 0; i < p.count; ++i) cache.edges[cache.size + i] = p.edges[i];
+    cache.size += p.count;
+    return true;
+}
+
+void load_cache(EscCache& c, const u32* edges, u32 size, u64 epoch) {
+    if (size > kEscCapacity) throw std::invalid_argument("escape cache holds at most 64 edges");
+    c.size = size; c.epoch = epoch;
+    for (u32 i = 0; i < kEscCapacity; ++i) c.edges[i] = i < size ? EscEdge{edges[2 * i], edges[2 * i + 1]} : EscEdge{0, 0};
+}
+
+void store_proposal(const EscProposal& p, u32* out) {
+    // status, count, expected size, expected epoch (low, high), edges [3][2]
+    out[0] = p.status; out[1] = p.count; out[2] = p.expected_size;
+    out[3] = static_cast<u32>(p.expected_epoch); out[4] = static_cast<u32>(p.expected_epoch >> 32);
+    for (u32 i = 0; i < 3; ++i) { out[5 + 2 * i] = p.edges[i].a; out[6 + 2 * i] = p.edges[i].b; }
+}
+
+
 struct LsStats {
     u64 processed = 0, checks = 0, evaluations = 0, accepted = 0, reactivations = 0;
     u64 rejections = 0;  // Hard mode: evaluated moves that would leave the graph (faco_cpu.cpp:180-183)
+    u64 capacity_rejections = 0, new_edges = 0;  // Escape mode: full-cache rejections, committed new edges
     u32 limit_reached = 0;
 };
+
+// Escape mode's local search context (cuda/escape_device.cuh:111-128,
+// cuda/faco_device.cuh:326-336, 435-447): this ant's LS rows (an anchor's row
+// is its escape view), their replaced-slot flags, and the ant's cache of
+// out-of-graph edges. A move through a replaced slot may add new edges.
+struct EscapeLs {
+    const u32* rows;
+    const std::uint8_t* replaced;
+    EscCache* cache;
+    std::vector<u32>* events;   // or null: accepted moves as (kind, a, b, slot, cache size, permit)
+};
+
+// prepare_escape_two_opt (escape_edges.hpp:117-122).
+EscProposal escape_two_opt_proposal(const Problem& p, const EscCache& cache, u32 a, u32 an, u32 b, u32 bn,
+                                    bool permit) {
+    const EscEdge removed[2] = {{a, an}, {b, bn}};
+    const EscEdge added[2] = {{a, b}, {an, bn}};
+    return prepare_edges(2, removed, added, p.graph_offsets, p.graph_neighbors, p.n, cache, permit);
+}
+
+// prepare_escape_relocation (escape_edges.hpp:101-115): move `node` to
+// directly after `target` in the working tour.
+EscProposal escape_relocation_proposal(const Problem& p, const Tour& t, const EscCache& cache, u32 target, u32 node,
+                                       bool permit) {
+    EscProposal q;
+    q.expected_size = cache.size; q.expected_epoch = cache.epoch;
+    if (target >= p.n || node >= p.n || target == node) { q.status = kInvalidEdge; return q; }
+    const u32 target_after = t.succ(target);
+    if (target_after == node) return q;
+    const u32 before = t.pred(node), after = t.succ(node);
+    const EscEdge removed[3] = {{before, node}, {node, after}, {target, target_after}};
+    const EscEdge added[3] = {{before, after}, {target, node}, {node, target_after}};
+    return prepare_edges(3, removed, added, p.graph_offsets, p.graph_neighbors, p.n, cache, permit);
+}
 
 // Checklist 2-opt: restates CpuTour::checklist_two_opt
 // (cpp/src/faco_cpu.cpp:148-217). `checklist` is the whole FIFO history; the
@@ -317,8 +525,9 @@ void two_opt_move(Tour& t, u32 x1, u32 x2, u32 y1, u32 y2) {
 }  // namespace
 
 LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
-                std::vector<std::uint8_t>& queued, u64 limit, bool or_opt) {
+                std::vector<std::uint8_t>& queued, u64 limit, bool or_opt, const EscapeLs* esc = nullptr) {
     LsStats st;
+    if (esc) ls = esc->rows;
     const u32 N = t.n();
     size_t next = 0;
     i64 gain_sum = 0;
@@ -331,6 +540,7 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
         const i64 to_next = t.p->d(a, a_next), to_prev = t.p->d(a_prev, a);
         i64 best = -1;
         u32 move[4] = {0, 0, 0, 0};
+        u32 best_slot = 0, best_kind = 0, best_b = 0, best_nb = 0;
         bool interrupted = false, or_best = false;
         u32 or_move[6] = {0, 0, 0, 0, 0, 0};  // p, a, last, nx, c, d
         u32 or_forward = 0;
@@ -351,12 +561,24 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
                 // Hard mode: a move that leaves the graph never wins
                 // (cuda/faco_device.cuh:383-394); it still counts as evaluated
                 const bool identity = nb == a || b == closing;
-                const bool allowed = identity || !t.p->hard() || hard_two_opt_allowed(*t.p, a, closing, b, nb);
+                bool allowed;
+                if (esc) {
+                    // Escape mode: graph edges, cached edges, or new edges
+                    // through a replaced slot while the cache has room; an
+                    // identity move always passes (its added edges cancel)
+                    const EscProposal q = escape_two_opt_proposal(*t.p, *esc->cache, a, closing, b, nb,
+                                                                  esc->replaced[static_cast<size_t>(a) * lw + j] != 0);
+                    st.capacity_rejections += q.status == kCapacity ? 1 : 0;
+                    allowed = q.status == kAllowed;
+                } else {
+                    allowed = identity || !t.p->hard() || hard_two_opt_allowed(*t.p, a, closing, b, nb);
+                }
                 st.rejections += allowed ? 0 : 1;
                 const i64 gain = identity ? 0
                     : allowed ? current + t.p->d(b, nb) - ab - t.p->d(closing, nb) : std::numeric_limits<i64>::min();
                 if (gain > best) {
                     best = gain;
+                    best_slot = j; best_kind = static_cast<u32>(kind); best_b = b; best_nb = nb;
                     if (kind == 0) { move[0] = a_next; move[1] = nb; move[2] = a; move[3] = b; }
                     else { move[0] = a; move[1] = b; move[2] = a_prev; move[3] = nb; }
                 }
@@ -412,6 +634,17 @@ LsStats two_opt(Tour& t, const u32* ls, u32 lw, std::vector<u32>& checklist,
                     if (!queued[v]) { queued[v] = 1; checklist.push_back(v); ++st.reactivations; }
                 }
             } else {
+                if (esc) {
+                    // commit the move's new edges to the ant's cache
+                    const EscProposal q = escape_two_opt_proposal(
+                        *t.p, *esc->cache, a, best_kind == 0 ? a_next : a_prev, best_b, best_nb,
+                        esc->replaced[static_cast<size_t>(a) * lw + best_slot] != 0);
+                    if (!commit_edges(*esc->cache, q)) throw std::runtime_error("escape 2-opt commit refused");
+                    st.new_edges += q.count;
+                    if (esc->events)
+                        esc->events->insert(esc->events->end(), {best_kind + 1, a, best_b, best_slot, esc->cache->size,
+                                                                 esc->replaced[static_cast<size_t>(a) * lw + best_slot]});
+                }
                 t.flip(move[0], move[1]);
                 for (u32 v : move) {
                     if (!queued[v]) { queued[v] = 1; checklist.push_back(v); ++st.reactivations; }
@@ -475,12 +708,21 @@ u32 select_next(const Problem& p, u32 current, const double* products,
 
 struct ConstructionStats { u32 mne = 0, steps = 0, relocations = 0, exhausted = 0; };
 
+// Escape counters of one ant, in the order of the reference's EscapeStats
+// (escape_slots.hpp:70-75).
+enum EscCounter : u32 {
+    kOpportunities, kGates, kReplacedSlots, kEscapeRelocations, kAnchorNodes, kLsReplacedSlots,
+    kOldViewReactivations, kAnchorReactivations, kNewEdges, kConstructionCapacity, kLsCapacity, kEscCounters
+};
+
 struct AntResult {
     std::vector<u32> construction_tour, tour, positions, checklist;
     i64 construction_cost = 0, final_cost = 0;
     ConstructionStats c;
     LsStats ls;
     u32 start = 0;
+    std::vector<std::uint8_t> footprint;   // Escape mode: cities whose LS row had replaced slots
+    u64 esc[kEscCounters] = {};
 };
 
 // One ant: FocusedConstruction (cpp/src/faco_cpu.cpp:219-245) driven by a
@@ -605,6 +847,11 @@ struct ColonyRun {
     u32 iteration_best = 0, source_is_epoch = 0;
     std::vector<double> trails, products, eta;
     u64 construction_steps = 0, ls_evaluations = 0, constraint_rejections = 0;
+    // Escape mode: footprints (per city: the LS row of that city had
+    // replaced slots in the ant that produced the tour) of the parent, epoch
+    // and global tours, and the colony's escape counters
+    std::vector<std::uint8_t> parent_fp, epoch_fp, global_fp;
+    u64 esc[kEscCounters] = {};
 };
 
 void colony_reset(const Problem& p, ColonyRun& r) {
@@ -624,6 +871,8 @@ void colony_reset(const Problem& p, ColonyRun& r) {
     }
     r.construction_steps = r.ls_evaluations = r.constraint_rejections = 0;
     r.source_uniform = 0; r.source_is_epoch = 0; r.iteration_best = 0;
+    r.parent_fp.assign(n, 0); r.epoch_fp.assign(n, 0); r.global_fp.assign(n, 0);
+    std::fill(r.esc, r.esc + kEscCounters, 0ull);
 }
 
 // reduce_and_select + update_pheromone for one batch given every ant's result.
@@ -635,6 +884,7 @@ void colony_select_and_update(const Problem& p, ColonyRun& r, const std::vector<
         r.construction_steps += ants[a].c.steps;
         r.ls_evaluations += ants[a].ls.evaluations;
         r.constraint_rejections += ants[a].ls.rejections;
+        for (u32 k = 0; k < kEscCounters; ++k) r.esc[k] += ants[a].esc[k];
         if (ants[a].final_cost < ants[best].final_cost) best = a;
     }
     r.iteration_best = best;
@@ -648,6 +898,13 @@ void colony_select_and_update(const Problem& p, ColonyRun& r, const std::vector<
     r.source_is_epoch = r.source_uniform < p.s.epoch_source_probability;
     r.parent_cost = r.source_is_epoch ? r.epoch_cost : cost;
     r.parent = r.source_is_epoch ? r.epoch : ants[best].tour;
+    if (p.escape) {
+        // footprints follow their tours (cuda/colony_state.cuh:201-206)
+        const std::vector<std::uint8_t>& value = ants[best].footprint;
+        if (improve_epoch) r.epoch_fp = value;
+        if (improve_global) r.global_fp = value;
+        r.parent_fp = r.source_is_epoch ? r.epoch_fp : value;
+    }
     // Evaporate, then deposit once on each directed candidate entry whose
     // target is the parent predecessor or successor (SparsePheromone
     // evaporate/deposit, cpp/src/faco_cpu.cpp:322-343).
@@ -668,9 +925,184 @@ void colony_select_and_update(const Problem& p, ColonyRun& r, const std::vector<
     r.default_trail = std::max(r.minimum, r.default_trail * p.s.retention);
 }
 
+// One Escape-mode ant: construct_and_search with EscapeChoices and
+// EscapeEdges (cuda/faco_device.cuh:161-300, 326-336, 435-447;
+// cuda/escape_device.cuh:41-128). Each construction step draws the escape
+// gate (one word in 16 opens it) from the escape stream of (key, batch, ant,
+// step); an open gate swaps about a quarter of the current city's primary
+// row for backup cities (slot view). A replaced slot weighs
+// default_trail * d^-beta (the registered backup rows never hold primary
+// cities, so the reference's pheromone lookup returns the default trail)
+// and may add out-of-graph edges while the ant's 64-edge cache has room;
+// other moves keep to graph or cached edges. A new-edge relocation that
+// moves a city marks five anchors (current, its old successor, the city, its
+// old neighbours). After construction every anchor's LS row becomes its
+// escape view (escape stream of (key, batch, ant, city) in the LS domain),
+// sorted by (distance, city); the cities the parent's footprint names and
+// the anchors join the checklist in city order; the 2-opt then follows the
+// same permission rule per slot.
+// What one Escape ant did, for replay checks (the reference's diagnostic
+// capture, cuda/batch_engine.cu:169-176): its accepted moves in order as
+// (kind 0 relocation / 1 successor 2-opt / 2 predecessor 2-opt, a, b, slot,
+// cache size after the move, permit), its anchors, LS rows, replaced flags
+// and cache.
+struct EscapeTrace {
+    std::vector<u32> events, rows;
+    std::vector<std::uint8_t> anchors, replaced;
+    EscCache cache{};
+};
+
+AntResult escape_ant(const Problem& p, const ColonyRun& r, u64 key, u32 batch, u32 ant, u32 start, u32 target,
+                     EscapeTrace* trace = nullptr) {
+    AntResult res;
+    const u32 n = p.n, pw = p.s.primary_width, bw = p.s.backup_width, lw = p.s.ls_width;
+    const bool disabled = p.escape == 2;
+    Tour reference, work;
+    reference.assign(r.parent.data(), n, &p);
+    work = reference;
+    std::vector<std::uint8_t> visited(n, 0), queued(n, 0), anchors(n, 0);
+    std::vector<u32> checklist;
+    EscCache cache{};
+    u64* st = res.esc;
+    auto available = [&](u32 current, u32 node) {   // available_node with the ant's cache, no new edges
+        return node < n && !visited[node] &&
+               escape_relocation_proposal(p, work, cache, current, node, false).status == kAllowed;
+    };
+    u32 current = start;
+    res.start = start;
+    visited[current] = 1;
+    while (res.c.mne < target && res.c.steps + 1 < n) {
+        const u32 step = res.c.steps + 1;
+        Stream ordinary(key, batch, ant, step);
+        const double u = ordinary.uniform53();
+        const u32* prow = p.primary.data() + static_cast<size_t>(current) * pw;
+        const u32* brow = bw ? p.backup.data() + static_cast<size_t>(current) * bw : nullptr;
+        EscRow row;
+        for (u32 j = 0; j < pw; ++j) row.members[j] = prow[j];
+        ++st[kOpportunities];
+        EscStream gate(key, batch, ant, step, false);
+        if (!disabled && (gate.draw() & 15u) == 0) {
+            ++st[kGates];
+            row = slot_view(prow, pw, brow, bw, n, current, gate);
+            st[kReplacedSlots] += row.replacements;
+        }
+        double weights[kEscLsWidth] = {};
+        bool open[kEscLsWidth] = {};
+        double total = 0;
+        u32 count = 0, chosen = n, slot = n;
+        bool novel = false;
+        for (u32 j = 0; j < pw; ++j) {
+            const u32 node = row.members[j];
+            if (node >= n || visited[node]) continue;
+            const EscProposal q = escape_relocation_proposal(p, work, cache, current, node, row.replaced[j] != 0);
+            st[kConstructionCapacity] += q.status == kCapacity ? 1 : 0;
+            if (q.status != kAllowed) continue;
+            open[j] = true;
+            weights[j] = row.replaced[j] ? r.default_trail * heuristic(p.d(current, node), p.s.beta)
+                                         : r.products[static_cast<size_t>(current) * pw + j];
+            total += weights[j];
+            ++count;
+            chosen = node; novel = row.replaced[j] != 0; slot = j;
+        }
+        if (count) {
+            if (total > 0) {
+                double prefix = 0;
+                const double threshold = u * total;
+                for (u32 j = 0; j < pw; ++j) if (open[j]) {
+                    prefix += weights[j];
+                    if (threshold < prefix) { chosen = row.members[j]; novel = row.replaced[j] != 0; slot = j; break; }
+                }
+            } else {
+                u32 k = static_cast<u32>(u * count);
+                if (k >= count) k = count - 1;
+                for (u32 j = 0; j < pw; ++j) if (open[j] && k-- == 0) {
+                    chosen = row.members[j]; novel = row.replaced[j] != 0; slot = j; break;
+                }
+            }
+        } else {
+            for (u32 j = 0; j < bw && chosen == n; ++j) if (available(current, brow[j])) chosen = brow[j];
+            if (chosen == n) {
+                i64 best = std::numeric_limits<i64>::max();
+                for (u32 v = 0; v < n; ++v) {
+                    if (!available(current, v)) continue;
+                    const i64 value = p.d(current, v);
+                    if (value < best) { best = value; chosen = v; }
+                }
+            }
+        }
+        if (chosen >= n) { res.c.exhausted = 1; break; }
+        const EscProposal q = escape_relocation_proposal(p, work, cache, current, chosen, novel);
+        if (!commit_edges(cache, q)) throw std::runtime_error("escape relocation commit refused");
+        st[kNewEdges] += q.count;
+        const u32 previous = work.pred(chosen), after = work.succ(chosen), target_after = work.succ(current);
+        visited[chosen] = 1;
+        ++res.c.steps;
+        if (work.relocate(current, chosen)) {
+            ++res.c.relocations;
+            if (trace) trace->events.insert(trace->events.end(), {0u, current, chosen, slot, cache.size, novel ? 1u : 0u});
+            if (novel) {
+                for (u32 v : {current, target_after, chosen, previous, after}) anchors[v] = 1;
+                ++st[kEscapeRelocations];
+            }
+        }
+        if (!reference.has_edge(current, chosen)) {
+            ++res.c.mne;
+            for (u32 v : {current, chosen, previous})
+                if (!queued[v]) { queued[v] = 1; checklist.push_back(v); }
+        }
+        current = chosen;
+    }
+    res.construction_tour = work.order;
+    res.construction_cost = work.cost;
+    // build_ls_row (escape_device.cuh:111-126): anchors get their escape views
+    std::vector<u32> rows(p.ls);
+    std::vector<std::uint8_t> replaced(static_cast<size_t>(n) * lw, 0);
+    res.footprint.assign(n, 0);
+    std::vector<double> distance_of(n);
+    for (u32 node = 0; node < n; ++node) {
+        if (!anchors[node] || disabled) continue;
+        EscStream random(key, batch, ant, node, true);
+        const u32* lrow = p.ls.data() + static_cast<size_t>(node) * lw;
+        EscRow view = slot_view(lrow, lw, bw ? p.backup.data() + static_cast<size_t>(node) * bw : nullptr, bw, n, node,
+                                random);
+        for (u32 v = 0; v < n; ++v) distance_of[v] = static_cast<double>(p.d(node, v));
+        sort_row(view, distance_of.data());
+        for (u32 j = 0; j < lw; ++j) {
+            rows[static_cast<size_t>(node) * lw + j] = view.members[j];
+            replaced[static_cast<size_t>(node) * lw + j] = view.replaced[j];
+        }
+        res.footprint[node] = view.replacements != 0;
+    }
+    // the parent's footprint and the anchors reactivate their cities
+    // (cuda/faco_device.cuh:273-291)
+    for (u32 node = 0; node < n; ++node) {
+        if (r.parent_fp[node] && !queued[node]) {
+            queued[node] = 1; checklist.push_back(node); ++st[kOldViewReactivations];
+        }
+        if (anchors[node]) {
+            ++st[kAnchorNodes];
+            if (!queued[node]) { queued[node] = 1; checklist.push_back(node); ++st[kAnchorReactivations]; }
+            for (u32 j = 0; j < lw; ++j) st[kLsReplacedSlots] += replaced[static_cast<size_t>(node) * lw + j];
+        }
+    }
+    const EscapeLs esc{rows.data(), replaced.data(), &cache, trace ? &trace->events : nullptr};
+    res.ls = two_opt(work, p.ls.data(), lw, checklist, queued, p.s.ls_evaluation_limit, false, &esc);
+    if (trace) {
+        trace->anchors = anchors; trace->rows = rows; trace->replaced = replaced; trace->cache = cache;
+    }
+    st[kNewEdges] += res.ls.new_edges;
+    st[kLsCapacity] += res.ls.capacity_rejections;
+    res.tour = work.order;
+    res.positions = work.pos;
+    res.checklist = checklist;
+    res.final_cost = work.cost;
+    return res;
+}
+
 AntResult stochastic_ant(const Problem& p, const ColonyRun& r, u64 key, u32 batch, u32 ant, u32 target) {
     const u32 pw = p.s.primary_width;
     const u32 start = start_node(key, batch, ant, p.n);
+    if (p.escape) return escape_ant(p, r, key, batch, ant, start, target);
     return run_ant(p, r.parent.data(), start, target, p.s.ls_evaluation_limit,
         [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited, const Tour& work) {
             Stream stream(key, batch, ant, step);
@@ -722,6 +1154,7 @@ struct Rec {
     std::vector<u32> order, pos;
     i64 cost = 0;
     Ident id;
+    std::vector<std::uint8_t> fp;   // Escape mode: the tour's footprint (empty: none)
     u32 n() const { return static_cast<u32>(order.size()); }
     u32 succ(u32 v) const { return order[pos[v] + 1 == n() ? 0 : pos[v] + 1]; }
     u32 pred(u32 v) const { return order[pos[v] == 0 ? n() - 1 : pos[v] - 1]; }
@@ -1100,6 +1533,13 @@ void control_restart(const Problem& p, ColonyRun& r, Control& ctl, u32 alternati
     ctl.active = ctl.epoch_id = alt.id;
     if (!initializing) restart_feedback(ctl.fb);
     r.parent = r.epoch = alt.order;
+    if (p.escape) {
+        // the restart carries the archive entry's footprint
+        // (reference tests/cpp/test_escape_engine.cpp:93-98)
+        if (alt.fp.empty()) std::fill(r.parent_fp.begin(), r.parent_fp.end(), 0);
+        else r.parent_fp = alt.fp;
+        r.epoch_fp = r.parent_fp;
+    }
     for (size_t i = 0; i < static_cast<size_t>(n) * pw; ++i) {
         r.trails[i] = r.maximum;
         r.products[i] = r.maximum * r.eta[i];
@@ -1140,7 +1580,10 @@ std::vector<u32> archive_select(const std::vector<Rec>& pool, u32 first, i64 glo
 void control_update(const Problem& p, const ColonyRun& r, Control& ctl, const std::vector<AntResult>& ants) {
     const u32 n = p.n, A = static_cast<u32>(ants.size());
     std::vector<Rec> pool = ctl.archive;
-    for (const auto& a : ants) pool.push_back(make_rec(a.tour.data(), n, a.final_cost));
+    for (const auto& a : ants) {
+        pool.push_back(make_rec(a.tour.data(), n, a.final_cost));
+        pool.back().fp = a.footprint;   // archive footprints follow their entries (control_state.cuh:177-181)
+    }
     const u32 archived = static_cast<u32>(ctl.archive.size());
     const bool improved = r.global_cost < ctl.tracked_global;
     const std::vector<u32> chosen = archive_select(pool, improved ? archived + r.iteration_best : 0u, r.global_cost);
@@ -1172,6 +1615,7 @@ AntResult controlled_ant(const Problem& p, const ColonyRun& r, u64 key, u32 batc
                          const u32* start_nodes, u32 start_count) {
     const u32 pw = p.s.primary_width;
     const u32 start = start_nodes[start_node(key, batch, ant, start_count)];
+    if (p.escape) return escape_ant(p, r, key, batch, ant, start, target);
     return run_ant(p, r.parent.data(), start, target, p.s.ls_evaluation_limit,
         [&](u32 current, u32 step, const std::vector<std::uint8_t>& visited, const Tour& work) {
             Stream stream(key, batch, ant, step);
@@ -1379,10 +1823,14 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
                     const u32* graph_neighbors /* [instances][neighbor_stride] */, u32 neighbor_stride,
                     u32 timed_from /* elapsed_search_seconds covers batches timed_from.. only */,
                     u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */,
-                    const u32* rows /* [instances][n (pw + bw + lw)] or null: a graph spec's own rows */) {
+                    const u32* rows /* [instances][n (pw + bw + lw)] or null: a graph spec's own rows */,
+                    u32 escape /* 0, or Escape mode on the graph: 1 gates on, 2 gates disabled */,
+                    u64* escape_counters /* [colonies][11] or null */) {
     return guarded([&] {
         if (!instances || !colonies) throw std::invalid_argument("empty panel");
         if (graph_offsets && local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
+        if (escape > 2 || (escape && (!graph_offsets || multipliers)))
+            throw std::invalid_argument("Escape mode needs a graph and takes no learned prior");
         std::vector<Problem> problems(instances);
         for (u32 g = 0; g < instances; ++g) {
             check_coordinates(n, xy_all + static_cast<size_t>(g) * 2 * n);
@@ -1391,6 +1839,7 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             prepare(problems[g]);
             if (rows) apply_rows(problems[g], rows + static_cast<size_t>(g) * row_cells(problems[g]));
             problems[g].or_opt = local_search != 0;
+            problems[g].escape = escape;
             if (graph_offsets) {
                 problems[g].graph_offsets = graph_offsets + static_cast<size_t>(g) * (n + 1);
                 problems[g].graph_neighbors = graph_neighbors + static_cast<size_t>(g) * neighbor_stride;
@@ -1449,7 +1898,58 @@ int oracle_run_faco(u32 n, u32 instances, const std::int32_t* xy_all, const u64*
             work[2 * c] = runs[c].construction_steps;
             work[2 * c + 1] = runs[c].ls_evaluations;
             if (rejections) rejections[c] = runs[c].constraint_rejections;
+            if (escape_counters) std::copy(runs[c].esc, runs[c].esc + kEscCounters, escape_counters + kEscCounters * c);
         }
+    });
+}
+
+// One Escape-mode ant on explicit colony state, with its trace: the parent
+// tour, the colony's products [n][pw] and default trail, the parent's
+// footprint; rows (pw + bw + lw per city, a graph spec's rows) or null for
+// the prepared rows. Outputs: construction and final tour, anchors, LS rows
+// [n][lw] and replaced flags, footprint, cache [64][2] and its size, events
+// [2n][6] and their count, escape counters [11], ls [4] = evaluations,
+// accepted moves, rejections, checklist size; costs [2] = construction, final.
+int oracle_escape_ant(u32 n, const std::int32_t* xy, const Settings* settings, const u32* rows,
+                      const u32* graph_offsets, const u32* graph_neighbors, const u32* parent, const double* products,
+                      double default_trail, const std::uint8_t* parent_fp, u64 key, u32 batch, u32 ant, u32 start,
+                      u32 target, u32 disabled, u32* construction_tour, u32* tour, std::uint8_t* anchors,
+                      u32* ls_rows, std::uint8_t* replaced, std::uint8_t* footprint, u32* cache_edges,
+                      u32* cache_size, u32* events, u32* event_count, u64* counters, u64* ls,
+                      std::int64_t* costs) {
+    return guarded([&] {
+        check_coordinates(n, xy);
+        Problem p; p.n = n; p.xy = xy; p.s = normalized(*settings, n);
+        prepare(p);
+        if (rows) apply_rows(p, rows);
+        p.graph_offsets = graph_offsets; p.graph_neighbors = graph_neighbors;
+        p.escape = disabled ? 2u : 1u;
+        if (start >= n) throw std::invalid_argument("start city out of range");
+        ColonyRun r;
+        colony_reset(p, r);
+        r.parent.assign(parent, parent + n);
+        r.products.assign(products, products + static_cast<size_t>(n) * p.s.primary_width);
+        r.default_trail = default_trail;
+        r.parent_fp.assign(parent_fp, parent_fp + n);
+        EscapeTrace trace;
+        const AntResult a = escape_ant(p, r, key, batch, ant, start, target, &trace);
+        std::copy(a.construction_tour.begin(), a.construction_tour.end(), construction_tour);
+        std::copy(a.tour.begin(), a.tour.end(), tour);
+        std::copy(trace.anchors.begin(), trace.anchors.end(), anchors);
+        std::copy(trace.rows.begin(), trace.rows.end(), ls_rows);
+        std::copy(trace.replaced.begin(), trace.replaced.end(), replaced);
+        std::copy(a.footprint.begin(), a.footprint.end(), footprint);
+        for (u32 i = 0; i < kEscCapacity; ++i) {
+            cache_edges[2 * i] = i < trace.cache.size ? trace.cache.edges[i].a : 0u;
+            cache_edges[2 * i + 1] = i < trace.cache.size ? trace.cache.edges[i].b : 0u;
+        }
+        *cache_size = trace.cache.size;
+        if (trace.events.size() > static_cast<size_t>(12) * n) throw std::runtime_error("more than 2n moves");
+        std::copy(trace.events.begin(), trace.events.end(), events);
+        *event_count = static_cast<u32>(trace.events.size() / 6);
+        std::copy(a.esc, a.esc + kEscCounters, counters);
+        ls[0] = a.ls.evaluations; ls[1] = a.ls.accepted; ls[2] = a.ls.rejections; ls[3] = a.checklist.size();
+        costs[0] = a.construction_cost; costs[1] = a.final_cost;
     });
 }
 
@@ -1589,7 +2089,9 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
                           const double* progress /* [batches]: the progress feature of each batch, or null:
                                                     completed FE / evaluation_limit */,
                           u64* rejections /* [colonies] or null: Hard-mode 2-opt moves rejected */,
-                          const u32* rows /* [instances][n (pw + bw + lw)] or null: a graph spec's own rows */) {
+                          const u32* rows /* [instances][n (pw + bw + lw)] or null: a graph spec's own rows */,
+                          u32 escape /* 0, or Escape mode on the graph: 1 gates on, 2 gates disabled */,
+                          u64* escape_counters /* [colonies][11] or null */) {
     // stop_batch (0 = the whole budget) ends the run early with the progress
     // feature still measured against evaluation_limit. At batch fork_batch
     // (UINT32_MAX = none) a paired fork (reference cuda/batch_engine.cu:839-845,
@@ -1600,6 +2102,8 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
         if (!instances || !colonies || colonies % instances) throw std::invalid_argument("invalid panel");
         if (mode < 1 || mode > 4) throw std::invalid_argument("unknown control mode");
         if (!(experiment_mask & 0xffffu)) throw std::invalid_argument("experiment mask must keep a hold action");
+        if (escape > 2 || (escape && (!graph_offsets || multipliers)))
+            throw std::invalid_argument("Escape mode needs a graph and takes no learned prior");
         g_force_collisions = force_collisions != 0;
         struct Reset { ~Reset() { g_force_collisions = false; } } reset_flag;
         std::vector<Problem> problems(instances);
@@ -1611,6 +2115,7 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
             prepare(problems[g]);
             if (rows) apply_rows(problems[g], rows + static_cast<size_t>(g) * row_cells(problems[g]));
             problems[g].or_opt = local_search != 0;
+            problems[g].escape = escape;
             if (graph_offsets) {
                 if (local_search) throw std::invalid_argument("Hard mode takes the plain 2-opt");
                 problems[g].graph_offsets = graph_offsets + static_cast<size_t>(g) * (n + 1);
@@ -1719,6 +2224,7 @@ int oracle_run_controlled(u32 n, u32 instances, const std::int32_t* xy_all, cons
             work[2 * c] = runs[c].construction_steps;
             work[2 * c + 1] = runs[c].ls_evaluations;
             if (rejections) rejections[c] = runs[c].constraint_rejections;
+            if (escape_counters) std::copy(runs[c].esc, runs[c].esc + kEscCounters, escape_counters + kEscCounters * c);
             if (states) {
                 const Control& k = controls[c];
                 states[c] = ControlOut{static_cast<u32>(k.archive.size()), 0, k.fb.stagnant, k.fb.epoch_batches,
@@ -1876,176 +2382,6 @@ int oracle_prior_features(u32 n, const std::int32_t* xy, u32 pw, const u32* prim
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Escape primitives (the E3 candidate-graph exit), restated from
-// cpp/include/gp_faco/escape_slots.hpp:18-71, escape_edges.hpp:13-122,
-// edge_constraints.hpp:15-33 and cuda/escape_device.cuh:8-27.
-namespace {
-
-constexpr u32 kEscCapacity = 64, kEscLsWidth = 20, kEscBackupWidth = 64;
-
-// EscapeRandom (cuda/escape_device.cuh:8-27) over the shared Philox: a word
-// stream per (batch, domain | ant, event) event, domain 0x40000000 for
-// construction gates and 0x80000000 for LS rows; counter block
-// (block, event, domain | ant, batch). Replaces cuRAND's stream (unpinned).
-struct EscStream {
-    u32 key[2], ant, batch, event, block = 0, index = 4, words[4] = {0, 0, 0, 0};
-    EscStream(u64 k, u32 b, u32 a, u32 e, bool ls) {
-        key[0] = static_cast<u32>(k); key[1] = static_cast<u32>(k >> 32);
-        batch = b; event = e; ant = (ls ? 0x80000000u : 0x40000000u) | a;
-    }
-    u32 draw() {
-        if (index == 4) {
-            const u32 ctr[4] = {block++, event, ant, batch};
-            philox4x32_10(ctr, key, words);
-            index = 0;
-        }
-        return words[index++];
-    }
-    u32 below(u32 count) {   // unbiased: reject words below 2^32 mod count
-        const u32 threshold = (0u - count) % count;
-        u32 v;
-        do { v = draw(); } while (v < threshold);
-        return v % count;
-    }
-};
-
-// The reference test's scripted draws (tests/cpp/test_escape_slots.cpp:13-17).
-struct ScriptDraws {
-    u32 rounding;
-    u32 draw() { return rounding; }
-    u32 below(u32 count) {
-        if (!count) throw std::invalid_argument("sampling an empty set");
-        return count - 1;
-    }
-};
-
-struct EscRow {
-    u32 members[kEscLsWidth] = {};
-    std::uint8_t replaced[kEscLsWidth] = {};
-    u32 valid = 0, replacements = 0;
-};
-
-// escape_slot_view (escape_slots.hpp:20-49).
-template<class Random>
-EscRow slot_view(const u32* original, u32 width, const u32* backup, u32 bw, u32 n, u32 from, Random& random) {
-    if (width > kEscLsWidth || bw > kEscBackupWidth) throw std::invalid_argument("escape row too wide");
-    EscRow row;
-    for (u32 j = 0; j < width; ++j) { row.members[j] = original[j]; row.valid += original[j] < n ? 1u : 0u; }
-    u32 wanted = row.valid / 4;
-    if (row.valid % 4) wanted += (random.draw() & 3u) < row.valid % 4 ? 1u : 0u;
-    if (!wanted) return row;
-    u32 eligible[kEscBackupWidth], count = 0;
-    for (u32 j = 0; j < bw; ++j) {
-        const u32 node = backup[j];
-        if (node >= n) break;
-        if (node == from) continue;
-        bool duplicate = false;
-        for (u32 k = 0; k < row.valid; ++k) duplicate |= original[k] == node;
-        for (u32 k = 0; k < count; ++k) duplicate |= eligible[k] == node;
-        if (!duplicate) eligible[count++] = node;
-    }
-    wanted = std::min(wanted, count);
-    u32 slots[kEscLsWidth];
-    for (u32 j = 0; j < row.valid; ++j) slots[j] = j;
-    for (u32 j = 0; j < wanted; ++j) {
-        const u32 pick_slot = j + random.below(row.valid - j);
-        std::swap(slots[pick_slot], slots[j]);
-        const u32 pick_node = j + random.below(count - j);
-        std::swap(eligible[pick_node], eligible[j]);
-        row.members[slots[j]] = eligible[j];
-        row.replaced[slots[j]] = 1;
-    }
-    row.replacements = wanted;
-    return row;
-}
-
-// sort_escape_ls_row (escape_slots.hpp:52-68): insertion sort of the valid
-// prefix by (distance, id); the replaced flags move with their members.
-void sort_row(EscRow& row, const double* distance_of) {
-    for (u32 j = 1; j < row.valid; ++j) {
-        const u32 node = row.members[j];
-        const std::uint8_t flag = row.replaced[j];
-        const double d = distance_of[node];
-        u32 at = j;
-        while (at) {
-            const u32 prior = row.members[at - 1];
-            const double previous = distance_of[prior];
-            if (previous < d || (previous == d && prior < node)) break;
-            row.members[at] = prior; row.replaced[at] = row.replaced[at - 1]; --at;
-        }
-        row.members[at] = node; row.replaced[at] = flag;
-    }
-}
-
-struct EscEdge { u32 a, b; };
-bool same_edge(EscEdge x, EscEdge y) { return (x.a == y.a && x.b == y.b) || (x.a == y.b && x.b == y.a); }
-
-// SparseGraphView::operator() (edge_constraints.hpp:15-28): binary search of b in a's CSR row.
-bool graph_has(const u32* offsets, const u32* neighbors, u32 dimension, u32 a, u32 b) {
-    if (a >= dimension || b >= dimension || a == b) return false;
-    u32 first = offsets[a], last = offsets[a + 1];
-    const u32 end = last;
-    while (first < last) {
-        const u32 middle = first + (last - first) / 2;
-        if (neighbors[middle] < b) first = middle + 1; else last = middle;
-    }
-    return first < end && neighbors[first] == b;
-}
-
-struct EscCache { EscEdge edges[kEscCapacity]; u32 size; u64 epoch; };
-bool cache_has(const EscCache& c, EscEdge e) {
-    for (u32 i = 0; i < c.size; ++i) if (same_edge(c.edges[i], e)) return true;
-    return false;
-}
-
-enum EscStatus : u32 { kAllowed = 0, kInvalidEdge = 1, kOutsideGraph = 2, kCapacity = 3 };
-struct EscProposal { EscEdge edges[3] = {}; u32 count = 0, expected_size = 0; u64 expected_epoch = 0; u32 status = kAllowed; };
-
-// prepare_escape_edges (escape_edges.hpp:45-72).
-EscProposal prepare_edges(u32 count, const EscEdge* removed, const EscEdge* added, const u32* offsets,
-                          const u32* neighbors, u32 dimension, const EscCache& cache, bool permit) {
-    EscProposal p;
-    p.expected_size = cache.size; p.expected_epoch = cache.epoch;
-    bool consumed[3] = {false, false, false};
-    for (u32 i = 0; i < count; ++i) {
-        const EscEdge e = added[i].a < added[i].b ? added[i] : EscEdge{added[i].b, added[i].a};
-        if (e.a >= dimension || e.b >= dimension || e.a == e.b) { p.status = kInvalidEdge; return p; }
-        bool unchanged = false;
-        for (u32 j = 0; j < count; ++j)
-            if (!consumed[j] && same_edge(e, removed[j])) { consumed[j] = true; unchanged = true; break; }
-        if (unchanged || graph_has(offsets, neighbors, dimension, e.a, e.b) || cache_has(cache, e)) continue;
-        if (!permit) { p.status = kOutsideGraph; return p; }
-        bool duplicate = false;
-        for (u32 j = 0; j < p.count; ++j) duplicate |= same_edge(e, p.edges[j]);
-        if (!duplicate) p.edges[p.count++] = e;
-    }
-    if (cache.size > kEscCapacity || p.count > kEscCapacity - cache.size) p.status = kCapacity;
-    return p;
-}
-
-// commit_escape_edges (escape_edges.hpp:74-84).
-bool commit_edges(EscCache& cache, const EscProposal& p) {
-    if (p.status != kAllowed || cache.epoch != p.expected_epoch || cache.size != p.expected_size ||
-        cache.size > kEscCapacity || p.count > kEscCapacity - cache.size) return false;
-    for (u32 i = 0; i < p.count; ++i) cache.edges[cache.size + i] = p.edges[i];
-    cache.size += p.count;
-    return true;
-}
-
-void load_cache(EscCache& c, const u32* edges, u32 size, u64 epoch) {
-    if (size > kEscCapacity) throw std::invalid_argument("escape cache holds at most 64 edges");
-    c.size = size; c.epoch = epoch;
-    for (u32 i = 0; i < kEscCapacity; ++i) c.edges[i] = i < size ? EscEdge{edges[2 * i], edges[2 * i + 1]} : EscEdge{0, 0};
-}
-
-void store_proposal(const EscProposal& p, u32* out) {
-    // status, count, expected size, expected epoch (low, high), edges [3][2]
-    out[0] = p.status; out[1] = p.count; out[2] = p.expected_size;
-    out[3] = static_cast<u32>(p.expected_epoch); out[4] = static_cast<u32>(p.expected_epoch >> 32);
-    for (u32 i = 0; i < 3; ++i) { out[5 + 2 * i] = p.edges[i].a; out[6 + 2 * i] = p.edges[i].b; }
-}
-
-}  // namespace
 
 extern "C" {
 

@@ -192,7 +192,12 @@ def lib():
                                       u32p, i64p, u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
-                                      ctypes.c_void_p, ctypes.c_void_p]
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        L.oracle_escape_ant.argtypes = [ctypes.c_uint32, i32p, ctypes.POINTER(Settings), ctypes.c_void_p, u32p, u32p,
+                                        u32p, f64p, ctypes.c_double, u8p, ctypes.c_uint64, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p,
+                                        u8p, u32p, u8p, u8p, u32p, ctypes.POINTER(ctypes.c_uint32), u32p,
+                                        ctypes.POINTER(ctypes.c_uint32), u64p, u64p, i64p]
         L.oracle_prior_features.argtypes = [ctypes.c_uint32, i32p, ctypes.c_uint32, u32p, f64p, ctypes.c_double, f32p]
         L.oracle_escape_slot_view.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
@@ -250,7 +255,7 @@ def lib():
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_void_p, ctypes.c_void_p]
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
         L.oracle_colony_select.argtypes = [ctypes.c_uint32, i32p, ctypes.POINTER(Settings), ctypes.c_uint64,
                                            ctypes.c_uint32, ctypes.c_uint32, u32p, i64p, i64p, u32p, f64p, f64p,
                                            f64p, f64p, ctypes.POINTER(ctypes.c_uint32),
@@ -461,8 +466,14 @@ def _spec_rows(rows, instances):
                                           for triple in rows]))
 
 
+ESCAPE_COUNTERS = ("construction_opportunities", "construction_gates", "construction_replaced_slots",
+                   "escape_relocations", "ls_anchor_nodes", "ls_replaced_slots", "old_view_reactivations",
+                   "anchor_reactivations", "new_edges", "construction_capacity_rejections", "ls_capacity_rejections")
+ESCAPE_MODES = {None: 0, "on": 1, "disabled": 2}
+
+
 def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8, threads=0, trace=False,
-             or_opt=False, multipliers=None, graphs=None, timed_from=0, rows=None):
+             or_opt=False, multipliers=None, graphs=None, timed_from=0, rows=None, escape=None):
     """Fixed-iteration FACO over colonies (colony i uses instance i % instances).
     search_seconds times batches timed_from..batches-1 only (a window of the run).
     `multipliers` [instances, n, pw] (float32): learned heuristic multipliers,
@@ -470,7 +481,9 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
     (offsets, neighbors) per instance (csr()) for the Hard mode, in which
     every move must keep the tour inside its graph; `rows`: one (primary,
     backup, ls) triple per instance replacing the prepared rows, as a graph
-    spec's own rows do."""
+    spec's own rows do. `escape` ("on" or "disabled", with `graphs`): the
+    Escape mode on those graphs; "escape_counters" [colonies][11] in
+    ESCAPE_COUNTERS order."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -480,6 +493,7 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
     costs = np.zeros(colonies, dtype=np.int64)
     work = np.zeros((colonies, 2), dtype=np.uint64)
     rejections = np.zeros(colonies, dtype=np.uint64)
+    escape_counters = np.zeros((colonies, len(ESCAPE_COUNTERS)), dtype=np.uint64)
     elapsed = ctypes.c_double()
     trace_parent = trace_trails = trace_costs = None
     if trace:
@@ -509,11 +523,48 @@ def run_faco(xy_instances, instance_keys, seeds, settings, batches, mne_target=8
                              work.reshape(-1), ptr(trace_parent), ptr(trace_trails), ptr(trace_costs),
                              ctypes.byref(elapsed), int(bool(or_opt)),
                              ptr(None if multipliers is None else mult), ptr(g_offsets), ptr(g_neighbors), stride,
-                             int(timed_from), ptr(rejections), ptr(spec_rows)), L)
+                             int(timed_from), ptr(rejections), ptr(spec_rows), ESCAPE_MODES[escape],
+                             ptr(escape_counters)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
-           "constraint_rejections": rejections}
+           "constraint_rejections": rejections, "escape_counters": escape_counters}
     if trace:
         out.update(trace_parent=trace_parent, trace_trails=trace_trails, trace_costs=trace_costs)
+    return out
+
+
+def escape_ant(xy, settings, graph, parent, products, default_trail, parent_footprint, key, batch, ant, start,
+               target, rows=None, disabled=False):
+    """One Escape-mode ant on explicit colony state (parent tour, products
+    [n][pw], default trail, parent footprint [n]) over the CSR `graph`,
+    with its trace: "events" [moves][6] = (kind 0 relocation / 1 successor
+    2-opt / 2 predecessor 2-opt, a, b, slot, cache size after, permit),
+    "anchors", "ls_rows" [n][lw], "replaced", "footprint", "cache" [size][2],
+    "counters" (ESCAPE_COUNTERS order), "ls" = evaluations, accepted,
+    rejections, checklist size, "costs" = construction, final."""
+    xy = np.ascontiguousarray(xy, dtype=np.int32).reshape(-1)
+    n = xy.size // 2
+    norm = prepare(xy, settings)["settings"]
+    spec_rows = _spec_rows(None if rows is None else [rows], 1)
+    offsets, neighbors = (np.ascontiguousarray(a, dtype=np.uint32) for a in graph)
+    out = {"construction_tour": np.zeros(n, np.uint32), "tour": np.zeros(n, np.uint32),
+           "anchors": np.zeros(n, np.uint8), "ls_rows": np.zeros((n, norm.ls_width), np.uint32),
+           "replaced": np.zeros((n, norm.ls_width), np.uint8), "footprint": np.zeros(n, np.uint8)}
+    cache = np.zeros((64, 2), np.uint32)
+    events = np.zeros((2 * n, 6), np.uint32)
+    counters = np.zeros(len(ESCAPE_COUNTERS), np.uint64)
+    ls = np.zeros(4, np.uint64)
+    costs = np.zeros(2, np.int64)
+    size, count = ctypes.c_uint32(), ctypes.c_uint32()
+    L = lib()
+    _check(L.oracle_escape_ant(
+        n, xy, ctypes.byref(settings), spec_rows.ctypes.data_as(ctypes.c_void_p) if spec_rows is not None else None,
+        offsets, neighbors, np.ascontiguousarray(parent, dtype=np.uint32),
+        np.ascontiguousarray(products, dtype=np.float64).reshape(-1), float(default_trail),
+        np.ascontiguousarray(parent_footprint, dtype=np.uint8), key, batch, ant, start, target, int(bool(disabled)),
+        out["construction_tour"], out["tour"], out["anchors"], out["ls_rows"].reshape(-1), out["replaced"].reshape(-1),
+        out["footprint"], cache.reshape(-1), ctypes.byref(size), events.reshape(-1), ctypes.byref(count), counters,
+        ls, costs), L)
+    out.update(cache=cache[:size.value], events=events[:count.value], counters=counters, ls=ls, costs=costs)
     return out
 
 
@@ -955,7 +1006,7 @@ FACTORIAL_VARIANTS = {"M00": 0, "M10": 1, "M01": 2, "M11": 3}
 def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limit, mode, programs=None,
                    controllers=None, baseline=None, variant="M11", experiment_mask=0xFFFFFFFF, threads=0,
                    trace=False, force_collisions=False, or_opt=False, stop_batch=0, fork=None, graphs=None,
-                   multipliers=None, progress=None, rows=None):
+                   multipliers=None, progress=None, rows=None, escape=None):
     """Controlled FACO: colony c uses instance c % instances and decider c // instances.
 
     multipliers [instances, n, pw] (float32): learned heuristic multipliers,
@@ -963,7 +1014,8 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
     each batch (a time-budget run's elapsed share) instead of completed FE
     over evaluation_limit. stop_batch (0 = all) ends the run after that many batches, the progress
     feature still over evaluation_limit; fork = (batch, seed, action) forces
-    that batch's action and re-keys every colony from its ants on."""
+    that batch's action and re-keys every colony from its ants on. `escape`
+    as for run_faco."""
     xy_instances = np.ascontiguousarray(xy_instances, dtype=np.int32)
     instances, n = xy_instances.shape[0], xy_instances.shape[1]
     seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
@@ -973,6 +1025,7 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
     costs = np.zeros(colonies, dtype=np.int64)
     work = np.zeros((colonies, 2), dtype=np.uint64)
     rejections = np.zeros(colonies, dtype=np.uint64)
+    escape_counters = np.zeros((colonies, len(ESCAPE_COUNTERS)), dtype=np.uint64)
     states = (ControlOut * colonies)()
     elapsed = ctypes.c_double()
     norm = prepare(xy_instances[0], settings)["settings"]
@@ -1021,9 +1074,10 @@ def run_controlled(xy_instances, instance_keys, seeds, settings, evaluation_limi
         ptr(traces.get("masks")), ptr(traces.get("actions")), ptr(traces.get("alternatives")),
         ptr(traces.get("regions")), ptr(traces.get("costs")), ctypes.byref(elapsed), int(bool(or_opt)),
         stop_batch, fork[0] if fork else 0xFFFFFFFF, fork[1] if fork else 0, fork[2] if fork else -1,
-        ptr(g_off), ptr(g_nb), g_stride, ptr(mult), ptr(prog), ptr(rejections), ptr(spec_rows)), L)
+        ptr(g_off), ptr(g_nb), g_stride, ptr(mult), ptr(prog), ptr(rejections), ptr(spec_rows),
+        ESCAPE_MODES[escape], ptr(escape_counters)), L)
     out = {"tours": tours, "costs": costs, "work": work, "search_seconds": elapsed.value,
-           "constraint_rejections": rejections,
+           "constraint_rejections": rejections, "escape_counters": escape_counters,
            "states": [{"archive_size": s.archive_size, "stagnant_batches": s.stagnant_batches,
                        "epoch_batches": s.epoch_batches, "restarts": s.restarts, "return_rate": s.return_rate,
                        "ls_work": s.ls_work} for s in states]}
