@@ -340,6 +340,7 @@ SIGNATURES = {
                                                      ctypes.c_uint32, vp]),
     "faco_engine_set_preparation_charges": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
     "faco_engine_constraint_rejections": (ctypes.c_int, [vp, u64p, ctypes.c_uint32]),
+    "faco_engine_escape_counters": (ctypes.c_int, [vp, u64p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     "faco_engine_evaluate_control": (ctypes.c_int, [vp, u64p, u64p, ctypes.c_uint32, ctypes.c_uint64,
                                                     ctypes.POINTER(ControlSpec), ctypes.c_uint32,
                                                     ctypes.POINTER(ControlTrace), ctypes.POINTER(Result), u32p, i64p,

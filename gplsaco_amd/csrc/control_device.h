@@ -485,6 +485,13 @@ __device__ __forceinline__ void control_decide(const ControlParams& p, uint32_t 
         if (lds_links) lds_links[i] = src_links[i];
         if (lds_ppos) lds_ppos[src[i]] = static_cast<uint16_t>(i);
     }
+    if (p.archive_fp) {
+        // Escape mode: the restart carries the entry's footprint
+        // (reference cuda/control_state.cuh:339-340)
+        const uint8_t* fp = p.archive_fp + ((static_cast<size_t>(colony) * 2 + c.parity) * kArchive + alt) * ns;
+        for (uint32_t i = tid; i < n; i += threads)
+            p.parent_fp[static_cast<size_t>(colony) * ns + i] = p.epoch_fp[static_cast<size_t>(colony) * ns + i] = fp[i];
+    }
     const uint32_t cells = n * pw;
     double* tr = p.trails + static_cast<size_t>(colony) * ns * pw;
     double* pr = p.products + static_cast<size_t>(colony) * ns * pw;
@@ -715,6 +722,17 @@ __device__ __forceinline__ void control_update(const ControlParams& p, uint32_t 
         const uint32_t pred = src[i == 0 ? n - 1 : i - 1], succ = src[i + 1 == n ? 0 : i + 1];
         new_tours[static_cast<size_t>(slot) * ns + i] = v;
         new_links[static_cast<size_t>(slot) * ns + v] = pred | (succ << 16);
+    }
+    if (p.archive_fp) {
+        // Escape mode: each entry's footprint moves with it
+        // (reference cuda/control_state.cuh:177-189)
+        const uint8_t* old_fp = p.archive_fp + (static_cast<size_t>(colony) * 2 + parity) * kArchive * ns;
+        uint8_t* new_fp = p.archive_fp + (static_cast<size_t>(colony) * 2 + (parity ^ 1u)) * kArchive * ns;
+        for (uint32_t k = tid; k < count * n; k += threads) {
+            const uint32_t slot = k / n, i = k - slot * n, e = s_chosen[slot];
+            new_fp[static_cast<size_t>(slot) * ns + i] =
+                e < kArchive ? old_fp[static_cast<size_t>(e) * ns + i] : p.ant_fp[(ant_base + e - kArchive) * ns + i];
+        }
     }
     double* work_terms = reinterpret_cast<double*>(cost);   // the costs are no longer read
     for (uint32_t a = tid; a < A; a += threads) {

@@ -102,14 +102,14 @@ size_t max_lds_per_block() {
 // Workgroup shape for the ant kernel: the most ants per workgroup that both
 // divides the ants of a colony and fits the LDS budget; neighbour rows are
 // staged in LDS when at least one ant still fits beside them.
-LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_waves) {
+LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_waves, uint32_t extra_wave_bytes = 0) {
     LaunchShape shape;
     // the kernels' static LDS (ant slot counter, the persistent kernel's
     // colony state) comes on top of the dynamic share
     constexpr size_t kStaticLds = 256;
     const size_t budget = max_lds_per_block() - kStaticLds;
     shape.nbr_lds_bytes = round16_host(2u * n * K);
-    const uint32_t per_wave = ant_wave_bytes(n);
+    const uint32_t per_wave = ant_wave_bytes(n) + extra_wave_bytes;
     if (forced_waves) {
         // a forced shape: any wave count 1..16 (waves take ants dynamically,
         // so it need not divide the ants); bit 8 keeps neighbour rows global
@@ -164,6 +164,20 @@ struct Engine {
     bool hard = false;                     // constraint mode Hard: every tour stays in its instance's graph
     uint32_t graph_stride = 0;             // neighbour capacity per slot: 2 n (primary_width + 1)
     DeviceArray<uint32_t> graph_offsets, graph_neighbors;   // [G][n + 1], [G][graph_stride]
+    // Escape mode (on the Hard graph): escape gates, slot views, per-ant
+    // edge caches and anchor LS views; an evaluation may disable the gates
+    // (FACO_RUN_NO_ESCAPE), which must reproduce the Hard run
+    bool escape = false, escape_disabled = false;
+    uint32_t wave_extra = 0;               // per-wave LDS of the ant kernel beyond the tour arrays
+    uint64_t escape_bytes = 0;             // device memory of the Escape buffers
+    DeviceArray<uint16_t> esc_rows;        // [A][n][lw] anchors' LS views (city | replaced << 15)
+    DeviceArray<uint8_t> esc_fp;           // [A][n] footprints of the ants' tours
+    DeviceArray<uint8_t> parent_fp, epoch_fp, global_fp;   // [C][n]
+    DeviceArray<uint8_t> archive_fp;       // [C][2][4][n] (the archive ring's layout)
+    DeviceArray<uint32_t> esc_stats;       // [A][12]
+    DeviceArray<uint64_t> esc_totals;      // [C][12]
+    DeviceArray<double> eta_backup;        // [G][n][bw] d^-beta of every backup slot
+    std::vector<uint64_t> last_escape;     // [C][11] escape counters of the last count evaluation
     faco_settings base{};                  // settings as given (ants 0 = per-n default)
     DeviceArray<ColonyShape> shapes;       // [C]
     DeviceArray<uint32_t> order;           // [C]
@@ -239,7 +253,7 @@ struct Engine {
         device = current_device();
         checked(hipDeviceGetAttribute(&compute_units, hipDeviceAttributeMultiprocessorCount, device), "CU count");
         p_root = std::pow(config.p_best, 1.0 / config.primary_width);
-        shape = choose_shape(n, K, config.ants, 0);
+        shape = choose_shape(n, K, config.ants, 0, wave_extra);
         checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         checked(hipEventCreate(&ev_start), "hipEventCreate");
         checked(hipEventCreate(&ev_stop), "hipEventCreate");
@@ -258,6 +272,40 @@ struct Engine {
         if (ev_start) (void)hipEventDestroy(ev_start);
         if (ev_stop) (void)hipEventDestroy(ev_stop);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Escape-mode buffers (constraint mode 2, on top of the Hard graph).
+    void enable_escape() {
+        escape = true;
+        wave_extra = kEscapeWaveBytes;
+        shape = choose_shape(n, K, config.ants, 0, wave_extra);
+        const size_t C = colonies_capacity, A = C * config.ants, G = panel;
+        auto alloc = [&](auto& buffer, size_t count) { buffer.allocate(count); allocated += buffer.bytes(); };
+        alloc(esc_rows, A * n * config.ls_width); alloc(esc_fp, A * n);
+        alloc(parent_fp, C * n); alloc(epoch_fp, C * n); alloc(global_fp, C * n);
+        alloc(archive_fp, C * 2 * kArchive * n);
+        alloc(esc_stats, A * kEscStats); alloc(esc_totals, C * kEscStats);
+        alloc(eta_backup, G * n * std::max(config.backup_width, 1u));
+        escape_bytes = esc_rows.bytes() + esc_fp.bytes() + parent_fp.bytes() + epoch_fp.bytes() + global_fp.bytes() +
+                       archive_fp.bytes() + esc_stats.bytes() + esc_totals.bytes() + eta_backup.bytes();
+    }
+    // the escape counters of the last evaluation's `tasks` colonies
+    void read_escape(uint32_t tasks) {
+        last_escape.clear();
+        if (!escape) return;
+        std::vector<uint64_t> totals(static_cast<size_t>(tasks) * kEscStats);
+        esc_totals.download(totals.data(), totals.size(), stream);
+        checked(hipStreamSynchronize(stream), "escape counters");
+        for (uint32_t c = 0; c < tasks; ++c)
+            last_escape.insert(last_escape.end(), totals.begin() + static_cast<size_t>(c) * kEscStats,
+                               totals.begin() + static_cast<size_t>(c) * kEscStats + FACO_ESCAPE_COUNTERS);
+    }
+    // every footprint and counter starts at zero with each evaluation
+    void reset_escape() {
+        if (!escape) return;
+        for (auto* b : {&parent_fp, &epoch_fp, &global_fp, &archive_fp})
+            checked(hipMemsetAsync(b->data(), 0, b->bytes(), stream), "escape reset");
+        checked(hipMemsetAsync(esc_totals.data(), 0, esc_totals.bytes(), stream), "escape reset");
     }
 
     void ensure_control() {
@@ -313,7 +361,7 @@ struct Engine {
     }
 
     void set_waves(uint32_t waves, uint32_t blocks_per_colony) {
-        shape = choose_shape(n, K, config.ants, waves);
+        shape = choose_shape(n, K, config.ants, waves, wave_extra);
         forced_waves = waves;
         forced_blocks_per_colony = blocks_per_colony;
         drop_graph();
@@ -352,6 +400,8 @@ struct Engine {
             geo.nbr.upload(p.nbr.data(), nbr_stride, stream, static_cast<size_t>(g) * nbr_stride);
             geo.nbr_dist.upload(p.nbr_dist.data(), nbr_stride, stream, static_cast<size_t>(g) * nbr_stride);
             if (bw) geo.backup.upload(p.backup16.data(), static_cast<size_t>(m) * bw, stream, static_cast<size_t>(g) * n * bw);
+            if (escape && bw)
+                eta_backup.upload(p.eta_backup.data(), static_cast<size_t>(m) * bw, stream, static_cast<size_t>(g) * n * bw);
             if (p.prior.empty()) {
                 geo.eta.upload(p.eta.data(), static_cast<size_t>(m) * pw, stream, static_cast<size_t>(g) * n * pw);
             } else {
@@ -387,7 +437,7 @@ struct Engine {
         p.n = n; p.ants = config.ants; p.pw = config.primary_width; p.bw = config.backup_width;
         p.lw = config.ls_width; p.K = K; p.ls_offset = ls_offset; p.waves = shape.waves;
         p.nbr_lds_bytes = shape.nbr_lds_bytes; p.nbr_stride = nbr_stride;
-        p.per_wave_bytes = ant_wave_bytes(n);
+        p.per_wave_bytes = ant_wave_bytes(n) + wave_extra;
         p.xcd_remap = shapes_active ? 0u : xcd_remap;   // mixed: keep the largest-first order
         // Split each colony over just enough workgroups to fill every CU:
         // fewer, larger chunks let the waves of a workgroup balance uneven
@@ -413,6 +463,11 @@ struct Engine {
             p.graph_offsets = graph_offsets.data(); p.graph_neighbors = graph_neighbors.data();
             p.graph_stride = graph_stride;
         }
+        if (escape) {
+            p.escape_disabled = escape_disabled ? 1u : 0u;
+            p.esc_rows = esc_rows.data(); p.esc_footprint = esc_fp.data(); p.esc_stats = esc_stats.data();
+            p.parent_fp = parent_fp.data(); p.eta_backup = eta_backup.data();
+        }
         (void)colonies;
         return p;
     }
@@ -426,6 +481,11 @@ struct Engine {
         p.links = links.data(); p.epoch = epoch.data(); p.global = global.data(); p.trails = trails.data();
         p.products = products.data();
         p.shapes = shapes_active ? shapes.data() : nullptr;
+        if (escape) {
+            p.ant_fp = esc_fp.data(); p.esc_stats = esc_stats.data();
+            p.parent_fp = parent_fp.data(); p.epoch_fp = epoch_fp.data(); p.global_fp = global_fp.data();
+            p.esc_totals = esc_totals.data();
+        }
         return p;
     }
     void reset(uint32_t colonies, uint32_t mne) {
@@ -437,6 +497,7 @@ struct Engine {
         r.epoch = epoch.data(); r.global = global.data(); r.trails = trails.data(); r.products = products.data();
         r.shapes = shapes_active ? shapes.data() : nullptr;
         checked(launch_colony_reset(r, colonies, stream), "colony reset launch");
+        reset_escape();
     }
     void iteration(uint32_t colonies, const AntParams& ap, const ColonyParams& cp) {
         const uint32_t blocks = colonies * ap.blocks_per_colony;
@@ -646,9 +707,12 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
               uint32_t mne, uint32_t flags, faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
               uint64_t limit, bool any_count = false, const faco_checkpoints* checkpoints = nullptr) {
     e.last_rejections.clear();
+    e.last_escape.clear();
     const uint64_t launches_before = e.persistent_launch_count;
     const bool use_graph = (flags & FACO_RUN_GRAPH) != 0, profile = (flags & FACO_RUN_PROFILE) != 0;
     const bool events = (flags & FACO_RUN_EVENTS) != 0, resume = (flags & FACO_RUN_CONTINUE) != 0;
+    require(!(flags & FACO_RUN_NO_ESCAPE) || e.escape, "only an Escape-mode engine can disable its escape");
+    require(!resume || !e.escape, "an Escape-mode run is not continued");
     double kernel_ms[FACO_STAGE_COUNT] = {0};
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
@@ -658,6 +722,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
             "task count must equal the engine's colonies");
     require(mne > 0, "mne_target must be positive");
     for (uint32_t c = 0; c < tasks; ++c) (void)e.problem(instance_keys[c]);
+    e.escape_disabled = (flags & FACO_RUN_NO_ESCAPE) != 0;
     e.upload_geometry(instance_keys, tasks);
     std::vector<uint64_t> keys(tasks);
     for (uint32_t c = 0; c < tasks; ++c) keys[c] = colony_key(instance_keys[c], seeds[c]);
@@ -711,6 +776,7 @@ void evaluate(Engine& e, const uint64_t* instance_keys, const uint64_t* seeds, u
         e.last_rejections[c] = states[c].work_constraints - (resume ? before[c].work_constraints : 0);
         rejections += e.last_rejections[c];
     }
+    e.read_escape(tasks);
     fill_result(result, e, tasks, limit, batches, e.mixed ? limit : batches * e.config.ants, steps, evals,
                 seconds_since(started), search, gpu_ms);
     if (result) {
@@ -866,6 +932,10 @@ ControlParams control_params(Engine& e, uint32_t panel, const faco_control_spec&
     cp.features = b.features.data(); cp.scores = b.scores.data(); cp.masks = b.masks.data();
     cp.actions = b.actions.data(); cp.alternatives = b.alternatives.data(); cp.regions = b.regions.data();
     cp.shapes = e.shapes_active ? e.shapes.data() : nullptr;
+    if (e.escape) {
+        cp.archive_fp = e.archive_fp.data(); cp.ant_fp = e.esc_fp.data();
+        cp.parent_fp = e.parent_fp.data(); cp.epoch_fp = e.epoch_fp.data();
+    }
     return cp;
 }
 
@@ -894,6 +964,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
                       faco_result* result, uint32_t* tours, int64_t* costs, uint64_t* work,
                       faco_control_state* states, const faco_checkpoints* checkpoints, const StateIO* io = nullptr) {
     e.last_rejections.clear();
+    e.last_escape.clear();
     const uint64_t launches_before = e.persistent_launch_count;
     const auto started = Clock::now();
     std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
@@ -903,6 +974,8 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     require(spec.mode >= FACO_CONTROL_PROGRAM && spec.mode <= FACO_CONTROL_FACTORIAL, "unknown control mode");
     require(!e.hard || !e.local_search, "a Hard-mode engine uses the plain 2-opt");
     require((spec.experiment_mask & 0xffffu) != 0, "the experiment mask must keep at least one hold action");
+    require(!(flags & FACO_RUN_NO_ESCAPE) || e.escape, "only an Escape-mode engine can disable its escape");
+    e.escape_disabled = (flags & FACO_RUN_NO_ESCAPE) != 0;
     const uint32_t ants = e.config.ants;   // the ant stride in a mixed-size batch
     if (!e.mixed)
         require(limit % ants == 0 && limit / ants <= UINT32_MAX, "evaluation limit must be a whole number of ant batches");
@@ -1150,6 +1223,7 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
     marks.read_back(e.stream, e.n, [&](uint32_t c) { return e.problem(inst[c]).n; });
     uint64_t steps = 0, evals = 0, rejections = 0;
     e.last_rejections.assign(C, 0);
+    e.read_escape(C);
     for (uint32_t c = 0; c < C; ++c) {
         e.last_rejections[c] = st[c].work_constraints - (resuming ? resumed[c].work_constraints : 0);
         rejections += e.last_rejections[c];
@@ -1754,13 +1828,20 @@ int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_setting
                        uint32_t constraint_mode, uint32_t population_size, faco_engine** out) {
     return guarded([&] {
         require(out && settings, "null argument");
-        require(constraint_mode <= 1, "constraint mode must be 0 (unrestricted) or 1 (hard); escape is not available");
-        require(constraint_mode == 0 || population_size == 1, "a Hard-mode engine holds one population member");
+        require(constraint_mode <= 2, "constraint mode must be 0 (unrestricted), 1 (hard) or 2 (escape)");
+        require(constraint_mode == 0 || population_size == 1, "a Hard- or Escape-mode engine holds one population member");
         auto engine = std::make_unique<faco_engine>();
         engine->impl = std::make_unique<Engine>(dimension, colonies, *settings, population_size, false,
-                                                constraint_mode == 1);
+                                                constraint_mode != 0);
         Engine& e = *engine->impl;
-        if (constraint_mode == 1) {
+        if (constraint_mode == 2) {
+            // the reference's frozen Escape shape (batch_engine.cu:84-86)
+            require(e.config.primary_width <= 16 && e.config.backup_width <= 64 && e.config.ls_width <= 20,
+                    "Escape mode takes at most 16 primary, 64 backup and 20 LS slots");
+            require(e.n < 32768, "Escape mode takes fewer than 32768 cities");
+            e.enable_escape();
+        }
+        if (constraint_mode != 0) {
             e.hard = true;
             e.graph_stride = 2u * e.n * (e.config.primary_width + 1u);
             e.graph_offsets.allocate(static_cast<size_t>(e.panel) * (e.n + 1));
@@ -1814,7 +1895,7 @@ int faco_engine_autotune(faco_engine* engine, const uint64_t* instance_keys, con
             for (uint32_t lds : {1u, 0u}) {
                 const uint32_t forced = w | (lds ? 0u : 0x100u);
                 try {
-                    (void)choose_shape(e.n, e.K, e.config.ants, forced);
+                    (void)choose_shape(e.n, e.K, e.config.ants, forced, e.wave_extra);
                 } catch (const std::invalid_argument&) {
                     continue;   // does not fit the LDS budget
                 }
@@ -1867,7 +1948,7 @@ int faco_engine_autotune_control(faco_engine* engine, const uint64_t* panel_keys
             for (uint32_t lds : {1u, 0u}) {
                 const uint32_t forced = w | (lds ? 0u : 0x100u);
                 try {
-                    (void)choose_shape(e.n, e.K, e.config.ants, forced);
+                    (void)choose_shape(e.n, e.K, e.config.ants, forced, e.wave_extra);
                 } catch (const std::invalid_argument&) {
                     continue;   // does not fit the LDS budget
                 }
@@ -1937,7 +2018,7 @@ int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint3
         if (waves) *waves = e.shape.waves;
         if (nbr_in_lds) *nbr_in_lds = e.shape.nbr_in_lds;
         if (lds_bytes) *lds_bytes = ant_shared_bytes(e.n, e.shape.nbr_in_lds ? e.shape.nbr_lds_bytes : 0u) +
-                                    e.shape.waves * ant_wave_bytes(e.n);
+                                    e.shape.waves * (ant_wave_bytes(e.n) + e.wave_extra);
     });
 }
 
@@ -2073,6 +2154,18 @@ int faco_engine_register_graph_problem(faco_engine* engine, uint64_t key, const 
                     b >= n ? 0.0 : db > 0 ? (e.config.beta == 1.0 ? 1.0 / db : 1.0 / std::pow(db, e.config.beta)) : 1.0;
             }
         }
+        if (e.escape) {
+            // the weight of a backup city in a replaced slot is
+            // default_trail * d^-beta (reference escape_device.cuh:74-77)
+            p.eta_backup.assign(static_cast<size_t>(n) * bw, 0.0);
+            for (uint32_t a = 0; a < n; ++a)
+                for (uint32_t j = 0; j < bw; ++j) {
+                    const uint32_t b = p.backup[static_cast<size_t>(a) * bw + j];
+                    const double db = b < n ? static_cast<double>(d(a, b)) : 0.0;
+                    p.eta_backup[static_cast<size_t>(a) * bw + j] =
+                        b >= n ? 0.0 : db > 0 ? (e.config.beta == 1.0 ? 1.0 / db : 1.0 / std::pow(db, e.config.beta)) : 1.0;
+                }
+        }
         p.preparation_seconds += seconds_since(started);
         const auto existing = e.registry.find(key);
         if (existing != e.registry.end()) {
@@ -2133,6 +2226,7 @@ int faco_engine_set_edge_prior(faco_engine* engine, uint64_t key, const float* m
         auto it = e.registry.find(key);
         require(it != e.registry.end(), "task instance has not been registered");
         const size_t cells = static_cast<size_t>(it->second.n) * e.config.primary_width;
+        require(!multipliers || !e.escape, "an Escape-mode engine takes no learned prior");
         if (multipliers) {
             for (size_t i = 0; i < cells; ++i)
                 require(std::isfinite(multipliers[i]) && multipliers[i] > 0.0f, "prior multipliers must be positive");
@@ -2229,6 +2323,20 @@ int faco_engine_constraint_rejections(faco_engine* engine, uint64_t* out, uint32
         require(lock.owns_lock(), "rejections can not be read during an evaluation");
         require(count == e.last_rejections.size(), "one entry per task of the last count evaluation");
         std::copy(e.last_rejections.begin(), e.last_rejections.end(), out);
+    });
+}
+
+int faco_engine_escape_counters(faco_engine* engine, uint64_t* out, uint32_t count, uint64_t* reserved_bytes) {
+    return guarded([&] {
+        require(engine && out, "null argument");
+        Engine& e = *engine->impl;
+        std::unique_lock<std::mutex> lock(e.mutex, std::try_to_lock);
+        require(lock.owns_lock(), "escape counters can not be read during an evaluation");
+        require(e.escape, "only an Escape-mode engine counts escapes");
+        require(static_cast<size_t>(count) * FACO_ESCAPE_COUNTERS == e.last_escape.size(),
+                "one entry per task of the last count evaluation");
+        std::copy(e.last_escape.begin(), e.last_escape.end(), out);
+        if (reserved_bytes) *reserved_bytes = e.escape_bytes;
     });
 }
 

@@ -6,14 +6,16 @@
 // cpp/include/gp_faco/escape_slots.hpp:18-71, escape_edges.hpp:13-122,
 // edge_constraints.hpp:15-33.
 //
-// Design: one wave per row or transaction. Backup eligibility (first
-// occurrence, before the first padding entry, not the row's own city, not
-// already in the row) is one ballot per test and the eligible list is
-// compacted with mbcnt; the LS row order is a rank sort over (distance, id)
-// (the keys are distinct, so it equals the reference's insertion sort); a
-// cache lookup checks the 64 cache slots in one lane each. The draws that
-// pick slots and cities stay sequential, as the stream order defines them.
-#include "faco_common.h"
+// Design: one wave per row or transaction. The stream, the slot view and
+// the LS order are the ant kernel's own (escape_device.h): backup
+// eligibility (first occurrence, before the first padding entry, not the
+// row's own city, not already in the row) is one ballot per test and the
+// eligible list is compacted with mbcnt; the LS row order is a rank sort
+// over (distance, id) (the keys are distinct, so it equals the reference's
+// insertion sort); a cache lookup checks the 64 cache slots in one lane
+// each. The draws that pick slots and cities stay sequential, as the stream
+// order defines them.
+#include "escape_device.h"
 
 #include <algorithm>
 #include <stdexcept>
@@ -30,96 +32,6 @@ constexpr uint32_t kEscAllowed = 0, kEscInvalid = 1, kEscOutside = 2, kEscCapaci
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 __device__ __forceinline__ uint64_t wave_ballot(bool v) { return __ballot(v); }
-__device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
-    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
-}
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-}
-
-// Escape word stream of (key, batch, domain | ant, event): counter block
-// (block, event, domain | ant, batch), four words per block. Wave-uniform.
-struct EscapeStream {
-    uint32_t k0, k1, ant, batch, event, block, index, w[4];
-    __device__ EscapeStream(uint64_t key, uint32_t b, uint32_t a, uint32_t e, bool ls)
-        : k0(static_cast<uint32_t>(key)), k1(static_cast<uint32_t>(key >> 32)),
-          ant((ls ? 0x80000000u : 0x40000000u) | a), batch(b), event(e), block(0), index(4), w{0, 0, 0, 0} {}
-    __device__ uint32_t draw() {
-        if (index == 4) {
-            w[0] = block++; w[1] = event; w[2] = ant; w[3] = batch;
-            philox4x32_10(w, k0, k1);
-            index = 0;
-        }
-        return w[index++];
-    }
-    __device__ uint32_t below(uint32_t count) {
-        const uint32_t threshold = (0u - count) % count;
-        uint32_t v;
-        do { v = draw(); } while (v < threshold);
-        return v % count;
-    }
-};
-
-// Slot view of one row by one wave; members/replaced live in the caller's
-// LDS (width entries each). Returns valid | replacements << 16.
-__device__ uint32_t wave_slot_view(const uint32_t* original, uint32_t width, const uint32_t* backup, uint32_t bw,
-                                   uint32_t n, uint32_t from, EscapeStream& rng, uint32_t* members, uint8_t* replaced,
-                                   uint32_t* eligible, uint32_t* slots) {
-    const uint32_t lane = lane_id();
-    const uint32_t m = lane < width ? original[lane] : n;
-    if (lane < width) { members[lane] = m; replaced[lane] = 0; }
-    const uint32_t valid = static_cast<uint32_t>(__builtin_popcountll(wave_ballot(lane < width && m < n)));
-    uint32_t wanted = valid / 4;
-    if (valid % 4) wanted += (rng.draw() & 3u) < valid % 4 ? 1u : 0u;
-    if (!wanted) { __builtin_amdgcn_wave_barrier(); return valid; }
-    // eligible backups: before the first padding entry, not the row's city,
-    // not in the row's valid prefix, first occurrence
-    const uint32_t b = lane < bw ? backup[lane] : n;
-    const uint64_t pad = wave_ballot(lane < bw && b >= n) | (bw < 64 ? (~0ull << bw) : 0ull);
-    const uint32_t first_pad = pad ? static_cast<uint32_t>(__builtin_ctzll(pad)) : 64u;
-    bool ok = lane < first_pad && b != from;
-    for (uint32_t k = 0; k < valid; ++k) ok = ok && bcast(m, k) != b;
-    for (uint32_t k = 0; k < first_pad; ++k) ok = ok && !(k < lane && bcast(b, k) == b);
-    const uint64_t emask = wave_ballot(ok);
-    const uint32_t count = static_cast<uint32_t>(__builtin_popcountll(emask));
-    if (ok) eligible[lanes_below(emask)] = b;
-    if (lane < valid) slots[lane] = lane;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (wanted > count) wanted = count;
-    // the draws that pick slots and cities, in stream order (uniform)
-    for (uint32_t j = 0; j < wanted; ++j) {
-        const uint32_t pick_slot = j + rng.below(valid - j);
-        const uint32_t pick_node = j + rng.below(count - j);
-        if (lane == 0) {
-            const uint32_t slot = slots[pick_slot];
-            slots[pick_slot] = slots[j]; slots[j] = slot;
-            const uint32_t node = eligible[pick_node];
-            eligible[pick_node] = eligible[j]; eligible[j] = node;
-            members[slot] = node; replaced[slot] = 1;
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
-    return valid | (wanted << 16);
-}
-
-// LS form of a view: the valid prefix ordered by (distance to `from`, id).
-__device__ void wave_sort_row(uint32_t* members, uint8_t* replaced, uint32_t valid, uint32_t from, const uint32_t* coords) {
-    const uint32_t lane = lane_id();
-    const uint32_t m = lane < valid ? members[lane] : 0u;
-    const uint8_t f = lane < valid ? replaced[lane] : 0;
-    const uint64_t key = lane < valid ? (static_cast<uint64_t>(euc2d(coords[from], coords[m])) << 32) | m : ~0ull;
-    uint32_t rank = 0;
-    for (uint32_t k = 0; k < valid; ++k) {
-        const uint64_t other = (static_cast<uint64_t>(bcast(static_cast<uint32_t>(key >> 32), k)) << 32) | bcast(static_cast<uint32_t>(key), k);
-        rank += other < key ? 1u : 0u;
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (lane < valid) { members[rank] = m; replaced[rank] = f; }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
 
 struct Edge { uint32_t a, b; };
 __device__ __forceinline__ bool same_edge(Edge x, Edge y) { return (x.a == y.a && x.b == y.b) || (x.a == y.b && x.b == y.a); }
@@ -199,20 +111,20 @@ struct RowParams {
 __global__ void __launch_bounds__(64) faco_escape_rows_kernel(const RowParams p) {
     const uint32_t row = blockIdx.x;
     if (row >= p.rows) return;
-    __shared__ uint32_t members[kEscRow], eligible[kEscBackup], slots[kEscRow];
-    __shared__ uint8_t replaced[kEscRow];
-    EscapeStream rng(p.key, p.batch, row, p.events[row], p.ls != 0);
-    const uint32_t packed = wave_slot_view(p.original + static_cast<size_t>(row) * p.width, p.width,
-                                           p.backup + static_cast<size_t>(row) * p.bw, p.bw, p.n, p.from[row], rng,
-                                           members, replaced, eligible, slots);
-    const uint32_t valid = packed & 0xffffu;
-    if (p.coords) wave_sort_row(members, replaced, valid, p.from[row], p.coords);
+    __shared__ uint8_t scratch[128];
+    __shared__ uint32_t order[32];
     const uint32_t lane = lane_id();
+    EscapeStream rng(p.key, p.batch, row, p.events[row], p.ls != 0);
+    const uint32_t orig = lane < p.width ? p.original[static_cast<size_t>(row) * p.width + lane] : p.n;
+    const uint32_t b = lane < p.bw ? p.backup[static_cast<size_t>(row) * p.bw + lane] : p.n;
+    const LaneView v = lane_slot_view(orig, p.width, b, p.bw, p.n, p.from[row], rng, scratch);
+    uint32_t member = v.member, source = v.source;
+    if (p.coords) lane_sort_row(member, source, v.valid, p.from[row], p.coords, order);
     if (lane < p.width) {
-        p.members[static_cast<size_t>(row) * p.width + lane] = members[lane];
-        p.replaced[static_cast<size_t>(row) * p.width + lane] = replaced[lane];
+        p.members[static_cast<size_t>(row) * p.width + lane] = member;
+        p.replaced[static_cast<size_t>(row) * p.width + lane] = source != kNoSource ? 1 : 0;
     }
-    if (lane == 0) { p.counts[2 * row] = valid; p.counts[2 * row + 1] = packed >> 16; }
+    if (lane == 0) { p.counts[2 * row] = v.valid; p.counts[2 * row + 1] = v.replacements; }
 }
 
 struct TxParams {

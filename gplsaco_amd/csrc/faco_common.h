@@ -276,7 +276,23 @@ struct AntParams {
     const uint32_t* graph_offsets;
     const uint32_t* graph_neighbors;
     uint32_t graph_stride;
+    // Escape mode (on a Hard graph; null esc_rows: off): per ant its LS
+    // rows [A][n][lw] (an anchor's escape view, city | replaced << 15; other
+    // rows are not written), footprint [A][n] and counters [A][12] (the
+    // reference's EscapeStats order); the parent footprint of every colony
+    // [C][n]; d^-beta of every backup slot [G][n][bw]
+    uint32_t escape_disabled;           // gates never open (the run equals the Hard run)
+    uint16_t* esc_rows;
+    uint8_t* esc_footprint;
+    uint32_t* esc_stats;
+    const uint8_t* parent_fp;
+    const double* eta_backup;
 };
+// Escape counters per ant, in EscapeStats order (escape_slots.hpp:70-75).
+constexpr uint32_t kEscStats = 12;
+// Per-wave LDS the Escape ant adds: its edge cache (64 packed edges), the
+// slot-view scratch (128 B) and the LS order scratch (128 B).
+constexpr uint32_t kEscapeWaveBytes = 256u + 128u + 128u;
 
 struct ColonyParams {
     uint32_t n, ants, pw, K, nbr_stride;
@@ -296,6 +312,15 @@ struct ColonyParams {
     double* trails;                     // [C][n][pw]
     double* products;                   // [C][n][pw]
     const ColonyShape* shapes;          // [C] or null (fixed size)
+    // Escape mode (null: off): the ants' footprints [A][n] and counters
+    // [A][12], the colonies' parent / epoch / global footprints [C][n] and
+    // counter totals [C][12]
+    const uint8_t* ant_fp;
+    const uint32_t* esc_stats;
+    uint8_t* parent_fp;
+    uint8_t* epoch_fp;
+    uint8_t* global_fp;
+    uint64_t* esc_totals;
 };
 
 struct ResetParams {
@@ -359,6 +384,14 @@ struct ControlParams {
     uint32_t* alternatives;             // [C] or null
     uint32_t* regions;                  // [C][2][4][16] or null
     const ColonyShape* shapes;          // [C] or null (fixed size); n and ants above are then strides
+    // Escape mode (null archive_fp: off): footprints of the archive
+    // [C][2][4][n] (the tours' ring), of the ants [A][n], and the colonies'
+    // parent and epoch footprints [C][n], which a restart takes from the
+    // archive entry
+    uint8_t* archive_fp;
+    const uint8_t* ant_fp;
+    uint8_t* parent_fp;
+    uint8_t* epoch_fp;
 };
 // Device preparation (prepare_kernels.hip): instances g = 0..count-1 of
 // sizes[g] cities at stride `stride` in packed coordinates [count][stride].

@@ -20,6 +20,7 @@
 //   [n][primary] entries (each entry owned by one thread, no atomics).
 //   Semantics: reference cuda/colony_state.cuh:148-229.
 #include "control_device.h"
+#include "escape_device.h"
 #include "faco_common.h"
 
 namespace faco {
@@ -71,7 +72,7 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
     return v;
 }
 
-constexpr uint8_t kVisited = 1, kQueued = 2;
+constexpr uint8_t kVisited = 1, kQueued = 2, kAnchor = 4;   // kAnchor: Escape-mode anchor city
 
 
 __device__ __forceinline__ uint32_t round16(uint32_t v) { return (v + 15u) & ~15u; }
@@ -141,6 +142,9 @@ __device__ __forceinline__ double row_scan4(double v) {
 // `s` is the workgroup's copy of the colony state and `s_links` [n] receives
 // the new parent links (LDS); `s_ppos` [n] (LDS, may be null) the parent
 // position of every city.
+// Footprints: the Escape-mode footprint and counter updates are compiled in
+// (the per-batch update kernel; the persistent kernels never run Escape).
+template<bool Footprints = false>
 __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t colony, uint32_t tid, uint32_t threads,
                                               uint64_t* red, ColonyState& s, uint32_t* s_links, uint16_t* s_ppos) {
     const uint32_t ns = p.n;
@@ -218,6 +222,25 @@ __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t co
         if (s_ppos) s_ppos[v] = static_cast<uint16_t>(i);
         if (v == 0) s.parent_zero = i | (succ < pred ? 0x80000000u : 0u);
     }
+    if (Footprints && p.ant_fp) {
+        // Escape mode: footprints follow their tours (reference
+        // cuda/colony_state.cuh:201-206); the ants' counters add up per colony
+        const uint8_t* best_fp = p.ant_fp + (static_cast<size_t>(colony) * p.ants + s.iteration_best) * ns;
+        uint8_t* efp = p.epoch_fp + static_cast<size_t>(colony) * ns;
+        uint8_t* gfp = p.global_fp + static_cast<size_t>(colony) * ns;
+        uint8_t* pfp = p.parent_fp + static_cast<size_t>(colony) * ns;
+        for (uint32_t i = tid; i < n; i += threads) {
+            const uint8_t v = best_fp[i];
+            if (improve & 1u) efp[i] = v;
+            if (improve & 2u) gfp[i] = v;
+            pfp[i] = s.source_is_epoch ? efp[i] : v;
+        }
+        if (tid < kEscStats) {
+            uint64_t sum = 0;
+            for (uint32_t a = 0; a < ants; ++a) sum += p.esc_stats[(static_cast<size_t>(colony) * p.ants + a) * kEscStats + tid];
+            p.esc_totals[static_cast<size_t>(colony) * kEscStats + tid] += sum;
+        }
+    }
     __syncthreads();
     // Evaporate, clamp below, deposit on parent edges, clamp above, refresh
     // products (reference cuda/colony_state.cuh:210-229).
@@ -260,7 +283,7 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
     __shared__ uint64_t red[4 * 4];
     __shared__ ColonyState s;
     extern __shared__ uint32_t s_links[];
-    update_colony(p, colony, tid, blockDim.x, red, s, s_links, nullptr);
+    update_colony<true>(p, colony, tid, blockDim.x, red, s, s_links, nullptr);
 }
 
 // ant_search: the body of the ant kernel. Persistent: the workgroup runs
@@ -271,10 +294,12 @@ __global__ void __launch_bounds__(256) faco_colony_update(const ColonyParams p) 
 // decision and followed by the archive / feedback update of `ctl`, both in
 // the same workgroup, on the LDS the ants leave free between batches.
 template<bool NbrLds, bool Profile, bool Diag, bool OrOpt, bool Persistent, bool Hard = false,
-         bool Controlled = false>
+         bool Controlled = false, bool Escape = false>
 __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParams& cp, uint32_t rounds,
                                            ColonyState* s_state, const ControlParams* ctl = nullptr) {
     static_assert(!Controlled || Persistent, "the fused control runs in the persistent kernel");
+    static_assert(!Escape || (Hard && !Persistent && !Diag && !Profile && !OrOpt),
+                  "Escape mode runs per batch on a Hard graph with the plain 2-opt");
     // Diagnostic features (explicit visits, per-ant parents, traces and
     // logs) exist only in the Diag instantiation: the production kernel
     // keeps none of their pointers or flags live.
@@ -362,6 +387,12 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     uint16_t* pos = reinterpret_cast<uint16_t*>(mine + round16(2u * n));
     uint16_t* queue = reinterpret_cast<uint16_t*>(mine + 2u * round16(2u * n));
     uint8_t* flags = reinterpret_cast<uint8_t*>(mine + 3u * round16(2u * n));
+    // Escape mode: this ant's cache of out-of-graph edges (min | max << 16)
+    // and the scratch of its slot views and LS orders, after its own arrays
+    unsigned char* const esc_area = mine + 3u * round16(2u * ns) + round16(ns);
+    uint32_t* const esc_cache = reinterpret_cast<uint32_t*>(esc_area);
+    uint8_t* const esc_scratch = esc_area + 4u * kEscapeCapacity;
+    uint32_t* const esc_order = reinterpret_cast<uint32_t*>(esc_area + 4u * kEscapeCapacity + 128u);
 
     const uint32_t pw = p.pw, K = p.K, lw = p.lw;
     const uint64_t key = explicit_mode ? 0ull : p.keys[colony];
@@ -419,6 +450,9 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         __syncthreads();
     }
     const uint32_t batch = explicit_mode ? 0u : (Persistent ? s_state->batch : p.state[colony].batch);
+    // Escape mode: the colony's default trail and the backup rows' d^-beta
+    const double dtrail = Escape ? p.state[colony].default_trail : 0.0;
+    const double* const eta_backup = Escape ? p.eta_backup + static_cast<size_t>(geom) * ns * p.bw : nullptr;
     for (;;) {
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(&next_slot, 1u);
@@ -560,6 +594,82 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             return edges_allowed(3, removed, added);
         };
 
+        // Escape mode (reference escape_edges.hpp:13-122): the ant's cache of
+        // out-of-graph edges holds esc_size packed edges (wave-uniform); an
+        // edge is known when it is a graph or cache edge. A group of added
+        // edges passes when each one cancels a removed edge (as a multiset)
+        // or is known, or, with permission, is new; the new ones must fit in
+        // the cache. Status 0 allowed, 1 invalid edge, 2 outside the graph,
+        // 3 cache full; `novel` receives the new edges.
+        uint32_t esc_size = 0;
+        uint32_t e_gates = 0, e_replaced = 0, e_relocations = 0, e_anchor_nodes = 0, e_ls_replaced = 0;
+        uint32_t e_old_views = 0, e_anchor_pushes = 0, e_new = 0, e_construction_full = 0, e_ls_full = 0;
+        auto esc_known = [&](uint32_t x, uint32_t y) -> bool {
+            if (in_graph(x, y)) return true;
+            const uint32_t packed = x < y ? (x | (y << 16)) : (y | (x << 16));
+            bool hit = false;
+            for (uint32_t k = 0; k < esc_size; ++k) hit |= esc_cache[k] == packed;
+            return hit;
+        };
+        auto esc_status = [&](uint32_t count, const uint32_t (&removed)[3][2], const uint32_t (&added)[3][2],
+                              bool permit, uint32_t (&novel)[3], uint32_t& novel_count) -> uint32_t {
+            bool consumed[3] = {false, false, false};
+            novel_count = 0;
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t x = added[i][0], y = added[i][1];
+                if (x >= n || y >= n || x == y) return 1u;
+                bool unchanged = false;
+                for (uint32_t j = 0; j < count; ++j) {
+                    const bool same = (x == removed[j][0] && y == removed[j][1]) || (x == removed[j][1] && y == removed[j][0]);
+                    if (!unchanged && !consumed[j] && same) { consumed[j] = true; unchanged = true; }
+                }
+                if (unchanged || esc_known(x, y)) continue;
+                if (!permit) return 2u;
+                const uint32_t packed = x < y ? (x | (y << 16)) : (y | (x << 16));
+                bool duplicate = false;
+                for (uint32_t j = 0; j < novel_count; ++j) duplicate |= novel[j] == packed;
+                if (!duplicate) novel[novel_count++] = packed;
+            }
+            return novel_count > kEscapeCapacity - esc_size ? 3u : 0u;
+        };
+        // moving c to follow `cur` (whose successor is cur_after)
+        auto esc_relocation = [&](uint32_t c, uint32_t cur, uint32_t cur_after, bool permit, uint32_t (&novel)[3],
+                                  uint32_t& novel_count) -> uint32_t {
+            novel_count = 0;
+            if (c >= n || c == cur) return 1u;
+            if (cur_after == c) return 0u;
+            uint32_t before, after;
+            if (linked) {
+                const uint32_t l = lnk[c];
+                before = l & 0xffffu; after = l >> 16;
+            } else {
+                const uint32_t pc_c = pos[c];
+                before = tour[pc_c == 0 ? n - 1 : pc_c - 1];
+                after = tour[pc_c + 1 == n ? 0 : pc_c + 1];
+            }
+            const uint32_t removed[3][2] = {{before, c}, {c, after}, {cur, cur_after}};
+            const uint32_t added[3][2] = {{before, after}, {cur, c}, {c, cur_after}};
+            return esc_status(3, removed, added, permit, novel, novel_count);
+        };
+        // a move's new edges join the cache (the values are wave-uniform;
+        // a chosen move always fits, its availability checked the room)
+        auto esc_commit = [&](const uint32_t (&novel)[3], uint32_t novel_count) {
+            if (novel_count > kEscapeCapacity - esc_size) novel_count = 0;
+            if (lane == 0) for (uint32_t k = 0; k < novel_count; ++k) esc_cache[esc_size + k] = novel[k];
+            esc_size += novel_count;
+            e_new += novel_count;
+            wave_sync();
+        };
+        // availability without new edges (backup and nearest-city fallbacks)
+        auto reloc_ok = [&](uint32_t c, uint32_t cur, uint32_t cur_after) -> bool {
+            if constexpr (Escape) {
+                uint32_t novel[3], count;
+                return esc_relocation(c, cur, cur_after, false, novel, count) == 0u;
+            } else {
+                return relocation_ok(c, cur, cur_after);
+            }
+        };
+
         // ------------------------------------------------------------ construction
         // The product rows of both parent neighbours of the current city are
         // prefetched at the start of a step: the roulette mostly follows a
@@ -599,6 +709,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             uint32_t selected;
             double u = 0.0;
             uint32_t link;
+            bool novel = false;   // Escape mode: the move may open new edges
             if (explicit_mode) {
                 selected = uniform(visits[step]);
                 link = links[current];
@@ -620,10 +731,41 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     u_pool = step_uniform((static_cast<uint64_t>(k_hi) << 32) | k_lo, batch, local_ant, step + lane);
                 }
                 u = read_lane(u_pool, step - u_base);
-                const uint32_t cand = cur_cand;
+                uint32_t cand = cur_cand;
                 uint32_t cur_after = 0;
                 if constexpr (Hard) cur_after = linked ? (cur_link >> 16) : tour[pc + 1 == n ? 0 : pc + 1];
-                const bool avail = cur_free && (!Hard || relocation_ok(cand, current, cur_after));
+                double weight = prod;
+                bool avail;
+                uint32_t source = kNoSource;
+                if constexpr (Escape) {
+                    // the escape gate (one word in 16 of the ant's escape
+                    // stream of this step) opens a slot view of the row:
+                    // a replaced slot weighs default_trail * d^-beta (the
+                    // backup rows never hold primary cities, so the
+                    // reference's pheromone lookup gives the default trail)
+                    // and may open new edges (cuda/escape_device.cuh:41-80)
+                    bool free_city = cur_free;
+                    EscapeStream gate(key, batch, local_ant, step, false);
+                    if (!p.escape_disabled && (gate.draw() & 15u) == 0u) {
+                        ++e_gates;
+                        const uint32_t bval = lane < p.bw ? backup[static_cast<size_t>(current) * p.bw + lane] : n;
+                        const LaneView view = lane_slot_view(cur_cand, pw, bval, p.bw, n, current, gate, esc_scratch);
+                        e_replaced += view.replacements;
+                        source = view.source;
+                        if (source != kNoSource) {
+                            cand = view.member;
+                            free_city = !(flags[cand] & kVisited);
+                            weight = dtrail * eta_backup[static_cast<size_t>(current) * p.bw + source];
+                        }
+                    }
+                    uint32_t novel[3], count;
+                    const uint32_t status = free_city ? esc_relocation(cand, current, cur_after, source != kNoSource,
+                                                                       novel, count) : 1u;
+                    e_construction_full += static_cast<uint32_t>(__builtin_popcountll(ballot(free_city && status == 3u)));
+                    avail = free_city && status == 0u;
+                } else {
+                    avail = cur_free && (!Hard || relocation_ok(cand, current, cur_after));
+                }
                 const uint64_t amask = ballot(avail);
                 if (amask) {
                     // Roulette of select_next: the first available slot j with
@@ -643,7 +785,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     double total = 0.0;
                     uint32_t sel_lane = kWave;
                     if (pw <= 16) {
-                        const double w = avail ? prod : 0.0;
+                        const double w = avail ? weight : 0.0;
                         const double fast = row_scan4(w);
                         const double fast_total = read_lane(fast, 15);
                         if (fast_total > 0.0) {
@@ -663,12 +805,28 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                         acc = 0.0;
                         for (uint64_t m = amask; m; m &= m - 1) {
                             const uint32_t i = ctz64(m);
-                            const double v = read_lane(prod, i);
+                            const double v = read_lane(weight, i);
                             acc = (i <= lane) ? acc + v : acc;
                         }
                         total = read_lane(acc, 63);
                     }
-                    if (sel_lane < kWave) {
+                    if constexpr (Escape) {
+                        // the chosen slot also tells whether the move may open new edges
+                        if (sel_lane == kWave) {
+                            if (total > 0.0) {
+                                const double threshold = u * total;
+                                const uint64_t hit = ballot(avail && threshold < acc);
+                                sel_lane = hit ? ctz64(hit) : last64(amask);
+                            } else {
+                                const uint32_t count = static_cast<uint32_t>(__builtin_popcountll(amask));
+                                uint32_t k = static_cast<uint32_t>(u * count);
+                                if (k >= count) k = count - 1;
+                                sel_lane = nth_set_bit(amask, k);
+                            }
+                        }
+                        selected = read_lane(cand, sel_lane);
+                        novel = read_lane(source, sel_lane) != kNoSource;
+                    } else if (sel_lane < kWave) {
                         selected = read_lane(cand, sel_lane);
                     } else if (total > 0.0) {
                         const double threshold = u * total;
@@ -686,7 +844,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                         const uint32_t j = base + lane;
                         const uint32_t b = j < p.bw ? backup[static_cast<size_t>(current) * p.bw + j] : n;
                         const uint64_t bm = ballot(b < n && !(flags[b] & kVisited) &&
-                                                   (!Hard || relocation_ok(b, current, cur_after)));
+                                                   (!Hard || reloc_ok(b, current, cur_after)));
                         if (bm) selected = read_lane(b, ctz64(bm));
                     }
                     if (selected == n) {
@@ -694,7 +852,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                         const uint32_t cxy = s_xy[current];
                         uint64_t best = ~0ull;
                         for (uint32_t v = lane; v < n; v += kWave) {
-                            if (!(flags[v] & kVisited) && (!Hard || relocation_ok(v, current, cur_after))) {
+                            if (!(flags[v] & kVisited) && (!Hard || reloc_ok(v, current, cur_after))) {
                                 const uint64_t kv = (static_cast<uint64_t>(euc2d(cxy, s_xy[v])) << 32) | v;
                                 best = kv < best ? kv : best;
                             }
@@ -704,6 +862,14 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     }
                 }
                 selected = uniform(selected);
+                if constexpr (Escape) {
+                    // the move's new edges join the ant's cache
+                    if (selected < n) {
+                        uint32_t novel_edges[3], count;
+                        (void)esc_relocation(selected, current, cur_after, novel, novel_edges, count);
+                        esc_commit(novel_edges, count);
+                    }
+                }
                 if (selected < n) {
                     const uint32_t c = s_nbr[selected * K + row_lane];
                     const uint8_t fl = flags[c < n ? c : 0u];
@@ -740,6 +906,20 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             cur_link = current | (after << 16);   // selected's links when the move is the identity
             if (target_after != selected) {
                 ++relocations;
+                if constexpr (Escape) {
+                    // a move that may open new edges marks its five endpoints
+                    // as anchors (cuda/faco_device.cuh:213-218)
+                    if (novel) {
+                        if (lane == 0) {
+                            flags[current] = static_cast<uint8_t>(flags[current] | kAnchor);
+                            flags[target_after] = static_cast<uint8_t>(flags[target_after] | kAnchor);
+                            flags[selected] = static_cast<uint8_t>(flags[selected] | kAnchor);
+                            flags[old_prev] = static_cast<uint8_t>(flags[old_prev] | kAnchor);
+                            flags[after] = static_cast<uint8_t>(flags[after] | kAnchor);
+                        }
+                        ++e_relocations;
+                    }
+                }
                 if (linked) {
                     // Unlink `selected`, relink it between `current` and its
                     // successor; the array's first city moves on when it is
@@ -954,6 +1134,60 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             qhead = 0; qtail = 0; qsize = n;
             wave_sync();
         }
+        // Escape mode: each anchor's LS row becomes its escape view, sorted
+        // by (distance, city) (escape stream of the city in the LS domain),
+        // and the footprint marks the rows with replaced slots
+        // (cuda/escape_device.cuh:111-126); then the parent's footprint and
+        // the anchors put their cities on the checklist, in city order
+        // (cuda/faco_device.cuh:270-291).
+        uint16_t* const my_rows = Escape ? p.esc_rows + static_cast<size_t>(ant) * ns * lw : nullptr;
+        if constexpr (Escape) {
+            uint8_t* my_fp = p.esc_footprint + static_cast<size_t>(ant) * ns;
+            for (uint32_t v0 = 0; v0 < n; v0 += kWave) {
+                const uint32_t v = v0 + lane, vi = v < n ? v : 0u;
+                const bool anchor = v < n && (flags[vi] & kAnchor);
+                uint64_t am = ballot(anchor);
+                e_anchor_nodes += static_cast<uint32_t>(__builtin_popcountll(am));
+                if (v < n && !anchor) my_fp[v] = 0;
+                while (am) {
+                    const uint32_t node = v0 + ctz64(am);
+                    am &= am - 1;
+                    EscapeStream rows_rng(key, batch, local_ant, node, true);
+                    const uint32_t orig = lane < lw ? s_nbr[node * K + p.ls_offset + lane] : n;
+                    const uint32_t bval = lane < p.bw ? backup[static_cast<size_t>(node) * p.bw + lane] : n;
+                    const LaneView view = lane_slot_view(orig, lw, bval, p.bw, n, node, rows_rng, esc_scratch);
+                    uint32_t member = view.member, src = view.source;
+                    lane_sort_row(member, src, view.valid, node, s_xy, esc_order);
+                    if (lane < lw)
+                        my_rows[static_cast<size_t>(node) * lw + lane] = static_cast<uint16_t>(
+                            lane < view.valid ? (member | (src != kNoSource ? 0x8000u : 0u)) : n);
+                    e_ls_replaced += view.replacements;
+                    if (lane == 0) my_fp[node] = view.replacements ? 1 : 0;
+                }
+            }
+            const uint8_t* pfp = p.parent_fp + static_cast<size_t>(colony) * ns;
+            for (uint32_t v0 = 0; v0 < n; v0 += kWave) {
+                const uint32_t v = v0 + lane, vi = v < n ? v : 0u;
+                const uint8_t f = v < n ? flags[vi] : 0;
+                const bool old_view = v < n && pfp[vi] != 0;
+                const bool anchor = (f & kAnchor) != 0, queued = (f & kQueued) != 0;
+                const bool fresh = (old_view || anchor) && !queued;
+                const uint64_t fm = ballot(fresh);
+                e_old_views += static_cast<uint32_t>(__builtin_popcountll(ballot(old_view && !queued)));
+                e_anchor_pushes += static_cast<uint32_t>(__builtin_popcountll(ballot(anchor && !old_view && !queued)));
+                if (fresh) {
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(fm >> 32),
+                                          __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(fm), 0u));
+                    flags[v] = static_cast<uint8_t>(f | kQueued);
+                    queue[(qtail + rank) % n] = static_cast<uint16_t>(v);
+                }
+                const uint32_t pushed = static_cast<uint32_t>(__builtin_popcountll(fm));
+                qtail = (qtail + pushed) % n;
+                qsize += pushed;
+                logged += pushed;
+                wave_sync();
+            }
+        }
         const uint64_t tick2 = Profile ? wall_clock64() : 0;
 
         // ------------------------------------------------------------ checklist 2-opt
@@ -1013,12 +1247,29 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             qhead = qhead + 1 == n ? 0u : qhead + 1;
             --qsize;
             // only the queued bit matters after construction: clear the byte
-            if (lane == 0) flags[a] = 0;
+            // (Escape mode keeps the anchor bit: an anchor reads its escape row)
+            bool a_anchor = false;
+            if constexpr (Escape) {
+                const uint8_t fa = flags[a];
+                a_anchor = (fa & kAnchor) != 0;
+                wave_sync();
+                if (lane == 0) flags[a] = static_cast<uint8_t>(fa & ~kQueued);
+            } else {
+                if (lane == 0) flags[a] = 0;
+            }
             ++processed;
             // every lane reads (in-bounds addresses, no exec masks); lanes
             // outside the scan or past the row end see b = n and count as
             // failing, their other values are not used
-            const uint32_t row_b = s_nbr[a * K + column];
+            uint32_t row_b = s_nbr[a * K + column];
+            uint32_t permit = 0;   // Escape mode: the slot was replaced, its move may open new edges
+            if constexpr (Escape) {
+                if (a_anchor) {
+                    const uint32_t e = my_rows[static_cast<size_t>(a) * lw + (cslot < lw ? cslot : lw - 1u)];
+                    row_b = e & 0x7fffu;
+                    permit = e >> 15;
+                }
+            }
             const uint32_t b = min(row_b | off_scan, n);   // row entries are at most n
             const bool real = b < n;
             const uint32_t bi = real ? b : a;
@@ -1032,7 +1283,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             const uint32_t closing = (a_prev & kind_mask) | (a_next & ~kind_mask);
             const uint32_t xclosing = s_xy[closing];
             const int32_t current_edge = static_cast<int32_t>(euc2d(xa, xclosing));
-            const int32_t dab = g_nd[a * K + column];
+            const int32_t dab = (Escape && a_anchor) ? static_cast<int32_t>(euc2d(xa, xb)) : static_cast<int32_t>(g_nd[a * K + column]);
             const bool bad = !real || !(current_edge > dab);
             // First failing slot per kind; lanes past lw count as failing.
             const uint64_t bad_mask = ballot(bad);
@@ -1043,15 +1294,29 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             const uint32_t prefix1 = ctz64(static_cast<uint64_t>(bad1) | (1ull << 32));
             // Budget: kind 0 first, then kind 1; the limit check precedes every
             // evaluation and a node cut short commits nothing.
+            // (Hard mode: a node cut short still counts the rejections of the
+            // moves it evaluated, as the reference's sequential scan does,
+            // cuda/faco_device.cuh:316-336; it stops after counting them)
+            bool cut = false;
             const uint32_t used0 = min(limit - evaluations, prefix0);
             evaluations += used0;
             checks += min(used0 + 1u, lw);   // the failing slot counts as checked
-            if (used0 < prefix0) { limit_hit = 1; break; }
-            const uint32_t used1 = min(limit - evaluations, prefix1);
-            evaluations += used1;
-            checks += min(used1 + 1u, lw);
-            if (used1 < prefix1) { limit_hit = 1; break; }
-            if (used0 + used1 == 0) continue;
+            uint32_t used1 = 0;
+            if (used0 < prefix0) {
+                limit_hit = 1;
+                if constexpr (!Hard) break;
+                cut = true;
+            } else {
+                used1 = min(limit - evaluations, prefix1);
+                evaluations += used1;
+                checks += min(used1 + 1u, lw);
+                if (used1 < prefix1) {
+                    limit_hit = 1;
+                    if constexpr (!Hard) break;
+                    cut = true;
+                }
+            }
+            if (!cut && used0 + used1 == 0) continue;
 
             // evaluated slots of this lane's kind (lanes past lw never qualify)
             const bool active = cslot < ((used1 & kind_mask) | (used0 & ~kind_mask));
@@ -1067,14 +1332,25 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             // (reference cuda/faco_device.cuh:383-394)
             bool move_allowed = true;
             if constexpr (Hard) {
+                uint32_t status = 0;   // Escape mode: 3 = the cache is full
                 if (active && nb != a && b != closing) {
                     const uint32_t removed[3][2] = {{a, closing}, {b, nb}, {0u, 0u}};
                     const uint32_t added[3][2] = {{a, b}, {closing, nb}, {0u, 0u}};
-                    move_allowed = edges_allowed(2, removed, added);
+                    if constexpr (Escape) {
+                        // graph or cache edges, or new ones through a replaced
+                        // slot while the cache has room (escape_edges.hpp:117-122)
+                        uint32_t novel[3], count;
+                        status = esc_status(2, removed, added, permit != 0, novel, count);
+                        move_allowed = status == 0u;
+                    } else {
+                        move_allowed = edges_allowed(2, removed, added);
+                    }
                 }
                 // every evaluated move that would leave the graph counts
                 // (reference cpp/src/faco_cpu.cpp:180-183)
                 rejections += static_cast<uint32_t>(__builtin_popcountll(ballot(active && !move_allowed)));
+                if constexpr (Escape) e_ls_full += static_cast<uint32_t>(__builtin_popcountll(ballot(status == 3u)));
+                if (cut) break;
             }
             const uint32_t key_value = active && move_allowed
                 ? ((static_cast<uint32_t>(gain + (1 << 20)) << 6) | (63u - tie_index)) : 0u;
@@ -1178,6 +1454,15 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             uint32_t move[4];
             if (best_kind == 0) { move[0] = a_next; move[1] = nnode; move[2] = a; move[3] = bnode; }
             else { move[0] = a; move[1] = bnode; move[2] = a_prev; move[3] = nnode; }
+            if constexpr (Escape) {
+                // the move's new edges join the ant's cache
+                const uint32_t closing_node = best_kind == 0 ? a_next : a_prev;
+                const uint32_t removed[3][2] = {{a, closing_node}, {bnode, nnode}, {0u, 0u}};
+                const uint32_t added[3][2] = {{a, bnode}, {closing_node, nnode}, {0u, 0u}};
+                uint32_t novel[3], count;
+                (void)esc_status(2, removed, added, read_lane(permit, best_lane) != 0, novel, count);
+                esc_commit(novel, count);
+            }
             flip_nodes(move[0], move[1]);
             ++accepted;
             gain_sum += best_gain;
@@ -1263,6 +1548,14 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             info.constraint_rejections = rejections;
             info.pad = 0;
             p.out_info[ant] = info;
+            if constexpr (Escape) {
+                // EscapeStats order (escape_slots.hpp:70-75); every step
+                // (and an exhausted one) was one construction opportunity
+                uint32_t* st = p.esc_stats + static_cast<size_t>(ant) * kEscStats;
+                st[0] = steps + exhausted; st[1] = e_gates; st[2] = e_replaced; st[3] = e_relocations;
+                st[4] = e_anchor_nodes; st[5] = e_ls_replaced; st[6] = e_old_views; st[7] = e_anchor_pushes;
+                st[8] = e_new; st[9] = e_construction_full; st[10] = e_ls_full; st[11] = 0;
+            }
             if constexpr (Profile) {
                 uint64_t* t = p.phase_ticks + kPhaseSlots * static_cast<size_t>(ant);
                 t[0] = tick1 - tick0; t[1] = tick2 - tick1; t[2] = tick3 - tick2;
@@ -1302,6 +1595,13 @@ __global__ void __launch_bounds__(1024) faco_ant_search(const AntParams p) {
 template<bool NbrLds>
 __global__ void __launch_bounds__(1024) faco_ant_search_hard(const AntParams p) {
     ant_search<NbrLds, false, false, false, false, true>(p, ColonyParams{}, 1u, nullptr);
+}
+
+// Escape mode (the Hard graph with escape gates, slot views, per-ant edge
+// caches and anchor LS views), per batch.
+template<bool NbrLds>
+__global__ void __launch_bounds__(1024) faco_ant_search_escape(const AntParams p) {
+    ant_search<NbrLds, false, false, false, false, true, false, true>(p, ColonyParams{}, 1u, nullptr);
 }
 
 template<bool NbrLds, bool OrOpt>
@@ -1437,7 +1737,8 @@ hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t st
     const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
     if (p.graph_offsets) {
         if (diag || p.phase_ticks || oo) return hipErrorInvalidValue;   // Hard mode: plain 2-opt, no diagnostics
-        nl ? go(faco_ant_search_hard<true>) : go(faco_ant_search_hard<false>);
+        if (p.esc_rows) nl ? go(faco_ant_search_escape<true>) : go(faco_ant_search_escape<false>);
+        else nl ? go(faco_ant_search_hard<true>) : go(faco_ant_search_hard<false>);
         return hipGetLastError();
     }
     if (diag) {

@@ -32,6 +32,8 @@ struct Prepared {
     // Hard mode: the registered undirected graph as CSR (offsets [n + 1],
     // sorted neighbours), empty for an unrestricted instance
     std::vector<uint32_t> graph_offsets, graph_neighbors;
+    // Escape mode: 1/d^beta of every backup slot [n][bw] (0 for padding)
+    std::vector<double> eta_backup;
 };
 
 faco_settings normalized_settings(faco_settings settings, uint32_t n);

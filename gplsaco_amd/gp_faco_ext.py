@@ -28,6 +28,11 @@ from ._native import (BaselinePolicy, Checkpoints, ControlSpec, ControlState, Co
 numeric_backend = "euc2d_int32"
 campaign_interface_version = 1
 
+# Escape counters in the reference's EscapeStats order (escape_slots.hpp:70-75)
+ESCAPE_COUNTERS = ("construction_opportunities", "construction_gates", "construction_replaced_slots",
+                   "escape_relocations", "ls_anchor_nodes", "ls_replaced_slots", "old_view_reactivations",
+                   "anchor_reactivations", "new_edges", "construction_capacity_rejections", "ls_capacity_rejections")
+
 _SETTING_FIELDS = ("ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
                    "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")
 
@@ -473,14 +478,17 @@ class FacoBatchEngine:
         population = _unsigned(population_size, "population_size", 32)
         if constraint_mode not in ("unrestricted", "hard", "escape"):
             raise ValueError("unknown constraint mode")
-        if constraint_mode == "escape":
-            raise ValueError("the escape constraint mode is not available in this engine")
         if not isinstance(settings, FixedFacoSettings):
             raise TypeError("settings must be FixedFacoSettings")
         handle = ctypes.c_void_p()
         check(lib().faco_engine_create(dimension, colonies, ctypes.byref(settings._native()),
-                                       1 if constraint_mode == "hard" else 0, population, ctypes.byref(handle)))
+                                       {"unrestricted": 0, "hard": 1, "escape": 2}[constraint_mode], population,
+                                       ctypes.byref(handle)))
         self.constraint_mode = constraint_mode
+        # diagnostic (the reference's BatchDiagnosticControls::disable_escape):
+        # an Escape engine's evaluations keep every escape gate shut, which
+        # must reproduce the Hard run
+        self.disable_escape = False
         self._handle = handle
         self.dimension, self.colonies, self.population_size = dimension, colonies, population
         s = Settings()
@@ -642,17 +650,40 @@ class FacoBatchEngine:
         self._graph_edges[key] = int(edges.shape[0])
         return {"cheap_seconds": cheap.value, "preparation_seconds": prep.value}
 
+    def _escape_flag(self) -> int:
+        if self.disable_escape and self.constraint_mode != "escape":
+            raise ValueError("only an Escape-mode engine can disable its escape")
+        return 16 if self.disable_escape else 0
+
     def _constraint_output(self, out: dict, keys, rejections: int) -> dict:
-        """The Hard-mode keys of a result (reference bindings.cpp:364-371):
-        the graph's undirected edge count per colony and the 2-opt moves
-        rejected for leaving the graph."""
-        if self.constraint_mode == "hard":
-            out.update({"constraint_mode": "hard", "graph_spec_id": 1,
+        """The Hard- and Escape-mode keys of a result (reference
+        bindings.cpp:364-388): the graph's undirected edge count per colony,
+        the 2-opt moves rejected for leaving the graph and, for Escape, the
+        escape counters summed over the colonies."""
+        if self.constraint_mode != "unrestricted":
+            reserved = 0
+            if self.constraint_mode == "escape":
+                counters, reserved = self._escape_counters(len(keys))
+                out.update({"escape_spec_id": 1, "escape_edge_capacity_per_ant": 64,
+                            "escape_counters": {name: int(v) for name, v in zip(ESCAPE_COUNTERS, counters.sum(axis=0))}})
+            out.update({"constraint_mode": self.constraint_mode, "graph_spec_id": 1,
                         "graph_edges_per_colony": [self._graph_edges[int(k)] for k in keys],
                         "completed_constraint_rejections": int(rejections),
                         "preparation_scope": "engine_only; external candidate graph cached",
-                        "reserved_escape_device_bytes": 0})
+                        "reserved_escape_device_bytes": int(reserved)})
         return out
+
+    def _escape_counters(self, tasks):
+        out = np.zeros((_unsigned(tasks, "tasks", 32), len(ESCAPE_COUNTERS)), dtype=np.uint64)
+        reserved = ctypes.c_uint64()
+        check(lib().faco_engine_escape_counters(self._handle, out.reshape(-1), out.shape[0], ctypes.byref(reserved)))
+        return out, reserved.value
+
+    def colony_escape_counters(self, tasks) -> np.ndarray:
+        """Escape mode: each task's escape counters of the last count
+        evaluation, [tasks][11] in ESCAPE_COUNTERS order (the reference's
+        EscapeStats, escape_slots.hpp:70-75)."""
+        return self._escape_counters(tasks)[0]
 
     def colony_constraint_rejections(self, tasks) -> np.ndarray:
         """Hard mode: the 2-opt moves each task's colony rejected in the last
@@ -728,7 +759,7 @@ class FacoBatchEngine:
         result = Result()
         started = time.perf_counter()
         flags = (1 if use_graph else 0) | (2 if profile_kernels else 0) | (4 if time_kernels or profile_events_only else 0) \
-            | (8 if continue_run else 0)
+            | (8 if continue_run else 0) | self._escape_flag()
         check(lib().faco_engine_evaluate_faco(self._handle, keys, seeds, tasks, limit, mne, flags,
                                               ctypes.byref(result), tours.reshape(-1), costs, work.reshape(-1),
                                               request.argument()))
@@ -928,12 +959,12 @@ class FacoBatchEngine:
             for name, array in traces.items():
                 setattr(trace, name, array.ctypes.data_as(ctypes.c_void_p))
         started = time.perf_counter()
-        check(call(ctypes.byref(spec), (1 if use_graph else 0) | (4 if profile_events_only else 0),
+        check(call(ctypes.byref(spec), (1 if use_graph else 0) | (4 if profile_events_only else 0) | self._escape_flag(),
                    ctypes.byref(trace) if trace else None, ctypes.byref(result), tours.reshape(-1), costs,
                    work.reshape(-1), ctypes.addressof(states), request.argument()))
         del prog_arr, ctrl_arr
         rejections = None
-        if self.constraint_mode == "hard":
+        if self.constraint_mode != "unrestricted":
             rejections = self.colony_constraint_rejections(C)
         return {"result": result, "tours": tours, "costs": costs, "work": work, "elapsed": time.perf_counter() - started,
                 "keys": np.resize(panel_keys, C), "rejections": rejections,
@@ -1214,6 +1245,7 @@ class MixedFacoBatchEngine(FacoBatchEngine):
         check(lib().faco_engine_settings(self._handle, ctypes.byref(s)))
         self.settings = FixedFacoSettings._from_native(s)
         self.constraint_mode = "unrestricted"
+        self.disable_escape = False
         self._charges = {}
         self._graph_edges = {}
         self._sizes = {}

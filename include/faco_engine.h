@@ -198,7 +198,10 @@ int faco_prepare_common_initial(const double* xy, uint32_t n, const faco_setting
 /* FacoBatchEngine(dimension, colonies, settings, constraint_mode, population_size)
  * (reference cpp/include/gp_faco/batch_engine.hpp:68-70; bindings.cpp:557-573).
  * constraint_mode: 0 unrestricted, 1 Hard (population 1; instances come
- * through faco_engine_register_graph_problem). population_size in [1, 128]. */
+ * through faco_engine_register_graph_problem), 2 Escape (the Hard graph
+ * with escape gates: reference cuda/escape_device.cuh:29-128,
+ * batch_engine.cu:1014-1025; at most 16 primary, 64 backup and 20 LS slots,
+ * fewer than 32768 cities, no learned prior). population_size in [1, 128]. */
 int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_settings* settings,
                        uint32_t constraint_mode, uint32_t population_size, faco_engine** out);
 /* Mixed-size batch engine: instances of any n up to max_dimension in one
@@ -346,6 +349,16 @@ int faco_engine_problem_data(faco_engine* engine, uint64_t instance_key, uint32_
                                only. No counterpart in the reference (whose plain entry point
                                always starts from the prepared tour); a steady-state window of a
                                long run is measured this way */
+#define FACO_RUN_NO_ESCAPE 16u /* Escape-mode engines: no escape gate opens, so the run must equal
+                                  the Hard run of the same graph (reference
+                                  BatchDiagnosticControls::disable_escape, batch_engine.hpp) */
+/* Escape counters per colony, in the reference's EscapeStats order
+ * (cpp/include/gp_faco/escape_slots.hpp:70-75; bindings.cpp:376-386):
+ * construction_opportunities, construction_gates,
+ * construction_replaced_slots, escape_relocations, ls_anchor_nodes,
+ * ls_replaced_slots, old_view_reactivations, anchor_reactivations,
+ * new_edges, construction_capacity_rejections, ls_capacity_rejections. */
+#define FACO_ESCAPE_COUNTERS 11u
 
 /* Curve checkpoints (reference cuda/batch_engine.cu:635-653, the
  * checkpoint_iterations argument of bindings.cpp:701-757,818-853): the
@@ -474,6 +487,14 @@ int faco_engine_evaluate_deadline(faco_engine* engine, const uint64_t* instance_
  * cuda/batch_engine.cu:432, colony_work[index][2]). `count` must equal the
  * last evaluation's task count. */
 int faco_engine_constraint_rejections(faco_engine* engine, uint64_t* out, uint32_t count);
+
+/* Per-task escape counters of an Escape-mode engine's last count
+ * evaluation (plain or controlled), [count][FACO_ESCAPE_COUNTERS] in task
+ * order; their sums are the reference's result["escape_counters"]
+ * (bindings.cpp:372-388). `count` must equal the last evaluation's task
+ * count. reserved_bytes (may be null): the device memory of the Escape
+ * buffers (result["reserved_escape_device_bytes"]). */
+int faco_engine_escape_counters(faco_engine* engine, uint64_t* out, uint32_t count, uint64_t* reserved_bytes);
 
 /* set_preparation_charges(instance_key, cheap_seconds, preparation_seconds)
  * (reference batch_engine.cu:318-330, bindings.cpp:651-661): frozen

@@ -144,7 +144,7 @@ def test_hard_registration_rules(ext):
     with pytest.raises(ValueError):
         hard.set_local_search("2opt+oropt")
     with pytest.raises(ValueError):
-        ext.FacoBatchEngine(n, 2, s, constraint_mode="escape")
+        ext.FacoBatchEngine(n, 2, s, constraint_mode="sideways")
 
 
 
@@ -344,3 +344,35 @@ def test_matched_prior_graphs_run_in_hard_mode_like_the_oracle(ext, oracle_mod):
             assert _cycle(got["items"][c]["tour"]) <= edges
         np.testing.assert_array_equal(got["colony_work"], want["work"])
         np.testing.assert_array_equal(hard.colony_constraint_rejections(2), want["constraint_rejections"])
+
+
+def test_hard_rejections_of_a_node_cut_short_by_the_limit(ext, oracle_mod):
+    """A node the LS evaluation limit cuts short still counts the rejected
+    moves it evaluated before the limit, as the reference's sequential scan
+    does (cuda/faco_device.cuh:316-336): small limits stop most ants inside
+    a node, and device and oracle agree on tours, work and rejections."""
+    O = oracle_mod
+    n, batches = 200, 4
+    xy = clustered_instance(n, seed=71, instance=0)
+    s = ext.FixedFacoSettings()
+    s.ants = 16
+    plain = ext.FacoBatchEngine(n, 2, s)
+    plain.register_problem(1, xy)
+    spec, edges = _spec(plain.problem_data(1), [])
+    graph = O.csr(sorted(edges), n)
+    seeds = np.array([8, 9], dtype=np.uint64)
+    for limit in (7, 37, 101):
+        s.ls_evaluation_limit = limit
+        hard = ext.FacoBatchEngine(n, 2, s, constraint_mode="hard")
+        hard.register_graph_problem(1, xy, spec)
+        settings = O.default_settings(**{f: getattr(s, f) for f in (
+            "ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
+            "epoch_source_probability", "ls_evaluation_limit", "initial_ls_evaluation_limit")})
+        announce("hard-cut", limit)
+        got = hard.evaluate_faco_evaluations(np.array([1, 1], dtype=np.uint64), seeds, s.ants * batches)
+        want = O.run_faco(xy[None].astype(np.int32), [1], seeds, settings, batches, 8, graphs=[graph])
+        for c in range(2):
+            assert got["items"][c]["tour"] == want["tours"][c].tolist(), (limit, c)
+        np.testing.assert_array_equal(got["colony_work"], want["work"])
+        np.testing.assert_array_equal(hard.colony_constraint_rejections(2), want["constraint_rejections"])
+        assert want["work"][:, 1].tolist() == [limit * s.ants * batches] * 2 or limit > 100
