@@ -45,6 +45,21 @@ Parity status of each oracle function and what pins it:
   in tests/test_oracle_control.py (archive = archive_update of the batch's
   pool, restarts reset the trails to the new upper bound, feedback = the
   reference's update_feedback).
+- Hard mode (run_faco / run_controlled with graphs=): move predicates against
+  complete before/after edge sets (tests/cpp/test_hard_edges.cpp's method),
+  the Hard 2-opt and selection live against the reference core (oracle/_ref)
+  in tests/test_oracle_hard.py.
+- escape primitives (slot views, edge transactions, streams): the reference's
+  scripted slot cases, exhaustive move checks and stream separation
+  (tests/test_oracle_escape.py).
+- Escape mode (escape_ant, run_faco / run_controlled with escape=): the
+  reference's per-ant replay checks of tests/cpp/test_escape_engine.cpp
+  (moves rebuild the tour, new edges only through permitted moves and within
+  the cache, anchors = the five endpoints, LS views keep their valid counts,
+  order and replacement bound, footprints mark replaced rows) and
+  disabled-escape runs equal to Hard runs (tests/test_oracle_escape_mode.py).
+  The escape stream is this Philox's, not cuRAND's (unpinned at that
+  boundary, as above).
 """
 from __future__ import annotations
 
