@@ -155,7 +155,7 @@ struct Engine {
     LaunchShape shape;
     uint32_t forced_waves = 0, forced_blocks_per_colony = 0, xcd_remap = 1;
     uint32_t local_search = 0;  // 0: checklist 2-opt, 1: 2-opt + Or-opt
-    double roulette_margin = 0x1.0p-46;  // fast roulette margin (relative to the row total)
+    double roulette_margin = kRouletteMargin;  // fast roulette margin (relative to the row total)
     // Mixed-size batch: instances of any n up to `n` (which is then the
     // stride of every per-city array; config.ants is the ant stride, the
     // ants of the largest n). Each evaluation uploads every colony's shape
@@ -1628,7 +1628,7 @@ double register_problems_device(Engine& e, const uint64_t* keys, const double* x
         ap.nbr_lds_bytes = round16_host(2u * stride * K); ap.per_wave_bytes = ant_wave_bytes(stride);
         ap.explicit_mode = 1; ap.per_ant_parent = 1; ap.full_checklist = 1; ap.block_n = d_sizes.data();
         ap.ls_limit = std::min<uint64_t>(e.config.initial_ls_evaluation_limit, UINT32_MAX);
-        ap.roulette_margin = 0x1.0p-46;
+        ap.roulette_margin = kRouletteMargin;
         ap.coords = d_coords.data(); ap.nbr = d_nbr.data(); ap.nbr_dist = d_nbr_dist.data();
         ap.parent = d_nn.data(); ap.links = d_links.data(); ap.visits = d_nn.data(); ap.targets = d_targets.data();
         ap.parent_cost = d_costs.data(); ap.out_tours = d_out.data(); ap.out_info = d_info.data();
@@ -1990,7 +1990,7 @@ int faco_engine_autotune_control(faco_engine* engine, const uint64_t* panel_keys
 int faco_engine_set_roulette_margin(faco_engine* engine, double margin) {
     return guarded([&] {
         require(engine, "null argument");
-        require(margin >= 0x1.0p-46, "the roulette margin can only be widened (2^-46 or more)");
+        require(margin >= kRouletteMargin, "the roulette margin can only be widened (2^-18 or more)");
         engine->impl->roulette_margin = margin;
         engine->impl->drop_graph();
     });
@@ -2602,7 +2602,7 @@ int faco_diagnostic_construct_ls(const double* xy, uint32_t n, const uint32_t* l
         p.chunk = chunk; p.blocks_per_colony = 1;
         p.nbr_lds_bytes = round16_host(2u * n * K); p.nbr_stride = stride; p.per_wave_bytes = ant_wave_bytes(n);
         p.xcd_remap = 0; p.explicit_mode = 1; p.per_ant_parent = 1; p.log_checklist = 1;
-        p.roulette_margin = 0x1.0p-46;
+        p.roulette_margin = kRouletteMargin;
         require(local_search <= 1, "local search mode must be 0 or 1");
         p.or_opt = local_search;
         p.ls_limit = std::min<uint64_t>(limit, UINT32_MAX);

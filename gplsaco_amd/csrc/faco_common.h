@@ -197,6 +197,10 @@ constexpr uint32_t kPhaseSlots = 10;
 // fallback list ranking keeps kMaxLinked / 64 entries per lane in
 // registers); such workgroups also stage the parent positions in LDS.
 constexpr uint32_t kMaxLinked = 1024;
+// Default margin of the ant kernel's FP32 roulette fast path, relative to the
+// row total (the FP32 and exact FP64 comparisons provably agree outside
+// 2^-20.4 of it; tests may only widen it).
+constexpr double kRouletteMargin = 0x1.0p-18;
 
 // Per-colony shape of a mixed-size batch: the colony's city count, ants and
 // its own iteration budget (a colony past its budget is left unchanged).

@@ -117,21 +117,24 @@ __device__ __forceinline__ double row_prefix_sum(double v) {
     return acc;
 }
 
-// Prefix sums over the 16 lanes of each row in 4 doubling steps (lane j ends
-// with v[0] + ... + v[j] in a tree order, not left to right).
-template<int Shift> __device__ __forceinline__ double row_shifted(double v) {
-    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(bits), 0x110 + Shift, 0xF, 0xF, true));
-    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(bits >> 32), 0x110 + Shift, 0xF, 0xF, true));
-    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+// Prefix sums over the 16 lanes of each row in 4 doubling steps in FP32 (one
+// DPP add per level; lane j ends with v[0] + ... + v[j] in a tree order, not
+// left to right): every term passes through at most 4 roundings, so each
+// prefix is within 4 * 2^-24 of the exact sum of the (non-negative) terms,
+// relative to the row total.
+template<int Shift> __device__ __forceinline__ float row_shifted_f32(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x110 + Shift, 0xF, 0xF, true));
 }
-__device__ __forceinline__ double row_scan4(double v) {
-    double acc = v;
-    acc = acc + row_shifted<1>(acc);
-    acc = acc + row_shifted<2>(acc);
-    acc = acc + row_shifted<4>(acc);
-    acc = acc + row_shifted<8>(acc);
+__device__ __forceinline__ float row_scan4_f32(float v) {
+    float acc = v;
+    acc = acc + row_shifted_f32<1>(acc);
+    acc = acc + row_shifted_f32<2>(acc);
+    acc = acc + row_shifted_f32<4>(acc);
+    acc = acc + row_shifted_f32<8>(acc);
     return acc;
+}
+__device__ __forceinline__ float read_lane(float v, uint32_t lane) {
+    return __builtin_bit_cast(float, read_lane(__builtin_bit_cast(uint32_t, v), lane));
 }
 
 // Colony update of one colony by the calling workgroup (threads tid of
@@ -689,6 +692,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         // links (LDS).
         uint32_t cur_cand = n, cur_link_parent = 0;
         bool cur_free = false;
+        const float margin32 = static_cast<float>(p.roulette_margin);
         if (!explicit_mode) {
             const uint32_t c = s_nbr[current * K + row_lane];
             const uint8_t fl = flags[c < n ? c : 0u];
@@ -702,6 +706,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         // Philox block per step, as step_uniform), so the serial scalar
         // Philox of every step becomes one vector Philox per 64 steps.
         double u_pool = 0.0;
+        float u_pool32 = 0.0f;
         uint32_t u_base = 0;
         while (mne < target && steps + 1 < n) {
             if constexpr (Profile) mark = clock64();
@@ -729,6 +734,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     uint32_t k_lo = static_cast<uint32_t>(key), k_hi = static_cast<uint32_t>(key >> 32);
                     asm volatile("" : "+s"(k_lo), "+s"(k_hi));
                     u_pool = step_uniform((static_cast<uint64_t>(k_hi) << 32) | k_lo, batch, local_ant, step + lane);
+                    u_pool32 = static_cast<float>(u_pool);
                 }
                 u = read_lane(u_pool, step - u_base);
                 uint32_t cand = cur_cand;
@@ -770,35 +776,39 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                 if (amask) {
                     // Roulette of select_next: the first available slot j with
                     // u * total < prefix_j, where total and the prefixes are
-                    // the exact left-to-right sums of the available weights.
-                    // Fast path (pw <= 16): tree-order prefix sums. Any order
-                    // of summing at most 16 non-negative doubles is within
-                    // 15 * 2^-53 * sum of the exact sum, so the tree-order
-                    // threshold and prefixes differ from the sequential ones
-                    // by less than 64 * 2^-53 * total; when no available
-                    // prefix lies within the margin (2^-46 * total unless a
-                    // test widens it) of the threshold, every
-                    // comparison has the same outcome as with the sequential
-                    // sums and so has the selection. Otherwise (and never in
-                    // practice) the sequential sums are formed.
+                    // the exact left-to-right FP64 sums of the available
+                    // weights (T = u * total; last available slot if none).
+                    // Fast path (pw <= 16): FP32 doubling-scan prefixes Q_j of
+                    // the weights rounded to FP32. With S the exact total:
+                    // the sequential FP64 prefixes and T lie within 2^-48 S
+                    // of the exact prefixes and u S; each Q_j within
+                    // (2^-24 + 4 * 2^-24) S, and T' = fl32(fl32(u) * Q_15)
+                    // within 2^-21.3 S of u S; so T - P_j and T' - Q_j differ
+                    // by less than 2^-20.4 S. When every available slot has
+                    // |T' - Q_j| above the margin (2^-18 Q_15 unless a test
+                    // widens it; Q_15 >= S (1 - 2^-21)), each comparison and
+                    // so the selection equal the sequential FP64 ones.
+                    // Otherwise (about one step in 10^4), or when the FP32
+                    // total leaves [2^-100, 2^100] (underflowing or huge
+                    // weights), the sequential FP64 sums are formed.
                     double acc = 0.0;
                     double total = 0.0;
                     uint32_t sel_lane = kWave;
                     if (pw <= 16) {
-                        const double w = avail ? weight : 0.0;
-                        const double fast = row_scan4(w);
-                        const double fast_total = read_lane(fast, 15);
-                        if (fast_total > 0.0) {
-                            const double threshold = u * fast_total;
-                            const double diff = threshold - fast;
-                            const double margin = fast_total * p.roulette_margin;
-                            if (!ballot(avail && diff <= margin && diff >= -margin)) {
+                        const float w32 = avail ? static_cast<float>(weight) : 0.0f;
+                        const float fast = row_scan4_f32(w32);
+                        const float fast_total = read_lane(fast, 15);
+                        if (fast_total >= 0x1p-100f && fast_total <= 0x1p100f) {
+                            const float threshold = read_lane(u_pool32, step - u_base) * fast_total;
+                            const float diff = threshold - fast;
+                            const float margin = fast_total * margin32;
+                            if (!ballot(avail && __builtin_fabsf(diff) <= margin)) {
                                 const uint64_t hit = ballot(avail && threshold < fast);
                                 sel_lane = hit ? ctz64(hit) : last64(amask);
                             }
                         }
                         if (sel_lane == kWave) {
-                            acc = row_prefix_sum(w);
+                            acc = row_prefix_sum(avail ? weight : 0.0);
                             total = read_lane(acc, 15);
                         }
                     } else {

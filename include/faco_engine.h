@@ -236,11 +236,11 @@ int faco_engine_launch_shape(const faco_engine* engine, uint32_t colonies, uint3
 int faco_engine_set_waves(faco_engine* engine, uint32_t waves, uint32_t blocks_per_colony);
 
 /* Test hook of the roulette (no counterpart in the reference): the ant
- * kernel decides a roulette step from tree-order prefix sums when no
+ * kernel decides a roulette step from FP32 tree-order prefix sums when no
  * available prefix lies within margin x row total of the threshold, and from
- * the exact left-to-right sums otherwise. The default margin is 2^-46, well
- * above the two orders' rounding difference; a wider margin (up to +inf:
- * left-to-right sums at every step) must give identical results. */
+ * the exact left-to-right FP64 sums otherwise. The default margin is 2^-18,
+ * above the 2^-20.4 bound on the two computations' difference; a wider margin
+ * (up to +inf: left-to-right sums at every step) must give identical results. */
 int faco_engine_set_roulette_margin(faco_engine* engine, double margin);
 
 /* One timed launch shape of faco_engine_autotune. */
