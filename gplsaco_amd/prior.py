@@ -25,7 +25,8 @@ DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "data" / "edge_prior_k20.npz
 # Multipliers are sigmoid(logit) ** PRIOR_EXPONENT. 2 was chosen among
 # 1, 1.5, 2, 2.5, 3 and 4 on validation instances (instance seed 4242, 16
 # TSP1000 instances x 8 seeds, 300 iterations), not on the comparison
-# instances (seed 7919).
+# instances (seed 7919); with labels from 1000-iteration 2-opt + Or-opt runs
+# it stays the best of 1.5, 2 and 3 on the same validation instances.
 PRIOR_EXPONENT = 2.0
 
 
@@ -149,12 +150,13 @@ def synthetic_instance(n, clustered, index, seed):
 
 
 def build_dataset(count: int, *, iterations: int = 200, primary_width: int = 20, seed: int = 0,
-                  n_range=(200, 1000), size_step: int = 100, log=None):
+                  n_range=(200, 1000), size_step: int = 100, local_search: str = "2opt", log=None):
     """Training edges of `count` synthetic instances (training_instances):
     each instance is solved by plain FACO (one colony, `iterations`
-    iterations, default ants) on the GPU, and its candidate edges are
-    labelled by membership in the colony's best tour. Returns (features
-    [E, 8] float32, labels [E] uint8, per-size summary)."""
+    iterations, default ants, `local_search` "2opt" or "2opt+oropt") on the
+    GPU, and its candidate edges are labelled by membership in the colony's
+    best tour. Returns (features [E, 8] float32, labels [E] uint8, per-size
+    summary)."""
     from . import gp_faco_ext as ext
     groups = {}
     for n, clustered, index in training_instances(count, seed, n_range, size_step):
@@ -165,6 +167,8 @@ def build_dataset(count: int, *, iterations: int = 200, primary_width: int = 20,
         settings = ext.FixedFacoSettings()
         settings.primary_width = primary_width
         engine = ext.FacoBatchEngine(n, len(items), settings)
+        if local_search != "2opt":
+            engine.set_local_search(local_search)
         keys = []
         for clustered, index in items:
             key, xy = synthetic_instance(n, clustered, index, seed)
@@ -240,7 +244,9 @@ def main(argv=None) -> int:
     sub = parser.add_subparsers(dest="command", required=True)
     t = sub.add_parser("train")
     t.add_argument("--instances", type=int, default=2000)
-    t.add_argument("--iterations", type=int, default=300, help="FACO iterations per labelling run")
+    t.add_argument("--iterations", type=int, default=1000, help="FACO iterations per labelling run")
+    t.add_argument("--local-search", default="2opt+oropt", choices=["2opt", "2opt+oropt"],
+                   help="local search of the labelling runs")
     t.add_argument("--epochs", type=int, default=8)
     t.add_argument("--batch", type=int, default=8192)
     t.add_argument("--lr", type=float, default=5e-4)
@@ -257,12 +263,13 @@ def main(argv=None) -> int:
     if args.command == "train":
         started = time.perf_counter()
         f, y, summary = build_dataset(args.instances, iterations=args.iterations, seed=args.seed,
-                                      log=lambda m: print(m, flush=True))
+                                      local_search=args.local_search, log=lambda m: print(m, flush=True))
         built = time.perf_counter()
         prior = EdgePrior(seed=args.seed + 1)
         losses = prior.train(f, y, batch=args.batch, epochs=args.epochs, lr=args.lr, pos_weight=args.pos_weight,
                              seed=args.seed + 3)
-        info = {"instances": args.instances, "labelling_iterations": args.iterations, "edges": int(f.shape[0]),
+        info = {"instances": args.instances, "labelling_iterations": args.iterations,
+                "labelling_local_search": args.local_search, "edges": int(f.shape[0]),
                 "positive_rate": float(y.mean()), "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
                 "pos_weight": args.pos_weight, "seed": args.seed, "epoch_losses": [float(v) for v in losses],
                 "dataset_seconds": built - started, "training_seconds": time.perf_counter() - built,

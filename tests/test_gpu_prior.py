@@ -203,15 +203,19 @@ def test_apply_edge_prior_uses_sigmoid_of_logits(ext, oracle_mod, prior_mod):
 
 def test_trained_prior_gains_over_plain_faco(prior_mod):
     """The committed trained network, on TSP1000 instances it was not trained
-    on (8 instances, half clustered, x 8 seeds, k = 20 candidates, 100
-    iterations of 128 ants): at the full budget the learned-prior FACO's mean
-    final tour length is below plain FACO's, and it reaches plain's mean with
-    at most 90% of the evaluations."""
+    on (16 instances, half clustered, x 8 seeds, k = 20 candidates, 128
+    ants): at the full budget the learned-prior FACO's mean final tour length
+    is below plain FACO's at 100 and at 300 iterations, and with the
+    300-iteration budget it reaches plain's mean with at most 70% of the
+    evaluations."""
     assert prior_mod.DEFAULT_WEIGHTS.exists(), "trained weights missing"
     net = prior_mod.EdgePrior.load()
     announce("prior-gain")
-    r = prior_mod.equal_budget_comparison(net, instances=8, seeds=8, iterations=100,
-                                          fractions=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0])
+    short = prior_mod.equal_budget_comparison(net, instances=16, seeds=8, iterations=100, fractions=[1.0])
+    print(short, flush=True)
+    assert short["gap_at_equal_fe"] < 0, short
+    r = prior_mod.equal_budget_comparison(net, instances=16, seeds=8, iterations=300,
+                                          fractions=[0.5, 0.6, 0.7, 1.0])
     print(r, flush=True)
     assert r["gap_at_equal_fe"] < 0, r
-    assert r["fe_fraction_to_reach_plain"] is not None and r["fe_fraction_to_reach_plain"] <= 0.9, r
+    assert r["fe_fraction_to_reach_plain"] is not None and r["fe_fraction_to_reach_plain"] <= 0.7, r
