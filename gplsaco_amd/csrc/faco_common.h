@@ -193,10 +193,12 @@ FACO_HD uint32_t pack_xy(uint32_t x, uint32_t y) { return x | (y << 16); }
 // ---------------------------------------------------------------------------
 // Kernel parameter blocks (passed by value as kernel arguments).
 constexpr uint32_t kPhaseSlots = 10;
-// Largest instance whose construction runs on the linked-list tour (the
-// fallback list ranking keeps kMaxLinked / 64 entries per lane in
-// registers); such workgroups also stage the parent positions in LDS.
-constexpr uint32_t kMaxLinked = 1024;
+// Largest instance whose construction runs on the linked-list tour; such
+// workgroups also stage the parent positions in LDS. The fallback list
+// ranking keeps its entries in registers (kRankedInRegisters / 64 per lane)
+// up to kRankedInRegisters cities and works in LDS above that.
+constexpr uint32_t kMaxLinked = 2048;
+constexpr uint32_t kRankedInRegisters = 1024;
 // Default margin of the ant kernel's FP32 roulette fast path, relative to the
 // row total (the FP32 and exact FP64 comparisons provably agree outside
 // 2^-20.4 of it; tests may only widen it).

@@ -1000,13 +1000,14 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         uint32_t edge_sum = 0;
         if (linked) {
             // Edge lengths from the final links (each city and its predecessor).
-            constexpr uint32_t kPer = kMaxLinked / kWave;
+            constexpr uint32_t kPer = kRankedInRegisters / kWave;
             const uint32_t chunks = (n + kWave - 1) / kWave;
 #pragma unroll
             for (uint32_t k = 0; k < kPer; ++k) {
                 const uint32_t v = lane + k * kWave;
                 if (k < chunks && v < n) edge_sum += euc2d(s_xy[lnk[v] & 0xffffu], s_xy[v]);
             }
+            for (uint32_t v = kRankedInRegisters + lane; v < n; v += kWave) edge_sum += euc2d(s_xy[lnk[v] & 0xffffu], s_xy[v]);
             wave_sync();
             if (chain_rec && logged + steps < n) {
                 // The working tour is the visit chain [start, s1, .., sk]
@@ -1066,7 +1067,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     placed += static_cast<uint32_t>(__builtin_popcountll(fm));
                 }
                 wave_sync();
-            } else {
+            } else if (n <= kRankedInRegisters) {
                 // Array position of every city = its distance from `head`
                 // along predecessor links: list ranking by pointer jumping,
                 // each lane holding (distance | link << 16) for cities
@@ -1119,6 +1120,39 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                         tour[d] = static_cast<uint16_t>(v);
                         pos[v] = static_cast<uint16_t>(d);
                     }
+                }
+                wave_sync();
+            } else {
+                // The same ranking for n above kRankedInRegisters: pointer
+                // jumping in place in LDS (every entry is one packed word, so
+                // the argument above holds), then the ranks are staged in the
+                // ant's tour row in HBM, which the final tour overwrites,
+                // before the array is written over the links.
+                for (uint32_t v = lane; v < n; v += kWave) {
+                    const uint32_t l = lnk[v];
+                    lnk[v] = v == head ? (head << 16) : (1u | ((l & 0xffffu) << 16));
+                }
+                wave_sync();
+                for (uint32_t round = 0; round < 17; ++round) {
+                    bool pending = false;
+                    for (uint32_t v = lane; v < n; v += kWave) {
+                        const uint32_t e = lnk[v];
+                        const uint32_t t = lnk[e >> 16];
+                        const uint32_t jumped = ((e & 0xffffu) + (t & 0xffffu)) | (t & 0xffff0000u);
+                        lnk[v] = jumped;
+                        pending |= (jumped >> 16) != head;
+                    }
+                    wave_sync();
+                    if (!ballot(pending)) break;
+                }
+                uint32_t* const stage = p.out_tours + static_cast<size_t>(ant) * ns;
+                for (uint32_t v = lane; v < n; v += kWave) stage[v] = lnk[v] & 0xffffu;
+                __threadfence_block();
+                wave_sync();
+                for (uint32_t v = lane; v < n; v += kWave) {
+                    const uint32_t d = stage[v];
+                    tour[d] = static_cast<uint16_t>(v);
+                    pos[v] = static_cast<uint16_t>(d);
                 }
                 wave_sync();
             }
