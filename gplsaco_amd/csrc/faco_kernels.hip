@@ -41,6 +41,14 @@ __device__ __forceinline__ double read_lane(double v, uint32_t lane) {
     return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// The value itself, opaque to the optimiser: a loop-invariant condition
+// written on it is evaluated where it is used (one scalar compare) instead
+// of being hoisted as a 64-bit lane mask that the register allocator then
+// spills and reloads inside the loop.
+__device__ __forceinline__ uint32_t fresh(uint32_t v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
 // LDS writes by one lane become visible to the other lanes of the same wave
 // in program order; this keeps the compiler from moving LDS accesses across
 // the hand-off points.
@@ -433,6 +441,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         return ok;
     };
     const bool in_row = lane < pw;
+    const uint64_t row_bits = pw >= kWave ? ~0ull : (1ull << pw) - 1ull;   // ballot(in_row)
     // Product-row loads are issued by every lane (lanes past the row re-read
     // its last slot), not under an exec mask: with no branch around them the
     // compiler counts the loads in flight exactly and waits only for the row
@@ -495,8 +504,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         }
         wave_sync();
 
-        const uint32_t target = explicit_mode ? p.targets[ant]
-                              : (Persistent ? s_state->target : p.state[colony].target);
+        const uint32_t target = uniform(explicit_mode ? p.targets[ant]
+                                      : (Persistent ? s_state->target : p.state[colony].target));
         const uint32_t* visits = p.visits + static_cast<size_t>(ant) * ns;
         uint32_t* checklist_log = diag_checklist ? diag_checklist + static_cast<size_t>(ant) * 5u * ns : nullptr;
         uint32_t current;
@@ -523,6 +532,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         // only if the two never met (logged + steps < n at the end).
         const bool chain_rec = linked && with_ppos && !explicit_mode && !per_ant_parent;
         if (chain_rec && lane == 0) queue[n - 1] = static_cast<uint16_t>(current);
+        const uint32_t record_limit = chain_rec ? n : 0u;   // record step s while logged + s < record_limit
         // array mode: position of `current`; linked mode: its links
         uint32_t pc = linked ? 0u : pos[current];
         uint32_t cur_link = linked ? uniform(lnk[current]) : 0u;
@@ -692,22 +702,28 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         // links (LDS).
         uint32_t cur_cand = n, cur_link_parent = 0;
         bool cur_free = false;
+        // ballot of cur_free, taken where the flags are read (as ballots of
+        // single compares: a ballot of a combined condition goes through a
+        // vector register)
+        uint64_t cur_mask = 0;
         const float margin32 = static_cast<float>(p.roulette_margin);
         if (!explicit_mode) {
             const uint32_t c = s_nbr[current * K + row_lane];
             const uint8_t fl = flags[c < n ? c : 0u];
             cur_cand = in_row ? c : n;
             cur_free = in_row && c < n && !(fl & kVisited);
+            cur_mask = ballot(c < n) & ballot(!(fl & kVisited)) & row_bits;
             miss_row = products[static_cast<size_t>(current) * pw + row_lane];
-            cur_link_parent = s_plinks[current];
+            cur_link_parent = uniform(s_plinks[current]);
         }
         // Roulette draws come in pools of 64 computed lane-parallel on the
-        // vector unit: lane l holds the uniform of step u_base + l (one
-        // Philox block per step, as step_uniform), so the serial scalar
-        // Philox of every step becomes one vector Philox per 64 steps.
+        // vector unit: a pool drawn at step s holds the uniform of step s + l
+        // in lane l (one Philox block per step, as step_uniform), so the
+        // serial scalar Philox of every step becomes one vector Philox per
+        // 64 steps.
         double u_pool = 0.0;
         float u_pool32 = 0.0f;
-        uint32_t u_base = 0;
+        uint32_t u_left = 0;   // uniforms of the pool not yet used
         while (mne < target && steps + 1 < n) {
             if constexpr (Profile) mark = clock64();
             const uint32_t step = steps + 1;
@@ -725,8 +741,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                 pf_node1 = link >> 16;
                 pf_row0 = products[static_cast<size_t>(pf_node0) * pw + row_lane];
                 pf_row1 = products[static_cast<size_t>(pf_node1) * pw + row_lane];
-                if (u_base == 0 || step - u_base >= kWave) {
-                    u_base = step;
+                if (u_left == 0) {
+                    u_left = kWave;
                     // The key passes through an empty asm so the compiler
                     // derives the Philox round keys here, every 64 steps,
                     // instead of keeping them live in scalar registers
@@ -736,7 +752,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     u_pool = step_uniform((static_cast<uint64_t>(k_hi) << 32) | k_lo, batch, local_ant, step + lane);
                     u_pool32 = static_cast<float>(u_pool);
                 }
-                u = read_lane(u_pool, step - u_base);
+                const uint32_t u_index = kWave - u_left;   // lane of this step's uniform
+                --u_left;
                 uint32_t cand = cur_cand;
                 uint32_t cur_after = 0;
                 if constexpr (Hard) cur_after = linked ? (cur_link >> 16) : tour[pc + 1 == n ? 0 : pc + 1];
@@ -772,7 +789,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                 } else {
                     avail = cur_free && (!Hard || relocation_ok(cand, current, cur_after));
                 }
-                const uint64_t amask = ballot(avail);
+                const uint64_t amask = (Hard || Escape) ? ballot(avail) : cur_mask;
                 if (amask) {
                     // Roulette of select_next: the first available slot j with
                     // u * total < prefix_j, where total and the prefixes are
@@ -790,64 +807,51 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     // so the selection equal the sequential FP64 ones.
                     // Otherwise (about one step in 10^4), or when the FP32
                     // total leaves [2^-100, 2^100] (underflowing or huge
-                    // weights), the sequential FP64 sums are formed.
-                    double acc = 0.0;
-                    double total = 0.0;
+                    // weights), the sequential FP64 sums decide (exact_lane).
+                    auto exact_lane = [&]() -> uint32_t {
+                        const double uu = read_lane(u_pool, u_index);
+                        double acc = 0.0, total = 0.0;
+                        if (pw <= 16) {
+                            acc = row_prefix_sum(avail ? weight : 0.0);
+                            total = read_lane(acc, 15);
+                        } else {
+                            for (uint64_t m = amask; m; m &= m - 1) {
+                                const uint32_t i = ctz64(m);
+                                const double v = read_lane(weight, i);
+                                acc = (i <= lane) ? acc + v : acc;
+                            }
+                            total = read_lane(acc, 63);
+                        }
+                        if (total > 0.0) {
+                            const double threshold = uu * total;
+                            const uint64_t hit = ballot(avail && threshold < acc);
+                            return hit ? ctz64(hit) : last64(amask);
+                        }
+                        const uint32_t count = static_cast<uint32_t>(__builtin_popcountll(amask));
+                        uint32_t k = static_cast<uint32_t>(uu * count);
+                        if (k >= count) k = count - 1;
+                        return nth_set_bit(amask, k);
+                    };
                     uint32_t sel_lane = kWave;
-                    if (pw <= 16) {
+                    if (fresh(pw) <= 16u) {
                         const float w32 = avail ? static_cast<float>(weight) : 0.0f;
                         const float fast = row_scan4_f32(w32);
                         const float fast_total = read_lane(fast, 15);
                         if (fast_total >= 0x1p-100f && fast_total <= 0x1p100f) {
-                            const float threshold = read_lane(u_pool32, step - u_base) * fast_total;
+                            const float threshold = read_lane(u_pool32, u_index) * fast_total;
                             const float diff = threshold - fast;
                             const float margin = fast_total * margin32;
-                            if (!ballot(avail && __builtin_fabsf(diff) <= margin)) {
-                                const uint64_t hit = ballot(avail && threshold < fast);
+                            if (!(ballot(__builtin_fabsf(diff) <= margin) & amask)) {
+                                const uint64_t hit = ballot(threshold < fast) & amask;
                                 sel_lane = hit ? ctz64(hit) : last64(amask);
                             }
                         }
-                        if (sel_lane == kWave) {
-                            acc = row_prefix_sum(avail ? weight : 0.0);
-                            total = read_lane(acc, 15);
-                        }
-                    } else {
-                        acc = 0.0;
-                        for (uint64_t m = amask; m; m &= m - 1) {
-                            const uint32_t i = ctz64(m);
-                            const double v = read_lane(weight, i);
-                            acc = (i <= lane) ? acc + v : acc;
-                        }
-                        total = read_lane(acc, 63);
                     }
-                    if constexpr (Escape) {
-                        // the chosen slot also tells whether the move may open new edges
-                        if (sel_lane == kWave) {
-                            if (total > 0.0) {
-                                const double threshold = u * total;
-                                const uint64_t hit = ballot(avail && threshold < acc);
-                                sel_lane = hit ? ctz64(hit) : last64(amask);
-                            } else {
-                                const uint32_t count = static_cast<uint32_t>(__builtin_popcountll(amask));
-                                uint32_t k = static_cast<uint32_t>(u * count);
-                                if (k >= count) k = count - 1;
-                                sel_lane = nth_set_bit(amask, k);
-                            }
-                        }
-                        selected = read_lane(cand, sel_lane);
-                        novel = read_lane(source, sel_lane) != kNoSource;
-                    } else if (sel_lane < kWave) {
-                        selected = read_lane(cand, sel_lane);
-                    } else if (total > 0.0) {
-                        const double threshold = u * total;
-                        const uint64_t hit = ballot(avail && threshold < acc);
-                        selected = read_lane(cand, hit ? ctz64(hit) : last64(amask));
-                    } else {
-                        const uint32_t count = static_cast<uint32_t>(__builtin_popcountll(amask));
-                        uint32_t k = static_cast<uint32_t>(u * count);
-                        if (k >= count) k = count - 1;
-                        selected = read_lane(cand, nth_set_bit(amask, k));
-                    }
+                    if (sel_lane == kWave) sel_lane = uniform(exact_lane());
+                    selected = read_lane(cand, sel_lane);
+                    // Escape mode: the chosen slot also tells whether the move
+                    // may open new edges
+                    if constexpr (Escape) novel = read_lane(source, sel_lane) != kNoSource;
                 } else {
                     selected = n;
                     for (uint32_t base = 0; base < p.bw && selected == n; base += kWave) {
@@ -855,7 +859,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                         const uint32_t b = j < p.bw ? backup[static_cast<size_t>(current) * p.bw + j] : n;
                         const uint64_t bm = ballot(b < n && !(flags[b] & kVisited) &&
                                                    (!Hard || reloc_ok(b, current, cur_after)));
-                        if (bm) selected = read_lane(b, ctz64(bm));
+                        if (bm) selected = uniform(read_lane(b, ctz64(bm)));
                     }
                     if (selected == n) {
                         // Nearest unvisited city, lowest id on ties.
@@ -868,7 +872,7 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                             }
                         }
                         best = wave_min_u64(best);
-                        selected = best == ~0ull ? n : static_cast<uint32_t>(best);
+                        selected = uniform(best == ~0ull ? n : static_cast<uint32_t>(best));
                     }
                 }
                 selected = uniform(selected);
@@ -883,65 +887,78 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                 if (selected < n) {
                     const uint32_t c = s_nbr[selected * K + row_lane];
                     const uint8_t fl = flags[c < n ? c : 0u];
-                    cur_cand = in_row ? c : n;
-                    cur_free = in_row && c < n && !(fl & kVisited);
-                    cur_link_parent = s_plinks[selected];
+                    const bool row_lane_now = lane < fresh(pw);   // = in_row
+                    cur_cand = row_lane_now ? c : n;
+                    cur_free = row_lane_now && c < n && !(fl & kVisited);
+                    cur_mask = ballot(c < n) & ballot(!(fl & kVisited)) & row_bits;
+                    cur_link_parent = uniform(s_plinks[selected]);
                     row_src = selected == pf_node0 ? 0u : (selected == pf_node1 ? 1u : 2u);
                     if (row_src == 2u) miss_row = products[static_cast<size_t>(selected) * pw + row_lane];
                 }
             }
-            if (trace_selected && lane == 0) {
-                trace_selected[static_cast<size_t>(ant) * ns + step] = selected;
-                trace_uniform[static_cast<size_t>(ant) * ns + step] = u;
+            if (trace_selected) {
+                if (!explicit_mode) u = read_lane(u_pool, kWave - 1u - u_left);
+                if (lane == 0) {
+                    trace_selected[static_cast<size_t>(ant) * ns + step] = selected;
+                    trace_uniform[static_cast<size_t>(ant) * ns + step] = u;
+                }
             }
             if constexpr (Profile) { const uint64_t t = clock64(); sub[0] += t - mark; mark = t; }
             if (selected >= n) { exhausted = 1; break; }
 
             ++steps;
-            if (chain_rec && lane == 0 && logged + steps < n) queue[n - 1 - steps] = static_cast<uint16_t>(selected);
             uint32_t ps = 0, old_prev, after, target_after;
-            if (linked) {
-                const uint32_t sel_link = lnk[selected];
+            const bool linked_now = fresh(n) <= kMaxLinked;   // = linked
+            if (linked_now) {
+                const uint32_t sel_link = uniform(lnk[selected]);
                 old_prev = sel_link & 0xffffu;
                 after = sel_link >> 16;
                 target_after = cur_link >> 16;
             } else {
-                ps = pos[selected];
-                old_prev = tour[ps == 0 ? n - 1 : ps - 1];
-                after = tour[ps + 1 == n ? 0 : ps + 1];
-                target_after = tour[pc + 1 == n ? 0 : pc + 1];
+                ps = uniform(pos[selected]);
+                old_prev = uniform(tour[ps == 0 ? n - 1 : ps - 1]);
+                after = uniform(tour[ps + 1 == n ? 0 : ps + 1]);
+                target_after = uniform(tour[pc + 1 == n ? 0 : pc + 1]);
             }
-            if (lane == 0) flags[selected] = static_cast<uint8_t>(flags[selected] | kVisited);
             uint32_t next_pc = ps;
-            cur_link = current | (after << 16);   // selected's links when the move is the identity
-            if (target_after != selected) {
-                ++relocations;
+            const bool moved = target_after != selected;
+            const bool record = logged + steps < fresh(record_limit);
+            // Lane 0's LDS writes of the step, in one region: the visit
+            // record, the visited flag, Escape anchors and, in linked mode,
+            // the relinking: unlink `selected` and relink it between
+            // `current` and its successor.
+            if (lane == 0) {
+                if (record) queue[fresh(n) - 1u - steps] = static_cast<uint16_t>(selected);
+                flags[selected] = static_cast<uint8_t>(flags[selected] | kVisited);
                 if constexpr (Escape) {
                     // a move that may open new edges marks its five endpoints
                     // as anchors (cuda/faco_device.cuh:213-218)
-                    if (novel) {
-                        if (lane == 0) {
-                            flags[current] = static_cast<uint8_t>(flags[current] | kAnchor);
-                            flags[target_after] = static_cast<uint8_t>(flags[target_after] | kAnchor);
-                            flags[selected] = static_cast<uint8_t>(flags[selected] | kAnchor);
-                            flags[old_prev] = static_cast<uint8_t>(flags[old_prev] | kAnchor);
-                            flags[after] = static_cast<uint8_t>(flags[after] | kAnchor);
-                        }
-                        ++e_relocations;
+                    if (moved && novel) {
+                        flags[current] = static_cast<uint8_t>(flags[current] | kAnchor);
+                        flags[target_after] = static_cast<uint8_t>(flags[target_after] | kAnchor);
+                        flags[selected] = static_cast<uint8_t>(flags[selected] | kAnchor);
+                        flags[old_prev] = static_cast<uint8_t>(flags[old_prev] | kAnchor);
+                        flags[after] = static_cast<uint8_t>(flags[after] | kAnchor);
                     }
                 }
-                if (linked) {
-                    // Unlink `selected`, relink it between `current` and its
-                    // successor; the array's first city moves on when it is
-                    // the one relocated (the array shift starts at position 0).
-                    if (lane == 0) {
-                        uint16_t* half = reinterpret_cast<uint16_t*>(lnk);
-                        half[2 * old_prev + 1] = static_cast<uint16_t>(after);
-                        half[2 * after] = static_cast<uint16_t>(old_prev);
-                        half[2 * current + 1] = static_cast<uint16_t>(selected);
-                        lnk[selected] = current | (target_after << 16);
-                        half[2 * target_after] = static_cast<uint16_t>(selected);
-                    }
+                if (linked_now && moved) {
+                    uint16_t* half = reinterpret_cast<uint16_t*>(lnk);
+                    half[2 * old_prev + 1] = static_cast<uint16_t>(after);
+                    half[2 * after] = static_cast<uint16_t>(old_prev);
+                    half[2 * current + 1] = static_cast<uint16_t>(selected);
+                    lnk[selected] = current | (target_after << 16);
+                    half[2 * target_after] = static_cast<uint16_t>(selected);
+                }
+            }
+            cur_link = current | (after << 16);   // selected's links when the move is the identity
+            if (moved) {
+                ++relocations;
+                if constexpr (Escape) {
+                    if (novel) ++e_relocations;
+                }
+                if (linked_now) {
+                    // the array's first city moves on when it is the one
+                    // relocated (the array shift starts at position 0)
                     if (selected == head) head = after;
                     cur_link = current | (target_after << 16);
                 }
