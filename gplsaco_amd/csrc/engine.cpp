@@ -164,6 +164,8 @@ struct Engine {
     bool hard = false;                     // constraint mode Hard: every tour stays in its instance's graph
     uint32_t graph_stride = 0;             // neighbour capacity per slot: 2 n (primary_width + 1)
     DeviceArray<uint32_t> graph_offsets, graph_neighbors;   // [G][n + 1], [G][graph_stride]
+    uint32_t graph_words = 0;              // 32-bit words per adjacency row: ceil(n / 32)
+    DeviceArray<uint32_t> graph_bits;      // [G][n][graph_words] adjacency bits of the same graph
     // Escape mode (on the Hard graph): escape gates, slot views, per-ant
     // edge caches and anchor LS views; an evaluation may disable the gates
     // (FACO_RUN_NO_ESCAPE), which must reproduce the Hard run
@@ -418,6 +420,14 @@ struct Engine {
                 graph_offsets.upload(p.graph_offsets.data(), m + 1, stream, static_cast<size_t>(g) * (n + 1));
                 graph_neighbors.upload(p.graph_neighbors.data(), p.graph_neighbors.size(), stream,
                                        static_cast<size_t>(g) * graph_stride);
+                std::vector<uint32_t> bits(static_cast<size_t>(n) * graph_words, 0u);
+                for (uint32_t a = 0; a < m; ++a)
+                    for (uint32_t k = p.graph_offsets[a]; k < p.graph_offsets[a + 1]; ++k) {
+                        const uint32_t b = p.graph_neighbors[k];
+                        bits[static_cast<size_t>(a) * graph_words + (b >> 5)] |= 1u << (b & 31u);
+                    }
+                graph_bits.upload(bits.data(), bits.size(), stream, static_cast<size_t>(g) * n * graph_words);
+                checked(hipStreamSynchronize(stream), "graph upload");
             }
             if (ctl) {
                 const auto samples = sample_nodes(m, slots[g]);
@@ -462,6 +472,7 @@ struct Engine {
         if (hard) {
             p.graph_offsets = graph_offsets.data(); p.graph_neighbors = graph_neighbors.data();
             p.graph_stride = graph_stride;
+            p.graph_bits = graph_bits.data(); p.graph_words = graph_words;
         }
         if (escape) {
             p.escape_disabled = escape_disabled ? 1u : 0u;
@@ -580,7 +591,11 @@ struct Engine {
     // kernel: up to kRoundsPerLaunch batches per launch, each colony's
     // workgroup doing its own colony update between batches.
     static constexpr uint64_t kRoundsPerLaunch = 1024;
-    static bool persistent(const AntParams& ap) { return ap.blocks_per_colony == 1 && !ap.graph_offsets; }
+    // (Hard mode too, with plain 2-opt; Escape mode runs per batch: its
+    // footprint update is compiled into the per-batch colony update only)
+    static bool persistent(const AntParams& ap) {
+        return ap.blocks_per_colony == 1 && !ap.esc_rows && !(ap.graph_offsets && ap.or_opt);
+    }
     // Run `iterations` batches: persistent launches when the shape allows,
     // else per batch an ant launch and a colony-update launch, through a
     // cached HIP graph of 16 batches when use_graph is set.
@@ -1846,7 +1861,9 @@ int faco_engine_create(uint32_t dimension, uint32_t colonies, const faco_setting
             e.graph_stride = 2u * e.n * (e.config.primary_width + 1u);
             e.graph_offsets.allocate(static_cast<size_t>(e.panel) * (e.n + 1));
             e.graph_neighbors.allocate(static_cast<size_t>(e.panel) * e.graph_stride);
-            e.allocated += e.graph_offsets.bytes() + e.graph_neighbors.bytes();
+            e.graph_words = (e.n + 31u) / 32u;
+            e.graph_bits.allocate(static_cast<size_t>(e.panel) * e.n * e.graph_words);
+            e.allocated += e.graph_offsets.bytes() + e.graph_neighbors.bytes() + e.graph_bits.bytes();
         }
         *out = engine.release();
     });

@@ -282,6 +282,10 @@ struct AntParams {
     const uint32_t* graph_offsets;
     const uint32_t* graph_neighbors;
     uint32_t graph_stride;
+    // the same graph as adjacency bits, [G][n][graph_words] (bit b of row a:
+    // b is in a's neighbour list), so that a membership test is one load
+    const uint32_t* graph_bits;
+    uint32_t graph_words;
     // Escape mode (on a Hard graph; null esc_rows: off): per ant its LS
     // rows [A][n][lw] (an anchor's escape view, city | replaced << 15; other
     // rows are not written), footprint [A][n] and counters [A][12] (the

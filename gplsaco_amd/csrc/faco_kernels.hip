@@ -414,17 +414,13 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     // the instance's undirected CSR graph; a move is allowed when every
     // added edge that does not cancel a removed one (as a multiset) is a
     // graph edge
-    const uint32_t* g_off = Hard ? p.graph_offsets + static_cast<size_t>(geom) * (ns + 1) : nullptr;
-    const uint32_t* g_adj = Hard ? p.graph_neighbors + static_cast<size_t>(geom) * p.graph_stride : nullptr;
+    // (held as adjacency bits: b is a neighbour of a when bit b of row a is
+    // set, the rows built from the sorted CSR lists)
+    const uint32_t graph_words = p.graph_words;
+    const uint32_t* g_bits = Hard ? p.graph_bits + static_cast<size_t>(geom) * ns * graph_words : nullptr;
     auto in_graph = [&](uint32_t a, uint32_t b) -> bool {
         if (a >= n || b >= n || a == b) return false;
-        uint32_t lo = g_off[a], hi = g_off[a + 1];
-        const uint32_t end = hi;
-        while (lo < hi) {
-            const uint32_t mid = lo + (hi - lo) / 2;
-            if (g_adj[mid] < b) lo = mid + 1; else hi = mid;
-        }
-        return lo < end && g_adj[lo] == b;
+        return (g_bits[a * graph_words + (b >> 5)] >> (b & 31u)) & 1u;
     };
     auto edges_allowed = [&](uint32_t count, const uint32_t (&removed)[3][2], const uint32_t (&added)[3][2]) -> bool {
         bool consumed[3] = {false, false, false};
@@ -1665,10 +1661,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search_escape(const AntParams p
     ant_search<NbrLds, false, false, false, false, true, false, true>(p, ColonyParams{}, 1u, nullptr);
 }
 
-template<bool NbrLds, bool OrOpt>
+template<bool NbrLds, bool OrOpt, bool Hard>
 __global__ void __launch_bounds__(1024) faco_colony_run(const AntParams p, const ColonyParams cp, uint32_t rounds) {
     __shared__ ColonyState s_state;
-    ant_search<NbrLds, false, false, OrOpt, true>(p, cp, rounds, &s_state);
+    ant_search<NbrLds, false, false, OrOpt, true, Hard>(p, cp, rounds, &s_state);
 }
 
 // GP-controlled FACO over `rounds` batches in one launch: one workgroup per
@@ -1819,15 +1815,20 @@ hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t st
 hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_t colonies, uint32_t rounds,
                              hipStream_t stream) {
     if (p.blocks_per_colony != 1 || p.phase_ticks || p.explicit_mode || p.per_ant_parent || p.trace_selected ||
-        p.graph_offsets ||
+        p.esc_rows || (p.graph_offsets && p.or_opt) ||
         p.diag_construction || p.diag_positions || p.diag_checklist || p.control || p.out_identity)
-        return hipErrorInvalidValue;   // one workgroup per colony, production plain FACO only
+        return hipErrorInvalidValue;   // one workgroup per colony, production plain or Hard FACO only
     const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     const dim3 grid(colonies), block(64 * p.waves);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p, cp, rounds); };
     const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
-    if (nl) oo ? go(faco_colony_run<true, true>) : go(faco_colony_run<true, false>);
-    else oo ? go(faco_colony_run<false, true>) : go(faco_colony_run<false, false>);
+    if (p.graph_offsets) {   // Hard mode: plain 2-opt
+        nl ? go(faco_colony_run<true, false, true>) : go(faco_colony_run<false, false, true>);
+    } else if (nl) {
+        oo ? go(faco_colony_run<true, true, false>) : go(faco_colony_run<true, false, false>);
+    } else {
+        oo ? go(faco_colony_run<false, true, false>) : go(faco_colony_run<false, false, false>);
+    }
     return hipGetLastError();
 }
 

@@ -7,7 +7,7 @@ It must equal, bit for bit, the per-batch launches of the same engine
 (tours, costs, work counters, every control summary) and the oracle's
 controlled run, for programs, conditional controllers, rule baselines,
 factorial masks, Or-opt, neighbour rows in LDS or in global memory, the
-linked-list (n <= 1024) and the array construction, mixed-size panels and
+linked-list (n <= 2048) and the array construction, mixed-size panels and
 runs split by checkpoints. `persistent_launches` shows which path ran.
 """
 import math
@@ -63,7 +63,7 @@ def _both(e, keys, seeds, limit, mode, **kw):
 
 
 @pytest.mark.parametrize("n,ants,batches,waves", [(200, 32, 30, 8), (90, 16, 40, 2), (1100, 64, 5, 12),
-                                                  (300, 32, 12, 6 | 0x100), (400, 64, 10, 16)])
+                                                  (300, 32, 12, 6 | 0x100), (400, 64, 10, 16), (2100, 32, 3, 4 | 0x100)])
 def test_program_population(ext, oracle_mod, n, ants, batches, waves):
     O = oracle_mod
     count = 2

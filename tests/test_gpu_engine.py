@@ -67,7 +67,7 @@ def test_device_philox_matches_oracle(ext, oracle_mod):
 CAPS = [0, 1, 7, 127, U64]
 
 
-@pytest.mark.parametrize("n", [3, 4, 5, 7, 8, 17, 100, 500, 1000, 1024, 1025, 1500])
+@pytest.mark.parametrize("n", [3, 4, 5, 7, 8, 17, 100, 500, 1000, 1024, 1025, 1500, 2048, 2049])
 @pytest.mark.parametrize("metric", ["uniform", "grid", "repeated"])
 @pytest.mark.parametrize("ls_mode", ["2opt", "2opt+oropt"])
 def test_explicit_construct_ls_matches_oracle(ext, oracle_mod, n, metric, ls_mode):

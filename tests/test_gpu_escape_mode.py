@@ -6,7 +6,7 @@ complete graph on a degenerate instance, and the initial tour plus each
 city's four nearest): tours, costs, work, rejections and the eleven escape
 counters bit for bit, plain and controlled (a rule baseline that restarts
 every batch, so footprints travel through the archive), on the linked-list
-(n <= 1024) and the array (n > 1024) construction. With the escape gates
+(n <= 2048) and the array (n > 2048) construction. With the escape gates
 disabled the Escape engine reproduces the Hard engine (the reference's
 same(hard, disabled)).
 """
@@ -38,7 +38,8 @@ def _cycle(tour):
     return {tuple(sorted((int(tour[i]), int(tour[(i + 1) % len(tour)])))) for i in range(len(tour))}
 
 
-@pytest.mark.parametrize("n,full,batches", [(31, True, 16), (31, False, 16), (200, False, 8), (1100, False, 3)])
+@pytest.mark.parametrize("n,full,batches", [(31, True, 16), (31, False, 16), (200, False, 8), (1100, False, 3),
+                                             (2100, False, 2)])
 def test_escape_runs_match_the_oracle(ext, oracle_mod, n, full, batches):
     O = oracle_mod
     ants = 4 if n <= 31 else 16

@@ -1,8 +1,9 @@
 """Hard mode on the device (constraint_mode "hard", register_graph_problem)
 against the oracle's Hard runs (oracle.run_faco(..., graphs=...), itself
 pinned in tests/test_oracle_hard.py): the same tours, costs and work counters
-bit for bit, every tour inside its graph, on the linked-list (n <= 1024) and
-the array (n > 1024) construction paths. The graph is the reference's usual
+bit for bit, every tour inside its graph, on the linked-list (n <= 2048) and
+the array (n > 2048) construction paths, per batch and in the persistent
+kernel (one workgroup per colony running its batches and colony updates). The graph is the reference's usual
 candidate graph: the primary and LS rows plus the initial tour
 (SparseUndirectedGraph::from_candidates, cpp/src/sparse_graph.cpp:27-40),
 optionally with a few random extra edges. Controlled runs (a GP program,
@@ -35,8 +36,9 @@ def _spec(data, extra_edges):
             "backup": data["backup"].tolist(), "ls": data["ls"].tolist()}, edges
 
 
-@pytest.mark.parametrize("n,extra", [(200, 0), (300, 150), (1100, 0)])
-def test_hard_mode_matches_the_oracle_and_stays_in_the_graph(ext, oracle_mod, n, extra):
+@pytest.mark.parametrize("n,extra,persistent", [(200, 0, False), (300, 150, False), (1100, 0, False), (2100, 0, False),
+                                                (300, 150, True), (1100, 0, True), (2100, 0, True)])
+def test_hard_mode_matches_the_oracle_and_stays_in_the_graph(ext, oracle_mod, n, extra, persistent):
     O = oracle_mod
     count, batches = 2, 5
     xys = [(clustered_instance if i % 2 else uniform_instance)(n, seed=41, instance=i) for i in range(count)]
@@ -56,8 +58,13 @@ def test_hard_mode_matches_the_oracle_and_stays_in_the_graph(ext, oracle_mod, n,
         members.append(edges)
     task_keys = np.array([keys[c % count] for c in range(4)], dtype=np.uint64)
     seeds = np.array([1, 1, 2, 2], dtype=np.uint64)
-    announce("hard", n, extra)
+    if persistent:
+        # one workgroup per colony: the persistent kernel runs every batch
+        hard.set_waves_per_block(hard.launch_shape(4)["waves_per_block"], 1)
+        assert hard.launch_shape(4)["blocks_per_colony"] == 1
+    announce("hard", n, extra, persistent)
     got = hard.evaluate_faco_evaluations(task_keys, seeds, s.ants * batches)
+    assert (got["persistent_launches"] > 0) == persistent, got["persistent_launches"]
     ref = plain.evaluate_faco_evaluations(task_keys, seeds, s.ants * batches)
     settings = O.default_settings(**{f: getattr(s, f) for f in (
         "ants", "primary_width", "backup_width", "ls_width", "beta", "retention", "p_best",
