@@ -2268,7 +2268,7 @@ int faco_engine_apply_edge_prior(faco_engine* engine, faco_prior* prior, uint64_
         DeviceArray<float> f, s;
         f.allocate(cells * 8); s.allocate(cells);
         f.upload(features.data(), f.size(), e.stream);
-        checked(launch_prior_infer(prior_device_params(prior), f.data(), static_cast<uint32_t>(cells), s.data(),
+        checked(launch_prior_infer(prior_device_image(prior), f.data(), static_cast<uint32_t>(cells), s.data(),
                                    e.stream), "prior inference launch");
         s.download(logits.data(), cells, e.stream);
         checked(hipStreamSynchronize(e.stream), "prior inference");

@@ -425,14 +425,20 @@ uint32_t prior_grad_rows(uint32_t edges);
 hipError_t launch_prior_features(uint32_t n, uint32_t pw, const uint32_t* coords, const uint32_t* primary,
                                  const double* local_scale, double epsilon, float mean_scale, float* out,
                                  hipStream_t stream);
-hipError_t launch_prior_infer(const float* params, const float* features, uint32_t edges, float* logits,
+// The kernels read the weights from an image in their own LDS layout
+// (prior_image_bytes bytes), built from the FP32 parameters by
+// launch_prior_image and kept current by every Adam step of
+// launch_prior_reduce.
+uint32_t prior_image_bytes();
+hipError_t launch_prior_image(const float* params, void* image, hipStream_t stream);
+hipError_t launch_prior_infer(const void* image, const float* features, uint32_t edges, float* logits,
                               hipStream_t stream);
-hipError_t launch_prior_grad(const float* params, const float* features, const uint8_t* labels,
+hipError_t launch_prior_grad(const void* image, const float* features, const uint8_t* labels,
                              const uint32_t* indices, uint32_t offset, uint32_t edges, float pos_weight,
                              float inv_total, float* partial, hipStream_t stream);
 hipError_t launch_prior_reduce(const float* partial, uint32_t rows, float* grad, float* params, float* m, float* v,
                                bool adam, float lr, float beta1, float beta2, float eps, uint32_t step,
-                               float* loss_sum, hipStream_t stream);
+                               float* loss_sum, void* image, hipStream_t stream);
 hipError_t launch_prior_mma_check(const float* A, const float* B, float* C, hipStream_t stream);
 
 // Launch wrappers (defined in faco_kernels.hip).

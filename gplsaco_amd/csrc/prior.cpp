@@ -10,6 +10,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace faco {
@@ -45,6 +46,24 @@ template<class T> struct Buffer {
     }
 };
 
+// Pinned host memory (asynchronous uploads read it while the host works on).
+template<class T> struct PinnedBuffer {
+    T* ptr = nullptr;
+    explicit PinnedBuffer(size_t n) {
+        if (n) checked(hipHostMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    }
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() { if (ptr) (void)hipHostFree(ptr); }
+};
+
+// A worker thread joined on every exit path.
+struct Worker {
+    std::thread thread;
+    void join() { if (thread.joinable()) thread.join(); }
+    ~Worker() { join(); }
+};
+
 // SplitMix64 stream for initial weights and epoch shuffles.
 struct Mix {
     uint64_t state;
@@ -56,10 +75,11 @@ struct Mix {
 
 struct PriorImpl {
     Buffer<float> params, m, v, grad, loss;
+    Buffer<uint8_t> image;   // the weights in the kernels' layout, rebuilt with every parameter change
     uint32_t step = 0;
     hipStream_t stream = nullptr;
     PriorImpl() : params(prior_param_count()), m(prior_param_count()), v(prior_param_count()),
-                  grad(prior_param_count() + 1), loss(1) {
+                  grad(prior_param_count() + 1), loss(1), image(prior_image_bytes()) {
         checked(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     }
     ~PriorImpl() { if (stream) (void)hipStreamDestroy(stream); }
@@ -69,12 +89,13 @@ struct PriorImpl {
         checked(hipMemcpyAsync(params.ptr, host, P * sizeof(float), hipMemcpyHostToDevice, stream), "upload");
         checked(hipMemsetAsync(m.ptr, 0, P * sizeof(float), stream), "memset");
         checked(hipMemsetAsync(v.ptr, 0, P * sizeof(float), stream), "memset");
+        checked(launch_prior_image(params.ptr, image.ptr, stream), "weight image launch");
         checked(hipStreamSynchronize(stream), "prior set");
         step = 0;
     }
 };
 
-const float* prior_device_params(const faco_prior* prior) { return reinterpret_cast<const PriorImpl*>(prior)->params.ptr; }
+const void* prior_device_image(const faco_prior* prior) { return reinterpret_cast<const PriorImpl*>(prior)->image.ptr; }
 
 // Scratch of faco_prior_gradients_noalloc: one minibatch of at most
 // FACO_PRIOR_NOALLOC_EDGES edges (the training loop's batch size). Layout in
@@ -171,7 +192,7 @@ int faco_prior_logits(faco_prior* prior, const float* features, uint32_t edges, 
         auto* p = reinterpret_cast<PriorImpl*>(prior);
         faco::Buffer<float> f(static_cast<size_t>(edges) * 8), s(edges);
         faco::checked(hipMemcpyAsync(f.ptr, features, f.count * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
-        faco::checked(faco::launch_prior_infer(p->params.ptr, f.ptr, edges, s.ptr, p->stream), "prior inference launch");
+        faco::checked(faco::launch_prior_infer(p->image.ptr, f.ptr, edges, s.ptr, p->stream), "prior inference launch");
         faco::checked(hipMemcpyAsync(logits, s.ptr, edges * sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");
         faco::checked(hipStreamSynchronize(p->stream), "prior inference");
     });
@@ -189,10 +210,10 @@ int faco_prior_gradients(faco_prior* prior, const float* features, const uint8_t
         faco::Buffer<float> partial(static_cast<size_t>(rows) * (faco::prior_param_count() + 1));
         faco::checked(hipMemcpyAsync(f.ptr, features, f.count * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
         faco::checked(hipMemcpyAsync(y.ptr, labels, edges, hipMemcpyHostToDevice, p->stream), "upload");
-        faco::checked(faco::launch_prior_grad(p->params.ptr, f.ptr, y.ptr, nullptr, 0, edges, pos_weight,
+        faco::checked(faco::launch_prior_grad(p->image.ptr, f.ptr, y.ptr, nullptr, 0, edges, pos_weight,
                                               1.0f / static_cast<float>(edges), partial.ptr, p->stream), "gradient launch");
         faco::checked(faco::launch_prior_reduce(partial.ptr, rows, p->grad.ptr, p->params.ptr, p->m.ptr, p->v.ptr, false,
-                                                0, 0, 0, 0, 1, nullptr, p->stream), "reduce launch");
+                                                0, 0, 0, 0, 1, nullptr, p->image.ptr, p->stream), "reduce launch");
         std::vector<float> g(faco::prior_param_count() + 1);
         faco::checked(hipMemcpyAsync(g.data(), p->grad.ptr, g.size() * sizeof(float), hipMemcpyDeviceToHost, p->stream),
                       "download");
@@ -222,10 +243,10 @@ int faco_prior_gradients_noalloc(faco_prior* prior, const float* features, const
         faco::checked(hipMemcpyAsync(staged, features, static_cast<size_t>(edges) * 8 * sizeof(float),
                                      hipMemcpyHostToDevice, p->stream), "upload");
         faco::checked(hipMemcpyAsync(y, labels, edges, hipMemcpyHostToDevice, p->stream), "upload");
-        faco::checked(faco::launch_prior_grad(p->params.ptr, staged, y, nullptr, 0, edges, pos_weight,
+        faco::checked(faco::launch_prior_grad(p->image.ptr, staged, y, nullptr, 0, edges, pos_weight,
                                               1.0f / static_cast<float>(edges), partial, p->stream), "gradient launch");
         faco::checked(faco::launch_prior_reduce(partial, rows, grad, p->params.ptr, p->m.ptr, p->v.ptr, false,
-                                                0, 0, 0, 0, 1, nullptr, p->stream), "reduce launch");
+                                                0, 0, 0, 0, 1, nullptr, p->image.ptr, p->stream), "reduce launch");
         faco::checked(hipMemcpyAsync(grads, grad, P * sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");
         if (loss) faco::checked(hipMemcpyAsync(loss, grad + P, sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");
         faco::checked(hipStreamSynchronize(p->stream), "prior gradients");
@@ -243,32 +264,43 @@ int faco_prior_train(faco_prior* prior, const float* features, const uint8_t* la
         const uint32_t P = faco::prior_param_count();
         faco::Buffer<float> f(static_cast<size_t>(edges) * 8);
         faco::Buffer<uint8_t> y(edges);
-        faco::Buffer<uint32_t> order(edges);
         const uint32_t rows = faco::prior_grad_rows(batch < edges ? batch : edges);
         faco::Buffer<float> partial(static_cast<size_t>(rows) * (P + 1));
         faco::checked(hipMemcpyAsync(f.ptr, features, f.count * sizeof(float), hipMemcpyHostToDevice, p->stream), "upload");
         faco::checked(hipMemcpyAsync(y.ptr, labels, edges, hipMemcpyHostToDevice, p->stream), "upload");
-        std::vector<uint32_t> perm(edges);
-        for (uint32_t e = 0; e < epochs; ++e) {
-            // Fisher-Yates shuffle from a SplitMix64 stream of (seed, epoch)
+        // Fisher-Yates shuffle of epoch e from a SplitMix64 stream of (seed,
+        // e). The host draws epoch e + 1's order (pinned buffer, own thread)
+        // while the device runs epoch e from the other device buffer.
+        auto shuffle = [edges, shuffle_seed](uint32_t e, uint32_t* perm) {
             for (uint32_t i = 0; i < edges; ++i) perm[i] = i;
             faco::Mix mix{faco::mixed(shuffle_seed ^ faco::mixed(e + 0x2545f4914f6cdd1dULL))};
             for (uint32_t i = edges - 1; i > 0; --i) {
                 const uint32_t j = static_cast<uint32_t>(mix.next() % (static_cast<uint64_t>(i) + 1));
                 std::swap(perm[i], perm[j]);
             }
-            faco::checked(hipMemcpyAsync(order.ptr, perm.data(), edges * sizeof(uint32_t), hipMemcpyHostToDevice,
+        };
+        faco::Buffer<uint32_t> orders[2] = {faco::Buffer<uint32_t>(edges), faco::Buffer<uint32_t>(edges)};
+        faco::PinnedBuffer<uint32_t> perms[2] = {faco::PinnedBuffer<uint32_t>(edges), faco::PinnedBuffer<uint32_t>(edges)};
+        shuffle(0, perms[0].ptr);
+        faco::Worker next;
+        for (uint32_t e = 0; e < epochs; ++e) {
+            next.join();
+            faco::Buffer<uint32_t>& order = orders[e & 1u];
+            faco::checked(hipMemcpyAsync(order.ptr, perms[e & 1u].ptr, edges * sizeof(uint32_t), hipMemcpyHostToDevice,
                                          p->stream), "upload");
+            // the pinned buffer of epoch e - 1 is free once the device has
+            // taken epoch e - 1's order (the stream synchronised after it)
+            if (e + 1 < epochs) next.thread = std::thread(shuffle, e + 1, perms[(e + 1) & 1u].ptr);
             faco::checked(hipMemsetAsync(p->loss.ptr, 0, sizeof(float), p->stream), "memset");
             uint32_t batches = 0;
             for (uint32_t offset = 0; offset < edges; offset += batch, ++batches) {
                 const uint32_t count = edges - offset < batch ? edges - offset : batch;
-                faco::checked(faco::launch_prior_grad(p->params.ptr, f.ptr, y.ptr, order.ptr, offset, count, pos_weight,
+                faco::checked(faco::launch_prior_grad(p->image.ptr, f.ptr, y.ptr, order.ptr, offset, count, pos_weight,
                                                       1.0f / static_cast<float>(count), partial.ptr, p->stream),
                               "gradient launch");
                 faco::checked(faco::launch_prior_reduce(partial.ptr, faco::prior_grad_rows(count), p->grad.ptr,
                                                         p->params.ptr, p->m.ptr, p->v.ptr, true, lr, 0.9f, 0.999f, 1e-8f,
-                                                        ++p->step, p->loss.ptr, p->stream), "adam launch");
+                                                        ++p->step, p->loss.ptr, p->image.ptr, p->stream), "adam launch");
             }
             float total = 0;
             faco::checked(hipMemcpyAsync(&total, p->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, p->stream), "download");

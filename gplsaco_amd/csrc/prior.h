@@ -5,6 +5,7 @@
 #include "../../include/faco_engine.h"
 
 namespace faco {
-// Device pointer of the FP32 master parameters of a faco_prior.
-const float* prior_device_params(const faco_prior* prior);
+// Device pointer of the weight image the prior's kernels read (current
+// after every call that changed the parameters has returned).
+const void* prior_device_image(const faco_prior* prior);
 }  // namespace faco

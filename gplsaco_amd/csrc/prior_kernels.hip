@@ -6,10 +6,13 @@
 //
 // Every product runs on v_mfma_f32_16x16x32_bf16 (one wave: a 16x16 tile
 // with K = 32; lane l holds A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16] and
-// D[4(l/16)+r][l%16]). Operands are staged in LDS as bf16; a wave works on
-// tiles of 32 edges. bf16 rounding points: edge features, hidden
-// activations, output-layer inputs, the two backward deltas, and all
-// weights; everything else is FP32.
+// D[4(l/16)+r][l%16]). Operands are staged in LDS as bf16 with K contiguous
+// for both operands (A row-major, B column-major), so each lane reads each
+// operand with one 16-byte LDS load; a wave works on tiles of 32 edges. The
+// weights reach a workgroup as one image in that layout (kept current by the
+// Adam step), copied with 16-byte loads. bf16 rounding points: edge
+// features, hidden activations, output-layer inputs, the two backward
+// deltas, and all weights; everything else is FP32.
 #include "faco_common.h"
 
 namespace faco {
@@ -29,15 +32,13 @@ __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// C (16x16 fragment) += A[16][32] (row stride lda) x B[32][16] (row stride ldb).
-__device__ __forceinline__ f32x4 mma(const __bf16* A, int lda, const __bf16* B, int ldb, f32x4 c, int lane) {
+// C (16x16 fragment) += A[16][32] (row-major, row stride lda) x B[32][16]
+// given column-major as Bc[16][32] (column stride ldb): Bc[c][k] = B[k][c].
+// Strides and bases are multiples of 8 elements (16-byte aligned loads).
+__device__ __forceinline__ f32x4 mma(const __bf16* A, int lda, const __bf16* Bc, int ldb, f32x4 c, int lane) {
     const int i = lane & 15, kb = (lane >> 4) * 8;
-    bf16x8 a, b;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        a[t] = A[i * lda + kb + t];
-        b[t] = B[(kb + t) * ldb + i];
-    }
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(A + i * lda + kb);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bc + i * ldb + kb);
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
@@ -53,20 +54,27 @@ __device__ __forceinline__ float gelu_grad(float z) {
 __device__ __forceinline__ float sigmoid(float s) { return 1.0f / (1.0f + expf(-s)); }
 __device__ __forceinline__ float softplus(float s) { return s > 0.0f ? s + log1pf(expf(-s)) : log1pf(expf(s)); }
 
-// Block-shared weights in the layouts the products read.
-struct WeightsLds {
-    __bf16 w1t[32 * 64];   // [k][j] = W1[j][k], rows k >= 8 zero
-    __bf16 w2t[64 * 64];   // [k][j] = W2[j][k]
+}  // namespace
+
+// The weights in the layouts the products read (B operands column-major):
+// the block-shared LDS copy and, byte for byte, the image in device memory.
+struct alignas(16) WeightsLds {
+    __bf16 w1n[64 * 32];   // [j][k] = W1[j][k], k >= 8 zero
     __bf16 w2n[64 * 64];   // [j][k] = W2[j][k]
-    __bf16 w3t[64 * 16];   // [k][0] = w3[k], other columns zero
-    float b1[64], b2[64], w3[64], b3;
+    __bf16 w2t[64 * 64];   // [k][j] = W2[j][k]
+    __bf16 w3n[16 * 64];   // [0][k] = w3[k], rows 1..15 zero
+    float b1[64], b2[64], w3[64], b3;   // w3 rounded through bf16
 };
 
-// Per-wave tile buffers.
+namespace {
+
+// Per-wave tile buffers (B operands column-major).
 struct TileLds {
-    __bf16 xb[kTile * 32];     // features, K padded to 32
+    __bf16 xb[kTile * 32];     // features [e][k], K padded to 32
+    __bf16 xt[32 * kTile];     // the same transposed, [k][e]
     float z1[kTile * 64];
-    __bf16 h1b[kTile * 64];
+    __bf16 h1b[kTile * 64];    // [e][j]
+    __bf16 h1t[64 * kTile];    // [j][e]
     float z2[kTile * 64];      // reused for the FP32 first-layer delta in backward
     __bf16 h2b[kTile * 64];
     __bf16 db[kTile * 64];     // second-layer delta, [e][j]
@@ -74,29 +82,38 @@ struct TileLds {
     float s[kTile], ds[kTile];
 };
 
-__device__ void load_weights(const float* params, WeightsLds& w) {
-    for (int i = threadIdx.x; i < 32 * 64; i += blockDim.x) {
-        const int k = i / 64, j = i % 64;
-        w.w1t[i] = static_cast<__bf16>(k < 8 ? params[oW1 + j * 8 + k] : 0.0f);
-    }
-    for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
-        const int k = i / 64, j = i % 64;
-        w.w2t[i] = static_cast<__bf16>(params[oW2 + j * 64 + k]);
-        w.w2n[i] = static_cast<__bf16>(params[oW2 + i]);
-    }
-    for (int i = threadIdx.x; i < 64 * 16; i += blockDim.x) {
-        const int k = i / 16, j = i % 16;
-        w.w3t[i] = static_cast<__bf16>(j == 0 ? params[oW3 + k] : 0.0f);
-    }
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
-        w.b1[i] = params[oB1 + i];
-        w.b2[i] = params[oB2 + i];
-        w.w3[i] = static_cast<float>(static_cast<__bf16>(params[oW3 + i]));
-    }
-    if (threadIdx.x == 0) w.b3 = params[oB3];
+__device__ void load_weights(const WeightsLds* image, WeightsLds& w) {
+    const uint4* src = reinterpret_cast<const uint4*>(image);
+    uint4* dst = reinterpret_cast<uint4*>(&w);
+    for (uint32_t i = threadIdx.x; i < sizeof(WeightsLds) / 16; i += blockDim.x) dst[i] = src[i];
 }
 
-// Forward pass of one tile of `count` edges starting at `first`.
+// Parameter p's entries of the weight image (the zero padding is written
+// once, when the image is cleared).
+__device__ __forceinline__ void image_entry(WeightsLds* w, uint32_t p, float value) {
+    const __bf16 h = static_cast<__bf16>(value);
+    if (p < oB1) {
+        const uint32_t j = (p - oW1) / 8, k = (p - oW1) % 8;
+        w->w1n[j * 32 + k] = h;
+    } else if (p < oW2) {
+        w->b1[p - oB1] = value;
+    } else if (p < oB2) {
+        const uint32_t j = (p - oW2) / 64, k = (p - oW2) % 64;
+        w->w2n[j * 64 + k] = h;
+        w->w2t[k * 64 + j] = h;
+    } else if (p < oW3) {
+        w->b2[p - oB2] = value;
+    } else if (p < oB3) {
+        w->w3n[p - oW3] = h;
+        w->w3[p - oW3] = static_cast<float>(h);
+    } else {
+        w->b3 = value;
+    }
+}
+
+// Forward pass of one tile of `count` edges starting at `first`; with
+// Backward, the transposed features and first hidden layer too.
+template<bool Backward>
 __device__ void forward_tile(const float* features, const uint32_t* indices, uint32_t first, uint32_t count,
                              const WeightsLds& w, TileLds& t, int lane) {
     for (int i = lane; i < kTile * 32; i += 64) {
@@ -107,19 +124,22 @@ __device__ void forward_tile(const float* features, const uint32_t* indices, uin
             v = features[static_cast<size_t>(id) * 8 + k];
         }
         t.xb[i] = static_cast<__bf16>(v);
+        if (Backward) t.xt[k * kTile + e] = static_cast<__bf16>(v);
     }
     wsync();
     const int fi = 4 * (lane >> 4), fj = lane & 15;
     for (int rt = 0; rt < 2; ++rt)
         for (int ct = 0; ct < 4; ++ct) {
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mma(t.xb + rt * 16 * 32, 32, w.w1t + ct * 16, 64, acc, lane);
+            acc = mma(t.xb + rt * 16 * 32, 32, w.w1n + ct * 16 * 32, 32, acc, lane);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int e = rt * 16 + fi + r, j = ct * 16 + fj;
                 const float z = acc[r] + w.b1[j];
+                const __bf16 h = static_cast<__bf16>(gelu(z));
                 t.z1[e * 64 + j] = z;
-                t.h1b[e * 64 + j] = static_cast<__bf16>(gelu(z));
+                t.h1b[e * 64 + j] = h;
+                if (Backward) t.h1t[j * kTile + e] = h;
             }
         }
     wsync();
@@ -127,7 +147,7 @@ __device__ void forward_tile(const float* features, const uint32_t* indices, uin
         for (int ct = 0; ct < 4; ++ct) {
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int ks = 0; ks < 2; ++ks)
-                acc = mma(t.h1b + rt * 16 * 64 + ks * 32, 64, w.w2t + ks * 32 * 64 + ct * 16, 64, acc, lane);
+                acc = mma(t.h1b + rt * 16 * 64 + ks * 32, 64, w.w2n + ct * 16 * 64 + ks * 32, 64, acc, lane);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int e = rt * 16 + fi + r, j = ct * 16 + fj;
@@ -140,7 +160,7 @@ __device__ void forward_tile(const float* features, const uint32_t* indices, uin
     for (int rt = 0; rt < 2; ++rt) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int ks = 0; ks < 2; ++ks)
-            acc = mma(t.h2b + rt * 16 * 64 + ks * 32, 64, w.w3t + ks * 32 * 16, 16, acc, lane);
+            acc = mma(t.h2b + rt * 16 * 64 + ks * 32, 64, w.w3n + ks * 32, 64, acc, lane);
         if (fj == 0)
 #pragma unroll
             for (int r = 0; r < 4; ++r) t.s[rt * 16 + fi + r] = acc[r] + w.b3;
@@ -183,18 +203,18 @@ __global__ void faco_prior_features_kernel(uint32_t n, uint32_t pw, const uint32
 }
 
 // Logits of `edges` edges.
-__global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_infer_kernel(const float* params, const float* features,
+__global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_infer_kernel(const WeightsLds* image, const float* features,
                                                                                uint32_t edges, float* logits) {
     __shared__ WeightsLds w;
     __shared__ TileLds tiles[kWavesPerBlock];
-    load_weights(params, w);
+    load_weights(image, w);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     TileLds& t = tiles[wave];
     const uint32_t waves = gridDim.x * kWavesPerBlock;
     for (uint32_t tile = blockIdx.x * kWavesPerBlock + wave; tile * kTile < edges; tile += waves) {
         const uint32_t first = tile * kTile, count = edges - first < kTile ? edges - first : kTile;
-        forward_tile(features, nullptr, first, count, w, t, lane);
+        forward_tile<false>(features, nullptr, first, count, w, t, lane);
         if (lane < static_cast<int>(count)) logits[first + lane] = t.s[lane];
         wsync();
     }
@@ -202,14 +222,14 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_infer_kernel(c
 
 // Gradient partials of the weighted BCE loss over `edges` edges: one row
 // of kParams + 1 (the loss) per wave.
-__global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_grad_kernel(const float* params, const float* features,
+__global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_grad_kernel(const WeightsLds* image, const float* features,
                                                                               const uint8_t* labels,
                                                                               const uint32_t* indices, uint32_t offset,
                                                                               uint32_t edges, float pos_weight,
                                                                               float inv_total, float* partial) {
     __shared__ WeightsLds w;
     __shared__ TileLds tiles[kWavesPerBlock];
-    load_weights(params, w);
+    load_weights(image, w);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     TileLds& t = tiles[wave];
@@ -223,7 +243,7 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_grad_kernel(co
     float gb1 = 0.0f, gb2 = 0.0f, gw3 = 0.0f, gb3 = 0.0f, loss = 0.0f;
     for (uint32_t tile = blockIdx.x * kWavesPerBlock + wave; tile * kTile < edges; tile += waves) {
         const uint32_t first = tile * kTile, count = edges - first < kTile ? edges - first : kTile;
-        forward_tile(features, indices, offset + first, count, w, t, lane);
+        forward_tile<true>(features, indices, offset + first, count, w, t, lane);
         if (lane < kTile) {
             float ds = 0.0f;
             if (lane < static_cast<int>(count)) {
@@ -265,13 +285,13 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_grad_kernel(co
         wsync();
         // gW2 += dz2^T h1 (K = the 32 edges of the tile)
         for (int jt = 0; jt < 4; ++jt)
-            for (int kt = 0; kt < 4; ++kt) g2[jt][kt] = mma(t.dbt + jt * 16 * kTile, kTile, t.h1b + kt * 16, 64, g2[jt][kt], lane);
+            for (int kt = 0; kt < 4; ++kt) g2[jt][kt] = mma(t.dbt + jt * 16 * kTile, kTile, t.h1t + kt * 16 * kTile, kTile, g2[jt][kt], lane);
         // first-layer delta: dz1 = (dz2 W2) * gelu'(z1), FP32 into the z2 buffer
         for (int rt = 0; rt < 2; ++rt)
             for (int ct = 0; ct < 4; ++ct) {
                 f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 for (int ks = 0; ks < 2; ++ks)
-                    acc = mma(t.db + rt * 16 * 64 + ks * 32, 64, w.w2n + ks * 32 * 64 + ct * 16, 64, acc, lane);
+                    acc = mma(t.db + rt * 16 * 64 + ks * 32, 64, w.w2t + ct * 16 * 64 + ks * 32, 64, acc, lane);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int e = rt * 16 + fi + r, k = ct * 16 + fj;
@@ -291,7 +311,7 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_grad_kernel(co
         }
         wsync();
         // gW1 += dz1^T x
-        for (int jt = 0; jt < 4; ++jt) g1[jt] = mma(t.dbt + jt * 16 * kTile, kTile, t.xb, 32, g1[jt], lane);
+        for (int jt = 0; jt < 4; ++jt) g1[jt] = mma(t.dbt + jt * 16 * kTile, kTile, t.xt, kTile, g1[jt], lane);
         wsync();
     }
     float* out = partial + static_cast<size_t>(blockIdx.x * kWavesPerBlock + wave) * (kParams + 1);
@@ -315,7 +335,7 @@ __global__ void __launch_bounds__(64 * kWavesPerBlock) faco_prior_grad_kernel(co
 // the FP32 master weights (bias-corrected, step number `step` >= 1).
 __global__ void faco_prior_reduce_kernel(const float* partial, uint32_t rows, float* grad, float* params, float* m,
                                          float* v, uint32_t adam, float lr, float beta1, float beta2, float eps,
-                                         uint32_t step, float* loss_sum) {
+                                         uint32_t step, float* loss_sum, WeightsLds* image) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > static_cast<uint32_t>(kParams)) return;
     float g = 0.0f;
@@ -333,16 +353,27 @@ __global__ void faco_prior_reduce_kernel(const float* partial, uint32_t rows, fl
     const float mh = mm / (1.0f - powf(beta1, static_cast<float>(step)));
     const float vh = vv / (1.0f - powf(beta2, static_cast<float>(step)));
     params[p] -= lr * mh / (sqrtf(vh) + eps);
+    image_entry(image, p, params[p]);
+}
+
+// Every parameter's entries of the weight image (after the image is cleared).
+__global__ void faco_prior_image_kernel(const float* params, WeightsLds* image) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < static_cast<uint32_t>(kParams)) image_entry(image, p, params[p]);
 }
 
 // 16x16x32 bf16 product of one wave on global operands (layout check).
 __global__ void faco_prior_mma_check_kernel(const float* A, const float* B, float* C) {
-    __shared__ __bf16 a[16 * 32], b[32 * 16];
+    __shared__ __attribute__((aligned(16))) __bf16 a[16 * 32];
+    __shared__ __attribute__((aligned(16))) __bf16 bc[16 * 32];   // B column-major: bc[c][k] = B[k][c]
     const int lane = threadIdx.x;
-    for (int i = lane; i < 16 * 32; i += 64) { a[i] = static_cast<__bf16>(A[i]); b[i] = static_cast<__bf16>(B[i]); }
+    for (int i = lane; i < 16 * 32; i += 64) {
+        a[i] = static_cast<__bf16>(A[i]);
+        bc[(i % 16) * 32 + i / 16] = static_cast<__bf16>(B[i]);
+    }
     wsync();
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    acc = mma(a, 32, b, 16, acc, lane);
+    acc = mma(a, 32, bc, 32, acc, lane);
 #pragma unroll
     for (int r = 0; r < 4; ++r) C[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[r];
 }
@@ -365,28 +396,40 @@ uint32_t prior_grad_rows(uint32_t edges) {
     return (blocks ? blocks : 1u) * kWavesPerBlock;
 }
 
-hipError_t launch_prior_infer(const float* params, const float* features, uint32_t edges, float* logits,
-                              hipStream_t stream) {
-    const uint32_t blocks = prior_grad_rows(edges) / kWavesPerBlock;
-    hipLaunchKernelGGL(faco_prior_infer_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, stream, params, features,
-                       edges, logits);
+uint32_t prior_image_bytes() { return static_cast<uint32_t>(sizeof(WeightsLds)); }
+
+hipError_t launch_prior_image(const float* params, void* image, hipStream_t stream) {
+    hipError_t status = hipMemsetAsync(image, 0, sizeof(WeightsLds), stream);
+    if (status != hipSuccess) return status;
+    hipLaunchKernelGGL(faco_prior_image_kernel, dim3((kParams + 255) / 256), dim3(256), 0, stream, params,
+                       static_cast<WeightsLds*>(image));
     return hipGetLastError();
 }
 
-hipError_t launch_prior_grad(const float* params, const float* features, const uint8_t* labels,
+hipError_t launch_prior_infer(const void* image, const float* features, uint32_t edges, float* logits,
+                              hipStream_t stream) {
+    const uint32_t blocks = prior_grad_rows(edges) / kWavesPerBlock;
+    hipLaunchKernelGGL(faco_prior_infer_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, stream,
+                       static_cast<const WeightsLds*>(image), features, edges, logits);
+    return hipGetLastError();
+}
+
+hipError_t launch_prior_grad(const void* image, const float* features, const uint8_t* labels,
                              const uint32_t* indices, uint32_t offset, uint32_t edges, float pos_weight,
                              float inv_total, float* partial, hipStream_t stream) {
     const uint32_t blocks = prior_grad_rows(edges) / kWavesPerBlock;
-    hipLaunchKernelGGL(faco_prior_grad_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, stream, params, features,
-                       labels, indices, offset, edges, pos_weight, inv_total, partial);
+    hipLaunchKernelGGL(faco_prior_grad_kernel, dim3(blocks), dim3(64 * kWavesPerBlock), 0, stream,
+                       static_cast<const WeightsLds*>(image), features, labels, indices, offset, edges, pos_weight,
+                       inv_total, partial);
     return hipGetLastError();
 }
 
 hipError_t launch_prior_reduce(const float* partial, uint32_t rows, float* grad, float* params, float* m, float* v,
                                bool adam, float lr, float beta1, float beta2, float eps, uint32_t step,
-                               float* loss_sum, hipStream_t stream) {
+                               float* loss_sum, void* image, hipStream_t stream) {
     hipLaunchKernelGGL(faco_prior_reduce_kernel, dim3((kParams + 1 + 255) / 256), dim3(256), 0, stream, partial, rows,
-                       grad, params, m, v, adam ? 1u : 0u, lr, beta1, beta2, eps, step, loss_sum);
+                       grad, params, m, v, adam ? 1u : 0u, lr, beta1, beta2, eps, step, loss_sum,
+                       static_cast<WeightsLds*>(image));
     return hipGetLastError();
 }
 
