@@ -138,6 +138,27 @@ LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_
     throw std::invalid_argument("instance too large for the LDS budget of one ant");
 }
 
+// The shape with the most ants in flight for instances of up to n cities,
+// neighbour rows in LDS where the same count fits beside them (waves are
+// taken dynamically, so any count 1..16); waves 0 when none fits.
+LaunchShape most_waves_shape(uint32_t n, uint32_t K, uint32_t extra_wave_bytes) {
+    LaunchShape shape;
+    constexpr size_t kStaticLds = 256;
+    const size_t budget = max_lds_per_block() - kStaticLds;
+    shape.nbr_lds_bytes = round16_host(2u * n * K);
+    const uint32_t per_wave = ant_wave_bytes(n) + extra_wave_bytes;
+    for (uint32_t w = 16; w >= 1; --w)
+        for (uint32_t lds_rows : {1u, 0u}) {
+            const size_t shared = ant_shared_bytes(n, lds_rows ? shape.nbr_lds_bytes : 0u);
+            if (shared + static_cast<size_t>(w) * per_wave <= budget) {
+                shape.waves = w; shape.nbr_in_lds = lds_rows;
+                return shape;
+            }
+        }
+    shape.waves = 0;
+    return shape;
+}
+
 struct PreparedSlot { uint64_t key; const Prepared* problem; };
 
 }  // namespace
@@ -185,6 +206,15 @@ struct Engine {
     DeviceArray<uint32_t> order;           // [C]
     bool shapes_active = false;
     uint32_t min_ants = 0;                 // smallest ants of the current batch
+    // Size classes of a mixed batch on the persistent paths: the colonies of
+    // at most small_n cities (listed after the others in `order`) run as a
+    // second launch, beside the first on the side stream, with small_shape:
+    // more ants in flight than the engine's shape, which is sized for the
+    // largest instance, allows. split_small = 0: one launch.
+    uint32_t split_big = 0, split_small = 0, small_n = 0;
+    LaunchShape small_shape;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int compute_units = 1;
     double p_root = 0;
     // device state
@@ -273,6 +303,9 @@ struct Engine {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (ev_start) (void)hipEventDestroy(ev_start);
         if (ev_stop) (void)hipEventDestroy(ev_stop);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -354,6 +387,25 @@ struct Engine {
             return static_cast<uint64_t>(host[x].n) * host[x].ants * host[x].batches >
                    static_cast<uint64_t>(host[y].n) * host[y].ants * host[y].batches;
         });
+        // Size classes: the largest colony size whose workgroup holds more
+        // ants than the engine's shape bounds the small class; both classes
+        // keep the largest-work-first order.
+        split_big = tasks; split_small = 0; small_n = 0;
+        if (!hard) {
+            std::vector<uint32_t> sizes;
+            for (uint32_t c = 0; c < tasks; ++c) sizes.push_back(host[c].n);
+            std::sort(sizes.begin(), sizes.end());
+            sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+            for (auto it = sizes.rbegin(); it != sizes.rend(); ++it) {
+                const LaunchShape s = most_waves_shape(*it, K, wave_extra);
+                if (s.waves > shape.waves) { small_n = *it; small_shape = s; break; }
+            }
+            if (small_n) {
+                std::stable_partition(ord.begin(), ord.end(), [&](uint32_t c) { return host[c].n > small_n; });
+                for (uint32_t c = 0; c < tasks; ++c) split_small += host[c].n <= small_n ? 1u : 0u;
+                split_big = tasks - split_small;
+            }
+        }
         shapes.upload(host.data(), tasks, stream);
         order.upload(ord.data(), tasks, stream);
         if (smallest != min_ants) drop_graph();   // the workgroup split depends on it
@@ -371,6 +423,40 @@ struct Engine {
     void drop_graph() {
         if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
         graph_colonies = graph_iterations = 0;
+    }
+    // The small class's launch parameters from a launch's `ap` (the large
+    // class keeps `ap`, with the whole batch's colony count).
+    AntParams small_class(const AntParams& ap, uint32_t colonies) const {
+        AntParams s = ap;
+        s.waves = small_shape.waves;
+        s.nbr_in_lds = small_shape.nbr_in_lds;
+        s.nbr_lds_bytes = small_shape.nbr_lds_bytes;
+        s.per_wave_bytes = ant_wave_bytes(small_n) + wave_extra;
+        s.lds_n = small_n;
+        s.order = ap.order + split_big;
+        s.total_colonies = colonies;
+        return s;
+    }
+    // Persistent launches of a mixed batch of `colonies` colonies by size
+    // class: `launch(params, count, stream)` runs one class; the small class
+    // runs on the side stream, forked from and joined to the engine stream.
+    template<class Launch> void by_class(const AntParams& ap, uint32_t colonies, bool small_fits, Launch&& launch) {
+        if (!shapes_active || !split_small || !small_fits) { launch(ap, colonies, stream); return; }
+        const AntParams small = small_class(ap, colonies);
+        if (!split_big) { launch(small, colonies, stream); return; }
+        if (!side) {
+            checked(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate");
+            checked(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+            checked(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
+        }
+        AntParams big = ap;
+        big.total_colonies = colonies;
+        checked(hipEventRecord(ev_fork, stream), "event");
+        checked(hipStreamWaitEvent(side, ev_fork, 0), "stream wait");
+        launch(big, split_big, stream);
+        launch(small, split_small, side);
+        checked(hipEventRecord(ev_join, side), "event");
+        checked(hipStreamWaitEvent(stream, ev_join, 0), "stream wait");
     }
 
     const Prepared& problem(uint64_t key) const {
@@ -540,7 +626,9 @@ struct Engine {
             for (uint64_t done = 0; done < iterations;) {
                 const uint64_t rounds = std::min(iterations - done, kRoundsPerLaunch);
                 checked(hipEventRecord(ev[0], stream), "event");
-                checked(launch_colony_run(ap, cp, colonies, static_cast<uint32_t>(rounds), stream), "colony run launch");
+                by_class(ap, colonies, true, [&](const AntParams& p, uint32_t count, hipStream_t s) {
+                    checked(launch_colony_run(p, cp, count, static_cast<uint32_t>(rounds), s), "colony run launch");
+                });
                 ++persistent_launch_count;
                 checked(hipEventRecord(ev[1], stream), "event");
                 checked(hipEventSynchronize(ev[1]), "event sync");
@@ -605,7 +693,9 @@ struct Engine {
         if (persistent(ap)) {
             for (uint64_t done = 0; done < iterations;) {
                 const uint64_t rounds = std::min(iterations - done, kRoundsPerLaunch);
-                checked(launch_colony_run(ap, cp, colonies, static_cast<uint32_t>(rounds), stream), "colony run launch");
+                by_class(ap, colonies, true, [&](const AntParams& p, uint32_t count, hipStream_t s) {
+                    checked(launch_colony_run(p, cp, count, static_cast<uint32_t>(rounds), s), "colony run launch");
+                });
                 ++persistent_launch_count;
                 done += rounds;
             }
@@ -1197,8 +1287,11 @@ void evaluate_control(Engine& e, const uint64_t* panel_keys, const uint64_t* pan
             uint64_t left = stop - done;
             for (; fused && left;) {
                 const uint64_t rounds = std::min(left, Engine::kRoundsPerLaunch);
-                checked(launch_control_run(ap, colony, cp, b.params.data(), C, static_cast<uint32_t>(rounds), e.stream),
-                        "control run launch");
+                e.by_class(ap, C, fused_control(e.small_class(ap, C)),
+                           [&](const AntParams& p, uint32_t count, hipStream_t s) {
+                               checked(launch_control_run(p, colony, cp, b.params.data(), count,
+                                                          static_cast<uint32_t>(rounds), s), "control run launch");
+                           });
                 ++e.persistent_launch_count;
                 left -= rounds;
             }

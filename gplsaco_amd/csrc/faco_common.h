@@ -228,6 +228,11 @@ struct AntParams {
     uint32_t waves;                     // waves per workgroup
     uint32_t chunk;                     // ants of one colony handled by one workgroup
     uint32_t blocks_per_colony;         // workgroups per colony (explicit mode: unused)
+    // A launch over part of a mixed batch (its colonies listed from
+    // `order`): the whole batch's colony count (0: the grid) and the largest
+    // colony n of the launch, which sizes its LDS (0: n)
+    uint32_t total_colonies;
+    uint32_t lds_n;
     uint32_t nbr_lds_bytes;             // neighbour rows in LDS (16-byte rounded)
     uint32_t nbr_stride;                // u16 elements per geometry (nbr_lds_bytes / 2)
     uint32_t per_wave_bytes;            // per-ant LDS block
@@ -473,5 +478,7 @@ inline uint32_t ant_shared_bytes(uint32_t n, uint32_t nbr_lds_bytes) {
     return 2u * round16_host(4u * n) + nbr_lds_bytes + (n <= kMaxLinked ? round16_host(2u * n) : 0u);
 }
 inline uint32_t ant_wave_bytes(uint32_t n) { return 3u * round16_host(2u * n) + round16_host(n); }
+// the n that sizes a launch's LDS
+inline uint32_t launch_lds_n(const AntParams& p) { return p.lds_n ? p.lds_n : p.n; }
 
 }  // namespace faco

@@ -451,7 +451,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     if constexpr (Controlled) {
         // this batch's decision: features, scoring, action, restart (which
         // also refreshes the LDS parent links and positions)
-        control_decide(*ctl, colony, nblocks, threadIdx.x, blockDim.x, *reinterpret_cast<DecideShared*>(lds + shared_bytes),
+        control_decide(*ctl, colony, p.total_colonies ? p.total_colonies : nblocks, threadIdx.x, blockDim.x,
+                       *reinterpret_cast<DecideShared*>(lds + shared_bytes),
                        s_plinks, with_ppos ? s_ppos : nullptr);
         __syncthreads();
         if (threadIdx.x == 0) *s_state = p.state[colony];
@@ -1775,7 +1776,7 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t str
 
 // ---------------------------------------------------------------------------
 int ant_search_blocks_per_cu(const AntParams& p) {
-    const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
+    const uint32_t lds = ant_shared_bytes(launch_lds_n(p), p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     int blocks = 0;
     const hipError_t e = p.nbr_in_lds
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, faco_ant_search<true, false, false, false>, 64 * p.waves, lds)
@@ -1784,7 +1785,7 @@ int ant_search_blocks_per_cu(const AntParams& p) {
 }
 
 hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t stream) {
-    const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
+    const uint32_t lds = ant_shared_bytes(launch_lds_n(p), p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     const dim3 grid(blocks), block(64 * p.waves);
     const bool diag = p.explicit_mode || p.per_ant_parent || p.trace_selected || p.diag_construction ||
                       p.diag_positions || p.diag_checklist;
@@ -1818,7 +1819,7 @@ hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_
         p.esc_rows || (p.graph_offsets && p.or_opt) ||
         p.diag_construction || p.diag_positions || p.diag_checklist || p.control || p.out_identity)
         return hipErrorInvalidValue;   // one workgroup per colony, production plain or Hard FACO only
-    const uint32_t lds = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
+    const uint32_t lds = ant_shared_bytes(launch_lds_n(p), p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     const dim3 grid(colonies), block(64 * p.waves);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p, cp, rounds); };
     const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
@@ -1835,7 +1836,7 @@ hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_
 // LDS of the persistent controlled kernel: the ants' share, or the
 // control bodies' if larger (they reuse the ants' per-wave space).
 uint32_t control_run_lds_bytes(const AntParams& p) {
-    const uint32_t shared = ant_shared_bytes(p.n, p.nbr_in_lds ? p.nbr_lds_bytes : 0u);
+    const uint32_t shared = ant_shared_bytes(launch_lds_n(p), p.nbr_in_lds ? p.nbr_lds_bytes : 0u);
     const uint32_t ants = p.waves * p.per_wave_bytes;
     const uint32_t decide = static_cast<uint32_t>(sizeof(DecideShared));
     const uint32_t update = round16_host(static_cast<uint32_t>(sizeof(UpdateShared))) +
