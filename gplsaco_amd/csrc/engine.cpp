@@ -387,18 +387,22 @@ struct Engine {
             return static_cast<uint64_t>(host[x].n) * host[x].ants * host[x].batches >
                    static_cast<uint64_t>(host[y].n) * host[y].ants * host[y].batches;
         });
-        // Size classes: the largest colony size whose workgroup holds more
-        // ants than the engine's shape bounds the small class; both classes
-        // keep the largest-work-first order.
+        // Size classes: the small class is bounded by the largest colony
+        // size whose workgroup holds the most ants a workgroup can (16), or
+        // failing that more than the engine's shape; both classes keep the
+        // largest-work-first order.
         split_big = tasks; split_small = 0; small_n = 0;
         if (!hard) {
             std::vector<uint32_t> sizes;
             for (uint32_t c = 0; c < tasks; ++c) sizes.push_back(host[c].n);
             std::sort(sizes.begin(), sizes.end());
             sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
-            for (auto it = sizes.rbegin(); it != sizes.rend(); ++it) {
-                const LaunchShape s = most_waves_shape(*it, K, wave_extra);
-                if (s.waves > shape.waves) { small_n = *it; small_shape = s; break; }
+            for (const uint32_t need : {16u, shape.waves + 1u}) {
+                if (need <= shape.waves || small_n) continue;
+                for (auto it = sizes.rbegin(); it != sizes.rend(); ++it) {
+                    const LaunchShape s = most_waves_shape(*it, K, wave_extra);
+                    if (s.waves >= need) { small_n = *it; small_shape = s; break; }
+                }
             }
             if (small_n) {
                 std::stable_partition(ord.begin(), ord.end(), [&](uint32_t c) { return host[c].n > small_n; });
