@@ -101,7 +101,7 @@ size_t max_lds_per_block() {
 
 // Workgroup shape for the ant kernel: the most ants per workgroup that both
 // divides the ants of a colony and fits the LDS budget; neighbour rows are
-// staged in LDS when at least one ant still fits beside them.
+// staged in LDS when that many ants still fit beside them.
 LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_waves, uint32_t extra_wave_bytes = 0) {
     LaunchShape shape;
     // the kernels' static LDS (ant slot counter, the persistent kernel's
@@ -125,10 +125,12 @@ LaunchShape choose_shape(uint32_t n, uint32_t K, uint32_t ants, uint32_t forced_
         }
         throw std::invalid_argument("forced shape exceeds the LDS budget");
     }
-    for (uint32_t lds_rows : {1u, 0u}) {
-        const size_t shared = ant_shared_bytes(n, lds_rows ? shape.nbr_lds_bytes : 0u);
-        for (uint32_t w : {16u, 8u, 4u, 2u, 1u}) {
-            if (ants % w) continue;
+    // most ants in flight first (the ant kernel is bound by issue and
+    // latency, which more resident waves hide), rows in LDS at equal count
+    for (uint32_t w : {16u, 8u, 4u, 2u, 1u}) {
+        if (ants % w) continue;
+        for (uint32_t lds_rows : {1u, 0u}) {
+            const size_t shared = ant_shared_bytes(n, lds_rows ? shape.nbr_lds_bytes : 0u);
             if (shared + static_cast<size_t>(w) * per_wave <= budget) {
                 shape.waves = w; shape.nbr_in_lds = lds_rows;
                 return shape;
