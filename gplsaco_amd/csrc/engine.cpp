@@ -685,10 +685,9 @@ struct Engine {
     // kernel: up to kRoundsPerLaunch batches per launch, each colony's
     // workgroup doing its own colony update between batches.
     static constexpr uint64_t kRoundsPerLaunch = 1024;
-    // (Hard mode too, with plain 2-opt; Escape mode runs per batch: its
-    // footprint update is compiled into the per-batch colony update only)
+    // (Hard and Escape mode too, with plain 2-opt)
     static bool persistent(const AntParams& ap) {
-        return ap.blocks_per_colony == 1 && !ap.esc_rows && !(ap.graph_offsets && ap.or_opt);
+        return ap.blocks_per_colony == 1 && !(ap.graph_offsets && ap.or_opt);
     }
     // Run `iterations` batches: persistent launches when the shape allows,
     // else per batch an ant launch and a colony-update launch, through a

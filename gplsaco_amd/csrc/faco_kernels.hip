@@ -309,8 +309,8 @@ template<bool NbrLds, bool Profile, bool Diag, bool OrOpt, bool Persistent, bool
 __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParams& cp, uint32_t rounds,
                                            ColonyState* s_state, const ControlParams* ctl = nullptr) {
     static_assert(!Controlled || Persistent, "the fused control runs in the persistent kernel");
-    static_assert(!Escape || (Hard && !Persistent && !Diag && !Profile && !OrOpt),
-                  "Escape mode runs per batch on a Hard graph with the plain 2-opt");
+    static_assert(!Escape || (Hard && !Controlled && !Diag && !Profile && !OrOpt),
+                  "Escape mode runs on a Hard graph with the plain 2-opt");
     // Diagnostic features (explicit visits, per-ant parents, traces and
     // logs) exist only in the Diag instantiation: the production kernel
     // keeps none of their pointers or flags live.
@@ -1628,8 +1628,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
     } else {
         // every ant of the batch is done: the colony update, in place
         __syncthreads();
-        update_colony(cp, colony, threadIdx.x, blockDim.x, reinterpret_cast<uint64_t*>(lds + shared_bytes),
-                      *s_state, s_plinks, with_ppos ? s_ppos : nullptr);
+        update_colony<Escape>(cp, colony, threadIdx.x, blockDim.x, reinterpret_cast<uint64_t*>(lds + shared_bytes),
+                              *s_state, s_plinks, with_ppos ? s_ppos : nullptr);
         if constexpr (Controlled) {
             __syncthreads();
             control_update(*ctl, colony, threadIdx.x, blockDim.x, *reinterpret_cast<UpdateShared*>(lds + shared_bytes),
@@ -1662,10 +1662,10 @@ __global__ void __launch_bounds__(1024) faco_ant_search_escape(const AntParams p
     ant_search<NbrLds, false, false, false, false, true, false, true>(p, ColonyParams{}, 1u, nullptr);
 }
 
-template<bool NbrLds, bool OrOpt, bool Hard>
+template<bool NbrLds, bool OrOpt, bool Hard, bool Escape = false>
 __global__ void __launch_bounds__(1024) faco_colony_run(const AntParams p, const ColonyParams cp, uint32_t rounds) {
     __shared__ ColonyState s_state;
-    ant_search<NbrLds, false, false, OrOpt, true, Hard>(p, cp, rounds, &s_state);
+    ant_search<NbrLds, false, false, OrOpt, true, Hard, false, Escape>(p, cp, rounds, &s_state);
 }
 
 // GP-controlled FACO over `rounds` batches in one launch: one workgroup per
@@ -1816,14 +1816,16 @@ hipError_t launch_ant_search(const AntParams& p, uint32_t blocks, hipStream_t st
 hipError_t launch_colony_run(const AntParams& p, const ColonyParams& cp, uint32_t colonies, uint32_t rounds,
                              hipStream_t stream) {
     if (p.blocks_per_colony != 1 || p.phase_ticks || p.explicit_mode || p.per_ant_parent || p.trace_selected ||
-        p.esc_rows || (p.graph_offsets && p.or_opt) ||
+        (p.graph_offsets && p.or_opt) ||
         p.diag_construction || p.diag_positions || p.diag_checklist || p.control || p.out_identity)
-        return hipErrorInvalidValue;   // one workgroup per colony, production plain or Hard FACO only
+        return hipErrorInvalidValue;   // one workgroup per colony, production plain, Hard or Escape FACO only
     const uint32_t lds = ant_shared_bytes(launch_lds_n(p), p.nbr_in_lds ? p.nbr_lds_bytes : 0u) + p.waves * p.per_wave_bytes;
     const dim3 grid(colonies), block(64 * p.waves);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, lds, stream, p, cp, rounds); };
     const bool nl = p.nbr_in_lds != 0, oo = p.or_opt != 0;
-    if (p.graph_offsets) {   // Hard mode: plain 2-opt
+    if (p.esc_rows) {        // Escape mode (on the Hard graph): plain 2-opt
+        nl ? go(faco_colony_run<true, false, true, true>) : go(faco_colony_run<false, false, true, true>);
+    } else if (p.graph_offsets) {   // Hard mode: plain 2-opt
         nl ? go(faco_colony_run<true, false, true>) : go(faco_colony_run<false, false, true>);
     } else if (nl) {
         oo ? go(faco_colony_run<true, true, false>) : go(faco_colony_run<true, false, false>);

@@ -38,16 +38,23 @@ def _cycle(tour):
     return {tuple(sorted((int(tour[i]), int(tour[(i + 1) % len(tour)])))) for i in range(len(tour))}
 
 
-@pytest.mark.parametrize("n,full,batches", [(31, True, 16), (31, False, 16), (200, False, 8), (1100, False, 3),
-                                             (2100, False, 2)])
-def test_escape_runs_match_the_oracle(ext, oracle_mod, n, full, batches):
+@pytest.mark.parametrize("n,full,batches,persistent", [(31, True, 16, False), (31, False, 16, False), (200, False, 8, False),
+                                                        (1100, False, 3, False), (2100, False, 2, False),
+                                                        (31, True, 16, True), (200, False, 8, True), (1100, False, 3, True),
+                                                        (2100, False, 2, True)])
+def test_escape_runs_match_the_oracle(ext, oracle_mod, n, full, batches, persistent):
     O = oracle_mod
     ants = 4 if n <= 31 else 16
     xy, engine, settings, graph, rows, members, s = _setup(ext, O, n, full, ants, 4)
     keys = np.ones(4, dtype=np.uint64)
     seeds = np.array([17, 29, 31, 37], dtype=np.uint64)
-    announce("escape", n, full)
+    # one workgroup per colony: the persistent kernel runs every batch; two:
+    # per-batch launches of the ant search and the colony update
+    engine.set_waves_per_block(engine.launch_shape(4)["waves_per_block"], 1 if persistent else 2)
+    assert engine.launch_shape(4)["blocks_per_colony"] == (1 if persistent else 2)
+    announce("escape", n, full, persistent)
     got = engine.evaluate_faco_evaluations(keys, seeds, ants * batches)
+    assert (got["persistent_launches"] > 0) == persistent, got["persistent_launches"]
     want = O.run_faco(xy[None].astype(np.int32), [1], seeds, settings, batches, 8, graphs=[graph], rows=[rows],
                       escape="on")
     for c in range(4):
