@@ -186,13 +186,16 @@ def hard_mode_rate(ext, iterations: int) -> dict:
     """Hard mode (every relocation and 2-opt move kept inside a candidate
     graph): the headline panel (32 TSP1000 instances x 8 seeds, 128 ants,
     2-opt) with each instance's graph = primary and LS rows plus the initial
-    tour, per-batch launches; beside it the same panel unrestricted."""
+    tour, in the persistent kernel; beside it the same panel in Escape mode
+    (escape gates open slot views that may add out-of-graph edges, up to 64
+    per ant) and unrestricted."""
     from gplsaco_amd.instances import instance_key, uniform_instance
     n, count, seeds_per = 1000, 32, 8
     xys = [uniform_instance(n, seed=0, instance=i) for i in range(count)]
     keys = [instance_key(n, 0, i, False) for i in range(count)]
     plain = ext.FacoBatchEngine(n, count * seeds_per, ext.FixedFacoSettings())
     hard = ext.FacoBatchEngine(n, count * seeds_per, ext.FixedFacoSettings(), constraint_mode="hard")
+    escape = ext.FacoBatchEngine(n, count * seeds_per, ext.FixedFacoSettings(), constraint_mode="escape")
     edges_total = 0
     for k, xy in zip(keys, xys):
         plain.register_problem(k, xy)
@@ -203,14 +206,17 @@ def hard_mode_rate(ext, iterations: int) -> dict:
             for a in range(n):
                 edges.update(tuple(sorted((a, int(b)))) for b in data[name][a])
         edges_total += len(edges)
-        hard.register_graph_problem(k, xy, {"graph_spec_id": 1, "common_initial_tour": tour.tolist(),
-                                            "edges": sorted(edges), "primary": data["primary"].tolist(),
-                                            "backup": data["backup"].tolist(), "ls": data["ls"].tolist()})
+        spec = {"graph_spec_id": 1, "common_initial_tour": tour.tolist(), "edges": sorted(edges),
+                "primary": data["primary"].tolist(), "backup": data["backup"].tolist(), "ls": data["ls"].tolist()}
+        hard.register_graph_problem(k, xy, spec)
+        escape.register_graph_problem(k, xy, spec)
     task_keys = np.array([keys[c % count] for c in range(count * seeds_per)], dtype=np.uint64)
     seeds = np.array([c // count for c in range(count * seeds_per)], dtype=np.uint64)
     ants = hard.settings.ants
     hard.evaluate_faco_evaluations(task_keys, seeds, ants * 4)      # warm-up (graph capture)
     out = hard.evaluate_faco_evaluations(task_keys, seeds, ants * iterations)
+    escape.evaluate_faco_evaluations(task_keys, seeds, ants * 4)    # warm-up
+    esc = escape.evaluate_faco_evaluations(task_keys, seeds, ants * iterations)
     ref = plain.evaluate_faco_evaluations(task_keys, seeds, ants * iterations)
     mean = lambda o: float(np.mean([i["cost"] for i in o["items"]]))  # noqa: E731
     return {"workload": f"Hard mode, {count} x TSP1000 x {seeds_per} seeds, {ants} ants, {iterations} iterations, "
@@ -218,7 +224,13 @@ def hard_mode_rate(ext, iterations: int) -> dict:
             "ant_tours_per_s": out["total_tour_evaluations"] / out["search_seconds"],
             "unrestricted_ant_tours_per_s": ref["total_tour_evaluations"] / ref["search_seconds"],
             "mean_final_tour_length": mean(out), "unrestricted_mean_final_tour_length": mean(ref),
-            "ls_evaluations_per_tour": out["completed_ls_evaluations"] / out["total_tour_evaluations"]}
+            "ls_evaluations_per_tour": out["completed_ls_evaluations"] / out["total_tour_evaluations"],
+            "persistent_launches": out["persistent_launches"],
+            "escape_mode": {"ant_tours_per_s": esc["total_tour_evaluations"] / esc["search_seconds"],
+                            "mean_final_tour_length": mean(esc),
+                            "ls_evaluations_per_tour": esc["completed_ls_evaluations"] / esc["total_tour_evaluations"],
+                            "persistent_launches": esc["persistent_launches"],
+                            "new_edges_per_tour": esc["escape_counters"]["new_edges"] / esc["total_tour_evaluations"]}}
 
 
 def uneven_batch_rate(ext, rounds: int) -> dict:
