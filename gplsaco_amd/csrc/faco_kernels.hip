@@ -1,15 +1,23 @@
 // HIP/CDNA4 kernels of the FACO batch engine (gfx950).
 //
-// ant_search: one wave64 per ant. The instance's packed coordinates and
+// ant_search: one wave64 per ant. The instance's packed coordinates, the
+//   colony's parent links and positions and (when they fit) the
 //   distance-ordered neighbour rows sit in LDS once per workgroup; each wave
 //   keeps its ant's tour, inverse positions, FIFO checklist and visited/queued
-//   flags in LDS (7 bytes per city). Scalar FACO state (current city, step,
-//   MNE, queue head/tail, cost) is wave-uniform and lives in SGPRs; lanes work
-//   on the parallel parts: the 16-slot roulette (ballot availability + an
-//   in-order FP64 prefix), relocation shifts and 2-opt flips (64 elements per
-//   instruction), and the 2x20 2-opt move scan with a wave-wide
-//   (gain, lower index) maximum. Distances are EUC_2D from LDS coordinates,
-//   computed in registers (no distance table, no HBM gather).
+//   flags in LDS (7 bytes per city; during construction the tour is a doubly
+//   linked list over the same bytes up to kMaxLinked cities). Scalar FACO
+//   state (current city, step, MNE, queue head/tail, cost) is wave-uniform
+//   and lives in SGPRs; lanes work on the parallel parts: the 16-slot
+//   roulette (ballot availability, an FP32 prefix with an exact FP64
+//   fallback), list ranking, relocation shifts and 2-opt flips (64 elements
+//   per instruction), and the 2x20 2-opt move scan with a wave-wide (gain,
+//   lower index) maximum. Distances are EUC_2D from LDS coordinates computed
+//   in registers, except d(a, b) of the 2-opt scan, read from the instance's
+//   table of neighbour-row distances (u16, L2-resident).
+//   Instantiations: per batch or persistent (the workgroup also runs its
+//   colony's update between batches, and in controlled runs the decision and
+//   the archive update), unrestricted, Hard or Escape, plain 2-opt or with
+//   Or-opt, production, clock-profiled or diagnostic.
 //   Semantics: reference cuda/faco_device.cuh:100-501 (construct_and_search),
 //   cuda/faco_choices.cuh:86-142 (StochasticChoices::next).
 //
