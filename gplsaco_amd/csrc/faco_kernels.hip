@@ -729,6 +729,11 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
         double u_pool = 0.0;
         float u_pool32 = 0.0f;
         uint32_t u_left = 0;   // uniforms of the pool not yet used
+        // Escape mode: the gates of the same 64 steps, from the first word
+        // of each step's escape stream computed in lane l for step s + l
+        // (bit l set when that word opens the gate); an open gate rebuilds
+        // its step's stream and skips the gate word.
+        uint64_t gate_pool = 0;
         while (mne < target && steps + 1 < n) {
             if constexpr (Profile) mark = clock64();
             const uint32_t step = steps + 1;
@@ -756,6 +761,11 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     asm volatile("" : "+s"(k_lo), "+s"(k_hi));
                     u_pool = step_uniform((static_cast<uint64_t>(k_hi) << 32) | k_lo, batch, local_ant, step + lane);
                     u_pool32 = static_cast<float>(u_pool);
+                    if constexpr (Escape) {
+                        uint32_t w[4] = {0u, step + lane, 0x40000000u | local_ant, batch};
+                        philox4x32_10(w, k_lo, k_hi);
+                        gate_pool = ballot((w[0] & 15u) == 0u);
+                    }
                 }
                 const uint32_t u_index = kWave - u_left;   // lane of this step's uniform
                 --u_left;
@@ -773,8 +783,9 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     // reference's pheromone lookup gives the default trail)
                     // and may open new edges (cuda/escape_device.cuh:41-80)
                     bool free_city = cur_free;
-                    EscapeStream gate(key, batch, local_ant, step, false);
-                    if (!p.escape_disabled && (gate.draw() & 15u) == 0u) {
+                    if (!p.escape_disabled && ((gate_pool >> u_index) & 1ull)) {
+                        EscapeStream gate(key, batch, local_ant, step, false);
+                        gate.draw();
                         ++e_gates;
                         const uint32_t bval = lane < p.bw ? backup[static_cast<size_t>(current) * p.bw + lane] : n;
                         const LaneView view = lane_slot_view(cur_cand, pw, bval, p.bw, n, current, gate, esc_scratch);
