@@ -893,8 +893,10 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                 }
                 selected = uniform(selected);
                 if constexpr (Escape) {
-                    // the move's new edges join the ant's cache
-                    if (selected < n) {
+                    // the move's new edges join the ant's cache (a move through
+                    // a slot that kept its own city was checked without
+                    // permission for new edges, so it has none)
+                    if (selected < n && novel) {
                         uint32_t novel_edges[3], count;
                         (void)esc_relocation(selected, current, cur_after, novel, novel_edges, count);
                         esc_commit(novel_edges, count);
@@ -1532,13 +1534,17 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
             if (best_kind == 0) { move[0] = a_next; move[1] = nnode; move[2] = a; move[3] = bnode; }
             else { move[0] = a; move[1] = bnode; move[2] = a_prev; move[3] = nnode; }
             if constexpr (Escape) {
-                // the move's new edges join the ant's cache
-                const uint32_t closing_node = best_kind == 0 ? a_next : a_prev;
-                const uint32_t removed[3][2] = {{a, closing_node}, {bnode, nnode}, {0u, 0u}};
-                const uint32_t added[3][2] = {{a, bnode}, {closing_node, nnode}, {0u, 0u}};
-                uint32_t novel[3], count;
-                (void)esc_status(2, removed, added, read_lane(permit, best_lane) != 0, novel, count);
-                esc_commit(novel, count);
+                // the move's new edges join the ant's cache (only a replaced
+                // slot's move may have any: the others passed without
+                // permission for new edges)
+                if (read_lane(permit, best_lane) != 0) {
+                    const uint32_t closing_node = best_kind == 0 ? a_next : a_prev;
+                    const uint32_t removed[3][2] = {{a, closing_node}, {bnode, nnode}, {0u, 0u}};
+                    const uint32_t added[3][2] = {{a, bnode}, {closing_node, nnode}, {0u, 0u}};
+                    uint32_t novel[3], count;
+                    (void)esc_status(2, removed, added, true, novel, count);
+                    esc_commit(novel, count);
+                }
             }
             flip_nodes(move[0], move[1]);
             ++accepted;
