@@ -162,7 +162,7 @@ __device__ __forceinline__ float read_lane(float v, uint32_t lane) {
 // the new parent links (LDS); `s_ppos` [n] (LDS, may be null) the parent
 // position of every city.
 // Footprints: the Escape-mode footprint and counter updates are compiled in
-// (the per-batch update kernel; the persistent kernels never run Escape).
+// (the per-batch update kernel and the persistent Escape kernel).
 template<bool Footprints = false>
 __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t colony, uint32_t tid, uint32_t threads,
                                               uint64_t* red, ColonyState& s, uint32_t* s_links, uint16_t* s_ppos) {
@@ -254,10 +254,17 @@ __device__ __forceinline__ void update_colony(const ColonyParams& p, uint32_t co
             if (improve & 2u) gfp[i] = v;
             pfp[i] = s.source_is_epoch ? efp[i] : v;
         }
-        if (tid < kEscStats) {
+        // one wave per counter, its lanes over the ants (callers launch
+        // whole waves)
+        const uint32_t lane = tid & 63u, waves = threads >> 6;
+        for (uint32_t k = tid >> 6; k < kEscStats; k += waves) {
             uint64_t sum = 0;
-            for (uint32_t a = 0; a < ants; ++a) sum += p.esc_stats[(static_cast<size_t>(colony) * p.ants + a) * kEscStats + tid];
-            p.esc_totals[static_cast<size_t>(colony) * kEscStats + tid] += sum;
+            for (uint32_t a = lane; a < ants; a += 64u)
+                sum += p.esc_stats[(static_cast<size_t>(colony) * p.ants + a) * kEscStats + k];
+#pragma unroll
+            for (int offset = 32; offset; offset >>= 1)
+                sum += static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(sum), offset, 64));
+            if (lane == 0) p.esc_totals[static_cast<size_t>(colony) * kEscStats + k] += sum;
         }
     }
     __syncthreads();
