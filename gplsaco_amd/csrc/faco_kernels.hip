@@ -790,7 +790,8 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     // reference's pheromone lookup gives the default trail)
                     // and may open new edges (cuda/escape_device.cuh:41-80)
                     bool free_city = cur_free;
-                    if (!p.escape_disabled && ((gate_pool >> u_index) & 1ull)) {
+                    const bool gated = !p.escape_disabled && ((gate_pool >> u_index) & 1ull);
+                    if (gated) {
                         EscapeStream gate(key, batch, local_ant, step, false);
                         gate.draw();
                         ++e_gates;
@@ -804,9 +805,15 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                             weight = dtrail * eta_backup[static_cast<size_t>(current) * p.bw + source];
                         }
                     }
-                    uint32_t novel[3], count;
-                    const uint32_t status = free_city ? esc_relocation(cand, current, cur_after, source != kNoSource,
-                                                                       novel, count) : 1u;
+                    // with no gate (no lane may open edges) and an empty
+                    // cache the test is the Hard one
+                    uint32_t status = 1u;
+                    if (!gated && esc_size == 0) {
+                        status = free_city ? (relocation_ok(cand, current, cur_after) ? 0u : 2u) : 1u;
+                    } else {
+                        uint32_t novel[3], count;
+                        status = free_city ? esc_relocation(cand, current, cur_after, source != kNoSource, novel, count) : 1u;
+                    }
                     e_construction_full += static_cast<uint32_t>(__builtin_popcountll(ballot(free_city && status == 3u)));
                     avail = free_city && status == 0u;
                 } else {
@@ -1425,9 +1432,15 @@ __device__ __forceinline__ void ant_search(const AntParams& p, const ColonyParam
                     if constexpr (Escape) {
                         // graph or cache edges, or new ones through a replaced
                         // slot while the cache has room (escape_edges.hpp:117-122)
-                        uint32_t novel[3], count;
-                        status = esc_status(2, removed, added, permit != 0, novel, count);
-                        move_allowed = status == 0u;
+                        // (a non-anchor row has no replaced slots: with an
+                        // empty cache the test is the Hard one)
+                        if (!a_anchor && esc_size == 0) {
+                            move_allowed = edges_allowed(2, removed, added);
+                        } else {
+                            uint32_t novel[3], count;
+                            status = esc_status(2, removed, added, permit != 0, novel, count);
+                            move_allowed = status == 0u;
+                        }
                     } else {
                         move_allowed = edges_allowed(2, removed, added);
                     }
